@@ -1,0 +1,219 @@
+/*
+ * vf_trace.h — C ABI of the MI355X (gfx950) voxel traversal library (libvftrace.so).
+ *
+ * Drop-in boundary for VoxelForge's traced passes.  Each entry point replaces one GPU call of the
+ * reference renderer; the structs keep the reference's #[repr(C)] / std140 byte layouts so the
+ * Rust host can pass its own `bytemuck::Pod` values straight through.  Citations are relative to
+ * the reference repository root.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative VF_E_* code; vf_last_error() gives text.
+ *     No exception or abort crosses the ABI; invalid input is an error, never a crash.
+ *   - Pointers named `d_*` are DEVICE pointers (caller-owned, e.g. from vf_device_alloc); all
+ *     other pointers are host pointers, read during the call only.
+ *   - Work is enqueued on the context's stream (vf_set_stream) and is asynchronous; vf_sync() is
+ *     the fence.  The world must not be updated while a trace on the same context is in flight.
+ *   - One context per host thread, or external synchronisation.
+ */
+#ifndef VF_TRACE_H
+#define VF_TRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VF_ABI_VERSION 1
+
+enum {
+    VF_OK = 0,
+    VF_E_INVALID = -1,     /* bad argument (null pointer, dimension, slot out of range) */
+    VF_E_DEVICE = -2,      /* HIP runtime error */
+    VF_E_NO_WORLD = -3,    /* a trace before vf_world_upload */
+    VF_E_OOM = -4,         /* device allocation failed */
+    VF_E_UNSUPPORTED = -5  /* a feature outside this library (e.g. bodies) */
+};
+
+/* ---- layouts shared with the reference host ------------------------------------------------ */
+
+/* Palette entry, 48 B (src/voxel/world.rs:21-31; common.glsl Material). */
+typedef struct vf_material {
+    float r, g, b, emissive;
+    float props[4];   /* density, jitter, kind (1 water, 2 grass, 3 foliage), variant */
+    float surf[4];    /* roughness, metalness, dielectric F0, detail */
+} vf_material;
+
+/* SceneUbo, std140, 1536 B (src/gfx/ubo.rs:10-86; offsets asserted there and in vf_abi.c). */
+typedef struct vf_scene_ubo {
+    uint32_t atlas_bricks[3]; float night;
+    int32_t world_bricks[3];  int32_t num_objects;
+    float sun_dir[3];         float grass_dist;
+    float sun_rgb[3];         float sun_angle;
+    float fog_sun_rgb[3];     float fog_density;
+    float fog_ambient[3];     float fog_g;
+    float sky_zenith[3];      float sky_scale;
+    float sky_horizon[3];     float gi_scale;
+    float sunset_rgb[3];      int32_t sky_model;
+    float fog_height, fog_falloff, fog_noise; int32_t fog_steps;
+    float fog_shadow_max; int32_t tonemap_op; float saturation, contrast;
+    float white_balance[3];   int32_t grass_shadows;
+    float sh_right[3];        float sh_texel0;
+    float sh_up[3];           float sh_texel1;
+    float sh_lz_top, sh_texel2; int32_t sh_n, std_lighting;
+    int32_t sh_win_a[4];
+    int32_t sh_win_b[4];
+    int32_t probe_origin[3];  float probe_spacing;
+    int32_t probe_dims[3];    int32_t probe_update_offset;
+    float probe_hyst, probe_normal_bias, std_soft_taps, std_cascade_blend;
+    int32_t std_cascade_bias; float std_view_bias, sh_texel3, std_soft_min;
+    int32_t probe_origin_c[3]; float probe_spacing_c;
+    int32_t world_win[4];     /* ring window min corner (bricks) in .x/.z; .w != 0 = ring mode */
+    int32_t hor_a[4];
+    int32_t hor_b[4];
+    uint32_t chunk_fade[256];
+    float gi_cache_cam[3];    float gi_cache_lod;
+    int32_t gi_cache_p[4];
+    float gi_cache_q[4];
+    int32_t hor_c[4];
+    float cloud_a[4];
+    float cloud_b[4];
+    float rain_a[4];
+} vf_scene_ubo;
+
+/* PassUbo, std140, 288 B (src/gfx/ubo.rs:91-131). */
+typedef struct vf_pass_ubo {
+    float cam_pos[3];      float lock_active;
+    float cam_right[3];    float fog_start;
+    float cam_up[3];       float haze_k;
+    float cam_fwd[3];      float tan_half_fov;
+    float prev_cam_pos[3]; float prev_tan_half_fov;
+    float prev_cam_right[3]; float haze_sea;
+    float prev_cam_up[3];  float _pf;
+    float prev_cam_fwd[3]; uint32_t frame_idx;
+    float exposure; int32_t u_step, u_iteration, debug_mode;
+    float time, prev_time; float jitter[2];
+    float water_p[4];
+    float look_p[4];
+    float fog_p[4];
+    float under_p[4];
+    float opt_p[4];
+    float wave_p[4];
+    float shift_p[4];
+    float facet_p[4];
+} vf_pass_ubo;
+
+/* Explicit ray, 24 B: origin and direction (voxel units; direction need not be normalised). */
+typedef struct vf_ray { float o[3]; float d[3]; } vf_ray;
+
+/* Hit record, 20 B (common.glsl:454-461 Hit, without the body index).
+ * face: 0 = none (the ray started inside the world box), 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z
+ * (the outward normal of the face the ray entered through).  Miss: hit = 0, t = -1, rest 0. */
+typedef struct vf_hit {
+    int32_t cell[3];   /* absolute voxel coordinates */
+    float t;
+    uint8_t mat, face, hit, pad;
+} vf_hit;
+
+/* The world as GpuWorld::upload (src/gfx/gpu_world.rs:228-437) packs it. */
+typedef struct vf_world_desc {
+    int32_t world_bricks[3];     /* grid dims; ring mode: x and z powers of two */
+    int32_t world_win[4];        /* SceneUbo.world_win */
+    const uint32_t *grid;        /* [wbx*wby*wbz], (z*wby + y)*wbx + x: 0 empty, 1-255 uniform, 256+slot */
+    const uint8_t *atlas;        /* [n_slots*512], per slot voxel lz*64 + ly*8 + lx (Brick::vox order) */
+    uint32_t n_slots;
+    const uint32_t *masks;       /* [n_slots*16] brick_mask layout (sparse_volume.rs:48-63), or NULL */
+    const uint8_t *coarse1;      /* [ceil(wb/4)^3-shaped] or NULL (then built here) */
+    const uint8_t *coarse2;      /* [ceil(wb/16)] or NULL (then built here) */
+    const vf_material *palette;  /* [256] */
+} vf_world_desc;
+
+typedef struct vf_ctx vf_ctx;
+
+/* ---- context ------------------------------------------------------------------------------- */
+/* Creates a context on HIP device `device` with its own default-priority stream.
+ * Replaces the Vulkan device/queue the reference passes run on (src/gfx/context.rs:67). */
+int vf_ctx_create(int device, vf_ctx **out);
+void vf_ctx_destroy(vf_ctx *ctx);
+/* Last error text of ctx (or of the calling thread when ctx is NULL); never NULL. */
+const char *vf_last_error(const vf_ctx *ctx);
+int vf_abi_version(void);
+/* Sizes of the shared structs, for the host's layout asserts (ubo.rs:76-86, 132-139). */
+int vf_abi_sizes(size_t *scene_ubo, size_t *pass_ubo, size_t *material, size_t *ray, size_t *hit);
+/* Run on a caller stream (a hipStream_t passed as void*; NULL = the context's own stream). */
+int vf_set_stream(vf_ctx *ctx, void *hip_stream);
+/* Fence: waits for all work enqueued on the context's stream (end_one_shot_and_wait). */
+int vf_sync(vf_ctx *ctx);
+
+/* ---- device memory helpers (the Rust host's Buffer equivalents) ------------------------------ */
+int vf_device_alloc(vf_ctx *ctx, size_t bytes, void **d_out);
+int vf_device_free(vf_ctx *ctx, void *d_ptr);
+int vf_copy_to_device(vf_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int vf_copy_to_host(vf_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int vf_memset_device(vf_ctx *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- world mirror -------------------------------------------------------------------------- */
+/* GpuWorld::upload (gpu_world.rs:228): grid, atlas, masks, coarse pyramid and palette, then the full
+ * brick distance field (gpu_world.rs:422) and the sky-top map (grasstop.comp skyTop). */
+int vf_world_upload(vf_ctx *ctx, const vf_world_desc *desc);
+/* GpuWorld::sync (gpu_world.rs:985) minus the Vulkan staging: set grid cells and (re)write atlas
+ * slots.  cells: grid indices + new values; slots: slot ids + 512 B each.  The caller then rebuilds
+ * the distance field over the dirty box with vf_build_brickdist, as the reference does. */
+int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_val, uint32_t n_cells,
+                    const uint32_t *slot_idx, const uint8_t *slot_vox, uint32_t n_slot_updates);
+/* GpuWorld::record_distance_field (gpu_world.rs:1376-1428): the separable L-inf brick distance
+ * field (brickdist.comp) over bricks [lo, hi] and the dry occupancy over [dirty_lo, dirty_hi],
+ * inclusive, absolute brick coordinates.  Also refreshes the coarse pyramid and sky-top columns. */
+int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dirty_lo[3],
+                       const int32_t dirty_hi[3]);
+/* Read back the two distance fields the traversal uses (brickDistTex, brickDistDryTex), host. */
+int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry);
+/* Read back the sky-top map (R16UI, (wbx*8) x (wbz*8)), host. */
+int vf_read_skytop(vf_ctx *ctx, uint16_t *skytop);
+
+/* ---- per-frame uniforms -------------------------------------------------------------------- */
+/* Renderer::set_frame_uniforms (frame.rs:549): the scene block for the following traces. */
+int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
+
+/* ---- traced passes ------------------------------------------------------------------------- */
+/* P::Gbuf (record.rs:327; gbuffer.comp:189-266, 470-480): one pinhole primary ray per pixel,
+ * camera in air, world only.  Writes hits [w*h], and when non-NULL gbPos (RGBA32F: P, t; sky
+ * (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5, a=255; sky 0).  Row 0 is the image bottom. */
+int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
+                     float *d_gb_pos, uint32_t *d_gb_normal);
+/* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map / foliage / grass
+ * extras): one exact sun ray per G-buffer hit from P + N*0.03 through the dry field (water clear).
+ * shadowRaw is RG16F (vis 0/1, min(tBlocker, 60000)); d_hits (nullable) gets the sun ray's hit. */
+int vf_trace_shadow(vf_ctx *ctx, int32_t width, int32_t height, const float *d_gb_pos,
+                    const uint32_t *d_gb_normal, uint32_t *d_shadow_raw, vf_hit *d_hits);
+/* P::Gi (record.rs:350/361; gi.comp:34-84 + giPath 1 bounce): cosine-hemisphere GI ray from the
+ * quantised G-buffer normal (R2 sequence, GI_LDS), its sun ray, sky escape via the sky-top map.
+ * half_res != 0: the reference's half-resolution dispatch ((w+1)/2 x (h+1)/2 outputs); else one
+ * ray per full-res pixel.  radiance: RGBA32F (L.rgb, 1; sky pixel 0).  d_bounce (nullable): the
+ * bounce ray's hit.  gi_reach: GI_REACH in voxels (r.gi.reach, default 48). */
+int vf_trace_gi(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, int32_t half_res,
+                float gi_reach, int32_t lights_nee, const float *d_gb_pos, const uint32_t *d_gb_normal,
+                float *d_radiance, vf_hit *d_bounce);
+/* traceVolumeS over the world for an explicit batch (common.glsl:541-658): skip_mat 0 (water is a
+ * hit) or 5 (water is clear, dry field).  t_cap is tCap (1e30 = none). */
+int vf_trace_rays(vf_ctx *ctx, const vf_ray *d_rays, int64_t n, float t_cap, uint32_t skip_mat, vf_hit *d_hits);
+/* probe_trace.comp ray source (probes.glsl:155-170) for a probe grid: n_rays spherical-Fibonacci
+ * directions per probe, rotated per frame; probe p's origin = origin + spacing*(p.x,p.y,p.z),
+ * p = (i % dims.x, i / dims.x % dims.y, i / dims.x / dims.y).  Hits [n_probes * n_rays]. */
+int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int32_t dims[3], int32_t n_rays,
+                    uint32_t frame, float reach, vf_hit *d_hits);
+
+/* ---- measurement --------------------------------------------------------------------------- */
+/* When on, the traces launch their counting variants, which add the reference visit sequence's
+ * fetch bytes (grid 4 B per brick step, 1 B per distance/dry/coarse/atlas fetch) to counters. */
+int vf_set_counting(vf_ctx *ctx, int32_t on);
+/* counters: {rays, brick steps, voxel steps, fetch bytes}; reset = 1 zeroes them after reading. */
+int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset);
+/* Kernel time of the last trace entry point on the context's stream, from HIP events (ms). */
+int vf_last_kernel_ms(vf_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

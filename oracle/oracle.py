@@ -1,0 +1,297 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, the smoke check and the CPU-baseline leg of bench.py as the checker; the product
+library (libvftrace.so) never imports, links or runs anything here.  See vf_oracle.h for what each
+function restates (reference file:line) and how it is pinned.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
+                      ("pad", "u1")])
+
+
+class VfoWorld(C.Structure):
+    _fields_ = [("wb", C.c_int32 * 3), ("win", C.c_int32 * 4), ("grid", C.c_void_p), ("atlas", C.c_void_p),
+                ("n_slots", C.c_uint32), ("dist", C.c_void_p), ("dry", C.c_void_p), ("coarse1", C.c_void_p),
+                ("coarse2", C.c_void_p), ("c1", C.c_int32 * 3), ("c2", C.c_int32 * 3), ("use_dist", C.c_int32),
+                ("coarse_skip", C.c_int32)]
+
+
+class VfoHit(C.Structure):
+    _fields_ = [("hit", C.c_int32), ("t", C.c_float), ("n", C.c_int32 * 3), ("mat", C.c_uint32),
+                ("cell", C.c_int32 * 3)]
+
+
+class VfoCount(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("outer", C.c_uint64), ("voxel", C.c_uint64), ("bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {"rays": self.rays, "outer": self.outer, "voxel": self.voxel, "bytes": self.bytes}
+
+
+class VfoCam(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3), ("fwd", C.c_float * 3),
+                ("tan_half_fov", C.c_float), ("jitter", C.c_float * 2), ("frame_idx", C.c_uint32)]
+
+
+class VfoSky(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("sun_rgb", C.c_float * 3), ("sky_zenith", C.c_float * 3),
+                ("sky_horizon", C.c_float * 3), ("sunset_rgb", C.c_float * 3), ("sky_scale", C.c_float),
+                ("gi_scale", C.c_float), ("night", C.c_float), ("sky_model", C.c_int32), ("gi_reach", C.c_float),
+                ("lights_nee", C.c_int32), ("palette", C.c_void_p), ("skytop", C.c_void_p)]
+
+
+_L = None
+
+
+def build():
+    """Compiles liboracle.so with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "vfo_trace": (None, [P, P, P, C.c_float, C.c_uint32, P, P]),
+            "vfo_trace_batch": (None, [P, P, C.c_int64, C.c_float, C.c_uint32, P, P, C.c_int]),
+            "vfo_march_f64": (None, [P, P, P, C.c_double, C.c_uint32, P, P, P, P, P]),
+            "vfo_raycast_cpu": (None, [P, P, P, C.c_float, C.c_uint32, P, P, P, P, P]),
+            "vfo_voxel": (C.c_uint32, [P, C.c_int32, C.c_int32, C.c_int32]),
+            "vfo_brick_mask": (None, [P, P]),
+            "vfo_coarse": (None, [P, P, P, P]),
+            "vfo_brickdist_pass": (None, [P, P, P, P, P, P, P, C.c_int32, C.c_int32]),
+            "vfo_record_distance_field": (None, [P, P, P, C.c_int32, P, P, P, P, P, P, P, P]),
+            "vfo_brickdist_brute": (None, [P, P, P, C.c_int32, C.c_int32, P]),
+            "vfo_skytop": (None, [P, P, P]),
+            "vfo_pcg": (C.c_uint32, [C.c_uint32]),
+            "vfo_pixel_seed": (C.c_uint32, [C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]),
+            "vfo_ld_sample2": (None, [C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, P]),
+            "vfo_cosine_hemisphere_u": (None, [P, P, P]),
+            "vfo_sin": (C.c_float, [C.c_float]),
+            "vfo_cos": (C.c_float, [C.c_float]),
+            "vfo_primary_dir": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]),
+            "vfo_probe_dir": (None, [C.c_int32, C.c_int32, C.c_uint32, P]),
+            "vfo_pass_primary": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, C.c_int]),
+            "vfo_pass_shadow": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
+            "vfo_pass_gi": (None, [P, P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
+            "vfo_gi_path": (None, [P, P, P, P, P, P, P, P]),
+            "vfo_sky_open": (C.c_float, [P, P, P]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _L = L
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleWorld:
+    """The oracle's own copy of a world: its grid and atlas plus the derived fields it builds itself
+    (coarse pyramid, both distance fields by record_distance_field, sky-top map)."""
+
+    def __init__(self, arrays, use_dist=True, coarse_skip=True, build_fields=True):
+        self.wb = tuple(int(x) for x in arrays["world_bricks"])
+        self.win = tuple(int(x) for x in arrays["world_win"])
+        self.grid = np.ascontiguousarray(arrays["grid"], dtype=np.uint32)
+        self.atlas = np.ascontiguousarray(arrays["atlas"], dtype=np.uint8)
+        if self.atlas.size == 0:
+            self.atlas = np.zeros(512, np.uint8)
+        self.atlas = np.where(self.atlas == 254, 0, self.atlas).astype(np.uint8)   # gpu_world.rs:58-67
+        self.palette = np.ascontiguousarray(arrays["palette"], dtype=np.float32).reshape(256, 12)
+        n = self.wb[0] * self.wb[1] * self.wb[2]
+        self.c1d = tuple((x + 3) // 4 for x in self.wb)
+        self.c2d = tuple((x + 15) // 16 for x in self.wb)
+        self.c1 = np.zeros(int(np.prod(self.c1d)), np.uint8)
+        self.c2 = np.zeros(int(np.prod(self.c2d)), np.uint8)
+        wb = (C.c_int32 * 3)(*self.wb)
+        lib().vfo_coarse(_p(self.grid), wb, _p(self.c1), _p(self.c2))
+        self.d = [np.zeros(n, np.uint8) for _ in range(4)]
+        if build_fields:
+            self.rebuild_fields()
+        self.s = VfoWorld()
+        self.s.wb[:] = self.wb
+        self.s.win[:] = self.win
+        self.s.grid = self.grid.ctypes.data
+        self.s.atlas = self.atlas.ctypes.data
+        self.s.n_slots = self.atlas.size // 512
+        self.s.dist = self.d[0].ctypes.data
+        self.s.dry = self.d[2].ctypes.data
+        self.s.coarse1 = self.c1.ctypes.data
+        self.s.coarse2 = self.c2.ctypes.data
+        self.s.c1[:] = self.c1d
+        self.s.c2[:] = self.c2d
+        self.s.use_dist = int(bool(use_dist))
+        self.s.coarse_skip = int(bool(coarse_skip))
+        self.skytop = np.zeros(self.wb[0] * 8 * self.wb[2] * 8, np.uint16)
+        kinds = np.ascontiguousarray(self.palette[:, 6])
+        lib().vfo_skytop(C.byref(self.s), _p(kinds), _p(self.skytop))
+
+    def ring(self):
+        return self.win[3] != 0
+
+    def full_box(self):
+        lo = [self.win[0] if self.ring() else 0, 0, self.win[2] if self.ring() else 0]
+        hi = [lo[0] + self.wb[0] - 1, self.wb[1] - 1, lo[2] + self.wb[2] - 1]
+        return lo, hi
+
+    def rebuild_fields(self, lo=None, hi=None, dlo=None, dhi=None):
+        flo, fhi = self.full_box()
+        lo = flo if lo is None else lo
+        hi = fhi if hi is None else hi
+        dlo = lo if dlo is None else dlo
+        dhi = hi if dhi is None else dhi
+        i3 = lambda v: (C.c_int32 * 3)(*[int(x) for x in v])
+        lib().vfo_record_distance_field(_p(self.grid), _p(self.atlas), i3(self.wb), int(self.ring()), i3(lo), i3(hi),
+                                        i3(dlo), i3(dhi), _p(self.d[0]), _p(self.d[1]), _p(self.d[2]),
+                                        _p(self.d[3]))
+
+    def voxel(self, x, y, z):
+        return int(lib().vfo_voxel(C.byref(self.s), int(x), int(y), int(z)))
+
+    def trace_rays(self, rays, t_cap=1e30, skip_mat=0, threads=0):
+        """rays: float32 [N, 6] (origin, direction) -> (hits, counters)."""
+        r = f32(rays).reshape(-1, 6)
+        out = np.zeros(r.shape[0], HIT_DTYPE)
+        cnt = VfoCount()
+        lib().vfo_trace_batch(C.byref(self.s), _p(r), r.shape[0], float(t_cap), int(skip_mat), _p(out), C.byref(cnt),
+                              int(threads))
+        return out, cnt.as_dict()
+
+    def trace_one(self, ro, rd, t_cap=1e30, skip_mat=0):
+        h = VfoHit()
+        cnt = VfoCount()
+        ro32, rd32 = f32(ro), f32(rd)
+        lib().vfo_trace(C.byref(self.s), _p(ro32), _p(rd32), float(t_cap), int(skip_mat), C.byref(h), C.byref(cnt))
+        return h, cnt.as_dict()
+
+    def march_f64(self, ro, rd, t_cap=1e30, skip_mat=0):
+        hit, t, mat = C.c_int32(), C.c_double(), C.c_uint32()
+        cell, n = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+        ro64 = np.ascontiguousarray(ro, dtype=np.float64)
+        rd64 = np.ascontiguousarray(rd, dtype=np.float64)
+        lib().vfo_march_f64(C.byref(self.s), _p(ro64), _p(rd64), float(t_cap), int(skip_mat), C.byref(hit),
+                            C.byref(t), cell, n, C.byref(mat))
+        return bool(hit.value), t.value, tuple(cell), tuple(n), mat.value
+
+    def raycast_cpu(self, ro, rd, max_t, skip_mat=0):
+        hit, t, mat = C.c_int32(), C.c_float(), C.c_uint32()
+        cell, n = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+        ro32, rd32 = f32(ro), f32(rd)
+        lib().vfo_raycast_cpu(C.byref(self.s), _p(ro32), _p(rd32), float(max_t), int(skip_mat), C.byref(hit),
+                              C.byref(t), cell, n, C.byref(mat))
+        return bool(hit.value), t.value, tuple(cell), tuple(n), mat.value
+
+    def pass_primary(self, cam, width, height, threads=0):
+        c = VfoCam()
+        c.pos[:] = [float(x) for x in cam["pos"]]
+        c.right[:] = [float(x) for x in cam["right"]]
+        c.up[:] = [float(x) for x in cam["up"]]
+        c.fwd[:] = [float(x) for x in cam["fwd"]]
+        c.tan_half_fov = float(cam["tan_half_fov"])
+        c.jitter[:] = [float(x) for x in cam.get("jitter", (0.0, 0.0))]
+        n = width * height
+        hits = np.zeros(n, HIT_DTYPE)
+        pos = np.zeros((n, 4), np.float32)
+        nrm = np.zeros(n, np.uint32)
+        cnt = VfoCount()
+        lib().vfo_pass_primary(C.byref(self.s), C.byref(c), int(width), int(height), _p(hits), _p(pos), _p(nrm),
+                               C.byref(cnt), int(threads))
+        return hits, pos, nrm, cnt.as_dict()
+
+    def pass_shadow(self, sun, width, height, gb_pos, gb_normal, threads=0):
+        n = width * height
+        sh = np.zeros((n, 2), np.float32)
+        hits = np.zeros(n, HIT_DTYPE)
+        cnt = VfoCount()
+        sun32, pos32, nrm = f32(sun), f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_pass_shadow(C.byref(self.s), _p(sun32), int(width), int(height), _p(pos32), _p(nrm), _p(sh),
+                              _p(hits), C.byref(cnt), int(threads))
+        return sh, hits, cnt.as_dict()
+
+    def sky(self, scene, gi_reach=48.0, lights_nee=True):
+        s = VfoSky()
+        for k in ("sun_dir", "sun_rgb", "sky_zenith", "sky_horizon", "sunset_rgb"):
+            getattr(s, k)[:] = [float(x) for x in getattr(scene, k)]
+        s.sky_scale, s.gi_scale, s.night = scene.sky_scale, scene.gi_scale, scene.night
+        s.sky_model = scene.sky_model
+        s.gi_reach = float(gi_reach)
+        s.lights_nee = int(bool(lights_nee))
+        s.palette = self.palette.ctypes.data
+        s.skytop = self.skytop.ctypes.data
+        self._sky_keep = s
+        return s
+
+    def pass_gi(self, scene, frame, width, height, gb_pos, gb_normal, half_res=True, gi_reach=48.0, lights_nee=True,
+                threads=0):
+        s = self.sky(scene, gi_reach, lights_nee)
+        ow, oh = ((width + 1) // 2, (height + 1) // 2) if half_res else (width, height)
+        rad = np.zeros((ow * oh, 4), np.float32)
+        bounce = np.zeros(ow * oh, HIT_DTYPE)
+        cnt = VfoCount()
+        pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_pass_gi(C.byref(self.s), C.byref(s), int(frame), int(width), int(height), int(bool(half_res)),
+                          _p(pos32), _p(nrm), _p(rad), _p(bounce), C.byref(cnt), int(threads))
+        return rad, bounce, cnt.as_dict()
+
+
+def pcg(v):
+    return int(lib().vfo_pcg(int(v) & 0xFFFFFFFF))
+
+
+def brick_mask(vox):
+    v = np.ascontiguousarray(vox, dtype=np.uint8).reshape(512)
+    out = np.zeros(16, np.uint32)
+    lib().vfo_brick_mask(_p(v), _p(out))
+    return out
+
+
+def brickdist_brute(grid, atlas, wb, ring, dry):
+    out = np.zeros(int(np.prod(wb)), np.uint8)
+    g = np.ascontiguousarray(grid, dtype=np.uint32)
+    a = np.ascontiguousarray(atlas, dtype=np.uint8)
+    if a.size == 0:
+        a = np.zeros(512, np.uint8)
+    lib().vfo_brickdist_brute(_p(g), _p(a), (C.c_int32 * 3)(*wb), int(bool(ring)), int(bool(dry)), _p(out))
+    return out
+
+
+def primary_dirs(cam, width, height):
+    out = np.zeros((height * width, 3), np.float32)
+    jit = np.zeros(2, np.float32)
+    L = lib()
+    R, U, F = f32(cam["right"]), f32(cam["up"]), f32(cam["fwd"])
+    tmp = np.zeros(3, np.float32)
+    for py in range(height):
+        for px in range(width):
+            L.vfo_primary_dir(_p(R), _p(U), _p(F), float(cam["tan_half_fov"]), width, height, px, py, _p(jit), _p(tmp))
+            out[py * width + px] = tmp
+    return out
+
+
+def probe_dirs(n_rays, frame):
+    out = np.zeros((n_rays, 3), np.float32)
+    tmp = np.zeros(3, np.float32)
+    for i in range(n_rays):
+        lib().vfo_probe_dir(i, n_rays, int(frame), _p(tmp))
+        out[i] = tmp
+    return out

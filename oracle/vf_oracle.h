@@ -1,0 +1,152 @@
+/*
+ * CPU oracle for the brick-DDA voxel traversal — TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of the reference's traversal (shaders/common.glsl
+ * traceVolumeS / brickVoxels), its distance-field build (shaders/brickdist.comp), its
+ * ray generators (gbuffer.comp, gi.comp, probes.glsl) and its CPU-side mirror builders
+ * (gpu_world.rs, sparse_volume.rs).  It exists only to check the HIP kernels: tests,
+ * the smoke check and the CPU-baseline leg of the benchmark may call it; the product
+ * library never links or loads it.
+ *
+ * Numerics: FP32 in the reference's operation order, compiled with -ffp-contract=off and
+ * IEEE division / square root, so that GPU results can be compared bit for bit.  GLSL's
+ * min/max/clamp are restated with the GLSL definitions (min(x,y) = y < x ? y : x), which
+ * matters for signed zeros.
+ *
+ * Pinning (see the tests under tests/):
+ *   - traversal: brute-force FP32 voxel DDA without skips (vfo_trace_noskip) and an
+ *     independent FP64 plain voxel march (vfo_march_f64); analytic scenes; the reference's
+ *     raycast_hits_floor case (sparse_volume.rs:669-679) through vfo_raycast_cpu.
+ *   - brick masks: the reference's mask_bits vector (sparse_volume.rs:660-668).
+ *   - distance field: the closed form min(16, L-inf distance) by brute force, incl. ring wrap.
+ *   - coarse pyramid and sky-top map: closed forms on small grids.
+ *   - RNG: pcgHash is a bijection check plus the published RXS-M-XS structure; the exact
+ *     values are "parity unpinned" beyond GPU-vs-oracle agreement (no reference vectors).
+ *   - GI radiance: estimator structure only (statistical), "parity unpinned" bitwise.
+ */
+#ifndef VF_ORACLE_H
+#define VF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One world as the traversal sees it (the GPU world mirror's textures, common.glsl:11-44). */
+typedef struct vfo_world {
+    int32_t wb[3];            /* worldBricks (SceneUbo@16) */
+    int32_t win[4];           /* worldWin (SceneUbo@352); .w != 0 -> ring (toroidal x/z) */
+    const uint32_t *grid;     /* gridTex R32UI, index (z*wby + y)*wbx + x (wrapped coords) */
+    const uint8_t *atlas;     /* slot-major: slot*512 + lz*64 + ly*8 + lx */
+    uint32_t n_slots;
+    const uint8_t *dist;      /* brickDistTex R8UI, grid layout */
+    const uint8_t *dry;       /* brickDistDryTex R8UI, grid layout */
+    const uint8_t *coarse1;   /* coarse1Tex R8UI, ceil(wb/4) */
+    const uint8_t *coarse2;   /* coarse2Tex R8UI, ceil(wb/16) */
+    int32_t c1[3], c2[3];
+    int32_t use_dist;         /* USE_DIST_FIELD (default on, cvars.rs:110) */
+    int32_t coarse_skip;      /* kCoarseSkip (common.glsl:20); 0 = plain brick-by-brick DDA */
+} vfo_world;
+
+typedef struct vfo_hit {
+    int32_t hit;
+    float t;
+    int32_t n[3];             /* face normal, one axis +-1, or all 0 (ray started inside the box) */
+    uint32_t mat;
+    int32_t cell[3];          /* absolute voxel coordinates */
+} vfo_hit;
+
+/* Fetch counters of the algorithmic byte model: grid 4 B per outer iteration, 1 B per
+ * distance / dry / coarse fetch, 1 B per atlas voxel fetch. */
+typedef struct vfo_count {
+    uint64_t rays, outer, voxel, bytes;
+} vfo_count;
+
+/* Explicit ray record (24 B) and hit record (20 B), the same as the C-ABI's vf_ray / vf_hit. */
+typedef struct vfo_ray { float o[3]; float d[3]; } vfo_ray;
+typedef struct vfo_hitrec { int32_t cell[3]; float t; uint8_t mat, face, hit, pad; } vfo_hitrec;
+
+/* --- traversal (common.glsl:474-658) --- */
+void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip_mat,
+               vfo_hit *out, vfo_count *cnt);
+void vfo_trace_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip_mat,
+                     vfo_hitrec *out, vfo_count *cnt, int threads);
+/* FP64 plain voxel march over the same voxels (independent pin for t). */
+void vfo_march_f64(const vfo_world *w, const double ro[3], const double rd[3], double tcap, uint32_t skip_mat,
+                   int32_t *hit, double *t, int32_t cell[3], int32_t n[3], uint32_t *mat);
+/* The reference's CPU raycast (sparse_volume.rs:573-637) over a world's voxels. */
+void vfo_raycast_cpu(const vfo_world *w, const float ro[3], const float rd[3], float max_t, uint32_t skip_mat,
+                     int32_t *hit, float *t, int32_t cell[3], int32_t n[3], uint32_t *mat);
+uint32_t vfo_voxel(const vfo_world *w, int32_t x, int32_t y, int32_t z);
+int32_t vfo_face_code(const int32_t n[3]);
+
+/* --- mirror builders (gpu_world.rs:300-313, sparse_volume.rs:48-63, brickdist.comp) --- */
+void vfo_brick_mask(const uint8_t vox[512], uint32_t out[16]);
+void vfo_coarse(const uint32_t *grid, const int32_t wb[3], uint8_t *c1, uint8_t *c2);
+/* One brickdist.comp dispatch (exact restatement).  src: grid (u32) for first_pass 1/2,
+ * else a u8 field.  dst: u8 field.  atlas: slot-major. */
+void vfo_brickdist_pass(const uint32_t *grid, const uint8_t *src_u8, uint8_t *dst, const uint8_t *atlas,
+                        const int32_t wb[3], const int32_t box_min[3], const int32_t box_max[3],
+                        int32_t axis, int32_t first_pass);
+/* record_distance_field (gpu_world.rs:1376-1428): 7 passes over 4 fields (d0 = normal, d2 = dry). */
+void vfo_record_distance_field(const uint32_t *grid, const uint8_t *atlas, const int32_t wb[3], int32_t ring,
+                               const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3],
+                               uint8_t *d0, uint8_t *d1, uint8_t *d2, uint8_t *d3);
+/* Closed form: min(16, L-inf distance to the nearest occupied brick), brute force. dry: water-only = empty. */
+void vfo_brickdist_brute(const uint32_t *grid, const uint8_t *atlas, const int32_t wb[3], int32_t ring, int32_t dry,
+                         uint8_t *out);
+/* Sky-top map (grasstop.comp:121-146): per voxel column, 1 + the highest voxel whose kind is not
+ * water (1) or foliage (3); 0 if none.  kinds[256] = palette props.z. */
+void vfo_skytop(const vfo_world *w, const float *kinds, uint16_t *out);
+
+/* --- RNG and ray generators (common.glsl:1233-1288, gbuffer.comp:194-202, probes.glsl:155-170) --- */
+uint32_t vfo_pcg(uint32_t v);
+uint32_t vfo_pixel_seed(int32_t px, int32_t py, uint32_t frame, uint32_t salt);
+void vfo_ld_sample2(int32_t px, int32_t py, uint32_t k, uint32_t salt, float out[2]);
+void vfo_cosine_hemisphere_u(const float n[3], const float u[2], float out[3]);
+float vfo_sin(float x);
+float vfo_cos(float x);
+void vfo_primary_dir(const float cam_right[3], const float cam_up[3], const float cam_fwd[3], float tan_half_fov,
+                     int32_t w, int32_t h, int32_t px, int32_t py, const float jitter[2], float out[3]);
+void vfo_probe_dir(int32_t i, int32_t n_rays, uint32_t frame, float out[3]);
+
+/* --- passes --- */
+typedef struct vfo_cam {
+    float pos[3], right[3], up[3], fwd[3];
+    float tan_half_fov;
+    float jitter[2];
+    uint32_t frame_idx;
+} vfo_cam;
+
+typedef struct vfo_sky {
+    float sun_dir[3], sun_rgb[3];
+    float sky_zenith[3], sky_horizon[3], sunset_rgb[3];
+    float sky_scale, gi_scale, night;
+    int32_t sky_model;
+    float gi_reach;
+    int32_t lights_nee;
+    const float *palette;     /* 256 x 12 floats (Material, 48 B) */
+    const uint16_t *skytop;   /* (wbx*8) x (wbz*8), index z*(wbx*8) + x */
+} vfo_sky;
+
+/* gbuffer.comp primary pass (world only, camera in air): hits + gbPos (xyz, t) + gbNormal (RGBA8). */
+void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height,
+                      vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads);
+/* shadow_trace.comp exact sun ray per pixel: shadowRaw as two floats (vis, min(tBlock, 60000)). */
+void vfo_pass_shadow(const vfo_world *w, const float sun_dir[3], int32_t width, int32_t height,
+                     const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
+                     vfo_count *cnt, int threads);
+/* gi.comp sampling + giPath (1 bounce): radiance rgb + the bounce ray's hit. half_res: the reference's
+ * half-resolution dispatch (gi.comp:34-84); else one ray per full-res pixel. */
+void vfo_pass_gi(const vfo_world *w, const vfo_sky *s, uint32_t frame_idx, int32_t width, int32_t height,
+                 int32_t half_res, const float *gb_pos, const uint32_t *gb_normal, float *radiance,
+                 vfo_hitrec *bounce, vfo_count *cnt, int threads);
+/* GI path from an explicit point/normal/direction (giPathImpl with lastStage 1). */
+void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                 float L[3], vfo_hit *bounce, vfo_count *cnt);
+float vfo_sky_open(const vfo_world *w, const vfo_sky *s, const float p[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+"""GPU parity of the traced passes against the CPU oracle, through the C ABI.
+
+Bar: hit records (cell, t, material, face, hit flag) bit-identical to the oracle's FP32 restatement
+of traceVolumeS / brickVoxels (reference shaders/common.glsl:474-658) on the same inputs; G-buffer
+position bits identical; shadowRaw halves identical; GI radiance within rtol 2e-4 / atol 2e-5 (the
+sky term goes through the device's powf / expf, which are not correctly rounded).
+"""
+import numpy as np
+import pytest
+import torch
+
+import vftrace as vf
+import oracle as O
+from helpers import camera, config1_camera, dense_world, hit_diff, random_rays, world
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vf.Context(0)
+    yield c
+    c.close()
+
+
+def dev_hits(n):
+    return torch.zeros(max(n, 1) * 20, dtype=torch.uint8, device=DEV)
+
+
+def upload(ctx, w, a):
+    ctx.upload(w)
+    scene = vf.default_scene(a["world_bricks"], a["world_win"])
+    ctx.set_scene(scene)
+    return scene
+
+
+def trace_rays_gpu(ctx, rays, t_cap=1e30, skip_mat=0):
+    r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(DEV)
+    h = dev_hits(len(rays))
+    ctx.trace_rays(r, len(rays), h, t_cap, skip_mat)
+    ctx.sync()
+    return vf.hits_to_numpy(h)[:len(rays)]
+
+
+def primary_gpu(ctx, pu, W, H):
+    hits = dev_hits(W * H)
+    gbp = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
+    gbn = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    ctx.trace_primary(pu, W, H, hits, gbp, gbn)
+    ctx.sync()
+    return hits, gbp, gbn
+
+
+def test_primary_config1_full_res(ctx):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    pu, cam = config1_camera()
+    W, H = 640, 360
+    hits, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    oh, opos, onrm, _ = ow.pass_primary(cam, W, H)
+    g = vf.hits_to_numpy(hits)
+    bad = hit_diff(g, oh)
+    assert bad.size == 0, f"{bad.size} primary hits differ, first {bad[:5]}: {g[bad[:3]]} vs {oh[bad[:3]]}"
+    assert 0.3 < g["hit"].mean() <= 1.0
+    assert np.array_equal(gbp.cpu().numpy().reshape(-1, 4).view(np.uint32), opos.view(np.uint32))
+    assert np.array_equal(gbn.cpu().numpy().view(np.uint32), onrm)
+
+
+def test_shadow_config1(ctx):
+    w, a, ow = world(1)
+    scene = upload(ctx, w, a)
+    pu, cam = config1_camera()
+    W, H = 640, 360
+    _, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    shh = dev_hits(W * H)
+    ctx.trace_shadow(W, H, gbp, gbn, sh, shh)
+    ctx.sync()
+    _, opos, onrm, _ = ow.pass_primary(cam, W, H)
+    osh, ohits, _ = ow.pass_shadow(np.array(scene.sun_dir, np.float32), W, H, opos, onrm)
+    g = vf.hits_to_numpy(shh)
+    bad = hit_diff(g, ohits)
+    assert bad.size == 0, f"{bad.size} shadow hits differ"
+    gsh = sh.cpu().numpy().view(np.uint32)
+    vis = (gsh & 0xFFFF).astype(np.uint16).view(np.float16)
+    tb = (gsh >> 16).astype(np.uint16).view(np.float16)
+    assert np.array_equal(vis, osh[:, 0].astype(np.float16))
+    assert np.array_equal(tb, osh[:, 1].astype(np.float16))
+    assert 0.0 < (vis == 0).mean() < 1.0
+
+
+@pytest.mark.parametrize("half_res,frame", [(True, 0), (True, 5), (False, 2)])
+def test_gi_config1(ctx, half_res, frame):
+    w, a, ow = world(1)
+    scene = upload(ctx, w, a)
+    pu, cam = config1_camera(frame)
+    W, H = 320, 180
+    _, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    OW, OH = ((W + 1) // 2, (H + 1) // 2) if half_res else (W, H)
+    rad = torch.zeros(OW * OH * 4, dtype=torch.float32, device=DEV)
+    bh = dev_hits(OW * OH)
+    ctx.trace_gi(pu, W, H, gbp, gbn, rad, bh, half_res=half_res, gi_reach=48.0)
+    ctx.sync()
+    _, opos, onrm, _ = ow.pass_primary(cam, W, H)
+    orad, obounce, _ = ow.pass_gi(scene, frame, W, H, opos, onrm, half_res=half_res, gi_reach=48.0)
+    g = vf.hits_to_numpy(bh)
+    bad = hit_diff(g, obounce)
+    assert bad.size == 0, f"{bad.size} GI bounce hits differ"
+    gr = rad.cpu().numpy().reshape(-1, 4)
+    np.testing.assert_allclose(gr, orad, rtol=2e-4, atol=2e-5)
+    assert gr[:, :3].max() > 0.0
+
+
+@pytest.mark.parametrize("skip_mat", [0, 5])
+def test_explicit_rays_config1(ctx, skip_mat):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    # origins inside and around the world box; 10 % with tiny / zero direction components
+    rays = random_rays(200_000, [-40, -20, -40], [296, 150, 296], seed=3 + skip_mat)
+    g = trace_rays_gpu(ctx, rays, 1e30, skip_mat)
+    oh, _ = ow.trace_rays(rays, 1e30, skip_mat)
+    bad = hit_diff(g, oh)
+    assert bad.size == 0, f"{bad.size} ray hits differ, first {bad[:4]}"
+
+
+@pytest.mark.parametrize("t_cap", [0.0, 5.0, 48.0, 640.0])
+def test_explicit_rays_tcap(ctx, t_cap):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    rays = random_rays(50_000, [0, 40, 0], [256, 128, 256], seed=11)
+    g = trace_rays_gpu(ctx, rays, t_cap, 0)
+    oh, _ = ow.trace_rays(rays, t_cap, 0)
+    assert hit_diff(g, oh).size == 0
+
+
+def test_counters_match_oracle(ctx):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    rays = random_rays(40_000, [0, 0, 0], [256, 128, 256], seed=5)
+    ctx.set_counting(True)
+    try:
+        ctx.counters(reset=True)
+        trace_rays_gpu(ctx, rays, 1e30, 5)
+        c = ctx.counters(reset=True)
+    finally:
+        ctx.set_counting(False)
+    _, oc = ow.trace_rays(rays, 1e30, 5)
+    assert c == oc
+
+
+def test_probe_rays(ctx):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    dims, nr, origin, spacing, frame = (4, 3, 4), 256, (10.5, 60.25, 12.75), 20.0, 7
+    h = dev_hits(int(np.prod(dims)) * nr)
+    ctx.trace_probes(origin, spacing, dims, nr, frame, 256.0, h)
+    ctx.sync()
+    g = vf.hits_to_numpy(h)[:int(np.prod(dims)) * nr]
+    dirs = O.probe_dirs(nr, frame)
+    rays = []
+    for p in range(int(np.prod(dims))):
+        px, py, pz = p % dims[0], (p // dims[0]) % dims[1], p // (dims[0] * dims[1])
+        o = np.array([origin[0] + np.float32(spacing) * np.float32(px), origin[1] + np.float32(spacing) * np.float32(py),
+                      origin[2] + np.float32(spacing) * np.float32(pz)], np.float32)
+        for i in range(nr):
+            rays.append(np.concatenate([o, dirs[i]]))
+    oh, _ = ow.trace_rays(np.array(rays, np.float32), 256.0, 5)
+    assert hit_diff(g, oh).size == 0
+
+
+def test_brickdist_and_skytop_match_oracle(ctx):
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    d, r = ctx.read_brickdist()
+    assert np.array_equal(d, ow.d[0]) and np.array_equal(r, ow.d[2])
+    assert np.array_equal(ctx.read_skytop(), ow.skytop)
+
+
+def test_brickdist_incremental_update(ctx):
+    """GpuWorld::sync-style edit: clear a box of bricks and fill another, rebuild dirty +- 16."""
+    w, a, _ = world(1)
+    a = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+    upload(ctx, w, a)
+    wb = a["world_bricks"]
+    grid = a["grid"]
+    idx, val = [], []
+    for z in range(10, 14):
+        for y in range(2, 6):
+            for x in range(5, 9):
+                i = (z * wb[1] + y) * wb[0] + x
+                idx.append(i); val.append(0)
+    for z in range(20, 22):
+        for y in range(12, 14):
+            for x in range(20, 23):
+                i = (z * wb[1] + y) * wb[0] + x
+                idx.append(i); val.append(9)
+    ctx.update(idx, val)
+    grid[np.array(idx)] = np.array(val, np.uint32)
+    lo, hi = [5 - 16, 2 - 16, 10 - 16], [22 + 16, 13 + 16, 21 + 16]
+    dlo, dhi = [5, 2, 10], [22, 13, 21]
+    ctx.build_brickdist(lo, hi, dlo, dhi)
+    d, r = ctx.read_brickdist()
+    full = O.OracleWorld(a)
+    assert np.array_equal(d, full.d[0])
+    assert np.array_equal(r, full.d[2])
+    # the traversal sees the edit
+    rays = random_rays(30_000, [0, 0, 0], [256, 128, 256], seed=9)
+    g = trace_rays_gpu(ctx, rays, 1e30, 0)
+    oh, _ = full.trace_rays(rays, 1e30, 0)
+    assert hit_diff(g, oh).size == 0
+
+
+def test_ring_world_primary_and_rays(ctx):
+    """Endless-world shape in miniature: 3x3 chunks in a 4x4-chunk ring grid (negative absolute bricks)."""
+    w, a, ow = world(2, world_bricks=(64, 32, 64), chunks=3, trees_per_chunk=60)
+    assert a["world_win"][3] == 1 and a["world_win"][0] < 0
+    upload(ctx, w, a)
+    gy = w.ground(-37, 21)
+    pu, cam = camera([-37.4, gy + 17.0, 21.6], 2.1, -0.12)
+    W, H = 320, 200
+    hits, gbp, _ = primary_gpu(ctx, pu, W, H)
+    oh, opos, _, _ = ow.pass_primary(cam, W, H)
+    g = vf.hits_to_numpy(hits)
+    assert hit_diff(g, oh).size == 0
+    assert np.array_equal(gbp.cpu().numpy().reshape(-1, 4).view(np.uint32), opos.view(np.uint32))
+    win = a["world_win"]
+    lo = [win[0] * 8 - 30, -10, win[2] * 8 - 30]
+    hi = [(win[0] + 64) * 8 + 30, 266, (win[2] + 64) * 8 + 30]
+    rays = random_rays(100_000, lo, hi, seed=21)
+    for skip in (0, 5):
+        assert hit_diff(trace_rays_gpu(ctx, rays, 1e30, skip), ow.trace_rays(rays, 1e30, skip)[0]).size == 0
+    d, r = ctx.read_brickdist()
+    assert np.array_equal(d, ow.d[0]) and np.array_equal(r, ow.d[2])
+
+
+def test_analytic_scenes(ctx):
+    """Empty world, one voxel, a water slab over a floor (skipMat semantics), a full solid world."""
+    dims = (64, 32, 48)   # z, y, x
+    rays = random_rays(20_000, [-8, -8, -8], [56, 40, 72], seed=4)
+    empty = np.zeros(dims, np.uint8)
+    one = empty.copy(); one[20, 13, 30] = 7
+    water = empty.copy(); water[:, 0:8, :] = 3; water[:, 8:14, :] = 5
+    solid = np.full(dims, 2, np.uint8)
+    for vox in (empty, one, water, solid):
+        w, a, ow = dense_world(vox)
+        upload(ctx, w, a)
+        for skip in (0, 5):
+            g = trace_rays_gpu(ctx, rays, 1e30, skip)
+            oh, _ = ow.trace_rays(rays, 1e30, skip)
+            assert hit_diff(g, oh).size == 0
+    # the water slab: a ray straight down stops at the water with skipMat 0, at the floor with 5
+    w, a, ow = dense_world(water)
+    upload(ctx, w, a)
+    down = np.array([[10.5, 30.0, 10.5, 0.0, -1.0, 0.0]], np.float32)
+    g0 = trace_rays_gpu(ctx, down, 1e30, 0)[0]
+    g5 = trace_rays_gpu(ctx, down, 1e30, 5)[0]
+    assert g0["hit"] and g0["mat"] == 5 and tuple(g0["cell"]) == (10, 13, 10) and g0["face"] == 3
+    assert g5["hit"] and g5["mat"] == 3 and tuple(g5["cell"]) == (10, 7, 10) and g5["face"] == 3
+    assert g0["t"] == np.float32(16.0) and g5["t"] == np.float32(22.0)

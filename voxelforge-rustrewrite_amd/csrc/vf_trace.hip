@@ -1,0 +1,1556 @@
+// MI355X (gfx950) brick-DDA voxel traversal: HIP kernels and the C ABI of include/vf_trace.h.
+//
+// The traversal restates traceVolumeS / brickVoxels (reference shaders/common.glsl:474-658) with
+// the same FP32 operation order (built with -ffp-contract=off, IEEE division and square root),
+// so hits match the reference algorithm bit for bit.  What changes is how the data reaches the
+// arithmetic:
+//   * one 8-byte "cell word" per brick packs the grid value (gridTex), both distance fields
+//     (brickDistTex, brickDistDryTex) and the two coarse-pyramid flags of the brick's cells
+//     (coarse1Tex, coarse2Tex), so an outer DDA step is ONE load instead of the reference's
+//     dependent chain grid -> dist -> coarse2 -> coarse1.  Cell words are stored in 4x4x4-brick
+//     tiles (512 B) so a ray's neighbouring bricks share cache lines.
+//   * each detailed brick has two 512-bit occupancy masks (m != 0, and m not in {0, water}),
+//     fetched as four 16-byte loads when the ray enters the brick; the voxel DDA then tests bits
+//     in registers and touches the 512-byte atlas brick once, for the hit's material.
+//   * 256-thread workgroups trace 16x16 pixel tiles (one 8x8 tile per wavefront), and workgroup
+//     ids are remapped so that each of the 8 XCDs (each with its own L2) gets a contiguous band
+//     of tiles.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vf_trace.h"
+#include "vf_internal.h"
+
+// ------------------------------------------------------------------------------------------------
+// device-side world
+// ------------------------------------------------------------------------------------------------
+struct DevWorld {
+    int32_t wb[3];
+    int32_t minb[3];     // worldMinB(): (win.x, 0, win.z) in ring mode, else 0
+    int32_t ring;
+    int32_t use_dist;    // USE_DIST_FIELD
+    int32_t tdim[3];     // 4^3-brick tiles per axis of the cell-word array
+    int32_t max_outer;   // 3 * (wbx + wby + wbz) + 4
+    const uint2 *cells;  // .x grid value, .y meta: dist | dry << 8 | c2empty << 16 | c1empty << 17
+    const uint8_t *atlas;
+    const uint4 *occ;     // per slot 4 x uint4: bit (lz*64 + ly*8 + lx) = voxel != 0
+    const uint4 *occdry;  // per slot: voxel != 0 && voxel != 5
+    const uint16_t *skytop;
+    const float4 *palette;  // 256 x 3 float4 (Material)
+};
+
+struct SkyParams {
+    float sun_dir[3], sun_rgb[3], sky_zenith[3], sky_horizon[3], sunset_rgb[3];
+    float sky_scale, gi_scale, night;
+    int32_t sky_model;
+};
+
+struct Cnt {
+    unsigned long long rays, outer, voxel, bytes;
+};
+
+struct Hit {
+    int32_t hit;
+    float t;
+    int32_t face;
+    uint32_t mat;
+    int32_t cell[3];
+};
+
+// GLSL built-ins with their specified definitions (signed zeros matter for bit equality).
+__device__ __forceinline__ float gmin(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ int32_t imin(int32_t x, int32_t y) { return y < x ? y : x; }
+__device__ __forceinline__ int32_t imax(int32_t x, int32_t y) { return x < y ? y : x; }
+__device__ __forceinline__ int32_t iclamp(int32_t x, int32_t lo, int32_t hi) { return imin(imax(x, lo), hi); }
+__device__ __forceinline__ float fclampg(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__device__ __forceinline__ int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
+    float x = a[1] * b[2] - b[1] * a[2];
+    float y = a[2] * b[0] - b[2] * a[0];
+    float z = a[0] * b[1] - b[0] * a[1];
+    o[0] = x; o[1] = y; o[2] = z;
+}
+__device__ __forceinline__ void normalize3(float v[3]) {
+    float len = sqrtf(dot3(v, v));
+    v[0] = v[0] / len; v[1] = v[1] / len; v[2] = v[2] / len;
+}
+
+// Cell word index of absolute brick (x, y, z) inside the window (gridWrap, common.glsl:134-136).
+__device__ __forceinline__ uint32_t cell_index(const DevWorld &W, int32_t x, int32_t y, int32_t z) {
+    if (W.ring) { x &= W.wb[0] - 1; z &= W.wb[2] - 1; }
+    uint32_t tile = ((uint32_t)(z >> 2) * (uint32_t)W.tdim[1] + (uint32_t)(y >> 2)) * (uint32_t)W.tdim[0] + (uint32_t)(x >> 2);
+    return tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
+}
+
+// 64 occupancy bits of brick layer lz (two 32-bit words) out of the 512-bit mask.
+__device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, const uint4 &m2, const uint4 &m3, int lz,
+                                           uint32_t &lo, uint32_t &hi) {
+    uint4 a = (lz & 2) ? m1 : m0;
+    uint4 b = (lz & 2) ? m3 : m2;
+    uint4 q = (lz & 4) ? b : a;
+    lo = (lz & 1) ? q.z : q.x;
+    hi = (lz & 1) ? q.w : q.y;
+}
+
+// brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
+// arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
+template <int SKIP, bool COUNT>
+__device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
+                                             const int32_t st[3], const float up[3], const int32_t faceA[3],
+                                             const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c) {
+    const uint32_t slot = g - 256u;
+    const uint4 *mp = (SKIP == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
+    const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
+    int32_t bmin[3], vc[3];
+    float tM[3], tD[3];
+    const float e = t + 1e-3f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bmin[a] = bc[a] * 8;
+        float q = ro[a] + sd[a] * e;
+        vc[a] = iclamp(f2i(floorf(q)), bmin[a], bmin[a] + 7);
+        tM[a] = (((float)vc[a] + up[a]) - ro[a]) * inv[a];
+        tD[a] = fabsf(inv[a]);
+    }
+    int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
+    uint32_t wlo, whi;
+    mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+    float tv = t;
+    int32_t f2 = face;
+    for (int j = 0; j < 26; ++j) {
+        if (COUNT) { c.voxel++; c.bytes += 1; }
+        const uint32_t wv = (ly & 4) ? whi : wlo;
+        if ((wv >> (((ly & 3) << 3) | lx)) & 1u) {
+            h.hit = 1;
+            h.t = tv;
+            h.face = f2;
+            h.mat = W.atlas[(size_t)slot * 512u + (uint32_t)(lz * 64 + ly * 8 + lx)];
+            h.cell[0] = bmin[0] + lx; h.cell[1] = bmin[1] + ly; h.cell[2] = bmin[2] + lz;
+            return true;
+        }
+        if (tM[0] < tM[1] && tM[0] < tM[2]) {
+            lx += st[0]; tv = tM[0]; tM[0] += tD[0]; f2 = faceA[0];
+            if ((uint32_t)lx > 7u) break;
+        } else if (tM[1] < tM[2]) {
+            ly += st[1]; tv = tM[1]; tM[1] += tD[1]; f2 = faceA[1];
+            if ((uint32_t)ly > 7u) break;
+        } else {
+            lz += st[2]; tv = tM[2]; tM[2] += tD[2]; f2 = faceA[2];
+            if ((uint32_t)lz > 7u) break;
+            mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+        }
+    }
+    return false;
+}
+
+// traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5).
+template <int SKIP, bool COUNT>
+__device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
+                                            Cnt &c) {
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+    if (COUNT) c.rays++;
+    float sd[3], inv[3], up[3], tmin3[3], tmax3[3];
+    int32_t st[3], faceA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        inv[a] = 1.0f / sd[a];
+        const float lo = (float)(W.minb[a] * 8);
+        const float dims = (float)(W.wb[a] * 8);
+        const float t0 = (lo - ro[a]) * inv[a];
+        const float t1 = ((lo + dims) - ro[a]) * inv[a];
+        tmin3[a] = gmin(t0, t1);
+        tmax3[a] = gmax(t0, t1);
+        st[a] = sd[a] > 0.0f ? 1 : -1;
+        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
+        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);   // face code of normal -st on axis a
+    }
+    const float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
+    const float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
+    if (tExit < gmax(tEnter, 0.0f) || tEnter > tcap) return;
+
+    float t = gmax(tEnter, 0.0f);
+    int32_t bc[3];
+    float tD8[3], tM8[3];
+    {
+        const float e = t + 1e-3f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float entry = ro[a] + sd[a] * e;
+            bc[a] = iclamp(f2i(floorf(entry * 0.125f)), W.minb[a], W.minb[a] + W.wb[a] - 1);
+            tD8[a] = fabsf(inv[a]) * 8.0f;
+            tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
+        }
+    }
+    int32_t face = 0;
+    if (tEnter > 0.0f) {
+        if (tEnter == tmin3[0]) face = faceA[0];
+        else if (tEnter == tmin3[1]) face = faceA[1];
+        else face = faceA[2];
+    }
+    for (int i = 0; i < W.max_outer; ++i) {
+        if (t > tcap) return;
+        const uint2 cw = W.cells[cell_index(W, bc[0], bc[1], bc[2])];
+        uint32_t g = cw.x;
+        const uint32_t meta = cw.y;
+        if (COUNT) { c.outer++; c.bytes += 4; }
+        if (SKIP == 5) {
+            if (g == 5u) g = 0u;
+            else if (g >= 256u) {
+                if (COUNT) c.bytes += 1;
+                if (((meta >> 8) & 255u) > 0u) g = 0u;
+            }
+        }
+        if (g >= 256u) {
+            if (brick_voxels<SKIP, COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c)) return;
+        } else if (g != 0u && g != (uint32_t)SKIP) {
+            const float e = t + 1e-3f;
+            h.hit = 1; h.t = t; h.face = face; h.mat = g;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) h.cell[a] = f2i(floorf(ro[a] + sd[a] * e));
+            return;
+        }
+        bool stepped = false;
+        if (g == 0u) {
+            int32_t Cf = 0, cb[3] = {0, 0, 0};
+            int32_t d = 0;
+            if (W.use_dist) {
+                d = (int32_t)(SKIP == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
+                if (COUNT) c.bytes += 1;
+            }
+            if (W.use_dist && d > 1) {
+                Cf = 2 * d - 1;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (d - 1);
+            } else {
+                if (COUNT) c.bytes += 1;
+                if (meta & (1u << 16)) Cf = 16;
+                else {
+                    if (COUNT) c.bytes += 1;
+                    Cf = (meta & (1u << 17)) ? 4 : 0;
+                }
+                if (Cf > 0) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (bc[a] & (Cf - 1));
+                }
+            }
+            if (Cf > 0) {
+                float tFar[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float cmin = (float)cb[a] * 8.0f;
+                    const float cmax = cmin + (float)Cf * 8.0f;
+                    tFar[a] = ((up[a] != 0.0f ? cmax : cmin) - ro[a]) * inv[a];
+                }
+                const bool ax0 = tFar[0] < tFar[1] && tFar[0] < tFar[2];
+                const bool ax1 = !ax0 && tFar[1] < tFar[2];
+                const bool ax2 = !ax0 && !ax1;
+                t = ax0 ? tFar[0] : (ax1 ? tFar[1] : tFar[2]);
+                face = ax0 ? faceA[0] : (ax1 ? faceA[1] : faceA[2]);
+                const float e = t + 1e-3f;
+                const bool axs[3] = {ax0, ax1, ax2};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    int32_t v = iclamp(f2i(floorf((ro[a] + sd[a] * e) * 0.125f)), cb[a], cb[a] + Cf - 1);
+                    if (axs[a]) v = st[a] > 0 ? cb[a] + Cf : cb[a] - 1;
+                    bc[a] = v;
+                    tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
+                }
+                stepped = true;
+            }
+        }
+        if (!stepped) {
+            if (tM8[0] < tM8[1] && tM8[0] < tM8[2]) {
+                bc[0] += st[0]; t = tM8[0]; tM8[0] += tD8[0]; face = faceA[0];
+            } else if (tM8[1] < tM8[2]) {
+                bc[1] += st[1]; t = tM8[1]; tM8[1] += tD8[1]; face = faceA[1];
+            } else {
+                bc[2] += st[2]; t = tM8[2]; tM8[2] += tD8[2]; face = faceA[2];
+            }
+        }
+        if (bc[0] < W.minb[0] || bc[1] < W.minb[1] || bc[2] < W.minb[2] || bc[0] >= W.minb[0] + W.wb[0] ||
+            bc[1] >= W.minb[1] + W.wb[1] || bc[2] >= W.minb[2] + W.wb[2])
+            return;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// shared helpers of the passes
+// ------------------------------------------------------------------------------------------------
+// Workgroup id -> logical tile so that the 8 XCDs (workgroups are dealt round-robin) each get a
+// contiguous run of tiles and keep their L2 on one region of the world.
+__device__ __forceinline__ int32_t xcd_remap(int32_t i, int32_t n) {
+    const int32_t q = n >> 3, r = n & 7, x = i & 7;
+    return x * q + imin(x, r) + (i >> 3);
+}
+
+__device__ __forceinline__ void store_hit(vf_hit *out, const Hit &h) {
+    uint32_t *o = reinterpret_cast<uint32_t *>(out);
+    if (h.hit) {
+        o[0] = (uint32_t)h.cell[0]; o[1] = (uint32_t)h.cell[1]; o[2] = (uint32_t)h.cell[2];
+        o[3] = __float_as_uint(h.t);
+        o[4] = (h.mat & 255u) | ((uint32_t)h.face << 8) | (1u << 16);
+    } else {
+        o[0] = 0u; o[1] = 0u; o[2] = 0u;
+        o[3] = __float_as_uint(-1.0f);
+        o[4] = 0u;
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_count(const Cnt &c, unsigned long long *cnt) {
+    if (COUNT) {
+        atomicAdd(cnt + 0, c.rays);
+        atomicAdd(cnt + 1, c.outer);
+        atomicAdd(cnt + 2, c.voxel);
+        atomicAdd(cnt + 3, c.bytes);
+    }
+}
+
+__device__ __forceinline__ uint32_t unorm8(float v) { return (uint32_t)floorf(fclampg(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
+__device__ __forceinline__ float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
+
+__device__ __forceinline__ void face_normal(int32_t face, float n[3]) {
+    n[0] = n[1] = n[2] = 0.0f;
+    if (face > 0) n[(face - 1) >> 1] = (face & 1) ? 1.0f : -1.0f;
+}
+
+// PCG hash and samplers (common.glsl:1233-1288)
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    v = v * 747796405u + 2891336453u;
+    v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
+    return (v >> 22u) ^ v;
+}
+__device__ __forceinline__ float fractg(float x) { return x - floorf(x); }
+
+// sin / cos by Cody-Waite reduction with pi/2 = 1.5703125 + 4.838267948966e-4 and Taylor
+// polynomials on [-pi/4, pi/4]; deterministic, so GI and probe directions are reproducible.
+__device__ __forceinline__ void sincos_red(float x, float &s, float &c) {
+    const float kf = floorf(x * 0.63661977f + 0.5f);
+    const int32_t k = f2i(kf);
+    const float r = (x - kf * 1.5703125f) - kf * 4.838267948966e-4f;
+    const float r2 = r * r;
+    const float ps = r + r * r2 * (-0.16666667f + r2 * (0.0083333333f + r2 * (-1.9841270e-4f + r2 * 2.7557319e-6f)));
+    const float pc = 1.0f + r2 * (-0.5f + r2 * (0.041666667f + r2 * (-1.3888889e-3f + r2 * 2.4801587e-5f)));
+    switch (k & 3) {
+    case 0: s = ps; c = pc; break;
+    case 1: s = pc; c = -ps; break;
+    case 2: s = -ps; c = -pc; break;
+    default: s = -pc; c = ps; break;
+    }
+}
+
+__device__ __forceinline__ void cosine_hemisphere_u(const float n[3], float u0, float u1, float out[3]) {
+    float t1[3], t2[3];
+    const float upv[3] = {0.0f, 1.0f, 0.0f}, uxv[3] = {1.0f, 0.0f, 0.0f};
+    cross3(n, fabsf(n[1]) < 0.9f ? upv : uxv, t1);
+    normalize3(t1);
+    cross3(n, t1, t2);
+    const float r = sqrtf(u0);
+    const float ph = 6.2831853f * u1;
+    float s, c;
+    sincos_red(ph, s, c);
+    const float a = r * c, b = r * s, z = sqrtf(1.0f - u0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = (t1[i] * a + t2[i] * b) + n[i] * z;
+    normalize3(out);
+}
+
+// ------------------------------------------------------------------------------------------------
+// pass kernels
+// ------------------------------------------------------------------------------------------------
+struct PrimArgs {
+    float pos[3], right[3], up[3], fwd[3];
+    float thf;
+    float jit[2];
+    int32_t w, h, tiles_x, ntiles;
+};
+
+// gbuffer.comp primary ray (gbuffer.comp:194-202, 260-266, 470-480) for a 16x16 tile per workgroup.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn,
+                                                 unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.w || py >= A.h) return;
+    const float sx = ((float)px + 0.5f) + A.jit[0];
+    const float sy = ((float)py + 0.5f) + A.jit[1];
+    const float nx = (sx / (float)A.w) * 2.0f - 1.0f;
+    const float ny = (sy / (float)A.h) * 2.0f - 1.0f;
+    const float aspect = (float)A.w / (float)A.h;
+    const float a = (nx * aspect) * A.thf;
+    const float b = ny * A.thf;
+    float rd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rd[i] = (A.fwd[i] + A.right[i] * a) + A.up[i] * b;
+    normalize3(rd);
+    Cnt c = {0, 0, 0, 0};
+    Hit h;
+    trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
+    h.t = h.t + 0.0f;   // h.t += tSkip (0: camera in air)
+    const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
+    store_hit(hits + i, h);
+    if (gbpos) {
+        float4 p;
+        if (h.hit) p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
+        else p = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        gbpos[i] = p;
+    }
+    if (gbn) {
+        uint32_t v = 0u;
+        if (h.hit) {
+            float n[3];
+            face_normal(h.face, n);
+            v = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
+                (255u << 24);
+        }
+        gbn[i] = v;
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
+struct TileArgs {
+    int32_t w, h, tiles_x, ntiles;
+};
+
+// shadow_trace.comp exact sun ray (shadow_trace.comp:17-34, 87, 101-105).
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx, float sy, float sz, const float4 *gbpos,
+                                                const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
+                                                unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.w || py >= A.h) return;
+    const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
+    const float4 pt = gbpos[i];
+    const float sun[3] = {sx, sy, sz};
+    Cnt c = {0, 0, 0, 0};
+    Hit h;
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+    float vis, tb;
+    if (pt.w < 0.0f) {
+        vis = 1.0f; tb = 0.0f;
+    } else {
+        const uint32_t nb = gbn[i];
+        const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+        if (dot3(N, sun) <= 0.0f) {
+            vis = 0.0f; tb = 0.0f;
+        } else {
+            const float so[3] = {pt.x + N[0] * 3e-2f, pt.y + N[1] * 3e-2f, pt.z + N[2] * 3e-2f};
+            trace_world<5, COUNT>(W, so, sun, 1e30f, h, c);
+            vis = h.hit ? 0.0f : 1.0f;
+            tb = h.hit ? h.t : 0.0f;
+        }
+    }
+    const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
+    const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
+    shadow[i] = hv | (ht << 16);
+    if (hits) store_hit(hits + i, h);
+    flush_count<COUNT>(c, cnt);
+}
+
+// skyOpen with colLoaded (common.glsl:232-258); a bounded world has no horizon (outside -> 1).
+__device__ __forceinline__ float sky_open(const DevWorld &W, const float p[3]) {
+    if (p[1] < 0.0f) return 0.0f;
+    const int32_t bx = f2i(floorf(p[0] / 8.0f)), bz = f2i(floorf(p[2] / 8.0f));
+    const bool inside = bx >= W.minb[0] && bz >= W.minb[2] && bx < W.minb[0] + W.wb[0] && bz < W.minb[2] + W.wb[2];
+    if (!inside || W.cells[cell_index(W, bx, 0, bz)].x == 0u) return 1.0f;
+    const int32_t sxn = W.wb[0] * 8, szn = W.wb[2] * 8;
+    const int32_t cx = f2i(floorf(p[0])), cz = f2i(floorf(p[2]));
+    const int32_t ix = ((cx % sxn) + sxn) % sxn, iz = ((cz % szn) + szn) % szn;
+    const float top = (float)W.skytop[(size_t)iz * (size_t)sxn + (size_t)ix];
+    const float e0 = top - 0.5f, e1 = top + 1.0f;
+    const float tt = fclampg((p[1] - e0) / (e1 - e0), 0.0f, 1.0f);
+    return tt * tt * (3.0f - 2.0f * tt);
+}
+
+// giEscape (common.glsl:2131-2148)
+__device__ float gi_escape(const DevWorld &W, const float o[3], const float d[3], float reach) {
+    float lo[3], hi[3], tf[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = (float)(W.minb[a] * 8);
+        hi[a] = (float)((W.minb[a] + W.wb[a]) * 8);
+        const float s = fabsf(d[a]) < 1e-6f ? 1e-6f : d[a];
+        const float t1 = (lo[a] - o[a]) / s, t2 = (hi[a] - o[a]) / s;
+        tf[a] = gmax(t1, t2);
+    }
+    const float tExit = gmax(gmin(gmin(tf[0], tf[1]), tf[2]), 0.0f);
+    if (tExit > reach) {
+        float e[3], e1[3], e2[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) e[a] = o[a] + d[a] * reach;
+        float so = sky_open(W, e);
+        if (so > 0.0f) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { e1[a] = e[a] + d[a] * 16.0f; e2[a] = e[a] + d[a] * 32.0f; }
+            so *= sky_open(W, e1) * sky_open(W, e2);
+        }
+        return so;
+    }
+    float f[3], f2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = o[a] + d[a] * tExit;
+    if (f[1] <= lo[1] + 1.0f) return 0.0f;
+    if (f[1] >= hi[1] - 1.0f) return 1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f2[a] = f[a] + d[a] * 2.0f;
+    return sky_open(W, f2);
+}
+
+// skyColor(rd, sunDisk 0) (common.glsl:1082-1099)
+__device__ void sky_color0(const SkyParams &S, const float rd[3], float out[3]) {
+    const float horizon = fclampg(rd[1] * 0.5f + 0.5f, 0.0f, 1.0f);
+    const float mu = gmax(dot3(rd, S.sun_dir), 0.0f);
+    const float glow = powf(mu, 8.0f) * (1.0f - 0.9f * S.night);
+    const float ph = powf(horizon, 0.7f);
+    const float tint[3] = {1.0f, 0.9f, 0.7f};
+    if (S.sky_model == 0) {
+        const float a[3] = {0.75f, 0.82f, 0.90f}, b[3] = {0.25f, 0.45f, 0.85f}, nt[3] = {0.012f, 0.02f, 0.045f};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float sky = a[i] * (1.0f - ph) + b[i] * ph;
+            sky = sky * (1.0f - S.night) + nt[i] * S.night;
+            out[i] = (sky + tint[i] * (glow * 0.25f) * (1.0f - 0.7f * S.night)) * S.sky_scale;
+        }
+        return;
+    }
+    const float m = gmax(gmax(S.sun_rgb[0], S.sun_rgb[1]), 1e-3f);
+    const float k1 = powf(mu, 3.0f) * expf(-gmax(rd[1], 0.0f) * 5.0f) + 0.25f * expf(-fabsf(rd[1]) * 9.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float sky = S.sky_horizon[i] * (1.0f - ph) + S.sky_zenith[i] * ph;
+        sky += S.sunset_rgb[i] * k1;
+        out[i] = (sky + (S.sun_rgb[i] / m) * tint[i] * (glow * 0.25f) * (1.0f - 0.7f * S.night)) * S.sky_scale;
+    }
+}
+
+struct GiArgs {
+    int32_t w, h, ow, oh, tiles_x, ntiles, half_res, lights_nee;
+    uint32_t frame;
+    float reach;
+};
+
+// gi.comp (gi.comp:34-84) + giPathImpl with one bounce (common.glsl:2250-2358).
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, const float4 *gbpos, const uint32_t *gbn,
+                                            float4 *radiance, vf_hit *bounce, unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.ow || py >= A.oh) return;
+    const size_t oi = (size_t)py * (size_t)A.ow + (size_t)px;
+    int32_t fx = px, fy = py;
+    bool sky = false;
+    if (A.half_res) {
+        const int32_t ox = (int32_t)(A.frame & 1u), oy = (int32_t)((A.frame >> 1u) & 1u);
+        fx = imin(px * 2 + ox, A.w - 1);
+        fy = imin(py * 2 + oy, A.h - 1);
+        int32_t gx = imin(px * 2, A.w - 1), gy = imin(py * 2, A.h - 1);
+        for (int k = 0; k < 4; ++k) {
+            const int32_t qx = imin(px * 2 + (k & 1), A.w - 1), qy = imin(py * 2 + (k >> 1), A.h - 1);
+            if (gbpos[(size_t)qy * A.w + qx].w >= 0.0f) { gx = qx; gy = qy; break; }
+        }
+        if (gbpos[(size_t)gy * A.w + gx].w < 0.0f) sky = true;
+        else if (gbpos[(size_t)fy * A.w + fx].w < 0.0f) { fx = gx; fy = gy; }
+    } else if (gbpos[(size_t)fy * A.w + fx].w < 0.0f) {
+        sky = true;
+    }
+    Cnt c = {0, 0, 0, 0};
+    Hit bh;
+    bh.hit = 0; bh.t = -1.0f; bh.face = 0; bh.mat = 0; bh.cell[0] = bh.cell[1] = bh.cell[2] = 0;
+    float Lr[3] = {0.0f, 0.0f, 0.0f};
+    if (!sky) {
+        const size_t fi = (size_t)fy * A.w + fx;
+        const float4 P4 = gbpos[fi];
+        const uint32_t nb = gbn[fi];
+        const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+        // ldSample2(fpix, frame >> 2, 0x68e31da4) (common.glsl:1284-1288)
+        const uint32_t salt = 0x68e31da4u;
+        const uint32_t hh = pcg((uint32_t)fx + pcg((uint32_t)fy + salt));
+        const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
+        const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
+        const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
+        const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
+        float rdir[3], ro[3];
+        cosine_hemisphere_u(N, u0, u1, rdir);
+        const float P[3] = {P4.x, P4.y, P4.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ro[a] = P[a] + N[a] * 3e-2f;
+        float hitAlb[3] = {0.0f, 0.0f, 0.0f}, hitNdl = 0.0f, reach = A.reach;
+        int stage = 0;
+        bool extended = false;
+        for (int it = 0; it < 6; ++it) {
+            const bool sunRay = (stage & 1) == 1;
+            Hit h;
+            trace_world<0, COUNT>(W, ro, rdir, sunRay ? 1e30f : reach, h, c);
+            if (sunRay) {
+                if (!h.hit) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) Lr[a] += hitAlb[a] * S.sun_rgb[a] * hitNdl;
+                }
+                break;
+            }
+            if (stage == 0) bh = h;
+            if (h.hit && h.mat == 5u) h.hit = 0;
+            if (!h.hit) {
+                const float esc = gi_escape(W, ro, rdir, reach);
+                if (esc < 0.999f && !extended && stage == 0 && rdir[1] > 0.3f) {
+                    float e[3], p1[3], p2[3], p3[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        e[a] = ro[a] + rdir[a] * reach;
+                        p1[a] = e[a] + rdir[a] * 32.0f;
+                        p2[a] = e[a] + rdir[a] * 64.0f;
+                        p3[a] = e[a] + rdir[a] * 96.0f;
+                    }
+                    const float m = gmax(sky_open(W, p1), gmax(sky_open(W, p2), sky_open(W, p3)));
+                    if (m > 0.0f) {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) ro[a] = e[a];
+                        reach = 96.0f;
+                        extended = true;
+                        continue;
+                    }
+                }
+                float skyc[3];
+                sky_color0(S, rdir, skyc);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) Lr[a] += gmin(skyc[a], 3.0f) * esc;
+                break;
+            }
+            const float4 alb4 = W.palette[h.mat * 3u];
+            float emit = alb4.w * 8.0f;
+            if (A.lights_nee && alb4.w >= 0.5f) emit = 0.0f;
+            const float alb[3] = {alb4.x, alb4.y, alb4.z};
+            float hitN[3], hitP[3];
+            face_normal(h.face, hitN);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                Lr[a] += alb[a] * emit;
+                hitP[a] = ro[a] + rdir[a] * h.t;
+                hitAlb[a] = alb[a];
+            }
+            reach = A.reach;
+            extended = false;
+            hitNdl = gmax(dot3(hitN, S.sun_dir), 0.0f);
+            if (hitNdl > 0.0f) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) { ro[a] = hitP[a] + hitN[a] * 3e-2f; rdir[a] = S.sun_dir[a]; }
+                stage += 1;
+            } else {
+                break;
+            }
+        }
+    }
+    radiance[oi] = make_float4(Lr[0], Lr[1], Lr[2], sky ? 0.0f : 1.0f);
+    if (bounce) store_hit(bounce + oi, bh);
+    flush_count<COUNT>(c, cnt);
+}
+
+// Explicit ray batch: traceVolumeS per ray (1-D, XCD-remapped 256-ray blocks).
+template <int SKIP, bool COUNT>
+__global__ void __launch_bounds__(256) k_rays(DevWorld W, const vf_ray *rays, long long n, float tcap, vf_hit *hits,
+                                              unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const long long i = (long long)L * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *r = reinterpret_cast<const float *>(rays + i);
+    const float ro[3] = {r[0], r[1], r[2]}, rd[3] = {r[3], r[4], r[5]};
+    Cnt c = {0, 0, 0, 0};
+    Hit h;
+    trace_world<SKIP, COUNT>(W, ro, rd, tcap, h, c);
+    store_hit(hits + i, h);
+    flush_count<COUNT>(c, cnt);
+}
+
+struct ProbeArgs {
+    float origin[3];
+    float spacing;
+    int32_t dims[3];
+    int32_t n_rays;
+    uint32_t frame;
+    float reach;
+    long long n;
+};
+
+// probe_trace.comp ray source (probes.glsl:155-170) traced with skipMat 5 (kSkipMat).
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_probes(DevWorld W, ProbeArgs A, vf_hit *hits, unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const long long idx = (long long)L * 256 + threadIdx.x;
+    if (idx >= A.n) return;
+    const int32_t i = (int32_t)(idx % A.n_rays);
+    const long long p = idx / A.n_rays;
+    const int32_t pxi = (int32_t)(p % A.dims[0]), pyi = (int32_t)((p / A.dims[0]) % A.dims[1]),
+                  pzi = (int32_t)(p / ((long long)A.dims[0] * A.dims[1]));
+    const float ro[3] = {A.origin[0] + A.spacing * (float)pxi, A.origin[1] + A.spacing * (float)pyi,
+                         A.origin[2] + A.spacing * (float)pzi};
+    const float golden = 2.39996323f;
+    const float z = 1.0f - (2.0f * (float)i + 1.0f) / (float)A.n_rays;
+    const float rr = sqrtf(gmax(0.0f, 1.0f - z * z));
+    const float ph = (float)i * golden;
+    float s, c;
+    sincos_red(ph, s, c);
+    const float d[3] = {c * rr, z, s * rr};
+    const uint32_t h0 = pcg(A.frame * 747796405u + 1u);
+    const float u1 = (float)(h0 & 0xFFFFu) / 65535.0f, u2 = (float)(h0 >> 16) / 65535.0f;
+    const uint32_t h2 = pcg(h0);
+    const float u3 = (float)(h2 & 0xFFFFu) / 65535.0f;
+    float s2, c2, s3, c3;
+    sincos_red(6.2831853f * u2, s2, c2);
+    sincos_red(6.2831853f * u3, s3, c3);
+    const float qa = sqrtf(1.0f - u1), qb = sqrtf(u1);
+    const float q[3] = {qa * s2, qa * c2, qb * s3};
+    const float qw = qb * c3;
+    float c1v[3], tmp[3], c2v[3], rd[3];
+    cross3(q, d, c1v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tmp[k] = c1v[k] + qw * d[k];
+    cross3(q, tmp, c2v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rd[k] = d[k] + 2.0f * c2v[k];
+    Cnt cc = {0, 0, 0, 0};
+    Hit h;
+    trace_world<5, COUNT>(W, ro, rd, A.reach, h, cc);
+    store_hit(hits + idx, h);
+    flush_count<COUNT>(cc, cnt);
+}
+
+// ------------------------------------------------------------------------------------------------
+// world build kernels: occupancy masks, coarse pyramid, brick distance field, cell words, sky-top
+// ------------------------------------------------------------------------------------------------
+// One thread per (slot, 32-bit word): 32 voxels = 4 rows of 8.
+__global__ void __launch_bounds__(256) k_masks(const uint8_t *atlas, const uint32_t *slots, uint32_t n_list, uint32_t n_slots,
+                                               uint32_t *occ, uint32_t *occdry) {
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t k = gid >> 4;
+    const uint32_t wd = (uint32_t)(gid & 15u);
+    if (k >= (slots ? n_list : n_slots)) return;
+    const uint32_t slot = slots ? slots[k] : (uint32_t)k;
+    if (slot >= n_slots) return;
+    const uint8_t *b = atlas + (size_t)slot * 512u + (size_t)wd * 32u;
+    uint32_t m0 = 0u, m5 = 0u;
+    for (int i = 0; i < 32; ++i) {
+        const uint32_t v = b[i];
+        m0 |= (uint32_t)(v != 0u) << i;
+        m5 |= (uint32_t)(v != 0u && v != 5u) << i;
+    }
+    occ[(size_t)slot * 16u + wd] = m0;
+    occdry[(size_t)slot * 16u + wd] = m5;
+}
+
+// coarse1: one thread per 4^3-brick cell (gpu_world.rs:300-313).
+__global__ void __launch_bounds__(256) k_coarse1(const uint32_t *grid, int32_t wbx, int32_t wby, int32_t wbz, int32_t c1x,
+                                                 int32_t c1y, int32_t c1z, uint8_t *c1) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)c1x * c1y * c1z) return;
+    const int32_t x = (int32_t)(i % c1x), y = (int32_t)((i / c1x) % c1y), z = (int32_t)(i / ((int64_t)c1x * c1y));
+    uint8_t any = 0;
+    for (int32_t dz = 0; dz < 4 && !any; ++dz)
+        for (int32_t dy = 0; dy < 4 && !any; ++dy)
+            for (int32_t dx = 0; dx < 4 && !any; ++dx) {
+                const int32_t bx = x * 4 + dx, by = y * 4 + dy, bz = z * 4 + dz;
+                if (bx >= wbx || by >= wby || bz >= wbz) continue;
+                any = grid[((size_t)bz * wby + by) * wbx + bx] != 0u;
+            }
+    c1[i] = any;
+}
+// coarse2 from coarse1: a 16^3-brick cell is 4^3 coarse1 cells.
+__global__ void __launch_bounds__(256) k_coarse2(const uint8_t *c1, int32_t c1x, int32_t c1y, int32_t c1z, int32_t c2x,
+                                                 int32_t c2y, int32_t c2z, uint8_t *c2) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)c2x * c2y * c2z) return;
+    const int32_t x = (int32_t)(i % c2x), y = (int32_t)((i / c2x) % c2y), z = (int32_t)(i / ((int64_t)c2x * c2y));
+    uint8_t any = 0;
+    for (int32_t dz = 0; dz < 4 && !any; ++dz)
+        for (int32_t dy = 0; dy < 4 && !any; ++dy)
+            for (int32_t dx = 0; dx < 4 && !any; ++dx) {
+                const int32_t qx = x * 4 + dx, qy = y * 4 + dy, qz = z * 4 + dz;
+                if (qx >= c1x || qy >= c1y || qz >= c1z) continue;
+                any = c1[((size_t)qz * c1y + qy) * c1x + qx];
+            }
+    c2[i] = any;
+}
+
+struct DistArgs {
+    int32_t bmin[3], bmax[3];
+    int32_t dims[3];
+    int32_t axis, fp, ring;
+};
+
+// One brickdist.comp dispatch (brickdist.comp:31-68): fp 1 = grid occupancy + pass, 0 = pass over
+// a u8 field, 2 = dry occupancy (from the dry occupancy mask: any bit set = a solid non-water voxel).
+__global__ void __launch_bounds__(256) k_dist(DistArgs A, const uint32_t *grid, const uint8_t *src, uint8_t *dst,
+                                              const uint4 *occdry) {
+    const int32_t ex = A.bmax[0] - A.bmin[0] + 1, ey = A.bmax[1] - A.bmin[1] + 1, ez = A.bmax[2] - A.bmin[2] + 1;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)ex * ey * ez) return;
+    const int32_t c[3] = {A.bmin[0] + (int32_t)(i % ex), A.bmin[1] + (int32_t)((i / ex) % ey),
+                          A.bmin[2] + (int32_t)(i / ((int64_t)ex * ey))};
+    const int32_t cw[3] = {A.ring ? (c[0] & (A.dims[0] - 1)) : c[0], c[1], A.ring ? (c[2] & (A.dims[2] - 1)) : c[2]};
+    const size_t ci = ((size_t)cw[2] * A.dims[1] + cw[1]) * A.dims[0] + cw[0];
+    const uint32_t CAP = 16u;
+    if (A.fp == 2) {
+        const uint32_t g = grid[ci];
+        bool solid = g != 0u && g != 5u;
+        if (g >= 256u) {
+            const uint4 *m = occdry + (size_t)(g - 256u) * 4u;
+            const uint4 a = m[0], b = m[1], cc = m[2], d = m[3];
+            solid = ((a.x | a.y | a.z | a.w) | (b.x | b.y | b.z | b.w) | (cc.x | cc.y | cc.z | cc.w) |
+                     (d.x | d.y | d.z | d.w)) != 0u;
+        }
+        dst[ci] = solid ? 0u : (uint8_t)CAP;
+        return;
+    }
+    uint32_t best = CAP;
+    for (int32_t k = -16; k <= 16; ++k) {
+        int32_t q[3] = {c[0], c[1], c[2]};
+        q[A.axis] += k;
+        if (A.ring) {
+            if (q[1] < 0 || q[1] >= A.dims[1]) continue;
+            q[0] &= A.dims[0] - 1;
+            q[2] &= A.dims[2] - 1;
+        } else if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= A.dims[0] || q[1] >= A.dims[1] || q[2] >= A.dims[2]) {
+            continue;
+        }
+        const size_t qi = ((size_t)q[2] * A.dims[1] + q[1]) * A.dims[0] + q[0];
+        const uint32_t f = A.fp ? (grid[qi] != 0u ? 0u : CAP) : (uint32_t)src[qi];
+        const uint32_t ak = (uint32_t)(k < 0 ? -k : k);
+        const uint32_t m = ak > f ? ak : f;
+        best = m < best ? m : best;
+    }
+    dst[ci] = (uint8_t)best;
+}
+
+// Cell words over the whole grid: grid value, both distance fields and the coarse-cell flags.
+__global__ void __launch_bounds__(256) k_pack(const uint32_t *grid, const uint8_t *d0, const uint8_t *d2, const uint8_t *c1,
+                                              const uint8_t *c2, int32_t wbx, int32_t wby, int32_t wbz, int32_t c1x,
+                                              int32_t c1y, int32_t c2x, int32_t c2y, int32_t tdx, int32_t tdy,
+                                              uint2 *cells) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)wbx * wby * wbz) return;
+    const int32_t x = (int32_t)(i % wbx), y = (int32_t)((i / wbx) % wby), z = (int32_t)(i / ((int64_t)wbx * wby));
+    const uint32_t e2 = c2[((size_t)(z >> 4) * c2y + (y >> 4)) * c2x + (x >> 4)] == 0u ? 1u : 0u;
+    const uint32_t e1 = c1[((size_t)(z >> 2) * c1y + (y >> 2)) * c1x + (x >> 2)] == 0u ? 1u : 0u;
+    const uint32_t meta = (uint32_t)d0[i] | ((uint32_t)d2[i] << 8) | (e2 << 16) | (e1 << 17);
+    const uint32_t tile = ((uint32_t)(z >> 2) * (uint32_t)tdy + (uint32_t)(y >> 2)) * (uint32_t)tdx + (uint32_t)(x >> 2);
+    cells[(size_t)tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3))] = make_uint2(grid[i], meta);
+}
+
+// Sky-top map (grasstop.comp:121-146 skyTop): 1 + the highest voxel of the column whose material
+// kind is neither water (1) nor foliage (3); 0 if none.  Columns in grid (wrapped) coordinates.
+__global__ void __launch_bounds__(256) k_skytop(const uint32_t *grid, const uint8_t *atlas, const float4 *palette,
+                                                int32_t wbx, int32_t wby, int32_t wbz, uint16_t *skytop) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int32_t sx = wbx * 8;
+    if (i >= (int64_t)sx * wbz * 8) return;
+    const int32_t x = (int32_t)(i % sx), z = (int32_t)(i / sx);
+    uint32_t sky = 0u;
+    for (int32_t by = wby - 1; by >= 0 && sky == 0u; --by) {
+        const uint32_t g = grid[((size_t)(z >> 3) * wby + by) * wbx + (x >> 3)];
+        if (g == 0u) continue;
+        for (int32_t ly = 7; ly >= 0; --ly) {
+            const uint32_t m = g < 256u ? g : atlas[(size_t)(g - 256u) * 512u + (uint32_t)((z & 7) * 64 + ly * 8 + (x & 7))];
+            if (m == 0u) continue;
+            const float kind = palette[m * 3u + 1u].z;
+            if (kind != 1.0f && kind != 3.0f) { sky = (uint32_t)(by * 8 + ly + 1); break; }
+        }
+    }
+    skytop[i] = (uint16_t)sky;
+}
+
+__global__ void k_update_cells(uint32_t *grid, const uint32_t *idx, const uint32_t *val, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) grid[idx[i]] = val[i];
+}
+__global__ void k_update_slots(uint8_t *atlas, const uint32_t *slots, const uint8_t *vox, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;   // one thread per 4 bytes
+    if (i >= n * 128u) return;
+    const uint32_t k = i >> 7, w = i & 127u;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(vox + (size_t)k * 512u);
+    uint32_t v = src[w];
+    // TORCH_FLAME (254) is air for every ray (gpu_world.rs:58-67)
+    uint32_t o = 0u;
+    for (int b = 0; b < 4; ++b) {
+        uint32_t byte = (v >> (8 * b)) & 255u;
+        if (byte == 254u) byte = 0u;
+        o |= byte << (8 * b);
+    }
+    reinterpret_cast<uint32_t *>(atlas + (size_t)slots[k] * 512u)[w] = o;
+}
+
+// ------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------
+struct vf_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_world = false;
+    bool has_scene = false;
+    DevWorld W{};
+    SkyParams S{};
+    vf_scene_ubo scene{};
+    // owned device buffers
+    uint32_t *grid = nullptr;
+    uint2 *cells = nullptr;
+    uint8_t *atlas = nullptr;
+    uint32_t *occ = nullptr, *occdry = nullptr, *masks_ref = nullptr;
+    uint8_t *dist[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t *c1 = nullptr, *c2 = nullptr;
+    uint16_t *skytop = nullptr;
+    float4 *palette = nullptr;
+    int32_t c1d[3] = {0, 0, 0}, c2d[3] = {0, 0, 0};
+    uint32_t n_slots = 0;
+    size_t n_cells = 0, n_cells_tiled = 0;
+    bool counting = false;
+    unsigned long long *d_cnt = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+namespace vf {
+std::string &thread_error() {
+    static thread_local std::string e;
+    return e;
+}
+void set_ctx_error(vf_ctx *ctx, const std::string &msg) { ctx->err = msg; }
+}  // namespace vf
+
+#define VF_HIP(ctx, call)                                                                                         \
+    do {                                                                                                          \
+        hipError_t e_ = (call);                                                                                   \
+        if (e_ != hipSuccess)                                                                                     \
+            return vf::fail((ctx), e_ == hipErrorOutOfMemory ? VF_E_OOM : VF_E_DEVICE, "%s failed: %s (%s:%d)", #call, \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                                           \
+    } while (0)
+
+namespace {
+
+int set_device(vf_ctx *ctx) {
+    VF_HIP(ctx, hipSetDevice(ctx->device));
+    return VF_OK;
+}
+
+void free_world(vf_ctx *ctx) {
+    void *ptrs[] = {ctx->grid, ctx->cells, ctx->atlas, ctx->occ, ctx->occdry, ctx->masks_ref, ctx->dist[0], ctx->dist[1],
+                    ctx->dist[2], ctx->dist[3], ctx->c1, ctx->c2, ctx->skytop, ctx->palette};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ctx->grid = nullptr; ctx->cells = nullptr; ctx->atlas = nullptr; ctx->occ = nullptr; ctx->occdry = nullptr;
+    ctx->masks_ref = nullptr;
+    for (auto &d : ctx->dist) d = nullptr;
+    ctx->c1 = ctx->c2 = nullptr;
+    ctx->skytop = nullptr;
+    ctx->palette = nullptr;
+    ctx->has_world = false;
+}
+
+inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+int launch_check(vf_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return vf::fail(ctx, VF_E_DEVICE, "%s launch failed: %s", what, hipGetErrorString(e));
+    return VF_OK;
+}
+
+int rebuild_coarse(vf_ctx *ctx) {
+    const int32_t *wb = ctx->W.wb;
+    const int64_t n1 = (int64_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2];
+    const int64_t n2 = (int64_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
+    hipLaunchKernelGGL(k_coarse1, dim3(blocks_for(n1)), dim3(256), 0, ctx->stream, ctx->grid, wb[0], wb[1], wb[2], ctx->c1d[0],
+                       ctx->c1d[1], ctx->c1d[2], ctx->c1);
+    hipLaunchKernelGGL(k_coarse2, dim3(blocks_for(n2)), dim3(256), 0, ctx->stream, ctx->c1, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2],
+                       ctx->c2d[0], ctx->c2d[1], ctx->c2d[2], ctx->c2);
+    return launch_check(ctx, "coarse pyramid");
+}
+
+int repack(vf_ctx *ctx) {
+    const int32_t *wb = ctx->W.wb;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)ctx->n_cells)), dim3(256), 0, ctx->stream, ctx->grid, ctx->dist[0],
+                       ctx->dist[2], ctx->c1, ctx->c2, wb[0], wb[1], wb[2], ctx->c1d[0], ctx->c1d[1], ctx->c2d[0],
+                       ctx->c2d[1], ctx->W.tdim[0], ctx->W.tdim[1], ctx->cells);
+    return launch_check(ctx, "cell words");
+}
+
+int rebuild_skytop(vf_ctx *ctx) {
+    const int32_t *wb = ctx->W.wb;
+    const int64_t n = (int64_t)wb[0] * 8 * wb[2] * 8;
+    hipLaunchKernelGGL(k_skytop, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, ctx->grid, ctx->atlas, ctx->palette, wb[0],
+                       wb[1], wb[2], ctx->skytop);
+    return launch_check(ctx, "sky-top");
+}
+
+int dist_pass(vf_ctx *ctx, const int32_t bmin[3], const int32_t bmax[3], int axis, int fp, const uint8_t *src, uint8_t *dst) {
+    DistArgs A;
+    for (int a = 0; a < 3; ++a) {
+        A.bmin[a] = bmin[a];
+        A.bmax[a] = bmax[a];
+        A.dims[a] = ctx->W.wb[a];
+        if (bmax[a] < bmin[a]) return VF_OK;
+    }
+    A.axis = axis;
+    A.fp = fp;
+    A.ring = ctx->W.ring;
+    const int64_t n = (int64_t)(bmax[0] - bmin[0] + 1) * (bmax[1] - bmin[1] + 1) * (bmax[2] - bmin[2] + 1);
+    hipLaunchKernelGGL(k_dist, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, A, ctx->grid, src, dst,
+                       reinterpret_cast<const uint4 *>(ctx->occdry));
+    return launch_check(ctx, "brickdist pass");
+}
+
+// record_distance_field (gpu_world.rs:1376-1428): box clamping, then 7 passes over 4 fields.
+int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], const int32_t dlo_in[3], const int32_t dhi_in[3]) {
+    const int32_t *wb = ctx->W.wb;
+    int32_t lo[3], hi[3], dlo[3], dhi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = lo_in[k]; hi[k] = hi_in[k]; dlo[k] = dlo_in[k]; dhi[k] = dhi_in[k];
+        if (ctx->W.ring && k != 1) {
+            dhi[k] = std::min(dhi[k], dlo[k] + wb[k] - 1);
+            hi[k] = std::min(hi[k], lo[k] + wb[k] - 1);
+            continue;
+        }
+        lo[k] = std::clamp(lo[k], 0, wb[k] - 1);
+        hi[k] = std::clamp(hi[k], 0, wb[k] - 1);
+        dlo[k] = std::clamp(dlo[k], 0, wb[k] - 1);
+        dhi[k] = std::clamp(dhi[k], 0, wb[k] - 1);
+    }
+    uint8_t **d = ctx->dist;
+    int rc;
+    if ((rc = dist_pass(ctx, lo, hi, 0, 1, nullptr, d[0]))) return rc;
+    if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[0], d[1]))) return rc;
+    if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[0]))) return rc;
+    if ((rc = dist_pass(ctx, dlo, dhi, 0, 2, nullptr, d[3]))) return rc;
+    if ((rc = dist_pass(ctx, lo, hi, 0, 0, d[3], d[2]))) return rc;
+    if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[2], d[1]))) return rc;
+    if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[2]))) return rc;
+    return VF_OK;
+}
+
+int begin_timed(vf_ctx *ctx) {
+    VF_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return VF_OK;
+}
+int end_timed(vf_ctx *ctx) {
+    VF_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    return VF_OK;
+}
+
+int check_ready(vf_ctx *ctx, const char *what) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "%s: null context", what);
+    if (!ctx->has_world) return vf::fail(ctx, VF_E_NO_WORLD, "%s: no world uploaded (vf_world_upload)", what);
+    return set_device(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vf_abi_version(void) { return VF_ABI_VERSION; }
+
+int vf_abi_sizes(size_t *scene, size_t *pass, size_t *mat, size_t *ray, size_t *hit) {
+    if (scene) *scene = sizeof(vf_scene_ubo);
+    if (pass) *pass = sizeof(vf_pass_ubo);
+    if (mat) *mat = sizeof(vf_material);
+    if (ray) *ray = sizeof(vf_ray);
+    if (hit) *hit = sizeof(vf_hit);
+    return VF_OK;
+}
+
+const char *vf_last_error(const vf_ctx *ctx) {
+    if (ctx) return ctx->err.c_str();
+    return vf::thread_error().c_str();
+}
+
+int vf_ctx_create(int device, vf_ctx **out) {
+    if (!out) return vf::fail(nullptr, VF_E_INVALID, "vf_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return vf::fail(nullptr, VF_E_DEVICE, "vf_ctx_create: no HIP device (%s)", e != hipSuccess ? hipGetErrorString(e) : "count 0");
+    if (device < 0 || device >= n) return vf::fail(nullptr, VF_E_INVALID, "vf_ctx_create: device %d of %d", device, n);
+    vf_ctx *ctx = new (std::nothrow) vf_ctx();
+    if (!ctx) return vf::fail(nullptr, VF_E_OOM, "vf_ctx_create: out of host memory");
+    ctx->device = device;
+    int rc = VF_OK;
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&ctx->own_stream)) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_cnt, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMemset(ctx->d_cnt, 0, 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess) {
+        rc = vf::fail(nullptr, VF_E_DEVICE, "vf_ctx_create: %s", hipGetErrorString(e));
+        vf_ctx_destroy(ctx);
+        return rc;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return VF_OK;
+}
+
+void vf_ctx_destroy(vf_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_world(ctx);
+    if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+int vf_set_stream(vf_ctx *ctx, void *s) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_set_stream: null context");
+    ctx->stream = s ? reinterpret_cast<hipStream_t>(s) : ctx->own_stream;
+    return VF_OK;
+}
+
+int vf_sync(vf_ctx *ctx) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_sync: null context");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int vf_device_alloc(vf_ctx *ctx, size_t bytes, void **d_out) {
+    if (!ctx || !d_out) return vf::fail(ctx, VF_E_INVALID, "vf_device_alloc: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    *d_out = nullptr;
+    VF_HIP(ctx, hipMalloc(d_out, bytes ? bytes : 1));
+    return VF_OK;
+}
+int vf_device_free(vf_ctx *ctx, void *p) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_device_free: null context");
+    if (!p) return VF_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    VF_HIP(ctx, hipFree(p));
+    return VF_OK;
+}
+int vf_copy_to_device(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_copy_to_device: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+int vf_copy_to_host(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_copy_to_host: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+int vf_memset_device(vf_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!ctx || (!dst && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_memset_device: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return VF_OK;
+}
+
+int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
+    if (!ctx || !d) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: null argument");
+    if (!d->grid || !d->palette) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: grid and palette are required");
+    for (int a = 0; a < 3; ++a)
+        if (d->world_bricks[a] <= 0 || d->world_bricks[a] > 4096)
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: world_bricks[%d] = %d out of range", a, d->world_bricks[a]);
+    const int ring = d->world_win[3] != 0;
+    if (ring) {
+        const int32_t x = d->world_bricks[0], z = d->world_bricks[2];
+        if ((x & (x - 1)) || (z & (z - 1)))
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: ring mode needs power-of-two x/z brick dims");
+    }
+    if (d->n_slots && !d->atlas) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: n_slots > 0 without an atlas");
+    if (d->n_slots > (1u << 24)) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: n_slots %u > 2^24", d->n_slots);
+    const size_t n = (size_t)d->world_bricks[0] * d->world_bricks[1] * d->world_bricks[2];
+    for (size_t i = 0; i < n; ++i)
+        if (d->grid[i] >= 256u && d->grid[i] - 256u >= d->n_slots)
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: grid[%zu] = %u names slot %u of %u", i, d->grid[i],
+                            d->grid[i] - 256u, d->n_slots);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_world(ctx);
+    DevWorld &W = ctx->W;
+    W = DevWorld{};
+    for (int a = 0; a < 3; ++a) {
+        W.wb[a] = d->world_bricks[a];
+        W.tdim[a] = (W.wb[a] + 3) / 4;
+        ctx->c1d[a] = (W.wb[a] + 3) / 4;
+        ctx->c2d[a] = (W.wb[a] + 15) / 16;
+    }
+    W.ring = ring;
+    W.minb[0] = ring ? d->world_win[0] : 0;
+    W.minb[1] = 0;
+    W.minb[2] = ring ? d->world_win[2] : 0;
+    W.use_dist = 1;
+    W.max_outer = 3 * (W.wb[0] + W.wb[1] + W.wb[2]) + 4;
+    ctx->n_cells = n;
+    ctx->n_cells_tiled = (size_t)W.tdim[0] * W.tdim[1] * W.tdim[2] * 64;
+    ctx->n_slots = d->n_slots;
+    const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
+    const size_t slots = std::max<size_t>(d->n_slots, 1);
+    const size_t nsky = (size_t)W.wb[0] * 8 * W.wb[2] * 8;
+#define VF_ALLOC(ptr, bytes)                                                     \
+    do {                                                                         \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                     \
+        if (e_ != hipSuccess) {                                                  \
+            free_world(ctx);                                                     \
+            return vf::fail(ctx, VF_E_OOM, "vf_world_upload: hipMalloc(%zu): %s", \
+                            (size_t)(bytes), hipGetErrorString(e_));             \
+        }                                                                        \
+    } while (0)
+    VF_ALLOC(ctx->grid, n * 4);
+    VF_ALLOC(ctx->cells, ctx->n_cells_tiled * 8);
+    VF_ALLOC(ctx->atlas, slots * 512);
+    VF_ALLOC(ctx->occ, slots * 64);
+    VF_ALLOC(ctx->occdry, slots * 64);
+    VF_ALLOC(ctx->masks_ref, slots * 64);
+    for (auto &p : ctx->dist) VF_ALLOC(p, n);
+    VF_ALLOC(ctx->c1, n1);
+    VF_ALLOC(ctx->c2, n2);
+    VF_ALLOC(ctx->skytop, nsky * 2);
+    VF_ALLOC(ctx->palette, 256 * 48);
+#undef VF_ALLOC
+    VF_HIP(ctx, hipMemsetAsync(ctx->cells, 0, ctx->n_cells_tiled * 8, ctx->stream));
+    VF_HIP(ctx, hipMemcpyAsync(ctx->grid, d->grid, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (d->n_slots) {
+        // TORCH_FLAME -> air as the GPU atlas sees it (gpu_world.rs:58-67)
+        std::vector<uint8_t> atlas(d->atlas, d->atlas + (size_t)d->n_slots * 512);
+        for (auto &v : atlas)
+            if (v == 254) v = 0;
+        VF_HIP(ctx, hipMemcpyAsync(ctx->atlas, atlas.data(), atlas.size(), hipMemcpyHostToDevice, ctx->stream));
+        VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (d->masks)
+            VF_HIP(ctx, hipMemcpyAsync(ctx->masks_ref, d->masks, (size_t)d->n_slots * 64, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        VF_HIP(ctx, hipMemsetAsync(ctx->atlas, 0, 512, ctx->stream));
+    }
+    VF_HIP(ctx, hipMemsetAsync(ctx->occ, 0, slots * 64, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->occdry, 0, slots * 64, ctx->stream));
+    VF_HIP(ctx, hipMemcpyAsync(ctx->palette, d->palette, 256 * 48, hipMemcpyHostToDevice, ctx->stream));
+    if (d->n_slots) {
+        hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)d->n_slots * 16)), dim3(256), 0, ctx->stream, ctx->atlas, nullptr, 0u,
+                           d->n_slots, ctx->occ, ctx->occdry);
+        if ((rc = launch_check(ctx, "occupancy masks"))) return rc;
+    }
+    if (d->coarse1 && d->coarse2) {
+        VF_HIP(ctx, hipMemcpyAsync(ctx->c1, d->coarse1, n1, hipMemcpyHostToDevice, ctx->stream));
+        VF_HIP(ctx, hipMemcpyAsync(ctx->c2, d->coarse2, n2, hipMemcpyHostToDevice, ctx->stream));
+    } else if ((rc = rebuild_coarse(ctx))) {
+        return rc;
+    }
+    W.cells = ctx->cells;
+    W.atlas = ctx->atlas;
+    W.occ = reinterpret_cast<const uint4 *>(ctx->occ);
+    W.occdry = reinterpret_cast<const uint4 *>(ctx->occdry);
+    W.skytop = ctx->skytop;
+    W.palette = ctx->palette;
+    const int32_t lo[3] = {W.minb[0], 0, W.minb[2]};
+    const int32_t hi[3] = {W.minb[0] + W.wb[0] - 1, W.wb[1] - 1, W.minb[2] + W.wb[2] - 1};
+    if ((rc = distance_field(ctx, lo, hi, lo, hi))) return rc;
+    if ((rc = repack(ctx))) return rc;
+    if ((rc = rebuild_skytop(ctx))) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->has_world = true;
+    return VF_OK;
+}
+
+int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_val, uint32_t n_cells, const uint32_t *slot_idx,
+                    const uint8_t *slot_vox, uint32_t n_slot_updates) {
+    int rc = check_ready(ctx, "vf_world_update");
+    if (rc) return rc;
+    if ((n_cells && (!cell_idx || !cell_val)) || (n_slot_updates && (!slot_idx || !slot_vox)))
+        return vf::fail(ctx, VF_E_INVALID, "vf_world_update: null arrays");
+    for (uint32_t i = 0; i < n_cells; ++i) {
+        if (cell_idx[i] >= ctx->n_cells) return vf::fail(ctx, VF_E_INVALID, "vf_world_update: cell %u out of range", cell_idx[i]);
+        if (cell_val[i] >= 256u && cell_val[i] - 256u >= ctx->n_slots)
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_update: value %u names a slot beyond %u", cell_val[i], ctx->n_slots);
+    }
+    for (uint32_t i = 0; i < n_slot_updates; ++i)
+        if (slot_idx[i] >= ctx->n_slots) return vf::fail(ctx, VF_E_INVALID, "vf_world_update: slot %u out of range", slot_idx[i]);
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // staging (the reference's staging ring): temporary device buffers for this update
+    void *d_ci = nullptr, *d_cv = nullptr, *d_si = nullptr, *d_sv = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {d_ci, d_cv, d_si, d_sv})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t e = hipSuccess;
+    if (n_cells) {
+        if ((e = hipMalloc(&d_ci, (size_t)n_cells * 4)) != hipSuccess || (e = hipMalloc(&d_cv, (size_t)n_cells * 4)) != hipSuccess ||
+            (e = hipMemcpy(d_ci, cell_idx, (size_t)n_cells * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d_cv, cell_val, (size_t)n_cells * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+            cleanup();
+            return vf::fail(ctx, VF_E_DEVICE, "vf_world_update: %s", hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(k_update_cells, dim3(blocks_for(n_cells)), dim3(256), 0, ctx->stream, ctx->grid, (const uint32_t *)d_ci,
+                           (const uint32_t *)d_cv, n_cells);
+    }
+    if (n_slot_updates) {
+        if ((e = hipMalloc(&d_si, (size_t)n_slot_updates * 4)) != hipSuccess ||
+            (e = hipMalloc(&d_sv, (size_t)n_slot_updates * 512)) != hipSuccess ||
+            (e = hipMemcpy(d_si, slot_idx, (size_t)n_slot_updates * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d_sv, slot_vox, (size_t)n_slot_updates * 512, hipMemcpyHostToDevice)) != hipSuccess) {
+            cleanup();
+            return vf::fail(ctx, VF_E_DEVICE, "vf_world_update: %s", hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(k_update_slots, dim3(blocks_for((int64_t)n_slot_updates * 128)), dim3(256), 0, ctx->stream, ctx->atlas,
+                           (const uint32_t *)d_si, (const uint8_t *)d_sv, n_slot_updates);
+        hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)n_slot_updates * 16)), dim3(256), 0, ctx->stream, ctx->atlas,
+                           (const uint32_t *)d_si, n_slot_updates, ctx->n_slots, ctx->occ, ctx->occdry);
+    }
+    if ((rc = launch_check(ctx, "world update"))) { cleanup(); return rc; }
+    if ((rc = rebuild_coarse(ctx)) || (rc = repack(ctx)) || (rc = rebuild_skytop(ctx))) { cleanup(); return rc; }
+    e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return vf::fail(ctx, VF_E_DEVICE, "vf_world_update: %s", hipGetErrorString(e));
+    return VF_OK;
+}
+
+int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]) {
+    int rc = check_ready(ctx, "vf_build_brickdist");
+    if (rc) return rc;
+    if (!lo || !hi || !dlo || !dhi) return vf::fail(ctx, VF_E_INVALID, "vf_build_brickdist: null box");
+    for (int a = 0; a < 3; ++a)
+        if (hi[a] - lo[a] > (1 << 20) || dhi[a] - dlo[a] > (1 << 20))
+            return vf::fail(ctx, VF_E_INVALID, "vf_build_brickdist: box too large");
+    if ((rc = begin_timed(ctx))) return rc;
+    if ((rc = distance_field(ctx, lo, hi, dlo, dhi))) return rc;
+    if ((rc = repack(ctx))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry) {
+    int rc = check_ready(ctx, "vf_read_brickdist");
+    if (rc) return rc;
+    if (dist) VF_HIP(ctx, hipMemcpyAsync(dist, ctx->dist[0], ctx->n_cells, hipMemcpyDeviceToHost, ctx->stream));
+    if (dry) VF_HIP(ctx, hipMemcpyAsync(dry, ctx->dist[2], ctx->n_cells, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int vf_read_skytop(vf_ctx *ctx, uint16_t *sky) {
+    int rc = check_ready(ctx, "vf_read_skytop");
+    if (rc) return rc;
+    if (!sky) return vf::fail(ctx, VF_E_INVALID, "vf_read_skytop: null output");
+    VF_HIP(ctx, hipMemcpyAsync(sky, ctx->skytop, (size_t)ctx->W.wb[0] * 8 * ctx->W.wb[2] * 8 * 2, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *s) {
+    if (!ctx || !s) return vf::fail(ctx, VF_E_INVALID, "vf_set_scene: null argument");
+    if (ctx->has_world) {
+        for (int a = 0; a < 3; ++a)
+            if (s->world_bricks[a] != ctx->W.wb[a])
+                return vf::fail(ctx, VF_E_INVALID, "vf_set_scene: world_bricks differ from the uploaded world");
+        if ((s->world_win[3] != 0) != (ctx->W.ring != 0))
+            return vf::fail(ctx, VF_E_INVALID, "vf_set_scene: world_win.w (ring mode) differs from the uploaded world");
+        if (ctx->W.ring) { ctx->W.minb[0] = s->world_win[0]; ctx->W.minb[2] = s->world_win[2]; }
+    }
+    if (s->num_objects != 0)
+        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_set_scene: num_objects = %d: body traversal is not part of this library",
+                        s->num_objects);
+    ctx->scene = *s;
+    SkyParams &S = ctx->S;
+    for (int a = 0; a < 3; ++a) {
+        S.sun_dir[a] = s->sun_dir[a];
+        S.sun_rgb[a] = s->sun_rgb[a];
+        S.sky_zenith[a] = s->sky_zenith[a];
+        S.sky_horizon[a] = s->sky_horizon[a];
+        S.sunset_rgb[a] = s->sunset_rgb[a];
+    }
+    S.sky_scale = s->sky_scale;
+    S.gi_scale = s->gi_scale;
+    S.night = s->night;
+    S.sky_model = s->sky_model;
+    ctx->has_scene = true;
+    return VF_OK;
+}
+
+int vf_set_counting(vf_ctx *ctx, int32_t on) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_set_counting: null context");
+    ctx->counting = on != 0;
+    return VF_OK;
+}
+
+int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset) {
+    if (!ctx || !out) return vf::fail(ctx, VF_E_INVALID, "vf_read_counters: null argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    unsigned long long h[4];
+    VF_HIP(ctx, hipMemcpyAsync(h, ctx->d_cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < 4; ++i) out[i] = h[i];
+    if (reset) VF_HIP(ctx, hipMemsetAsync(ctx->d_cnt, 0, sizeof(h), ctx->stream));
+    return VF_OK;
+}
+
+int vf_last_kernel_ms(vf_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: null argument");
+    if (!ctx->timed) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: nothing timed yet");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    VF_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return VF_OK;
+}
+
+int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos, uint32_t *gb_normal) {
+    int rc = check_ready(ctx, "vf_trace_primary");
+    if (rc) return rc;
+    if (!p || !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary: null pass or hits");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary: bad size %dx%d", w, h);
+    PrimArgs A;
+    for (int a = 0; a < 3; ++a) {
+        A.pos[a] = p->cam_pos[a]; A.right[a] = p->cam_right[a]; A.up[a] = p->cam_up[a]; A.fwd[a] = p->cam_fwd[a];
+    }
+    A.thf = p->tan_half_fov;
+    A.jit[0] = p->jitter[0];
+    A.jit[1] = p->jitter[1];
+    A.w = w; A.h = h;
+    A.tiles_x = (w + 15) / 16;
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_primary<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits,
+                           reinterpret_cast<float4 *>(gb_pos), gb_normal, ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_primary<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits,
+                           reinterpret_cast<float4 *>(gb_pos), gb_normal, ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_primary"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_shadow(vf_ctx *ctx, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal, uint32_t *shadow, vf_hit *hits) {
+    int rc = check_ready(ctx, "vf_trace_shadow");
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: no scene (vf_set_scene)");
+    if (!gb_pos || !gb_normal || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: null buffer");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: bad size %dx%d", w, h);
+    TileArgs A{w, h, (w + 15) / 16, 0};
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    const float *s = ctx->S.sun_dir;
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_shadow"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_gi(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, int32_t half_res, float gi_reach, int32_t lights_nee,
+                const float *gb_pos, const uint32_t *gb_normal, float *radiance, vf_hit *bounce) {
+    int rc = check_ready(ctx, "vf_trace_gi");
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi: no scene (vf_set_scene)");
+    if (!p || !gb_pos || !gb_normal || !radiance) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi: null buffer");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi: bad size %dx%d", w, h);
+    if (!(gi_reach > 0.0f) || gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi: gi_reach %g out of range", gi_reach);
+    GiArgs A;
+    A.w = w; A.h = h;
+    A.half_res = half_res ? 1 : 0;
+    A.ow = half_res ? (w + 1) / 2 : w;
+    A.oh = half_res ? (h + 1) / 2 : h;
+    A.tiles_x = (A.ow + 15) / 16;
+    A.ntiles = A.tiles_x * ((A.oh + 15) / 16);
+    A.lights_nee = lights_nee ? 1 : 0;
+    A.frame = p->frame_idx;
+    A.reach = gi_reach;
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_gi<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(radiance), bounce,
+                           ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_gi<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(radiance), bounce,
+                           ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_gi"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_rays(vf_ctx *ctx, const vf_ray *rays, int64_t n, float tcap, uint32_t skip_mat, vf_hit *hits) {
+    int rc = check_ready(ctx, "vf_trace_rays");
+    if (rc) return rc;
+    if (n < 0 || n > (1ll << 31) * 64) return vf::fail(ctx, VF_E_INVALID, "vf_trace_rays: bad count %lld", (long long)n);
+    if (n == 0) return VF_OK;
+    if (!rays || !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_rays: null buffer");
+    if (skip_mat != 0u && skip_mat != 5u) return vf::fail(ctx, VF_E_INVALID, "vf_trace_rays: skip_mat must be 0 or 5");
+    const unsigned nb = blocks_for(n);
+    if ((rc = begin_timed(ctx))) return rc;
+    if (skip_mat == 5u) {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_rays<5, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_rays<5, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+    } else {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_rays<0, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_rays<0, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+    }
+    if ((rc = launch_check(ctx, "k_rays"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int32_t dims[3], int32_t n_rays, uint32_t frame,
+                    float reach, vf_hit *hits) {
+    int rc = check_ready(ctx, "vf_trace_probes");
+    if (rc) return rc;
+    if (!origin || !dims || !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_probes: null argument");
+    if (n_rays <= 0 || n_rays > 65536 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0)
+        return vf::fail(ctx, VF_E_INVALID, "vf_trace_probes: bad grid or ray count");
+    ProbeArgs A;
+    for (int a = 0; a < 3; ++a) { A.origin[a] = origin[a]; A.dims[a] = dims[a]; }
+    A.spacing = spacing;
+    A.n_rays = n_rays;
+    A.frame = frame;
+    A.reach = reach;
+    A.n = (long long)dims[0] * dims[1] * dims[2] * n_rays;
+    if (A.n > (1ll << 31) * 64) return vf::fail(ctx, VF_E_INVALID, "vf_trace_probes: too many rays");
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_probes<true>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, A, hits, ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_probes<false>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, A, hits, ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_probes"))) return rc;
+    return end_timed(ctx);
+}
+
+}  // extern "C"

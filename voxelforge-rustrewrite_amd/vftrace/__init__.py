@@ -1,0 +1,381 @@
+"""Host-side mirror of VoxelForge's traced-pass interface over libvftrace.so (MI355X / gfx950).
+
+The reference records its traced passes in Rust (src/gfx/renderer/record.rs `Renderer::go` per
+pass, src/gfx/gpu_world.rs `GpuWorld::upload` / `sync` / `record_distance_field`).  Rust is not part
+of this build, so this module is the host side above the C ABI (include/vf_trace.h): the same calls,
+argument meanings and error behaviour (a failing call raises VfError with the library's message;
+nothing falls back to another implementation).  Device buffers are torch tensors on the HIP device
+(only their data pointers cross the ABI).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from .abi import (GenParams, Hit, HIT_DTYPE, Material, PassUbo, Ray, RAY_DTYPE, SceneUbo, WorldDesc)
+
+__all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
+           "RAY_DTYPE", "fly_camera", "sun_dir", "make_pass", "default_scene", "library_path"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def library_path():
+    return os.path.join(_HERE, "libvftrace.so")
+
+
+class VfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vftrace error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+_P = C.c_void_p
+_SIGS = {
+    "vf_abi_version": (C.c_int, []),
+    "vf_abi_sizes": (C.c_int, [C.POINTER(C.c_size_t)] * 5),
+    "vf_last_error": (C.c_char_p, [_P]),
+    "vf_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "vf_ctx_destroy": (None, [_P]),
+    "vf_set_stream": (C.c_int, [_P, _P]),
+    "vf_sync": (C.c_int, [_P]),
+    "vf_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "vf_device_free": (C.c_int, [_P, _P]),
+    "vf_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "vf_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "vf_memset_device": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    "vf_world_upload": (C.c_int, [_P, C.POINTER(WorldDesc)]),
+    "vf_world_update": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32]),
+    "vf_build_brickdist": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
+    "vf_read_brickdist": (C.c_int, [_P, _P, _P]),
+    "vf_read_skytop": (C.c_int, [_P, _P]),
+    "vf_set_scene": (C.c_int, [_P, C.POINTER(SceneUbo)]),
+    "vf_trace_primary": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P]),
+    "vf_trace_shadow": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    "vf_trace_gi": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
+                              _P, _P]),
+    "vf_trace_rays": (C.c_int, [_P, _P, C.c_int64, C.c_float, C.c_uint32, _P]),
+    "vf_trace_probes": (C.c_int, [_P, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int32), C.c_int32, C.c_uint32,
+                                  C.c_float, _P]),
+    "vf_set_counting": (C.c_int, [_P, C.c_int32]),
+    "vf_read_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32]),
+    "vf_last_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "vf_gen_defaults": (C.c_int, [C.c_int32, C.POINTER(GenParams)]),
+    "vf_gen_world_create": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
+    "vf_gen_world_from_dense": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, C.POINTER(_P)]),
+    "vf_gen_world_destroy": (None, [_P]),
+    "vf_gen_world_desc": (C.c_int, [_P, C.POINTER(WorldDesc)]),
+    "vf_gen_world_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "vf_gen_world_voxel": (C.c_uint32, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "vf_gen_world_ground": (C.c_int32, [_P, C.c_int32, C.c_int32]),
+}
+
+
+def lib():
+    """Loads libvftrace.so (built in-tree by __graft_entry__.build()); raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"libvftrace.so not built: {path} is missing (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = lib().vf_last_error(ctx).decode(errors="replace")
+        raise VfError(rc, msg)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _i3(v):
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+# --- uniforms ---------------------------------------------------------------------------------
+def fly_camera(pos, yaw, pitch):
+    """FlyCamera basis (src/camera.rs:34-42): forward = (cos p sin y, sin p, cos p cos y),
+    right = normalize(forward x Y), up = right x forward; float32."""
+    f32 = np.float32
+    y, p = f32(yaw), f32(pitch)
+    fwd = np.array([np.cos(p) * np.sin(y), np.sin(p), np.cos(p) * np.cos(y)], dtype=np.float32)
+    r = np.cross(fwd, np.array([0, 1, 0], dtype=np.float32)).astype(np.float32)
+    n = np.float32(np.sqrt(np.float32(r @ r)))
+    right = (r / n).astype(np.float32) if n > 1e-6 else np.array([0, 1, 0], dtype=np.float32)
+    up = np.cross(right, fwd).astype(np.float32)
+    return np.asarray(pos, dtype=np.float32), right, up, fwd
+
+
+def sun_dir(az_deg, el_deg):
+    """Renderer::set_sun (src/gfx/renderer/frame.rs:44-48)."""
+    az, el = math.radians(az_deg), math.radians(el_deg)
+    s = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)], dtype=np.float32)
+    return (s / np.float32(np.linalg.norm(s))).astype(np.float32)
+
+
+def make_pass(pos, right, up, fwd, fov_deg=70.0, frame_idx=0, jitter=(0.0, 0.0)):
+    """The camera part of PassUbo (set_frame_uniforms, src/gfx/renderer/frame.rs:549-566)."""
+    p = PassUbo()
+    thf = np.float32(math.tan(math.radians(fov_deg * 0.5)))
+    for i in range(3):
+        p.cam_pos[i] = float(pos[i]); p.cam_right[i] = float(right[i])
+        p.cam_up[i] = float(up[i]); p.cam_fwd[i] = float(fwd[i])
+        p.prev_cam_pos[i] = float(pos[i]); p.prev_cam_right[i] = float(right[i])
+        p.prev_cam_up[i] = float(up[i]); p.prev_cam_fwd[i] = float(fwd[i])
+    p.tan_half_fov = float(thf)
+    p.prev_tan_half_fov = float(thf)
+    p.frame_idx = int(frame_idx)
+    p.jitter[0], p.jitter[1] = float(jitter[0]), float(jitter[1])
+    return p
+
+
+def default_scene(world_bricks, world_win=(0, 0, 0, 0), sun=(32.0, 46.0), sky_model=0):
+    """SceneUbo with the reference defaults the traced passes read (cvars.rs:152-160, 209-210):
+    sun azimuth 32, elevation 46, classic sky, sky and GI intensity 1."""
+    s = SceneUbo()
+    for i in range(3):
+        s.world_bricks[i] = int(world_bricks[i])
+    for i in range(4):
+        s.world_win[i] = int(world_win[i])
+    d = sun_dir(*sun)
+    for i in range(3):
+        s.sun_dir[i] = float(d[i])
+    s.sun_rgb[0], s.sun_rgb[1], s.sun_rgb[2] = 3.0, 2.85, 2.6
+    s.sky_zenith[0], s.sky_zenith[1], s.sky_zenith[2] = 0.25, 0.45, 0.85
+    s.sky_horizon[0], s.sky_horizon[1], s.sky_horizon[2] = 0.75, 0.82, 0.90
+    s.sunset_rgb[0], s.sunset_rgb[1], s.sunset_rgb[2] = 0.3, 0.15, 0.05
+    s.sky_scale = 1.0
+    s.gi_scale = 1.0
+    s.sky_model = int(sky_model)
+    s.num_objects = 0
+    return s
+
+
+# --- worlds -----------------------------------------------------------------------------------
+class GenWorld:
+    """A synthetic world packed like GpuWorld::upload (include/vf_worldgen.h)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def defaults(cls, kind):
+        p = GenParams()
+        _check(lib().vf_gen_defaults(int(kind), C.byref(p)))
+        return p
+
+    @classmethod
+    def generate(cls, kind=1, **overrides):
+        p = cls.defaults(kind)
+        for k, v in overrides.items():
+            if k == "world_bricks":
+                for i in range(3):
+                    p.world_bricks[i] = int(v[i])
+            else:
+                setattr(p, k, v)
+        h = C.c_void_p()
+        _check(lib().vf_gen_world_create(C.byref(p), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_dense(cls, vox, world_win=(0, 0, 0, 0), palette=None):
+        """vox: uint8 array [z, y, x] of shape world_bricks*8 (z, y, x order)."""
+        vox = np.ascontiguousarray(vox, dtype=np.uint8)
+        nz, ny, nx = vox.shape
+        if nx % 8 or ny % 8 or nz % 8:
+            raise ValueError("dense world dims must be multiples of 8")
+        wb = (C.c_int32 * 3)(nx // 8, ny // 8, nz // 8)
+        win = (C.c_int32 * 4)(*[int(x) for x in world_win])
+        pal = None
+        if palette is not None:
+            pal = (Material * 256)()
+            arr = np.ascontiguousarray(palette, dtype=np.float32).reshape(256, 12)
+            C.memmove(pal, arr.ctypes.data, 256 * 48)
+        h = C.c_void_p()
+        _check(lib().vf_gen_world_from_dense(vox.ctypes.data_as(C.c_void_p), wb, win,
+                                             C.cast(pal, C.c_void_p) if pal is not None else None, C.byref(h)))
+        return cls(h.value)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().vf_gen_world_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def desc(self):
+        d = WorldDesc()
+        _check(lib().vf_gen_world_desc(self._h, C.byref(d)))
+        return d
+
+    def stats(self):
+        out = (C.c_uint64 * 4)()
+        _check(lib().vf_gen_world_stats(self._h, out))
+        return {"nonempty_bricks": out[0], "detailed_bricks": out[1], "uniform_bricks": out[2],
+                "atlas_solid_voxels": out[3]}
+
+    def voxel(self, x, y, z):
+        return int(lib().vf_gen_world_voxel(self._h, int(x), int(y), int(z)))
+
+    def ground(self, x, z):
+        return int(lib().vf_gen_world_ground(self._h, int(x), int(z)))
+
+    def arrays(self):
+        """Copies of the packed arrays: grid, atlas, masks, coarse1, coarse2, palette, dims."""
+        d = self.desc()
+        wb = tuple(d.world_bricks)
+        n = wb[0] * wb[1] * wb[2]
+        c1 = tuple((x + 3) // 4 for x in wb)
+        c2 = tuple((x + 15) // 16 for x in wb)
+        ns = d.n_slots
+
+        def arr(ptr, count, dtype):
+            if count == 0 or not ptr:
+                return np.zeros(0, dtype=dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+        pal = np.ctypeslib.as_array(C.cast(d.palette, C.POINTER(C.c_float)), shape=(256 * 12,)).copy()
+        return {
+            "world_bricks": wb, "world_win": tuple(d.world_win), "n_slots": ns,
+            "grid": arr(d.grid, n, np.uint32),
+            "atlas": arr(d.atlas, ns * 512, np.uint8),
+            "masks": arr(d.masks, ns * 16, np.uint32),
+            "coarse1": arr(d.coarse1, c1[0] * c1[1] * c1[2], np.uint8),
+            "coarse2": arr(d.coarse2, c2[0] * c2[1] * c2[2], np.uint8),
+            "palette": pal.reshape(256, 12),
+        }
+
+
+# --- context ----------------------------------------------------------------------------------
+class Context:
+    """One vf_ctx: owns the world mirror on the device and a HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().vf_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.world_bricks = None
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().vf_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise VfError(rc, lib().vf_last_error(self._h).decode(errors="replace"))
+
+    # state
+    def set_stream(self, stream=None):
+        """Use a torch.cuda.Stream (or None for the context's own stream)."""
+        self._ck(lib().vf_set_stream(self._h, C.c_void_p(stream.cuda_stream) if stream is not None else None))
+
+    def sync(self):
+        self._ck(lib().vf_sync(self._h))
+
+    def upload(self, world):
+        d = world.desc() if isinstance(world, GenWorld) else world
+        self._ck(lib().vf_world_upload(self._h, C.byref(d)))
+        self.world_bricks = tuple(d.world_bricks)
+
+    def update(self, cell_idx=None, cell_val=None, slot_idx=None, slot_vox=None):
+        ci = np.ascontiguousarray(cell_idx if cell_idx is not None else [], dtype=np.uint32)
+        cv = np.ascontiguousarray(cell_val if cell_val is not None else [], dtype=np.uint32)
+        si = np.ascontiguousarray(slot_idx if slot_idx is not None else [], dtype=np.uint32)
+        sv = np.ascontiguousarray(slot_vox if slot_vox is not None else np.zeros((0, 512)), dtype=np.uint8)
+        if ci.size != cv.size or sv.size != si.size * 512:
+            raise ValueError("update: mismatched array sizes")
+        self._ck(lib().vf_world_update(self._h, ci.ctypes.data if ci.size else None,
+                                       cv.ctypes.data if cv.size else None, ci.size,
+                                       si.ctypes.data if si.size else None, sv.ctypes.data if sv.size else None,
+                                       si.size))
+
+    def build_brickdist(self, lo, hi, dirty_lo=None, dirty_hi=None):
+        dlo = lo if dirty_lo is None else dirty_lo
+        dhi = hi if dirty_hi is None else dirty_hi
+        self._ck(lib().vf_build_brickdist(self._h, _i3(lo), _i3(hi), _i3(dlo), _i3(dhi)))
+
+    def read_brickdist(self):
+        n = int(np.prod(self.world_bricks))
+        d = np.zeros(n, np.uint8)
+        r = np.zeros(n, np.uint8)
+        self._ck(lib().vf_read_brickdist(self._h, d.ctypes.data, r.ctypes.data))
+        return d, r
+
+    def read_skytop(self):
+        wb = self.world_bricks
+        s = np.zeros(wb[0] * 8 * wb[2] * 8, np.uint16)
+        self._ck(lib().vf_read_skytop(self._h, s.ctypes.data))
+        return s
+
+    def set_scene(self, scene):
+        self._ck(lib().vf_set_scene(self._h, C.byref(scene)))
+
+    # passes (device buffers: torch tensors)
+    def trace_primary(self, pass_ubo, width, height, hits, gb_pos=None, gb_normal=None):
+        self._ck(lib().vf_trace_primary(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
+                                        _ptr(gb_pos), _ptr(gb_normal)))
+
+    def trace_shadow(self, width, height, gb_pos, gb_normal, shadow_raw, hits=None):
+        self._ck(lib().vf_trace_shadow(self._h, int(width), int(height), _ptr(gb_pos), _ptr(gb_normal),
+                                       _ptr(shadow_raw), _ptr(hits)))
+
+    def trace_gi(self, pass_ubo, width, height, gb_pos, gb_normal, radiance, bounce=None, half_res=True,
+                 gi_reach=48.0, lights_nee=True):
+        self._ck(lib().vf_trace_gi(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(half_res)),
+                                   float(gi_reach), int(bool(lights_nee)), _ptr(gb_pos), _ptr(gb_normal),
+                                   _ptr(radiance), _ptr(bounce)))
+
+    def trace_rays(self, rays, n, hits, t_cap=1e30, skip_mat=0):
+        self._ck(lib().vf_trace_rays(self._h, _ptr(rays), int(n), float(t_cap), int(skip_mat), _ptr(hits)))
+
+    def trace_probes(self, origin, spacing, dims, n_rays, frame, reach, hits):
+        o = (C.c_float * 3)(*[float(x) for x in origin])
+        self._ck(lib().vf_trace_probes(self._h, o, float(spacing), _i3(dims), int(n_rays), int(frame), float(reach),
+                                       _ptr(hits)))
+
+    # measurement
+    def set_counting(self, on):
+        self._ck(lib().vf_set_counting(self._h, int(bool(on))))
+
+    def counters(self, reset=True):
+        out = (C.c_uint64 * 4)()
+        self._ck(lib().vf_read_counters(self._h, out, int(bool(reset))))
+        return {"rays": out[0], "outer": out[1], "voxel": out[2], "bytes": out[3]}
+
+    def last_kernel_ms(self):
+        ms = C.c_float()
+        self._ck(lib().vf_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+
+def hits_to_numpy(t):
+    """A torch uint8/int32 tensor holding vf_hit records -> structured numpy array."""
+    raw = t.detach().cpu().contiguous().view(-1).numpy().view(np.uint8)
+    return raw.view(HIT_DTYPE)

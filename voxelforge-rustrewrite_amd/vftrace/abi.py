@@ -1,0 +1,121 @@
+"""ctypes mirrors of the C ABI structs (include/vf_trace.h, include/vf_worldgen.h).
+
+The field order and sizes follow the reference's #[repr(C)] / std140 structs: SceneUbo and PassUbo
+(src/gfx/ubo.rs:10-131, offsets asserted at :76-86 and :132-139) and Material
+(src/voxel/world.rs:21-31).  The asserts at the bottom of this module are the same ones.
+"""
+import ctypes as C
+
+import numpy as np
+
+f3 = C.c_float * 3
+f4 = C.c_float * 4
+i3 = C.c_int32 * 3
+i4 = C.c_int32 * 4
+
+
+class Material(C.Structure):
+    _fields_ = [("r", C.c_float), ("g", C.c_float), ("b", C.c_float), ("emissive", C.c_float),
+                ("props", f4), ("surf", f4)]
+
+
+class SceneUbo(C.Structure):
+    _fields_ = [
+        ("atlas_bricks", C.c_uint32 * 3), ("night", C.c_float),
+        ("world_bricks", i3), ("num_objects", C.c_int32),
+        ("sun_dir", f3), ("grass_dist", C.c_float),
+        ("sun_rgb", f3), ("sun_angle", C.c_float),
+        ("fog_sun_rgb", f3), ("fog_density", C.c_float),
+        ("fog_ambient", f3), ("fog_g", C.c_float),
+        ("sky_zenith", f3), ("sky_scale", C.c_float),
+        ("sky_horizon", f3), ("gi_scale", C.c_float),
+        ("sunset_rgb", f3), ("sky_model", C.c_int32),
+        ("fog_height", C.c_float), ("fog_falloff", C.c_float), ("fog_noise", C.c_float), ("fog_steps", C.c_int32),
+        ("fog_shadow_max", C.c_float), ("tonemap_op", C.c_int32), ("saturation", C.c_float), ("contrast", C.c_float),
+        ("white_balance", f3), ("grass_shadows", C.c_int32),
+        ("sh_right", f3), ("sh_texel0", C.c_float),
+        ("sh_up", f3), ("sh_texel1", C.c_float),
+        ("sh_lz_top", C.c_float), ("sh_texel2", C.c_float), ("sh_n", C.c_int32), ("std_lighting", C.c_int32),
+        ("sh_win_a", i4), ("sh_win_b", i4),
+        ("probe_origin", i3), ("probe_spacing", C.c_float),
+        ("probe_dims", i3), ("probe_update_offset", C.c_int32),
+        ("probe_hyst", C.c_float), ("probe_normal_bias", C.c_float), ("std_soft_taps", C.c_float),
+        ("std_cascade_blend", C.c_float),
+        ("std_cascade_bias", C.c_int32), ("std_view_bias", C.c_float), ("sh_texel3", C.c_float),
+        ("std_soft_min", C.c_float),
+        ("probe_origin_c", i3), ("probe_spacing_c", C.c_float),
+        ("world_win", i4), ("hor_a", i4), ("hor_b", i4),
+        ("chunk_fade", C.c_uint32 * 256),
+        ("gi_cache_cam", f3), ("gi_cache_lod", C.c_float),
+        ("gi_cache_p", i4), ("gi_cache_q", f4), ("hor_c", i4),
+        ("cloud_a", f4), ("cloud_b", f4), ("rain_a", f4),
+    ]
+
+
+class PassUbo(C.Structure):
+    _fields_ = [
+        ("cam_pos", f3), ("lock_active", C.c_float),
+        ("cam_right", f3), ("fog_start", C.c_float),
+        ("cam_up", f3), ("haze_k", C.c_float),
+        ("cam_fwd", f3), ("tan_half_fov", C.c_float),
+        ("prev_cam_pos", f3), ("prev_tan_half_fov", C.c_float),
+        ("prev_cam_right", f3), ("haze_sea", C.c_float),
+        ("prev_cam_up", f3), ("_pf", C.c_float),
+        ("prev_cam_fwd", f3), ("frame_idx", C.c_uint32),
+        ("exposure", C.c_float), ("u_step", C.c_int32), ("u_iteration", C.c_int32), ("debug_mode", C.c_int32),
+        ("time", C.c_float), ("prev_time", C.c_float), ("jitter", C.c_float * 2),
+        ("water_p", f4), ("look_p", f4), ("fog_p", f4), ("under_p", f4),
+        ("opt_p", f4), ("wave_p", f4), ("shift_p", f4), ("facet_p", f4),
+    ]
+
+
+class Ray(C.Structure):
+    _fields_ = [("o", f3), ("d", f3)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("cell", i3), ("t", C.c_float), ("mat", C.c_uint8), ("face", C.c_uint8),
+                ("hit", C.c_uint8), ("pad", C.c_uint8)]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("world_bricks", i3), ("world_win", i4),
+        ("grid", C.POINTER(C.c_uint32)), ("atlas", C.POINTER(C.c_uint8)), ("n_slots", C.c_uint32),
+        ("masks", C.POINTER(C.c_uint32)), ("coarse1", C.POINTER(C.c_uint8)), ("coarse2", C.POINTER(C.c_uint8)),
+        ("palette", C.POINTER(Material)),
+    ]
+
+
+class GenParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("seed", C.c_uint32), ("world_bricks", i3), ("chunks", C.c_int32),
+        ("octaves", C.c_int32), ("surface_mean", C.c_float), ("amplitude", C.c_float), ("period", C.c_float),
+        ("water_level", C.c_int32), ("trees_per_chunk", C.c_int32), ("threads", C.c_int32),
+    ]
+
+
+# numpy views of the 24-byte ray and 20-byte hit records
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("d", "<f4", (3,))])
+HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
+                      ("pad", "u1")])
+
+# Layout asserts (src/gfx/ubo.rs:76-86, 132-139; src/voxel/world.rs:31).
+assert C.sizeof(SceneUbo) == 1536
+assert SceneUbo.fog_height.offset == 144
+assert SceneUbo.sh_win_a.offset == 240
+assert SceneUbo.probe_origin.offset == 272
+assert SceneUbo.world_win.offset == 352
+assert SceneUbo.chunk_fade.offset == 400
+assert SceneUbo.gi_cache_cam.offset == 1424
+assert SceneUbo.hor_c.offset == 1472
+assert SceneUbo.rain_a.offset == 1520
+assert C.sizeof(PassUbo) == 288
+assert PassUbo.prev_tan_half_fov.offset == 76
+assert PassUbo.prev_cam_fwd.offset == 112
+assert PassUbo.frame_idx.offset == 124
+assert PassUbo.jitter.offset == 152
+assert PassUbo.facet_p.offset == 272
+assert C.sizeof(Material) == 48
+assert C.sizeof(Ray) == 24 and RAY_DTYPE.itemsize == 24
+assert C.sizeof(Hit) == 20 and HIT_DTYPE.itemsize == 20
