@@ -45,7 +45,7 @@ typedef struct vf_material {
     float surf[4];    /* roughness, metalness, dielectric F0, detail */
 } vf_material;
 
-/* SceneUbo, std140, 1536 B (src/gfx/ubo.rs:10-86; offsets asserted there and in vf_abi.c). */
+/* SceneUbo, std140, 1536 B (src/gfx/ubo.rs:10-86; offsets asserted there and in vftrace/abi.py). */
 typedef struct vf_scene_ubo {
     uint32_t atlas_bricks[3]; float night;
     int32_t world_bricks[3];  int32_t num_objects;
@@ -203,6 +203,78 @@ int vf_trace_rays(vf_ctx *ctx, const vf_ray *d_rays, int64_t n, float t_cap, uin
  * p = (i % dims.x, i / dims.x % dims.y, i / dims.x / dims.y).  Hits [n_probes * n_rays]. */
 int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int32_t dims[3], int32_t n_rays,
                     uint32_t frame, float reach, vf_hit *d_hits);
+
+/* ---- persistent work-queue batches ------------------------------------------------------- */
+/* Queue parameters: the pixel tile that consecutive ray ids cover (8x8, 16x8, 16x16, 8x16, 32x8), rays
+ * a wavefront claims per queue fetch (64..4096, multiple of 64), whether GI rays are bucket-sorted by
+ * direction octant and origin cell first, and an optional cap on workgroups per CU (0 = as many as
+ * fit).  Results never depend on these. */
+typedef struct vf_queue_cfg {
+    int32_t tile_w, tile_h, chunk, sort, groups_per_cu;
+} vf_queue_cfg;
+int vf_queue_defaults(vf_queue_cfg *cfg);
+
+/* Config-3 style batch: one GI ray per full-resolution G-buffer pixel (gi.comp:55-79 with fpix = pix:
+ * cosine-hemisphere direction from the quantised normal and the R2 sequence, origin P + N*0.03,
+ * t_cap = reach) and one sun ray (giPathImpl's, common.glsl:2318-2330: from hit + N*0.03 toward
+ * sunDir, no cap) per GI hit that is not water and faces the sun.  Both go through the persistent
+ * queue kernel.  gi_hits / sun_hits: one record per pixel (miss where no ray).  Not a reference
+ * entry point: it batches what P::Gi traces, for incoherent-ray throughput. */
+int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, float reach,
+                      const float *d_gb_pos, const uint32_t *d_gb_normal, vf_hit *d_gi_hits, vf_hit *d_sun_hits,
+                      const vf_queue_cfg *cfg);
+
+#define VF_MAX_REQUESTS 24
+enum { VF_REQ_VIEW = 0, VF_REQ_PROBES = 1 };
+/* One request of an uneven batch: a camera view (primary rays as vf_trace_primary, hits [w*h]) or a
+ * DDGI probe grid (as vf_trace_probes, hits [n_probes*n_rays]). */
+typedef struct vf_request {
+    int32_t kind;
+    int32_t width, height;
+    float cam_pos[3], cam_right[3], cam_up[3], cam_fwd[3];
+    float tan_half_fov;
+    float jitter[2];
+    float origin[3];
+    float spacing;
+    int32_t dims[3];
+    int32_t n_rays;
+    uint32_t frame;
+    float reach;
+    vf_hit *d_hits;
+} vf_request;
+/* All requests in ONE persistent launch; each request's completion time is stamped on the device.
+ * Request order is the service order (put the latency-critical ones first).  Recordable into a graph. */
+int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg);
+/* Per-request completion times of the last batch, ms after the launch started (host, synchronises). */
+int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms);
+
+/* Autotuner: times every tile shape {8x8, 16x8, 16x16} x chunk {64..1024} (x sort off/on for the GI
+ * batch) on the given batch (median of 3 runs) and returns the fastest configuration. */
+enum { VF_TUNE_GI_BATCH = 0, VF_TUNE_BATCH = 1 };
+typedef struct vf_tune_result { vf_queue_cfg cfg; float ms; } vf_tune_result;
+typedef struct vf_tune_gi_args {
+    const vf_pass_ubo *pass;
+    int32_t width, height;
+    float reach;
+    const float *d_gb_pos;
+    const uint32_t *d_gb_normal;
+    vf_hit *d_gi_hits, *d_sun_hits;
+} vf_tune_gi_args;
+typedef struct vf_tune_batch_args { const vf_request *reqs; int32_t n_req; } vf_tune_batch_args;
+int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, vf_queue_cfg *best, vf_tune_result *results,
+                int32_t max_results, int32_t *n_results);
+
+/* ---- frame graphs ------------------------------------------------------------------------- */
+/* A frame's trace calls recorded once and replayed (the reference records a command buffer per
+ * frame, record.rs:320).  Between vf_graph_begin and vf_graph_end the trace entry points are
+ * recorded, not run; calls that synchronise or copy (uploads, reads, vf_sync) are refused.  The
+ * recorded calls keep their arguments (buffers, uniforms) as they were at capture time.  The
+ * context's stream must not be the null stream. */
+typedef struct vf_graph vf_graph;
+int vf_graph_begin(vf_ctx *ctx);
+int vf_graph_end(vf_ctx *ctx, vf_graph **out);
+int vf_graph_launch(vf_ctx *ctx, vf_graph *graph);
+void vf_graph_destroy(vf_graph *graph);
 
 /* ---- measurement --------------------------------------------------------------------------- */
 /* When on, the traces launch their counting variants, which add the reference visit sequence's

@@ -84,10 +84,13 @@ def lib():
             "vfo_primary_dir": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]),
             "vfo_probe_dir": (None, [C.c_int32, C.c_int32, C.c_uint32, P]),
             "vfo_pass_primary": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, C.c_int]),
+            "vfo_pass_primary_rows": (None, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
+                                              C.c_int]),
             "vfo_pass_shadow": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
             "vfo_pass_gi": (None, [P, P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
             "vfo_gi_path": (None, [P, P, P, P, P, P, P, P]),
             "vfo_sky_open": (C.c_float, [P, P, P]),
+            "vfo_gi_rays": (None, [C.c_int32, C.c_int32, C.c_uint32, P, P, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -109,7 +112,7 @@ class OracleWorld:
     """The oracle's own copy of a world: its grid and atlas plus the derived fields it builds itself
     (coarse pyramid, both distance fields by record_distance_field, sky-top map)."""
 
-    def __init__(self, arrays, use_dist=True, coarse_skip=True, build_fields=True):
+    def __init__(self, arrays, use_dist=True, coarse_skip=True, build_fields=True, build_skytop=True):
         self.wb = tuple(int(x) for x in arrays["world_bricks"])
         self.win = tuple(int(x) for x in arrays["world_win"])
         self.grid = np.ascontiguousarray(arrays["grid"], dtype=np.uint32)
@@ -143,8 +146,9 @@ class OracleWorld:
         self.s.use_dist = int(bool(use_dist))
         self.s.coarse_skip = int(bool(coarse_skip))
         self.skytop = np.zeros(self.wb[0] * 8 * self.wb[2] * 8, np.uint16)
-        kinds = np.ascontiguousarray(self.palette[:, 6])
-        lib().vfo_skytop(C.byref(self.s), _p(kinds), _p(self.skytop))
+        if build_skytop:
+            kinds = np.ascontiguousarray(self.palette[:, 6])
+            lib().vfo_skytop(C.byref(self.s), _p(kinds), _p(self.skytop))
 
     def ring(self):
         return self.win[3] != 0
@@ -216,6 +220,22 @@ class OracleWorld:
         cnt = VfoCount()
         lib().vfo_pass_primary(C.byref(self.s), C.byref(c), int(width), int(height), _p(hits), _p(pos), _p(nrm),
                                C.byref(cnt), int(threads))
+        return hits, pos, nrm, cnt.as_dict()
+
+    def pass_primary_rows(self, cam, width, height, y0, ystep, nrows, threads=0):
+        c = VfoCam()
+        c.pos[:] = [float(x) for x in cam["pos"]]
+        c.right[:] = [float(x) for x in cam["right"]]
+        c.up[:] = [float(x) for x in cam["up"]]
+        c.fwd[:] = [float(x) for x in cam["fwd"]]
+        c.tan_half_fov = float(cam["tan_half_fov"])
+        n = width * nrows
+        hits = np.zeros(n, HIT_DTYPE)
+        pos = np.zeros((n, 4), np.float32)
+        nrm = np.zeros(n, np.uint32)
+        cnt = VfoCount()
+        lib().vfo_pass_primary_rows(C.byref(self.s), C.byref(c), int(width), int(height), int(y0), int(ystep),
+                                    int(nrows), _p(hits), _p(pos), _p(nrm), C.byref(cnt), int(threads))
         return hits, pos, nrm, cnt.as_dict()
 
     def pass_shadow(self, sun, width, height, gb_pos, gb_normal, threads=0):
@@ -295,3 +315,48 @@ def probe_dirs(n_rays, frame):
         lib().vfo_probe_dir(i, n_rays, int(frame), _p(tmp))
         out[i] = tmp
     return out
+
+
+def gi_rays(width, height, frame, gb_pos, gb_normal):
+    """GI batch rays (float32 [N, 6]) and the non-sky mask of a full-resolution G-buffer."""
+    rays = np.zeros((width * height, 6), np.float32)
+    valid = np.zeros(width * height, np.uint8)
+    pos32 = f32(gb_pos)
+    nrm = np.ascontiguousarray(gb_normal, dtype=np.uint32)
+    lib().vfo_gi_rays(int(width), int(height), int(frame), _p(pos32), _p(nrm), _p(rays), _p(valid))
+    return rays, valid.astype(bool)
+
+
+def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_counts=False):
+    """The GI batch on the oracle: GI rays (t_cap = reach, skipMat 0), then for each hit that is not
+    water and faces the sun, its sun ray from hit + N*0.03 (giPathImpl stage 1).  Per-pixel hits."""
+    rays, valid = gi_rays(width, height, frame, gb_pos, gb_normal)
+    gi = np.zeros(width * height, HIT_DTYPE)
+    gi["t"] = -1.0
+    idx = np.nonzero(valid)[0]
+    h, cnt = ow.trace_rays(rays[idx], reach, 0)
+    gi[idx] = h
+    sun = np.asarray(sun, np.float32)
+    face = h["face"].astype(np.int64)
+    hn = np.zeros((len(h), 3), np.float32)
+    ax = (face - 1) // 2
+    sgn = np.where(face % 2 == 1, 1.0, -1.0).astype(np.float32)
+    has = face > 0
+    hn[np.nonzero(has)[0], ax[has]] = sgn[has]
+    ndl = (hn[:, 0] * sun[0] + hn[:, 1] * sun[1]) + hn[:, 2] * sun[2]
+    emit = (h["hit"] == 1) & (h["mat"] != 5) & (np.maximum(ndl, np.float32(0.0)) > 0)
+    o = rays[idx, :3]
+    d = rays[idx, 3:]
+    t = h["t"].astype(np.float32)[:, None]
+    hp = o + d * t
+    so = hp + hn * np.float32(3e-2)
+    sun_rays = np.concatenate([so, np.broadcast_to(sun, so.shape)], 1)[emit].astype(np.float32)
+    sh = np.zeros(width * height, HIT_DTYPE)
+    sh["t"] = -1.0
+    if len(sun_rays):
+        s_hits, c2 = ow.trace_rays(sun_rays, 1e30, 0)
+        sh[idx[emit]] = s_hits
+        cnt = {k: cnt[k] + c2[k] for k in cnt}
+    if return_counts:
+        return gi, sh, cnt
+    return gi, sh

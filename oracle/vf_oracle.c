@@ -684,6 +684,41 @@ void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H
     if (cnt) add_cnt(cnt, &total);
 }
 
+/* Rows y0, y0 + ystep, ... (nrows of them) of the primary pass of a width x height image: the CPU
+ * baseline's bounded sample.  Outputs are packed row after row. */
+void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H, int32_t y0, int32_t ystep,
+                           int32_t nrows, vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt,
+                           int threads) {
+    vfo_count total = {0, 0, 0, 0};
+#pragma omp parallel num_threads(nthreads(threads))
+    {
+        vfo_count cc = {0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)W * nrows; ++i) {
+            int32_t px = (int32_t)(i % W), py = y0 + ystep * (int32_t)(i / W);
+            float rd[3];
+            vfo_primary_dir(c->right, c->up, c->fwd, c->tan_half_fov, W, H, px, py, c->jitter, rd);
+            vfo_hit h;
+            vfo_trace(w, c->pos, rd, 1e30f, 0u, &h, &cc);
+            h.t = h.t + 0.0f;
+            to_rec(&h, &hits[i]);
+            if (h.hit) {
+                for (int a = 0; a < 3; ++a) gb_pos[i * 4 + a] = c->pos[a] + rd[a] * h.t;
+                gb_pos[i * 4 + 3] = h.t;
+                gb_normal[i] = unorm8((float)h.n[0] * 0.5f + 0.5f) | (unorm8((float)h.n[1] * 0.5f + 0.5f) << 8) |
+                               (unorm8((float)h.n[2] * 0.5f + 0.5f) << 16) | (255u << 24);
+            } else {
+                gb_pos[i * 4 + 0] = gb_pos[i * 4 + 1] = gb_pos[i * 4 + 2] = 0.0f;
+                gb_pos[i * 4 + 3] = -1.0f;
+                gb_normal[i] = 0;
+            }
+        }
+#pragma omp critical
+        add_cnt(&total, &cc);
+    }
+    if (cnt) add_cnt(cnt, &total);
+}
+
 void vfo_pass_shadow(const vfo_world *w, const float sun[3], int32_t W, int32_t H, const float *gb_pos,
                      const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits, vfo_count *cnt, int threads) {
     vfo_count total = {0, 0, 0, 0};
@@ -915,4 +950,24 @@ void vfo_pass_gi(const vfo_world *w, const vfo_sky *s, uint32_t frame, int32_t W
         add_cnt(&total, &cc);
     }
     if (cnt) add_cnt(cnt, &total);
+}
+
+/* The GI rays of one full-resolution G-buffer (gi.comp:55-79 with fpix = pix): for each non-sky pixel,
+ * origin P + N*0.03 and direction cosineHemisphereU(N, ldSample2(pix, frame >> 2, 0x68e31da4)) from
+ * the quantised normal; rays[i] for pixel i, valid[i] = 0 for sky. */
+void vfo_gi_rays(int32_t W, int32_t H, uint32_t frame, const float *gb_pos, const uint32_t *gb_normal, vfo_ray *rays,
+                 uint8_t *valid) {
+    for (int64_t i = 0; i < (int64_t)W * H; ++i) {
+        int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
+        const float *P = gb_pos + i * 4;
+        valid[i] = P[3] >= 0.0f;
+        memset(&rays[i], 0, sizeof(rays[i]));
+        if (!valid[i]) continue;
+        uint32_t nb = gb_normal[i];
+        float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+        float u[2];
+        vfo_ld_sample2(px, py, frame >> 2u, 0x68e31da4u, u);
+        vfo_cosine_hemisphere_u(N, u, rays[i].d);
+        for (int a = 0; a < 3; ++a) rays[i].o[a] = P[a] + N[a] * 3e-2f;
+    }
 }

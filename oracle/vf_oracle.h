@@ -132,6 +132,9 @@ typedef struct vfo_sky {
 /* gbuffer.comp primary pass (world only, camera in air): hits + gbPos (xyz, t) + gbNormal (RGBA8). */
 void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height,
                       vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads);
+void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height, int32_t y0,
+                           int32_t ystep, int32_t nrows, vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal,
+                           vfo_count *cnt, int threads);
 /* shadow_trace.comp exact sun ray per pixel: shadowRaw as two floats (vis, min(tBlock, 60000)). */
 void vfo_pass_shadow(const vfo_world *w, const float sun_dir[3], int32_t width, int32_t height,
                      const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
@@ -145,6 +148,9 @@ void vfo_pass_gi(const vfo_world *w, const vfo_sky *s, uint32_t frame_idx, int32
 void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
                  float L[3], vfo_hit *bounce, vfo_count *cnt);
 float vfo_sky_open(const vfo_world *w, const vfo_sky *s, const float p[3]);
+/* GI rays of a full-resolution G-buffer (the GI batch: one per non-sky pixel). */
+void vfo_gi_rays(int32_t width, int32_t height, uint32_t frame, const float *gb_pos, const uint32_t *gb_normal,
+                 vfo_ray *rays, uint8_t *valid);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,165 @@
+"""GPU parity of the persistent work-queue paths: the GI batch (GI ray per pixel + sun ray per GI hit)
+against the oracle, and the uneven view + probe batch against per-request plain launches.  Hit records
+must be bit-identical whatever the tile shape, chunk size or sorting."""
+import numpy as np
+import pytest
+import torch
+
+import vftrace as vf
+import oracle as O
+from helpers import camera, config1_camera, hit_diff, world
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vf.Context(0)
+    w, a, _ = world(1)
+    c.upload(w)
+    c.set_scene(vf.default_scene(a["world_bricks"], a["world_win"]))
+    yield c
+    c.close()
+
+
+def dev_hits(n):
+    return torch.zeros(max(n, 1) * 20, dtype=torch.uint8, device=DEV)
+
+
+def gbuffer(ctx, pu, W, H):
+    hits = dev_hits(W * H)
+    gbp = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
+    gbn = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    ctx.trace_primary(pu, W, H, hits, gbp, gbn)
+    ctx.sync()
+    return gbp, gbn
+
+
+@pytest.mark.parametrize("tile,chunk,sort", [((8, 8), 64, False), ((16, 8), 256, True), ((16, 16), 1024, True),
+                                             ((8, 8), 512, True)])
+def test_gi_batch_matches_oracle(ctx, tile, chunk, sort):
+    _, a, ow = world(1)
+    pu, _ = config1_camera(frame=6)
+    W, H = 640, 360
+    gbp, gbn = gbuffer(ctx, pu, W, H)
+    gi = dev_hits(W * H)
+    sun = dev_hits(W * H)
+    ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg(tile, chunk, sort))
+    ctx.sync()
+    scene = vf.default_scene(a["world_bricks"])
+    og, osun = O.gi_batch(ow, W, H, 6, gbp.cpu().numpy().reshape(-1, 4), gbn.cpu().numpy().view(np.uint32),
+                          np.array(scene.sun_dir, np.float32), 640.0)
+    g = vf.hits_to_numpy(gi)[:W * H]
+    s = vf.hits_to_numpy(sun)[:W * H]
+    assert hit_diff(g, og).size == 0
+    assert hit_diff(s, osun).size == 0
+    assert 0.05 < g["hit"].mean() < 0.95 and s["hit"].sum() > 0
+
+
+def test_gi_batch_counters_match_oracle(ctx):
+    _, a, ow = world(1)
+    pu, _ = config1_camera(frame=2)
+    W, H = 320, 180
+    gbp, gbn = gbuffer(ctx, pu, W, H)
+    gi, sun = dev_hits(W * H), dev_hits(W * H)
+    ctx.set_counting(True)
+    try:
+        ctx.counters(reset=True)
+        ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg((16, 8), 128, True))
+        c = ctx.counters(reset=True)
+    finally:
+        ctx.set_counting(False)
+    scene = vf.default_scene(a["world_bricks"])
+    _, _, oc = O.gi_batch(ow, W, H, 2, gbp.cpu().numpy().reshape(-1, 4), gbn.cpu().numpy().view(np.uint32),
+                          np.array(scene.sun_dir, np.float32), 640.0, return_counts=True)
+    assert c == oc
+
+
+def _requests(ctx, sizes, probes):
+    reqs, outs, passes = [], [], []
+    rng = np.random.default_rng(5)
+    for (w, h) in sizes:
+        pos = [rng.uniform(20, 230), rng.uniform(70, 120), rng.uniform(20, 230)]
+        pu, _ = camera(pos, rng.uniform(0, 6.28), rng.uniform(-0.6, 0.1))
+        out = dev_hits(w * h)
+        reqs.append(ctx.view_request(pu, w, h, out))
+        outs.append(out)
+        passes.append((pu, w, h))
+    for (origin, spacing, dims, nr, frame, reach) in probes:
+        out = dev_hits(int(np.prod(dims)) * nr)
+        reqs.append(ctx.probe_request(origin, spacing, dims, nr, frame, reach, out))
+        outs.append(out)
+        passes.append((origin, spacing, dims, nr, frame, reach))
+    return reqs, outs, passes
+
+
+def test_uneven_batch_matches_plain_launches(ctx):
+    sizes = [(64, 64), (256, 144), (130, 75), (480, 270)]
+    probes = [((-64.5, -32.25, -64.75), 20.0, (8, 6, 8), 256, 3, 256.0)]
+    reqs, outs, passes = _requests(ctx, sizes, probes)
+    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 1024), ctx.queue_cfg((16, 8), 256)):
+        for o in outs:
+            o.zero_()
+        ctx.trace_batch(reqs, cfg)
+        ctx.sync()
+        times = ctx.batch_times(len(reqs))
+        assert all(t > 0 for t in times), times
+        for k, p in enumerate(passes):
+            if k < len(sizes):
+                pu, w, h = p
+                ref = dev_hits(w * h)
+                ctx.trace_primary(pu, w, h, ref)
+                n = w * h
+            else:
+                origin, spacing, dims, nr, frame, reach = p
+                n = int(np.prod(dims)) * nr
+                ref = dev_hits(n)
+                ctx.trace_probes(origin, spacing, dims, nr, frame, reach, ref)
+            ctx.sync()
+            assert torch.equal(outs[k][:n * 20], ref[:n * 20]), (k, cfg.tile_w, cfg.chunk)
+
+
+def test_batch_and_gi_batch_in_a_graph(ctx):
+    sizes = [(96, 64), (200, 120)]
+    probes = [((10.5, 60.5, 10.5), 20.0, (4, 2, 4), 64, 1, 128.0)]
+    reqs, outs, _ = _requests(ctx, sizes, probes)
+    pu, _ = config1_camera(frame=3)
+    W, H = 160, 96
+    gbp, gbn = gbuffer(ctx, pu, W, H)
+    gi, sun = dev_hits(W * H), dev_hits(W * H)
+    s = torch.cuda.Stream()
+    ctx.set_stream(s)
+    try:
+        cfg = ctx.queue_cfg((16, 8), 128, True)
+        ctx.trace_batch(reqs, cfg)          # eager run first: sizes the queue buffers
+        ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, cfg)
+        ctx.sync()
+        eager = [o.clone() for o in outs] + [gi.clone(), sun.clone()]
+        ctx.graph_begin()
+        ctx.trace_batch(reqs, cfg)
+        ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, cfg)
+        g = ctx.graph_end()
+        for rep in range(3):
+            for o in outs + [gi, sun]:
+                o.zero_()
+            torch.cuda.synchronize()
+            ctx.graph_launch(g)
+            ctx.sync()
+            for e, o in zip(eager, outs + [gi, sun]):
+                assert torch.equal(e, o), rep
+        g.close()
+    finally:
+        ctx.set_stream(None)
+
+
+def test_autotuner_picks_a_measured_config(ctx):
+    pu, _ = config1_camera(frame=1)
+    W, H = 256, 144
+    gbp, gbn = gbuffer(ctx, pu, W, H)
+    gi, sun = dev_hits(W * H), dev_hits(W * H)
+    best, table = ctx.autotune_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun)
+    assert len(table) == 3 * 5 * 2
+    fastest = min(table, key=lambda r: r[3])
+    assert (best.tile_w, best.tile_h) == fastest[0] and best.chunk == fastest[1] and bool(best.sort) == fastest[2]
+    assert all(r[3] > 0 for r in table)

@@ -259,3 +259,42 @@ def test_analytic_scenes(ctx):
     assert g0["hit"] and g0["mat"] == 5 and tuple(g0["cell"]) == (10, 13, 10) and g0["face"] == 3
     assert g5["hit"] and g5["mat"] == 3 and tuple(g5["cell"]) == (10, 7, 10) and g5["face"] == 3
     assert g0["t"] == np.float32(16.0) and g5["t"] == np.float32(22.0)
+
+
+def test_frame_graph_replays_match_eager(ctx):
+    """A frame recorded with vf_graph_begin/end replays to the eager result, several times, with eager
+    work and synchronisations between the replays; calls that synchronise are refused while recording."""
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    pu, _ = config1_camera()
+    W, H = 320, 180
+    s = torch.cuda.Stream()
+    ctx.set_stream(s)
+    try:
+        e_hits, e_gbp, e_gbn = primary_gpu(ctx, pu, W, H)
+        e_sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        ctx.trace_shadow(W, H, e_gbp, e_gbn, e_sh)
+        ctx.sync()
+        hits = dev_hits(W * H)
+        gbp = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
+        gbn = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        ctx.graph_begin()
+        with pytest.raises(vf.VfError):
+            ctx.sync()
+        ctx.trace_primary(pu, W, H, hits, gbp, gbn)
+        ctx.trace_shadow(W, H, gbp, gbn, sh)
+        g = ctx.graph_end()
+        for k in range(3):
+            hits.zero_(); gbp.zero_(); gbn.zero_(); sh.zero_()
+            torch.cuda.synchronize()
+            ctx.graph_launch(g)
+            ctx.sync()
+            assert torch.equal(hits, e_hits) and torch.equal(gbp, e_gbp) and torch.equal(sh, e_sh), k
+            rays = torch.from_numpy(random_rays(1000, [0, 60, 0], [256, 128, 256], seed=k)).to(DEV)
+            tmp = dev_hits(1000)
+            ctx.trace_rays(rays, 1000, tmp)    # eager work between replays
+            ctx.sync()
+        g.close()
+    finally:
+        ctx.set_stream(None)

@@ -1,0 +1,71 @@
+// The context object behind the C ABI's opaque vf_ctx (host side, shared by the library's .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "vf_device.h"
+#include "vf_internal.h"
+
+#define VF_HIP(ctx, call)                                                                                         \
+    do {                                                                                                          \
+        hipError_t e_ = (call);                                                                                   \
+        if (e_ != hipSuccess)                                                                                     \
+            return vf::fail((ctx), e_ == hipErrorOutOfMemory ? VF_E_OOM : VF_E_DEVICE, "%s failed: %s (%s:%d)", #call, \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                                           \
+    } while (0)
+
+struct vf_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_world = false;
+    bool has_scene = false;
+    DevWorld W{};
+    SkyParams S{};
+    vf_scene_ubo scene{};
+    // owned device buffers
+    uint32_t *grid = nullptr;
+    uint2 *cells = nullptr;
+    uint8_t *atlas = nullptr;
+    uint32_t *occ = nullptr, *occdry = nullptr, *masks_ref = nullptr;
+    uint8_t *dist[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t *c1 = nullptr, *c2 = nullptr;
+    uint16_t *skytop = nullptr;
+    float4 *palette = nullptr;
+    int32_t c1d[3] = {0, 0, 0}, c2d[3] = {0, 0, 0};
+    uint32_t n_slots = 0;
+    size_t n_cells = 0, n_cells_tiled = 0;
+    bool counting = false;
+    unsigned long long *d_cnt = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    bool capturing = false;   // inside vf_graph_begin / vf_graph_end: no syncs, no timing events
+    int n_cu = 256;           // compute units (persistent grids)
+    long long wall_khz = 100000;   // wall clock rate (batch completion stamps)
+    // persistent-queue buffers (vf_queue.hip), grown on first use outside a capture
+    unsigned long long *q_ctr = nullptr, *q_stamps = nullptr;
+    float4 *q_rays = nullptr, *q_rays2 = nullptr, *q_sun = nullptr;
+    uint32_t *q_keys = nullptr, *q_buckets = nullptr;
+    int64_t q_cap = 0;
+};
+
+struct vf_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace vf {
+// helpers defined in vf_trace.hip
+int set_device(vf_ctx *ctx);
+int launch_check(vf_ctx *ctx, const char *what);
+int begin_timed(vf_ctx *ctx);
+int end_timed(vf_ctx *ctx);
+int check_ready(vf_ctx *ctx, const char *what);
+int no_capture(vf_ctx *ctx, const char *what);
+void free_queue(vf_ctx *ctx);
+inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+}  // namespace vf

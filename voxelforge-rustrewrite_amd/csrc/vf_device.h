@@ -1,0 +1,534 @@
+// Device-side traversal shared by the pass kernels (vf_trace.hip) and the queue kernels (vf_queue.hip):
+// the world view, traceVolumeS / brickVoxels (reference shaders/common.glsl:474-658) in the reference's
+// FP32 operation order, and the RNG / sampling helpers of the passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/vf_trace.h"
+
+// ------------------------------------------------------------------------------------------------
+// device-side world
+// ------------------------------------------------------------------------------------------------
+struct DevWorld {
+    int32_t wb[3];
+    int32_t minb[3];     // worldMinB(): (win.x, 0, win.z) in ring mode, else 0
+    int32_t ring;
+    int32_t use_dist;    // USE_DIST_FIELD
+    int32_t tdim[3];     // 4^3-brick tiles per axis of the cell-word array
+    int32_t max_outer;   // 3 * (wbx + wby + wbz) + 4
+    const uint2 *cells;  // .x grid value, .y meta: dist | dry << 8 | c2empty << 16 | c1empty << 17
+    const uint8_t *atlas;
+    const uint4 *occ;     // per slot 4 x uint4: bit (lz*64 + ly*8 + lx) = voxel != 0
+    const uint4 *occdry;  // per slot: voxel != 0 && voxel != 5
+    const uint16_t *skytop;
+    const float4 *palette;  // 256 x 3 float4 (Material)
+};
+
+struct SkyParams {
+    float sun_dir[3], sun_rgb[3], sky_zenith[3], sky_horizon[3], sunset_rgb[3];
+    float sky_scale, gi_scale, night;
+    int32_t sky_model;
+};
+
+struct Cnt {
+    unsigned long long rays, outer, voxel, bytes;
+};
+
+struct Hit {
+    int32_t hit;
+    float t;
+    int32_t face;
+    uint32_t mat;
+    int32_t cell[3];
+};
+
+// GLSL built-ins with their specified definitions (signed zeros matter for bit equality).
+__device__ __forceinline__ float gmin(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ int32_t imin(int32_t x, int32_t y) { return y < x ? y : x; }
+__device__ __forceinline__ int32_t imax(int32_t x, int32_t y) { return x < y ? y : x; }
+__device__ __forceinline__ int32_t iclamp(int32_t x, int32_t lo, int32_t hi) { return imin(imax(x, lo), hi); }
+__device__ __forceinline__ float fclampg(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__device__ __forceinline__ int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
+    float x = a[1] * b[2] - b[1] * a[2];
+    float y = a[2] * b[0] - b[2] * a[0];
+    float z = a[0] * b[1] - b[0] * a[1];
+    o[0] = x; o[1] = y; o[2] = z;
+}
+__device__ __forceinline__ void normalize3(float v[3]) {
+    float len = sqrtf(dot3(v, v));
+    v[0] = v[0] / len; v[1] = v[1] / len; v[2] = v[2] / len;
+}
+
+// Cell word index of absolute brick (x, y, z) inside the window (gridWrap, common.glsl:134-136).
+__device__ __forceinline__ uint32_t cell_index(const DevWorld &W, int32_t x, int32_t y, int32_t z) {
+    if (W.ring) { x &= W.wb[0] - 1; z &= W.wb[2] - 1; }
+    uint32_t tile = ((uint32_t)(z >> 2) * (uint32_t)W.tdim[1] + (uint32_t)(y >> 2)) * (uint32_t)W.tdim[0] + (uint32_t)(x >> 2);
+    return tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
+}
+
+// 64 occupancy bits of brick layer lz (two 32-bit words) out of the 512-bit mask.
+__device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, const uint4 &m2, const uint4 &m3, int lz,
+                                           uint32_t &lo, uint32_t &hi) {
+    uint4 a = (lz & 2) ? m1 : m0;
+    uint4 b = (lz & 2) ? m3 : m2;
+    uint4 q = (lz & 4) ? b : a;
+    lo = (lz & 1) ? q.z : q.x;
+    hi = (lz & 1) ? q.w : q.y;
+}
+
+// brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
+// arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
+template <bool COUNT>
+__device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
+                                             const int32_t st[3], const float up[3], const int32_t faceA[3],
+                                             const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c,
+                                             const int skip) {
+    const uint32_t slot = g - 256u;
+    const uint4 *mp = (skip == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
+    const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
+    int32_t bmin[3], vc[3];
+    float tM[3], tD[3];
+    const float e = t + 1e-3f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bmin[a] = bc[a] * 8;
+        float q = ro[a] + sd[a] * e;
+        vc[a] = iclamp(f2i(floorf(q)), bmin[a], bmin[a] + 7);
+        tM[a] = (((float)vc[a] + up[a]) - ro[a]) * inv[a];
+        tD[a] = fabsf(inv[a]);
+    }
+    int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
+    uint32_t wlo, whi;
+    mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+    float tv = t;
+    int32_t f2 = face;
+    bool hit = false;
+    // One DDA step per trip, written with selects instead of a 3-way branch: the axis choice and
+    // the updates are the reference's (strict <, ties x then y then z; tMax += tDelta); the mask
+    // layer is re-selected only after a step along z.  Scalars, not arrays, so nothing is indexed.
+    float tMx = tM[0], tMy = tM[1], tMz = tM[2];
+    const float tDx = tD[0], tDy = tD[1], tDz = tD[2];
+    const int32_t fA0 = faceA[0], fA1 = faceA[1], fA2 = faceA[2];
+    const int32_t s0 = st[0], s1 = st[1], s2 = st[2];
+    for (int j = 0; j < 26; ++j) {
+        if (COUNT) { c.voxel++; c.bytes += 1; }
+        const uint32_t wv = (ly & 4) ? whi : wlo;
+        if ((wv >> (((ly & 3) << 3) | lx)) & 1u) { hit = true; break; }
+        const bool a0 = tMx < tMy && tMx < tMz;
+        const bool a1 = !a0 && tMy < tMz;
+        const bool a2 = !a0 && !a1;
+        tv = a0 ? tMx : (a1 ? tMy : tMz);
+        f2 = a0 ? fA0 : (a1 ? fA1 : fA2);
+        const float nx = tMx + tDx, ny = tMy + tDy, nz = tMz + tDz;
+        tMx = a0 ? nx : tMx;
+        tMy = a1 ? ny : tMy;
+        tMz = a2 ? nz : tMz;
+        lx += a0 ? s0 : 0;
+        ly += a1 ? s1 : 0;
+        lz += a2 ? s2 : 0;
+        if (((uint32_t)lx | (uint32_t)ly | (uint32_t)lz) > 7u) break;
+        if (a2) mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+    }
+    if (hit) {
+        h.hit = 1;
+        h.t = tv;
+        h.face = f2;
+        h.mat = W.atlas[(size_t)slot * 512u + (uint32_t)(lz * 64 + ly * 8 + lx)];
+        h.cell[0] = bmin[0] + lx; h.cell[1] = bmin[1] + ly; h.cell[2] = bmin[2] + lz;
+        return true;
+    }
+    return false;
+}
+
+// traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5).
+template <int SKIP, bool COUNT>
+__device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
+                                            Cnt &c) {
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+    if (COUNT) c.rays++;
+    float sd[3], inv[3], up[3], tmin3[3], tmax3[3];
+    int32_t st[3], faceA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        inv[a] = 1.0f / sd[a];
+        const float lo = (float)(W.minb[a] * 8);
+        const float dims = (float)(W.wb[a] * 8);
+        const float t0 = (lo - ro[a]) * inv[a];
+        const float t1 = ((lo + dims) - ro[a]) * inv[a];
+        tmin3[a] = gmin(t0, t1);
+        tmax3[a] = gmax(t0, t1);
+        st[a] = sd[a] > 0.0f ? 1 : -1;
+        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
+        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);   // face code of normal -st on axis a
+    }
+    const float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
+    const float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
+    if (tExit < gmax(tEnter, 0.0f) || tEnter > tcap) return;
+
+    float t = gmax(tEnter, 0.0f);
+    int32_t bc[3];
+    float tD8[3], tM8[3];
+    {
+        const float e = t + 1e-3f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float entry = ro[a] + sd[a] * e;
+            bc[a] = iclamp(f2i(floorf(entry * 0.125f)), W.minb[a], W.minb[a] + W.wb[a] - 1);
+            tD8[a] = fabsf(inv[a]) * 8.0f;
+            tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
+        }
+    }
+    int32_t face = 0;
+    if (tEnter > 0.0f) {
+        if (tEnter == tmin3[0]) face = faceA[0];
+        else if (tEnter == tmin3[1]) face = faceA[1];
+        else face = faceA[2];
+    }
+    for (int i = 0; i < W.max_outer; ++i) {
+        if (t > tcap) return;
+        const uint2 cw = W.cells[cell_index(W, bc[0], bc[1], bc[2])];
+        uint32_t g = cw.x;
+        const uint32_t meta = cw.y;
+        if (COUNT) { c.outer++; c.bytes += 4; }
+        if (SKIP == 5) {
+            if (g == 5u) g = 0u;
+            else if (g >= 256u) {
+                if (COUNT) c.bytes += 1;
+                if (((meta >> 8) & 255u) > 0u) g = 0u;
+            }
+        }
+        if (g >= 256u) {
+            if (brick_voxels<COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, SKIP)) return;
+        } else if (g != 0u && g != (uint32_t)SKIP) {
+            const float e = t + 1e-3f;
+            h.hit = 1; h.t = t; h.face = face; h.mat = g;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) h.cell[a] = f2i(floorf(ro[a] + sd[a] * e));
+            return;
+        }
+        bool stepped = false;
+        if (g == 0u) {
+            int32_t Cf = 0, cb[3] = {0, 0, 0};
+            int32_t d = 0;
+            if (W.use_dist) {
+                d = (int32_t)(SKIP == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
+                if (COUNT) c.bytes += 1;
+            }
+            if (W.use_dist && d > 1) {
+                Cf = 2 * d - 1;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (d - 1);
+            } else {
+                if (COUNT) c.bytes += 1;
+                if (meta & (1u << 16)) Cf = 16;
+                else {
+                    if (COUNT) c.bytes += 1;
+                    Cf = (meta & (1u << 17)) ? 4 : 0;
+                }
+                if (Cf > 0) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (bc[a] & (Cf - 1));
+                }
+            }
+            if (Cf > 0) {
+                float tFar[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float cmin = (float)cb[a] * 8.0f;
+                    const float cmax = cmin + (float)Cf * 8.0f;
+                    tFar[a] = ((up[a] != 0.0f ? cmax : cmin) - ro[a]) * inv[a];
+                }
+                const bool ax0 = tFar[0] < tFar[1] && tFar[0] < tFar[2];
+                const bool ax1 = !ax0 && tFar[1] < tFar[2];
+                const bool ax2 = !ax0 && !ax1;
+                t = ax0 ? tFar[0] : (ax1 ? tFar[1] : tFar[2]);
+                face = ax0 ? faceA[0] : (ax1 ? faceA[1] : faceA[2]);
+                const float e = t + 1e-3f;
+                const bool axs[3] = {ax0, ax1, ax2};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    int32_t v = iclamp(f2i(floorf((ro[a] + sd[a] * e) * 0.125f)), cb[a], cb[a] + Cf - 1);
+                    if (axs[a]) v = st[a] > 0 ? cb[a] + Cf : cb[a] - 1;
+                    bc[a] = v;
+                    tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
+                }
+                stepped = true;
+            }
+        }
+        if (!stepped) {   // one brick step, strict <, ties x then y then z (common.glsl:644-655)
+            const bool a0 = tM8[0] < tM8[1] && tM8[0] < tM8[2];
+            const bool a1 = !a0 && tM8[1] < tM8[2];
+            const bool a2 = !a0 && !a1;
+            t = a0 ? tM8[0] : (a1 ? tM8[1] : tM8[2]);
+            face = a0 ? faceA[0] : (a1 ? faceA[1] : faceA[2]);
+            const float n0 = tM8[0] + tD8[0], n1 = tM8[1] + tD8[1], n2 = tM8[2] + tD8[2];
+            tM8[0] = a0 ? n0 : tM8[0];
+            tM8[1] = a1 ? n1 : tM8[1];
+            tM8[2] = a2 ? n2 : tM8[2];
+            bc[0] += a0 ? st[0] : 0;
+            bc[1] += a1 ? st[1] : 0;
+            bc[2] += a2 ? st[2] : 0;
+        }
+        if (bc[0] < W.minb[0] || bc[1] < W.minb[1] || bc[2] < W.minb[2] || bc[0] >= W.minb[0] + W.wb[0] ||
+            bc[1] >= W.minb[1] + W.wb[1] || bc[2] >= W.minb[2] + W.wb[2])
+            return;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// shared helpers of the passes
+// ------------------------------------------------------------------------------------------------
+// Workgroup id -> logical tile so that the 8 XCDs (workgroups are dealt round-robin) each get a
+// contiguous run of tiles and keep their L2 on one region of the world.
+__device__ __forceinline__ int32_t xcd_remap(int32_t i, int32_t n) {
+    const int32_t q = n >> 3, r = n & 7, x = i & 7;
+    return x * q + imin(x, r) + (i >> 3);
+}
+
+__device__ __forceinline__ void store_hit(vf_hit *out, const Hit &h) {
+    uint32_t *o = reinterpret_cast<uint32_t *>(out);
+    if (h.hit) {
+        o[0] = (uint32_t)h.cell[0]; o[1] = (uint32_t)h.cell[1]; o[2] = (uint32_t)h.cell[2];
+        o[3] = __float_as_uint(h.t);
+        o[4] = (h.mat & 255u) | ((uint32_t)h.face << 8) | (1u << 16);
+    } else {
+        o[0] = 0u; o[1] = 0u; o[2] = 0u;
+        o[3] = __float_as_uint(-1.0f);
+        o[4] = 0u;
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_count(const Cnt &c, unsigned long long *cnt) {
+    if (COUNT) {
+        atomicAdd(cnt + 0, c.rays);
+        atomicAdd(cnt + 1, c.outer);
+        atomicAdd(cnt + 2, c.voxel);
+        atomicAdd(cnt + 3, c.bytes);
+    }
+}
+
+__device__ __forceinline__ uint32_t unorm8(float v) { return (uint32_t)floorf(fclampg(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
+__device__ __forceinline__ float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
+
+__device__ __forceinline__ void face_normal(int32_t face, float n[3]) {
+    n[0] = n[1] = n[2] = 0.0f;
+    if (face > 0) n[(face - 1) >> 1] = (face & 1) ? 1.0f : -1.0f;
+}
+
+// PCG hash and samplers (common.glsl:1233-1288)
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    v = v * 747796405u + 2891336453u;
+    v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
+    return (v >> 22u) ^ v;
+}
+__device__ __forceinline__ float fractg(float x) { return x - floorf(x); }
+
+// sin / cos by Cody-Waite reduction with pi/2 = 1.5703125 + 4.838267948966e-4 and Taylor
+// polynomials on [-pi/4, pi/4]; deterministic, so GI and probe directions are reproducible.
+__device__ __forceinline__ void sincos_red(float x, float &s, float &c) {
+    const float kf = floorf(x * 0.63661977f + 0.5f);
+    const int32_t k = f2i(kf);
+    const float r = (x - kf * 1.5703125f) - kf * 4.838267948966e-4f;
+    const float r2 = r * r;
+    const float ps = r + r * r2 * (-0.16666667f + r2 * (0.0083333333f + r2 * (-1.9841270e-4f + r2 * 2.7557319e-6f)));
+    const float pc = 1.0f + r2 * (-0.5f + r2 * (0.041666667f + r2 * (-1.3888889e-3f + r2 * 2.4801587e-5f)));
+    switch (k & 3) {
+    case 0: s = ps; c = pc; break;
+    case 1: s = pc; c = -ps; break;
+    case 2: s = -ps; c = -pc; break;
+    default: s = -pc; c = ps; break;
+    }
+}
+
+__device__ __forceinline__ void cosine_hemisphere_u(const float n[3], float u0, float u1, float out[3]) {
+    float t1[3], t2[3];
+    const float upv[3] = {0.0f, 1.0f, 0.0f}, uxv[3] = {1.0f, 0.0f, 0.0f};
+    cross3(n, fabsf(n[1]) < 0.9f ? upv : uxv, t1);
+    normalize3(t1);
+    cross3(n, t1, t2);
+    const float r = sqrtf(u0);
+    const float ph = 6.2831853f * u1;
+    float s, c;
+    sincos_red(ph, s, c);
+    const float a = r * c, b = r * s, z = sqrtf(1.0f - u0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = (t1[i] * a + t2[i] * b) + n[i] * z;
+    normalize3(out);
+}
+
+
+// ------------------------------------------------------------------------------------------------
+// traceVolumeS split into a resumable form for the persistent queue kernels: ray_begin is the
+// set-up before the outer loop, ray_outer one iteration of it (including the voxel walk of a
+// detailed brick).  The arithmetic is trace_world's, statement for statement, so a ray traced in
+// pieces returns the same hit.
+// ------------------------------------------------------------------------------------------------
+struct RayState {
+    float ro[3], sd[3], inv[3], tM8[3];
+    float t, tcap;
+    int32_t bc[3];
+    int32_t face;
+    int32_t iter;
+};
+
+enum : int32_t { RAY_RUNNING = 0, RAY_HIT = 1, RAY_MISS = 2 };
+
+template <bool COUNT>
+__device__ __forceinline__ int32_t ray_begin(const DevWorld &W, const float ro[3], const float rd[3], float tcap, RayState &r,
+                                             Hit &h, Cnt &c) {
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+    if (COUNT) c.rays++;
+    float tmin3[3], tmax3[3], up[3];
+    int32_t st[3], faceA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.ro[a] = ro[a];
+        r.sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        r.inv[a] = 1.0f / r.sd[a];
+        const float lo = (float)(W.minb[a] * 8);
+        const float dims = (float)(W.wb[a] * 8);
+        const float t0 = (lo - ro[a]) * r.inv[a];
+        const float t1 = ((lo + dims) - ro[a]) * r.inv[a];
+        tmin3[a] = gmin(t0, t1);
+        tmax3[a] = gmax(t0, t1);
+        st[a] = r.sd[a] > 0.0f ? 1 : -1;
+        up[a] = r.sd[a] < 0.0f ? 0.0f : 1.0f;
+        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);
+    }
+    const float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
+    const float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
+    if (tExit < gmax(tEnter, 0.0f) || tEnter > tcap) return RAY_MISS;
+    r.t = gmax(tEnter, 0.0f);
+    r.tcap = tcap;
+    const float e = r.t + 1e-3f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float entry = ro[a] + r.sd[a] * e;
+        r.bc[a] = iclamp(f2i(floorf(entry * 0.125f)), W.minb[a], W.minb[a] + W.wb[a] - 1);
+        r.tM8[a] = (((float)r.bc[a] + up[a]) * 8.0f - ro[a]) * r.inv[a];
+    }
+    r.face = 0;
+    if (tEnter > 0.0f) {
+        if (tEnter == tmin3[0]) r.face = faceA[0];
+        else if (tEnter == tmin3[1]) r.face = faceA[1];
+        else r.face = faceA[2];
+    }
+    r.iter = 0;
+    return RAY_RUNNING;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ int32_t ray_outer(const DevWorld &W, RayState &r, Hit &h, Cnt &c, const int skip) {
+    if (r.iter >= W.max_outer) return RAY_MISS;
+    r.iter++;
+    if (r.t > r.tcap) return RAY_MISS;
+    float up[3];
+    int32_t st[3], faceA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        st[a] = r.sd[a] > 0.0f ? 1 : -1;
+        up[a] = r.sd[a] < 0.0f ? 0.0f : 1.0f;
+        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);
+    }
+    const uint2 cw = W.cells[cell_index(W, r.bc[0], r.bc[1], r.bc[2])];
+    uint32_t g = cw.x;
+    const uint32_t meta = cw.y;
+    if (COUNT) { c.outer++; c.bytes += 4; }
+    if (skip == 5) {
+        if (g == 5u) g = 0u;
+        else if (g >= 256u) {
+            if (COUNT) c.bytes += 1;
+            if (((meta >> 8) & 255u) > 0u) g = 0u;
+        }
+    }
+    if (g >= 256u) {
+        if (brick_voxels<COUNT>(W, r.ro, r.sd, r.inv, st, up, faceA, r.bc, g, r.t, r.face, h, c, skip)) return RAY_HIT;
+    } else if (g != 0u && g != (uint32_t)skip) {
+        const float e = r.t + 1e-3f;
+        h.hit = 1; h.t = r.t; h.face = r.face; h.mat = g;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h.cell[a] = f2i(floorf(r.ro[a] + r.sd[a] * e));
+        return RAY_HIT;
+    }
+    bool stepped = false;
+    if (g == 0u) {
+        int32_t Cf = 0, cb[3] = {0, 0, 0};
+        int32_t d = 0;
+        if (W.use_dist) {
+            d = (int32_t)(skip == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
+            if (COUNT) c.bytes += 1;
+        }
+        if (W.use_dist && d > 1) {
+            Cf = 2 * d - 1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) cb[a] = r.bc[a] - (d - 1);
+        } else {
+            if (COUNT) c.bytes += 1;
+            if (meta & (1u << 16)) Cf = 16;
+            else {
+                if (COUNT) c.bytes += 1;
+                Cf = (meta & (1u << 17)) ? 4 : 0;
+            }
+            if (Cf > 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) cb[a] = r.bc[a] - (r.bc[a] & (Cf - 1));
+            }
+        }
+        if (Cf > 0) {
+            float tFar[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float cmin = (float)cb[a] * 8.0f;
+                const float cmax = cmin + (float)Cf * 8.0f;
+                tFar[a] = ((up[a] != 0.0f ? cmax : cmin) - r.ro[a]) * r.inv[a];
+            }
+            const bool ax0 = tFar[0] < tFar[1] && tFar[0] < tFar[2];
+            const bool ax1 = !ax0 && tFar[1] < tFar[2];
+            const bool ax2 = !ax0 && !ax1;
+            r.t = ax0 ? tFar[0] : (ax1 ? tFar[1] : tFar[2]);
+            r.face = ax0 ? faceA[0] : (ax1 ? faceA[1] : faceA[2]);
+            const float e = r.t + 1e-3f;
+            const bool axs[3] = {ax0, ax1, ax2};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                int32_t v = iclamp(f2i(floorf((r.ro[a] + r.sd[a] * e) * 0.125f)), cb[a], cb[a] + Cf - 1);
+                if (axs[a]) v = st[a] > 0 ? cb[a] + Cf : cb[a] - 1;
+                r.bc[a] = v;
+                r.tM8[a] = (((float)r.bc[a] + up[a]) * 8.0f - r.ro[a]) * r.inv[a];
+            }
+            stepped = true;
+        }
+    }
+    if (!stepped) {
+        const bool a0 = r.tM8[0] < r.tM8[1] && r.tM8[0] < r.tM8[2];
+        const bool a1 = !a0 && r.tM8[1] < r.tM8[2];
+        const bool a2 = !a0 && !a1;
+        r.t = a0 ? r.tM8[0] : (a1 ? r.tM8[1] : r.tM8[2]);
+        r.face = a0 ? faceA[0] : (a1 ? faceA[1] : faceA[2]);
+        const float n0 = r.tM8[0] + fabsf(r.inv[0]) * 8.0f, n1 = r.tM8[1] + fabsf(r.inv[1]) * 8.0f,
+                    n2 = r.tM8[2] + fabsf(r.inv[2]) * 8.0f;
+        r.tM8[0] = a0 ? n0 : r.tM8[0];
+        r.tM8[1] = a1 ? n1 : r.tM8[1];
+        r.tM8[2] = a2 ? n2 : r.tM8[2];
+        r.bc[0] += a0 ? st[0] : 0;
+        r.bc[1] += a1 ? st[1] : 0;
+        r.bc[2] += a2 ? st[2] : 0;
+    }
+    if (r.bc[0] < W.minb[0] || r.bc[1] < W.minb[1] || r.bc[2] < W.minb[2] || r.bc[0] >= W.minb[0] + W.wb[0] ||
+        r.bc[1] >= W.minb[1] + W.wb[1] || r.bc[2] >= W.minb[2] + W.wb[2])
+        return RAY_MISS;
+    return RAY_RUNNING;
+}

@@ -1,0 +1,686 @@
+// Persistent work-queue traversal for incoherent and uneven ray batches (MI355X / gfx950).
+//
+// A grid of exactly as many workgroups as fit on the chip at once loops until its queue is empty.
+// Each wavefront claims `chunk` ray ids at a time from a global counter and hands them to its
+// lanes; a lane whose ray finishes takes the next id between two outer DDA iterations (a
+// "while-while" loop with dynamic fetch), so lanes do not idle behind the wave's longest ray.  The
+// arithmetic is ray_begin / ray_outer (vf_device.h), traceVolumeS statement for statement, so each
+// ray's hit is exactly what a plain launch returns.
+//
+//   * vf_trace_gi_batch: 1 GI ray per G-buffer pixel + 1 sun ray per GI hit.  A generation kernel
+//     compacts the non-sky pixels into a ray list (optionally bucket-sorted by direction octant and
+//     origin cell), a persistent kernel traces it and compacts the sun rays of the hits into a
+//     second list (wave ballot, one atomic per wave), a second persistent launch drains that.
+//   * vf_trace_batch: camera views of any size + DDGI probe grids in ONE persistent launch.  Each
+//     request owns a chunk-aligned id range (smallest views first); rays are generated on the fly;
+//     the wave that retires a request's last ray stamps its completion time (100 MHz wall clock).
+// Queue counters are reset by a kernel (no memset), so a batch can be recorded into a graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/vf_trace.h"
+#include "vf_ctx.h"
+#include "vf_device.h"
+#include "vf_internal.h"
+
+namespace {
+
+constexpr int kMaxReq = VF_MAX_REQUESTS;
+// counters: [0] ray-list length / batch head, [1] GI head, [2] sun tail, [3] sun head,
+// [8 + r] remaining rays of request r
+constexpr int kCtr = 8 + kMaxReq;
+constexpr int kBuckets = 8 * 4096;   // direction octant x 16^3 cells of 8^3 bricks
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
+    return (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+__device__ __forceinline__ void miss(Hit &h) {
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+}
+
+// Clears the GI-batch counters and the sort histogram.
+__global__ void k_queue_reset(unsigned long long *ctr, uint32_t *buckets, int nbuckets) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kCtr) ctr[i] = 0ull;
+    if (buckets && i < nbuckets) buckets[i] = 0u;
+}
+
+// ---- GI batch ----------------------------------------------------------------------------------
+struct GenArgs {
+    int32_t w, h, tile_w, tile_h, tiles_x;
+    uint32_t frame;
+    float reach;
+    int32_t sort;
+    int32_t minb[3];
+};
+
+__device__ __forceinline__ uint32_t spread4(uint32_t v) {   // interleave 4 bits with 2 zero bits
+    v &= 15u;
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+}
+
+// Pixel -> GI ray of gi.comp at full resolution (fpix = pix): origin P + N*0.03, direction
+// cosineHemisphereU(N, ldSample2(pix, frame >> 2)).  Sky pixels get miss records.
+__global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, const uint32_t *gbn, float4 *rays,
+                                                uint32_t *keys, uint32_t *buckets, unsigned long long *ctr,
+                                                vf_hit *gi_hits, vf_hit *sun_hits) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int32_t per = A.tile_w * A.tile_h;
+    const int64_t tile = i / per;
+    const int32_t in = (int32_t)(i % per);
+    const int32_t px = (int32_t)(tile % A.tiles_x) * A.tile_w + in % A.tile_w;
+    const int32_t py = (int32_t)(tile / A.tiles_x) * A.tile_h + in / A.tile_w;
+    const bool inside = px < A.w && py < A.h;
+    const size_t fi = (size_t)py * A.w + px;
+    const bool valid = inside && gbpos[fi].w >= 0.0f;
+    const unsigned long long m = __ballot(valid);
+    unsigned long long base = 0;
+    if (lane_id() == 0 && m) base = atomicAdd(ctr + 0, (unsigned long long)__popcll(m));
+    base = __shfl(base, 0);
+    if (!inside) return;
+    if (!valid) {
+        Hit h;
+        miss(h);
+        store_hit(gi_hits + fi, h);
+        store_hit(sun_hits + fi, h);
+        return;
+    }
+    const float4 P4 = gbpos[fi];
+    const uint32_t nb = gbn[fi];
+    const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+    const uint32_t salt = 0x68e31da4u;
+    const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
+    const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
+    const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
+    const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
+    const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
+    float d[3];
+    cosine_hemisphere_u(N, u0, u1, d);
+    const float o[3] = {P4.x + N[0] * 3e-2f, P4.y + N[1] * 3e-2f, P4.z + N[2] * 3e-2f};
+    const uint64_t slot = base + rank_in(m);
+    rays[slot * 2 + 0] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)fi));
+    rays[slot * 2 + 1] = make_float4(d[0], d[1], d[2], A.reach);
+    if (A.sort) {
+        const uint32_t oct = (d[0] < 0.0f ? 1u : 0u) | (d[1] < 0.0f ? 2u : 0u) | (d[2] < 0.0f ? 4u : 0u);
+        const uint32_t cx = (uint32_t)((f2i(floorf(o[0] * 0.125f)) - A.minb[0]) >> 3);
+        const uint32_t cy = (uint32_t)((f2i(floorf(o[1] * 0.125f)) - A.minb[1]) >> 3);
+        const uint32_t cz = (uint32_t)((f2i(floorf(o[2] * 0.125f)) - A.minb[2]) >> 3);
+        const uint32_t key = oct * 4096u + ((spread4(cx) | (spread4(cy) << 1) | (spread4(cz) << 2)) & 4095u);
+        keys[slot] = key;
+        atomicAdd(buckets + key, 1u);
+    }
+}
+
+// Exclusive scan of the bucket histogram (one workgroup of 1024 threads, 32 buckets each).
+__global__ void __launch_bounds__(1024) k_bucket_scan(uint32_t *buckets) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    uint32_t local[32];
+    uint32_t s = 0;
+    for (int k = 0; k < 32; ++k) { local[k] = buckets[t * 32 + k]; s += local[k]; }
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (int k = 0; k < 32; ++k) { buckets[t * 32 + k] = run; run += local[k]; }
+}
+
+// Scatter into bucket order (order inside a bucket is arbitrary; every result is per ray).
+__global__ void __launch_bounds__(256) k_bucket_scatter(const float4 *rays, const uint32_t *keys, uint32_t *buckets,
+                                                        float4 *out, const unsigned long long *ctr) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ctr[0]) return;
+    const uint32_t dst = atomicAdd(buckets + keys[i], 1u);
+    out[(size_t)dst * 2 + 0] = rays[i * 2 + 0];
+    out[(size_t)dst * 2 + 1] = rays[i * 2 + 1];
+}
+
+// Claims the next chunk of ray ids for the wave; false when the list is exhausted.
+__device__ __forceinline__ bool claim(unsigned long long *head, uint64_t n, int32_t chunk, uint64_t &pool,
+                                      uint64_t &pool_end) {
+    unsigned long long b = 0;
+    if (lane_id() == 0) b = atomicAdd(head, (unsigned long long)chunk);
+    b = __shfl(b, 0);
+    if (b >= n) return false;
+    pool = b;
+    pool_end = b + (uint64_t)chunk < n ? b + (uint64_t)chunk : n;
+    return true;
+}
+
+// Traces a ray list (o.xyz, tag) (d.xyz, tcap) of length *len.  SUN: the list is the GI batch, and
+// each GI hit that is not water and faces the sun appends its sun ray (giPathImpl stage 1,
+// common.glsl:2318-2330) to sun_list; every GI ray writes one sun_hits record (miss when no ray).
+template <bool SUN, bool COUNT>
+__global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *rays, const unsigned long long *len,
+                                                      unsigned long long *head, vf_hit *hits, float4 *sun_list,
+                                                      unsigned long long *sun_tail, vf_hit *sun_hits, float sx, float sy,
+                                                      float sz, int32_t chunk, unsigned long long *cnt) {
+    const uint64_t n = *len;
+    const float sun_dir[3] = {sx, sy, sz};
+    Cnt c = {0, 0, 0, 0};
+    RayState r;
+    Hit h;
+    float rd[3] = {0.0f, 0.0f, 0.0f};
+    bool active = false;
+    uint32_t tag = 0;
+    uint64_t pool = 0, pool_end = 0;
+    bool drained = false;
+    while (true) {
+        bool finished = false;   // this lane finished a ray during this trip
+        unsigned long long idle = __ballot(!active);
+        while (idle != 0ull && !drained) {
+            if (pool >= pool_end && !claim(head, n, chunk, pool, pool_end)) { drained = true; break; }
+            const uint64_t avail = pool_end - pool;
+            const uint32_t need = (uint32_t)__popcll(idle);
+            const uint32_t take = (uint32_t)(avail < need ? avail : need);
+            const uint32_t rk = rank_in(idle);
+            if (((idle >> lane_id()) & 1ull) != 0ull && rk < take) {
+                const uint64_t id = pool + rk;
+                const float4 a = rays[id * 2 + 0], b = rays[id * 2 + 1];
+                const float ro[3] = {a.x, a.y, a.z};
+                rd[0] = b.x; rd[1] = b.y; rd[2] = b.z;
+                tag = __float_as_uint(a.w);
+                if (ray_begin<COUNT>(W, ro, rd, b.w, r, h, c) == RAY_RUNNING) active = true;
+                else finished = true;   // missed the world box: done at once
+            }
+            pool += take;
+            idle = __ballot(!active && !finished);
+        }
+        if (active) {
+            const int32_t st = ray_outer<COUNT>(W, r, h, c, 0);
+            if (st != RAY_RUNNING) {
+                if (st == RAY_MISS) miss(h);
+                active = false;
+                finished = true;
+            }
+        }
+        bool emit = false;
+        float so[3] = {0.0f, 0.0f, 0.0f};
+        if (finished) {
+            store_hit(hits + tag, h);
+            if (SUN) {
+                if (h.hit && h.mat != 5u) {
+                    float hn[3];
+                    face_normal(h.face, hn);
+                    if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
+                        emit = true;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const float hp = r.ro[a] + rd[a] * h.t;
+                            so[a] = hp + hn[a] * 3e-2f;
+                        }
+                    }
+                }
+                if (!emit) {
+                    Hit mh;
+                    miss(mh);
+                    store_hit(sun_hits + tag, mh);
+                }
+            }
+        }
+        if (SUN) {
+            const unsigned long long em = __ballot(emit);
+            if (em != 0ull) {
+                unsigned long long base = 0;
+                if (lane_id() == 0) base = atomicAdd(sun_tail, (unsigned long long)__popcll(em));
+                base = __shfl(base, 0);
+                if (emit) {
+                    const uint64_t slot = base + rank_in(em);
+                    sun_list[slot * 2 + 0] = make_float4(so[0], so[1], so[2], __uint_as_float(tag));
+                    sun_list[slot * 2 + 1] = make_float4(sx, sy, sz, 1e30f);
+                }
+            }
+        }
+        if (drained && !__any(active)) break;
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
+// ---- uneven batch: views + probe grids -----------------------------------------------------
+struct ReqDev {
+    int32_t kind;   // 0 view, 1 probe grid
+    int32_t w, h, tiles_x, tile_w, tile_h;
+    float v[12];    // view: pos, right, up, fwd;  probes: origin (3), spacing
+    float thf, jit0, jit1, reach;
+    int32_t dims[3], n_rays;
+    uint32_t frame;
+    uint64_t first, count;   // id range [first, first + count); first is chunk-aligned
+    vf_hit *hits;
+};
+
+struct BatchArgs {
+    int32_t n_req;
+    uint64_t total;   // end of the last request's chunk-aligned range
+    ReqDev req[kMaxReq];
+};
+
+// Batch counters from the request table itself (a kernel argument): recordable into a graph.
+__global__ void k_batch_reset(BatchArgs B, unsigned long long *ctr, unsigned long long *stamps) {
+    const int i = threadIdx.x;
+    if (i < kCtr) ctr[i] = (i >= 8 && i - 8 < B.n_req) ? (unsigned long long)B.req[i - 8].count : 0ull;
+    if (i < kMaxReq) stamps[i] = 0ull;
+    if (i == kMaxReq) stamps[i] = ~0ull;
+}
+
+__device__ __forceinline__ int find_req(const BatchArgs &B, uint64_t id) {
+    int r = 0;
+    for (int k = 1; k < B.n_req; ++k)
+        if (id >= B.req[k].first) r = k;
+    return r;
+}
+
+// Ray `local` of request q: a primary ray of a view (gbuffer.comp:194-202, tile order), or probe ray
+// (probes.glsl:155-170).  out: the output index; skip: the ray's skipMat.
+__device__ __forceinline__ void batch_ray(const ReqDev &q, uint64_t local, float ro[3], float rd[3], float &tcap,
+                                          uint32_t &out, int &skip) {
+    if (q.kind == 0) {
+        const int32_t per = q.tile_w * q.tile_h;
+        const int64_t tile = (int64_t)(local / (uint64_t)per);
+        const int32_t in = (int32_t)(local % (uint64_t)per);
+        const int32_t px = (int32_t)(tile % q.tiles_x) * q.tile_w + in % q.tile_w;
+        const int32_t py = (int32_t)(tile / q.tiles_x) * q.tile_h + in / q.tile_w;
+        const float sx = ((float)px + 0.5f) + q.jit0;
+        const float sy = ((float)py + 0.5f) + q.jit1;
+        const float nx = (sx / (float)q.w) * 2.0f - 1.0f;
+        const float ny = (sy / (float)q.h) * 2.0f - 1.0f;
+        const float aspect = (float)q.w / (float)q.h;
+        const float a = (nx * aspect) * q.thf;
+        const float b = ny * q.thf;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ro[i] = q.v[i];
+            rd[i] = (q.v[9 + i] + q.v[3 + i] * a) + q.v[6 + i] * b;
+        }
+        normalize3(rd);
+        tcap = 1e30f;
+        skip = 0;
+        out = (px < q.w && py < q.h) ? (uint32_t)(py * q.w + px) : 0xFFFFFFFFu;
+        return;
+    }
+    const int32_t i = (int32_t)(local % (uint64_t)q.n_rays);
+    const int64_t p = (int64_t)(local / (uint64_t)q.n_rays);
+    const int32_t pxi = (int32_t)(p % q.dims[0]), pyi = (int32_t)((p / q.dims[0]) % q.dims[1]),
+                  pzi = (int32_t)(p / ((int64_t)q.dims[0] * q.dims[1]));
+    ro[0] = q.v[0] + q.v[3] * (float)pxi;
+    ro[1] = q.v[1] + q.v[3] * (float)pyi;
+    ro[2] = q.v[2] + q.v[3] * (float)pzi;
+    const float z = 1.0f - (2.0f * (float)i + 1.0f) / (float)q.n_rays;
+    const float rr = sqrtf(gmax(0.0f, 1.0f - z * z));
+    float s, c;
+    sincos_red((float)i * 2.39996323f, s, c);
+    const float d[3] = {c * rr, z, s * rr};
+    const uint32_t h0 = pcg(q.frame * 747796405u + 1u);
+    const float u1 = (float)(h0 & 0xFFFFu) / 65535.0f, u2 = (float)(h0 >> 16) / 65535.0f;
+    const uint32_t h2 = pcg(h0);
+    const float u3 = (float)(h2 & 0xFFFFu) / 65535.0f;
+    float s2, c2, s3, c3;
+    sincos_red(6.2831853f * u2, s2, c2);
+    sincos_red(6.2831853f * u3, s3, c3);
+    const float qa = sqrtf(1.0f - u1), qb = sqrtf(u1);
+    const float qv[3] = {qa * s2, qa * c2, qb * s3};
+    const float qw = qb * c3;
+    float c1v[3], tmp[3], c2v[3];
+    cross3(qv, d, c1v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tmp[k] = c1v[k] + qw * d[k];
+    cross3(qv, tmp, c2v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rd[k] = d[k] + 2.0f * c2v[k];
+    tcap = q.reach;
+    skip = 5;   // probe_trace.comp: traceWorldProbe(..., kSkipMat)
+    out = (uint32_t)local;
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, unsigned long long *ctr,
+                                                       unsigned long long *stamps, int32_t chunk,
+                                                       unsigned long long *cnt) {
+    if (threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
+    // every lane of the first trip waits for the start stamp of its own workgroup only; requests are
+    // stamped relative to the earliest workgroup start
+    Cnt c = {0, 0, 0, 0};
+    RayState r;
+    Hit h;
+    bool active = false;
+    int32_t req = 0, skip = 0;
+    uint32_t out = 0;
+    uint64_t pool = 0, pool_end = 0;
+    int32_t pool_req = 0;
+    bool drained = false;
+    unsigned long long *head = ctr + 0;
+    while (true) {
+        bool finished = false;
+        unsigned long long idle = __ballot(!active);
+        while (idle != 0ull && !drained) {
+            if (pool >= pool_end) {
+                unsigned long long b = 0;
+                if (lane_id() == 0) b = atomicAdd(head, (unsigned long long)chunk);
+                b = __shfl(b, 0);
+                if (b >= B.total) { drained = true; break; }
+                pool_req = find_req(B, b);
+                const uint64_t rend = B.req[pool_req].first + B.req[pool_req].count;
+                pool = b;
+                pool_end = b + (uint64_t)chunk < rend ? b + (uint64_t)chunk : rend;   // aligned: no straddling
+                if (pool >= pool_end) continue;   // a chunk of the alignment padding: claim again
+            }
+            const uint64_t avail = pool_end - pool;
+            const uint32_t need = (uint32_t)__popcll(idle);
+            const uint32_t take = (uint32_t)(avail < need ? avail : need);
+            const uint32_t rk = rank_in(idle);
+            if (((idle >> lane_id()) & 1ull) != 0ull && rk < take) {
+                const uint64_t id = pool + rk;
+                req = pool_req;
+                float ro[3], rd[3], tcap;
+                batch_ray(B.req[req], id - B.req[req].first, ro, rd, tcap, out, skip);
+                if (out == 0xFFFFFFFFu) finished = true;   // tile padding outside the view: no ray
+                else if (ray_begin<COUNT>(W, ro, rd, tcap, r, h, c) == RAY_RUNNING) active = true;
+                else finished = true;
+            }
+            pool += take;
+            idle = __ballot(!active && !finished);
+        }
+        if (active) {
+            const int32_t st = ray_outer<COUNT>(W, r, h, c, skip);
+            if (st != RAY_RUNNING) {
+                if (st == RAY_MISS) miss(h);
+                active = false;
+                finished = true;
+            }
+        }
+        if (finished && out != 0xFFFFFFFFu) store_hit(B.req[req].hits + out, h);
+        // retire finished rays per request: one atomic per (wave, request); the last one stamps the time
+        unsigned long long fm = __ballot(finished);
+        while (fm != 0ull) {
+            const int leader = __ffsll((long long)fm) - 1;
+            const int rq = __shfl(req, leader);
+            const unsigned long long same = __ballot(finished && req == rq);
+            if ((int)lane_id() == leader) {
+                const unsigned long long k = (unsigned long long)__popcll(same);
+                const unsigned long long old = atomicSub(ctr + 8 + rq, k);
+                if (old == k) stamps[rq] = wall_clock64();
+            }
+            fm &= ~same;
+        }
+        if (drained && !__any(active)) break;
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------
+namespace vf {
+
+void free_queue(vf_ctx *ctx) {
+    void *ptrs[] = {ctx->q_ctr, ctx->q_stamps, ctx->q_rays, ctx->q_rays2, ctx->q_sun, ctx->q_keys, ctx->q_buckets};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ctx->q_ctr = ctx->q_stamps = nullptr;
+    ctx->q_rays = ctx->q_rays2 = ctx->q_sun = nullptr;
+    ctx->q_keys = ctx->q_buckets = nullptr;
+    ctx->q_cap = 0;
+}
+
+}  // namespace vf
+
+namespace {
+
+int reserve_queue(vf_ctx *ctx, int64_t rays) {
+    if (ctx->q_ctr && ctx->q_cap >= rays) return VF_OK;
+    if (ctx->capturing)
+        return vf::fail(ctx, VF_E_INVALID, "queue buffers for %lld rays are not allocated: run the batch once before recording it",
+                        (long long)rays);
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    vf::free_queue(ctx);
+    const int64_t cap = std::max<int64_t>(rays, 1);
+#define VF_QALLOC(ptr, bytes)                                                                        \
+    do {                                                                                             \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                                         \
+        if (e_ != hipSuccess) {                                                                      \
+            vf::free_queue(ctx);                                                                     \
+            return vf::fail(ctx, VF_E_OOM, "queue buffers: hipMalloc(%zu): %s", (size_t)(bytes), hipGetErrorString(e_)); \
+        }                                                                                            \
+    } while (0)
+    VF_QALLOC(ctx->q_ctr, kCtr * 8);
+    VF_QALLOC(ctx->q_stamps, (kMaxReq + 1) * 8);
+    VF_QALLOC(ctx->q_rays, (size_t)cap * 32);
+    VF_QALLOC(ctx->q_rays2, (size_t)cap * 32);
+    VF_QALLOC(ctx->q_sun, (size_t)cap * 32);
+    VF_QALLOC(ctx->q_keys, (size_t)cap * 4);
+    VF_QALLOC(ctx->q_buckets, (size_t)kBuckets * 4);
+#undef VF_QALLOC
+    ctx->q_cap = cap;
+    return VF_OK;
+}
+
+int persistent_groups(vf_ctx *ctx, const void *kernel, const vf_queue_cfg *cfg) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    if (cfg && cfg->groups_per_cu > 0) per_cu = std::min(per_cu, (int)cfg->groups_per_cu);
+    return std::max(1, per_cu * ctx->n_cu);
+}
+
+int check_cfg(vf_ctx *ctx, const vf_queue_cfg *cfg, const char *what) {
+    if (!cfg) return VF_OK;
+    const bool tiles = (cfg->tile_w == 8 && cfg->tile_h == 8) || (cfg->tile_w == 16 && cfg->tile_h == 8) ||
+                       (cfg->tile_w == 16 && cfg->tile_h == 16) || (cfg->tile_w == 8 && cfg->tile_h == 16) ||
+                       (cfg->tile_w == 32 && cfg->tile_h == 8);
+    if (!tiles) return vf::fail(ctx, VF_E_INVALID, "%s: tile %dx%d not supported", what, cfg->tile_w, cfg->tile_h);
+    if (cfg->chunk < 64 || cfg->chunk > 4096 || (cfg->chunk & 63))
+        return vf::fail(ctx, VF_E_INVALID, "%s: chunk %d must be a multiple of 64 in [64, 4096]", what, cfg->chunk);
+    return VF_OK;
+}
+
+vf_queue_cfg default_cfg() {
+    vf_queue_cfg c;
+    c.tile_w = 8; c.tile_h = 8; c.chunk = 256; c.sort = 0; c.groups_per_cu = 0;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vf_queue_defaults(vf_queue_cfg *cfg) {
+    if (!cfg) return vf::fail(nullptr, VF_E_INVALID, "vf_queue_defaults: null cfg");
+    *cfg = default_cfg();
+    return VF_OK;
+}
+
+int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h, float reach, const float *gb_pos,
+                      const uint32_t *gb_normal, vf_hit *gi_hits, vf_hit *sun_hits, const vf_queue_cfg *cfg_in) {
+    int rc = vf::check_ready(ctx, "vf_trace_gi_batch");
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: no scene (vf_set_scene)");
+    if (!pass || !gb_pos || !gb_normal || !gi_hits || !sun_hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: null buffer");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: bad size %dx%d", w, h);
+    if (!(reach > 0.0f)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: reach must be > 0");
+    if ((rc = check_cfg(ctx, cfg_in, "vf_trace_gi_batch"))) return rc;
+    const vf_queue_cfg cfg = cfg_in ? *cfg_in : default_cfg();
+    const int64_t n = (int64_t)w * h;
+    if ((rc = reserve_queue(ctx, n))) return rc;
+    GenArgs A;
+    A.w = w; A.h = h; A.tile_w = cfg.tile_w; A.tile_h = cfg.tile_h;
+    A.tiles_x = (w + cfg.tile_w - 1) / cfg.tile_w;
+    const int64_t tiles_y = (h + cfg.tile_h - 1) / cfg.tile_h;
+    A.frame = pass->frame_idx;
+    A.reach = reach;
+    A.sort = cfg.sort ? 1 : 0;
+    for (int a = 0; a < 3; ++a) A.minb[a] = ctx->W.minb[a];
+    const int64_t ngen = (int64_t)A.tiles_x * tiles_y * cfg.tile_w * cfg.tile_h;
+    const float *s = ctx->S.sun_dir;
+    unsigned long long *ctr = ctx->q_ctr;
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    hipLaunchKernelGGL(k_queue_reset, dim3((kBuckets + 255) / 256), dim3(256), 0, ctx->stream, ctr, ctx->q_buckets,
+                       kBuckets);
+    hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen)), dim3(256), 0, ctx->stream, A,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctx->q_keys, ctx->q_buckets, ctr,
+                       gi_hits, sun_hits);
+    const float4 *list = ctx->q_rays;
+    if (cfg.sort) {
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->q_buckets);
+        hipLaunchKernelGGL(k_bucket_scatter, dim3(vf::blocks_for(n)), dim3(256), 0, ctx->stream, ctx->q_rays, ctx->q_keys,
+                           ctx->q_buckets, ctx->q_rays2, ctr);
+        list = ctx->q_rays2;
+    }
+    if (ctx->counting) {
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_list_persist<true, true>), &cfg);
+        hipLaunchKernelGGL((k_list_persist<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1, gi_hits,
+                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
+        hipLaunchKernelGGL((k_list_persist<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2, ctr + 3,
+                           sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
+    } else {
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_list_persist<true, false>), &cfg);
+        hipLaunchKernelGGL((k_list_persist<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1, gi_hits,
+                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
+        hipLaunchKernelGGL((k_list_persist<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2, ctr + 3,
+                           sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
+    }
+    if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
+    return vf::end_timed(ctx);
+}
+
+int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in) {
+    int rc = vf::check_ready(ctx, "vf_trace_batch");
+    if (rc) return rc;
+    if (!reqs || n_req <= 0 || n_req > kMaxReq)
+        return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: 1..%d requests required", kMaxReq);
+    if ((rc = check_cfg(ctx, cfg_in, "vf_trace_batch"))) return rc;
+    const vf_queue_cfg cfg = cfg_in ? *cfg_in : default_cfg();
+    BatchArgs B;
+    std::memset(&B, 0, sizeof(B));
+    B.n_req = n_req;
+    uint64_t at = 0;
+    uint64_t counts[kMaxReq] = {};
+    for (int k = 0; k < n_req; ++k) {
+        const vf_request &q = reqs[k];
+        ReqDev &d = B.req[k];
+        if (!q.d_hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d has no output", k);
+        d.kind = q.kind;
+        d.hits = q.d_hits;
+        if (q.kind == VF_REQ_VIEW) {
+            if (q.width <= 0 || q.height <= 0 || (int64_t)q.width * q.height > (1ll << 28))
+                return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: bad view size", k);
+            d.w = q.width; d.h = q.height;
+            d.tile_w = cfg.tile_w; d.tile_h = cfg.tile_h;
+            d.tiles_x = (q.width + cfg.tile_w - 1) / cfg.tile_w;
+            const int64_t ty = (q.height + cfg.tile_h - 1) / cfg.tile_h;
+            for (int a = 0; a < 3; ++a) {
+                d.v[a] = q.cam_pos[a]; d.v[3 + a] = q.cam_right[a]; d.v[6 + a] = q.cam_up[a]; d.v[9 + a] = q.cam_fwd[a];
+            }
+            d.thf = q.tan_half_fov;
+            d.jit0 = q.jitter[0];
+            d.jit1 = q.jitter[1];
+            d.count = (uint64_t)d.tiles_x * ty * cfg.tile_w * cfg.tile_h;   // tile padding: no output
+            counts[k] = d.count;
+        } else if (q.kind == VF_REQ_PROBES) {
+            if (q.n_rays <= 0 || q.n_rays > 65536 || q.dims[0] <= 0 || q.dims[1] <= 0 || q.dims[2] <= 0)
+                return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: bad probe grid", k);
+            for (int a = 0; a < 3; ++a) { d.v[a] = q.origin[a]; d.dims[a] = q.dims[a]; }
+            d.v[3] = q.spacing;
+            d.n_rays = q.n_rays;
+            d.frame = q.frame;
+            d.reach = q.reach;
+            d.count = (uint64_t)q.dims[0] * q.dims[1] * q.dims[2] * (uint64_t)q.n_rays;
+            counts[k] = d.count;
+        } else {
+            return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: unknown kind %d", k, q.kind);
+        }
+        d.first = at;
+        at += (d.count + (uint64_t)cfg.chunk - 1) / (uint64_t)cfg.chunk * (uint64_t)cfg.chunk;
+    }
+    B.total = at;
+    (void)counts;
+    if ((rc = reserve_queue(ctx, 1))) return rc;
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctx->q_ctr, ctx->q_stamps);
+    if (ctx->counting) {
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_batch_persist<true>), &cfg);
+        hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                           cfg.chunk, ctx->d_cnt);
+    } else {
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_batch_persist<false>), &cfg);
+        hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                           cfg.chunk, ctx->d_cnt);
+    }
+    if ((rc = vf::launch_check(ctx, "vf_trace_batch"))) return rc;
+    return vf::end_timed(ctx);
+}
+
+int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms) {
+    if (!ctx || !ms || n_req <= 0 || n_req > kMaxReq) return vf::fail(ctx, VF_E_INVALID, "vf_read_batch_times: bad argument");
+    if (int rc = vf::no_capture(ctx, "vf_read_batch_times")) return rc;
+    if (!ctx->q_stamps) return vf::fail(ctx, VF_E_INVALID, "vf_read_batch_times: no batch has run");
+    unsigned long long st[kMaxReq + 1];
+    VF_HIP(ctx, hipMemcpyAsync(st, ctx->q_stamps, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double tick_ms = 1.0 / (double)ctx->wall_khz;
+    for (int k = 0; k < n_req; ++k) ms[k] = st[k] ? (double)(st[k] - st[kMaxReq]) * tick_ms : -1.0;
+    return VF_OK;
+}
+
+int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, vf_queue_cfg *best, vf_tune_result *results,
+                int32_t max_results, int32_t *n_results) {
+    if (!ctx || !args || !best) return vf::fail(ctx, VF_E_INVALID, "vf_autotune: null argument");
+    if (int rc = vf::no_capture(ctx, "vf_autotune")) return rc;
+    static const int tiles[3][2] = {{8, 8}, {16, 8}, {16, 16}};
+    static const int chunks[5] = {64, 128, 256, 512, 1024};
+    const int sorts = workload == VF_TUNE_GI_BATCH ? 2 : 1;
+    int k = 0;
+    float best_ms = 1e30f;
+    hipEvent_t e0, e1;
+    VF_HIP(ctx, hipEventCreate(&e0));
+    VF_HIP(ctx, hipEventCreate(&e1));
+    int rc = VF_OK;
+    for (int so = 0; so < sorts && rc == VF_OK; ++so)
+        for (const auto &t : tiles)
+            for (int ch : chunks) {
+                vf_queue_cfg c = default_cfg();
+                c.tile_w = t[0]; c.tile_h = t[1]; c.chunk = ch; c.sort = so;
+                auto run = [&]() -> int {
+                    if (workload == VF_TUNE_GI_BATCH) {
+                        const vf_tune_gi_args *a = static_cast<const vf_tune_gi_args *>(args);
+                        return vf_trace_gi_batch(ctx, a->pass, a->width, a->height, a->reach, a->d_gb_pos, a->d_gb_normal,
+                                                 a->d_gi_hits, a->d_sun_hits, &c);
+                    }
+                    const vf_tune_batch_args *a = static_cast<const vf_tune_batch_args *>(args);
+                    return vf_trace_batch(ctx, a->reqs, a->n_req, &c);
+                };
+                if ((rc = run())) break;   // warm-up (and buffer allocation)
+                float ms_med[3];
+                for (int rep = 0; rep < 3 && rc == VF_OK; ++rep) {
+                    if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = VF_E_DEVICE; break; }
+                    if ((rc = run())) break;
+                    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms_med[rep], e0, e1) != hipSuccess) {
+                        rc = VF_E_DEVICE;
+                        break;
+                    }
+                }
+                if (rc) break;
+                std::sort(ms_med, ms_med + 3);
+                if (results && k < max_results) { results[k].cfg = c; results[k].ms = ms_med[1]; }
+                ++k;
+                if (ms_med[1] < best_ms) { best_ms = ms_med[1]; *best = c; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == VF_E_DEVICE ? vf::fail(ctx, rc, "vf_autotune: timing failed") : rc;
+    if (n_results) *n_results = k;
+    return VF_OK;
+}
+
+}  // extern "C"

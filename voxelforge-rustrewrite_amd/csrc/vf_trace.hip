@@ -27,349 +27,9 @@
 #include <vector>
 
 #include "../../include/vf_trace.h"
+#include "vf_ctx.h"
+#include "vf_device.h"
 #include "vf_internal.h"
-
-// ------------------------------------------------------------------------------------------------
-// device-side world
-// ------------------------------------------------------------------------------------------------
-struct DevWorld {
-    int32_t wb[3];
-    int32_t minb[3];     // worldMinB(): (win.x, 0, win.z) in ring mode, else 0
-    int32_t ring;
-    int32_t use_dist;    // USE_DIST_FIELD
-    int32_t tdim[3];     // 4^3-brick tiles per axis of the cell-word array
-    int32_t max_outer;   // 3 * (wbx + wby + wbz) + 4
-    const uint2 *cells;  // .x grid value, .y meta: dist | dry << 8 | c2empty << 16 | c1empty << 17
-    const uint8_t *atlas;
-    const uint4 *occ;     // per slot 4 x uint4: bit (lz*64 + ly*8 + lx) = voxel != 0
-    const uint4 *occdry;  // per slot: voxel != 0 && voxel != 5
-    const uint16_t *skytop;
-    const float4 *palette;  // 256 x 3 float4 (Material)
-};
-
-struct SkyParams {
-    float sun_dir[3], sun_rgb[3], sky_zenith[3], sky_horizon[3], sunset_rgb[3];
-    float sky_scale, gi_scale, night;
-    int32_t sky_model;
-};
-
-struct Cnt {
-    unsigned long long rays, outer, voxel, bytes;
-};
-
-struct Hit {
-    int32_t hit;
-    float t;
-    int32_t face;
-    uint32_t mat;
-    int32_t cell[3];
-};
-
-// GLSL built-ins with their specified definitions (signed zeros matter for bit equality).
-__device__ __forceinline__ float gmin(float x, float y) { return y < x ? y : x; }
-__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; }
-__device__ __forceinline__ int32_t imin(int32_t x, int32_t y) { return y < x ? y : x; }
-__device__ __forceinline__ int32_t imax(int32_t x, int32_t y) { return x < y ? y : x; }
-__device__ __forceinline__ int32_t iclamp(int32_t x, int32_t lo, int32_t hi) { return imin(imax(x, lo), hi); }
-__device__ __forceinline__ float fclampg(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
-__device__ __forceinline__ int32_t f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT32_MAX;
-    if (x <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)x;
-}
-__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
-__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
-    float x = a[1] * b[2] - b[1] * a[2];
-    float y = a[2] * b[0] - b[2] * a[0];
-    float z = a[0] * b[1] - b[0] * a[1];
-    o[0] = x; o[1] = y; o[2] = z;
-}
-__device__ __forceinline__ void normalize3(float v[3]) {
-    float len = sqrtf(dot3(v, v));
-    v[0] = v[0] / len; v[1] = v[1] / len; v[2] = v[2] / len;
-}
-
-// Cell word index of absolute brick (x, y, z) inside the window (gridWrap, common.glsl:134-136).
-__device__ __forceinline__ uint32_t cell_index(const DevWorld &W, int32_t x, int32_t y, int32_t z) {
-    if (W.ring) { x &= W.wb[0] - 1; z &= W.wb[2] - 1; }
-    uint32_t tile = ((uint32_t)(z >> 2) * (uint32_t)W.tdim[1] + (uint32_t)(y >> 2)) * (uint32_t)W.tdim[0] + (uint32_t)(x >> 2);
-    return tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
-}
-
-// 64 occupancy bits of brick layer lz (two 32-bit words) out of the 512-bit mask.
-__device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, const uint4 &m2, const uint4 &m3, int lz,
-                                           uint32_t &lo, uint32_t &hi) {
-    uint4 a = (lz & 2) ? m1 : m0;
-    uint4 b = (lz & 2) ? m3 : m2;
-    uint4 q = (lz & 4) ? b : a;
-    lo = (lz & 1) ? q.z : q.x;
-    hi = (lz & 1) ? q.w : q.y;
-}
-
-// brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
-// arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
-template <int SKIP, bool COUNT>
-__device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
-                                             const int32_t st[3], const float up[3], const int32_t faceA[3],
-                                             const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c) {
-    const uint32_t slot = g - 256u;
-    const uint4 *mp = (SKIP == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
-    const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
-    int32_t bmin[3], vc[3];
-    float tM[3], tD[3];
-    const float e = t + 1e-3f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        bmin[a] = bc[a] * 8;
-        float q = ro[a] + sd[a] * e;
-        vc[a] = iclamp(f2i(floorf(q)), bmin[a], bmin[a] + 7);
-        tM[a] = (((float)vc[a] + up[a]) - ro[a]) * inv[a];
-        tD[a] = fabsf(inv[a]);
-    }
-    int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
-    uint32_t wlo, whi;
-    mask_layer(m0, m1, m2, m3, lz, wlo, whi);
-    float tv = t;
-    int32_t f2 = face;
-    for (int j = 0; j < 26; ++j) {
-        if (COUNT) { c.voxel++; c.bytes += 1; }
-        const uint32_t wv = (ly & 4) ? whi : wlo;
-        if ((wv >> (((ly & 3) << 3) | lx)) & 1u) {
-            h.hit = 1;
-            h.t = tv;
-            h.face = f2;
-            h.mat = W.atlas[(size_t)slot * 512u + (uint32_t)(lz * 64 + ly * 8 + lx)];
-            h.cell[0] = bmin[0] + lx; h.cell[1] = bmin[1] + ly; h.cell[2] = bmin[2] + lz;
-            return true;
-        }
-        if (tM[0] < tM[1] && tM[0] < tM[2]) {
-            lx += st[0]; tv = tM[0]; tM[0] += tD[0]; f2 = faceA[0];
-            if ((uint32_t)lx > 7u) break;
-        } else if (tM[1] < tM[2]) {
-            ly += st[1]; tv = tM[1]; tM[1] += tD[1]; f2 = faceA[1];
-            if ((uint32_t)ly > 7u) break;
-        } else {
-            lz += st[2]; tv = tM[2]; tM[2] += tD[2]; f2 = faceA[2];
-            if ((uint32_t)lz > 7u) break;
-            mask_layer(m0, m1, m2, m3, lz, wlo, whi);
-        }
-    }
-    return false;
-}
-
-// traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5).
-template <int SKIP, bool COUNT>
-__device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
-                                            Cnt &c) {
-    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
-    if (COUNT) c.rays++;
-    float sd[3], inv[3], up[3], tmin3[3], tmax3[3];
-    int32_t st[3], faceA[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
-        inv[a] = 1.0f / sd[a];
-        const float lo = (float)(W.minb[a] * 8);
-        const float dims = (float)(W.wb[a] * 8);
-        const float t0 = (lo - ro[a]) * inv[a];
-        const float t1 = ((lo + dims) - ro[a]) * inv[a];
-        tmin3[a] = gmin(t0, t1);
-        tmax3[a] = gmax(t0, t1);
-        st[a] = sd[a] > 0.0f ? 1 : -1;
-        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
-        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);   // face code of normal -st on axis a
-    }
-    const float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
-    const float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
-    if (tExit < gmax(tEnter, 0.0f) || tEnter > tcap) return;
-
-    float t = gmax(tEnter, 0.0f);
-    int32_t bc[3];
-    float tD8[3], tM8[3];
-    {
-        const float e = t + 1e-3f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float entry = ro[a] + sd[a] * e;
-            bc[a] = iclamp(f2i(floorf(entry * 0.125f)), W.minb[a], W.minb[a] + W.wb[a] - 1);
-            tD8[a] = fabsf(inv[a]) * 8.0f;
-            tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
-        }
-    }
-    int32_t face = 0;
-    if (tEnter > 0.0f) {
-        if (tEnter == tmin3[0]) face = faceA[0];
-        else if (tEnter == tmin3[1]) face = faceA[1];
-        else face = faceA[2];
-    }
-    for (int i = 0; i < W.max_outer; ++i) {
-        if (t > tcap) return;
-        const uint2 cw = W.cells[cell_index(W, bc[0], bc[1], bc[2])];
-        uint32_t g = cw.x;
-        const uint32_t meta = cw.y;
-        if (COUNT) { c.outer++; c.bytes += 4; }
-        if (SKIP == 5) {
-            if (g == 5u) g = 0u;
-            else if (g >= 256u) {
-                if (COUNT) c.bytes += 1;
-                if (((meta >> 8) & 255u) > 0u) g = 0u;
-            }
-        }
-        if (g >= 256u) {
-            if (brick_voxels<SKIP, COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c)) return;
-        } else if (g != 0u && g != (uint32_t)SKIP) {
-            const float e = t + 1e-3f;
-            h.hit = 1; h.t = t; h.face = face; h.mat = g;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) h.cell[a] = f2i(floorf(ro[a] + sd[a] * e));
-            return;
-        }
-        bool stepped = false;
-        if (g == 0u) {
-            int32_t Cf = 0, cb[3] = {0, 0, 0};
-            int32_t d = 0;
-            if (W.use_dist) {
-                d = (int32_t)(SKIP == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
-                if (COUNT) c.bytes += 1;
-            }
-            if (W.use_dist && d > 1) {
-                Cf = 2 * d - 1;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (d - 1);
-            } else {
-                if (COUNT) c.bytes += 1;
-                if (meta & (1u << 16)) Cf = 16;
-                else {
-                    if (COUNT) c.bytes += 1;
-                    Cf = (meta & (1u << 17)) ? 4 : 0;
-                }
-                if (Cf > 0) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) cb[a] = bc[a] - (bc[a] & (Cf - 1));
-                }
-            }
-            if (Cf > 0) {
-                float tFar[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float cmin = (float)cb[a] * 8.0f;
-                    const float cmax = cmin + (float)Cf * 8.0f;
-                    tFar[a] = ((up[a] != 0.0f ? cmax : cmin) - ro[a]) * inv[a];
-                }
-                const bool ax0 = tFar[0] < tFar[1] && tFar[0] < tFar[2];
-                const bool ax1 = !ax0 && tFar[1] < tFar[2];
-                const bool ax2 = !ax0 && !ax1;
-                t = ax0 ? tFar[0] : (ax1 ? tFar[1] : tFar[2]);
-                face = ax0 ? faceA[0] : (ax1 ? faceA[1] : faceA[2]);
-                const float e = t + 1e-3f;
-                const bool axs[3] = {ax0, ax1, ax2};
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    int32_t v = iclamp(f2i(floorf((ro[a] + sd[a] * e) * 0.125f)), cb[a], cb[a] + Cf - 1);
-                    if (axs[a]) v = st[a] > 0 ? cb[a] + Cf : cb[a] - 1;
-                    bc[a] = v;
-                    tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
-                }
-                stepped = true;
-            }
-        }
-        if (!stepped) {
-            if (tM8[0] < tM8[1] && tM8[0] < tM8[2]) {
-                bc[0] += st[0]; t = tM8[0]; tM8[0] += tD8[0]; face = faceA[0];
-            } else if (tM8[1] < tM8[2]) {
-                bc[1] += st[1]; t = tM8[1]; tM8[1] += tD8[1]; face = faceA[1];
-            } else {
-                bc[2] += st[2]; t = tM8[2]; tM8[2] += tD8[2]; face = faceA[2];
-            }
-        }
-        if (bc[0] < W.minb[0] || bc[1] < W.minb[1] || bc[2] < W.minb[2] || bc[0] >= W.minb[0] + W.wb[0] ||
-            bc[1] >= W.minb[1] + W.wb[1] || bc[2] >= W.minb[2] + W.wb[2])
-            return;
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// shared helpers of the passes
-// ------------------------------------------------------------------------------------------------
-// Workgroup id -> logical tile so that the 8 XCDs (workgroups are dealt round-robin) each get a
-// contiguous run of tiles and keep their L2 on one region of the world.
-__device__ __forceinline__ int32_t xcd_remap(int32_t i, int32_t n) {
-    const int32_t q = n >> 3, r = n & 7, x = i & 7;
-    return x * q + imin(x, r) + (i >> 3);
-}
-
-__device__ __forceinline__ void store_hit(vf_hit *out, const Hit &h) {
-    uint32_t *o = reinterpret_cast<uint32_t *>(out);
-    if (h.hit) {
-        o[0] = (uint32_t)h.cell[0]; o[1] = (uint32_t)h.cell[1]; o[2] = (uint32_t)h.cell[2];
-        o[3] = __float_as_uint(h.t);
-        o[4] = (h.mat & 255u) | ((uint32_t)h.face << 8) | (1u << 16);
-    } else {
-        o[0] = 0u; o[1] = 0u; o[2] = 0u;
-        o[3] = __float_as_uint(-1.0f);
-        o[4] = 0u;
-    }
-}
-
-template <bool COUNT>
-__device__ __forceinline__ void flush_count(const Cnt &c, unsigned long long *cnt) {
-    if (COUNT) {
-        atomicAdd(cnt + 0, c.rays);
-        atomicAdd(cnt + 1, c.outer);
-        atomicAdd(cnt + 2, c.voxel);
-        atomicAdd(cnt + 3, c.bytes);
-    }
-}
-
-__device__ __forceinline__ uint32_t unorm8(float v) { return (uint32_t)floorf(fclampg(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
-__device__ __forceinline__ float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
-
-__device__ __forceinline__ void face_normal(int32_t face, float n[3]) {
-    n[0] = n[1] = n[2] = 0.0f;
-    if (face > 0) n[(face - 1) >> 1] = (face & 1) ? 1.0f : -1.0f;
-}
-
-// PCG hash and samplers (common.glsl:1233-1288)
-__device__ __forceinline__ uint32_t pcg(uint32_t v) {
-    v = v * 747796405u + 2891336453u;
-    v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
-    return (v >> 22u) ^ v;
-}
-__device__ __forceinline__ float fractg(float x) { return x - floorf(x); }
-
-// sin / cos by Cody-Waite reduction with pi/2 = 1.5703125 + 4.838267948966e-4 and Taylor
-// polynomials on [-pi/4, pi/4]; deterministic, so GI and probe directions are reproducible.
-__device__ __forceinline__ void sincos_red(float x, float &s, float &c) {
-    const float kf = floorf(x * 0.63661977f + 0.5f);
-    const int32_t k = f2i(kf);
-    const float r = (x - kf * 1.5703125f) - kf * 4.838267948966e-4f;
-    const float r2 = r * r;
-    const float ps = r + r * r2 * (-0.16666667f + r2 * (0.0083333333f + r2 * (-1.9841270e-4f + r2 * 2.7557319e-6f)));
-    const float pc = 1.0f + r2 * (-0.5f + r2 * (0.041666667f + r2 * (-1.3888889e-3f + r2 * 2.4801587e-5f)));
-    switch (k & 3) {
-    case 0: s = ps; c = pc; break;
-    case 1: s = pc; c = -ps; break;
-    case 2: s = -ps; c = -pc; break;
-    default: s = -pc; c = ps; break;
-    }
-}
-
-__device__ __forceinline__ void cosine_hemisphere_u(const float n[3], float u0, float u1, float out[3]) {
-    float t1[3], t2[3];
-    const float upv[3] = {0.0f, 1.0f, 0.0f}, uxv[3] = {1.0f, 0.0f, 0.0f};
-    cross3(n, fabsf(n[1]) < 0.9f ? upv : uxv, t1);
-    normalize3(t1);
-    cross3(n, t1, t2);
-    const float r = sqrtf(u0);
-    const float ph = 6.2831853f * u1;
-    float s, c;
-    sincos_red(ph, s, c);
-    const float a = r * c, b = r * s, z = sqrtf(1.0f - u0);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) out[i] = (t1[i] * a + t2[i] * b) + n[i] * z;
-    normalize3(out);
-}
 
 // ------------------------------------------------------------------------------------------------
 // pass kernels
@@ -906,33 +566,6 @@ __global__ void k_update_slots(uint8_t *atlas, const uint32_t *slots, const uint
 // ------------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------------
-struct vf_ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    std::string err;
-    bool has_world = false;
-    bool has_scene = false;
-    DevWorld W{};
-    SkyParams S{};
-    vf_scene_ubo scene{};
-    // owned device buffers
-    uint32_t *grid = nullptr;
-    uint2 *cells = nullptr;
-    uint8_t *atlas = nullptr;
-    uint32_t *occ = nullptr, *occdry = nullptr, *masks_ref = nullptr;
-    uint8_t *dist[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint8_t *c1 = nullptr, *c2 = nullptr;
-    uint16_t *skytop = nullptr;
-    float4 *palette = nullptr;
-    int32_t c1d[3] = {0, 0, 0}, c2d[3] = {0, 0, 0};
-    uint32_t n_slots = 0;
-    size_t n_cells = 0, n_cells_tiled = 0;
-    bool counting = false;
-    unsigned long long *d_cnt = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-};
 
 namespace vf {
 std::string &thread_error() {
@@ -942,20 +575,55 @@ std::string &thread_error() {
 void set_ctx_error(vf_ctx *ctx, const std::string &msg) { ctx->err = msg; }
 }  // namespace vf
 
-#define VF_HIP(ctx, call)                                                                                         \
-    do {                                                                                                          \
-        hipError_t e_ = (call);                                                                                   \
-        if (e_ != hipSuccess)                                                                                     \
-            return vf::fail((ctx), e_ == hipErrorOutOfMemory ? VF_E_OOM : VF_E_DEVICE, "%s failed: %s (%s:%d)", #call, \
-                            hipGetErrorString(e_), __FILE__, __LINE__);                                           \
-    } while (0)
-
-namespace {
+namespace vf {
 
 int set_device(vf_ctx *ctx) {
     VF_HIP(ctx, hipSetDevice(ctx->device));
     return VF_OK;
 }
+
+int launch_check(vf_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return vf::fail(ctx, VF_E_DEVICE, "%s launch failed: %s", what, hipGetErrorString(e));
+    return VF_OK;
+}
+
+int begin_timed(vf_ctx *ctx) {
+    if (ctx->capturing) return VF_OK;
+    VF_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return VF_OK;
+}
+
+int end_timed(vf_ctx *ctx) {
+    if (ctx->capturing) return VF_OK;
+    VF_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    return VF_OK;
+}
+
+// Calls that synchronise or upload cannot be recorded into a graph.
+int no_capture(vf_ctx *ctx, const char *what) {
+    if (ctx && ctx->capturing) return vf::fail(ctx, VF_E_INVALID, "%s: not allowed while a graph is being captured", what);
+    return VF_OK;
+}
+
+int check_ready(vf_ctx *ctx, const char *what) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "%s: null context", what);
+    if (!ctx->has_world) return vf::fail(ctx, VF_E_NO_WORLD, "%s: no world uploaded (vf_world_upload)", what);
+    return set_device(ctx);
+}
+
+}  // namespace vf
+
+using vf::begin_timed;
+using vf::blocks_for;
+using vf::check_ready;
+using vf::end_timed;
+using vf::launch_check;
+using vf::no_capture;
+using vf::set_device;
+
+namespace {
 
 void free_world(vf_ctx *ctx) {
     void *ptrs[] = {ctx->grid, ctx->cells, ctx->atlas, ctx->occ, ctx->occdry, ctx->masks_ref, ctx->dist[0], ctx->dist[1],
@@ -971,13 +639,7 @@ void free_world(vf_ctx *ctx) {
     ctx->has_world = false;
 }
 
-inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
-int launch_check(vf_ctx *ctx, const char *what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return vf::fail(ctx, VF_E_DEVICE, "%s launch failed: %s", what, hipGetErrorString(e));
-    return VF_OK;
-}
 
 int rebuild_coarse(vf_ctx *ctx) {
     const int32_t *wb = ctx->W.wb;
@@ -1051,21 +713,8 @@ int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], 
     return VF_OK;
 }
 
-int begin_timed(vf_ctx *ctx) {
-    VF_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    return VF_OK;
-}
-int end_timed(vf_ctx *ctx) {
-    VF_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    ctx->timed = true;
-    return VF_OK;
-}
 
-int check_ready(vf_ctx *ctx, const char *what) {
-    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "%s: null context", what);
-    if (!ctx->has_world) return vf::fail(ctx, VF_E_NO_WORLD, "%s: no world uploaded (vf_world_upload)", what);
-    return set_device(ctx);
-}
+
 
 }  // namespace
 
@@ -1108,6 +757,11 @@ int vf_ctx_create(int device, vf_ctx **out) {
         return rc;
     }
     ctx->stream = ctx->own_stream;
+    {
+        int cu = 0, khz = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) ctx->n_cu = cu;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0) ctx->wall_khz = khz;
+    }
     *out = ctx;
     return VF_OK;
 }
@@ -1117,6 +771,7 @@ void vf_ctx_destroy(vf_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_world(ctx);
+    vf::free_queue(ctx);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1126,13 +781,16 @@ void vf_ctx_destroy(vf_ctx *ctx) {
 
 int vf_set_stream(vf_ctx *ctx, void *s) {
     if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_set_stream: null context");
+    if (int rc = no_capture(ctx, "vf_set_stream")) return rc;
     ctx->stream = s ? reinterpret_cast<hipStream_t>(s) : ctx->own_stream;
     return VF_OK;
 }
 
 int vf_sync(vf_ctx *ctx) {
     if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_sync: null context");
-    int rc = set_device(ctx);
+    int rc = no_capture(ctx, "vf_sync");
+    if (rc) return rc;
+    rc = set_device(ctx);
     if (rc) return rc;
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return VF_OK;
@@ -1140,6 +798,7 @@ int vf_sync(vf_ctx *ctx) {
 
 int vf_device_alloc(vf_ctx *ctx, size_t bytes, void **d_out) {
     if (!ctx || !d_out) return vf::fail(ctx, VF_E_INVALID, "vf_device_alloc: null argument");
+    if (int rc = no_capture(ctx, "vf_device_alloc")) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     *d_out = nullptr;
@@ -1148,6 +807,7 @@ int vf_device_alloc(vf_ctx *ctx, size_t bytes, void **d_out) {
 }
 int vf_device_free(vf_ctx *ctx, void *p) {
     if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_device_free: null context");
+    if (int rc = no_capture(ctx, "vf_device_free")) return rc;
     if (!p) return VF_OK;
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -1157,6 +817,7 @@ int vf_device_free(vf_ctx *ctx, void *p) {
 }
 int vf_copy_to_device(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_copy_to_device: null argument");
+    if (int rc = no_capture(ctx, "vf_copy_to_device")) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     VF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -1165,6 +826,7 @@ int vf_copy_to_device(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 int vf_copy_to_host(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_copy_to_host: null argument");
+    if (int rc = no_capture(ctx, "vf_copy_to_host")) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     VF_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -1173,6 +835,7 @@ int vf_copy_to_host(vf_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 int vf_memset_device(vf_ctx *ctx, void *dst, int value, size_t bytes) {
     if (!ctx || (!dst && bytes)) return vf::fail(ctx, VF_E_INVALID, "vf_memset_device: null argument");
+    if (int rc = no_capture(ctx, "vf_memset_device")) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     VF_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
@@ -1181,6 +844,7 @@ int vf_memset_device(vf_ctx *ctx, void *dst, int value, size_t bytes) {
 
 int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
     if (!ctx || !d) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: null argument");
+    if (int rc = no_capture(ctx, "vf_world_upload")) return rc;
     if (!d->grid || !d->palette) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: grid and palette are required");
     for (int a = 0; a < 3; ++a)
         if (d->world_bricks[a] <= 0 || d->world_bricks[a] > 4096)
@@ -1291,6 +955,7 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
                     const uint8_t *slot_vox, uint32_t n_slot_updates) {
     int rc = check_ready(ctx, "vf_world_update");
     if (rc) return rc;
+    if ((rc = no_capture(ctx, "vf_world_update"))) return rc;
     if ((n_cells && (!cell_idx || !cell_val)) || (n_slot_updates && (!slot_idx || !slot_vox)))
         return vf::fail(ctx, VF_E_INVALID, "vf_world_update: null arrays");
     for (uint32_t i = 0; i < n_cells; ++i) {
@@ -1339,6 +1004,60 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
     return VF_OK;
 }
 
+int vf_graph_begin(vf_ctx *ctx) {
+    if (!ctx) return vf::fail(nullptr, VF_E_INVALID, "vf_graph_begin: null context");
+    if (ctx->capturing) return vf::fail(ctx, VF_E_INVALID, "vf_graph_begin: already capturing");
+    if (!ctx->stream) return vf::fail(ctx, VF_E_INVALID, "vf_graph_begin: the null stream cannot be captured");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
+    return VF_OK;
+}
+
+int vf_graph_end(vf_ctx *ctx, vf_graph **out) {
+    if (!ctx || !out) return vf::fail(ctx, VF_E_INVALID, "vf_graph_end: null argument");
+    *out = nullptr;
+    if (!ctx->capturing) return vf::fail(ctx, VF_E_INVALID, "vf_graph_end: no capture in progress");
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    VF_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
+    vf_graph *G = new (std::nothrow) vf_graph();
+    if (!G) {
+        (void)hipGraphDestroy(g);
+        return vf::fail(ctx, VF_E_OOM, "vf_graph_end: out of host memory");
+    }
+    G->device = ctx->device;
+    G->graph = g;
+    hipError_t e = hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete G;
+        return vf::fail(ctx, VF_E_DEVICE, "vf_graph_end: hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *out = G;
+    return VF_OK;
+}
+
+int vf_graph_launch(vf_ctx *ctx, vf_graph *g) {
+    if (!ctx || !g) return vf::fail(ctx, VF_E_INVALID, "vf_graph_launch: null argument");
+    if (ctx->capturing) return vf::fail(ctx, VF_E_INVALID, "vf_graph_launch: capture in progress");
+    if (g->device != ctx->device) return vf::fail(ctx, VF_E_INVALID, "vf_graph_launch: graph of another device");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = begin_timed(ctx))) return rc;
+    VF_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return end_timed(ctx);
+}
+
+void vf_graph_destroy(vf_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
 int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]) {
     int rc = check_ready(ctx, "vf_build_brickdist");
     if (rc) return rc;
@@ -1355,6 +1074,7 @@ int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], co
 int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry) {
     int rc = check_ready(ctx, "vf_read_brickdist");
     if (rc) return rc;
+    if ((rc = no_capture(ctx, "vf_read_brickdist"))) return rc;
     if (dist) VF_HIP(ctx, hipMemcpyAsync(dist, ctx->dist[0], ctx->n_cells, hipMemcpyDeviceToHost, ctx->stream));
     if (dry) VF_HIP(ctx, hipMemcpyAsync(dry, ctx->dist[2], ctx->n_cells, hipMemcpyDeviceToHost, ctx->stream));
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1364,6 +1084,7 @@ int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry) {
 int vf_read_skytop(vf_ctx *ctx, uint16_t *sky) {
     int rc = check_ready(ctx, "vf_read_skytop");
     if (rc) return rc;
+    if ((rc = no_capture(ctx, "vf_read_skytop"))) return rc;
     if (!sky) return vf::fail(ctx, VF_E_INVALID, "vf_read_skytop: null output");
     VF_HIP(ctx, hipMemcpyAsync(sky, ctx->skytop, (size_t)ctx->W.wb[0] * 8 * ctx->W.wb[2] * 8 * 2, hipMemcpyDeviceToHost, ctx->stream));
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1408,6 +1129,7 @@ int vf_set_counting(vf_ctx *ctx, int32_t on) {
 
 int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset) {
     if (!ctx || !out) return vf::fail(ctx, VF_E_INVALID, "vf_read_counters: null argument");
+    if (int rc = no_capture(ctx, "vf_read_counters")) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     unsigned long long h[4];
@@ -1420,6 +1142,7 @@ int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset) {
 
 int vf_last_kernel_ms(vf_ctx *ctx, float *ms) {
     if (!ctx || !ms) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: null argument");
+    if (int rc = no_capture(ctx, "vf_last_kernel_ms")) return rc;
     if (!ctx->timed) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: nothing timed yet");
     int rc = set_device(ctx);
     if (rc) return rc;

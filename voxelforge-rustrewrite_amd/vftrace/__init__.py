@@ -13,7 +13,9 @@ import os
 
 import numpy as np
 
-from .abi import (GenParams, Hit, HIT_DTYPE, Material, PassUbo, Ray, RAY_DTYPE, SceneUbo, WorldDesc)
+from .abi import (GenParams, Hit, HIT_DTYPE, Material, PassUbo, QueueCfg, Ray, RAY_DTYPE, Request, SceneUbo,
+                  TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH, VF_TUNE_GI_BATCH,
+                  WorldDesc)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
            "RAY_DTYPE", "fly_camera", "sun_dir", "make_pass", "default_scene", "library_path"]
@@ -61,6 +63,17 @@ _SIGS = {
     "vf_trace_rays": (C.c_int, [_P, _P, C.c_int64, C.c_float, C.c_uint32, _P]),
     "vf_trace_probes": (C.c_int, [_P, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_int32), C.c_int32, C.c_uint32,
                                   C.c_float, _P]),
+    "vf_queue_defaults": (C.c_int, [C.POINTER(QueueCfg)]),
+    "vf_trace_gi_batch": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_float, _P, _P, _P, _P,
+                                    C.POINTER(QueueCfg)]),
+    "vf_trace_batch": (C.c_int, [_P, C.POINTER(Request), C.c_int32, C.POINTER(QueueCfg)]),
+    "vf_read_batch_times": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "vf_autotune": (C.c_int, [_P, C.c_int32, _P, C.POINTER(QueueCfg), C.POINTER(TuneResult), C.c_int32,
+                              C.POINTER(C.c_int32)]),
+    "vf_graph_begin": (C.c_int, [_P]),
+    "vf_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "vf_graph_launch": (C.c_int, [_P, _P]),
+    "vf_graph_destroy": (None, [_P]),
     "vf_set_counting": (C.c_int, [_P, C.c_int32]),
     "vf_read_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32]),
     "vf_last_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
@@ -360,6 +373,89 @@ class Context:
         self._ck(lib().vf_trace_probes(self._h, o, float(spacing), _i3(dims), int(n_rays), int(frame), float(reach),
                                        _ptr(hits)))
 
+    # persistent-queue batches
+    @staticmethod
+    def queue_cfg(tile=(8, 8), chunk=256, sort=False, groups_per_cu=0):
+        c = QueueCfg()
+        c.tile_w, c.tile_h = int(tile[0]), int(tile[1])
+        c.chunk, c.sort, c.groups_per_cu = int(chunk), int(bool(sort)), int(groups_per_cu)
+        return c
+
+    def trace_gi_batch(self, pass_ubo, width, height, reach, gb_pos, gb_normal, gi_hits, sun_hits, cfg=None):
+        self._ck(lib().vf_trace_gi_batch(self._h, C.byref(pass_ubo), int(width), int(height), float(reach),
+                                         _ptr(gb_pos), _ptr(gb_normal), _ptr(gi_hits), _ptr(sun_hits),
+                                         C.byref(cfg) if cfg is not None else None))
+
+    @staticmethod
+    def view_request(pass_ubo, width, height, hits):
+        r = Request()
+        r.kind = VF_REQ_VIEW
+        r.width, r.height = int(width), int(height)
+        for i in range(3):
+            r.cam_pos[i], r.cam_right[i] = pass_ubo.cam_pos[i], pass_ubo.cam_right[i]
+            r.cam_up[i], r.cam_fwd[i] = pass_ubo.cam_up[i], pass_ubo.cam_fwd[i]
+        r.tan_half_fov = pass_ubo.tan_half_fov
+        r.jitter[0], r.jitter[1] = pass_ubo.jitter[0], pass_ubo.jitter[1]
+        r.d_hits = hits.data_ptr()
+        return r
+
+    @staticmethod
+    def probe_request(origin, spacing, dims, n_rays, frame, reach, hits):
+        r = Request()
+        r.kind = VF_REQ_PROBES
+        for i in range(3):
+            r.origin[i] = float(origin[i])
+            r.dims[i] = int(dims[i])
+        r.spacing, r.n_rays, r.frame, r.reach = float(spacing), int(n_rays), int(frame), float(reach)
+        r.d_hits = hits.data_ptr()
+        return r
+
+    def trace_batch(self, requests, cfg=None):
+        arr = (Request * len(requests))(*requests)
+        self._ck(lib().vf_trace_batch(self._h, arr, len(requests), C.byref(cfg) if cfg is not None else None))
+
+    def batch_times(self, n):
+        out = (C.c_double * n)()
+        self._ck(lib().vf_read_batch_times(self._h, int(n), out))
+        return list(out)
+
+    def autotune_gi_batch(self, pass_ubo, width, height, reach, gb_pos, gb_normal, gi_hits, sun_hits):
+        a = TuneGiArgs()
+        a.pass_ = C.cast(C.pointer(pass_ubo), C.c_void_p)
+        a.width, a.height, a.reach = int(width), int(height), float(reach)
+        a.d_gb_pos, a.d_gb_normal = gb_pos.data_ptr(), gb_normal.data_ptr()
+        a.d_gi_hits, a.d_sun_hits = gi_hits.data_ptr(), sun_hits.data_ptr()
+        return self._autotune(VF_TUNE_GI_BATCH, a)
+
+    def autotune_batch(self, requests):
+        arr = (Request * len(requests))(*requests)
+        a = TuneBatchArgs()
+        a.reqs = C.cast(arr, C.c_void_p)
+        a.n_req = len(requests)
+        self._keep = arr
+        return self._autotune(VF_TUNE_BATCH, a)
+
+    def _autotune(self, workload, args):
+        best = QueueCfg()
+        res = (TuneResult * 64)()
+        n = C.c_int32()
+        self._ck(lib().vf_autotune(self._h, workload, C.cast(C.pointer(args), C.c_void_p), C.byref(best), res, 64,
+                                   C.byref(n)))
+        table = [((r.cfg.tile_w, r.cfg.tile_h), r.cfg.chunk, bool(r.cfg.sort), r.ms) for r in res[:n.value]]
+        return best, table
+
+    # frame graphs
+    def graph_begin(self):
+        self._ck(lib().vf_graph_begin(self._h))
+
+    def graph_end(self):
+        g = C.c_void_p()
+        self._ck(lib().vf_graph_end(self._h, C.byref(g)))
+        return Graph(g.value)
+
+    def graph_launch(self, graph):
+        self._ck(lib().vf_graph_launch(self._h, graph._h))
+
     # measurement
     def set_counting(self, on):
         self._ck(lib().vf_set_counting(self._h, int(bool(on))))
@@ -373,6 +469,24 @@ class Context:
         ms = C.c_float()
         self._ck(lib().vf_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+
+class Graph:
+    """A recorded frame (vf_graph); replay with Context.graph_launch."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().vf_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def hits_to_numpy(t):

@@ -95,6 +95,35 @@ class GenParams(C.Structure):
     ]
 
 
+class QueueCfg(C.Structure):
+    _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("chunk", C.c_int32), ("sort", C.c_int32),
+                ("groups_per_cu", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("cam_pos", f3), ("cam_right", f3), ("cam_up", f3), ("cam_fwd", f3), ("tan_half_fov", C.c_float),
+                ("jitter", C.c_float * 2), ("origin", f3), ("spacing", C.c_float), ("dims", i3),
+                ("n_rays", C.c_int32), ("frame", C.c_uint32), ("reach", C.c_float), ("d_hits", C.c_void_p)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("cfg", QueueCfg), ("ms", C.c_float)]
+
+
+class TuneGiArgs(C.Structure):
+    _fields_ = [("pass_", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("reach", C.c_float),
+                ("d_gb_pos", C.c_void_p), ("d_gb_normal", C.c_void_p), ("d_gi_hits", C.c_void_p),
+                ("d_sun_hits", C.c_void_p)]
+
+
+class TuneBatchArgs(C.Structure):
+    _fields_ = [("reqs", C.c_void_p), ("n_req", C.c_int32)]
+
+
+VF_REQ_VIEW, VF_REQ_PROBES = 0, 1
+VF_TUNE_GI_BATCH, VF_TUNE_BATCH = 0, 1
+
 # numpy views of the 24-byte ray and 20-byte hit records
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("d", "<f4", (3,))])
 HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
