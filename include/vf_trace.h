@@ -187,6 +187,12 @@ int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_
  * shadowRaw is RG16F (vis 0/1, min(tBlocker, 60000)); d_hits (nullable) gets the sun ray's hit. */
 int vf_trace_shadow(vf_ctx *ctx, int32_t width, int32_t height, const float *d_gb_pos,
                     const uint32_t *d_gb_normal, uint32_t *d_shadow_raw, vf_hit *d_hits);
+/* P::Gbuf followed by P::ShTrace (record.rs:436, 463: record_standard's order) in ONE kernel: the same
+ * outputs as vf_trace_primary + vf_trace_shadow (the sun ray starts from the written, quantised
+ * G-buffer values), without the second launch and the G-buffer read-back.  gb_pos / gb_normal may be
+ * NULL when only hits and shadowRaw are wanted; shadow_hits is nullable. */
+int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
+                            float *d_gb_pos, uint32_t *d_gb_normal, uint32_t *d_shadow_raw, vf_hit *d_shadow_hits);
 /* P::Gi (record.rs:350/361; gi.comp:34-84 + giPath 1 bounce): cosine-hemisphere GI ray from the
  * quantised G-buffer normal (R2 sequence, GI_LDS), its sun ray, sky escape via the sky-top map.
  * half_res != 0: the reference's half-resolution dispatch ((w+1)/2 x (h+1)/2 outputs); else one
@@ -206,11 +212,13 @@ int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int
 
 /* ---- persistent work-queue batches ------------------------------------------------------- */
 /* Queue parameters: the pixel tile that consecutive ray ids cover (8x8, 16x8, 16x16, 8x16, 32x8), rays
- * a wavefront claims per queue fetch (64..4096, multiple of 64), whether GI rays are bucket-sorted by
- * direction octant and origin cell first, and an optional cap on workgroups per CU (0 = as many as
- * fit).  Results never depend on these. */
+ * a wavefront claims per queue fetch (64..4096, multiple of 64), whether GI rays are grouped by
+ * direction octant within each 256-pixel block, an optional cap on workgroups per CU (0 = as many as
+ * fit) and how many lanes must be idle before a wavefront fetches new rays (1..64; 0 = chunk mode: a
+ * wavefront traces its claimed chunk 64 rays at a time, each to completion).  Results never depend
+ * on these. */
 typedef struct vf_queue_cfg {
-    int32_t tile_w, tile_h, chunk, sort, groups_per_cu;
+    int32_t tile_w, tile_h, chunk, sort, groups_per_cu, refill;
 } vf_queue_cfg;
 int vf_queue_defaults(vf_queue_cfg *cfg);
 
@@ -248,8 +256,9 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
 /* Per-request completion times of the last batch, ms after the launch started (host, synchronises). */
 int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms);
 
-/* Autotuner: times every tile shape {8x8, 16x8, 16x16} x chunk {64..1024} (x sort off/on for the GI
- * batch) on the given batch (median of 3 runs) and returns the fastest configuration. */
+/* Autotuner: times every refill mode {chunk mode, refill 32} x tile shape {8x8, 16x8, 16x16} x chunk
+ * {64..1024} (x sort off/on for the GI batch) on the given batch (median of 3 runs) and returns the
+ * fastest configuration; base (nullable) supplies groups_per_cu. */
 enum { VF_TUNE_GI_BATCH = 0, VF_TUNE_BATCH = 1 };
 typedef struct vf_tune_result { vf_queue_cfg cfg; float ms; } vf_tune_result;
 typedef struct vf_tune_gi_args {
@@ -261,8 +270,8 @@ typedef struct vf_tune_gi_args {
     vf_hit *d_gi_hits, *d_sun_hits;
 } vf_tune_gi_args;
 typedef struct vf_tune_batch_args { const vf_request *reqs; int32_t n_req; } vf_tune_batch_args;
-int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, vf_queue_cfg *best, vf_tune_result *results,
-                int32_t max_results, int32_t *n_results);
+int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_cfg *base, vf_queue_cfg *best,
+                vf_tune_result *results, int32_t max_results, int32_t *n_results);
 
 /* ---- frame graphs ------------------------------------------------------------------------- */
 /* A frame's trace calls recorded once and replayed (the reference records a command buffer per

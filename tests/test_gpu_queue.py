@@ -36,16 +36,17 @@ def gbuffer(ctx, pu, W, H):
     return gbp, gbn
 
 
-@pytest.mark.parametrize("tile,chunk,sort", [((8, 8), 64, False), ((16, 8), 256, True), ((16, 16), 1024, True),
-                                             ((8, 8), 512, True)])
-def test_gi_batch_matches_oracle(ctx, tile, chunk, sort):
+@pytest.mark.parametrize("tile,chunk,sort,refill", [((8, 8), 64, False, 32), ((16, 8), 256, True, 32),
+                                                    ((16, 16), 1024, True, 48), ((8, 8), 512, True, 0),
+                                                    ((8, 8), 64, False, 0)])
+def test_gi_batch_matches_oracle(ctx, tile, chunk, sort, refill):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=6)
     W, H = 640, 360
     gbp, gbn = gbuffer(ctx, pu, W, H)
     gi = dev_hits(W * H)
     sun = dev_hits(W * H)
-    ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg(tile, chunk, sort))
+    ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg(tile, chunk, sort, refill=refill))
     ctx.sync()
     scene = vf.default_scene(a["world_bricks"])
     og, osun = O.gi_batch(ow, W, H, 6, gbp.cpu().numpy().reshape(-1, 4), gbn.cpu().numpy().view(np.uint32),
@@ -57,7 +58,8 @@ def test_gi_batch_matches_oracle(ctx, tile, chunk, sort):
     assert 0.05 < g["hit"].mean() < 0.95 and s["hit"].sum() > 0
 
 
-def test_gi_batch_counters_match_oracle(ctx):
+@pytest.mark.parametrize("refill", [0, 32])
+def test_gi_batch_counters_match_oracle(ctx, refill):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=2)
     W, H = 320, 180
@@ -66,7 +68,7 @@ def test_gi_batch_counters_match_oracle(ctx):
     ctx.set_counting(True)
     try:
         ctx.counters(reset=True)
-        ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg((16, 8), 128, True))
+        ctx.trace_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun, ctx.queue_cfg((16, 8), 128, True, refill=refill))
         c = ctx.counters(reset=True)
     finally:
         ctx.set_counting(False)
@@ -98,7 +100,8 @@ def test_uneven_batch_matches_plain_launches(ctx):
     sizes = [(64, 64), (256, 144), (130, 75), (480, 270)]
     probes = [((-64.5, -32.25, -64.75), 20.0, (8, 6, 8), 256, 3, 256.0)]
     reqs, outs, passes = _requests(ctx, sizes, probes)
-    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 1024), ctx.queue_cfg((16, 8), 256)):
+    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 1024), ctx.queue_cfg((16, 8), 256),
+                ctx.queue_cfg((8, 8), 64, refill=0), ctx.queue_cfg((16, 16), 512, refill=0)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)
@@ -159,7 +162,7 @@ def test_autotuner_picks_a_measured_config(ctx):
     gbp, gbn = gbuffer(ctx, pu, W, H)
     gi, sun = dev_hits(W * H), dev_hits(W * H)
     best, table = ctx.autotune_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun)
-    assert len(table) == 3 * 5 * 2
+    assert len(table) == 2 * 3 * 5 * 2
     fastest = min(table, key=lambda r: r[3])
     assert (best.tile_w, best.tile_h) == fastest[0] and best.chunk == fastest[1] and bool(best.sort) == fastest[2]
     assert all(r[3] > 0 for r in table)

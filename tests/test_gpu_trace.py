@@ -298,3 +298,21 @@ def test_frame_graph_replays_match_eager(ctx):
         g.close()
     finally:
         ctx.set_stream(None)
+
+
+def test_fused_primary_shadow_equals_two_passes(ctx):
+    w, a, ow = world(2, world_bricks=(64, 32, 64), chunks=3, trees_per_chunk=60)
+    upload(ctx, w, a)
+    gy = w.ground(-37, 21)
+    pu, _ = camera([-37.4, gy + 17.0, 21.6], 4.1, -0.08)
+    W, H = 333, 197
+    hits, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    shh = dev_hits(W * H)
+    ctx.trace_shadow(W, H, gbp, gbn, sh, shh)
+    f_hits, f_gbp, f_gbn = dev_hits(W * H), torch.zeros_like(gbp), torch.zeros_like(gbn)
+    f_sh, f_shh = torch.zeros_like(sh), dev_hits(W * H)
+    ctx.trace_primary_shadow(pu, W, H, f_hits, f_gbp, f_gbn, f_sh, f_shh)
+    ctx.sync()
+    for x, y in ((hits, f_hits), (gbp, f_gbp), (gbn, f_gbn), (sh, f_sh), (shh, f_shh)):
+        assert torch.equal(x, y)

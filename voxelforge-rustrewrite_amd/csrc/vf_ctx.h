@@ -47,8 +47,7 @@ struct vf_ctx {
     long long wall_khz = 100000;   // wall clock rate (batch completion stamps)
     // persistent-queue buffers (vf_queue.hip), grown on first use outside a capture
     unsigned long long *q_ctr = nullptr, *q_stamps = nullptr;
-    float4 *q_rays = nullptr, *q_rays2 = nullptr, *q_sun = nullptr;
-    uint32_t *q_keys = nullptr, *q_buckets = nullptr;
+    float4 *q_rays = nullptr, *q_sun = nullptr;
     int64_t q_cap = 0;
 };
 

@@ -34,7 +34,6 @@ constexpr int kMaxReq = VF_MAX_REQUESTS;
 // counters: [0] ray-list length / batch head, [1] GI head, [2] sun tail, [3] sun head,
 // [8 + r] remaining rays of request r
 constexpr int kCtr = 8 + kMaxReq;
-constexpr int kBuckets = 8 * 4096;   // direction octant x 16^3 cells of 8^3 bricks
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
@@ -60,90 +59,60 @@ struct GenArgs {
     int32_t minb[3];
 };
 
-__device__ __forceinline__ uint32_t spread4(uint32_t v) {   // interleave 4 bits with 2 zero bits
-    v &= 15u;
-    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
-}
-
 // Pixel -> GI ray of gi.comp at full resolution (fpix = pix): origin P + N*0.03, direction
-// cosineHemisphereU(N, ldSample2(pix, frame >> 2)).  Sky pixels get miss records.
+// cosineHemisphereU(N, ldSample2(pix, frame >> 2)).  Sky pixels get miss records.  A workgroup covers
+// 256 consecutive pixels of the tile order and reserves room for its rays with one atomic; with
+// `sort` it writes them grouped by direction octant (a counting sort in LDS), so the rays of a
+// wavefront start close together AND share a direction octant.
 __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, const uint32_t *gbn, float4 *rays,
-                                                uint32_t *keys, uint32_t *buckets, unsigned long long *ctr,
-                                                vf_hit *gi_hits, vf_hit *sun_hits) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+                                                unsigned long long *ctr, vf_hit *gi_hits, vf_hit *sun_hits) {
+    __shared__ uint32_t oct_count[8], oct_base[8];
+    __shared__ unsigned long long wg_base;
+    const int t = threadIdx.x;
+    if (t < 8) oct_count[t] = 0u;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + t;
     const int32_t per = A.tile_w * A.tile_h;
     const int64_t tile = i / per;
     const int32_t in = (int32_t)(i % per);
     const int32_t px = (int32_t)(tile % A.tiles_x) * A.tile_w + in % A.tile_w;
     const int32_t py = (int32_t)(tile / A.tiles_x) * A.tile_h + in / A.tile_w;
     const bool inside = px < A.w && py < A.h;
-    const size_t fi = (size_t)py * A.w + px;
+    const size_t fi = inside ? (size_t)py * A.w + px : 0;
     const bool valid = inside && gbpos[fi].w >= 0.0f;
-    const unsigned long long m = __ballot(valid);
-    unsigned long long base = 0;
-    if (lane_id() == 0 && m) base = atomicAdd(ctr + 0, (unsigned long long)__popcll(m));
-    base = __shfl(base, 0);
-    if (!inside) return;
-    if (!valid) {
+    float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t oct = 0u, local = 0u;
+    if (valid) {
+        const float4 P4 = gbpos[fi];
+        const uint32_t nb = gbn[fi];
+        const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+        const uint32_t salt = 0x68e31da4u;
+        const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
+        const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
+        const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
+        const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
+        const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
+        cosine_hemisphere_u(N, u0, u1, d);
+        o[0] = P4.x + N[0] * 3e-2f; o[1] = P4.y + N[1] * 3e-2f; o[2] = P4.z + N[2] * 3e-2f;
+        oct = A.sort ? ((d[0] < 0.0f ? 1u : 0u) | (d[1] < 0.0f ? 2u : 0u) | (d[2] < 0.0f ? 4u : 0u)) : 0u;
+        local = atomicAdd(&oct_count[oct], 1u);
+    } else if (inside) {
         Hit h;
         miss(h);
         store_hit(gi_hits + fi, h);
         store_hit(sun_hits + fi, h);
-        return;
     }
-    const float4 P4 = gbpos[fi];
-    const uint32_t nb = gbn[fi];
-    const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
-    const uint32_t salt = 0x68e31da4u;
-    const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
-    const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
-    const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
-    const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
-    const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
-    float d[3];
-    cosine_hemisphere_u(N, u0, u1, d);
-    const float o[3] = {P4.x + N[0] * 3e-2f, P4.y + N[1] * 3e-2f, P4.z + N[2] * 3e-2f};
-    const uint64_t slot = base + rank_in(m);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0u;
+        for (int k = 0; k < 8; ++k) { oct_base[k] = run; run += oct_count[k]; }
+        wg_base = run ? atomicAdd(ctr + 0, (unsigned long long)run) : 0ull;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const uint64_t slot = wg_base + oct_base[oct] + local;
     rays[slot * 2 + 0] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)fi));
     rays[slot * 2 + 1] = make_float4(d[0], d[1], d[2], A.reach);
-    if (A.sort) {
-        const uint32_t oct = (d[0] < 0.0f ? 1u : 0u) | (d[1] < 0.0f ? 2u : 0u) | (d[2] < 0.0f ? 4u : 0u);
-        const uint32_t cx = (uint32_t)((f2i(floorf(o[0] * 0.125f)) - A.minb[0]) >> 3);
-        const uint32_t cy = (uint32_t)((f2i(floorf(o[1] * 0.125f)) - A.minb[1]) >> 3);
-        const uint32_t cz = (uint32_t)((f2i(floorf(o[2] * 0.125f)) - A.minb[2]) >> 3);
-        const uint32_t key = oct * 4096u + ((spread4(cx) | (spread4(cy) << 1) | (spread4(cz) << 2)) & 4095u);
-        keys[slot] = key;
-        atomicAdd(buckets + key, 1u);
-    }
-}
-
-// Exclusive scan of the bucket histogram (one workgroup of 1024 threads, 32 buckets each).
-__global__ void __launch_bounds__(1024) k_bucket_scan(uint32_t *buckets) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    uint32_t local[32];
-    uint32_t s = 0;
-    for (int k = 0; k < 32; ++k) { local[k] = buckets[t * 32 + k]; s += local[k]; }
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - s;
-    for (int k = 0; k < 32; ++k) { buckets[t * 32 + k] = run; run += local[k]; }
-}
-
-// Scatter into bucket order (order inside a bucket is arbitrary; every result is per ray).
-__global__ void __launch_bounds__(256) k_bucket_scatter(const float4 *rays, const uint32_t *keys, uint32_t *buckets,
-                                                        float4 *out, const unsigned long long *ctr) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= ctr[0]) return;
-    const uint32_t dst = atomicAdd(buckets + keys[i], 1u);
-    out[(size_t)dst * 2 + 0] = rays[i * 2 + 0];
-    out[(size_t)dst * 2 + 1] = rays[i * 2 + 1];
 }
 
 // Claims the next chunk of ray ids for the wave; false when the list is exhausted.
@@ -165,7 +134,7 @@ template <bool SUN, bool COUNT>
 __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *rays, const unsigned long long *len,
                                                       unsigned long long *head, vf_hit *hits, float4 *sun_list,
                                                       unsigned long long *sun_tail, vf_hit *sun_hits, float sx, float sy,
-                                                      float sz, int32_t chunk, unsigned long long *cnt) {
+                                                      float sz, int32_t chunk, int32_t refill, unsigned long long *cnt) {
     const uint64_t n = *len;
     const float sun_dir[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
@@ -176,9 +145,16 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
     uint32_t tag = 0;
     uint64_t pool = 0, pool_end = 0;
     bool drained = false;
+    // sun rays waiting to be appended: at most one per lane, flushed with ONE atomic per wave once
+    // enough lanes hold one (or a lane would need a second slot, or the wave is done)
+    bool pend = false;
+    float pso[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t ptag = 0;
     while (true) {
         bool finished = false;   // this lane finished a ray during this trip
         unsigned long long idle = __ballot(!active);
+        // refill only when enough lanes are idle, so ray set-up runs for many lanes at once
+        if (__popcll(idle) < (unsigned)refill && __any(active)) idle = 0ull;
         while (idle != 0ull && !drained) {
             if (pool >= pool_end && !claim(head, n, chunk, pool, pool_end)) { drained = true; break; }
             const uint64_t avail = pool_end - pool;
@@ -229,20 +205,42 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
                 }
             }
         }
+        const bool ending = drained && !__any(active);
         if (SUN) {
-            const unsigned long long em = __ballot(emit);
-            if (em != 0ull) {
+            const unsigned long long pm = __ballot(pend);
+            const unsigned long long clash = __ballot(pend && emit);
+            if (pm != 0ull && (clash != 0ull || __popcll(pm) >= 48 || ending)) {
                 unsigned long long base = 0;
-                if (lane_id() == 0) base = atomicAdd(sun_tail, (unsigned long long)__popcll(em));
+                if (lane_id() == 0) base = atomicAdd(sun_tail, (unsigned long long)__popcll(pm));
                 base = __shfl(base, 0);
-                if (emit) {
-                    const uint64_t slot = base + rank_in(em);
-                    sun_list[slot * 2 + 0] = make_float4(so[0], so[1], so[2], __uint_as_float(tag));
+                if (pend) {
+                    const uint64_t slot = base + rank_in(pm);
+                    sun_list[slot * 2 + 0] = make_float4(pso[0], pso[1], pso[2], __uint_as_float(ptag));
                     sun_list[slot * 2 + 1] = make_float4(sx, sy, sz, 1e30f);
+                }
+                pend = false;
+            }
+            if (emit) {
+                pend = true;
+                pso[0] = so[0]; pso[1] = so[1]; pso[2] = so[2];
+                ptag = tag;
+            }
+            if (ending) {
+                const unsigned long long pm2 = __ballot(pend);
+                if (pm2 != 0ull) {
+                    unsigned long long base = 0;
+                    if (lane_id() == 0) base = atomicAdd(sun_tail, (unsigned long long)__popcll(pm2));
+                    base = __shfl(base, 0);
+                    if (pend) {
+                        const uint64_t slot = base + rank_in(pm2);
+                        sun_list[slot * 2 + 0] = make_float4(pso[0], pso[1], pso[2], __uint_as_float(ptag));
+                        sun_list[slot * 2 + 1] = make_float4(sx, sy, sz, 1e30f);
+                    }
+                    pend = false;
                 }
             }
         }
-        if (drained && !__any(active)) break;
+        if (ending) break;
     }
     flush_count<COUNT>(c, cnt);
 }
@@ -342,9 +340,18 @@ __device__ __forceinline__ void batch_ray(const ReqDev &q, uint64_t local, float
     out = (uint32_t)local;
 }
 
+// Subtracts k retired rays from request rq; the wave whose subtraction reaches zero stamps the time.
+__device__ __forceinline__ void retire(unsigned long long *ctr, unsigned long long *stamps, int rq, unsigned long long k) {
+    if (k == 0ull) return;
+    if (lane_id() == 0) {
+        const unsigned long long old = atomicSub(ctr + 8 + rq, k);
+        if (old == k) stamps[rq] = wall_clock64();
+    }
+}
+
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, unsigned long long *ctr,
-                                                       unsigned long long *stamps, int32_t chunk,
+                                                       unsigned long long *stamps, int32_t chunk, int32_t refill,
                                                        unsigned long long *cnt) {
     if (threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
     // every lane of the first trip waits for the start stamp of its own workgroup only; requests are
@@ -359,15 +366,21 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
     int32_t pool_req = 0;
     bool drained = false;
     unsigned long long *head = ctr + 0;
+    int acc_req = 0, trips = 0;
+    unsigned long long acc_cnt = 0;
     while (true) {
         bool finished = false;
+        bool claimed = false;
         unsigned long long idle = __ballot(!active);
+        // refill only when enough lanes are idle, so ray set-up runs for many lanes at once
+        if (__popcll(idle) < (unsigned)refill && __any(active)) idle = 0ull;
         while (idle != 0ull && !drained) {
             if (pool >= pool_end) {
                 unsigned long long b = 0;
                 if (lane_id() == 0) b = atomicAdd(head, (unsigned long long)chunk);
                 b = __shfl(b, 0);
                 if (b >= B.total) { drained = true; break; }
+                claimed = true;
                 pool_req = find_req(B, b);
                 const uint64_t rend = B.req[pool_req].first + B.req[pool_req].count;
                 pool = b;
@@ -399,20 +412,129 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
             }
         }
         if (finished && out != 0xFFFFFFFFu) store_hit(B.req[req].hits + out, h);
-        // retire finished rays per request: one atomic per (wave, request); the last one stamps the time
+        // retired rays are counted per wave (wave-uniform acc_req / acc_cnt) and subtracted from the
+        // request's remaining count in one atomic when the request changes, the wave claimed a new
+        // chunk, every 64 trips, or at the end; the atomic that reaches zero stamps the completion
         unsigned long long fm = __ballot(finished);
         while (fm != 0ull) {
             const int leader = __ffsll((long long)fm) - 1;
             const int rq = __shfl(req, leader);
             const unsigned long long same = __ballot(finished && req == rq);
-            if ((int)lane_id() == leader) {
-                const unsigned long long k = (unsigned long long)__popcll(same);
-                const unsigned long long old = atomicSub(ctr + 8 + rq, k);
-                if (old == k) stamps[rq] = wall_clock64();
+            if (rq != acc_req) {
+                retire(ctr, stamps, acc_req, acc_cnt);
+                acc_req = rq;
+                acc_cnt = 0;
             }
+            acc_cnt += (unsigned long long)__popcll(same);
             fm &= ~same;
         }
-        if (drained && !__any(active)) break;
+        const bool ending = drained && !__any(active);
+        if (++trips >= 64 || claimed || ending) {
+            retire(ctr, stamps, acc_req, acc_cnt);
+            acc_cnt = 0;
+            trips = 0;
+        }
+        if (ending) break;
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
+// Chunk mode (refill == 0): a wavefront claims `chunk` consecutive rays and traces them 64 at a time,
+// each lane to completion with trace_world, then claims again.  No per-iteration bookkeeping, the
+// register footprint of the plain kernels; the sun rays of 64 GI rays go out with one atomic.
+template <bool SUN, bool COUNT>
+__global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, const float4 *rays, const unsigned long long *len,
+                                                      unsigned long long *head, vf_hit *hits, float4 *sun_list,
+                                                      unsigned long long *sun_tail, vf_hit *sun_hits, float sx, float sy,
+                                                      float sz, int32_t chunk, unsigned long long *cnt) {
+    const uint64_t n = *len;
+    const float sun_dir[3] = {sx, sy, sz};
+    Cnt c = {0, 0, 0, 0};
+    uint64_t pool = 0, pool_end = 0;
+    while (claim(head, n, chunk, pool, pool_end)) {
+        for (uint64_t base = pool; base < pool_end; base += 64) {
+            const uint64_t id = base + lane_id();
+            bool emit = false;
+            float so[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t tag = 0;
+            if (id < pool_end) {
+                const float4 a = rays[id * 2 + 0], b = rays[id * 2 + 1];
+                const float ro[3] = {a.x, a.y, a.z}, rd[3] = {b.x, b.y, b.z};
+                tag = __float_as_uint(a.w);
+                Hit h;
+                trace_world<0, COUNT>(W, ro, rd, b.w, h, c);
+                store_hit(hits + tag, h);
+                if (SUN) {
+                    if (h.hit && h.mat != 5u) {
+                        float hn[3];
+                        face_normal(h.face, hn);
+                        if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
+                            emit = true;
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                const float hp = ro[k] + rd[k] * h.t;
+                                so[k] = hp + hn[k] * 3e-2f;
+                            }
+                        }
+                    }
+                    if (!emit) {
+                        Hit mh;
+                        miss(mh);
+                        store_hit(sun_hits + tag, mh);
+                    }
+                }
+            }
+            if (SUN) {
+                const unsigned long long em = __ballot(emit);
+                if (em != 0ull) {
+                    unsigned long long sb = 0;
+                    if (lane_id() == 0) sb = atomicAdd(sun_tail, (unsigned long long)__popcll(em));
+                    sb = __shfl(sb, 0);
+                    if (emit) {
+                        const uint64_t slot = sb + rank_in(em);
+                        sun_list[slot * 2 + 0] = make_float4(so[0], so[1], so[2], __uint_as_float(tag));
+                        sun_list[slot * 2 + 1] = make_float4(sx, sy, sz, 1e30f);
+                    }
+                }
+            }
+        }
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
+// Chunk mode of the uneven batch: a claimed chunk belongs to one request (wave-uniform), so the
+// request's fields are scalar loads and views (skipMat 0) and probes (skipMat 5) never diverge.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, unsigned long long *ctr,
+                                                       unsigned long long *stamps, int32_t chunk,
+                                                       unsigned long long *cnt) {
+    if (threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
+    Cnt c = {0, 0, 0, 0};
+    while (true) {
+        unsigned long long b = 0;
+        if (lane_id() == 0) b = atomicAdd(ctr + 0, (unsigned long long)chunk);
+        b = __shfl(b, 0);
+        if (b >= B.total) break;
+        const int rq = find_req(B, b);
+        const ReqDev &q = B.req[rq];
+        const uint64_t rend = q.first + q.count;
+        const uint64_t end = b + (uint64_t)chunk < rend ? b + (uint64_t)chunk : rend;
+        for (uint64_t base = b; base < end; base += 64) {
+            const uint64_t id = base + lane_id();
+            if (id < end) {
+                float ro[3], rd[3], tcap;
+                uint32_t out;
+                int skip;
+                batch_ray(q, id - q.first, ro, rd, tcap, out, skip);
+                if (out != 0xFFFFFFFFu) {
+                    Hit h;
+                    if (q.kind == 0) trace_world<0, COUNT>(W, ro, rd, tcap, h, c);
+                    else trace_world<5, COUNT>(W, ro, rd, tcap, h, c);
+                    store_hit(q.hits + out, h);
+                }
+            }
+        }
+        if (end > b) retire(ctr, stamps, rq, (unsigned long long)(end - b));
     }
     flush_count<COUNT>(c, cnt);
 }
@@ -425,12 +547,11 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
 namespace vf {
 
 void free_queue(vf_ctx *ctx) {
-    void *ptrs[] = {ctx->q_ctr, ctx->q_stamps, ctx->q_rays, ctx->q_rays2, ctx->q_sun, ctx->q_keys, ctx->q_buckets};
+    void *ptrs[] = {ctx->q_ctr, ctx->q_stamps, ctx->q_rays, ctx->q_sun};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ctx->q_ctr = ctx->q_stamps = nullptr;
-    ctx->q_rays = ctx->q_rays2 = ctx->q_sun = nullptr;
-    ctx->q_keys = ctx->q_buckets = nullptr;
+    ctx->q_rays = ctx->q_sun = nullptr;
     ctx->q_cap = 0;
 }
 
@@ -457,10 +578,7 @@ int reserve_queue(vf_ctx *ctx, int64_t rays) {
     VF_QALLOC(ctx->q_ctr, kCtr * 8);
     VF_QALLOC(ctx->q_stamps, (kMaxReq + 1) * 8);
     VF_QALLOC(ctx->q_rays, (size_t)cap * 32);
-    VF_QALLOC(ctx->q_rays2, (size_t)cap * 32);
     VF_QALLOC(ctx->q_sun, (size_t)cap * 32);
-    VF_QALLOC(ctx->q_keys, (size_t)cap * 4);
-    VF_QALLOC(ctx->q_buckets, (size_t)kBuckets * 4);
 #undef VF_QALLOC
     ctx->q_cap = cap;
     return VF_OK;
@@ -481,12 +599,14 @@ int check_cfg(vf_ctx *ctx, const vf_queue_cfg *cfg, const char *what) {
     if (!tiles) return vf::fail(ctx, VF_E_INVALID, "%s: tile %dx%d not supported", what, cfg->tile_w, cfg->tile_h);
     if (cfg->chunk < 64 || cfg->chunk > 4096 || (cfg->chunk & 63))
         return vf::fail(ctx, VF_E_INVALID, "%s: chunk %d must be a multiple of 64 in [64, 4096]", what, cfg->chunk);
+    if (cfg->refill < 0 || cfg->refill > 64)
+        return vf::fail(ctx, VF_E_INVALID, "%s: refill %d must be in [0, 64]", what, cfg->refill);
     return VF_OK;
 }
 
 vf_queue_cfg default_cfg() {
     vf_queue_cfg c;
-    c.tile_w = 8; c.tile_h = 8; c.chunk = 256; c.sort = 0; c.groups_per_cu = 0;
+    c.tile_w = 8; c.tile_h = 8; c.chunk = 256; c.sort = 1; c.groups_per_cu = 0; c.refill = 32;
     return c;
 }
 
@@ -524,30 +644,36 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     const float *s = ctx->S.sun_dir;
     unsigned long long *ctr = ctx->q_ctr;
     if ((rc = vf::begin_timed(ctx))) return rc;
-    hipLaunchKernelGGL(k_queue_reset, dim3((kBuckets + 255) / 256), dim3(256), 0, ctx->stream, ctr, ctx->q_buckets,
-                       kBuckets);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(256), 0, ctx->stream, ctr, nullptr, 0);
     hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen)), dim3(256), 0, ctx->stream, A,
-                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctx->q_keys, ctx->q_buckets, ctr,
-                       gi_hits, sun_hits);
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctr, gi_hits, sun_hits);
     const float4 *list = ctx->q_rays;
-    if (cfg.sort) {
-        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->q_buckets);
-        hipLaunchKernelGGL(k_bucket_scatter, dim3(vf::blocks_for(n)), dim3(256), 0, ctx->stream, ctx->q_rays, ctx->q_keys,
-                           ctx->q_buckets, ctx->q_rays2, ctr);
-        list = ctx->q_rays2;
-    }
-    if (ctx->counting) {
-        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_list_persist<true, true>), &cfg);
-        hipLaunchKernelGGL((k_list_persist<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1, gi_hits,
-                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
-        hipLaunchKernelGGL((k_list_persist<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2, ctr + 3,
-                           sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
-    } else {
-        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_list_persist<true, false>), &cfg);
-        hipLaunchKernelGGL((k_list_persist<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1, gi_hits,
-                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
-        hipLaunchKernelGGL((k_list_persist<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2, ctr + 3,
-                           sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, ctx->d_cnt);
+    {
+        const void *kp = cfg.refill == 0 ? reinterpret_cast<const void *>(k_list_chunked<true, false>)
+                                         : reinterpret_cast<const void *>(k_list_persist<true, false>);
+        const int g = persistent_groups(ctx, kp, &cfg);
+        unsigned long long *C0 = ctx->d_cnt;
+        if (cfg.refill == 0 && ctx->counting) {
+            hipLaunchKernelGGL((k_list_chunked<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
+                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, C0);
+            hipLaunchKernelGGL((k_list_chunked<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
+                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, C0);
+        } else if (cfg.refill == 0) {
+            hipLaunchKernelGGL((k_list_chunked<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
+                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, C0);
+            hipLaunchKernelGGL((k_list_chunked<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
+                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, C0);
+        } else if (ctx->counting) {
+            hipLaunchKernelGGL((k_list_persist<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
+                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+            hipLaunchKernelGGL((k_list_persist<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
+                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+        } else {
+            hipLaunchKernelGGL((k_list_persist<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
+                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+            hipLaunchKernelGGL((k_list_persist<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
+                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+        }
     }
     if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
     return vf::end_timed(ctx);
@@ -607,14 +733,26 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
     if ((rc = reserve_queue(ctx, 1))) return rc;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctx->q_ctr, ctx->q_stamps);
-    if (ctx->counting) {
-        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_batch_persist<true>), &cfg);
-        hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                           cfg.chunk, ctx->d_cnt);
+    if (cfg.refill == 0) {
+        const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_chunked<true>)
+                                       : reinterpret_cast<const void *>(k_batch_chunked<false>);
+        const int g = persistent_groups(ctx, kp, &cfg);
+        if (ctx->counting)
+            hipLaunchKernelGGL(k_batch_chunked<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                               cfg.chunk, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL(k_batch_chunked<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                               cfg.chunk, ctx->d_cnt);
     } else {
-        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(k_batch_persist<false>), &cfg);
-        hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                           cfg.chunk, ctx->d_cnt);
+        const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_persist<true>)
+                                       : reinterpret_cast<const void *>(k_batch_persist<false>);
+        const int g = persistent_groups(ctx, kp, &cfg);
+        if (ctx->counting)
+            hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                               cfg.chunk, cfg.refill, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
+                               cfg.chunk, cfg.refill, ctx->d_cnt);
     }
     if ((rc = vf::launch_check(ctx, "vf_trace_batch"))) return rc;
     return vf::end_timed(ctx);
@@ -632,24 +770,28 @@ int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms) {
     return VF_OK;
 }
 
-int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, vf_queue_cfg *best, vf_tune_result *results,
-                int32_t max_results, int32_t *n_results) {
+int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_cfg *base_cfg, vf_queue_cfg *best,
+                vf_tune_result *results, int32_t max_results, int32_t *n_results) {
     if (!ctx || !args || !best) return vf::fail(ctx, VF_E_INVALID, "vf_autotune: null argument");
     if (int rc = vf::no_capture(ctx, "vf_autotune")) return rc;
     static const int tiles[3][2] = {{8, 8}, {16, 8}, {16, 16}};
     static const int chunks[5] = {64, 128, 256, 512, 1024};
     const int sorts = workload == VF_TUNE_GI_BATCH ? 2 : 1;
+    static const int refills[2] = {0, 32};
     int k = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
     VF_HIP(ctx, hipEventCreate(&e0));
     VF_HIP(ctx, hipEventCreate(&e1));
     int rc = VF_OK;
+    for (int rf : refills)
     for (int so = 0; so < sorts && rc == VF_OK; ++so)
         for (const auto &t : tiles)
             for (int ch : chunks) {
                 vf_queue_cfg c = default_cfg();
                 c.tile_w = t[0]; c.tile_h = t[1]; c.chunk = ch; c.sort = so;
+                c.refill = rf;
+                if (base_cfg) c.groups_per_cu = base_cfg->groups_per_cu;
                 auto run = [&]() -> int {
                     if (workload == VF_TUNE_GI_BATCH) {
                         const vf_tune_gi_args *a = static_cast<const vf_tune_gi_args *>(args);

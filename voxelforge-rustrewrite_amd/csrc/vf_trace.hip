@@ -130,6 +130,74 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx
     flush_count<COUNT>(c, cnt);
 }
 
+// gbuffer.comp then shadow_trace.comp for the same pixel in one kernel: the sun ray starts from the
+// G-buffer values this lane has just written (position, RGBA8-quantised normal), exactly what the
+// separate shadow pass would read back, so the results are those of the two passes.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
+                                                        float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
+                                                        unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.w || py >= A.h) return;
+    const float sxx = ((float)px + 0.5f) + A.jit[0];
+    const float syy = ((float)py + 0.5f) + A.jit[1];
+    const float nx = (sxx / (float)A.w) * 2.0f - 1.0f;
+    const float ny = (syy / (float)A.h) * 2.0f - 1.0f;
+    const float aspect = (float)A.w / (float)A.h;
+    const float a = (nx * aspect) * A.thf;
+    const float b = ny * A.thf;
+    float rd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rd[i] = (A.fwd[i] + A.right[i] * a) + A.up[i] * b;
+    normalize3(rd);
+    Cnt c = {0, 0, 0, 0};
+    Hit h;
+    trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
+    h.t = h.t + 0.0f;
+    const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
+    store_hit(hits + i, h);
+    float4 p;
+    uint32_t nq = 0u;
+    if (h.hit) {
+        p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
+        float n[3];
+        face_normal(h.face, n);
+        nq = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
+             (255u << 24);
+    } else {
+        p = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    }
+    if (gbpos) gbpos[i] = p;
+    if (gbn) gbn[i] = nq;
+    // shadow_trace.comp:21-34, 87, 101-105 on (p, nq)
+    const float sun[3] = {sx, sy, sz};
+    Hit hs;
+    hs.hit = 0; hs.t = -1.0f; hs.face = 0; hs.mat = 0; hs.cell[0] = hs.cell[1] = hs.cell[2] = 0;
+    float vis, tb;
+    if (p.w < 0.0f) {
+        vis = 1.0f; tb = 0.0f;
+    } else {
+        const float N[3] = {unorm8_dec(nq & 255u), unorm8_dec((nq >> 8) & 255u), unorm8_dec((nq >> 16) & 255u)};
+        if (dot3(N, sun) <= 0.0f) {
+            vis = 0.0f; tb = 0.0f;
+        } else {
+            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+            trace_world<5, COUNT>(W, so, sun, 1e30f, hs, c);
+            vis = hs.hit ? 0.0f : 1.0f;
+            tb = hs.hit ? hs.t : 0.0f;
+        }
+    }
+    const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
+    const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
+    shadow[i] = hv | (ht << 16);
+    if (shits) store_hit(shits + i, hs);
+    flush_count<COUNT>(c, cnt);
+}
+
 // skyOpen with colLoaded (common.glsl:232-258); a bounded world has no horizon (outside -> 1).
 __device__ __forceinline__ float sky_open(const DevWorld &W, const float p[3]) {
     if (p[1] < 0.0f) return 0.0f;
@@ -1194,6 +1262,36 @@ int vf_trace_shadow(vf_ctx *ctx, int32_t w, int32_t h, const float *gb_pos, cons
         hipLaunchKernelGGL(k_shadow<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
                            reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
     if ((rc = launch_check(ctx, "k_shadow"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos,
+                            uint32_t *gb_normal, uint32_t *shadow, vf_hit *shadow_hits) {
+    int rc = check_ready(ctx, "vf_trace_primary_shadow");
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: no scene (vf_set_scene)");
+    if (!p || !hits || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: null pass, hits or shadow");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30))
+        return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: bad size %dx%d", w, h);
+    PrimArgs A;
+    for (int a = 0; a < 3; ++a) {
+        A.pos[a] = p->cam_pos[a]; A.right[a] = p->cam_right[a]; A.up[a] = p->cam_up[a]; A.fwd[a] = p->cam_fwd[a];
+    }
+    A.thf = p->tan_half_fov;
+    A.jit[0] = p->jitter[0];
+    A.jit[1] = p->jitter[1];
+    A.w = w; A.h = h;
+    A.tiles_x = (w + 15) / 16;
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    const float *s = ctx->S.sun_dir;
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_primary_shadow<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], hits,
+                           reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits, ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_primary_shadow<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], hits,
+                           reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits, ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_primary_shadow"))) return rc;
     return end_timed(ctx);
 }
 

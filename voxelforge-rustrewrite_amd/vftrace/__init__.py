@@ -58,6 +58,7 @@ _SIGS = {
     "vf_set_scene": (C.c_int, [_P, C.POINTER(SceneUbo)]),
     "vf_trace_primary": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P]),
     "vf_trace_shadow": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    "vf_trace_primary_shadow": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "vf_trace_gi": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
                               _P, _P]),
     "vf_trace_rays": (C.c_int, [_P, _P, C.c_int64, C.c_float, C.c_uint32, _P]),
@@ -68,8 +69,8 @@ _SIGS = {
                                     C.POINTER(QueueCfg)]),
     "vf_trace_batch": (C.c_int, [_P, C.POINTER(Request), C.c_int32, C.POINTER(QueueCfg)]),
     "vf_read_batch_times": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
-    "vf_autotune": (C.c_int, [_P, C.c_int32, _P, C.POINTER(QueueCfg), C.POINTER(TuneResult), C.c_int32,
-                              C.POINTER(C.c_int32)]),
+    "vf_autotune": (C.c_int, [_P, C.c_int32, _P, C.POINTER(QueueCfg), C.POINTER(QueueCfg), C.POINTER(TuneResult),
+                              C.c_int32, C.POINTER(C.c_int32)]),
     "vf_graph_begin": (C.c_int, [_P]),
     "vf_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
     "vf_graph_launch": (C.c_int, [_P, _P]),
@@ -359,6 +360,10 @@ class Context:
         self._ck(lib().vf_trace_shadow(self._h, int(width), int(height), _ptr(gb_pos), _ptr(gb_normal),
                                        _ptr(shadow_raw), _ptr(hits)))
 
+    def trace_primary_shadow(self, pass_ubo, width, height, hits, gb_pos, gb_normal, shadow_raw, shadow_hits=None):
+        self._ck(lib().vf_trace_primary_shadow(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
+                                               _ptr(gb_pos), _ptr(gb_normal), _ptr(shadow_raw), _ptr(shadow_hits)))
+
     def trace_gi(self, pass_ubo, width, height, gb_pos, gb_normal, radiance, bounce=None, half_res=True,
                  gi_reach=48.0, lights_nee=True):
         self._ck(lib().vf_trace_gi(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(half_res)),
@@ -375,10 +380,10 @@ class Context:
 
     # persistent-queue batches
     @staticmethod
-    def queue_cfg(tile=(8, 8), chunk=256, sort=False, groups_per_cu=0):
+    def queue_cfg(tile=(8, 8), chunk=256, sort=True, groups_per_cu=0, refill=32):
         c = QueueCfg()
         c.tile_w, c.tile_h = int(tile[0]), int(tile[1])
-        c.chunk, c.sort, c.groups_per_cu = int(chunk), int(bool(sort)), int(groups_per_cu)
+        c.chunk, c.sort, c.groups_per_cu, c.refill = int(chunk), int(bool(sort)), int(groups_per_cu), int(refill)
         return c
 
     def trace_gi_batch(self, pass_ubo, width, height, reach, gb_pos, gb_normal, gi_hits, sun_hits, cfg=None):
@@ -435,12 +440,12 @@ class Context:
         self._keep = arr
         return self._autotune(VF_TUNE_BATCH, a)
 
-    def _autotune(self, workload, args):
+    def _autotune(self, workload, args, base=None):
         best = QueueCfg()
         res = (TuneResult * 64)()
         n = C.c_int32()
-        self._ck(lib().vf_autotune(self._h, workload, C.cast(C.pointer(args), C.c_void_p), C.byref(best), res, 64,
-                                   C.byref(n)))
+        self._ck(lib().vf_autotune(self._h, workload, C.cast(C.pointer(args), C.c_void_p),
+                                   C.byref(base) if base is not None else None, C.byref(best), res, 64, C.byref(n)))
         table = [((r.cfg.tile_w, r.cfg.tile_h), r.cfg.chunk, bool(r.cfg.sort), r.ms) for r in res[:n.value]]
         return best, table
 
