@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="minimum CPU-baseline time (repeats the frame)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the fixed-world, GI-batch and uneven-batch lines")
     ap.add_argument("--no-graph", action="store_true", help="launch the passes eagerly instead of replaying a graph")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (calibrates rocprof FETCH/WRITE_SIZE)")
@@ -159,7 +160,9 @@ def main():
         else:
             ctx.graph_launch(g)
 
-    g_step = record(lambda: (ctx.trace_primary(pu, W, H, hits, gbp, gbn), ctx.trace_shadow(W, H, gbp, gbn, sh)))
+    # the step: primary and sun rays fused in one kernel (same outputs as the two passes)
+    g_step = record(lambda: ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh))
+    g_two = record(lambda: (ctx.trace_primary(pu, W, H, hits, gbp, gbn), ctx.trace_shadow(W, H, gbp, gbn, sh)))
     g_prim = record(lambda: ctx.trace_primary(pu, W, H, hits, gbp, gbn))
     g_shad = record(lambda: ctx.trace_shadow(W, H, gbp, gbn, sh))
 
@@ -180,17 +183,26 @@ def main():
     wall, step_ms_ev = timed_loop(g_step, args.steps)
     ms_per_step = wall * 1e3 / args.steps
     value = rays_per_step * args.steps / wall / 1e9
+    _, two_ms = timed_loop(g_two, args.steps)
     _, prim_ms1 = timed_loop(g_prim, args.steps)
     _, shad_ms1 = timed_loop(g_shad, args.steps)
     prim_ms = np.array([prim_ms1])
     shad_ms = np.array([shad_ms1])
-    log(f"step {ms_per_step:.4f} ms wall ({step_ms_ev:.4f} ms events); k_primary {prim_ms1:.4f} ms, "
-        f"k_shadow {shad_ms1:.4f} ms")
+    log(f"step (fused) {ms_per_step:.4f} ms wall ({step_ms_ev:.4f} ms events); two passes {two_ms:.4f} ms; "
+        f"k_primary {prim_ms1:.4f} ms, k_shadow {shad_ms1:.4f} ms")
 
-    prim_gbs = prim_bytes / (prim_ms.mean() * 1e-3) / 1e9
-    shad_gbs = shad_bytes / (shad_ms.mean() * 1e-3) / 1e9
-    dominant = "k_primary" if prim_ms.mean() >= shad_ms.mean() else "k_shadow"
-    ach = prim_gbs if dominant == "k_primary" else shad_gbs
+    # the dominant (only) kernel of the step: k_primary_shadow; its algorithmic bytes = the fetches of
+    # both rays + the primary outputs (hit, gbPos, gbNormal) + shadowRaw (no G-buffer read-back).
+    # Its launch duration: 20 launches recorded back to back in one graph, timed with HIP events on
+    # the launch stream (per-replay timing would add the graph-launch gap to every kernel).
+    fused_bytes = cp["bytes"] + cs["bytes"] + n * (HIT_BYTES + GBPOS_BYTES + GBN_BYTES + SHADOW_BYTES)
+    dominant = "k_primary_shadow"
+    reps = 20
+    g_rep = record(lambda: [ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh) for _ in range(reps)])
+    _, rep_ms = timed_loop(g_rep, max(1, args.steps // reps))
+    kernel_ms = rep_ms / reps
+    g_rep.close()
+    ach = fused_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_endless_1080p.json")
     if os.path.exists(tf):
@@ -222,8 +234,10 @@ def main():
         },
         "primary_grays": round(cp["rays"] / (prim_ms.mean() * 1e-3) / 1e9, 4),
         "sun_grays": round(cs["rays"] / (shad_ms.mean() * 1e-3) / 1e9, 4),
-        "kernel_ms": {"k_primary": round(float(prim_ms.mean()), 5), "k_shadow": round(float(shad_ms.mean()), 5)},
-        "algorithmic_bytes_per_launch": {"k_primary": int(prim_bytes), "k_shadow": int(shad_bytes)},
+        "kernel_ms": {"k_primary_shadow": round(float(kernel_ms), 5), "step_replay": round(float(step_ms_ev), 5), "k_primary": round(float(prim_ms.mean()), 5),
+                      "k_shadow": round(float(shad_ms.mean()), 5), "two_pass_step": round(float(two_ms), 5)},
+        "algorithmic_bytes_per_launch": {"k_primary_shadow": int(fused_bytes), "k_primary": int(prim_bytes),
+                                         "k_shadow": int(shad_bytes)},
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
@@ -238,10 +252,124 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(world, win, arrays_wb, cam_pos, right, up, fwd, pu.tan_half_fov, scene,
                                               hits, args.cpu_seconds)
-    for g in (g_step, g_prim, g_shad):
+    for g in (g_step, g_two, g_prim, g_shad):
         g.close()
+    if not args.no_extra:
+        result["other_configs"] = other_configs(ctx, args)
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def graph_ms(ctx, stream, fn, k):
+    """Average time of one replay of fn's recorded graph over k back-to-back replays (HIP events)."""
+    ctx.graph_begin()
+    fn()
+    g = ctx.graph_end()
+    for _ in range(3):
+        ctx.graph_launch(g)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        ctx.graph_launch(g)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    g.close()
+    return e0.elapsed_time(e1) / k
+
+
+def other_configs(ctx, args):
+    """The fixed world (256x128x256 voxels): primary + sun rays at 640x360, the GI batch at 1920x1080
+    (1 GI ray per pixel, reach 640 voxels, 1 sun ray per GI hit; autotuned queue), and one launch of a
+    32x16x32 DDGI probe grid (256 rays each) plus 16 views of mixed sizes."""
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
+    out = {}
+    world = vf.GenWorld.generate(1, seed=args.seed)
+    ctx.upload(world)
+    scene = vf.default_scene((32, 16, 32))
+    ctx.set_scene(scene)
+    cam_pos, right, up, fwd = vf.fly_camera([37.3, 96.4, 41.7], 0.8, -0.35)
+    # fixed world, 640x360
+    w1, h1 = 640, 360
+    n1 = w1 * h1
+    pu = vf.make_pass(cam_pos, right, up, fwd, 70.0, 0)
+    hits = torch.empty(n1 * HIT_BYTES, dtype=torch.uint8, device=dev)
+    gbp = torch.empty(n1 * 4, dtype=torch.float32, device=dev)
+    gbn = torch.empty(n1, dtype=torch.int32, device=dev)
+    sh = torch.empty(n1, dtype=torch.int32, device=dev)
+    ctx.set_counting(True)
+    ctx.counters(reset=True)
+    ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh)
+    c1 = ctx.counters(reset=True)
+    ctx.set_counting(False)
+    ms = graph_ms(ctx, stream, lambda: ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh), args.steps)
+    out["fixed_world_640x360"] = {"rays": int(c1["rays"]), "ms": round(ms, 5),
+                                  "grays": round(c1["rays"] / (ms * 1e-3) / 1e9, 4)}
+    # GI batch at 1080p
+    w3, h3 = 1920, 1080
+    n3 = w3 * h3
+    pu3 = vf.make_pass(cam_pos, right, up, fwd, 70.0, 4)
+    hits3 = torch.empty(n3 * HIT_BYTES, dtype=torch.uint8, device=dev)
+    gbp3 = torch.empty(n3 * 4, dtype=torch.float32, device=dev)
+    gbn3 = torch.empty(n3, dtype=torch.int32, device=dev)
+    ctx.trace_primary(pu3, w3, h3, hits3, gbp3, gbn3)
+    gi = torch.empty(n3 * HIT_BYTES, dtype=torch.uint8, device=dev)
+    sun = torch.empty(n3 * HIT_BYTES, dtype=torch.uint8, device=dev)
+    best, table = ctx.autotune_gi_batch(pu3, w3, h3, 640.0, gbp3, gbn3, gi, sun)
+    ctx.set_counting(True)
+    ctx.counters(reset=True)
+    ctx.trace_gi_batch(pu3, w3, h3, 640.0, gbp3, gbn3, gi, sun, best)
+    c3 = ctx.counters(reset=True)
+    ctx.set_counting(False)
+    ms3 = graph_ms(ctx, stream, lambda: ctx.trace_gi_batch(pu3, w3, h3, 640.0, gbp3, gbn3, gi, sun, best), 50)
+    rad = torch.empty(n3 * 4, dtype=torch.float32, device=dev)
+    ms3p = graph_ms(ctx, stream, lambda: ctx.trace_gi(pu3, w3, h3, gbp3, gbn3, rad, gi, half_res=False, gi_reach=640.0), 20)
+    out["gi_batch_1080p"] = {"rays": int(c3["rays"]), "ms": round(ms3, 5), "grays": round(c3["rays"] / (ms3 * 1e-3) / 1e9, 4),
+                             "queue": {"tile": [best.tile_w, best.tile_h], "chunk": best.chunk, "sort": best.sort,
+                                       "refill": best.refill},
+                             "autotune_candidates": len(table),
+                             "plain_gi_pass_ms": round(ms3p, 5)}
+    # uneven batch: probes + 16 views, one launch
+    rng = np.random.default_rng(args.seed)
+    sizes = [(256, 256), (512, 288), (960, 540), (1024, 1024)]
+    views = []
+    for k in range(16):
+        vw, vh = sizes[int(rng.integers(0, 4))]
+        pos = [float(rng.uniform(20, 236)), float(rng.uniform(75, 120)), float(rng.uniform(20, 236))]
+        vp, vr, vu, vfw = vf.fly_camera(pos, float(rng.uniform(0, 2 * math.pi)), float(rng.uniform(-0.7, 0.1)))
+        views.append((vw, vh, vf.make_pass(vp, vr, vu, vfw, 70.0, 0)))
+    views.sort(key=lambda v: v[0] * v[1])        # smallest views first: they finish first
+    outs = [torch.empty(vw * vh * HIT_BYTES, dtype=torch.uint8, device=dev) for vw, vh, _ in views]
+    dims, nr = (32, 16, 32), 256
+    origin = (128.0 - 310.0 + 0.5, 64.0 - 150.0 + 0.5, 128.0 - 310.0 + 0.5)
+    probe_out = torch.empty(int(np.prod(dims)) * nr * HIT_BYTES, dtype=torch.uint8, device=dev)
+    reqs = [ctx.view_request(vp, vw, vh, o) for (vw, vh, vp), o in zip(views, outs)]
+    reqs.append(ctx.probe_request(origin, 20.0, dims, nr, 7, 256.0, probe_out))
+    bbest, btable = ctx.autotune_batch(reqs)
+    total_rays = sum(vw * vh for vw, vh, _ in views) + int(np.prod(dims)) * nr
+    ms5 = graph_ms(ctx, stream, lambda: ctx.trace_batch(reqs, bbest), 20)
+    ctx.trace_batch(reqs, bbest)
+    ctx.sync()
+    times = ctx.batch_times(len(reqs))
+    view_t = np.array(times[:16])
+
+    def plain():
+        for (vw, vh, vp), o in zip(views, outs):
+            ctx.trace_primary(vp, vw, vh, o)
+        ctx.trace_probes(origin, 20.0, dims, nr, 7, 256.0, probe_out)
+
+    ms5p = graph_ms(ctx, stream, plain, 20)
+    out["uneven_batch"] = {"rays": total_rays, "ms": round(ms5, 5), "grays": round(total_rays / (ms5 * 1e-3) / 1e9, 4),
+                           "view_latency_ms_p50": round(float(np.percentile(view_t, 50)), 5),
+                           "view_latency_ms_p99": round(float(np.percentile(view_t, 99)), 5),
+                           "probes_done_ms": round(float(times[16]), 5),
+                           "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill},
+                           "sequential_plain_launches_ms": round(ms5p, 5),
+                           "view_sizes": [[vw, vh] for vw, vh, _ in views]}
+    ctx.set_stream(None)
+    return out
 
 
 def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, min_seconds):
