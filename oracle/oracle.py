@@ -41,6 +41,16 @@ class VfoCam(C.Structure):
                 ("tan_half_fov", C.c_float), ("jitter", C.c_float * 2), ("frame_idx", C.c_uint32)]
 
 
+class VfoTail(C.Structure):
+    _fields_ = [("valid", C.c_int32), ("p", C.c_float * 3), ("n", C.c_float * 3), ("dir", C.c_float * 3),
+                ("thr", C.c_float * 3)]
+
+
+# 64-byte query / training record (vfo_record = the C ABI's vf_gi_record)
+RECORD_DTYPE = np.dtype([("pos", "<f4", (3,)), ("tag", "<u4"), ("n", "<f4", (3,)), ("pad0", "<f4"),
+                         ("view", "<f4", (3,)), ("pad1", "<f4"), ("rgb", "<f4", (3,)), ("pad2", "<f4")])
+
+
 class VfoSky(C.Structure):
     _fields_ = [("sun_dir", C.c_float * 3), ("sun_rgb", C.c_float * 3), ("sky_zenith", C.c_float * 3),
                 ("sky_horizon", C.c_float * 3), ("sunset_rgb", C.c_float * 3), ("sky_scale", C.c_float),
@@ -91,6 +101,14 @@ def lib():
             "vfo_gi_path": (None, [P, P, P, P, P, P, P, P]),
             "vfo_sky_open": (C.c_float, [P, P, P]),
             "vfo_gi_rays": (None, [C.c_int32, C.c_int32, C.c_uint32, P, P, P, P]),
+            "vfo_cosine_hemisphere": (None, [P, P, P]),
+            "vfo_gi_path_ex": (None, [P, P, P, P, P, P, C.c_int32, C.c_int32, P, P, P, P]),
+            "vfo_pass_gi_path": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32, P,
+                                        C.c_int]),
+            "vfo_gicache_prepass": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32, P, P,
+                                           C.c_int]),
+            "vfo_gicache_samples": (None, [P, P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_int32,
+                                           C.c_uint32, P, C.c_int]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -274,8 +292,67 @@ class OracleWorld:
         return rad, bounce, cnt.as_dict()
 
 
+    def gi_path(self, scene, P, N, d, seed, last_stage=3, trace_on=True, tail=False, gi_reach=48.0, lights_nee=True):
+        """One giPathImpl path; returns (L, seed after, tail dict or None)."""
+        s = self.sky(scene, gi_reach, lights_nee)
+        P32, N32, d32 = f32(P), f32(N), f32(d)
+        L = np.zeros(3, np.float32)
+        sd = C.c_uint32(int(seed) & 0xFFFFFFFF)
+        t = VfoTail()
+        lib().vfo_gi_path_ex(C.byref(self.s), C.byref(s), _p(P32), _p(N32), _p(d32), C.byref(sd), int(last_stage),
+                             int(bool(trace_on)), _p(L), None, C.byref(t) if tail else None, None)
+        tl = None
+        if tail:
+            tl = {"valid": bool(t.valid), "p": np.array(t.p[:], np.float32), "n": np.array(t.n[:], np.float32),
+                  "dir": np.array(t.dir[:], np.float32), "thr": np.array(t.thr[:], np.float32)}
+        return L, sd.value, tl
+
+    def pass_gi_path(self, scene, width, height, gb_pos, gb_normal, frame, spp, salt, gi_reach=48.0, lights_nee=True,
+                     threads=0):
+        """Plain traced two-bounce GI: RGBA per pixel (mean of spp paths)."""
+        s = self.sky(scene, gi_reach, lights_nee)
+        out = np.zeros((width * height, 4), np.float32)
+        pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_pass_gi_path(C.byref(self.s), C.byref(s), int(width), int(height), _p(pos32), _p(nrm),
+                               int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, _p(out), int(threads))
+        return out
+
+    def gicache_prepass(self, scene, width, height, gb_pos, gb_normal, frame, spp, salt, gi_reach=48.0,
+                        lights_nee=True, threads=0):
+        """The cached render's traced part: (pre RGBA per pixel, query record per pixel*spp slot)."""
+        s = self.sky(scene, gi_reach, lights_nee)
+        pre = np.zeros((width * height, 4), np.float32)
+        q = np.zeros(width * height * spp, RECORD_DTYPE)
+        pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_gicache_prepass(C.byref(self.s), C.byref(s), int(width), int(height), _p(pos32), _p(nrm),
+                                  int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, _p(pre), _p(q),
+                                  int(threads))
+        return pre, q
+
+    def gicache_samples(self, scene, cam_pos, width, height, gb_pos, gb_normal, frame, n, terminate, salt,
+                        gi_reach=48.0, lights_nee=True, threads=0):
+        """The training records of one frame."""
+        s = self.sky(scene, gi_reach, lights_nee)
+        out = np.zeros(n, RECORD_DTYPE)
+        cam = f32(cam_pos)
+        pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_gicache_samples(C.byref(self.s), C.byref(s), _p(cam), int(width), int(height), _p(pos32), _p(nrm),
+                                  int(frame) & 0xFFFFFFFF, int(n), int(terminate), int(salt) & 0xFFFFFFFF, _p(out),
+                                  int(threads))
+        return out
+
+
 def pcg(v):
     return int(lib().vfo_pcg(int(v) & 0xFFFFFFFF))
+
+
+def cosine_hemisphere(n, seed):
+    """cosineHemisphere(n, seed) (common.glsl:1271-1274): (direction, seed after)."""
+    n32 = f32(n)
+    out = np.zeros(3, np.float32)
+    sd = C.c_uint32(int(seed) & 0xFFFFFFFF)
+    lib().vfo_cosine_hemisphere(_p(n32), C.byref(sd), _p(out))
+    return out, sd.value
 
 
 def brick_mask(vox):

@@ -832,68 +832,237 @@ static void sky_color0(const vfo_sky *s, const float rd[3], float out[3]) {
     }
 }
 
-/* giPathImpl with lastStage 1 and traceOn (common.glsl:2250-2366); bodies, the radiance cache and
- * NEE light sampling are not part of this pass. */
-void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
-                 float L[3], vfo_hit *bounce, vfo_count *cnt) {
-    float thr[3] = {1.0f, 1.0f, 1.0f}, ro[3], rdir[3], hitAlb[3] = {0, 0, 0};
+/* rnd (common.glsl:1246-1251) and cosineHemisphere (common.glsl:1271-1274). */
+static float rnd(uint32_t *state) {
+    *state = vfo_pcg(*state);
+    return (float)(*state >> 8u) * (1.0f / 16777216.0f);
+}
+void vfo_cosine_hemisphere(const float n[3], uint32_t *state, float out[3]) {
+    float u[2];
+    u[0] = rnd(state);
+    u[1] = rnd(state);
+    vfo_cosine_hemisphere_u(n, u, out);
+}
+
+/* giPathImpl (common.glsl:2250-2366) without bodies, NEE lamps or the hash cache, for lastStage 1 (one
+ * bounce) or 3 (two bounces); traceOn = the rising-ray trace-on past the reach (common.glsl:2309-2320).
+ * seed: the path's RNG state (bounce 2's direction, cosineHemisphere(hitN, seed)).  With tail non-null
+ * the path stops where bounce 2 would start (after `thr *= hitAlb * giScale`, common.glsl:2277 and
+ * :2339) and reports that point; L then holds the light gathered before it. */
+void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                    uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
+                    vfo_count *cnt) {
+    float thr[3] = {1.0f, 1.0f, 1.0f}, ro[3], rdir[3], hitAlb[3] = {0, 0, 0}, hitP[3] = {0, 0, 0};
+    float hitN[3] = {0.0f, 1.0f, 0.0f}, hitD[3] = {0, 0, 0};
     float hitNdl = 0.0f, reach = s->gi_reach;
     int stage = 0, extended = 0;
     L[0] = L[1] = L[2] = 0.0f;
     if (bounce) { memset(bounce, 0, sizeof *bounce); bounce->t = -1.0f; }
+    if (tail) memset(tail, 0, sizeof *tail);
     for (int a = 0; a < 3; ++a) { ro[a] = P[a] + N[a] * 3e-2f; rdir[a] = dir[a]; }
     for (int it = 0; it < 6; ++it) {
         int sunRay = (stage & 1) == 1;
         vfo_hit h;
         vfo_trace(w, ro, rdir, sunRay ? 1e30f : reach, 0u, &h, cnt);
+        int next_bounce = 0;
         if (sunRay) {
             if (!h.hit)
                 for (int a = 0; a < 3; ++a) L[a] += thr[a] * hitAlb[a] * s->sun_rgb[a] * hitNdl;
-            break;   /* stage >= lastStage (1) */
-        }
-        if (stage == 0 && bounce) *bounce = h;
-        if (h.hit && h.mat == 5u) h.hit = 0;   /* WATER_CLEAR: water ends a bounce ray as a miss */
-        if (!h.hit) {
-            float esc = gi_escape(w, s, ro, rdir, reach);
-            if (esc < 0.999f && !extended && stage == 0 && rdir[1] > 0.3f) {
-                float e[3], p1[3], p2[3], p3[3];
-                for (int a = 0; a < 3; ++a) {
-                    e[a] = ro[a] + rdir[a] * reach;
-                    p1[a] = e[a] + rdir[a] * 32.0f;
-                    p2[a] = e[a] + rdir[a] * 64.0f;
-                    p3[a] = e[a] + rdir[a] * 96.0f;
+            if (stage >= last_stage) break;
+            next_bounce = 1;
+        } else {
+            if (stage == 0 && bounce) *bounce = h;
+            if (h.hit && h.mat == 5u) h.hit = 0;   /* WATER_CLEAR: water ends a bounce ray as a miss */
+            if (!h.hit) {
+                float esc = gi_escape(w, s, ro, rdir, reach);
+                if (trace_on && esc < 0.999f && !extended && stage == 0 && rdir[1] > 0.3f) {
+                    float e[3], p1[3], p2[3], p3[3];
+                    for (int a = 0; a < 3; ++a) {
+                        e[a] = ro[a] + rdir[a] * reach;
+                        p1[a] = e[a] + rdir[a] * 32.0f;
+                        p2[a] = e[a] + rdir[a] * 64.0f;
+                        p3[a] = e[a] + rdir[a] * 96.0f;
+                    }
+                    float m = gmax(vfo_sky_open(w, s, p1), gmax(vfo_sky_open(w, s, p2), vfo_sky_open(w, s, p3)));
+                    if (m > 0.0f) {
+                        for (int a = 0; a < 3; ++a) ro[a] = e[a];
+                        reach = 96.0f;
+                        extended = 1;
+                        continue;
+                    }
                 }
-                float m = gmax(vfo_sky_open(w, s, p1), gmax(vfo_sky_open(w, s, p2), vfo_sky_open(w, s, p3)));
-                if (m > 0.0f) {
-                    for (int a = 0; a < 3; ++a) ro[a] = e[a];
-                    reach = 96.0f;
-                    extended = 1;
-                    continue;
+                float sky[3];
+                sky_color0(s, rdir, sky);
+                for (int a = 0; a < 3; ++a) L[a] += thr[a] * gmin(sky[a], 3.0f) * esc;
+                break;
+            }
+            const float *mat = s->palette + (size_t)h.mat * 12;
+            float emit = mat[3] * 8.0f;
+            if (s->lights_nee && mat[3] >= 0.5f) emit = 0.0f;
+            for (int a = 0; a < 3; ++a) {
+                hitN[a] = (float)h.n[a];
+                L[a] += thr[a] * mat[a] * emit;
+                hitP[a] = ro[a] + rdir[a] * h.t;
+                hitD[a] = rdir[a];
+                hitAlb[a] = mat[a];
+            }
+            reach = s->gi_reach;
+            extended = 0;
+            hitNdl = gmax(dot3(hitN, s->sun_dir), 0.0f);
+            if (hitNdl > 0.0f) {
+                for (int a = 0; a < 3; ++a) { ro[a] = hitP[a] + hitN[a] * 3e-2f; rdir[a] = s->sun_dir[a]; }
+                stage += 1;
+                continue;
+            }
+            if (stage + 1 >= last_stage) break;
+            next_bounce = 1;
+        }
+        if (next_bounce) {   /* bounce 2 from the first hit */
+            for (int a = 0; a < 3; ++a) thr[a] = thr[a] * (hitAlb[a] * s->gi_scale);
+            if (tail) {
+                tail->valid = 1;
+                for (int a = 0; a < 3; ++a) {
+                    tail->p[a] = hitP[a];
+                    tail->n[a] = hitN[a];
+                    tail->dir[a] = hitD[a];
+                    tail->thr[a] = thr[a];
+                }
+                break;
+            }
+            vfo_cosine_hemisphere(hitN, seed, rdir);
+            for (int a = 0; a < 3; ++a) ro[a] = hitP[a] + hitN[a] * 3e-2f;
+            stage = 2;
+            reach = s->gi_reach;
+            extended = 0;
+        }
+    }
+}
+
+void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                 float L[3], vfo_hit *bounce, vfo_count *cnt) {
+    uint32_t seed = 0u;
+    vfo_gi_path_ex(w, s, P, N, dir, &seed, 1, 1, L, bounce, NULL, cnt);
+}
+
+static void gbuffer_point(const float *gb_pos, const uint32_t *gb_normal, size_t i, float P[3], float N[3]) {
+    uint32_t nb = gb_normal[i];
+    for (int a = 0; a < 3; ++a) P[a] = gb_pos[i * 4 + a];
+    N[0] = unorm8_dec(nb & 255u);
+    N[1] = unorm8_dec((nb >> 8) & 255u);
+    N[2] = unorm8_dec((nb >> 16) & 255u);
+}
+
+/* Bounce-1 direction and path seed of sample index k of pixel (px, py): gi.comp:62-72 with the
+ * sample index in place of the frame. */
+static void gi_sample_start(const float N[3], int32_t px, int32_t py, uint32_t k, uint32_t salt, float dir[3],
+                            uint32_t *seed) {
+    float u[2];
+    vfo_ld_sample2(px, py, k, salt, u);
+    vfo_cosine_hemisphere_u(N, u, dir);
+    *seed = vfo_pixel_seed(px, py, k, salt);
+}
+
+void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H, const float *gb_pos,
+                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *out, int threads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
+    for (int64_t i = 0; i < (int64_t)W * H; ++i) {
+        int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
+        float sum[3] = {0, 0, 0};
+        int sky = gb_pos[i * 4 + 3] < 0.0f;
+        if (!sky) {
+            float P[3], N[3];
+            gbuffer_point(gb_pos, gb_normal, (size_t)i, P, N);
+            for (int32_t sp = 0; sp < spp; ++sp) {
+                float dir[3], L[3];
+                uint32_t seed;
+                gi_sample_start(N, px, py, frame * (uint32_t)spp + (uint32_t)sp, salt, dir, &seed);
+                vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
+                for (int a = 0; a < 3; ++a) sum[a] += L[a];
+            }
+        }
+        for (int a = 0; a < 3; ++a) out[i * 4 + a] = sky ? 0.0f : sum[a] / (float)spp;
+        out[i * 4 + 3] = sky ? 0.0f : 1.0f;
+    }
+}
+
+/* The cache render's traced part with the tail at the bounce-1 hit: per pixel pre = the light gathered
+ * before the tail summed over the spp samples (a = 1, or 0 for sky), and per slot pixel*spp + s the
+ * query record (tag = slot, or 0xFFFFFFFF when the sample has no tail). */
+void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H, const float *gb_pos,
+                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *pre,
+                         vfo_record *queries, int threads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
+    for (int64_t i = 0; i < (int64_t)W * H; ++i) {
+        int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
+        float sum[3] = {0, 0, 0};
+        int sky = gb_pos[i * 4 + 3] < 0.0f;
+        float P[3] = {0, 0, 0}, N[3] = {0, 0, 0};
+        if (!sky) gbuffer_point(gb_pos, gb_normal, (size_t)i, P, N);
+        for (int32_t sp = 0; sp < spp; ++sp) {
+            vfo_record *q = &queries[i * spp + sp];
+            memset(q, 0, sizeof *q);
+            q->tag = 0xFFFFFFFFu;
+            if (sky) continue;
+            float dir[3], L[3];
+            uint32_t seed;
+            vfo_tail t;
+            gi_sample_start(N, px, py, frame * (uint32_t)spp + (uint32_t)sp, salt, dir, &seed);
+            vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, &t, NULL);
+            for (int a = 0; a < 3; ++a) sum[a] += L[a];
+            if (t.valid) {
+                q->tag = (uint32_t)(i * spp + sp);
+                for (int a = 0; a < 3; ++a) {
+                    q->pos[a] = t.p[a];
+                    q->n[a] = t.n[a];
+                    q->view[a] = -t.dir[a];
+                    q->rgb[a] = t.thr[a];
                 }
             }
-            float sky[3];
-            sky_color0(s, rdir, sky);
-            for (int a = 0; a < 3; ++a) L[a] += thr[a] * gmin(sky[a], 3.0f) * esc;
-            break;
         }
-        const float *mat = s->palette + (size_t)h.mat * 12;
-        float emit = mat[3] * 8.0f;
-        if (s->lights_nee && mat[3] >= 0.5f) emit = 0.0f;
-        float hitP[3], hitN[3] = {(float)h.n[0], (float)h.n[1], (float)h.n[2]};
-        for (int a = 0; a < 3; ++a) {
-            L[a] += thr[a] * mat[a] * emit;
-            hitP[a] = ro[a] + rdir[a] * h.t;
-            hitAlb[a] = mat[a];
+        for (int a = 0; a < 3; ++a) pre[i * 4 + a] = sum[a];
+        pre[i * 4 + 3] = sky ? 0.0f : 1.0f;
+    }
+}
+
+/* Training samples of one frame: sample i takes pixel floor((i + r) * W*H / n) (r = rnd of its own stream
+ * pcg(i + pcg(frame ^ salt))), a cosine-hemisphere bounce-1 direction from the same stream and the
+ * two-bounce path.  terminate 0: the record is the G-buffer point (view = towards the camera) with the
+ * path's radiance; terminate 1: the bounce-1 hit (view = back along bounce 1) with the radiance of the
+ * path's remainder from there (giPathImpl(hit, n, cosineHemisphere(n, seed), seed, 1, false): bounce 2 and
+ * its sun ray).  tag = 0 for a sky pixel or a sample without that hit. */
+void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_pos[3], int32_t W, int32_t H,
+                         const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t n,
+                         int32_t terminate, uint32_t salt, vfo_record *out, int threads) {
+    const float stratum = (float)((int64_t)W * H) / (float)n;
+    const uint32_t base = vfo_pcg(frame ^ salt);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
+    for (int32_t i = 0; i < n; ++i) {
+        vfo_record *r = &out[i];
+        memset(r, 0, sizeof *r);
+        uint32_t seed = vfo_pcg((uint32_t)i + base);
+        float u = rnd(&seed);
+        int64_t pix = (int64_t)floorf(((float)i + u) * stratum);
+        if (pix > (int64_t)W * H - 1) pix = (int64_t)W * H - 1;
+        if (gb_pos[pix * 4 + 3] < 0.0f) continue;
+        float P[3], N[3], dir[3], L[3];
+        gbuffer_point(gb_pos, gb_normal, (size_t)pix, P, N);
+        vfo_cosine_hemisphere(N, &seed, dir);
+        if (terminate == 0) {
+            vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
+            float v[3] = {cam_pos[0] - P[0], cam_pos[1] - P[1], cam_pos[2] - P[2]};
+            normalize3(v);
+            r->tag = 1u;
+            for (int a = 0; a < 3; ++a) { r->pos[a] = P[a]; r->n[a] = N[a]; r->view[a] = v[a]; r->rgb[a] = L[a]; }
+            continue;
         }
-        reach = s->gi_reach;
-        extended = 0;
-        hitNdl = gmax(dot3(hitN, s->sun_dir), 0.0f);
-        if (hitNdl > 0.0f) {
-            for (int a = 0; a < 3; ++a) { ro[a] = hitP[a] + hitN[a] * 3e-2f; rdir[a] = s->sun_dir[a]; }
-            stage += 1;
-        } else {
-            break;   /* stage + 1 >= lastStage */
-        }
+        vfo_tail t;
+        vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, &t, NULL);
+        if (!t.valid) continue;
+        float d2[3], L2[3];
+        vfo_cosine_hemisphere(t.n, &seed, d2);
+        vfo_gi_path_ex(w, s, t.p, t.n, d2, &seed, 1, 0, L2, NULL, NULL, NULL);
+        r->tag = 1u;
+        for (int a = 0; a < 3; ++a) { r->pos[a] = t.p[a]; r->n[a] = t.n[a]; r->view[a] = -t.dir[a]; r->rgb[a] = L2[a]; }
     }
 }
 

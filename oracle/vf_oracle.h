@@ -14,7 +14,7 @@
  * matters for signed zeros.
  *
  * Pinning (see the tests under tests/):
- *   - traversal: brute-force FP32 voxel DDA without skips (vfo_trace_noskip) and an
+ *   - traversal: the plain brick-by-brick DDA (vfo_world.use_dist = 0, coarse_skip = 0) and an
  *     independent FP64 plain voxel march (vfo_march_f64); analytic scenes; the reference's
  *     raycast_hits_floor case (sparse_volume.rs:669-679) through vfo_raycast_cpu.
  *   - brick masks: the reference's mask_bits vector (sparse_volume.rs:660-668).
@@ -23,6 +23,8 @@
  *   - RNG: pcgHash is a bijection check plus the published RXS-M-XS structure; the exact
  *     values are "parity unpinned" beyond GPU-vs-oracle agreement (no reference vectors).
  *   - GI radiance: estimator structure only (statistical), "parity unpinned" bitwise.
+ *   - two-bounce paths: a path stopped at its tail and continued from there by hand equals the whole
+ *     path; one-bounce = the pass the GI kernels were first pinned with.
  */
 #ifndef VF_ORACLE_H
 #define VF_ORACLE_H
@@ -147,6 +149,29 @@ void vfo_pass_gi(const vfo_world *w, const vfo_sky *s, uint32_t frame_idx, int32
 /* GI path from an explicit point/normal/direction (giPathImpl with lastStage 1). */
 void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
                  float L[3], vfo_hit *bounce, vfo_count *cnt);
+/* The point where a two-bounce path's bounce 2 would start: hit point, its normal, the bounce-1 ray's
+ * direction and the throughput after `thr *= hitAlb * giScale`. */
+typedef struct vfo_tail { int32_t valid; float p[3], n[3], dir[3], thr[3]; } vfo_tail;
+/* Query / training record, the same 64 B as the C-ABI's vf_gi_record. */
+typedef struct vfo_record {
+    float pos[3]; uint32_t tag; float n[3]; float pad0; float view[3]; float pad1; float rgb[3]; float pad2;
+} vfo_record;
+void vfo_cosine_hemisphere(const float n[3], uint32_t *state, float out[3]);
+/* giPathImpl (common.glsl:2250-2366) with lastStage 1 or 3, traceOn, and an optional stop at the tail. */
+void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                    uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
+                    vfo_count *cnt);
+/* Plain traced two-bounce GI with spp samples per pixel (RGBA32F out; a = 0 for sky). */
+void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t width, int32_t height, const float *gb_pos,
+                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *out, int threads);
+/* Cached render, traced part (tail at the bounce-1 hit). */
+void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t width, int32_t height, const float *gb_pos,
+                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *pre,
+                         vfo_record *queries, int threads);
+/* Training samples of one frame (terminate 0: at the G-buffer point; 1: at the bounce-1 hit). */
+void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_pos[3], int32_t width, int32_t height,
+                         const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t n,
+                         int32_t terminate, uint32_t salt, vfo_record *out, int threads);
 float vfo_sky_open(const vfo_world *w, const vfo_sky *s, const float p[3]);
 /* GI rays of a full-resolution G-buffer (the GI batch: one per non-sky pixel). */
 void vfo_gi_rays(int32_t width, int32_t height, uint32_t frame, const float *gb_pos, const uint32_t *gb_normal,

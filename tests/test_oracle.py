@@ -309,3 +309,56 @@ def test_gi_open_sky_and_water_semantics():
                         L.ctypes.data, None, None)
     # water is a miss: escape downward inside the world -> giEscape ends at the water surface column
     assert np.all(np.isfinite(L))
+
+
+def _gbuffer_points(n):
+    """Non-sky G-buffer points (position, quantised normal) of the fixed world's test camera."""
+    _, a, ow = world(1)
+    _, cam = config1_camera()
+    hits, pos, nrm, _ = ow.pass_primary(cam, 96, 54)
+    idx = np.nonzero(pos[:, 3] >= 0.0)[0][:n]
+    N = np.stack([((nrm[idx] >> (8 * k)) & 255).astype(np.float32) / np.float32(255.0) * np.float32(2.0)
+                  - np.float32(1.0) for k in range(3)], 1).astype(np.float32)
+    return ow, a, pos[idx, :3].copy(), N
+
+
+def test_two_bounce_path_without_bounce_light_is_the_one_bounce_path():
+    """giScale 0 zeroes the throughput of bounce 2 (common.glsl:2277, :2339): the two-bounce path then
+    returns exactly what the one-bounce path (pinned above) returns."""
+    ow, a, P, N = _gbuffer_points(300)
+    scene = vf.default_scene(a["world_bricks"])
+    scene.gi_scale = 0.0
+    rng = np.random.default_rng(4)
+    for i in range(len(P)):
+        seed = int(rng.integers(0, 2 ** 32))
+        d, seed = O.cosine_hemisphere(N[i], seed)
+        L3, _, _ = ow.gi_path(scene, P[i], N[i], d, seed, last_stage=3)
+        L1, _, _ = ow.gi_path(scene, P[i], N[i], d, seed, last_stage=1)
+        assert np.array_equal(L3, L1), i
+
+
+def test_two_bounce_path_is_its_head_plus_tail():
+    """Stopping a path where bounce 2 starts and continuing it from that point by hand (bounce 2 +
+    its sun ray, no trace-on: giPathImpl(..., 1, false)) gives the whole path: L = pre + thr * rest.
+    This is the split the learned cache's targets and queries rely on."""
+    ow, a, P, N = _gbuffer_points(400)
+    scene = vf.default_scene(a["world_bricks"])
+    rng = np.random.default_rng(5)
+    tails = 0
+    for i in range(len(P)):
+        seed0 = int(rng.integers(0, 2 ** 32))
+        d, seed = O.cosine_hemisphere(N[i], seed0)
+        L, _, _ = ow.gi_path(scene, P[i], N[i], d, seed, last_stage=3)
+        Lh, s_after, t = ow.gi_path(scene, P[i], N[i], d, seed, last_stage=3, tail=True)
+        assert s_after == seed   # nothing drawn before bounce 2
+        if not t["valid"]:
+            assert np.array_equal(L, Lh)
+            continue
+        tails += 1
+        d2, s2 = O.cosine_hemisphere(t["n"], seed)
+        Lr, _, _ = ow.gi_path(scene, t["p"], t["n"], d2, s2, last_stage=1, trace_on=False)
+        want = Lh.astype(np.float64) + t["thr"].astype(np.float64) * Lr
+        assert np.allclose(L, want, rtol=2e-6, atol=1e-7), (i, L, want)
+        # the throughput is albedo x giScale of the bounce-1 hit
+        assert np.all(t["thr"] >= 0.0) and np.all(t["thr"] <= 1.0)
+    assert tails > 40

@@ -1,0 +1,1246 @@
+// Two-bounce GI paths and the learned GI radiance cache (include/vf_gicache.h).
+//
+// The network (frequency encoding -> 5 x 64 ReLU -> RGB) runs on the matrix cores with
+// v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation.  Lane layout of that instruction
+// (lane l = 0..63, g = l / 16, c = l % 16):  A (16x32): A[c][8g + i];  B (32x16): B[8g + i][c];
+// D (16x16): D[4g + i][c].  Every layer is computed transposed, D = W^T (out x in) . X^T (in x samples),
+// so that a lane's B operand is 8 consecutive features of one sample: one 16-byte LDS read from a
+// [sample][feature] activation tile.  Weights live in "fragment" order (each lane's 8 BF16 of one
+// 16x32 operand block are contiguous), written by the optimizer kernel after every update, so a weight
+// operand is one coalesced 16-byte global load per lane.
+//
+// A training wavefront owns 64 samples: the forward pass keeps every layer's BF16 activations in LDS,
+// the backward pass overwrites each activation tile in place with its gradient once the weight gradient
+// that needs it has been accumulated.  Weight gradients are MFMA products with the samples as the K
+// dimension; each training wavefront writes its own partial, and the optimizer kernel sums the partials
+// in a fixed order (deterministic), then applies Adam to the FP32 master weights.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/vf_gicache.h"
+#include "vf_ctx.h"
+#include "vf_gi.h"
+
+using vf::blocks_for;
+using vf::launch_check;
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kP = VF_GICACHE_PARAMS;
+constexpr int kIn = VF_GICACHE_INPUTS;   // 78 features padded to 96
+// Layer shapes: (96, 64), (64, 64) x 4, (64, 3).
+__host__ __device__ constexpr int in_dim(int l) { return l == 0 ? 96 : 64; }
+__host__ __device__ constexpr int out_dim(int l) { return l == 5 ? 3 : 64; }
+__host__ __device__ constexpr int w_off(int l) { return l == 0 ? 0 : 96 * 64 + 64 + (l - 1) * (64 * 64 + 64); }
+__host__ __device__ constexpr int b_off(int l) { return w_off(l) + in_dim(l) * out_dim(l); }
+static_assert(b_off(5) + 3 == kP, "parameter count");
+// Forward fragments of layer l: (out padded to 16)/16 row tiles x in/32 K blocks; backward fragments
+// (layers 1..5): in/16 row tiles x (out padded to 32)/32 K blocks.  1 KB each.
+__host__ __device__ constexpr int fwd_mt(int l) { return l == 5 ? 1 : 4; }
+__host__ __device__ constexpr int fwd_kb(int l) { return in_dim(l) / 32; }
+__host__ __device__ constexpr int fwd_base(int l) { return l == 0 ? 0 : 12 + (l - 1) * 8; }
+__host__ __device__ constexpr int bwd_kb(int l) { return l == 5 ? 1 : 2; }
+__host__ __device__ constexpr int bwd_base(int l) { return (l - 1) * 8; }
+constexpr int kFwdFrags = 46, kBwdFrags = 36;
+static_assert(fwd_base(5) + fwd_mt(5) * fwd_kb(5) == kFwdFrags, "forward fragments");
+static_assert(bwd_base(5) + 4 * bwd_kb(5) == kBwdFrags, "backward fragments");
+
+// LDS row strides (BF16 elements) of the activation tiles: rows stay 16-byte aligned.
+constexpr int kS0 = 104;   // encoded input, 96 features
+constexpr int kSH = 72;    // hidden, 64 features
+constexpr int kSG = 40;    // output gradient, 3 used of 32
+constexpr int kTrainLds = 64 * kS0 + 5 * 64 * kSH + 64 * kSG;   // BF16 elements per training wavefront
+constexpr int kInferLds = 64 * kS0 + 2 * 64 * kSH;
+
+// sin(pi u), cos(pi u): u - rint(u) is exact, then Taylor polynomials on [-pi/2, pi/2] (terms to x^11
+// and x^12; truncation < 6e-8) and the sign of the half period.
+__device__ __forceinline__ void sincospi(float u, float &s, float &c) {
+    const float n = rintf(u);
+    const float x = (u - n) * 3.14159265358979f;
+    const float x2 = x * x;
+    float ps = x2 * -2.50521084e-08f + 2.75573192e-06f;
+    ps = ps * x2 + -1.98412698e-04f;
+    ps = ps * x2 + 8.33333333e-03f;
+    ps = ps * x2 + -1.66666667e-01f;
+    s = x + x * x2 * ps;
+    float pc = x2 * 2.08767570e-09f + -2.75573192e-07f;
+    pc = pc * x2 + 2.48015873e-05f;
+    pc = pc * x2 + -1.38888889e-03f;
+    pc = pc * x2 + 4.16666667e-02f;
+    pc = pc * x2 + -0.5f;
+    c = 1.0f + x2 * pc;
+    if ((int32_t)n & 1) { s = -s; c = -c; }
+}
+
+// Feature f of a record: f = 24 a + 2 k (+1): sin (cos) of pi * pos[a] * 2^(k - shift); 72..74 normal,
+// 75..77 view, 78..95 zero.  Group q (0..11) = features 8q..8q+7.
+__device__ __forceinline__ void encode_group(const float4 &r0, const float4 &r1, const float4 &r2, int shift, int q,
+                                             float f[8]) {
+    if (q < 9) {
+        const int a = q / 3, k0 = (q % 3) * 4;
+        const float x = a == 0 ? r0.x : (a == 1 ? r0.y : r0.z);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sincospi(ldexpf(x, k0 + j - shift), f[2 * j], f[2 * j + 1]);
+    } else if (q == 9) {
+        f[0] = r1.x; f[1] = r1.y; f[2] = r1.z;
+        f[3] = r2.x; f[4] = r2.y; f[5] = r2.z;
+        f[6] = 0.0f; f[7] = 0.0f;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(const float f[8]) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (bf16)f[j];
+    return v;
+}
+
+// The lane's record (64 B) into row `lane` of the input tile.
+__device__ __forceinline__ void encode_row(const float4 *rec, bool present, int shift, bf16 *act0, int lane) {
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+    if (present) { r0 = rec[0]; r1 = rec[1]; r2 = rec[2]; }
+    bf16 *row = act0 + lane * kS0;
+    for (int q = 0; q < 12; ++q) {
+        float f[8];
+        encode_group(r0, r1, r2, shift, q, f);
+        *reinterpret_cast<bf16x8 *>(row + 8 * q) = to_bf16x8(f);
+    }
+}
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Hidden layer l (KB = in/32): act_out = bf16(relu(W^T . act_in + b)) for the wavefront's 64 samples.
+template <int KB>
+__device__ __forceinline__ void hidden_layer(const bf16 *act_in, int s_in, bf16 *act_out, const bf16x8 *frag,
+                                             const float *bias, int lane) {
+    const int col = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int mt = 0; mt < 4; ++mt) {
+        bf16x8 a[KB];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
+        float bz[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bz[i] = bias[mt * 16 + 4 * g + i];
+        f32x4 acc[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(act_in + (nt * 16 + col) * s_in + kb * 32 + 8 * g);
+                acc[nt] = mfma(a[kb], b, acc[nt]);
+            }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            bf16x4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float v = acc[nt][i] + bz[i];
+                o[i] = (bf16)(v > 0.0f ? v : 0.0f);
+            }
+            *reinterpret_cast<bf16x4 *>(act_out + (nt * 16 + col) * kSH + mt * 16 + 4 * g) = o;
+        }
+    }
+}
+
+// Output layer (64 -> 3, rows padded to 16): y[nt][i] = output 4g + i of sample nt*16 + col (only g = 0,
+// i < 3 are outputs).
+__device__ __forceinline__ void output_layer(const bf16 *act5, const bf16x8 *frag, const float *bias, int lane,
+                                             f32x4 y[4]) {
+    const int col = lane & 15, g = lane >> 4;
+    const bf16x8 a0 = frag[0 * 64 + lane], a1 = frag[1 * 64 + lane];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const bf16 *row = act5 + (nt * 16 + col) * kSH + 8 * g;
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma(a0, *reinterpret_cast<const bf16x8 *>(row), acc);
+        acc = mfma(a1, *reinterpret_cast<const bf16x8 *>(row + 32), acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int o = 4 * g + i;
+            y[nt][i] = acc[i] + (o < 3 ? bias[o] : 0.0f);
+        }
+    }
+}
+
+// The forward pass of the wavefront's 64 encoded samples; hidden activations stay in act[1..5].
+__device__ __forceinline__ void forward(bf16 *act0, bf16 *const act[6], const bf16x8 *ffrag, const float *w, int lane,
+                                        f32x4 y[4]) {
+    hidden_layer<3>(act0, kS0, act[1], ffrag + fwd_base(0) * 64, w + b_off(0), lane);
+    __syncthreads();
+#pragma unroll 1
+    for (int l = 1; l < 5; ++l) {
+        hidden_layer<2>(act[l], kSH, act[l + 1], ffrag + fwd_base(l) * 64, w + b_off(l), lane);
+        __syncthreads();
+    }
+    output_layer(act[5], ffrag + fwd_base(5) * 64, w + b_off(5), lane, y);
+}
+
+// Gradient into the previous layer, in place: act[l] := bf16((W_l . G^T)^T * (act[l] > 0)), where G
+// (the gradient at layer l's output, KB = out/32 blocks) is read from g_in.
+template <int KB>
+__device__ __forceinline__ void backward_layer(const bf16 *g_in, int s_g, bf16 *act, const bf16x8 *frag, int lane) {
+    const int col = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int mt = 0; mt < 4; ++mt) {
+        bf16x8 a[KB];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
+        f32x4 acc[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(g_in + (nt * 16 + col) * s_g + kb * 32 + 8 * g);
+                acc[nt] = mfma(a[kb], b, acc[nt]);
+            }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            bf16x4 *p = reinterpret_cast<bf16x4 *>(act + (nt * 16 + col) * kSH + mt * 16 + 4 * g);
+            const bf16x4 m = *p;
+            bf16x4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[i] = (float)m[i] > 0.0f ? (bf16)acc[nt][i] : (bf16)0.0f;
+            *p = o;
+        }
+    }
+}
+
+// Weight gradient of layer l over the wavefront's 64 samples: dW[in][out] += sum_s act[s][in] G[s][out]
+// (MFMA with the samples as K; NMT = in/16 row tiles, NNT = out column tiles), bias gradient
+// db[out] += sum_s G[s][out]; written to (or added into) this wavefront's partial.
+template <int NMT, int NNT>
+__device__ __forceinline__ void weight_grad(const bf16 *act, int s_act, const bf16 *gr, int s_g, int nout, float *pw,
+                                            float *pb, bool accumulate, int lane) {
+    const int col = lane & 15, g = lane >> 4;
+    bf16x8 b[NNT][2];
+#pragma unroll
+    for (int nt = 0; nt < NNT; ++nt)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) b[nt][kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
+#pragma unroll 1
+    for (int mt = 0; mt < NMT; ++mt) {
+        bf16x8 a[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) a[kb][i] = act[(kb * 32 + 8 * g + i) * s_act + mt * 16 + col];
+#pragma unroll
+        for (int nt = 0; nt < NNT; ++nt) {
+            const int o = nt * 16 + col;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (accumulate && o < nout) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i] = pw[(mt * 16 + 4 * g + i) * nout + o];
+            }
+            acc = mfma(a[0], b[nt][0], acc);
+            acc = mfma(a[1], b[nt][1], acc);
+            if (o < nout) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) pw[(mt * 16 + 4 * g + i) * nout + o] = acc[i];
+            }
+        }
+    }
+    if (lane < nout) {
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += (float)gr[r * s_g + lane];
+        pb[lane] = accumulate ? pb[lane] + s : s;
+    }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+struct TrainArgs {
+    const float4 *samples;   // n records, 4 x float4 each
+    long long n;
+    int32_t n_chunks;        // ceil(n / 64)
+    int32_t shift;
+    float inv_n, loss_eps;
+    const bf16x8 *ffrag, *bfrag;
+    const float *w;          // FP32 master weights (biases are read from here)
+    float *part;             // [gridDim.x][kP]
+    float *loss_part;        // [gridDim.x]
+    int32_t *valid_part;     // [gridDim.x]
+};
+
+// Forward + backward of the relative L2 loss over 64-sample chunks; one wavefront per workgroup, chunks
+// strided over the grid.
+__global__ void __launch_bounds__(64) k_mlp_train(TrainArgs A) {
+    __shared__ __attribute__((aligned(16))) bf16 lds[kTrainLds];
+    const int lane = threadIdx.x;
+    bf16 *act0 = lds;
+    bf16 *act[6];
+    act[0] = act0;
+#pragma unroll
+    for (int l = 1; l < 6; ++l) act[l] = lds + 64 * kS0 + (l - 1) * 64 * kSH;
+    bf16 *g6 = lds + 64 * kS0 + 5 * 64 * kSH;
+    for (int j = 0; j < kSG; ++j) g6[lane * kSG + j] = (bf16)0.0f;
+    float *pw = A.part + (size_t)blockIdx.x * kP;
+    float loss = 0.0f;
+    int32_t nvalid = 0;
+    bool first = true;
+    for (int ch = blockIdx.x; ch < A.n_chunks; ch += gridDim.x) {
+        const long long sidx = (long long)ch * 64 + lane;
+        const bool present = sidx < A.n;
+        const float4 *rec = A.samples + (present ? sidx : 0) * 4;
+        float4 r3 = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool valid = false;
+        if (present) {
+            valid = __float_as_uint(rec[0].w) != 0u;
+            r3 = rec[3];
+        }
+        nvalid += valid ? 1 : 0;
+        encode_row(rec, present, A.shift, act0, lane);
+        __syncthreads();
+        f32x4 y[4];
+        forward(act0, act, A.ffrag, A.w, lane, y);
+        // loss gradient: lanes 0..15 hold outputs 0..2 of samples nt*16 + lane
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int s = nt * 16 + (lane & 15);
+            const float tx = __shfl(r3.x, s), ty = __shfl(r3.y, s), tz = __shfl(r3.z, s);
+            const bool v = __shfl((int)valid, s) != 0;
+            if (lane < 16) {
+                const float y0 = y[nt][0], y1 = y[nt][1], y2 = y[nt][2];
+                const float mean = ((y0 + y1) + y2) / 3.0f;
+                const float den = mean * mean + A.loss_eps;
+                const float d0 = y0 - tx, d1 = y1 - ty, d2 = y2 - tz;
+                bf16x4 o = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+                if (v) {
+                    loss += ((d0 * d0 + d1 * d1) + d2 * d2) / den / 3.0f;
+                    const float k = (2.0f / 3.0f) * A.inv_n / den;
+                    o[0] = (bf16)(d0 * k);
+                    o[1] = (bf16)(d1 * k);
+                    o[2] = (bf16)(d2 * k);
+                }
+                *reinterpret_cast<bf16x4 *>(g6 + s * kSG) = o;
+            }
+        }
+        __syncthreads();
+        // layer 5 (64 -> 3), then 4..1, then 0
+        weight_grad<4, 1>(act[5], kSH, g6, kSG, 3, pw + w_off(5), pw + b_off(5), !first, lane);
+        __syncthreads();
+        backward_layer<1>(g6, kSG, act[5], A.bfrag + bwd_base(5) * 64, lane);
+        __syncthreads();
+#pragma unroll 1
+        for (int l = 4; l >= 1; --l) {
+            weight_grad<4, 4>(act[l], kSH, act[l + 1], kSH, 64, pw + w_off(l), pw + b_off(l), !first, lane);
+            __syncthreads();
+            backward_layer<2>(act[l + 1], kSH, act[l], A.bfrag + bwd_base(l) * 64, lane);
+            __syncthreads();
+        }
+        weight_grad<6, 4>(act0, kS0, act[1], kSH, 64, pw + w_off(0), pw + b_off(0), !first, lane);
+        __syncthreads();
+        first = false;
+    }
+    if (first) {   // no chunk for this wavefront: its partial is zero
+        for (int p = lane; p < kP; p += 64) pw[p] = 0.0f;
+    }
+    loss = wave_sum(loss);
+    const float nv = wave_sum((float)nvalid);
+    if (lane == 0) {
+        A.loss_part[blockIdx.x] = loss * A.inv_n;
+        A.valid_part[blockIdx.x] = (int32_t)nv;
+    }
+}
+
+struct AdamArgs {
+    float *w, *m, *v;
+    const float *part;     // [nparts][kP] (mode 1, 2 without grad_in)
+    int32_t nparts;
+    const float *grad_in;  // mode 2: the gradient instead of the partials
+    float *grad_out;       // mode 1: the reduced gradient
+    int32_t mode;          // 0: fragments only; 1: reduce only; 2: reduce (or grad_in) + Adam + fragments
+    float lr, b1, b2, c1, c2, bc1, bc2, eps;
+    bf16 *ffrag, *bfrag;
+};
+
+__device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) {
+    int l = 0;
+    while (l < 5 && p >= w_off(l + 1)) ++l;
+    if (p >= b_off(l)) return;   // a bias: no fragment
+    const int q = p - w_off(l), nout = out_dim(l);
+    const int in = q / nout, out = q % nout;
+    const bf16 bv = (bf16)wv;
+    {
+        const int mt = out / 16, kb = in / 32, ln = (out & 15) + 16 * ((in & 31) >> 3), i = in & 7;
+        A.ffrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
+    }
+    if (l >= 1) {
+        const int mt = in / 16, kb = out / 32, ln = (in & 15) + 16 * ((out & 31) >> 3), i = out & 7;
+        A.bfrag[((size_t)(bwd_base(l) + mt * bwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
+    }
+}
+
+// 64 parameters per workgroup x 4 slices of the partials, summed in a fixed order.
+__global__ void __launch_bounds__(256) k_mlp_adam(AdamArgs A) {
+    __shared__ float red[4][64];
+    const int t = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + t;
+    float s = 0.0f;
+    if (A.mode != 0 && !A.grad_in && p < kP)
+        for (int q = sl; q < A.nparts; q += 4) s += A.part[(size_t)q * kP + p];
+    red[sl][t] = s;
+    __syncthreads();
+    if (sl != 0 || p >= kP) return;
+    if (A.mode == 0) {
+        write_frags(A, p, A.w[p]);
+        return;
+    }
+    const float gsum = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    const float g = A.grad_in ? A.grad_in[p] : gsum;
+    if (A.mode == 1) {
+        A.grad_out[p] = g;
+        return;
+    }
+    const float m = A.b1 * A.m[p] + A.c1 * g;
+    const float v = A.b2 * A.v[p] + A.c2 * (g * g);
+    const float mh = m / A.bc1, vh = v / A.bc2;
+    const float wv = A.w[p] - A.lr * mh / (sqrtf(vh) + A.eps);
+    A.m[p] = m;
+    A.v[p] = v;
+    A.w[p] = wv;
+    write_frags(A, p, wv);
+}
+
+// Device statistics of the cache (the fallback decision stays on the device, so a frame needs no sync).
+struct DevStats {
+    long long frames;
+    float last_loss, running_error, baseline;
+    int32_t fallback, fallback_frames, n_valid;
+};
+
+struct StatsArgs {
+    const float *loss_part;
+    const int32_t *valid_part;
+    int32_t nparts;
+    DevStats *st;
+    unsigned *qcount;
+    float *loss_out;        // grad mode: the loss only
+    int32_t warmup;
+    float ratio;
+};
+
+__global__ void __launch_bounds__(64) k_gc_stats(StatsArgs A) {
+    const int lane = threadIdx.x;
+    float s = 0.0f, nv = 0.0f;
+    for (int q = lane; q < A.nparts; q += 64) {
+        s += A.loss_part[q];
+        nv += (float)A.valid_part[q];
+    }
+    s = wave_sum(s);
+    nv = wave_sum(nv);
+    if (lane != 0) return;
+    if (A.loss_out) {
+        *A.loss_out = s;
+        return;
+    }
+    DevStats &st = *A.st;
+    st.frames += 1;
+    st.last_loss = s;
+    st.n_valid = (int32_t)nv;
+    st.running_error = st.frames == 1 ? s : 0.9f * st.running_error + 0.1f * s;
+    if (st.frames <= A.warmup) {
+        st.fallback = 1;
+        st.baseline = INFINITY;
+    } else {
+        st.baseline = fminf(st.baseline, st.running_error);
+        st.fallback = st.running_error > A.ratio * st.baseline ? 1 : 0;
+        st.fallback_frames = st.fallback ? st.fallback_frames + 1 : 0;
+        if (st.fallback_frames >= 64) {   // a lasting change (new lighting, edits): accept its error level
+            st.baseline = st.running_error;
+            st.fallback = 0;
+            st.fallback_frames = 0;
+        }
+    }
+    *A.qcount = 0u;
+}
+
+// ---- GI kernels -------------------------------------------------------------------------------
+
+struct PathArgs {
+    int32_t w, h, tiles_x, ntiles, spp, terminate;
+    uint32_t frame, salt;
+    GiPathCfg G;
+    float cam[3];
+};
+
+__device__ __forceinline__ void gbuffer_point(const float4 *gbpos, const uint32_t *gbn, size_t i, float P[3],
+                                              float N[3]) {
+    const float4 p4 = gbpos[i];
+    const uint32_t nb = gbn[i];
+    P[0] = p4.x; P[1] = p4.y; P[2] = p4.z;
+    N[0] = unorm8_dec(nb & 255u);
+    N[1] = unorm8_dec((nb >> 8) & 255u);
+    N[2] = unorm8_dec((nb >> 16) & 255u);
+}
+
+__device__ __forceinline__ bool tile_pixel(const PathArgs &A, int32_t &px, int32_t &py) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    return px < A.w && py < A.h;
+}
+
+// Sample k of pixel (px, py): bounce-1 direction cosineHemisphereU(N, ldSample2(pix, k, salt)) and the
+// path seed pixelSeed(pix, k, salt).
+__device__ __forceinline__ void sample_start(const float N[3], int32_t px, int32_t py, uint32_t k, uint32_t salt,
+                                             float dir[3], uint32_t &seed) {
+    float u0, u1;
+    ld_sample2(px, py, k, salt, u0, u1);
+    cosine_hemisphere_u(N, u0, u1, dir);
+    seed = pixel_seed(px, py, k, salt);
+}
+
+// Plain traced two-bounce GI: the mean of spp full paths per pixel.
+__device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &S, const PathArgs &A,
+                                            const float4 *gbpos, const uint32_t *gbn, int32_t px, int32_t py,
+                                            float4 *out) {
+    const size_t i = (size_t)py * A.w + px;
+    float sum[3] = {0.0f, 0.0f, 0.0f};
+    const bool sky = gbpos[i].w < 0.0f;
+    if (!sky) {
+        float P[3], N[3];
+        gbuffer_point(gbpos, gbn, i, P, N);
+        Cnt c = {0, 0, 0, 0};
+        for (int32_t s = 0; s < A.spp; ++s) {
+            float dir[3], L[3];
+            uint32_t seed;
+            sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
+            gi_path<3, false, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sum[a] += L[a];
+        }
+    }
+    const float fs = (float)A.spp;
+    out[i] = sky ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(sum[0] / fs, sum[1] / fs, sum[2] / fs, 1.0f);
+}
+
+__global__ void __launch_bounds__(256) k_gi_plain(DevWorld W, SkyParams S, PathArgs A, const float4 *gbpos,
+                                                  const uint32_t *gbn, float4 *out) {
+    int32_t px, py;
+    if (!tile_pixel(A, px, py)) return;
+    plain_pixel(W, S, A, gbpos, gbn, px, py, out);
+}
+
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
+    return (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
+}
+
+// Appends the lane's record (if `want`) to the query list with one atomic per wavefront.
+__device__ __forceinline__ void push_query(bool want, unsigned *qcount, float4 *queries, const float4 r[4]) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ull) return;
+    unsigned base = 0;
+    if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1)) base = atomicAdd(qcount, (unsigned)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    if (want) {
+        float4 *q = queries + (size_t)(base + lane_rank(m)) * 4;
+        q[0] = r[0]; q[1] = r[1]; q[2] = r[2]; q[3] = r[3];
+    }
+}
+
+// Cached render, traced part.  Fallback frames: the plain path, written to out.  Otherwise terminate 1:
+// per sample the path up to its tail (pre += its light; a query at the tail, or tailv[slot] = 0);
+// terminate 0: one query per non-sky pixel at the G-buffer point.
+__global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, PathArgs A, const float4 *gbpos,
+                                                   const uint32_t *gbn, const DevStats *st, float4 *out, float4 *pre,
+                                                   float4 *tailv, float4 *queries, unsigned *qcount) {
+    int32_t px, py;
+    const bool inside = tile_pixel(A, px, py);
+    if (st->fallback) {
+        if (inside) plain_pixel(W, S, A, gbpos, gbn, px, py, out);
+        return;
+    }
+    const size_t i = inside ? (size_t)py * A.w + px : 0;
+    const bool sky = !inside || gbpos[i].w < 0.0f;
+    float P[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f};
+    if (!sky) gbuffer_point(gbpos, gbn, i, P, N);
+    if (A.terminate == 0) {
+        float4 r[4];
+        if (!sky) {
+            float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
+            normalize3(v);
+            r[0] = make_float4(P[0], P[1], P[2], __uint_as_float((uint32_t)i));
+            r[1] = make_float4(N[0], N[1], N[2], 0.0f);
+            r[2] = make_float4(v[0], v[1], v[2], 0.0f);
+            r[3] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        }
+        push_query(!sky, qcount, queries, r);
+        if (inside) {
+            pre[i] = make_float4(0.0f, 0.0f, 0.0f, sky ? 0.0f : 1.0f);
+            if (sky) tailv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        return;
+    }
+    float sum[3] = {0.0f, 0.0f, 0.0f};
+    Cnt c = {0, 0, 0, 0};
+    for (int32_t s = 0; s < A.spp; ++s) {
+        const size_t slot = i * (size_t)A.spp + (size_t)s;
+        GiTail t;
+        t.valid = false;
+        if (!sky) {
+            float dir[3], L[3];
+            uint32_t seed;
+            sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
+            gi_path<3, true, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sum[a] += L[a];
+        }
+        float4 r[4];
+        if (t.valid) {
+            r[0] = make_float4(t.p[0], t.p[1], t.p[2], __uint_as_float((uint32_t)slot));
+            r[1] = make_float4(t.n[0], t.n[1], t.n[2], 0.0f);
+            r[2] = make_float4(-t.dir[0], -t.dir[1], -t.dir[2], 0.0f);
+            r[3] = make_float4(t.thr[0], t.thr[1], t.thr[2], 0.0f);
+        }
+        push_query(t.valid, qcount, queries, r);
+        if (inside && !t.valid) tailv[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (inside) pre[i] = make_float4(sum[0], sum[1], sum[2], sky ? 0.0f : 1.0f);
+}
+
+// out = (pre + sum of the slots' cache terms) / spp; nothing on fallback frames (out is already final).
+__global__ void __launch_bounds__(256) k_gc_resolve(const DevStats *st, int32_t npix, int32_t spp, const float4 *pre,
+                                                    const float4 *tailv, float4 *out) {
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npix || st->fallback) return;
+    const float4 p = pre[i];
+    float s[3] = {p.x, p.y, p.z};
+    for (int32_t k = 0; k < spp; ++k) {
+        const float4 t = tailv[(size_t)i * spp + k];
+        s[0] += t.x; s[1] += t.y; s[2] += t.z;
+    }
+    const float fs = (float)spp;
+    out[i] = p.w == 0.0f ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(s[0] / fs, s[1] / fs, s[2] / fs, 1.0f);
+}
+
+struct SampleArgs {
+    int32_t w, h, n, terminate;
+    uint32_t frame, salt;
+    GiPathCfg G;
+    float cam[3];
+};
+
+// Training samples: sample i takes pixel floor((i + r) * W*H / n) (r from its own stream
+// pcg(i + pcg(frame ^ salt))) and a cosine-hemisphere bounce-1 direction from the same stream.
+__global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, SampleArgs A, const float4 *gbpos,
+                                                    const uint32_t *gbn, float4 *out) {
+    const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= A.n) return;
+    float4 *r = out + (size_t)i * 4;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const long long npix = (long long)A.w * A.h;
+    const float stratum = (float)npix / (float)A.n;
+    uint32_t seed = pcg((uint32_t)i + pcg(A.frame ^ A.salt));
+    const float u = rnd(seed);
+    long long pix = (long long)floorf(((float)i + u) * stratum);
+    if (pix > npix - 1) pix = npix - 1;
+    if (gbpos[pix].w < 0.0f) {
+        r[0] = z; r[1] = z; r[2] = z; r[3] = z;
+        return;
+    }
+    float P[3], N[3], dir[3], L[3];
+    gbuffer_point(gbpos, gbn, (size_t)pix, P, N);
+    cosine_hemisphere(N, seed, dir);
+    Cnt c = {0, 0, 0, 0};
+    if (A.terminate == 0) {
+        gi_path<3, false, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c);
+        float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
+        normalize3(v);
+        r[0] = make_float4(P[0], P[1], P[2], __uint_as_float(1u));
+        r[1] = make_float4(N[0], N[1], N[2], 0.0f);
+        r[2] = make_float4(v[0], v[1], v[2], 0.0f);
+        r[3] = make_float4(L[0], L[1], L[2], 0.0f);
+        return;
+    }
+    GiTail t;
+    gi_path<3, true, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c);
+    if (!t.valid) {
+        r[0] = z; r[1] = z; r[2] = z; r[3] = z;
+        return;
+    }
+    float d2[3], L2[3];
+    cosine_hemisphere(t.n, seed, d2);
+    gi_path<1, false, false>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c);
+    r[0] = make_float4(t.p[0], t.p[1], t.p[2], __uint_as_float(1u));
+    r[1] = make_float4(t.n[0], t.n[1], t.n[2], 0.0f);
+    r[2] = make_float4(-t.dir[0], -t.dir[1], -t.dir[2], 0.0f);
+    r[3] = make_float4(L2[0], L2[1], L2[2], 0.0f);
+}
+
+struct InferArgs {
+    const float4 *recs;
+    long long n;             // records (mode 0)
+    const unsigned *count;   // mode 1: the query count on the device
+    int32_t shift;
+    const bf16x8 *ffrag;
+    const float *w;
+    uint16_t *out_h;         // mode 0: FP16 RGB(0) per record
+    float4 *tailv;           // mode 1: slot -> throughput * max(FP16 output, 0)
+};
+
+// MODE 0: n records -> FP16 outputs; MODE 1: the render's query list -> its slots.
+template <int MODE>
+__global__ void __launch_bounds__(64) k_mlp_infer(InferArgs A) {
+    __shared__ __attribute__((aligned(16))) bf16 lds[kInferLds];
+    const int lane = threadIdx.x;
+    const long long n = MODE == 0 ? A.n : (long long)*A.count;
+    const long long base = (long long)blockIdx.x * 64;
+    if (base >= n) return;
+    bf16 *act0 = lds, *h0 = lds + 64 * kS0, *h1 = h0 + 64 * kSH;
+    const long long idx = base + lane;
+    const bool present = idx < n;
+    const float4 *rec = A.recs + (present ? idx : 0) * 4;
+    encode_row(rec, present, A.shift, act0, lane);
+    __syncthreads();
+    hidden_layer<3>(act0, kS0, h0, A.ffrag + fwd_base(0) * 64, A.w + b_off(0), lane);
+    __syncthreads();
+    hidden_layer<2>(h0, kSH, h1, A.ffrag + fwd_base(1) * 64, A.w + b_off(1), lane);
+    __syncthreads();
+    hidden_layer<2>(h1, kSH, h0, A.ffrag + fwd_base(2) * 64, A.w + b_off(2), lane);
+    __syncthreads();
+    hidden_layer<2>(h0, kSH, h1, A.ffrag + fwd_base(3) * 64, A.w + b_off(3), lane);
+    __syncthreads();
+    hidden_layer<2>(h1, kSH, h0, A.ffrag + fwd_base(4) * 64, A.w + b_off(4), lane);
+    __syncthreads();
+    f32x4 y[4];
+    output_layer(h0, A.ffrag + fwd_base(5) * 64, A.w + b_off(5), lane, y);
+    // lanes 0..15 hold the outputs of samples nt*16 + lane; send them to the sample's lane
+    float yr[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float v = __shfl(y[nt][c], lane & 15);
+            if ((lane >> 4) == nt) yr[c] = v;
+        }
+    if (!present) return;
+    const __half h0v = __float2half_rn(yr[0]), h1v = __float2half_rn(yr[1]), h2v = __float2half_rn(yr[2]);
+    if (MODE == 0) {
+        ushort4 o;
+        o.x = __half_as_ushort(h0v);
+        o.y = __half_as_ushort(h1v);
+        o.z = __half_as_ushort(h2v);
+        o.w = 0;
+        reinterpret_cast<ushort4 *>(A.out_h)[idx] = o;
+    } else {
+        const float4 r0 = rec[0], r3 = rec[3];
+        const uint32_t slot = __float_as_uint(r0.w);
+        const float a = fmaxf(__half2float(h0v), 0.0f), b = fmaxf(__half2float(h1v), 0.0f),
+                    cc = fmaxf(__half2float(h2v), 0.0f);
+        A.tailv[slot] = make_float4(r3.x * a, r3.y * b, r3.z * cc, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_mlp_encode(const float4 *recs, long long n, int32_t shift, float *enc) {
+    const long long i = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const float4 *r = recs + i * 4;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    for (int q = 0; q < 12; ++q) {
+        float f[8];
+        encode_group(r0, r1, r2, shift, q, f);
+        float4 *o = reinterpret_cast<float4 *>(enc + i * kIn + 8 * q);
+        o[0] = make_float4(f[0], f[1], f[2], f[3]);
+        o[1] = make_float4(f[4], f[5], f[6], f[7]);
+    }
+}
+
+uint32_t host_pcg(uint32_t v) {
+    v = v * 747796405u + 2891336453u;
+    v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
+    return (v >> 22u) ^ v;
+}
+
+}  // namespace
+
+struct vf_gicache {
+    vf_ctx *ctx = nullptr;
+    vf_gicache_params p{};
+    float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    bf16 *ffrag = nullptr, *bfrag = nullptr;
+    float *part = nullptr, *loss_part = nullptr;
+    int32_t *valid_part = nullptr;
+    int32_t part_rows = 0;
+    float4 *samples = nullptr;
+    DevStats *st = nullptr;
+    unsigned *qcount = nullptr;
+    float4 *queries = nullptr, *pre = nullptr, *tailv = nullptr;
+    int64_t cap_pix = 0, cap_slots = 0;
+    long long steps = 0;
+};
+
+namespace {
+
+int gc_fail_null(vf_ctx *ctx, const char *what) { return vf::fail(ctx, VF_E_INVALID, "%s: null argument", what); }
+
+int check_cache(vf_ctx *ctx, vf_gicache *c, const char *what) {
+    if (!ctx) return VF_E_INVALID;
+    if (!c || c->ctx != ctx) return vf::fail(ctx, VF_E_INVALID, "%s: cache missing or created on another context", what);
+    return vf::set_device(ctx);
+}
+
+void free_all(vf_gicache *c) {
+    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->part, c->loss_part, c->valid_part,
+                    c->samples, c->st, c->qcount, c->queries, c->pre, c->tailv};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+}
+
+int launch_adam(vf_gicache *c, int mode, const float *grad_in, float *grad_out, int nparts) {
+    AdamArgs A;
+    A.w = c->w; A.m = c->m; A.v = c->v;
+    A.part = c->part;
+    A.nparts = nparts;
+    A.grad_in = grad_in;
+    A.grad_out = grad_out;
+    A.mode = mode;
+    A.lr = c->p.lr; A.b1 = c->p.beta1; A.b2 = c->p.beta2;
+    A.c1 = 1.0f - c->p.beta1; A.c2 = 1.0f - c->p.beta2;
+    A.bc1 = 1.0f; A.bc2 = 1.0f;
+    A.eps = c->p.eps;
+    if (mode == 2) {
+        c->steps += 1;
+        A.bc1 = (float)(1.0 - std::pow((double)c->p.beta1, (double)c->steps));
+        A.bc2 = (float)(1.0 - std::pow((double)c->p.beta2, (double)c->steps));
+    }
+    A.ffrag = c->ffrag; A.bfrag = c->bfrag;
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kP + 63) / 64), dim3(256), 0, c->ctx->stream, A);
+    return launch_check(c->ctx, "k_mlp_adam");
+}
+
+// Forward + backward over n samples into the partials; returns the number of partial rows used.
+int launch_train(vf_gicache *c, const float4 *samples, long long n, int *rows) {
+    TrainArgs A;
+    A.samples = samples;
+    A.n = n;
+    A.n_chunks = (int32_t)((n + 63) / 64);
+    A.shift = c->p.enc_shift;
+    A.inv_n = 1.0f / (float)n;
+    A.loss_eps = c->p.loss_eps;
+    A.ffrag = reinterpret_cast<const bf16x8 *>(c->ffrag);
+    A.bfrag = reinterpret_cast<const bf16x8 *>(c->bfrag);
+    A.w = c->w;
+    A.part = c->part;
+    A.loss_part = c->loss_part;
+    A.valid_part = c->valid_part;
+    const int g = A.n_chunks < c->part_rows ? A.n_chunks : c->part_rows;
+    hipLaunchKernelGGL(k_mlp_train, dim3(g), dim3(64), 0, c->ctx->stream, A);
+    *rows = g;
+    return launch_check(c->ctx, "k_mlp_train");
+}
+
+int launch_stats(vf_gicache *c, int rows, float *loss_out) {
+    StatsArgs S;
+    S.loss_part = c->loss_part;
+    S.valid_part = c->valid_part;
+    S.nparts = rows;
+    S.st = c->st;
+    S.qcount = c->qcount;
+    S.loss_out = loss_out;
+    S.warmup = c->p.warmup_frames;
+    S.ratio = c->p.fallback_ratio;
+    hipLaunchKernelGGL(k_gc_stats, dim3(1), dim3(64), 0, c->ctx->stream, S);
+    return launch_check(c->ctx, "k_gc_stats");
+}
+
+void init_weights(const vf_gicache_params &p, std::vector<float> &w) {
+    w.assign(kP, 0.0f);
+    uint32_t st = host_pcg(p.seed ^ 0x5eedu);
+    for (int l = 0; l < 6; ++l) {
+        const int fan_in = l == 0 ? 78 : 64;
+        float bound = std::sqrt(6.0f / (float)fan_in);
+        if (l == 5) bound *= 0.25f;
+        for (int i = 0; i < in_dim(l); ++i)
+            for (int o = 0; o < out_dim(l); ++o) {
+                st = host_pcg(st);
+                const float u = (float)(st >> 8u) * (1.0f / 16777216.0f);
+                w[w_off(l) + i * out_dim(l) + o] = (l == 0 && i >= 78) ? 0.0f : (2.0f * u - 1.0f) * bound;
+            }
+    }
+}
+
+int upload_params(vf_ctx *ctx, vf_gicache *c, const float *host) {
+    VF_HIP(ctx, hipMemcpyAsync(c->w, host, sizeof(float) * kP, hipMemcpyHostToDevice, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(c->m, 0, sizeof(float) * kP, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(c->v, 0, sizeof(float) * kP, ctx->stream));
+    c->steps = 0;
+    int rc = launch_adam(c, 0, nullptr, nullptr, 0);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int reset_stats(vf_ctx *ctx, vf_gicache *c) {
+    DevStats s{};
+    s.baseline = INFINITY;
+    s.fallback = 1;
+    VF_HIP(ctx, hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(c->qcount, 0, sizeof(unsigned), ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int check_params(vf_ctx *ctx, const vf_gicache_params *p) {
+    if (p->terminate != 0 && p->terminate != 1) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: terminate must be 0 or 1");
+    if (p->train_steps < 1 || p->train_steps > 64) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: train_steps must be in [1, 64]");
+    if (p->train_samples < 64 || p->train_samples > (1 << 22) || p->train_samples % (64 * p->train_steps))
+        return vf::fail(ctx, VF_E_INVALID, "vf_gicache: train_samples %d must be a multiple of 64 x train_steps in [64, 2^22]",
+                        p->train_samples);
+    if (p->spp < 1 || p->spp > 8) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: spp must be in [1, 8]");
+    if (!(p->lr > 0.0f) || !(p->beta1 >= 0.0f && p->beta1 < 1.0f) || !(p->beta2 >= 0.0f && p->beta2 < 1.0f) ||
+        !(p->eps > 0.0f) || !(p->loss_eps > 0.0f) || !(p->fallback_ratio >= 1.0f))
+        return vf::fail(ctx, VF_E_INVALID, "vf_gicache: optimizer or loss parameter out of range");
+    if (p->warmup_frames < 0 || p->enc_shift < 0 || p->enc_shift > 24)
+        return vf::fail(ctx, VF_E_INVALID, "vf_gicache: warmup_frames or enc_shift out of range");
+    if (!(p->gi_reach > 0.0f) || p->gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: gi_reach out of range");
+    return VF_OK;
+}
+
+int check_gbuffer(vf_ctx *ctx, const char *what, const vf_pass_ubo *pass, int32_t w, int32_t h, const float *gb_pos,
+                  const uint32_t *gb_normal) {
+    int rc = vf::check_ready(ctx, what);
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "%s: no scene (vf_set_scene)", what);
+    if (!pass || !gb_pos || !gb_normal) return gc_fail_null(ctx, what);
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 26)) return vf::fail(ctx, VF_E_INVALID, "%s: bad size %dx%d", what, w, h);
+    return VF_OK;
+}
+
+PathArgs path_args(const vf_pass_ubo *pass, int32_t w, int32_t h, uint32_t frame, int32_t spp, uint32_t salt, float reach,
+                   int32_t nee, int32_t terminate) {
+    PathArgs A;
+    A.w = w; A.h = h;
+    A.tiles_x = (w + 15) / 16;
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    A.spp = spp;
+    A.terminate = terminate;
+    A.frame = frame;
+    A.salt = salt;
+    A.G.reach = reach;
+    A.G.lights_nee = nee ? 1 : 0;
+    for (int a = 0; a < 3; ++a) A.cam[a] = pass->cam_pos[a];
+    return A;
+}
+
+int launch_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t w, int32_t h, const float *gb_pos,
+                   const uint32_t *gb_normal, uint32_t frame, float4 *out) {
+    SampleArgs A;
+    A.w = w; A.h = h;
+    A.n = c->p.train_samples;
+    A.terminate = c->p.terminate;
+    A.frame = frame;
+    A.salt = c->p.salt;
+    A.G.reach = c->p.gi_reach;
+    A.G.lights_nee = c->p.lights_nee ? 1 : 0;
+    for (int a = 0; a < 3; ++a) A.cam[a] = pass->cam_pos[a];
+    hipLaunchKernelGGL(k_gc_samples, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
+    return launch_check(ctx, "k_gc_samples");
+}
+
+}  // namespace
+
+extern "C" {
+
+int vf_gicache_defaults(vf_gicache_params *p) {
+    if (!p) return VF_E_INVALID;
+    std::memset(p, 0, sizeof *p);
+    p->terminate = 1;
+    p->train_samples = 65536;
+    p->train_steps = 4;
+    p->spp = 1;
+    p->lr = 1e-3f;
+    p->beta1 = 0.9f;
+    p->beta2 = 0.99f;
+    p->eps = 1e-8f;
+    p->loss_eps = 0.01f;
+    p->fallback_ratio = 2.0f;
+    p->warmup_frames = 16;
+    p->enc_shift = 8;
+    p->seed = 1337u;
+    p->salt = 0x2f6b9a31u;
+    p->gi_reach = 48.0f;
+    p->lights_nee = 1;
+    return VF_OK;
+}
+
+int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out) {
+    if (!ctx) return VF_E_INVALID;
+    if (!p || !out) return gc_fail_null(ctx, "vf_gicache_create");
+    *out = nullptr;
+    int rc = check_params(ctx, p);
+    if (rc) return rc;
+    if ((rc = vf::no_capture(ctx, "vf_gicache_create"))) return rc;
+    if ((rc = vf::set_device(ctx))) return rc;
+    vf_gicache *c = new (std::nothrow) vf_gicache();
+    if (!c) return vf::fail(ctx, VF_E_OOM, "vf_gicache_create: out of host memory");
+    c->ctx = ctx;
+    c->p = *p;
+    c->part_rows = 2 * ctx->n_cu;
+    hipError_t e = hipSuccess;
+#define VF_GC_ALLOC(ptr, bytes)                                              \
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes))
+    VF_GC_ALLOC(c->w, sizeof(float) * kP);
+    VF_GC_ALLOC(c->m, sizeof(float) * kP);
+    VF_GC_ALLOC(c->v, sizeof(float) * kP);
+    VF_GC_ALLOC(c->grad, sizeof(float) * kP);
+    VF_GC_ALLOC(c->ffrag, sizeof(bf16) * kFwdFrags * 512);
+    VF_GC_ALLOC(c->bfrag, sizeof(bf16) * kBwdFrags * 512);
+    VF_GC_ALLOC(c->part, sizeof(float) * (size_t)kP * c->part_rows);
+    VF_GC_ALLOC(c->loss_part, sizeof(float) * c->part_rows);
+    VF_GC_ALLOC(c->valid_part, sizeof(int32_t) * c->part_rows);
+    VF_GC_ALLOC(c->samples, sizeof(float4) * 4 * (size_t)p->train_samples);
+    VF_GC_ALLOC(c->st, sizeof(DevStats));
+    VF_GC_ALLOC(c->qcount, sizeof(unsigned));
+#undef VF_GC_ALLOC
+    if (e == hipSuccess) e = hipMemsetAsync(c->ffrag, 0, sizeof(bf16) * kFwdFrags * 512, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->bfrag, 0, sizeof(bf16) * kBwdFrags * 512, ctx->stream);
+    if (e != hipSuccess) {
+        free_all(c);
+        delete c;
+        return vf::fail(ctx, e == hipErrorOutOfMemory ? VF_E_OOM : VF_E_DEVICE, "vf_gicache_create: %s", hipGetErrorString(e));
+    }
+    if ((rc = vf_gicache_reset(ctx, c))) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return VF_OK;
+}
+
+void vf_gicache_destroy(vf_gicache *c) {
+    if (!c) return;
+    if (c->ctx) {
+        (void)vf::set_device(c->ctx);
+        (void)hipStreamSynchronize(c->ctx->stream);
+    }
+    free_all(c);
+    delete c;
+}
+
+int vf_gicache_reset(vf_ctx *ctx, vf_gicache *c) {
+    int rc = check_cache(ctx, c, "vf_gicache_reset");
+    if (rc) return rc;
+    if ((rc = vf::no_capture(ctx, "vf_gicache_reset"))) return rc;
+    std::vector<float> w;
+    init_weights(c->p, w);
+    if ((rc = upload_params(ctx, c, w.data()))) return rc;
+    return reset_stats(ctx, c);
+}
+
+int vf_trace_gi_path(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h, float gi_reach, int32_t lights_nee,
+                     const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt,
+                     float *out_rgba) {
+    int rc = check_gbuffer(ctx, "vf_trace_gi_path", pass, w, h, gb_pos, gb_normal);
+    if (rc) return rc;
+    if (!out_rgba) return gc_fail_null(ctx, "vf_trace_gi_path");
+    if (spp < 1 || spp > 4096) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: spp %d out of [1, 4096]", spp);
+    if (!(gi_reach > 0.0f) || gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: gi_reach out of range");
+    const PathArgs A = path_args(pass, w, h, frame, spp, salt, gi_reach, lights_nee, 1);
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    hipLaunchKernelGGL(k_gi_plain, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
+    if ((rc = launch_check(ctx, "k_gi_plain"))) return rc;
+    return vf::end_timed(ctx);
+}
+
+int vf_gicache_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t w, int32_t h, const float *gb_pos,
+                       const uint32_t *gb_normal, uint32_t frame, vf_gi_record *out) {
+    int rc = check_cache(ctx, c, "vf_gicache_samples");
+    if (rc) return rc;
+    if ((rc = check_gbuffer(ctx, "vf_gicache_samples", pass, w, h, gb_pos, gb_normal))) return rc;
+    if (!out) return gc_fail_null(ctx, "vf_gicache_samples");
+    return launch_samples(ctx, c, pass, w, h, gb_pos, gb_normal, frame, reinterpret_cast<float4 *>(out));
+}
+
+int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t w, int32_t h, const float *gb_pos,
+                     const uint32_t *gb_normal, uint32_t frame, float *out_rgba) {
+    int rc = check_cache(ctx, c, "vf_gicache_frame");
+    if (rc) return rc;
+    if ((rc = check_gbuffer(ctx, "vf_gicache_frame", pass, w, h, gb_pos, gb_normal))) return rc;
+    if (!out_rgba) return gc_fail_null(ctx, "vf_gicache_frame");
+    const int64_t npix = (int64_t)w * h, nslots = npix * (c->p.terminate ? c->p.spp : 1);
+    if (npix > c->cap_pix || nslots > c->cap_slots) {
+        if ((rc = vf::no_capture(ctx, "vf_gicache_frame (first frame of a new size)"))) return rc;
+        VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (c->queries) (void)hipFree(c->queries);
+        if (c->pre) (void)hipFree(c->pre);
+        if (c->tailv) (void)hipFree(c->tailv);
+        c->queries = c->pre = c->tailv = nullptr;
+        c->cap_pix = c->cap_slots = 0;
+        VF_HIP(ctx, hipMalloc(reinterpret_cast<void **>(&c->queries), sizeof(float4) * 4 * (size_t)nslots));
+        VF_HIP(ctx, hipMalloc(reinterpret_cast<void **>(&c->pre), sizeof(float4) * (size_t)npix));
+        VF_HIP(ctx, hipMalloc(reinterpret_cast<void **>(&c->tailv), sizeof(float4) * (size_t)nslots));
+        c->cap_pix = npix;
+        c->cap_slots = nslots;
+    }
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    // 1. fresh samples, 2. train_steps Adam steps (the first step's loss feeds the running error)
+    if ((rc = launch_samples(ctx, c, pass, w, h, gb_pos, gb_normal, frame, c->samples))) return rc;
+    const long long per = c->p.train_samples / c->p.train_steps;
+    for (int s = 0; s < c->p.train_steps; ++s) {
+        int rows = 0;
+        if ((rc = launch_train(c, c->samples + (size_t)s * per * 4, per, &rows))) return rc;
+        if (s == 0 && (rc = launch_stats(c, rows, nullptr))) return rc;
+        if ((rc = launch_adam(c, 2, nullptr, nullptr, rows))) return rc;
+    }
+    // 3. render: traced part + queries, the network on the queries, resolve
+    const int32_t spp = c->p.terminate ? c->p.spp : 1;
+    const PathArgs A = path_args(pass, w, h, frame, c->p.spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate);
+    PathArgs R = A;
+    R.spp = c->p.terminate ? c->p.spp : A.spp;
+    hipLaunchKernelGGL(k_gc_render, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, R,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
+                       c->pre, c->tailv, c->queries, c->qcount);
+    if ((rc = launch_check(ctx, "k_gc_render"))) return rc;
+    InferArgs I;
+    I.recs = c->queries;
+    I.n = 0;
+    I.count = c->qcount;
+    I.shift = c->p.enc_shift;
+    I.ffrag = reinterpret_cast<const bf16x8 *>(c->ffrag);
+    I.w = c->w;
+    I.out_h = nullptr;
+    I.tailv = c->tailv;
+    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(blocks_for(nslots, 64)), dim3(64), 0, ctx->stream, I);
+    if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
+    hipLaunchKernelGGL(k_gc_resolve, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, c->st, (int32_t)npix, spp, c->pre,
+                       c->tailv, reinterpret_cast<float4 *>(out_rgba));
+    if ((rc = launch_check(ctx, "k_gc_resolve"))) return rc;
+    return vf::end_timed(ctx);
+}
+
+int vf_gicache_read_stats(vf_ctx *ctx, vf_gicache *c, vf_gicache_stats *out) {
+    int rc = check_cache(ctx, c, "vf_gicache_read_stats");
+    if (rc) return rc;
+    if (!out) return gc_fail_null(ctx, "vf_gicache_read_stats");
+    if ((rc = vf::no_capture(ctx, "vf_gicache_read_stats"))) return rc;
+    DevStats s;
+    unsigned q = 0;
+    VF_HIP(ctx, hipMemcpyAsync(&s, c->st, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipMemcpyAsync(&q, c->qcount, sizeof q, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::memset(out, 0, sizeof *out);
+    out->frames = s.frames;
+    out->steps = c->steps;
+    out->last_loss = s.last_loss;
+    out->running_error = s.running_error;
+    out->baseline = s.baseline;
+    out->fallback = s.fallback;
+    out->n_queries = (int32_t)q;
+    out->n_valid_samples = s.n_valid;
+    return VF_OK;
+}
+
+int vf_gicache_get_params(vf_ctx *ctx, vf_gicache *c, float *host_out) {
+    int rc = check_cache(ctx, c, "vf_gicache_get_params");
+    if (rc) return rc;
+    if (!host_out) return gc_fail_null(ctx, "vf_gicache_get_params");
+    if ((rc = vf::no_capture(ctx, "vf_gicache_get_params"))) return rc;
+    VF_HIP(ctx, hipMemcpyAsync(host_out, c->w, sizeof(float) * kP, hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VF_OK;
+}
+
+int vf_gicache_set_params(vf_ctx *ctx, vf_gicache *c, const float *host_in) {
+    int rc = check_cache(ctx, c, "vf_gicache_set_params");
+    if (rc) return rc;
+    if (!host_in) return gc_fail_null(ctx, "vf_gicache_set_params");
+    if ((rc = vf::no_capture(ctx, "vf_gicache_set_params"))) return rc;
+    return upload_params(ctx, c, host_in);
+}
+
+int vf_gicache_encode(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64_t n, float *enc) {
+    int rc = check_cache(ctx, c, "vf_gicache_encode");
+    if (rc) return rc;
+    if (n < 0 || n > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_gicache_encode: bad count");
+    if (n == 0) return VF_OK;
+    if (!recs || !enc) return gc_fail_null(ctx, "vf_gicache_encode");
+    hipLaunchKernelGGL(k_mlp_encode, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       reinterpret_cast<const float4 *>(recs), (long long)n, c->p.enc_shift, enc);
+    return launch_check(ctx, "k_mlp_encode");
+}
+
+int vf_gicache_infer(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64_t n, uint16_t *out_rgbh) {
+    int rc = check_cache(ctx, c, "vf_gicache_infer");
+    if (rc) return rc;
+    if (n < 0 || n > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_gicache_infer: bad count");
+    if (n == 0) return VF_OK;
+    if (!recs || !out_rgbh) return gc_fail_null(ctx, "vf_gicache_infer");
+    InferArgs I;
+    I.recs = reinterpret_cast<const float4 *>(recs);
+    I.n = n;
+    I.count = nullptr;
+    I.shift = c->p.enc_shift;
+    I.ffrag = reinterpret_cast<const bf16x8 *>(c->ffrag);
+    I.w = c->w;
+    I.out_h = out_rgbh;
+    I.tailv = nullptr;
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    hipLaunchKernelGGL(k_mlp_infer<0>, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream, I);
+    if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
+    return vf::end_timed(ctx);
+}
+
+int vf_gicache_grad(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *samples, int64_t n, float *grad, float *loss) {
+    int rc = check_cache(ctx, c, "vf_gicache_grad");
+    if (rc) return rc;
+    if (n <= 0 || n > (1ll << 26)) return vf::fail(ctx, VF_E_INVALID, "vf_gicache_grad: bad count");
+    if (!samples || !grad || !loss) return gc_fail_null(ctx, "vf_gicache_grad");
+    int rows = 0;
+    if ((rc = launch_train(c, reinterpret_cast<const float4 *>(samples), n, &rows))) return rc;
+    if ((rc = launch_adam(c, 1, nullptr, grad, rows))) return rc;
+    return launch_stats(c, rows, loss);
+}
+
+int vf_gicache_adam(vf_ctx *ctx, vf_gicache *c, const float *grad) {
+    int rc = check_cache(ctx, c, "vf_gicache_adam");
+    if (rc) return rc;
+    if (!grad) return gc_fail_null(ctx, "vf_gicache_adam");
+    return launch_adam(c, 2, grad, nullptr, 0);
+}
+
+int vf_gicache_train(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *samples, int64_t n) {
+    int rc = check_cache(ctx, c, "vf_gicache_train");
+    if (rc) return rc;
+    if (n <= 0 || n > (1ll << 26)) return vf::fail(ctx, VF_E_INVALID, "vf_gicache_train: bad count");
+    if (!samples) return gc_fail_null(ctx, "vf_gicache_train");
+    int rows = 0;
+    if ((rc = vf::begin_timed(ctx))) return rc;
+    if ((rc = launch_train(c, reinterpret_cast<const float4 *>(samples), n, &rows))) return rc;
+    if ((rc = launch_adam(c, 2, nullptr, nullptr, rows))) return rc;
+    return vf::end_timed(ctx);
+}
+
+}  // extern "C"

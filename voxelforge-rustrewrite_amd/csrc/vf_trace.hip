@@ -29,6 +29,7 @@
 #include "../../include/vf_trace.h"
 #include "vf_ctx.h"
 #include "vf_device.h"
+#include "vf_gi.h"
 #include "vf_internal.h"
 
 // ------------------------------------------------------------------------------------------------
@@ -198,89 +199,13 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
     flush_count<COUNT>(c, cnt);
 }
 
-// skyOpen with colLoaded (common.glsl:232-258); a bounded world has no horizon (outside -> 1).
-__device__ __forceinline__ float sky_open(const DevWorld &W, const float p[3]) {
-    if (p[1] < 0.0f) return 0.0f;
-    const int32_t bx = f2i(floorf(p[0] / 8.0f)), bz = f2i(floorf(p[2] / 8.0f));
-    const bool inside = bx >= W.minb[0] && bz >= W.minb[2] && bx < W.minb[0] + W.wb[0] && bz < W.minb[2] + W.wb[2];
-    if (!inside || W.cells[cell_index(W, bx, 0, bz)].x == 0u) return 1.0f;
-    const int32_t sxn = W.wb[0] * 8, szn = W.wb[2] * 8;
-    const int32_t cx = f2i(floorf(p[0])), cz = f2i(floorf(p[2]));
-    const int32_t ix = ((cx % sxn) + sxn) % sxn, iz = ((cz % szn) + szn) % szn;
-    const float top = (float)W.skytop[(size_t)iz * (size_t)sxn + (size_t)ix];
-    const float e0 = top - 0.5f, e1 = top + 1.0f;
-    const float tt = fclampg((p[1] - e0) / (e1 - e0), 0.0f, 1.0f);
-    return tt * tt * (3.0f - 2.0f * tt);
-}
-
-// giEscape (common.glsl:2131-2148)
-__device__ float gi_escape(const DevWorld &W, const float o[3], const float d[3], float reach) {
-    float lo[3], hi[3], tf[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = (float)(W.minb[a] * 8);
-        hi[a] = (float)((W.minb[a] + W.wb[a]) * 8);
-        const float s = fabsf(d[a]) < 1e-6f ? 1e-6f : d[a];
-        const float t1 = (lo[a] - o[a]) / s, t2 = (hi[a] - o[a]) / s;
-        tf[a] = gmax(t1, t2);
-    }
-    const float tExit = gmax(gmin(gmin(tf[0], tf[1]), tf[2]), 0.0f);
-    if (tExit > reach) {
-        float e[3], e1[3], e2[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) e[a] = o[a] + d[a] * reach;
-        float so = sky_open(W, e);
-        if (so > 0.0f) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) { e1[a] = e[a] + d[a] * 16.0f; e2[a] = e[a] + d[a] * 32.0f; }
-            so *= sky_open(W, e1) * sky_open(W, e2);
-        }
-        return so;
-    }
-    float f[3], f2[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) f[a] = o[a] + d[a] * tExit;
-    if (f[1] <= lo[1] + 1.0f) return 0.0f;
-    if (f[1] >= hi[1] - 1.0f) return 1.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) f2[a] = f[a] + d[a] * 2.0f;
-    return sky_open(W, f2);
-}
-
-// skyColor(rd, sunDisk 0) (common.glsl:1082-1099)
-__device__ void sky_color0(const SkyParams &S, const float rd[3], float out[3]) {
-    const float horizon = fclampg(rd[1] * 0.5f + 0.5f, 0.0f, 1.0f);
-    const float mu = gmax(dot3(rd, S.sun_dir), 0.0f);
-    const float glow = powf(mu, 8.0f) * (1.0f - 0.9f * S.night);
-    const float ph = powf(horizon, 0.7f);
-    const float tint[3] = {1.0f, 0.9f, 0.7f};
-    if (S.sky_model == 0) {
-        const float a[3] = {0.75f, 0.82f, 0.90f}, b[3] = {0.25f, 0.45f, 0.85f}, nt[3] = {0.012f, 0.02f, 0.045f};
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            float sky = a[i] * (1.0f - ph) + b[i] * ph;
-            sky = sky * (1.0f - S.night) + nt[i] * S.night;
-            out[i] = (sky + tint[i] * (glow * 0.25f) * (1.0f - 0.7f * S.night)) * S.sky_scale;
-        }
-        return;
-    }
-    const float m = gmax(gmax(S.sun_rgb[0], S.sun_rgb[1]), 1e-3f);
-    const float k1 = powf(mu, 3.0f) * expf(-gmax(rd[1], 0.0f) * 5.0f) + 0.25f * expf(-fabsf(rd[1]) * 9.0f);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float sky = S.sky_horizon[i] * (1.0f - ph) + S.sky_zenith[i] * ph;
-        sky += S.sunset_rgb[i] * k1;
-        out[i] = (sky + (S.sun_rgb[i] / m) * tint[i] * (glow * 0.25f) * (1.0f - 0.7f * S.night)) * S.sky_scale;
-    }
-}
-
 struct GiArgs {
     int32_t w, h, ow, oh, tiles_x, ntiles, half_res, lights_nee;
     uint32_t frame;
     float reach;
 };
 
-// gi.comp (gi.comp:34-84) + giPathImpl with one bounce (common.glsl:2250-2358).
+// gi.comp (gi.comp:34-84) + giPathImpl with one bounce (common.glsl:2250-2366).
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, const float4 *gbpos, const uint32_t *gbn,
                                             float4 *radiance, vf_hit *bounce, unsigned long long *cnt) {
@@ -316,83 +241,14 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, c
         const float4 P4 = gbpos[fi];
         const uint32_t nb = gbn[fi];
         const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
-        // ldSample2(fpix, frame >> 2, 0x68e31da4) (common.glsl:1284-1288)
-        const uint32_t salt = 0x68e31da4u;
-        const uint32_t hh = pcg((uint32_t)fx + pcg((uint32_t)fy + salt));
-        const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
-        const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
-        const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
-        const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
-        float rdir[3], ro[3];
+        float u0, u1;
+        ld_sample2(fx, fy, A.frame >> 2u, 0x68e31da4u, u0, u1);   // gi.comp:70-72
+        float rdir[3];
         cosine_hemisphere_u(N, u0, u1, rdir);
         const float P[3] = {P4.x, P4.y, P4.z};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ro[a] = P[a] + N[a] * 3e-2f;
-        float hitAlb[3] = {0.0f, 0.0f, 0.0f}, hitNdl = 0.0f, reach = A.reach;
-        int stage = 0;
-        bool extended = false;
-        for (int it = 0; it < 6; ++it) {
-            const bool sunRay = (stage & 1) == 1;
-            Hit h;
-            trace_world<0, COUNT>(W, ro, rdir, sunRay ? 1e30f : reach, h, c);
-            if (sunRay) {
-                if (!h.hit) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) Lr[a] += hitAlb[a] * S.sun_rgb[a] * hitNdl;
-                }
-                break;
-            }
-            if (stage == 0) bh = h;
-            if (h.hit && h.mat == 5u) h.hit = 0;
-            if (!h.hit) {
-                const float esc = gi_escape(W, ro, rdir, reach);
-                if (esc < 0.999f && !extended && stage == 0 && rdir[1] > 0.3f) {
-                    float e[3], p1[3], p2[3], p3[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        e[a] = ro[a] + rdir[a] * reach;
-                        p1[a] = e[a] + rdir[a] * 32.0f;
-                        p2[a] = e[a] + rdir[a] * 64.0f;
-                        p3[a] = e[a] + rdir[a] * 96.0f;
-                    }
-                    const float m = gmax(sky_open(W, p1), gmax(sky_open(W, p2), sky_open(W, p3)));
-                    if (m > 0.0f) {
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) ro[a] = e[a];
-                        reach = 96.0f;
-                        extended = true;
-                        continue;
-                    }
-                }
-                float skyc[3];
-                sky_color0(S, rdir, skyc);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) Lr[a] += gmin(skyc[a], 3.0f) * esc;
-                break;
-            }
-            const float4 alb4 = W.palette[h.mat * 3u];
-            float emit = alb4.w * 8.0f;
-            if (A.lights_nee && alb4.w >= 0.5f) emit = 0.0f;
-            const float alb[3] = {alb4.x, alb4.y, alb4.z};
-            float hitN[3], hitP[3];
-            face_normal(h.face, hitN);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                Lr[a] += alb[a] * emit;
-                hitP[a] = ro[a] + rdir[a] * h.t;
-                hitAlb[a] = alb[a];
-            }
-            reach = A.reach;
-            extended = false;
-            hitNdl = gmax(dot3(hitN, S.sun_dir), 0.0f);
-            if (hitNdl > 0.0f) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) { ro[a] = hitP[a] + hitN[a] * 3e-2f; rdir[a] = S.sun_dir[a]; }
-                stage += 1;
-            } else {
-                break;
-            }
-        }
+        const GiPathCfg G = {A.reach, A.lights_nee};
+        uint32_t seed = 0u;
+        gi_path<1, false, COUNT>(W, S, G, P, N, rdir, seed, true, Lr, &bh, nullptr, c);
     }
     radiance[oi] = make_float4(Lr[0], Lr[1], Lr[2], sky ? 0.0f : 1.0f);
     if (bounce) store_hit(bounce + oi, bh);

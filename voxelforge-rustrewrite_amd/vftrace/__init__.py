@@ -13,12 +13,13 @@ import os
 
 import numpy as np
 
-from .abi import (GenParams, Hit, HIT_DTYPE, Material, PassUbo, QueueCfg, Ray, RAY_DTYPE, Request, SceneUbo,
-                  TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH, VF_TUNE_GI_BATCH,
-                  WorldDesc)
+from .abi import (GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams, GiCacheStats, Hit,
+                  HIT_DTYPE, Material, PassUbo, QueueCfg, Ray, RAY_DTYPE, Request, SceneUbo, TuneBatchArgs, TuneGiArgs,
+                  TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH, VF_TUNE_GI_BATCH, WorldDesc)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
-           "RAY_DTYPE", "fly_camera", "sun_dir", "make_pass", "default_scene", "library_path"]
+           "RAY_DTYPE", "GI_RECORD_DTYPE", "GiCache", "fly_camera", "sun_dir", "make_pass", "default_scene",
+           "library_path"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -78,6 +79,22 @@ _SIGS = {
     "vf_set_counting": (C.c_int, [_P, C.c_int32]),
     "vf_read_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32]),
     "vf_last_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "vf_gicache_defaults": (C.c_int, [C.POINTER(GiCacheParams)]),
+    "vf_gicache_create": (C.c_int, [_P, C.POINTER(GiCacheParams), C.POINTER(_P)]),
+    "vf_gicache_destroy": (None, [_P]),
+    "vf_gicache_reset": (C.c_int, [_P, _P]),
+    "vf_trace_gi_path": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
+                                   C.c_uint32, C.c_int32, C.c_uint32, _P]),
+    "vf_gicache_frame": (C.c_int, [_P, _P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, C.c_uint32, _P]),
+    "vf_gicache_read_stats": (C.c_int, [_P, _P, C.POINTER(GiCacheStats)]),
+    "vf_gicache_samples": (C.c_int, [_P, _P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, C.c_uint32, _P]),
+    "vf_gicache_get_params": (C.c_int, [_P, _P, _P]),
+    "vf_gicache_set_params": (C.c_int, [_P, _P, _P]),
+    "vf_gicache_encode": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "vf_gicache_infer": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "vf_gicache_grad": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
+    "vf_gicache_adam": (C.c_int, [_P, _P, _P]),
+    "vf_gicache_train": (C.c_int, [_P, _P, _P, C.c_int64]),
     "vf_gen_defaults": (C.c_int, [C.c_int32, C.POINTER(GenParams)]),
     "vf_gen_world_create": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
     "vf_gen_world_from_dense": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, C.POINTER(_P)]),
@@ -449,6 +466,13 @@ class Context:
         table = [((r.cfg.tile_w, r.cfg.tile_h), r.cfg.chunk, bool(r.cfg.sort), r.ms) for r in res[:n.value]]
         return best, table
 
+    def trace_gi_path(self, pass_ubo, width, height, gb_pos, gb_normal, out_rgba, frame=0, spp=1, salt=0x68e31da4,
+                      gi_reach=48.0, lights_nee=True):
+        """Plain traced two-bounce GI (mean of spp paths per pixel) into out_rgba (float32, 4 per pixel)."""
+        self._ck(lib().vf_trace_gi_path(self._h, C.byref(pass_ubo), int(width), int(height), float(gi_reach),
+                                        int(bool(lights_nee)), _ptr(gb_pos), _ptr(gb_normal), int(frame) & 0xFFFFFFFF,
+                                        int(spp), int(salt) & 0xFFFFFFFF, _ptr(out_rgba)))
+
     # frame graphs
     def graph_begin(self):
         self._ck(lib().vf_graph_begin(self._h))
@@ -474,6 +498,84 @@ class Context:
         ms = C.c_float()
         self._ck(lib().vf_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+
+class GiCache:
+    """The learned GI radiance cache (include/vf_gicache.h) on a Context."""
+
+    N_PARAMS = GICACHE_PARAMS
+    N_INPUTS = GICACHE_INPUTS
+
+    @staticmethod
+    def defaults():
+        p = GiCacheParams()
+        _check(lib().vf_gicache_defaults(C.byref(p)))
+        return p
+
+    def __init__(self, ctx, params=None, **overrides):
+        p = params if params is not None else GiCache.defaults()
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        self.params = p
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._ck(lib().vf_gicache_create(ctx._h, C.byref(p), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().vf_gicache_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        self.ctx._ck(rc)
+
+    def reset(self):
+        self._ck(lib().vf_gicache_reset(self.ctx._h, self._h))
+
+    def frame(self, pass_ubo, width, height, gb_pos, gb_normal, out_rgba, frame):
+        self._ck(lib().vf_gicache_frame(self.ctx._h, self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
+                                        _ptr(gb_normal), int(frame) & 0xFFFFFFFF, _ptr(out_rgba)))
+
+    def stats(self):
+        s = GiCacheStats()
+        self._ck(lib().vf_gicache_read_stats(self.ctx._h, self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in GiCacheStats._fields_}
+
+    def samples(self, pass_ubo, width, height, gb_pos, gb_normal, frame, out):
+        self._ck(lib().vf_gicache_samples(self.ctx._h, self._h, C.byref(pass_ubo), int(width), int(height),
+                                          _ptr(gb_pos), _ptr(gb_normal), int(frame) & 0xFFFFFFFF, _ptr(out)))
+
+    def get_params(self):
+        out = np.zeros(GICACHE_PARAMS, np.float32)
+        self._ck(lib().vf_gicache_get_params(self.ctx._h, self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == GICACHE_PARAMS
+        self._ck(lib().vf_gicache_set_params(self.ctx._h, self._h, w.ctypes.data))
+
+    def encode(self, recs, n, out):
+        self._ck(lib().vf_gicache_encode(self.ctx._h, self._h, _ptr(recs), int(n), _ptr(out)))
+
+    def infer(self, recs, n, out_h):
+        self._ck(lib().vf_gicache_infer(self.ctx._h, self._h, _ptr(recs), int(n), _ptr(out_h)))
+
+    def grad(self, samples, n, grad, loss):
+        self._ck(lib().vf_gicache_grad(self.ctx._h, self._h, _ptr(samples), int(n), _ptr(grad), _ptr(loss)))
+
+    def adam(self, grad):
+        self._ck(lib().vf_gicache_adam(self.ctx._h, self._h, _ptr(grad)))
+
+    def train(self, samples, n):
+        self._ck(lib().vf_gicache_train(self.ctx._h, self._h, _ptr(samples), int(n)))
 
 
 class Graph:

@@ -1,4 +1,4 @@
-"""ctypes mirrors of the C ABI structs (include/vf_trace.h, include/vf_worldgen.h).
+"""ctypes mirrors of the C ABI structs (include/vf_trace.h, include/vf_worldgen.h, include/vf_gicache.h).
 
 The field order and sizes follow the reference's #[repr(C)] / std140 structs: SceneUbo and PassUbo
 (src/gfx/ubo.rs:10-131, offsets asserted at :76-86 and :132-139) and Material
@@ -121,6 +121,23 @@ class TuneBatchArgs(C.Structure):
     _fields_ = [("reqs", C.c_void_p), ("n_req", C.c_int32)]
 
 
+class GiCacheParams(C.Structure):
+    _fields_ = [("terminate", C.c_int32), ("train_samples", C.c_int32), ("train_steps", C.c_int32), ("spp", C.c_int32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("loss_eps", C.c_float), ("fallback_ratio", C.c_float), ("warmup_frames", C.c_int32),
+                ("enc_shift", C.c_int32), ("seed", C.c_uint32), ("salt", C.c_uint32), ("gi_reach", C.c_float),
+                ("lights_nee", C.c_int32)]
+
+
+class GiCacheStats(C.Structure):
+    _fields_ = [("frames", C.c_int64), ("steps", C.c_int64), ("last_loss", C.c_float), ("running_error", C.c_float),
+                ("baseline", C.c_float), ("fallback", C.c_int32), ("n_queries", C.c_int32),
+                ("n_valid_samples", C.c_int32)]
+
+
+GICACHE_PARAMS = 23043
+GICACHE_INPUTS = 96
+
 VF_REQ_VIEW, VF_REQ_PROBES = 0, 1
 VF_TUNE_GI_BATCH, VF_TUNE_BATCH = 0, 1
 
@@ -128,6 +145,9 @@ VF_TUNE_GI_BATCH, VF_TUNE_BATCH = 0, 1
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("d", "<f4", (3,))])
 HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
                       ("pad", "u1")])
+# 64-byte query / training record (vf_gi_record)
+GI_RECORD_DTYPE = np.dtype([("pos", "<f4", (3,)), ("tag", "<u4"), ("n", "<f4", (3,)), ("pad0", "<f4"),
+                            ("view", "<f4", (3,)), ("pad1", "<f4"), ("rgb", "<f4", (3,)), ("pad2", "<f4")])
 
 # Layout asserts (src/gfx/ubo.rs:76-86, 132-139; src/voxel/world.rs:31).
 assert C.sizeof(SceneUbo) == 1536
@@ -148,3 +168,5 @@ assert PassUbo.facet_p.offset == 272
 assert C.sizeof(Material) == 48
 assert C.sizeof(Ray) == 24 and RAY_DTYPE.itemsize == 24
 assert C.sizeof(Hit) == 20 and HIT_DTYPE.itemsize == 20
+assert GI_RECORD_DTYPE.itemsize == 64
+assert C.sizeof(GiCacheParams) == 64 and C.sizeof(GiCacheStats) == 40
