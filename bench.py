@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the fixed-world, GI-batch and uneven-batch lines")
     ap.add_argument("--no-graph", action="store_true", help="launch the passes eagerly instead of replaying a graph")
+    ap.add_argument("--cache-frames", type=int, default=256, help="online-training frames before the GI cache is judged")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -280,8 +281,9 @@ def graph_ms(ctx, stream, fn, k):
 
 def other_configs(ctx, args):
     """The fixed world (256x128x256 voxels): primary + sun rays at 640x360, the GI batch at 1920x1080
-    (1 GI ray per pixel, reach 640 voxels, 1 sun ray per GI hit; autotuned queue), and one launch of a
-    32x16x32 DDGI probe grid (256 rays each) plus 16 views of mixed sizes."""
+    (1 GI ray per pixel, reach 640 voxels, 1 sun ray per GI hit; autotuned queue), the learned GI
+    cache at 1920x1080, and one launch of a 32x16x32 DDGI probe grid (256 rays each) plus 16 views of
+    mixed sizes."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
@@ -331,6 +333,7 @@ def other_configs(ctx, args):
                                        "refill": best.refill},
                              "autotune_candidates": len(table),
                              "plain_gi_pass_ms": round(ms3p, 5)}
+    out["learned_gi_cache_1080p"] = gi_cache_config(ctx, stream, args, pu3, w3, h3, gbp3, gbn3)
     # uneven batch: probes + 16 views, one launch
     rng = np.random.default_rng(args.seed)
     sizes = [(256, 256), (512, 288), (960, 540), (1024, 1024)]
@@ -370,6 +373,93 @@ def other_configs(ctx, args):
                            "view_sizes": [[vw, vh] for vw, vh, _ in views]}
     ctx.set_stream(None)
     return out
+
+
+def _errors(x, ref, mask):
+    """MSE and relMSE (squared error / (ref^2 + 0.01)) over the non-sky pixels' RGB."""
+    d = (x[mask, :3] - ref[mask, :3]).double()
+    r = ref[mask, :3].double()
+    return float((d * d).mean()), float((d * d / (r * r + 1e-2)).mean())
+
+
+def gi_cache_config(ctx, stream, args, pu, W, H, gbp, gbn):
+    """Learned GI radiance cache on the fixed world at 1920x1080: the two-bounce traced GI at 4 spp
+    against the cached GI (online training on 65,536 traced samples per frame in 4 Adam steps, then
+    the render), both judged by MSE / relMSE against a 64-spp traced reference of the same frame.
+    Each cache variant first trains for --cache-frames frames on a static camera; the timed frame
+    includes its sample tracing, training and render."""
+    dev = torch.device("cuda:0")
+    n = W * H
+    mask = (gbp.view(-1, 4)[:, 3] >= 0.0)
+    ref = torch.zeros(n * 4, dtype=torch.float32, device=dev)
+    part = torch.zeros_like(ref)
+    for c in range(8):   # samples 0..63 of the reference stream, 8 per launch
+        ctx.trace_gi_path(pu, W, H, gbp, gbn, part, frame=c, spp=8, salt=0x5EED)
+        ref += part
+    ref /= 8.0
+    refv = ref.view(-1, 4)
+    frame = args.cache_frames - 1
+    res = {"reference": "64-spp traced two-bounce GI (samples 0..63 of salt 0x5EED)",
+           "pixels_with_gi": int(mask.sum().item())}
+    plain = {}
+    for spp in (1, 4):
+        o = torch.zeros_like(ref)
+        ms = graph_ms(ctx, stream, lambda: ctx.trace_gi_path(pu, W, H, gbp, gbn, o, frame=frame, spp=spp,
+                                                             salt=0x2f6b9a31), 20)
+        mse, rel = _errors(o.view(-1, 4), refv, mask)
+        plain[spp] = ms
+        res[f"traced_{spp}spp"] = {"ms_per_frame": round(ms, 4), "mse": round(mse, 6), "relmse": round(rel, 5)}
+    for term in (0, 1):
+        g = vf.GiCache(ctx, terminate=term)
+        o = torch.zeros_like(ref)
+        for f in range(args.cache_frames):
+            g.frame(pu, W, H, gbp, gbn, o, f)
+        st = g.stats()
+        mse, rel = _errors(o.view(-1, 4), refv, mask)
+        ms = graph_ms(ctx, stream, lambda: g.frame(pu, W, H, gbp, gbn, o, frame), 20)
+        key = "cache_at_gbuffer_point" if term == 0 else "cache_after_bounce1"
+        res[key] = {"terminate": term, "ms_per_frame": round(ms, 4), "mse": round(mse, 6), "relmse": round(rel, 5),
+                    "speedup_vs_traced_4spp": round(plain[4] / ms, 3),
+                    "mse_vs_traced_4spp": round(mse / res["traced_4spp"]["mse"], 4),
+                    "frames_trained": args.cache_frames, "fallback": st["fallback"], "queries": st["n_queries"],
+                    "running_error": round(st["running_error"], 5)}
+        if term == 0:
+            # matrix-core rate: the training step and the inference on their own
+            samples = torch.zeros(65536 * 64, dtype=torch.uint8, device=dev)
+            g.samples(pu, W, H, gbp, gbn, frame, samples)
+            ms_t = graph_ms(ctx, stream, lambda: g.train(samples, 65536), 20)
+            q = _random_queries(n, dev)
+            h = torch.zeros(n * 4, dtype=torch.int16, device=dev)
+            ms_i = graph_ms(ctx, stream, lambda: g.infer(q, n, h), 20)
+            fl_t, fl_i = 65536 * 3 * FWD_FLOPS, n * FWD_FLOPS
+            res["mfma"] = {"peak_tflops_bf16_dense": 2500.0,
+                           "train_65536": {"ms": round(ms_t, 4), "tflops": round(fl_t / ms_t / 1e9, 2),
+                                           "frac": round(fl_t / ms_t / 1e9 / 2500.0, 5)},
+                           f"infer_{n}": {"ms": round(ms_i, 4), "tflops": round(fl_i / ms_i / 1e9, 2),
+                                          "frac": round(fl_i / ms_i / 1e9 / 2500.0, 5)},
+                           "flops_per_query": FWD_FLOPS}
+        g.close()
+    log(f"GI cache: traced 4spp {res['traced_4spp']}, cache@P {res['cache_at_gbuffer_point']}, "
+        f"cache@bounce1 {res['cache_after_bounce1']}")
+    return res
+
+
+# forward FLOPs per query: 2 * (96*64 + 4*64*64 + 64*3)
+FWD_FLOPS = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 3)
+
+
+def _random_queries(n, dev):
+    """n query records (64 B each) with positions in the fixed world's box and unit normal / view
+    vectors: the inference cost does not depend on the values."""
+    g = torch.Generator(device="cpu").manual_seed(3)
+    rec = torch.zeros(n, 16, dtype=torch.float32)
+    rec[:, 0:3] = torch.rand(n, 3, generator=g) * torch.tensor([256.0, 128.0, 256.0])
+    nv = torch.randn(n, 3, generator=g)
+    rec[:, 4:7] = nv / nv.norm(dim=1, keepdim=True)
+    vv = torch.randn(n, 3, generator=g)
+    rec[:, 8:11] = vv / vv.norm(dim=1, keepdim=True)
+    rec[:, 12:15] = 1.0
+    return rec.to(dev).view(torch.uint8).view(-1)
 
 
 def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, min_seconds):

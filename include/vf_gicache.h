@@ -76,7 +76,7 @@ typedef struct vf_gicache_stats {
     int32_t n_valid_samples; /* valid training samples of the last frame */
 } vf_gicache_stats;
 
-/* Defaults: terminate 1, 65536 samples in 4 steps, spp 1, Adam (1e-3, 0.9, 0.99, 1e-8), loss_eps 0.01,
+/* Defaults: terminate 0, 65536 samples in 4 steps, spp 1, Adam (1e-3, 0.9, 0.99, 1e-8), loss_eps 0.01,
  * fallback 2x, warmup 16, enc_shift 8, seed 1337, salt 0x2f6b9a31, gi_reach 48, lights_nee 1. */
 int vf_gicache_defaults(vf_gicache_params *p);
 int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out);

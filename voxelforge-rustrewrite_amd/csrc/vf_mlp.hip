@@ -975,7 +975,7 @@ extern "C" {
 int vf_gicache_defaults(vf_gicache_params *p) {
     if (!p) return VF_E_INVALID;
     std::memset(p, 0, sizeof *p);
-    p->terminate = 1;
+    p->terminate = 0;
     p->train_samples = 65536;
     p->train_steps = 4;
     p->spp = 1;
