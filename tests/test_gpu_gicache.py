@@ -3,7 +3,7 @@
 * traced parts (plain two-bounce GI, training samples, the cached render's traced head) against the C
   oracle: positions / normals / directions / tags bit-exact, radiance within rtol 2e-4 (the sky term's
   powf / expf differ in the last bits between the device and host libraries);
-* the network (encoding, inference, gradient, Adam) against oracle/mlp.py: the encoding within 5e-7,
+* the network (encoding, inference, gradient, Adam) against oracle/mlp.py: the encoding within 2e-6,
   inference and gradients against the oracle's BF16-emulating network (same rounding points, FP64 sums)
   to FP32-accumulation error, the gradient's direction against the plain FP64 network, Adam bit for bit;
 * the cache's behaviour: warm-up frames are the traced path exactly, and a trained cache lowers the
@@ -73,7 +73,10 @@ def test_encoding_matches_oracle(net):
     recs["pos"][:300] -= 300.0   # ring worlds have negative absolute coordinates
     got = gpu_encode(net, recs)
     want = M.encode(recs, 8)
-    assert np.abs(got - want).max() < 5e-7
+    # bands after the first of each group of four come from double-angle steps (error < 2e-6)
+    assert np.abs(got - want).max() < 2e-6
+    first = [24 * a + 2 * k + c for a in range(3) for k in (0, 4, 8) for c in (0, 1)]
+    assert np.abs(got[:, first] - want[:, first]).max() < 5e-7
 
 
 def test_inference_matches_emulated_oracle(net):

@@ -53,6 +53,11 @@ __host__ __device__ constexpr int fwd_base(int l) { return l == 0 ? 0 : 12 + (l 
 __host__ __device__ constexpr int bwd_kb(int l) { return l == 5 ? 1 : 2; }
 __host__ __device__ constexpr int bwd_base(int l) { return (l - 1) * 8; }
 constexpr int kFwdFrags = 46, kBwdFrags = 36;
+// The inference kernel keeps activations in registers: the output tile of a layer (lane g, c holds
+// features 16 mt + 4 g + i of sample c) is the next layer's B operand when K slot j of lane group g in
+// K block kb stands for input feature perm(kb, g, j) = (2 kb + j / 4) * 16 + 4 g + j % 4.  Its
+// fragments of layers 1..5 are packed in that K order (layer 0 reads the encoding in natural order).
+__host__ __device__ constexpr int perm_in(int kb, int g, int j) { return (2 * kb + (j >> 2)) * 16 + 4 * g + (j & 3); }
 static_assert(fwd_base(5) + fwd_mt(5) * fwd_kb(5) == kFwdFrags, "forward fragments");
 static_assert(bwd_base(5) + 4 * bwd_kb(5) == kBwdFrags, "backward fragments");
 
@@ -61,7 +66,7 @@ constexpr int kS0 = 104;   // encoded input, 96 features
 constexpr int kSH = 72;    // hidden, 64 features
 constexpr int kSG = 40;    // output gradient, 3 used of 32
 constexpr int kTrainLds = 64 * kS0 + 5 * 64 * kSH + 64 * kSG;   // BF16 elements per training wavefront
-constexpr int kInferLds = 64 * kS0 + 2 * 64 * kSH;
+constexpr int kBiasLds = 5 * 64 + 4;
 
 // sin(pi u), cos(pi u): u - rint(u) is exact, then Taylor polynomials on [-pi/2, pi/2] (terms to x^11
 // and x^12; truncation < 6e-8) and the sign of the half period.
@@ -84,14 +89,26 @@ __device__ __forceinline__ void sincospi(float u, float &s, float &c) {
 }
 
 // Feature f of a record: f = 24 a + 2 k (+1): sin (cos) of pi * pos[a] * 2^(k - shift); 72..74 normal,
-// 75..77 view, 78..95 zero.  Group q (0..11) = features 8q..8q+7.
+// 75..77 view, 78..95 zero.  Group q (0..11) = features 8q..8q+7 = four consecutive bands of one
+// coordinate: the first from sincospi, the next three by the double-angle formulas (sin 2t = 2 sin t
+// cos t, cos 2t = (cos t - sin t)(cos t + sin t)), which at most doubles the error per band (< 2e-6).
 __device__ __forceinline__ void encode_group(const float4 &r0, const float4 &r1, const float4 &r2, int shift, int q,
                                              float f[8]) {
     if (q < 9) {
         const int a = q / 3, k0 = (q % 3) * 4;
         const float x = a == 0 ? r0.x : (a == 1 ? r0.y : r0.z);
+        float s, c;
+        sincospi(ldexpf(x, k0 - shift), s, c);
+        f[0] = s;
+        f[1] = c;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sincospi(ldexpf(x, k0 + j - shift), f[2 * j], f[2 * j + 1]);
+        for (int j = 1; j < 4; ++j) {
+            const float s2 = 2.0f * s * c, c2 = (c - s) * (c + s);
+            s = s2;
+            c = c2;
+            f[2 * j] = s;
+            f[2 * j + 1] = c;
+        }
     } else if (q == 9) {
         f[0] = r1.x; f[1] = r1.y; f[2] = r1.z;
         f[3] = r2.x; f[4] = r2.y; f[5] = r2.z;
@@ -378,7 +395,7 @@ struct AdamArgs {
     float *grad_out;       // mode 1: the reduced gradient
     int32_t mode;          // 0: fragments only; 1: reduce only; 2: reduce (or grad_in) + Adam + fragments
     float lr, b1, b2, c1, c2, bc1, bc2, eps;
-    bf16 *ffrag, *bfrag;
+    bf16 *ffrag, *bfrag, *ifrag;
 };
 
 __device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) {
@@ -391,6 +408,13 @@ __device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) 
     {
         const int mt = out / 16, kb = in / 32, ln = (out & 15) + 16 * ((in & 31) >> 3), i = in & 7;
         A.ffrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
+        if (l == 0) {
+            A.ifrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
+        } else {   // in = perm_in(kb, g, j)
+            const int t = in >> 4, r = in & 15;
+            const int kb2 = t >> 1, g2 = r >> 2, j = ((t & 1) << 2) | (r & 3);
+            A.ifrag[((size_t)(fwd_base(l) + mt * 2 + kb2) * 64 + (out & 15) + 16 * g2) * 8 + j] = bv;
+        }
     }
     if (l >= 1) {
         const int mt = in / 16, kb = out / 32, ln = (in & 15) + 16 * ((out & 31) >> 3), i = out & 7;
@@ -702,62 +726,120 @@ struct InferArgs {
     long long n;             // records (mode 0)
     const unsigned *count;   // mode 1: the query count on the device
     int32_t shift;
-    const bf16x8 *ffrag;
+    const bf16x8 *ifrag;     // forward fragments in the inference K order
     const float *w;
     uint16_t *out_h;         // mode 0: FP16 RGB(0) per record
     float4 *tailv;           // mode 1: slot -> throughput * max(FP16 output, 0)
 };
 
+// 64 records per wavefront batch, 4 wavefronts per workgroup looping over batches; the workgroup
+// stages the weight fragments and biases in LDS once.  Activations never leave registers: each
+// layer's MFMA output tile is converted (bias, ReLU, BF16) into the next layer's B operand.
 // MODE 0: n records -> FP16 outputs; MODE 1: the render's query list -> its slots.
 template <int MODE>
-__global__ void __launch_bounds__(64) k_mlp_infer(InferArgs A) {
-    __shared__ __attribute__((aligned(16))) bf16 lds[kInferLds];
-    const int lane = threadIdx.x;
+__global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
+    __shared__ bf16x8 wl[kFwdFrags * 64];
+    __shared__ float bl[kBiasLds];
+    for (int i = threadIdx.x; i < kFwdFrags * 64; i += 256) wl[i] = A.ifrag[i];
+    for (int i = threadIdx.x; i < kBiasLds; i += 256) {
+        const int l = i >> 6, o = i & 63;
+        bl[i] = (l < 5) ? A.w[b_off(l) + o] : (o < 3 ? A.w[b_off(5) + o] : 0.0f);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, g = lane >> 4;
     const long long n = MODE == 0 ? A.n : (long long)*A.count;
-    const long long base = (long long)blockIdx.x * 64;
-    if (base >= n) return;
-    bf16 *act0 = lds, *h0 = lds + 64 * kS0, *h1 = h0 + 64 * kSH;
-    const long long idx = base + lane;
-    const bool present = idx < n;
-    const float4 *rec = A.recs + (present ? idx : 0) * 4;
-    encode_row(rec, present, A.shift, act0, lane);
-    __syncthreads();
-    hidden_layer<3>(act0, kS0, h0, A.ffrag + fwd_base(0) * 64, A.w + b_off(0), lane);
-    __syncthreads();
-    hidden_layer<2>(h0, kSH, h1, A.ffrag + fwd_base(1) * 64, A.w + b_off(1), lane);
-    __syncthreads();
-    hidden_layer<2>(h1, kSH, h0, A.ffrag + fwd_base(2) * 64, A.w + b_off(2), lane);
-    __syncthreads();
-    hidden_layer<2>(h0, kSH, h1, A.ffrag + fwd_base(3) * 64, A.w + b_off(3), lane);
-    __syncthreads();
-    hidden_layer<2>(h1, kSH, h0, A.ffrag + fwd_base(4) * 64, A.w + b_off(4), lane);
-    __syncthreads();
-    f32x4 y[4];
-    output_layer(h0, A.ffrag + fwd_base(5) * 64, A.w + b_off(5), lane, y);
-    // lanes 0..15 hold the outputs of samples nt*16 + lane; send them to the sample's lane
-    float yr[3] = {0.0f, 0.0f, 0.0f};
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long base = ((long long)blockIdx.x * 4 + wave) * 64; base < n; base += stride) {
+        // layer 0's B operand: group 4 kb + g of the features of sample nt*16 + col
+        bf16x8 b0[4][3];
+        float4 keep0[4], keep3[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
+        for (int nt = 0; nt < 4; ++nt) {
+            const long long s = base + nt * 16 + col;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+            if (s < n) {
+                const float4 *rec = A.recs + s * 4;
+                r0 = rec[0]; r1 = rec[1]; r2 = rec[2];
+                if (MODE == 1) r3 = rec[3];
+            }
+            keep0[nt] = r0;
+            keep3[nt] = r3;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float v = __shfl(y[nt][c], lane & 15);
-            if ((lane >> 4) == nt) yr[c] = v;
+            for (int kb = 0; kb < 3; ++kb) {
+                float f[8];
+                encode_group(r0, r1, r2, A.shift, 4 * kb + g, f);
+                b0[nt][kb] = to_bf16x8(f);
+            }
         }
-    if (!present) return;
-    const __half h0v = __float2half_rn(yr[0]), h1v = __float2half_rn(yr[1]), h2v = __float2half_rn(yr[2]);
-    if (MODE == 0) {
-        ushort4 o;
-        o.x = __half_as_ushort(h0v);
-        o.y = __half_as_ushort(h1v);
-        o.z = __half_as_ushort(h2v);
-        o.w = 0;
-        reinterpret_cast<ushort4 *>(A.out_h)[idx] = o;
-    } else {
-        const float4 r0 = rec[0], r3 = rec[3];
-        const uint32_t slot = __float_as_uint(r0.w);
-        const float a = fmaxf(__half2float(h0v), 0.0f), b = fmaxf(__half2float(h1v), 0.0f),
-                    cc = fmaxf(__half2float(h2v), 0.0f);
-        A.tailv[slot] = make_float4(r3.x * a, r3.y * b, r3.z * cc, 0.0f);
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) {
+                const bf16x8 a = wl[(fwd_base(0) + mt * 3 + kb) * 64 + lane];
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[nt][mt] = mfma(a, b0[nt][kb], acc[nt][mt]);
+            }
+        bf16x8 b[4][2];
+#pragma unroll 1
+        for (int l = 1; l < 6; ++l) {
+            // epilogue of layer l - 1 into layer l's B operand
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int mt = 2 * kb + (j >> 2), i = j & 3;
+                        const float v = acc[nt][mt][i] + bl[(l - 1) * 64 + mt * 16 + 4 * g + i];
+                        b[nt][kb][j] = (bf16)(v > 0.0f ? v : 0.0f);
+                    }
+            const int nmt = l == 5 ? 1 : 4;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) acc[nt][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                if (mt < nmt) {
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb) {
+                        const bf16x8 a = wl[(fwd_base(l) + mt * 2 + kb) * 64 + lane];
+#pragma unroll
+                        for (int nt = 0; nt < 4; ++nt) acc[nt][mt] = mfma(a, b[nt][kb], acc[nt][mt]);
+                    }
+                }
+            }
+        }
+        // outputs: lanes 0..15 hold rows 0..2 (R, G, B) of samples nt*16 + col
+        if (g == 0) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const long long s = base + nt * 16 + col;
+                if (s >= n) continue;
+                const __half hr = __float2half_rn(acc[nt][0][0] + bl[5 * 64 + 0]);
+                const __half hg = __float2half_rn(acc[nt][0][1] + bl[5 * 64 + 1]);
+                const __half hb = __float2half_rn(acc[nt][0][2] + bl[5 * 64 + 2]);
+                if (MODE == 0) {
+                    ushort4 o;
+                    o.x = __half_as_ushort(hr);
+                    o.y = __half_as_ushort(hg);
+                    o.z = __half_as_ushort(hb);
+                    o.w = 0;
+                    reinterpret_cast<ushort4 *>(A.out_h)[s] = o;
+                } else {
+                    const uint32_t slot = __float_as_uint(keep0[nt].w);
+                    const float4 t = keep3[nt];
+                    A.tailv[slot] = make_float4(t.x * fmaxf(__half2float(hr), 0.0f), t.y * fmaxf(__half2float(hg), 0.0f),
+                                                t.z * fmaxf(__half2float(hb), 0.0f), 0.0f);
+                }
+            }
+        }
     }
 }
 
@@ -787,7 +869,7 @@ struct vf_gicache {
     vf_ctx *ctx = nullptr;
     vf_gicache_params p{};
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
-    bf16 *ffrag = nullptr, *bfrag = nullptr;
+    bf16 *ffrag = nullptr, *bfrag = nullptr, *ifrag = nullptr;
     float *part = nullptr, *loss_part = nullptr;
     int32_t *valid_part = nullptr;
     int32_t part_rows = 0;
@@ -810,7 +892,7 @@ int check_cache(vf_ctx *ctx, vf_gicache *c, const char *what) {
 }
 
 void free_all(vf_gicache *c) {
-    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->part, c->loss_part, c->valid_part,
+    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->ifrag, c->part, c->loss_part, c->valid_part,
                     c->samples, c->st, c->qcount, c->queries, c->pre, c->tailv};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -833,7 +915,7 @@ int launch_adam(vf_gicache *c, int mode, const float *grad_in, float *grad_out, 
         A.bc1 = (float)(1.0 - std::pow((double)c->p.beta1, (double)c->steps));
         A.bc2 = (float)(1.0 - std::pow((double)c->p.beta2, (double)c->steps));
     }
-    A.ffrag = c->ffrag; A.bfrag = c->bfrag;
+    A.ffrag = c->ffrag; A.bfrag = c->bfrag; A.ifrag = c->ifrag;
     hipLaunchKernelGGL(k_mlp_adam, dim3((kP + 63) / 64), dim3(256), 0, c->ctx->stream, A);
     return launch_check(c->ctx, "k_mlp_adam");
 }
@@ -857,6 +939,12 @@ int launch_train(vf_gicache *c, const float4 *samples, long long n, int *rows) {
     hipLaunchKernelGGL(k_mlp_train, dim3(g), dim3(64), 0, c->ctx->stream, A);
     *rows = g;
     return launch_check(c->ctx, "k_mlp_train");
+}
+
+// Persistent inference grid: two workgroups per compute unit, fewer for small batches.
+unsigned infer_groups(const vf_gicache *c, int64_t n) {
+    const int64_t need = (n + 255) / 256, cap = 2ll * c->ctx->n_cu;
+    return (unsigned)(need < 1 ? 1 : (need < cap ? need : cap));
 }
 
 int launch_stats(vf_gicache *c, int rows, float *loss_out) {
@@ -1015,6 +1103,7 @@ int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out)
     VF_GC_ALLOC(c->v, sizeof(float) * kP);
     VF_GC_ALLOC(c->grad, sizeof(float) * kP);
     VF_GC_ALLOC(c->ffrag, sizeof(bf16) * kFwdFrags * 512);
+    VF_GC_ALLOC(c->ifrag, sizeof(bf16) * kFwdFrags * 512);
     VF_GC_ALLOC(c->bfrag, sizeof(bf16) * kBwdFrags * 512);
     VF_GC_ALLOC(c->part, sizeof(float) * (size_t)kP * c->part_rows);
     VF_GC_ALLOC(c->loss_part, sizeof(float) * c->part_rows);
@@ -1024,6 +1113,7 @@ int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out)
     VF_GC_ALLOC(c->qcount, sizeof(unsigned));
 #undef VF_GC_ALLOC
     if (e == hipSuccess) e = hipMemsetAsync(c->ffrag, 0, sizeof(bf16) * kFwdFrags * 512, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->ifrag, 0, sizeof(bf16) * kFwdFrags * 512, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bfrag, 0, sizeof(bf16) * kBwdFrags * 512, ctx->stream);
     if (e != hipSuccess) {
         free_all(c);
@@ -1129,11 +1219,11 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
     I.n = 0;
     I.count = c->qcount;
     I.shift = c->p.enc_shift;
-    I.ffrag = reinterpret_cast<const bf16x8 *>(c->ffrag);
+    I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
     I.w = c->w;
     I.out_h = nullptr;
     I.tailv = c->tailv;
-    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(blocks_for(nslots, 64)), dim3(64), 0, ctx->stream, I);
+    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(infer_groups(c, nslots)), dim3(256), 0, ctx->stream, I);
     if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
     hipLaunchKernelGGL(k_gc_resolve, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, c->st, (int32_t)npix, spp, c->pre,
                        c->tailv, reinterpret_cast<float4 *>(out_rgba));
@@ -1203,12 +1293,12 @@ int vf_gicache_infer(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64
     I.n = n;
     I.count = nullptr;
     I.shift = c->p.enc_shift;
-    I.ffrag = reinterpret_cast<const bf16x8 *>(c->ffrag);
+    I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
     I.w = c->w;
     I.out_h = out_rgbh;
     I.tailv = nullptr;
     if ((rc = vf::begin_timed(ctx))) return rc;
-    hipLaunchKernelGGL(k_mlp_infer<0>, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream, I);
+    hipLaunchKernelGGL(k_mlp_infer<0>, dim3(infer_groups(c, n)), dim3(256), 0, ctx->stream, I);
     if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
     return vf::end_timed(ctx);
 }
