@@ -126,29 +126,18 @@ __device__ __forceinline__ bf16x8 to_bf16x8(const float f[8]) {
     return v;
 }
 
-// The lane's record (64 B) into row `lane` of the input tile.
-__device__ __forceinline__ void encode_row(const float4 *rec, bool present, int shift, bf16 *act0, int lane) {
-    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
-    if (present) { r0 = rec[0]; r1 = rec[1]; r2 = rec[2]; }
-    bf16 *row = act0 + lane * kS0;
-    for (int q = 0; q < 12; ++q) {
-        float f[8];
-        encode_group(r0, r1, r2, shift, q, f);
-        *reinterpret_cast<bf16x8 *>(row + 8 * q) = to_bf16x8(f);
-    }
-}
-
 __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Hidden layer l (KB = in/32): act_out = bf16(relu(W^T . act_in + b)) for the wavefront's 64 samples.
+// Hidden layer l (KB = in/32): act_out = bf16(relu(W^T . act_in + b)) for the chunk's 64 samples; wave
+// w of nw computes the output row tiles mt = w, w + nw, ...
 template <int KB>
 __device__ __forceinline__ void hidden_layer(const bf16 *act_in, int s_in, bf16 *act_out, const bf16x8 *frag,
-                                             const float *bias, int lane) {
+                                             const float *bias, int lane, int w, int nw) {
     const int col = lane & 15, g = lane >> 4;
 #pragma unroll 1
-    for (int mt = 0; mt < 4; ++mt) {
+    for (int mt = w; mt < 4; mt += nw) {
         bf16x8 a[KB];
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
@@ -198,26 +187,28 @@ __device__ __forceinline__ void output_layer(const bf16 *act5, const bf16x8 *fra
     }
 }
 
-// The forward pass of the wavefront's 64 encoded samples; hidden activations stay in act[1..5].
+// The forward pass of the chunk's 64 encoded samples by the workgroup's nw waves; hidden activations
+// stay in act[1..5]; wave 0 computes the outputs.
 __device__ __forceinline__ void forward(bf16 *act0, bf16 *const act[6], const bf16x8 *ffrag, const float *w, int lane,
-                                        f32x4 y[4]) {
-    hidden_layer<3>(act0, kS0, act[1], ffrag + fwd_base(0) * 64, w + b_off(0), lane);
+                                        int wv, int nw, f32x4 y[4]) {
+    hidden_layer<3>(act0, kS0, act[1], ffrag + fwd_base(0) * 64, w + b_off(0), lane, wv, nw);
     __syncthreads();
 #pragma unroll 1
     for (int l = 1; l < 5; ++l) {
-        hidden_layer<2>(act[l], kSH, act[l + 1], ffrag + fwd_base(l) * 64, w + b_off(l), lane);
+        hidden_layer<2>(act[l], kSH, act[l + 1], ffrag + fwd_base(l) * 64, w + b_off(l), lane, wv, nw);
         __syncthreads();
     }
-    output_layer(act[5], ffrag + fwd_base(5) * 64, w + b_off(5), lane, y);
+    if (wv == 0) output_layer(act[5], ffrag + fwd_base(5) * 64, w + b_off(5), lane, y);
 }
 
 // Gradient into the previous layer, in place: act[l] := bf16((W_l . G^T)^T * (act[l] > 0)), where G
 // (the gradient at layer l's output, KB = out/32 blocks) is read from g_in.
 template <int KB>
-__device__ __forceinline__ void backward_layer(const bf16 *g_in, int s_g, bf16 *act, const bf16x8 *frag, int lane) {
+__device__ __forceinline__ void backward_layer(const bf16 *g_in, int s_g, bf16 *act, const bf16x8 *frag, int lane, int w,
+                                               int nw) {
     const int col = lane & 15, g = lane >> 4;
 #pragma unroll 1
-    for (int mt = 0; mt < 4; ++mt) {
+    for (int mt = w; mt < 4; mt += nw) {
         bf16x8 a[KB];
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
@@ -243,44 +234,39 @@ __device__ __forceinline__ void backward_layer(const bf16 *g_in, int s_g, bf16 *
     }
 }
 
-// Weight gradient of layer l over the wavefront's 64 samples: dW[in][out] += sum_s act[s][in] G[s][out]
-// (MFMA with the samples as K; NMT = in/16 row tiles, NNT = out column tiles), bias gradient
-// db[out] += sum_s G[s][out]; written to (or added into) this wavefront's partial.
+// Weight gradient of layer l over the chunk's 64 samples: dW[in][out] += sum_s act[s][in] G[s][out] (MFMA
+// with the samples as K; NMT = in/16 row tiles x NNT = out column tiles, tile t handled by wave t % nw),
+// bias gradient db[out] += sum_s G[s][out] by wave bw; written to (or added into) the workgroup's
+// partial.
 template <int NMT, int NNT>
 __device__ __forceinline__ void weight_grad(const bf16 *act, int s_act, const bf16 *gr, int s_g, int nout, float *pw,
-                                            float *pb, bool accumulate, int lane) {
+                                            float *pb, bool accumulate, int lane, int w, int nw, int bw) {
     const int col = lane & 15, g = lane >> 4;
-    bf16x8 b[NNT][2];
-#pragma unroll
-    for (int nt = 0; nt < NNT; ++nt)
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-            for (int i = 0; i < 8; ++i) b[nt][kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
 #pragma unroll 1
-    for (int mt = 0; mt < NMT; ++mt) {
-        bf16x8 a[2];
+    for (int t = w; t < NMT * NNT; t += nw) {
+        const int mt = t / NNT, nt = t % NNT;
+        bf16x8 a[2], b[2];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a[kb][i] = act[(kb * 32 + 8 * g + i) * s_act + mt * 16 + col];
-#pragma unroll
-        for (int nt = 0; nt < NNT; ++nt) {
-            const int o = nt * 16 + col;
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (accumulate && o < nout) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i] = pw[(mt * 16 + 4 * g + i) * nout + o];
+            for (int i = 0; i < 8; ++i) {
+                a[kb][i] = act[(kb * 32 + 8 * g + i) * s_act + mt * 16 + col];
+                b[kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
             }
-            acc = mfma(a[0], b[nt][0], acc);
-            acc = mfma(a[1], b[nt][1], acc);
-            if (o < nout) {
+        const int o = nt * 16 + col;
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (accumulate && o < nout) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) pw[(mt * 16 + 4 * g + i) * nout + o] = acc[i];
-            }
+            for (int i = 0; i < 4; ++i) acc[i] = pw[(mt * 16 + 4 * g + i) * nout + o];
+        }
+        acc = mfma(a[0], b[0], acc);
+        acc = mfma(a[1], b[1], acc);
+        if (o < nout) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) pw[(mt * 16 + 4 * g + i) * nout + o] = acc[i];
         }
     }
-    if (lane < nout) {
+    if (w == bw && lane < nout) {
         float s = 0.0f;
         for (int r = 0; r < 64; ++r) s += (float)gr[r * s_g + lane];
         pb[lane] = accumulate ? pb[lane] + s : s;
@@ -306,50 +292,60 @@ struct TrainArgs {
     int32_t *valid_part;     // [gridDim.x]
 };
 
-// Forward + backward of the relative L2 loss over 64-sample chunks; one wavefront per workgroup, chunks
-// strided over the grid.
-__global__ void __launch_bounds__(64) k_mlp_train(TrainArgs A) {
+// Forward + backward of the relative L2 loss over 64-sample chunks: the workgroup's four waves share
+// each chunk (every layer's tiles are split between them) and accumulate the weight gradients of their
+// chunks, strided over the grid, into the workgroup's partial.
+constexpr int kTrainWaves = 4;
+__global__ void __launch_bounds__(256) k_mlp_train(TrainArgs A) {
     __shared__ __attribute__((aligned(16))) bf16 lds[kTrainLds];
-    const int lane = threadIdx.x;
+    __shared__ float4 tgt[64];   // per sample: target RGB, valid
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int nw = kTrainWaves;
     bf16 *act0 = lds;
     bf16 *act[6];
     act[0] = act0;
 #pragma unroll
     for (int l = 1; l < 6; ++l) act[l] = lds + 64 * kS0 + (l - 1) * 64 * kSH;
     bf16 *g6 = lds + 64 * kS0 + 5 * 64 * kSH;
-    for (int j = 0; j < kSG; ++j) g6[lane * kSG + j] = (bf16)0.0f;
+    for (int j = tid; j < 64 * kSG; j += 256) g6[j] = (bf16)0.0f;
     float *pw = A.part + (size_t)blockIdx.x * kP;
-    float loss = 0.0f;
-    int32_t nvalid = 0;
+    float loss = 0.0f, nvalid = 0.0f;
     bool first = true;
     for (int ch = blockIdx.x; ch < A.n_chunks; ch += gridDim.x) {
-        const long long sidx = (long long)ch * 64 + lane;
-        const bool present = sidx < A.n;
-        const float4 *rec = A.samples + (present ? sidx : 0) * 4;
-        float4 r3 = make_float4(0.f, 0.f, 0.f, 0.f);
-        bool valid = false;
-        if (present) {
-            valid = __float_as_uint(rec[0].w) != 0u;
-            r3 = rec[3];
+        // encoding: four lanes per sample, each three of its twelve feature groups
+        {
+            const int s = wv * 16 + (lane >> 2), pq = lane & 3;
+            const long long sidx = (long long)ch * 64 + s;
+            const bool present = sidx < A.n;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+            if (present) {
+                const float4 *rec = A.samples + sidx * 4;
+                r0 = rec[0]; r1 = rec[1]; r2 = rec[2]; r3 = rec[3];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float f[8];
+                encode_group(r0, r1, r2, A.shift, pq + 4 * k, f);
+                *reinterpret_cast<bf16x8 *>(act0 + s * kS0 + 8 * (pq + 4 * k)) = to_bf16x8(f);
+            }
+            if (pq == 0) tgt[s] = make_float4(r3.x, r3.y, r3.z, (present && __float_as_uint(r0.w) != 0u) ? 1.0f : 0.0f);
         }
-        nvalid += valid ? 1 : 0;
-        encode_row(rec, present, A.shift, act0, lane);
         __syncthreads();
         f32x4 y[4];
-        forward(act0, act, A.ffrag, A.w, lane, y);
-        // loss gradient: lanes 0..15 hold outputs 0..2 of samples nt*16 + lane
+        forward(act0, act, A.ffrag, A.w, lane, wv, nw, y);
+        // loss gradient (wave 0): lanes 0..15 hold outputs 0..2 of samples nt*16 + lane
+        if (wv == 0 && lane < 16) {
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const int s = nt * 16 + (lane & 15);
-            const float tx = __shfl(r3.x, s), ty = __shfl(r3.y, s), tz = __shfl(r3.z, s);
-            const bool v = __shfl((int)valid, s) != 0;
-            if (lane < 16) {
+            for (int nt = 0; nt < 4; ++nt) {
+                const int s = nt * 16 + lane;
+                const float4 t = tgt[s];
                 const float y0 = y[nt][0], y1 = y[nt][1], y2 = y[nt][2];
                 const float mean = ((y0 + y1) + y2) / 3.0f;
                 const float den = mean * mean + A.loss_eps;
-                const float d0 = y0 - tx, d1 = y1 - ty, d2 = y2 - tz;
+                const float d0 = y0 - t.x, d1 = y1 - t.y, d2 = y2 - t.z;
                 bf16x4 o = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
-                if (v) {
+                if (t.w != 0.0f) {
+                    nvalid += 1.0f;
                     loss += ((d0 * d0 + d1 * d1) + d2 * d2) / den / 3.0f;
                     const float k = (2.0f / 3.0f) * A.inv_n / den;
                     o[0] = (bf16)(d0 * k);
@@ -361,29 +357,32 @@ __global__ void __launch_bounds__(64) k_mlp_train(TrainArgs A) {
         }
         __syncthreads();
         // layer 5 (64 -> 3), then 4..1, then 0
-        weight_grad<4, 1>(act[5], kSH, g6, kSG, 3, pw + w_off(5), pw + b_off(5), !first, lane);
+        weight_grad<4, 1>(act[5], kSH, g6, kSG, 3, pw + w_off(5), pw + b_off(5), !first, lane, wv, nw, 3);
         __syncthreads();
-        backward_layer<1>(g6, kSG, act[5], A.bfrag + bwd_base(5) * 64, lane);
+        backward_layer<1>(g6, kSG, act[5], A.bfrag + bwd_base(5) * 64, lane, wv, nw);
         __syncthreads();
 #pragma unroll 1
         for (int l = 4; l >= 1; --l) {
-            weight_grad<4, 4>(act[l], kSH, act[l + 1], kSH, 64, pw + w_off(l), pw + b_off(l), !first, lane);
+            weight_grad<4, 4>(act[l], kSH, act[l + 1], kSH, 64, pw + w_off(l), pw + b_off(l), !first, lane, wv, nw,
+                              l & 3);
             __syncthreads();
-            backward_layer<2>(act[l + 1], kSH, act[l], A.bfrag + bwd_base(l) * 64, lane);
+            backward_layer<2>(act[l + 1], kSH, act[l], A.bfrag + bwd_base(l) * 64, lane, wv, nw);
             __syncthreads();
         }
-        weight_grad<6, 4>(act0, kS0, act[1], kSH, 64, pw + w_off(0), pw + b_off(0), !first, lane);
+        weight_grad<6, 4>(act0, kS0, act[1], kSH, 64, pw + w_off(0), pw + b_off(0), !first, lane, wv, nw, 0);
         __syncthreads();
         first = false;
     }
-    if (first) {   // no chunk for this wavefront: its partial is zero
-        for (int p = lane; p < kP; p += 64) pw[p] = 0.0f;
+    if (first) {   // no chunk for this workgroup: its partial is zero
+        for (int p = tid; p < kP; p += 256) pw[p] = 0.0f;
     }
-    loss = wave_sum(loss);
-    const float nv = wave_sum((float)nvalid);
-    if (lane == 0) {
-        A.loss_part[blockIdx.x] = loss * A.inv_n;
-        A.valid_part[blockIdx.x] = (int32_t)nv;
+    if (wv == 0) {
+        loss = wave_sum(loss);
+        nvalid = wave_sum(nvalid);
+        if (lane == 0) {
+            A.loss_part[blockIdx.x] = loss * A.inv_n;
+            A.valid_part[blockIdx.x] = (int32_t)nvalid;
+        }
     }
 }
 
@@ -936,7 +935,7 @@ int launch_train(vf_gicache *c, const float4 *samples, long long n, int *rows) {
     A.loss_part = c->loss_part;
     A.valid_part = c->valid_part;
     const int g = A.n_chunks < c->part_rows ? A.n_chunks : c->part_rows;
-    hipLaunchKernelGGL(k_mlp_train, dim3(g), dim3(64), 0, c->ctx->stream, A);
+    hipLaunchKernelGGL(k_mlp_train, dim3(g), dim3(64 * kTrainWaves), 0, c->ctx->stream, A);
     *rows = g;
     return launch_check(c->ctx, "k_mlp_train");
 }
