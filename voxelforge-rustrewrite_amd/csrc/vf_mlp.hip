@@ -38,6 +38,8 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kP = VF_GICACHE_PARAMS;
+constexpr int kPP = (kP + 3) & ~3;      // row stride of the gradient partials (float4 aligned)
+constexpr int kSlices = 16;             // first-level slices of the partial reduction
 constexpr int kIn = VF_GICACHE_INPUTS;   // 78 features padded to 96
 // Layer shapes: (96, 64), (64, 64) x 4, (64, 3).
 __host__ __device__ constexpr int in_dim(int l) { return l == 0 ? 96 : 64; }
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(TrainArgs A) {
     for (int l = 1; l < 6; ++l) act[l] = lds + 64 * kS0 + (l - 1) * 64 * kSH;
     bf16 *g6 = lds + 64 * kS0 + 5 * 64 * kSH;
     for (int j = tid; j < 64 * kSG; j += 256) g6[j] = (bf16)0.0f;
-    float *pw = A.part + (size_t)blockIdx.x * kP;
+    float *pw = A.part + (size_t)blockIdx.x * kPP;
     float loss = 0.0f, nvalid = 0.0f;
     bool first = true;
     for (int ch = blockIdx.x; ch < A.n_chunks; ch += gridDim.x) {
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(TrainArgs A) {
         first = false;
     }
     if (first) {   // no chunk for this workgroup: its partial is zero
-        for (int p = tid; p < kP; p += 256) pw[p] = 0.0f;
+        for (int p = tid; p < kPP; p += 256) pw[p] = 0.0f;
     }
     if (wv == 0) {
         loss = wave_sum(loss);
@@ -386,9 +388,23 @@ __global__ void __launch_bounds__(256) k_mlp_train(TrainArgs A) {
     }
 }
 
+// First level of the deterministic gradient reduction: slice y of the partial rows, four parameters
+// per thread (float4), summed in row order into part2[y].
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int32_t nparts, float *part2) {
+    const int p4 = blockIdx.x * 256 + threadIdx.x;
+    if (p4 * 4 >= kPP) return;
+    const int r0 = (int)((long long)nparts * blockIdx.y / kSlices), r1 = (int)((long long)nparts * (blockIdx.y + 1) / kSlices);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int r = r0; r < r1; ++r) {
+        const float4 v = reinterpret_cast<const float4 *>(part + (size_t)r * kPP)[p4];
+        s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+    reinterpret_cast<float4 *>(part2 + (size_t)blockIdx.y * kPP)[p4] = s;
+}
+
 struct AdamArgs {
     float *w, *m, *v;
-    const float *part;     // [nparts][kP] (mode 1, 2 without grad_in)
+    const float *part;     // [kSlices][kPP]: the first-level sums (mode 1, 2 without grad_in)
     int32_t nparts;
     const float *grad_in;  // mode 2: the gradient instead of the partials
     float *grad_out;       // mode 1: the reduced gradient
@@ -421,23 +437,22 @@ __device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) 
     }
 }
 
-// 64 parameters per workgroup x 4 slices of the partials, summed in a fixed order.
+// One parameter per thread: the gradient (the first-level sums added in slice order, or grad_in), then
+// Adam on the FP32 master weight and the BF16 fragment copies.
 __global__ void __launch_bounds__(256) k_mlp_adam(AdamArgs A) {
-    __shared__ float red[4][64];
-    const int t = threadIdx.x & 63, sl = threadIdx.x >> 6;
-    const int p = blockIdx.x * 64 + t;
-    float s = 0.0f;
-    if (A.mode != 0 && !A.grad_in && p < kP)
-        for (int q = sl; q < A.nparts; q += 4) s += A.part[(size_t)q * kP + p];
-    red[sl][t] = s;
-    __syncthreads();
-    if (sl != 0 || p >= kP) return;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kP) return;
     if (A.mode == 0) {
         write_frags(A, p, A.w[p]);
         return;
     }
-    const float gsum = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
-    const float g = A.grad_in ? A.grad_in[p] : gsum;
+    float g;
+    if (A.grad_in) {
+        g = A.grad_in[p];
+    } else {
+        g = 0.0f;
+        for (int q = 0; q < kSlices; ++q) g += A.part[(size_t)q * kPP + p];
+    }
     if (A.mode == 1) {
         A.grad_out[p] = g;
         return;
@@ -869,7 +884,7 @@ struct vf_gicache {
     vf_gicache_params p{};
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     bf16 *ffrag = nullptr, *bfrag = nullptr, *ifrag = nullptr;
-    float *part = nullptr, *loss_part = nullptr;
+    float *part = nullptr, *part2 = nullptr, *loss_part = nullptr;
     int32_t *valid_part = nullptr;
     int32_t part_rows = 0;
     float4 *samples = nullptr;
@@ -891,16 +906,22 @@ int check_cache(vf_ctx *ctx, vf_gicache *c, const char *what) {
 }
 
 void free_all(vf_gicache *c) {
-    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->ifrag, c->part, c->loss_part, c->valid_part,
+    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->ifrag, c->part, c->part2, c->loss_part, c->valid_part,
                     c->samples, c->st, c->qcount, c->queries, c->pre, c->tailv};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
 
 int launch_adam(vf_gicache *c, int mode, const float *grad_in, float *grad_out, int nparts) {
+    if (mode != 0 && !grad_in) {
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((kPP / 4 + 255) / 256, kSlices), dim3(256), 0, c->ctx->stream, c->part,
+                           nparts, c->part2);
+        int rc = launch_check(c->ctx, "k_mlp_reduce");
+        if (rc) return rc;
+    }
     AdamArgs A;
     A.w = c->w; A.m = c->m; A.v = c->v;
-    A.part = c->part;
+    A.part = c->part2;
     A.nparts = nparts;
     A.grad_in = grad_in;
     A.grad_out = grad_out;
@@ -915,7 +936,7 @@ int launch_adam(vf_gicache *c, int mode, const float *grad_in, float *grad_out, 
         A.bc2 = (float)(1.0 - std::pow((double)c->p.beta2, (double)c->steps));
     }
     A.ffrag = c->ffrag; A.bfrag = c->bfrag; A.ifrag = c->ifrag;
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kP + 63) / 64), dim3(256), 0, c->ctx->stream, A);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kP + 255) / 256), dim3(256), 0, c->ctx->stream, A);
     return launch_check(c->ctx, "k_mlp_adam");
 }
 
@@ -1104,7 +1125,8 @@ int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out)
     VF_GC_ALLOC(c->ffrag, sizeof(bf16) * kFwdFrags * 512);
     VF_GC_ALLOC(c->ifrag, sizeof(bf16) * kFwdFrags * 512);
     VF_GC_ALLOC(c->bfrag, sizeof(bf16) * kBwdFrags * 512);
-    VF_GC_ALLOC(c->part, sizeof(float) * (size_t)kP * c->part_rows);
+    VF_GC_ALLOC(c->part, sizeof(float) * (size_t)kPP * c->part_rows);
+    VF_GC_ALLOC(c->part2, sizeof(float) * (size_t)kPP * kSlices);
     VF_GC_ALLOC(c->loss_part, sizeof(float) * c->part_rows);
     VF_GC_ALLOC(c->valid_part, sizeof(int32_t) * c->part_rows);
     VF_GC_ALLOC(c->samples, sizeof(float4) * 4 * (size_t)p->train_samples);
