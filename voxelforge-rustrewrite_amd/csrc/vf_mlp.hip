@@ -608,7 +608,7 @@ __device__ __forceinline__ void push_query(bool want, unsigned *qcount, float4 *
 
 // Cached render, traced part.  Fallback frames: the plain path, written to out.  Otherwise terminate 1:
 // per sample the path up to its tail (pre += its light; a query at the tail, or tailv[slot] = 0);
-// terminate 0: one query per non-sky pixel at the G-buffer point.
+// terminate 0: nothing (k_mlp_infer<2> answers at the G-buffer points).
 __global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, PathArgs A, const float4 *gbpos,
                                                    const uint32_t *gbn, const DevStats *st, float4 *out, float4 *pre,
                                                    float4 *tailv, float4 *queries, unsigned *qcount) {
@@ -618,27 +618,11 @@ __global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, Path
         if (inside) plain_pixel(W, S, A, gbpos, gbn, px, py, out);
         return;
     }
+    if (A.terminate == 0) return;   // answered at the G-buffer point by k_mlp_infer<2>
     const size_t i = inside ? (size_t)py * A.w + px : 0;
     const bool sky = !inside || gbpos[i].w < 0.0f;
     float P[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f};
     if (!sky) gbuffer_point(gbpos, gbn, i, P, N);
-    if (A.terminate == 0) {
-        float4 r[4];
-        if (!sky) {
-            float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
-            normalize3(v);
-            r[0] = make_float4(P[0], P[1], P[2], __uint_as_float((uint32_t)i));
-            r[1] = make_float4(N[0], N[1], N[2], 0.0f);
-            r[2] = make_float4(v[0], v[1], v[2], 0.0f);
-            r[3] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        }
-        push_query(!sky, qcount, queries, r);
-        if (inside) {
-            pre[i] = make_float4(0.0f, 0.0f, 0.0f, sky ? 0.0f : 1.0f);
-            if (sky) tailv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        return;
-    }
     float sum[3] = {0.0f, 0.0f, 0.0f};
     Cnt c = {0, 0, 0, 0};
     for (int32_t s = 0; s < A.spp; ++s) {
@@ -737,23 +721,32 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Sam
 
 struct InferArgs {
     const float4 *recs;
-    long long n;             // records (mode 0)
-    const unsigned *count;   // mode 1: the query count on the device
+    long long n;             // records (mode 0), pixels (mode 2)
+    const unsigned *count;   // mode 1: the query count on the device; mode 2: counts the answered pixels
     int32_t shift;
     const bf16x8 *ifrag;     // forward fragments in the inference K order
     const float *w;
     uint16_t *out_h;         // mode 0: FP16 RGB(0) per record
     float4 *tailv;           // mode 1: slot -> throughput * max(FP16 output, 0)
+    const float4 *gbpos;     // mode 2: the G-buffer
+    const uint32_t *gbn;
+    float cam[3];
+    float4 *out;             // mode 2: RGBA per pixel
+    const DevStats *st;      // mode 2: nothing on fallback frames
 };
 
 // 64 records per wavefront batch, 4 wavefronts per workgroup looping over batches; the workgroup
 // stages the weight fragments and biases in LDS once.  Activations never leave registers: each
 // layer's MFMA output tile is converted (bias, ReLU, BF16) into the next layer's B operand.
-// MODE 0: n records -> FP16 outputs; MODE 1: the render's query list -> its slots.
+// MODE 0: n records -> FP16 outputs; MODE 1: the render's query list -> its slots; MODE 2: the cache's
+// answer at every non-sky G-buffer pixel (query built from the G-buffer: position, quantised normal,
+// direction to the camera) straight into the output (batches of sky only are skipped).
 template <int MODE>
 __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
     __shared__ bf16x8 wl[kFwdFrags * 64];
     __shared__ float bl[kBiasLds];
+    __shared__ unsigned answered[4];
+    if (MODE == 2 && A.st->fallback) return;
     for (int i = threadIdx.x; i < kFwdFrags * 64; i += 256) wl[i] = A.ifrag[i];
     for (int i = threadIdx.x; i < kBiasLds; i += 256) {
         const int l = i >> 6, o = i & 63;
@@ -762,9 +755,18 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, g = lane >> 4;
-    const long long n = MODE == 0 ? A.n : (long long)*A.count;
+    const long long n = MODE == 1 ? (long long)*A.count : A.n;
     const long long stride = (long long)gridDim.x * 256;
+    unsigned n_answered = 0;
     for (long long base = ((long long)blockIdx.x * 4 + wave) * 64; base < n; base += stride) {
+        if (MODE == 2) {   // a batch of sky pixels: zeros, no network
+            const long long s = base + lane;
+            const bool ground = s < n && A.gbpos[s].w >= 0.0f;
+            if (__ballot(ground) == 0ull) {
+                if (s < n) A.out[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                continue;
+            }
+        }
         // layer 0's B operand: group 4 kb + g of the features of sample nt*16 + col
         bf16x8 b0[4][3];
         float4 keep0[4], keep3[4];
@@ -772,7 +774,20 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
         for (int nt = 0; nt < 4; ++nt) {
             const long long s = base + nt * 16 + col;
             float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
-            if (s < n) {
+            if (MODE == 2) {
+                if (s < n) {
+                    const float4 p4 = A.gbpos[s];
+                    if (p4.w >= 0.0f) {
+                        const uint32_t nb = A.gbn[s];
+                        float v[3] = {A.cam[0] - p4.x, A.cam[1] - p4.y, A.cam[2] - p4.z};
+                        normalize3(v);
+                        r0 = make_float4(p4.x, p4.y, p4.z, __uint_as_float(1u));
+                        r1 = make_float4(unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u),
+                                         unorm8_dec((nb >> 16) & 255u), 0.0f);
+                        r2 = make_float4(v[0], v[1], v[2], 0.0f);
+                    }
+                }
+            } else if (s < n) {
                 const float4 *rec = A.recs + s * 4;
                 r0 = rec[0]; r1 = rec[1]; r2 = rec[2];
                 if (MODE == 1) r3 = rec[3];
@@ -846,6 +861,14 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
                     o.z = __half_as_ushort(hb);
                     o.w = 0;
                     reinterpret_cast<ushort4 *>(A.out_h)[s] = o;
+                } else if (MODE == 2) {
+                    if (__float_as_uint(keep0[nt].w) != 0u) {
+                        n_answered += 1;
+                        A.out[s] = make_float4(fmaxf(__half2float(hr), 0.0f), fmaxf(__half2float(hg), 0.0f),
+                                               fmaxf(__half2float(hb), 0.0f), 1.0f);
+                    } else {
+                        A.out[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
                 } else {
                     const uint32_t slot = __float_as_uint(keep0[nt].w);
                     const float4 t = keep3[nt];
@@ -854,6 +877,14 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs A) {
                 }
             }
         }
+    }
+    if (MODE == 2) {   // answered pixels: one atomic per workgroup
+        float c = (float)n_answered;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) answered[wave] = (unsigned)c;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(const_cast<unsigned *>(A.count), ((answered[0] + answered[1]) + answered[2]) + answered[3]);
     }
 }
 
@@ -1201,8 +1232,8 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
     if (rc) return rc;
     if ((rc = check_gbuffer(ctx, "vf_gicache_frame", pass, w, h, gb_pos, gb_normal))) return rc;
     if (!out_rgba) return gc_fail_null(ctx, "vf_gicache_frame");
-    const int64_t npix = (int64_t)w * h, nslots = npix * (c->p.terminate ? c->p.spp : 1);
-    if (npix > c->cap_pix || nslots > c->cap_slots) {
+    const int64_t npix = (int64_t)w * h, nslots = npix * c->p.spp;
+    if (c->p.terminate == 1 && (npix > c->cap_pix || nslots > c->cap_slots)) {
         if ((rc = vf::no_capture(ctx, "vf_gicache_frame (first frame of a new size)"))) return rc;
         VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (c->queries) (void)hipFree(c->queries);
@@ -1227,22 +1258,30 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
         if ((rc = launch_adam(c, 2, nullptr, nullptr, rows))) return rc;
     }
     // 3. render: traced part + queries, the network on the queries, resolve
-    const int32_t spp = c->p.terminate ? c->p.spp : 1;
-    const PathArgs A = path_args(pass, w, h, frame, c->p.spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate);
-    PathArgs R = A;
-    R.spp = c->p.terminate ? c->p.spp : A.spp;
+    const int32_t spp = c->p.spp;
+    const PathArgs R = path_args(pass, w, h, frame, spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate);
     hipLaunchKernelGGL(k_gc_render, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, R,
                        reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
                        c->pre, c->tailv, c->queries, c->qcount);
     if ((rc = launch_check(ctx, "k_gc_render"))) return rc;
     InferArgs I;
-    I.recs = c->queries;
-    I.n = 0;
+    std::memset(&I, 0, sizeof I);
     I.count = c->qcount;
     I.shift = c->p.enc_shift;
     I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
     I.w = c->w;
-    I.out_h = nullptr;
+    if (c->p.terminate == 0) {
+        I.n = npix;
+        I.gbpos = reinterpret_cast<const float4 *>(gb_pos);
+        I.gbn = gb_normal;
+        for (int a = 0; a < 3; ++a) I.cam[a] = pass->cam_pos[a];
+        I.out = reinterpret_cast<float4 *>(out_rgba);
+        I.st = c->st;
+        hipLaunchKernelGGL(k_mlp_infer<2>, dim3(infer_groups(c, npix)), dim3(256), 0, ctx->stream, I);
+        if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
+        return vf::end_timed(ctx);
+    }
+    I.recs = c->queries;
     I.tailv = c->tailv;
     hipLaunchKernelGGL(k_mlp_infer<1>, dim3(infer_groups(c, nslots)), dim3(256), 0, ctx->stream, I);
     if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
@@ -1310,14 +1349,13 @@ int vf_gicache_infer(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64
     if (n == 0) return VF_OK;
     if (!recs || !out_rgbh) return gc_fail_null(ctx, "vf_gicache_infer");
     InferArgs I;
+    std::memset(&I, 0, sizeof I);
     I.recs = reinterpret_cast<const float4 *>(recs);
     I.n = n;
-    I.count = nullptr;
     I.shift = c->p.enc_shift;
     I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
     I.w = c->w;
     I.out_h = out_rgbh;
-    I.tailv = nullptr;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_mlp_infer<0>, dim3(infer_groups(c, n)), dim3(256), 0, ctx->stream, I);
     if ((rc = launch_check(ctx, "k_mlp_infer"))) return rc;
