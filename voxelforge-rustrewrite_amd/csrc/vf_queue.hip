@@ -61,9 +61,11 @@ struct GenArgs {
 
 // Pixel -> GI ray of gi.comp at full resolution (fpix = pix): origin P + N*0.03, direction
 // cosineHemisphereU(N, ldSample2(pix, frame >> 2)).  Sky pixels get miss records.  A workgroup covers
-// 256 consecutive pixels of the tile order and reserves room for its rays with one atomic; with
+// kGenPer x 256 consecutive pixels of the tile order and reserves room for its rays with one atomic
+// (atomics on the one list counter serialise, so fewer and larger reservations are faster); with
 // `sort` it writes them grouped by direction octant (a counting sort in LDS), so the rays of a
 // wavefront start close together AND share a direction octant.
+constexpr int kGenPer = 4;
 __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, const uint32_t *gbn, float4 *rays,
                                                 unsigned long long *ctr, vf_hit *gi_hits, vf_hit *sun_hits) {
     __shared__ uint32_t oct_count[8], oct_base[8];
@@ -71,36 +73,45 @@ __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, 
     const int t = threadIdx.x;
     if (t < 8) oct_count[t] = 0u;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * 256 + t;
     const int32_t per = A.tile_w * A.tile_h;
-    const int64_t tile = i / per;
-    const int32_t in = (int32_t)(i % per);
-    const int32_t px = (int32_t)(tile % A.tiles_x) * A.tile_w + in % A.tile_w;
-    const int32_t py = (int32_t)(tile / A.tiles_x) * A.tile_h + in / A.tile_w;
-    const bool inside = px < A.w && py < A.h;
-    const size_t fi = inside ? (size_t)py * A.w + px : 0;
-    const bool valid = inside && gbpos[fi].w >= 0.0f;
-    float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 0.0f};
-    uint32_t oct = 0u, local = 0u;
-    if (valid) {
-        const float4 P4 = gbpos[fi];
-        const uint32_t nb = gbn[fi];
-        const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
-        const uint32_t salt = 0x68e31da4u;
-        const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
-        const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
-        const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
-        const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
-        const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
-        cosine_hemisphere_u(N, u0, u1, d);
-        o[0] = P4.x + N[0] * 3e-2f; o[1] = P4.y + N[1] * 3e-2f; o[2] = P4.z + N[2] * 3e-2f;
-        oct = A.sort ? ((d[0] < 0.0f ? 1u : 0u) | (d[1] < 0.0f ? 2u : 0u) | (d[2] < 0.0f ? 4u : 0u)) : 0u;
-        local = atomicAdd(&oct_count[oct], 1u);
-    } else if (inside) {
-        Hit h;
-        miss(h);
-        store_hit(gi_hits + fi, h);
-        store_hit(sun_hits + fi, h);
+    float o[kGenPer][3], d[kGenPer][3];
+    uint32_t oct[kGenPer], local[kGenPer], tagv[kGenPer];
+    bool valid[kGenPer];
+#pragma unroll
+    for (int k = 0; k < kGenPer; ++k) {
+        const int64_t i = ((int64_t)blockIdx.x * kGenPer + k) * 256 + t;
+        const int64_t tile = i / per;
+        const int32_t in = (int32_t)(i % per);
+        const int32_t px = (int32_t)(tile % A.tiles_x) * A.tile_w + in % A.tile_w;
+        const int32_t py = (int32_t)(tile / A.tiles_x) * A.tile_h + in / A.tile_w;
+        const bool inside = px < A.w && py < A.h;
+        const size_t fi = inside ? (size_t)py * A.w + px : 0;
+        valid[k] = inside && gbpos[fi].w >= 0.0f;
+        tagv[k] = (uint32_t)fi;
+        oct[k] = 0u;
+        local[k] = 0u;
+        o[k][0] = o[k][1] = o[k][2] = 0.0f;
+        d[k][0] = d[k][1] = d[k][2] = 0.0f;
+        if (valid[k]) {
+            const float4 P4 = gbpos[fi];
+            const uint32_t nb = gbn[fi];
+            const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+            const uint32_t salt = 0x68e31da4u;
+            const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
+            const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
+            const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
+            const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
+            const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
+            cosine_hemisphere_u(N, u0, u1, d[k]);
+            o[k][0] = P4.x + N[0] * 3e-2f; o[k][1] = P4.y + N[1] * 3e-2f; o[k][2] = P4.z + N[2] * 3e-2f;
+            oct[k] = A.sort ? ((d[k][0] < 0.0f ? 1u : 0u) | (d[k][1] < 0.0f ? 2u : 0u) | (d[k][2] < 0.0f ? 4u : 0u)) : 0u;
+            local[k] = atomicAdd(&oct_count[oct[k]], 1u);
+        } else if (inside) {
+            Hit h;
+            miss(h);
+            store_hit(gi_hits + fi, h);
+            store_hit(sun_hits + fi, h);
+        }
     }
     __syncthreads();
     if (t == 0) {
@@ -109,10 +120,13 @@ __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, 
         wg_base = run ? atomicAdd(ctr + 0, (unsigned long long)run) : 0ull;
     }
     __syncthreads();
-    if (!valid) return;
-    const uint64_t slot = wg_base + oct_base[oct] + local;
-    rays[slot * 2 + 0] = make_float4(o[0], o[1], o[2], __uint_as_float((uint32_t)fi));
-    rays[slot * 2 + 1] = make_float4(d[0], d[1], d[2], A.reach);
+#pragma unroll
+    for (int k = 0; k < kGenPer; ++k) {
+        if (!valid[k]) continue;
+        const uint64_t slot = wg_base + oct_base[oct[k]] + local[k];
+        rays[slot * 2 + 0] = make_float4(o[k][0], o[k][1], o[k][2], __uint_as_float(tagv[k]));
+        rays[slot * 2 + 1] = make_float4(d[k][0], d[k][1], d[k][2], A.reach);
+    }
 }
 
 // Claims the next chunk of ray ids for the wave; false when the list is exhausted.
@@ -645,7 +659,7 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     unsigned long long *ctr = ctx->q_ctr;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(256), 0, ctx->stream, ctr, nullptr, 0);
-    hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen)), dim3(256), 0, ctx->stream, A,
+    hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen, 256 * kGenPer)), dim3(256), 0, ctx->stream, A,
                        reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctr, gi_hits, sun_hits);
     const float4 *list = ctx->q_rays;
     {
