@@ -177,9 +177,13 @@ int vf_read_skytop(vf_ctx *ctx, uint16_t *skytop);
 int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
 
 /* ---- traced passes ------------------------------------------------------------------------- */
-/* P::Gbuf (record.rs:327; gbuffer.comp:189-266, 470-480): one pinhole primary ray per pixel,
- * camera in air, world only.  Writes hits [w*h], and when non-NULL gbPos (RGBA32F: P, t; sky
- * (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5, a=255; sky 0).  Row 0 is the image bottom. */
+/* P::Gbuf (record.rs:327; gbuffer.comp:189-266, 470-480): one pinhole primary ray per pixel, world
+ * only.  A camera inside solid terrain starts each ray where it leaves the solid (exitSolid,
+ * common.glsl:990-1016; t counts from the camera, rays that only leave the world are misses); a
+ * camera in water traces with skipMat 5.  Writes hits [w*h], and when non-NULL gbPos (RGBA32F: P, t;
+ * sky (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5, a=255; sky 0).  Row 0 is the image bottom.
+ * A frame graph records the camera case with the pass: record it again after an edit of the
+ * camera's voxel. */
 int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
                      float *d_gb_pos, uint32_t *d_gb_normal);
 /* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map / foliage / grass

@@ -102,6 +102,8 @@ def lib():
             "vfo_sky_open": (C.c_float, [P, P, P]),
             "vfo_gi_rays": (None, [C.c_int32, C.c_int32, C.c_uint32, P, P, P, P]),
             "vfo_cosine_hemisphere": (None, [P, P, P]),
+            "vfo_voxel_mat_at": (C.c_uint32, [P, P]),
+            "vfo_exit_solid": (C.c_float, [P, P, P, C.c_int32]),
             "vfo_gi_path_ex": (None, [P, P, P, P, P, P, C.c_int32, C.c_int32, P, P, P, P]),
             "vfo_pass_gi_path": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32, P,
                                         C.c_int]),
@@ -340,6 +342,16 @@ class OracleWorld:
                                   int(frame) & 0xFFFFFFFF, int(n), int(terminate), int(salt) & 0xFFFFFFFF, _p(out),
                                   int(threads))
         return out
+
+
+    def voxel_mat_at(self, p):
+        p32 = f32(p)
+        return int(lib().vfo_voxel_mat_at(C.byref(self.s), _p(p32)))
+
+    def exit_solid(self, ro, rd, max_steps=2048):
+        """exitSolid (common.glsl:990-1016)."""
+        ro32, rd32 = f32(ro), f32(rd)
+        return float(lib().vfo_exit_solid(C.byref(self.s), _p(ro32), _p(rd32), int(max_steps)))
 
 
 def pcg(v):

@@ -641,6 +641,68 @@ void vfo_probe_dir(int32_t i, int32_t n_rays, uint32_t frame, float out[3]) {
     for (int k = 0; k < 3; ++k) out[k] = d[k] + 2.0f * c2v[k];
 }
 
+/* ---- camera inside solid terrain (gbuffer.comp:233-266) ---- */
+/* voxelMatAt (common.glsl:976-985) */
+uint32_t vfo_voxel_mat_at(const vfo_world *w, const float p[3]) {
+    return vfo_voxel(w, f2i(floorf(p[0])), f2i(floorf(p[1])), f2i(floorf(p[2])));
+}
+
+/* exitSolid (common.glsl:990-1016) */
+float vfo_exit_solid(const vfo_world *w, const float ro[3], const float rd[3], int32_t max_steps) {
+    int32_t mn[3] = {is_ring(w) ? w->win[0] : 0, 0, is_ring(w) ? w->win[2] : 0};
+    float sd[3], inv[3], up[3], t = 0.0f;
+    int32_t st[3], v[3];
+    for (int a = 0; a < 3; ++a) {
+        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        inv[a] = 1.0f / sd[a];
+        st[a] = sd[a] > 0.0f ? 1 : (sd[a] < 0.0f ? -1 : 0);
+        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
+        v[a] = f2i(floorf(ro[a]));
+    }
+    for (int32_t i = 0; i < max_steps; ++i) {
+        for (int a = 0; a < 3; ++a)
+            if (v[a] < mn[a] * 8 || v[a] >= (mn[a] + w->wb[a]) * 8) return -2.0f;
+        uint32_t g = w->grid[gidx(w, v[0] >> 3, v[1] >> 3, v[2] >> 3)];
+        if (g == 0u || g == 5u) return t;
+        int32_t size = 1, lo[3] = {v[0], v[1], v[2]};
+        if (g < 256u) {
+            size = 8;
+            for (int a = 0; a < 3; ++a) lo[a] = (v[a] >> 3) * 8;
+        } else {
+            uint32_t m = w->atlas[(size_t)(g - 256u) * 512u + (size_t)((v[2] & 7) * 64 + (v[1] & 7) * 8 + (v[0] & 7))];
+            if (m == 0u || m == 5u) return t;
+        }
+        float tf[3];
+        for (int a = 0; a < 3; ++a) tf[a] = (((float)lo[a] + up[a] * (float)size) - ro[a]) * inv[a];
+        int ax[3];
+        ax[0] = tf[0] < tf[1] && tf[0] < tf[2];
+        ax[1] = !ax[0] && tf[1] < tf[2];
+        ax[2] = !ax[0] && !ax[1];
+        t = ax[0] ? tf[0] : (ax[1] ? tf[1] : tf[2]);
+        for (int a = 0; a < 3; ++a) {
+            int32_t nv = imin(imax(f2i(floorf(ro[a] + sd[a] * t)), lo[a]), lo[a] + (size - 1));
+            v[a] = ax[a] ? (st[a] > 0 ? lo[a] + size : lo[a] - 1) : nv;
+        }
+    }
+    return -1.0f;
+}
+
+/* gbuffer.comp:233-266: the primary trace with the camera-inside-solid pre-step. */
+static void primary_trace(const vfo_world *w, const float cam[3], uint32_t cam_mat, const float rd[3], vfo_hit *h,
+                          vfo_count *cnt) {
+    float ro[3] = {cam[0], cam[1], cam[2]}, tskip = 0.0f;
+    int void_ray = 0;
+    if (cam_mat != 0u && cam_mat != 5u) {
+        tskip = vfo_exit_solid(w, cam, rd, 2048);
+        void_ray = tskip < 0.0f;
+        tskip = gmax(tskip, 0.0f);
+        for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
+    }
+    vfo_trace(w, ro, rd, 1e30f, cam_mat == 5u ? 5u : 0u, h, cnt);
+    h->t = h->t + tskip;
+    if (void_ray) h->hit = 0;
+}
+
 /* ---- passes ---- */
 static inline uint32_t unorm8(float v) { return (uint32_t)floorf(fclamp(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
 static inline float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
@@ -648,6 +710,7 @@ static inline float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f -
 void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H, vfo_hitrec *hits,
                       float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads) {
     vfo_count total = {0, 0, 0, 0};
+    const uint32_t cam_mat = vfo_voxel_mat_at(w, c->pos);
 #pragma omp parallel num_threads(nthreads(threads))
     {
         vfo_count cc = {0, 0, 0, 0};
@@ -657,8 +720,7 @@ void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H
             float rd[3];
             vfo_primary_dir(c->right, c->up, c->fwd, c->tan_half_fov, W, H, px, py, c->jitter, rd);
             vfo_hit h;
-            vfo_trace(w, c->pos, rd, 1e30f, 0u, &h, &cc);
-            h.t = h.t + 0.0f;   /* h.t += tSkip, tSkip = 0 with the camera in air (gbuffer.comp:266) */
+            primary_trace(w, c->pos, cam_mat, rd, &h, &cc);
             if (hits) to_rec(&h, &hits[i]);
             if (gb_pos) {
                 if (h.hit) {
@@ -690,6 +752,7 @@ void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int3
                            int32_t nrows, vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt,
                            int threads) {
     vfo_count total = {0, 0, 0, 0};
+    const uint32_t cam_mat = vfo_voxel_mat_at(w, c->pos);
 #pragma omp parallel num_threads(nthreads(threads))
     {
         vfo_count cc = {0, 0, 0, 0};
@@ -699,8 +762,7 @@ void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int3
             float rd[3];
             vfo_primary_dir(c->right, c->up, c->fwd, c->tan_half_fov, W, H, px, py, c->jitter, rd);
             vfo_hit h;
-            vfo_trace(w, c->pos, rd, 1e30f, 0u, &h, &cc);
-            h.t = h.t + 0.0f;
+            primary_trace(w, c->pos, cam_mat, rd, &h, &cc);
             to_rec(&h, &hits[i]);
             if (h.hit) {
                 for (int a = 0; a < 3; ++a) gb_pos[i * 4 + a] = c->pos[a] + rd[a] * h.t;

@@ -112,6 +112,11 @@ void vfo_primary_dir(const float cam_right[3], const float cam_up[3], const floa
                      int32_t w, int32_t h, int32_t px, int32_t py, const float jitter[2], float out[3]);
 void vfo_probe_dir(int32_t i, int32_t n_rays, uint32_t frame, float out[3]);
 
+/* --- camera inside solid terrain (gbuffer.comp:233-266) --- */
+uint32_t vfo_voxel_mat_at(const vfo_world *w, const float p[3]);
+/* exitSolid (common.glsl:990-1016): distance to the first non-solid voxel; -2 leaves the world, -1 too long. */
+float vfo_exit_solid(const vfo_world *w, const float ro[3], const float rd[3], int32_t max_steps);
+
 /* --- passes --- */
 typedef struct vfo_cam {
     float pos[3], right[3], up[3], fwd[3];
@@ -131,7 +136,8 @@ typedef struct vfo_sky {
     const uint16_t *skytop;   /* (wbx*8) x (wbz*8), index z*(wbx*8) + x */
 } vfo_sky;
 
-/* gbuffer.comp primary pass (world only, camera in air): hits + gbPos (xyz, t) + gbNormal (RGBA8). */
+/* gbuffer.comp primary pass (world only; a camera inside solid terrain or water as gbuffer.comp:233-266):
+ * hits + gbPos (xyz, t) + gbNormal (RGBA8). */
 void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height,
                       vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads);
 void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height, int32_t y0,

@@ -316,3 +316,72 @@ def test_fused_primary_shadow_equals_two_passes(ctx):
     ctx.sync()
     for x, y in ((hits, f_hits), (gbp, f_gbp), (gbn, f_gbn), (sh, f_sh), (shh, f_shh)):
         assert torch.equal(x, y)
+
+
+def _camera_spots(w):
+    """A camera inside solid terrain and one inside water of the fixed world."""
+    solid = water = None
+    for x in range(0, 256):
+        for z in range(0, 256):
+            gy = w.ground(x, z)
+            if solid is None and x >= 60 and z >= 60 and gy > 12 and w.voxel(x, gy - 6, z) not in (0, 5):
+                solid = [x + 0.37, gy - 5.6, z + 0.61]
+            if water is None and w.voxel(x, 39, z) == 5 and w.voxel(x, 38, z) == 5:   # water below level 40
+                water = [x + 0.52, 38.55, z + 0.33]
+            if solid is not None and water is not None:
+                return solid, water
+    return solid, water
+
+
+def test_camera_inside_solid_or_water_matches_oracle(ctx):
+    """gbuffer.comp:233-266: from inside solid terrain each ray starts where it leaves the solid (rays
+    that only leave the world are misses); from inside water the ray passes through water."""
+    w, a, ow = world(1)
+    upload(ctx, w, a)
+    solid, water = _camera_spots(w)
+    assert solid is not None and water is not None
+    assert ow.voxel_mat_at(solid) not in (0, 5) and ow.voxel_mat_at(water) == 5
+    W, H = 320, 180
+    for pos, yaw, pitch in ((solid, 0.7, -0.2), (solid, 2.5, 0.6), (water, 1.9, -0.1)):
+        pu, cam = camera(pos, yaw, pitch)
+        hits, gbp, gbn = primary_gpu(ctx, pu, W, H)
+        oh, opos, onrm, _ = ow.pass_primary(cam, W, H)
+        g = vf.hits_to_numpy(hits)
+        bad = hit_diff(g, oh)
+        assert bad.size == 0, (pos, bad.size, g[bad[:3]], oh[bad[:3]])
+        assert np.array_equal(gbp.cpu().numpy().reshape(-1, 4).view(np.uint32), opos.view(np.uint32))
+        assert np.array_equal(gbn.cpu().numpy().view(np.uint32), onrm)
+        assert g["hit"].mean() > 0.01
+        # the fused kernel takes the same pre-step
+        f_hits, f_gbp, f_gbn = dev_hits(W * H), torch.zeros_like(gbp), torch.zeros_like(gbn)
+        sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        ctx.trace_primary_shadow(pu, W, H, f_hits, f_gbp, f_gbn, sh)
+        ctx.sync()
+        assert torch.equal(hits, f_hits) and torch.equal(gbp, f_gbp) and torch.equal(gbn, f_gbn)
+    # inside solid the pre-step matters: some rays start past tSkip > 0
+    pu, cam = camera(solid, 0.7, -0.2)
+    oh, opos, _, _ = ow.pass_primary(cam, W, H)
+    assert np.any((oh["hit"] == 1) & (oh["t"] > 1.0))
+
+
+def test_batch_view_from_inside_solid_equals_plain_launch(ctx):
+    w, a, _ = world(1)
+    upload(ctx, w, a)
+    solid, water = _camera_spots(w)
+    views = [(solid, 0.7, -0.2, 200, 120), (water, 1.9, -0.1, 128, 96), ([60.5, 100.5, 60.5], 0.4, -0.3, 96, 64)]
+    reqs, outs, plain = [], [], []
+    for pos, yaw, pitch, vw, vh in views:
+        pu, _ = camera(pos, yaw, pitch)
+        o = dev_hits(vw * vh)
+        reqs.append(ctx.view_request(pu, vw, vh, o))
+        outs.append(o)
+        ref = dev_hits(vw * vh)
+        ctx.trace_primary(pu, vw, vh, ref)
+        plain.append(ref)
+    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 8), 128, refill=0)):
+        for o in outs:
+            o.zero_()
+        ctx.trace_batch(reqs, cfg)
+        ctx.sync()
+        for k, (o, ref) in enumerate(zip(outs, plain)):
+            assert torch.equal(o, ref), (k, cfg.refill)

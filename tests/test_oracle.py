@@ -362,3 +362,78 @@ def test_two_bounce_path_is_its_head_plus_tail():
         # the throughput is albedo x giScale of the bounce-1 hit
         assert np.all(t["thr"] >= 0.0) and np.all(t["thr"] <= 1.0)
     assert tails > 40
+
+
+# --- camera inside solid terrain (gbuffer.comp:233-266, exitSolid common.glsl:990-1016) ------------
+def _solid_world():
+    """64^3 voxels: a solid block [8,40) x [8,24) x [8,40) of material 3 (whole bricks inside, partial ones
+    at its faces) with a pocket of water at [24,28)^3 and a cave of air at [12,16) x [12,16) x [30,34)."""
+    vox = np.zeros((64, 64, 64), np.uint8)   # indexed (x, y, z); from_dense takes (z, y, x)
+    vox[8:40, 8:24, 8:40] = 3
+    vox[24:28, 16:20, 24:28] = 5
+    vox[12:16, 12:16, 30:34] = 0
+    return vox
+
+
+def _march_exit_f64(vox, ro, rd):
+    """First non-solid voxel along the ray by an FP64 voxel march: its entry t; -2 out of the box."""
+    ro, rd = np.asarray(ro, np.float64), np.asarray(rd, np.float64)
+    sd = np.where(np.abs(rd) < 1e-5, 1e-5, rd)
+    v = np.floor(ro).astype(np.int64)
+    st = np.where(sd > 0, 1, -1)
+    tmax = ((v + (st > 0)) - ro) / sd
+    tdel = np.abs(1.0 / sd)
+    t = 0.0
+    for _ in range(4096):
+        if np.any(v < 0) or np.any(v >= 64):
+            return -2.0
+        m = vox[v[0], v[1], v[2]]
+        if m == 0 or m == 5:
+            return t
+        a = int(np.argmin(tmax))
+        t = tmax[a]
+        v[a] += st[a]
+        tmax[a] += tdel[a]
+    return -1.0
+
+
+def test_exit_solid_closed_forms():
+    vox = _solid_world()
+    _, _, ow = dense_world(vox.transpose(2, 1, 0))
+    # axis-aligned from inside the block: the distance to its face
+    assert ow.exit_solid([20.25, 10.5, 20.5], [1.0, 0.0, 0.0]) == pytest.approx(40.0 - 20.25, abs=1e-5)
+    assert ow.exit_solid([20.25, 10.5, 20.5], [0.0, 1.0, 0.0]) == pytest.approx(24.0 - 10.5, abs=1e-5)
+    assert ow.exit_solid([20.25, 10.5, 20.5], [0.0, 0.0, -1.0]) == pytest.approx(20.5 - 8.0, abs=1e-5)
+    # into the water pocket: water is not solid
+    assert ow.exit_solid([26.5, 12.5, 26.5], [0.0, 1.0, 0.0]) == pytest.approx(16.0 - 12.5, abs=1e-5)
+    # a camera in air: 0; the material queries
+    assert ow.exit_solid([50.5, 50.5, 50.5], [0.3, 0.4, 0.5]) == 0.0
+    assert ow.voxel_mat_at([20.2, 10.1, 20.9]) == 3 and ow.voxel_mat_at([26.5, 17.5, 26.5]) == 5
+    assert ow.voxel_mat_at([50.5, 50.5, 50.5]) == 0 and ow.voxel_mat_at([-1.0, 5.0, 5.0]) == 0
+
+
+def test_exit_solid_matches_fp64_voxel_march():
+    """exitSolid crosses whole uniform bricks in one step; the voxel-by-voxel FP64 march must find the
+    same exit distance (and -2 when the ray leaves the world while still in solid)."""
+    vox = _solid_world()
+    vox[:, 0:8, :] = 2   # a solid floor reaching the world's edge: rays down leave the world in solid
+    _, _, ow = dense_world(vox.transpose(2, 1, 0))
+    rng = np.random.default_rng(21)
+    n_out = 0
+    for _ in range(600):
+        ro = rng.uniform([8.0, 1.0, 8.0], [40.0, 24.0, 40.0])
+        if vox[int(ro[0]), int(ro[1]), int(ro[2])] in (0, 5):
+            continue
+        d = rng.normal(size=3)
+        if rng.uniform() < 0.15:
+            d[rng.integers(0, 3)] = 0.0
+        d /= np.linalg.norm(d)
+        ro32, d32 = ro.astype(np.float32), d.astype(np.float32)
+        got = ow.exit_solid(ro32, d32)
+        want = _march_exit_f64(vox, ro32, d32)
+        if want == -2.0:
+            n_out += 1
+            assert got == -2.0
+        else:
+            assert got == pytest.approx(want, rel=1e-5, abs=1e-4), (ro32, d32)
+    assert n_out > 10

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "vf_device.h"
 #include "vf_internal.h"
@@ -37,6 +38,8 @@ struct vf_ctx {
     float4 *palette = nullptr;
     int32_t c1d[3] = {0, 0, 0}, c2d[3] = {0, 0, 0};
     uint32_t n_slots = 0;
+    std::vector<uint32_t> host_grid;   // host copies of the grid and of the slots' occupancy bits
+    std::vector<uint64_t> host_occ;    // (8 words per slot): classify the camera voxel without a device read
     size_t n_cells = 0, n_cells_tiled = 0;
     bool counting = false;
     unsigned long long *d_cnt = nullptr;
@@ -67,4 +70,6 @@ int check_ready(vf_ctx *ctx, const char *what);
 int no_capture(vf_ctx *ctx, const char *what);
 void free_queue(vf_ctx *ctx);
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+void slot_occupancy(const uint8_t *vox, uint64_t out[8]);
+void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n_slots);
 }  // namespace vf

@@ -77,6 +77,9 @@ __device__ __forceinline__ uint32_t cell_index(const DevWorld &W, int32_t x, int
     return tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
 }
 
+// voxelMatAt (common.glsl:976-985): the world voxel's material at p (0 = air or outside the loaded box).
+__device__ __forceinline__ uint32_t voxel_mat_at(const DevWorld &W, const float p[3]);
+
 // 64 occupancy bits of brick layer lz (two 32-bit words) out of the 512-bit mask.
 __device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, const uint4 &m2, const uint4 &m3, int lz,
                                            uint32_t &lo, uint32_t &hi) {
@@ -151,10 +154,12 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     return false;
 }
 
-// traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5).
+// traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5), or -1 to
+// take it from skip_rt (one inlined copy serving both).
 template <int SKIP, bool COUNT>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
-                                            Cnt &c) {
+                                            Cnt &c, const int skip_rt = 0) {
+    const int sk = SKIP >= 0 ? SKIP : skip_rt;
     h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
     if (COUNT) c.rays++;
     float sd[3], inv[3], up[3], tmin3[3], tmax3[3];
@@ -202,7 +207,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
         uint32_t g = cw.x;
         const uint32_t meta = cw.y;
         if (COUNT) { c.outer++; c.bytes += 4; }
-        if (SKIP == 5) {
+        if (sk == 5) {
             if (g == 5u) g = 0u;
             else if (g >= 256u) {
                 if (COUNT) c.bytes += 1;
@@ -210,8 +215,8 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             }
         }
         if (g >= 256u) {
-            if (brick_voxels<COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, SKIP)) return;
-        } else if (g != 0u && g != (uint32_t)SKIP) {
+            if (brick_voxels<COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
+        } else if (g != 0u && g != (uint32_t)sk) {
             const float e = t + 1e-3f;
             h.hit = 1; h.t = t; h.face = face; h.mat = g;
 #pragma unroll
@@ -223,7 +228,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             int32_t Cf = 0, cb[3] = {0, 0, 0};
             int32_t d = 0;
             if (W.use_dist) {
-                d = (int32_t)(SKIP == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
+                d = (int32_t)(sk == 5 ? ((meta >> 8) & 255u) : (meta & 255u));
                 if (COUNT) c.bytes += 1;
             }
             if (W.use_dist && d > 1) {
@@ -532,3 +537,88 @@ __device__ __forceinline__ int32_t ray_outer(const DevWorld &W, RayState &r, Hit
         return RAY_MISS;
     return RAY_RUNNING;
 }
+
+// ------------------------------------------------------------------------------------------------
+// camera inside solid terrain (gbuffer.comp:233-266)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t voxel_mat_at(const DevWorld &W, const float p[3]) {
+    const int32_t x = f2i(floorf(p[0])), y = f2i(floorf(p[1])), z = f2i(floorf(p[2]));
+    if (x < W.minb[0] * 8 || y < W.minb[1] * 8 || z < W.minb[2] * 8 || x >= (W.minb[0] + W.wb[0]) * 8 ||
+        y >= (W.minb[1] + W.wb[1]) * 8 || z >= (W.minb[2] + W.wb[2]) * 8)
+        return 0u;
+    const uint32_t g = W.cells[cell_index(W, x >> 3, y >> 3, z >> 3)].x;
+    if (g == 0u) return 0u;
+    if (g < 256u) return g;
+    return W.atlas[(size_t)(g - 256u) * 512u + (uint32_t)(((z & 7) << 6) | ((y & 7) << 3) | (x & 7))];
+}
+
+// exitSolid (common.glsl:990-1016): the distance along rd to the first voxel that is not solid (air or
+// water); -2 if the ray leaves the world first, -1 if it is still solid after max_steps.  Uniform
+// solid bricks are crossed in one step; each step crosses exactly one face.
+__device__ float exit_solid(const DevWorld &W, const float ro[3], const float rd[3], int32_t max_steps) {
+    float sd[3], inv[3], up[3];
+    int32_t st[3], v[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        inv[a] = 1.0f / sd[a];
+        st[a] = sd[a] > 0.0f ? 1 : -1;
+        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
+        v[a] = f2i(floorf(ro[a]));
+    }
+    float t = 0.0f;
+    for (int32_t i = 0; i < max_steps; ++i) {
+        bool out = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) out = out || v[a] < W.minb[a] * 8 || v[a] >= (W.minb[a] + W.wb[a]) * 8;
+        if (out) return -2.0f;
+        const uint32_t g = W.cells[cell_index(W, v[0] >> 3, v[1] >> 3, v[2] >> 3)].x;
+        if (g == 0u || g == 5u) return t;
+        int32_t size = 1, lo[3] = {v[0], v[1], v[2]};
+        if (g < 256u) {   // a uniform solid brick: its far side
+            size = 8;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) lo[a] = (v[a] >> 3) * 8;
+        } else {
+            const uint32_t m =
+                W.atlas[(size_t)(g - 256u) * 512u + (uint32_t)(((v[2] & 7) << 6) | ((v[1] & 7) << 3) | (v[0] & 7))];
+            if (m == 0u || m == 5u) return t;
+        }
+        float tf[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tf[a] = (((float)lo[a] + up[a] * (float)size) - ro[a]) * inv[a];
+        const bool ax = tf[0] < tf[1] && tf[0] < tf[2];
+        const bool ay = !ax && tf[1] < tf[2];
+        const bool az = !ax && !ay;
+        t = ax ? tf[0] : (ay ? tf[1] : tf[2]);
+        const bool sel[3] = {ax, ay, az};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int32_t nv = imin(imax(f2i(floorf(ro[a] + sd[a] * t)), lo[a]), lo[a] + (size - 1));
+            v[a] = sel[a] ? (st[a] > 0 ? lo[a] + size : lo[a] - 1) : nv;
+        }
+    }
+    return -1.0f;
+}
+
+// The primary ray of gbuffer.comp (233-266): a camera inside solid terrain (cam_mat = voxelMatAt(camPos)
+// neither air nor water) starts each ray where it leaves that solid (tSkip, added to h.t), and a ray that
+// only leaves it out of the world is a miss; a camera in water traces with skipMat 5 (WATER_CLEAR).
+template <bool COUNT>
+__device__ __forceinline__ void primary_trace(const DevWorld &W, const float cam[3], uint32_t cam_mat,
+                                              const float rd[3], Hit &h, Cnt &c) {
+    float ro[3] = {cam[0], cam[1], cam[2]};
+    float tskip = 0.0f;
+    bool void_ray = false;
+    if (cam_mat != 0u && cam_mat != 5u) {
+        tskip = exit_solid(W, cam, rd, 2048);
+        void_ray = tskip < 0.0f;
+        tskip = gmax(tskip, 0.0f);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
+    }
+    trace_world<-1, COUNT>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
+    h.t = h.t + tskip;
+    if (void_ray) h.hit = 0;
+}
+

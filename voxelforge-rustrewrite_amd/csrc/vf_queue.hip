@@ -292,10 +292,13 @@ __device__ __forceinline__ int find_req(const BatchArgs &B, uint64_t id) {
     return r;
 }
 
-// Ray `local` of request q: a primary ray of a view (gbuffer.comp:194-202, tile order), or probe ray
-// (probes.glsl:155-170).  out: the output index; skip: the ray's skipMat.
-__device__ __forceinline__ void batch_ray(const ReqDev &q, uint64_t local, float ro[3], float rd[3], float &tcap,
-                                          uint32_t &out, int &skip) {
+// Ray `local` of request q: a primary ray of a view (gbuffer.comp:194-202, tile order, with the
+// camera-inside-solid pre-step of 233-266: tskip is added to the hit's t, void_ray makes it a miss), or a
+// probe ray (probes.glsl:155-170).  out: the output index; skip: the ray's skipMat.
+__device__ __forceinline__ void batch_ray(const DevWorld &W, const ReqDev &q, uint64_t local, float ro[3], float rd[3],
+                                          float &tcap, uint32_t &out, int &skip, float &tskip, bool &void_ray) {
+    tskip = 0.0f;
+    void_ray = false;
     if (q.kind == 0) {
         const int32_t per = q.tile_w * q.tile_h;
         const int64_t tile = (int64_t)(local / (uint64_t)per);
@@ -316,7 +319,16 @@ __device__ __forceinline__ void batch_ray(const ReqDev &q, uint64_t local, float
         }
         normalize3(rd);
         tcap = 1e30f;
-        skip = 0;
+        const uint32_t cam_mat = voxel_mat_at(W, ro);
+        skip = cam_mat == 5u ? 5 : 0;
+        if (cam_mat != 0u && cam_mat != 5u) {
+            const float cam[3] = {ro[0], ro[1], ro[2]};
+            tskip = exit_solid(W, cam, rd, 2048);
+            void_ray = tskip < 0.0f;
+            tskip = gmax(tskip, 0.0f);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ro[i] = cam[i] + rd[i] * tskip;
+        }
         out = (px < q.w && py < q.h) ? (uint32_t)(py * q.w + px) : 0xFFFFFFFFu;
         return;
     }
@@ -376,6 +388,7 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
     bool active = false;
     int32_t req = 0, skip = 0;
     uint32_t out = 0;
+    float tskip = 0.0f;
     uint64_t pool = 0, pool_end = 0;
     int32_t pool_req = 0;
     bool drained = false;
@@ -409,10 +422,18 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
                 const uint64_t id = pool + rk;
                 req = pool_req;
                 float ro[3], rd[3], tcap;
-                batch_ray(B.req[req], id - B.req[req].first, ro, rd, tcap, out, skip);
-                if (out == 0xFFFFFFFFu) finished = true;   // tile padding outside the view: no ray
-                else if (ray_begin<COUNT>(W, ro, rd, tcap, r, h, c) == RAY_RUNNING) active = true;
-                else finished = true;
+                bool void_ray;
+                batch_ray(W, B.req[req], id - B.req[req].first, ro, rd, tcap, out, skip, tskip, void_ray);
+                if (out == 0xFFFFFFFFu) {
+                    finished = true;   // tile padding outside the view: no ray
+                } else if (void_ray) {
+                    miss(h);
+                    finished = true;
+                } else if (ray_begin<COUNT>(W, ro, rd, tcap, r, h, c) == RAY_RUNNING) {
+                    active = true;
+                } else {
+                    finished = true;
+                }
             }
             pool += take;
             idle = __ballot(!active && !finished);
@@ -425,7 +446,10 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
                 finished = true;
             }
         }
-        if (finished && out != 0xFFFFFFFFu) store_hit(B.req[req].hits + out, h);
+        if (finished && out != 0xFFFFFFFFu) {
+            h.t = h.t + tskip;
+            store_hit(B.req[req].hits + out, h);
+        }
         // retired rays are counted per wave (wave-uniform acc_req / acc_cnt) and subtracted from the
         // request's remaining count in one atomic when the request changes, the wave claimed a new
         // chunk, every 64 trips, or at the end; the atomic that reaches zero stamps the completion
@@ -536,14 +560,17 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, 
         for (uint64_t base = b; base < end; base += 64) {
             const uint64_t id = base + lane_id();
             if (id < end) {
-                float ro[3], rd[3], tcap;
+                float ro[3], rd[3], tcap, tskip;
                 uint32_t out;
                 int skip;
-                batch_ray(q, id - q.first, ro, rd, tcap, out, skip);
+                bool void_ray;
+                batch_ray(W, q, id - q.first, ro, rd, tcap, out, skip, tskip, void_ray);
                 if (out != 0xFFFFFFFFu) {
                     Hit h;
-                    if (q.kind == 0) trace_world<0, COUNT>(W, ro, rd, tcap, h, c);
+                    if (skip == 0) trace_world<0, COUNT>(W, ro, rd, tcap, h, c);
                     else trace_world<5, COUNT>(W, ro, rd, tcap, h, c);
+                    h.t = h.t + tskip;
+                    if (void_ray) h.hit = 0;
                     store_hit(q.hits + out, h);
                 }
             }

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vf_trace.h"
@@ -40,18 +41,19 @@ struct PrimArgs {
     float thf;
     float jit[2];
     int32_t w, h, tiles_x, ntiles;
+    int32_t cam_air;   // the host's grid copy shows the camera in an empty brick (or outside the world)
 };
 
-// gbuffer.comp primary ray (gbuffer.comp:194-202, 260-266, 470-480) for a 16x16 tile per workgroup.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn,
-                                                 unsigned long long *cnt) {
-    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
-    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
-    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
-    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
-    if (px >= A.w || py >= A.h) return;
+// The camera voxel decides which primary kernel variant's results stand (gbuffer.comp:233-241).  A camera
+// in air (the common case): every ray starts at the camera (a uniform origin, kept in scalar registers)
+// with skipMat 0; the camera voxel is loaded at the start but only checked before the results are
+// written, so its latency overlaps the trace.  A camera inside solid terrain (each ray starts where it
+// leaves the solid: a per-lane origin) or in water (skipMat 5): the SPECIAL variant, a small persistent
+// grid that returns at once for a camera in air.  Both are launched.
+__device__ __forceinline__ bool camera_special(uint32_t cam_mat) { return cam_mat != 0u; }
+
+// Pixel (px, py) of a 16x16-tile pass: its primary ray direction (gbuffer.comp:194-202).
+__device__ __forceinline__ void primary_dir(const PrimArgs &A, int32_t px, int32_t py, float rd[3]) {
     const float sx = ((float)px + 0.5f) + A.jit[0];
     const float sy = ((float)py + 0.5f) + A.jit[1];
     const float nx = (sx / (float)A.w) * 2.0f - 1.0f;
@@ -59,14 +61,35 @@ __global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit 
     const float aspect = (float)A.w / (float)A.h;
     const float a = (nx * aspect) * A.thf;
     const float b = ny * A.thf;
-    float rd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) rd[i] = (A.fwd[i] + A.right[i] * a) + A.up[i] * b;
     normalize3(rd);
-    Cnt c = {0, 0, 0, 0};
+}
+
+template <bool COUNT, bool SPECIAL>
+__device__ __forceinline__ void primary_hit(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, const float rd[3],
+                                            Hit &h, Cnt &c) {
+    if (SPECIAL) primary_trace<COUNT>(W, A.pos, cam_mat, rd, h, c);
+    else trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
+}
+
+// Tile L's pixel for this thread (8x8 per wave, 16x16 per workgroup).
+__device__ __forceinline__ void tile_pixel_of(int32_t L, int32_t tiles_x, int32_t &px, int32_t &py) {
+    const int32_t tx = L % tiles_x, ty = L / tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+}
+
+// gbuffer.comp primary ray (gbuffer.comp:194-202, 233-266, 470-480) for a 16x16 tile per workgroup.
+template <bool COUNT, bool SPECIAL>
+__device__ __forceinline__ void primary_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px, int32_t py,
+                                              vf_hit *hits, float4 *gbpos, uint32_t *gbn, Cnt &c) {
+    float rd[3];
+    primary_dir(A, px, py, rd);
     Hit h;
-    trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
-    h.t = h.t + 0.0f;   // h.t += tSkip (0: camera in air)
+    primary_hit<COUNT, SPECIAL>(W, A, cam_mat, rd, h, c);
+    if (camera_special(cam_mat) != SPECIAL) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     store_hit(hits + i, h);
     if (gbpos) {
@@ -85,7 +108,28 @@ __global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit 
         }
         gbn[i] = v;
     }
-    flush_count<COUNT>(c, cnt);
+}
+
+// SPECIAL = false: one workgroup per tile (XCD-remapped); SPECIAL = true: a small persistent grid over
+// the tiles.
+template <bool COUNT, bool SPECIAL>
+__global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn,
+                                                 unsigned long long *cnt) {
+    const uint32_t cam_mat = A.cam_air ? 0u : voxel_mat_at(W, A.pos);
+    if (SPECIAL && !camera_special(cam_mat)) return;
+    Cnt c = {0, 0, 0, 0};
+    if (!SPECIAL) {
+        int32_t px, py;
+        tile_pixel_of(xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x), A.tiles_x, px, py);
+        if (px < A.w && py < A.h) primary_pixel<COUNT, false>(W, A, cam_mat, px, py, hits, gbpos, gbn, c);
+    } else {
+        for (int32_t L = (int32_t)blockIdx.x; L < A.ntiles; L += (int32_t)gridDim.x) {
+            int32_t px, py;
+            tile_pixel_of(L, A.tiles_x, px, py);
+            if (px < A.w && py < A.h) primary_pixel<COUNT, true>(W, A, cam_mat, px, py, hits, gbpos, gbn, c);
+        }
+    }
+    if (camera_special(cam_mat) == SPECIAL) flush_count<COUNT>(c, cnt);   // only the variant whose results stand
 }
 
 struct TileArgs {
@@ -134,31 +178,15 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx
 // gbuffer.comp then shadow_trace.comp for the same pixel in one kernel: the sun ray starts from the
 // G-buffer values this lane has just written (position, RGBA8-quantised normal), exactly what the
 // separate shadow pass would read back, so the results are those of the two passes.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
-                                                        float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
-                                                        unsigned long long *cnt) {
-    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
-    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
-    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
-    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
-    if (px >= A.w || py >= A.h) return;
-    const float sxx = ((float)px + 0.5f) + A.jit[0];
-    const float syy = ((float)py + 0.5f) + A.jit[1];
-    const float nx = (sxx / (float)A.w) * 2.0f - 1.0f;
-    const float ny = (syy / (float)A.h) * 2.0f - 1.0f;
-    const float aspect = (float)A.w / (float)A.h;
-    const float a = (nx * aspect) * A.thf;
-    const float b = ny * A.thf;
+template <bool COUNT, bool SPECIAL>
+__device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px,
+                                                     int32_t py, const float sun[3], vf_hit *hits, float4 *gbpos,
+                                                     uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c) {
     float rd[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) rd[i] = (A.fwd[i] + A.right[i] * a) + A.up[i] * b;
-    normalize3(rd);
-    Cnt c = {0, 0, 0, 0};
+    primary_dir(A, px, py, rd);
     Hit h;
-    trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
-    h.t = h.t + 0.0f;
+    primary_hit<COUNT, SPECIAL>(W, A, cam_mat, rd, h, c);
+    if (camera_special(cam_mat) != SPECIAL) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     store_hit(hits + i, h);
     float4 p;
@@ -175,7 +203,6 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp:21-34, 87, 101-105 on (p, nq)
-    const float sun[3] = {sx, sy, sz};
     Hit hs;
     hs.hit = 0; hs.t = -1.0f; hs.face = 0; hs.mat = 0; hs.cell[0] = hs.cell[1] = hs.cell[2] = 0;
     float vis, tb;
@@ -196,7 +223,30 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
     const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
     shadow[i] = hv | (ht << 16);
     if (shits) store_hit(shits + i, hs);
-    flush_count<COUNT>(c, cnt);
+}
+
+template <bool COUNT, bool SPECIAL>
+__global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
+                                                        float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
+                                                        unsigned long long *cnt) {
+    const uint32_t cam_mat = A.cam_air ? 0u : voxel_mat_at(W, A.pos);
+    if (SPECIAL && !camera_special(cam_mat)) return;
+    const float sun[3] = {sx, sy, sz};
+    Cnt c = {0, 0, 0, 0};
+    if (!SPECIAL) {
+        int32_t px, py;
+        tile_pixel_of(xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x), A.tiles_x, px, py);
+        if (px < A.w && py < A.h)
+            primary_shadow_pixel<COUNT, false>(W, A, cam_mat, px, py, sun, hits, gbpos, gbn, shadow, shits, c);
+    } else {
+        for (int32_t L = (int32_t)blockIdx.x; L < A.ntiles; L += (int32_t)gridDim.x) {
+            int32_t px, py;
+            tile_pixel_of(L, A.tiles_x, px, py);
+            if (px < A.w && py < A.h)
+                primary_shadow_pixel<COUNT, true>(W, A, cam_mat, px, py, sun, hits, gbpos, gbn, shadow, shits, c);
+        }
+    }
+    if (camera_special(cam_mat) == SPECIAL) flush_count<COUNT>(c, cnt);   // only the variant whose results stand
 }
 
 struct GiArgs {
@@ -642,6 +692,81 @@ int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], 
 
 }  // namespace
 
+namespace {
+// The variants of a primary kernel: the tile grid for a camera in air, and a persistent grid of one
+// workgroup per compute unit for a camera in solid terrain or water (it returns at once otherwise); the
+// host's grid copy launches only the one that applies unless the camera is in a detailed brick.  A frame
+// graph records that choice with the camera (record it again after an edit of the camera's brick).
+}  // namespace
+
+namespace vf {
+// Occupancy bits of one 512-voxel slot: word lz, bit ly * 8 + lx = voxel != 0.
+void slot_occupancy(const uint8_t *vox, uint64_t out[8]) {
+    for (int w = 0; w < 8; ++w) {
+        uint64_t m = 0ull;
+        for (int b = 0; b < 64; ++b) m |= (uint64_t)(vox[w * 64 + b] != 0u) << b;
+        out[w] = m;
+    }
+}
+void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n_slots) {
+    occ.assign((size_t)n_slots * 8u, 0ull);
+    if (!n_slots || !atlas) return;
+    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (size_t sl = t; sl < n_slots; sl += nt) slot_occupancy(atlas + sl * 512u, &occ[sl * 8u]);
+        });
+    for (auto &x : th) x.join();
+}
+}  // namespace vf
+
+namespace {
+// The camera voxel from the host copies of the grid and of the slots' occupancy bits: 0 = air (or
+// outside the loaded world), 1 = solid or water (the SPECIAL variant decides which on the device).
+int camera_class(const vf_ctx *ctx, const float pos[3]) {
+    const DevWorld &W = ctx->W;
+    int32_t v[3];
+    for (int a = 0; a < 3; ++a) {
+        const float f = std::floor(pos[a]);
+        v[a] = f != f ? 0 : (f >= 2147483648.0f ? INT32_MAX : (f <= -2147483648.0f ? INT32_MIN : (int32_t)f));
+        if (v[a] < W.minb[a] * 8 || v[a] >= (W.minb[a] + W.wb[a]) * 8) return 0;
+    }
+    int32_t bx = v[0] >> 3, by = v[1] >> 3, bz = v[2] >> 3;
+    if (W.ring) { bx &= W.wb[0] - 1; bz &= W.wb[2] - 1; }
+    const uint32_t g = ctx->host_grid[((size_t)bz * W.wb[1] + by) * W.wb[0] + bx];
+    if (g < 256u) return g == 0u ? 0 : 1;
+    const uint32_t k = (uint32_t)(((v[2] & 7) << 6) | ((v[1] & 7) << 3) | (v[0] & 7));
+    return ((ctx->host_occ[(size_t)(g - 256u) * 8u + (k >> 6)] >> (k & 63u)) & 1ull) ? 1 : 0;
+}
+
+template <bool COUNT>
+void launch_primary(vf_ctx *ctx, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn) {
+    const int cls = camera_class(ctx, A.pos);
+    A.cam_air = cls == 0 ? 1 : 0;
+    if (cls != 1)
+        hipLaunchKernelGGL((k_primary<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits, gbpos,
+                           gbn, ctx->d_cnt);
+    const int32_t g = A.ntiles < ctx->n_cu ? A.ntiles : ctx->n_cu;
+    if (cls != 0)
+        hipLaunchKernelGGL((k_primary<COUNT, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, A, hits, gbpos, gbn,
+                           ctx->d_cnt);
+}
+template <bool COUNT>
+void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits, float4 *gbpos, uint32_t *gbn,
+                           uint32_t *shadow, vf_hit *shits) {
+    const int cls = camera_class(ctx, A.pos);
+    A.cam_air = cls == 0 ? 1 : 0;
+    if (cls != 1)
+        hipLaunchKernelGGL((k_primary_shadow<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0],
+                           s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
+    const int32_t g = A.ntiles < ctx->n_cu ? A.ntiles : ctx->n_cu;
+    if (cls != 0)
+        hipLaunchKernelGGL((k_primary_shadow<COUNT, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
+                           hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
+}
+}  // namespace
+
 extern "C" {
 
 int vf_abi_version(void) { return VF_ABI_VERSION; }
@@ -805,6 +930,8 @@ int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
     W.use_dist = 1;
     W.max_outer = 3 * (W.wb[0] + W.wb[1] + W.wb[2]) + 4;
     ctx->n_cells = n;
+    ctx->host_grid.assign(d->grid, d->grid + n);
+    vf::host_occupancy(ctx->host_occ, d->atlas, d->n_slots);
     ctx->n_cells_tiled = (size_t)W.tdim[0] * W.tdim[1] * W.tdim[2] * 64;
     ctx->n_slots = d->n_slots;
     const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
@@ -907,6 +1034,8 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
         hipLaunchKernelGGL(k_update_cells, dim3(blocks_for(n_cells)), dim3(256), 0, ctx->stream, ctx->grid, (const uint32_t *)d_ci,
                            (const uint32_t *)d_cv, n_cells);
     }
+    for (uint32_t i = 0; i < n_cells; ++i) ctx->host_grid[cell_idx[i]] = cell_val[i];
+    for (uint32_t i = 0; i < n_slot_updates; ++i) vf::slot_occupancy(slot_vox + (size_t)i * 512u, &ctx->host_occ[(size_t)slot_idx[i] * 8u]);
     if (n_slot_updates) {
         if ((e = hipMalloc(&d_si, (size_t)n_slot_updates * 4)) != hipSuccess ||
             (e = hipMalloc(&d_sv, (size_t)n_slot_updates * 512)) != hipSuccess ||
@@ -1092,11 +1221,9 @@ int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     if ((rc = begin_timed(ctx))) return rc;
     if (ctx->counting)
-        hipLaunchKernelGGL(k_primary<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits,
-                           reinterpret_cast<float4 *>(gb_pos), gb_normal, ctx->d_cnt);
+        launch_primary<true>(ctx, A, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal);
     else
-        hipLaunchKernelGGL(k_primary<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits,
-                           reinterpret_cast<float4 *>(gb_pos), gb_normal, ctx->d_cnt);
+        launch_primary<false>(ctx, A, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal);
     if ((rc = launch_check(ctx, "k_primary"))) return rc;
     return end_timed(ctx);
 }
@@ -1142,11 +1269,9 @@ int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_
     const float *s = ctx->S.sun_dir;
     if ((rc = begin_timed(ctx))) return rc;
     if (ctx->counting)
-        hipLaunchKernelGGL(k_primary_shadow<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], hits,
-                           reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits, ctx->d_cnt);
+        launch_primary_shadow<true>(ctx, A, s, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits);
     else
-        hipLaunchKernelGGL(k_primary_shadow<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], hits,
-                           reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits, ctx->d_cnt);
+        launch_primary_shadow<false>(ctx, A, s, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits);
     if ((rc = launch_check(ctx, "k_primary_shadow"))) return rc;
     return end_timed(ctx);
 }
