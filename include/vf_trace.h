@@ -186,11 +186,19 @@ int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
  * camera's voxel. */
 int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
                      float *d_gb_pos, uint32_t *d_gb_normal);
-/* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map / foliage / grass
- * extras): one exact sun ray per G-buffer hit from P + N*0.03 through the dry field (water clear).
- * shadowRaw is RG16F (vis 0/1, min(tBlocker, 60000)); d_hits (nullable) gets the sun ray's hit. */
-int vf_trace_shadow(vf_ctx *ctx, int32_t width, int32_t height, const float *d_gb_pos,
+/* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map, grass and under-water
+ * refraction): one sun ray per G-buffer hit from P + N*0.03 through the dry field (water clear), with
+ * FOLIAGE_LIGHT: sunVisibilityT (common.glsl:1964-1981, deterministic) lets pass->look_p[3]
+ * (r.foliage.light) of the light through each leaf voxel, up to 6 (0 = a hard shadow ray).  shadowRaw
+ * is RG16F (visibility, min(tFirstBlocker, 60000)); d_hits (nullable) gets the first segment's hit. */
+int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, const float *d_gb_pos,
                     const uint32_t *d_gb_normal, uint32_t *d_shadow_raw, vf_hit *d_hits);
+/* The SVGF lighting pass's sun ray (lighting.comp:44-84): per lit G-buffer texel one ray along
+ * sampleCone(sunDir, scene sun_angle) (common.glsl:1259-1265) seeded pixelSeed(pix, frame_idx, 0),
+ * through leaves with probability look_p[3] (sunVisibility, stochastic).  d_shadow: float per pixel,
+ * the raw sample (1 sky, 0 facing away / blocked, 1 lit); d_hits (nullable): the first segment. */
+int vf_trace_sun_cone(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, const float *d_gb_pos,
+                      const uint32_t *d_gb_normal, float *d_shadow, vf_hit *d_hits);
 /* P::Gbuf followed by P::ShTrace (record.rs:436, 463: record_standard's order) in ONE kernel: the same
  * outputs as vf_trace_primary + vf_trace_shadow (the sun ray starts from the written, quantised
  * G-buffer values), without the second launch and the G-buffer read-back.  gb_pos / gb_normal may be

@@ -96,7 +96,11 @@ def lib():
             "vfo_pass_primary": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, C.c_int]),
             "vfo_pass_primary_rows": (None, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
                                               C.c_int]),
-            "vfo_pass_shadow": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
+            "vfo_pass_shadow": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
+            "vfo_sun_visibility_t": (C.c_float, [P, P, P, P, C.c_float, P, C.c_int32, P, P, P]),
+            "vfo_sample_cone": (None, [P, C.c_float, P, P]),
+            "vfo_pass_sun_cone": (None, [P, P, P, C.c_float, C.c_float, C.c_uint32, C.c_int32, C.c_int32, P, P, P, P,
+                                         C.c_int]),
             "vfo_pass_gi": (None, [P, P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
             "vfo_gi_path": (None, [P, P, P, P, P, P, P, P]),
             "vfo_sky_open": (C.c_float, [P, P, P]),
@@ -258,15 +262,36 @@ class OracleWorld:
                                     int(nrows), _p(hits), _p(pos), _p(nrm), C.byref(cnt), int(threads))
         return hits, pos, nrm, cnt.as_dict()
 
-    def pass_shadow(self, sun, width, height, gb_pos, gb_normal, threads=0):
+    def pass_shadow(self, sun, width, height, gb_pos, gb_normal, threads=0, leaf_light=0.6):
+        """shadow_trace.comp with FOLIAGE_LIGHT (leaf_light = lookP.w, the reference default 0.6)."""
         n = width * height
         sh = np.zeros((n, 2), np.float32)
         hits = np.zeros(n, HIT_DTYPE)
         cnt = VfoCount()
         sun32, pos32, nrm = f32(sun), f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
-        lib().vfo_pass_shadow(C.byref(self.s), _p(sun32), int(width), int(height), _p(pos32), _p(nrm), _p(sh),
-                              _p(hits), C.byref(cnt), int(threads))
+        lib().vfo_pass_shadow(C.byref(self.s), _p(self.palette), _p(sun32), float(leaf_light), int(width), int(height),
+                              _p(pos32), _p(nrm), _p(sh), _p(hits), C.byref(cnt), int(threads))
         return sh, hits, cnt.as_dict()
+
+    def pass_sun_cone(self, sun, sun_angle, leaf_light, frame, width, height, gb_pos, gb_normal, threads=0):
+        """lighting.comp's stochastic cone-jittered sun sample per pixel, and the first segments' hits."""
+        n = width * height
+        out = np.zeros(n, np.float32)
+        hits = np.zeros(n, HIT_DTYPE)
+        sun32, pos32, nrm = f32(sun), f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
+        lib().vfo_pass_sun_cone(C.byref(self.s), _p(self.palette), _p(sun32), float(sun_angle), float(leaf_light),
+                                int(frame) & 0xFFFFFFFF, int(width), int(height), _p(pos32), _p(nrm), _p(out), _p(hits),
+                                int(threads))
+        return out, hits
+
+    def sun_visibility(self, o, d, leaf_light, seed=0, stochastic=False):
+        """sunVisibilityT: (visibility, tFirst, seed after)."""
+        o32, d32 = f32(o), f32(d)
+        sd = C.c_uint32(int(seed) & 0xFFFFFFFF)
+        tf = C.c_float()
+        v = lib().vfo_sun_visibility_t(C.byref(self.s), _p(self.palette), _p(o32), _p(d32), float(leaf_light),
+                                       C.byref(sd), int(bool(stochastic)), C.byref(tf), None, None)
+        return float(v), float(tf.value), sd.value
 
     def sky(self, scene, gi_reach=48.0, lights_nee=True):
         s = VfoSky()

@@ -781,8 +781,9 @@ void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int3
     if (cnt) add_cnt(cnt, &total);
 }
 
-void vfo_pass_shadow(const vfo_world *w, const float sun[3], int32_t W, int32_t H, const float *gb_pos,
-                     const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits, vfo_count *cnt, int threads) {
+void vfo_pass_shadow(const vfo_world *w, const float *palette, const float sun[3], float leaf_light, int32_t W,
+                     int32_t H, const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
+                     vfo_count *cnt, int threads) {
     vfo_count total = {0, 0, 0, 0};
 #pragma omp parallel num_threads(nthreads(threads))
     {
@@ -802,11 +803,11 @@ void vfo_pass_shadow(const vfo_world *w, const float sun[3], int32_t W, int32_t 
                 if (dot3(N, sun) <= 0.0f) {
                     vis = 0.0f; tb = 0.0f;
                 } else {
+                    /* FOLIAGE_LIGHT (shadow_trace.comp:96-99): sunVisibilityT, deterministic */
                     float so[3];
+                    uint32_t seed = 0u;
                     for (int a = 0; a < 3; ++a) so[a] = pt[a] + N[a] * 3e-2f;
-                    vfo_trace(w, so, sun, 1e30f, 5u, &h, &cc);
-                    vis = h.hit ? 0.0f : 1.0f;
-                    tb = h.hit ? h.t : 0.0f;
+                    vis = vfo_sun_visibility_t(w, palette, so, sun, leaf_light, &seed, 0, &tb, &h, &cc);
                 }
             }
             shadow[i * 2 + 0] = vis;
@@ -904,6 +905,83 @@ void vfo_cosine_hemisphere(const float n[3], uint32_t *state, float out[3]) {
     u[0] = rnd(state);
     u[1] = rnd(state);
     vfo_cosine_hemisphere_u(n, u, out);
+}
+
+/* sunVisibilityT (common.glsl:1964-1981) over the world: leaves (palette kind 3) let leaf_light of the
+ * light through (stochastic: with that probability), up to 6; the far-horizon and cloud factors are 1
+ * (neither is traced here).  first: the first segment's hit; the ray counter counts the ray once. */
+float vfo_sun_visibility_t(const vfo_world *w, const float *palette, const float o0[3], const float d[3],
+                           float leaf_light, uint32_t *seed, int32_t stochastic, float *tfirst, vfo_hit *first,
+                           vfo_count *cnt) {
+    float T = 1.0f, o[3] = {o0[0], o0[1], o0[2]};
+    *tfirst = 0.0f;
+    for (int i = 0; i < 6; ++i) {
+        vfo_hit h;
+        vfo_trace(w, o, d, 1e30f, 5u, &h, cnt);
+        if (cnt && i > 0) cnt->rays--;
+        if (i == 0 && first) *first = h;
+        if (!h.hit) return T;
+        float p[3];
+        for (int a = 0; a < 3; ++a) p[a] = o[a] + d[a] * h.t;
+        if (i == 0) *tfirst = h.t;
+        if (palette[(size_t)h.mat * 12 + 6] != 3.0f || leaf_light <= 0.0f) return 0.0f;
+        if (stochastic) {
+            if (rnd(seed) >= leaf_light) return 0.0f;
+        } else {
+            T = T * leaf_light;
+            if (T < 0.03f) return 0.0f;
+        }
+        float tx[3];
+        for (int a = 0; a < 3; ++a) {
+            float sd = fabsf(d[a]) < 1e-6f ? 1e-6f : d[a];
+            tx[a] = (((float)h.cell[a] + (sd < 0.0f ? 0.0f : 1.0f)) - p[a]) / sd;
+        }
+        float tm = gmin(gmin(tx[0], tx[1]), tx[2]) + 1e-3f;
+        for (int a = 0; a < 3; ++a) o[a] = p[a] + d[a] * tm;
+    }
+    return 0.0f;
+}
+
+/* sampleCone (common.glsl:1259-1265) */
+void vfo_sample_cone(const float dir[3], float ang, uint32_t *state, float out[3]) {
+    float t1[3], t2[3];
+    make_basis(dir, t1, t2);
+    float r = ang * sqrtf(rnd(state));
+    float ph = 6.2831853f * rnd(state);
+    float s, c;
+    sincos_red(ph, &s, &c);
+    for (int a = 0; a < 3; ++a) out[a] = (dir[a] + t1[a] * (r * c)) + t2[a] * (r * s);
+    normalize3(out);
+}
+
+/* lighting.comp's sun sample (lighting.comp:44-84): 1 sky, 0 facing away, else stochastic sunVisibility
+ * along sampleCone(sunDir, sun_angle), seed pixelSeed(pix, frame, 0). */
+void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun[3], float sun_angle, float leaf_light,
+                       uint32_t frame, int32_t W, int32_t H, const float *gb_pos, const uint32_t *gb_normal, float *out,
+                       vfo_hitrec *hits, int threads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
+    for (int64_t i = 0; i < (int64_t)W * H; ++i) {
+        int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
+        vfo_hit h;
+        memset(&h, 0, sizeof h);
+        h.t = -1.0f;
+        float v = 1.0f;
+        const float *pt = gb_pos + i * 4;
+        if (pt[3] >= 0.0f) {
+            uint32_t nb = gb_normal[i];
+            float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+            uint32_t seed = vfo_pixel_seed(px, py, frame, 0u);
+            v = 0.0f;
+            if (gmax(dot3(N, sun), 0.0f) > 0.0f) {
+                float sd[3], so[3], tf;
+                vfo_sample_cone(sun, sun_angle, &seed, sd);
+                for (int a = 0; a < 3; ++a) so[a] = pt[a] + N[a] * 3e-2f;
+                v = vfo_sun_visibility_t(w, palette, so, sd, leaf_light, &seed, 1, &tf, &h, NULL);
+            }
+        }
+        out[i] = v;
+        if (hits) to_rec(&h, &hits[i]);
+    }
 }
 
 /* giPathImpl (common.glsl:2250-2366) without bodies, NEE lamps or the hash cache, for lastStage 1 (one

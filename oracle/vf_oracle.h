@@ -143,10 +143,20 @@ void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t width, int32
 void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height, int32_t y0,
                            int32_t ystep, int32_t nrows, vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal,
                            vfo_count *cnt, int threads);
-/* shadow_trace.comp exact sun ray per pixel: shadowRaw as two floats (vis, min(tBlock, 60000)). */
-void vfo_pass_shadow(const vfo_world *w, const float sun_dir[3], int32_t width, int32_t height,
-                     const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
+/* shadow_trace.comp sun ray per pixel with FOLIAGE_LIGHT (leaf_light = lookP.w): shadowRaw as two floats
+ * (visibility, min(tFirst, 60000)); hits = the first segment. palette: 256 x 12 floats. */
+void vfo_pass_shadow(const vfo_world *w, const float *palette, const float sun_dir[3], float leaf_light, int32_t width,
+                     int32_t height, const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
                      vfo_count *cnt, int threads);
+/* sunVisibilityT (common.glsl:1964-1981); stochastic as SVGF's lighting pass, else as shadow_trace.comp. */
+float vfo_sun_visibility_t(const vfo_world *w, const float *palette, const float o[3], const float d[3],
+                           float leaf_light, uint32_t *seed, int32_t stochastic, float *tfirst, vfo_hit *first,
+                           vfo_count *cnt);
+void vfo_sample_cone(const float dir[3], float ang, uint32_t *state, float out[3]);
+/* lighting.comp's cone-jittered stochastic sun sample per pixel (1 sky, 0 / 1 otherwise). */
+void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun_dir[3], float sun_angle,
+                       float leaf_light, uint32_t frame, int32_t width, int32_t height, const float *gb_pos,
+                       const uint32_t *gb_normal, float *out, vfo_hitrec *hits, int threads);
 /* gi.comp sampling + giPath (1 bounce): radiance rgb + the bounce ray's hit. half_res: the reference's
  * half-resolution dispatch (gi.comp:34-84); else one ray per full-res pixel. */
 void vfo_pass_gi(const vfo_world *w, const vfo_sky *s, uint32_t frame_idx, int32_t width, int32_t height,

@@ -385,3 +385,83 @@ def test_batch_view_from_inside_solid_equals_plain_launch(ctx):
         ctx.sync()
         for k, (o, ref) in enumerate(zip(outs, plain)):
             assert torch.equal(o, ref), (k, cfg.refill)
+
+
+def _scattered_leaves():
+    """A stone floor (y < 8) under 3000 single leaf voxels (palette 7, kind 3) at y 12..55."""
+    rng = np.random.default_rng(8)
+    vox = np.zeros((64, 64, 64), np.uint8)   # (z, y, x)
+    vox[:, 0:8, :] = 3
+    idx = rng.integers(0, 64, (3000, 3))
+    idx[:, 1] = rng.integers(12, 56, 3000)
+    vox[idx[:, 0], idx[:, 1], idx[:, 2]] = 7
+    return dense_world(vox)
+
+
+def _forest(ctx):
+    w, a, ow = world(2, world_bricks=(64, 32, 64), chunks=3, trees_per_chunk=60)
+    scene = upload(ctx, w, a)
+    gy = w.ground(-37, 21)
+    pos = [-37.4, gy + 17.0, 21.6]
+    return w, a, ow, scene, pos
+
+
+@pytest.mark.parametrize("leaf", [0.6, 0.0])
+def test_shadow_through_leaves_matches_oracle(ctx, leaf):
+    """shadow_trace.comp with FOLIAGE_LIGHT on a forest: visibility leafLight^k through k leaves (exact
+    FP16 halves), tFirst and the first segment's hit bit-identical; leafLight 0 is the hard shadow."""
+    w, a, ow, scene, pos = _forest(ctx)
+    W, H = 320, 200
+    for yaw, pitch in ((4.1, -0.08), (1.0, 0.25)):
+        pu, cam = camera(pos, yaw, pitch)
+        pu.look_p[3] = leaf
+        _, gbp, gbn = primary_gpu(ctx, pu, W, H)
+        sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        shh = dev_hits(W * H)
+        ctx.trace_shadow(W, H, gbp, gbn, sh, shh, pass_ubo=pu)
+        ctx.sync()
+        _, opos, onrm, _ = ow.pass_primary(cam, W, H)
+        osh, ohits, _ = ow.pass_shadow(np.array(scene.sun_dir, np.float32), W, H, opos, onrm, leaf_light=leaf)
+        assert hit_diff(vf.hits_to_numpy(shh), ohits).size == 0
+        gsh = sh.cpu().numpy().view(np.uint32)
+        vis = (gsh & 0xFFFF).astype(np.uint16).view(np.float16)
+        tb = (gsh >> 16).astype(np.uint16).view(np.float16)
+        assert np.array_equal(vis, osh[:, 0].astype(np.float16))
+        assert np.array_equal(tb, osh[:, 1].astype(np.float16))
+    # scattered single leaves over a floor: many sun rays cross one to three of them
+    w2, a2, ow2 = _scattered_leaves()
+    scene2 = upload(ctx, w2, a2)
+    pu, cam = camera([32.5, 9.5, 32.5], 0.3, -0.45)
+    pu.look_p[3] = leaf
+    _, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    sh = torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    shh = dev_hits(W * H)
+    ctx.trace_shadow(W, H, gbp, gbn, sh, shh, pass_ubo=pu)
+    ctx.sync()
+    _, opos, onrm, _ = ow2.pass_primary(cam, W, H)
+    osh, ohits, _ = ow2.pass_shadow(np.array(scene2.sun_dir, np.float32), W, H, opos, onrm, leaf_light=leaf)
+    assert hit_diff(vf.hits_to_numpy(shh), ohits).size == 0
+    gsh = sh.cpu().numpy().view(np.uint32)
+    vis = (gsh & 0xFFFF).astype(np.uint16).view(np.float16)
+    assert np.array_equal(vis, osh[:, 0].astype(np.float16))
+    partial = ((vis > 0) & (vis < 1)).mean()
+    assert (partial > 0.01) == (leaf > 0.0), partial
+
+
+def test_sun_cone_matches_oracle(ctx):
+    """lighting.comp's cone-jittered, stochastic sun sample: bit-identical to the oracle."""
+    w, a, ow = _scattered_leaves()
+    scene = upload(ctx, w, a)
+    W, H = 320, 200
+    for frame in (0, 7):
+        pu, cam = camera([32.5, 9.5, 32.5], 0.3, -0.45, frame=frame)
+        _, gbp, gbn = primary_gpu(ctx, pu, W, H)
+        out = torch.zeros(W * H, dtype=torch.float32, device=DEV)
+        hh = dev_hits(W * H)
+        ctx.trace_sun_cone(pu, W, H, gbp, gbn, out, hh)
+        ctx.sync()
+        _, opos, onrm, _ = ow.pass_primary(cam, W, H)
+        o, oh = ow.pass_sun_cone(np.array(scene.sun_dir, np.float32), scene.sun_angle, 0.6, frame, W, H, opos, onrm)
+        assert np.array_equal(out.cpu().numpy(), o)
+        assert hit_diff(vf.hits_to_numpy(hh), oh).size == 0
+        assert 0.0 < (o == 0).mean() < 1.0

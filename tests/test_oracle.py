@@ -437,3 +437,59 @@ def test_exit_solid_matches_fp64_voxel_march():
         else:
             assert got == pytest.approx(want, rel=1e-5, abs=1e-4), (ro32, d32)
     assert n_out > 10
+
+
+# --- sun rays through leaves (FOLIAGE_LIGHT, sunVisibilityT common.glsl:1964-1981) ----------------------
+def _leaf_column_world(k, blocker=False):
+    """64^3 voxels: a floor of stone at y < 8 and, above (20, *, 20), k leaf voxels stacked from y = 12
+    (with a stone voxel on top when `blocker`).  Indexed (x, y, z); from_dense takes (z, y, x)."""
+    vox = np.zeros((64, 64, 64), np.uint8)
+    vox[:, 0:8, :] = 3
+    for j in range(k):
+        vox[20, 12 + 2 * j, 20] = 7          # leaves (palette kind 3), a gap between them
+    if blocker:
+        vox[20, 12 + 2 * k, 20] = 3
+    return vox
+
+
+def test_sun_visibility_through_leaves_closed_form():
+    """Straight up through k separated leaf voxels: deterministic visibility leafLight^k (k <= 5), 0 from
+    the sixth leaf on or at any non-leaf blocker; tFirst = distance to the first leaf; leafLight 0 is the
+    hard shadow ray."""
+    o = np.array([20.5, 8.03, 20.5], np.float32)
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    for k in range(0, 8):
+        _, _, ow = dense_world(_leaf_column_world(k).transpose(2, 1, 0))
+        v, tf, _ = ow.sun_visibility(o, up, 0.6)
+        want = 0.6 ** k if k <= 5 else 0.0
+        assert v == pytest.approx(want, rel=1e-6), (k, v)
+        assert tf == (pytest.approx(12.0 - 8.03, abs=1e-5) if k > 0 else 0.0)
+        v0, tf0, _ = ow.sun_visibility(o, up, 0.0)
+        assert v0 == (0.0 if k > 0 else 1.0) and tf0 == tf
+    _, _, ow = dense_world(_leaf_column_world(2, blocker=True).transpose(2, 1, 0))
+    assert ow.sun_visibility(o, up, 0.6)[0] == 0.0
+    # stochastic: pass-or-block with probability 0.6 per leaf
+    _, _, ow = dense_world(_leaf_column_world(2).transpose(2, 1, 0))
+    hits = [ow.sun_visibility(o, up, 0.6, seed=s, stochastic=True)[0] for s in range(4000)]
+    assert set(hits) <= {0.0, 1.0}
+    assert abs(np.mean(hits) - 0.36) < 0.03
+
+
+def test_sample_cone_stays_in_the_sun_disk():
+    rng = np.random.default_rng(6)
+    lib = O.lib()
+    for _ in range(20):
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        ang = float(rng.uniform(0.01, 0.2))
+        r2 = []
+        seed = ctypes.c_uint32(int(rng.integers(0, 2 ** 32)))
+        out = np.zeros(3, np.float32)
+        for _ in range(400):
+            lib.vfo_sample_cone(d.ctypes.data, ctypes.c_float(ang), ctypes.byref(seed), out.ctypes.data)
+            c = float(np.clip(np.dot(out, d), -1.0, 1.0))
+            th = math.acos(c)
+            assert th <= math.atan(ang) + 1e-5
+            r2.append(math.tan(th) ** 2)
+        # r = ang * sqrt(u): uniform over the disk, E[r^2] = ang^2 / 2
+        assert abs(np.mean(r2) / (ang * ang) - 0.5) < 0.06

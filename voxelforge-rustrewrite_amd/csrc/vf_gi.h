@@ -218,3 +218,67 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
         extended = false;
     }
 }
+
+// sampleCone (common.glsl:1259-1265): dir jittered within a cone of angular radius ang.
+__device__ __forceinline__ void sample_cone(const float dir[3], float ang, uint32_t &state, float out[3]) {
+    float t1[3], t2[3];
+    const float upv[3] = {0.0f, 1.0f, 0.0f}, uxv[3] = {1.0f, 0.0f, 0.0f};
+    cross3(dir, fabsf(dir[1]) < 0.9f ? upv : uxv, t1);
+    normalize3(t1);
+    cross3(dir, t1, t2);
+    const float r = ang * sqrtf(rnd(state));
+    const float ph = 6.2831853f * rnd(state);
+    float s, c;
+    sincos_red(ph, s, c);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = (dir[a] + t1[a] * (r * c)) + t2[a] * (r * s);
+    normalize3(out);
+}
+
+// sunVisibilityT (common.glsl:1964-1981) over the world: a leaf voxel (palette kind 3) lets the light
+// through with probability (STOCHASTIC) or fraction leaf_light, and the ray goes on from where it
+// leaves that voxel; up to 6 leaves.  tfirst = distance to the first blocker (0 = none).  The far
+// horizon's and the clouds' factors (horizonSunVisibility, cloudShadow) are 1 here: neither the
+// horizon heightfield nor clouds are traced by this library.  The first
+// segment's hit record at first_base[idx] (first_base nullable); the ray counter counts the whole
+// visibility ray once.
+template <bool STOCHASTIC, bool COUNT>
+__device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float o0[3], const float d[3], float leaf_light,
+                                                  uint32_t &seed, float &tfirst, vf_hit *first_base, size_t idx, Cnt &c) {
+    // the first segment: the plain sun ray (most rays end here: no hit, or a hit that is not a leaf)
+    Hit h;
+    trace_world<5, COUNT>(W, o0, d, 1e30f, h, c);
+    if (first_base) store_hit(first_base + idx, h);
+    tfirst = 0.0f;
+    if (!h.hit) return 1.0f;
+    tfirst = h.t;
+    if (W.palette[h.mat * 3u + 1u].z != 3.0f || leaf_light <= 0.0f) return 0.0f;
+    // through leaves: leave each leaf voxel across its exit face along d and trace on
+    float T = 1.0f;
+    float o[3] = {o0[0], o0[1], o0[2]};
+#pragma unroll 1
+    for (int i = 1; i <= 6; ++i) {
+        if (STOCHASTIC) {
+            if (rnd(seed) >= leaf_light) return 0.0f;
+        } else {
+            T = T * leaf_light;
+            if (T < 0.03f) return 0.0f;
+        }
+        if (i == 6) return 0.0f;   // six leaves: the loop of common.glsl:1967 ends without a pass
+        float p[3], tx[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            p[a] = o[a] + d[a] * h.t;
+            const float sd = fabsf(d[a]) < 1e-6f ? 1e-6f : d[a];
+            tx[a] = (((float)h.cell[a] + (sd < 0.0f ? 0.0f : 1.0f)) - p[a]) / sd;
+        }
+        const float tm = gmin(gmin(tx[0], tx[1]), tx[2]) + 1e-3f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o[a] = p[a] + d[a] * tm;
+        trace_world<5, COUNT>(W, o, d, 1e30f, h, c);
+        if (COUNT) c.rays--;
+        if (!h.hit) return T;
+        if (W.palette[h.mat * 3u + 1u].z != 3.0f) return 0.0f;
+    }
+    return 0.0f;
+}

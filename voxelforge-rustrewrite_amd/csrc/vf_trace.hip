@@ -42,6 +42,7 @@ struct PrimArgs {
     float jit[2];
     int32_t w, h, tiles_x, ntiles;
     int32_t cam_air;   // the host's grid copy shows the camera in an empty brick (or outside the world)
+    float leaf;        // lookP.w = r.foliage.light: sunlight passing through each leaf (FOLIAGE_LIGHT)
 };
 
 // The camera voxel decides which primary kernel variant's results stand (gbuffer.comp:233-241).  A camera
@@ -134,9 +135,42 @@ __global__ void __launch_bounds__(256) k_primary(DevWorld W, PrimArgs A, vf_hit 
 
 struct TileArgs {
     int32_t w, h, tiles_x, ntiles;
+    float leaf;        // lookP.w (FOLIAGE_LIGHT)
 };
 
-// shadow_trace.comp exact sun ray (shadow_trace.comp:17-34, 87, 101-105).
+__device__ __forceinline__ void store_miss(vf_hit *out) {
+    Hit m;
+    m.hit = 0; m.t = -1.0f; m.face = 0; m.mat = 0; m.cell[0] = m.cell[1] = m.cell[2] = 0;
+    store_hit(out, m);
+}
+
+// shadow_trace.comp's result for one G-buffer texel (shadow_trace.comp:21-34, 87, 96-105): sky -> lit,
+// facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
+// deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
+// to the first blocker.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
+                                                 vf_hit *hits, size_t i, Cnt &c) {
+    float vis = 0.0f, tb = 0.0f;
+    if (p.w < 0.0f) {
+        vis = 1.0f;
+        if (hits) store_miss(hits + i);
+    } else {
+        const float N[3] = {unorm8_dec(nq & 255u), unorm8_dec((nq >> 8) & 255u), unorm8_dec((nq >> 16) & 255u)};
+        if (dot3(N, sun) <= 0.0f) {
+            if (hits) store_miss(hits + i);
+        } else {
+            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+            uint32_t seed = 0u;
+            vis = sun_visibility_t<false, COUNT>(W, so, sun, leaf, seed, tb, hits, i, c);
+        }
+    }
+    const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
+    const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
+    return hv | (ht << 16);
+}
+
+// shadow_trace.comp (shadow_trace.comp:17-34, 87, 96-105) for a 16x16 tile per workgroup.
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx, float sy, float sz, const float4 *gbpos,
                                                 const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
@@ -148,30 +182,9 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx
     const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
     if (px >= A.w || py >= A.h) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
-    const float4 pt = gbpos[i];
     const float sun[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
-    Hit h;
-    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
-    float vis, tb;
-    if (pt.w < 0.0f) {
-        vis = 1.0f; tb = 0.0f;
-    } else {
-        const uint32_t nb = gbn[i];
-        const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
-        if (dot3(N, sun) <= 0.0f) {
-            vis = 0.0f; tb = 0.0f;
-        } else {
-            const float so[3] = {pt.x + N[0] * 3e-2f, pt.y + N[1] * 3e-2f, pt.z + N[2] * 3e-2f};
-            trace_world<5, COUNT>(W, so, sun, 1e30f, h, c);
-            vis = h.hit ? 0.0f : 1.0f;
-            tb = h.hit ? h.t : 0.0f;
-        }
-    }
-    const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
-    const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
-    shadow[i] = hv | (ht << 16);
-    if (hits) store_hit(hits + i, h);
+    shadow[i] = shadow_texel<COUNT>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -202,27 +215,8 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     }
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
-    // shadow_trace.comp:21-34, 87, 101-105 on (p, nq)
-    Hit hs;
-    hs.hit = 0; hs.t = -1.0f; hs.face = 0; hs.mat = 0; hs.cell[0] = hs.cell[1] = hs.cell[2] = 0;
-    float vis, tb;
-    if (p.w < 0.0f) {
-        vis = 1.0f; tb = 0.0f;
-    } else {
-        const float N[3] = {unorm8_dec(nq & 255u), unorm8_dec((nq >> 8) & 255u), unorm8_dec((nq >> 16) & 255u)};
-        if (dot3(N, sun) <= 0.0f) {
-            vis = 0.0f; tb = 0.0f;
-        } else {
-            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
-            trace_world<5, COUNT>(W, so, sun, 1e30f, hs, c);
-            vis = hs.hit ? 0.0f : 1.0f;
-            tb = hs.hit ? hs.t : 0.0f;
-        }
-    }
-    const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
-    const uint32_t ht = (uint32_t)__half_as_ushort(__float2half_rn(gmin(tb, 60000.0f)));
-    shadow[i] = hv | (ht << 16);
-    if (shits) store_hit(shits + i, hs);
+    // shadow_trace.comp on (p, nq)
+    shadow[i] = shadow_texel<COUNT>(W, p, nq, sun, A.leaf, shits, i, c);
 }
 
 template <bool COUNT, bool SPECIAL>
@@ -302,6 +296,43 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, c
     }
     radiance[oi] = make_float4(Lr[0], Lr[1], Lr[2], sky ? 0.0f : 1.0f);
     if (bounce) store_hit(bounce + oi, bh);
+    flush_count<COUNT>(c, cnt);
+}
+
+// lighting.comp's sun ray (SVGF, lighting.comp:44-84): per pixel seed pixelSeed(pix, frameIdx, 0), a
+// lit texel (N.L > 0) traces sunVisibility (FOLIAGE_LIGHT, stochastic) from P + N*0.03 along
+// sampleCone(sunDir, sunAngle); the raw sample is 1 for sky, 0 for a texel facing away, else 0 or 1.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, TileArgs A, float sx, float sy, float sz, float ang,
+                                                  uint32_t frame, const float4 *gbpos, const uint32_t *gbn, float *out,
+                                                  vf_hit *hits, unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t px = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.w || py >= A.h) return;
+    const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
+    const float4 p = gbpos[i];
+    bool stored = false;
+    Cnt c = {0, 0, 0, 0};
+    float v = 1.0f;
+    if (p.w >= 0.0f) {
+        const uint32_t nq = gbn[i];
+        const float N[3] = {unorm8_dec(nq & 255u), unorm8_dec((nq >> 8) & 255u), unorm8_dec((nq >> 16) & 255u)};
+        const float sun[3] = {sx, sy, sz};
+        uint32_t seed = pixel_seed(px, py, frame, 0u);
+        v = 0.0f;
+        if (gmax(dot3(N, sun), 0.0f) > 0.0f) {
+            float sd[3], tf;
+            sample_cone(sun, ang, seed, sd);
+            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+            v = sun_visibility_t<true, COUNT>(W, so, sd, A.leaf, seed, tf, hits, i, c);
+            stored = true;
+        }
+    }
+    if (hits && !stored) store_miss(hits + i);
+    out[i] = v;
     flush_count<COUNT>(c, cnt);
 }
 
@@ -1219,6 +1250,7 @@ int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf
     A.w = w; A.h = h;
     A.tiles_x = (w + 15) / 16;
     A.ntiles = A.tiles_x * ((h + 15) / 16);
+    A.leaf = p->look_p[3];
     if ((rc = begin_timed(ctx))) return rc;
     if (ctx->counting)
         launch_primary<true>(ctx, A, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal);
@@ -1228,13 +1260,14 @@ int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf
     return end_timed(ctx);
 }
 
-int vf_trace_shadow(vf_ctx *ctx, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal, uint32_t *shadow, vf_hit *hits) {
+int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal,
+                    uint32_t *shadow, vf_hit *hits) {
     int rc = check_ready(ctx, "vf_trace_shadow");
     if (rc) return rc;
     if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: no scene (vf_set_scene)");
-    if (!gb_pos || !gb_normal || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: null buffer");
+    if (!p || !gb_pos || !gb_normal || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: null pass or buffer");
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: bad size %dx%d", w, h);
-    TileArgs A{w, h, (w + 15) / 16, 0};
+    TileArgs A{w, h, (w + 15) / 16, 0, p->look_p[3]};
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     const float *s = ctx->S.sun_dir;
     if ((rc = begin_timed(ctx))) return rc;
@@ -1266,6 +1299,7 @@ int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_
     A.w = w; A.h = h;
     A.tiles_x = (w + 15) / 16;
     A.ntiles = A.tiles_x * ((h + 15) / 16);
+    A.leaf = p->look_p[3];
     const float *s = ctx->S.sun_dir;
     if ((rc = begin_timed(ctx))) return rc;
     if (ctx->counting)
@@ -1273,6 +1307,28 @@ int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_
     else
         launch_primary_shadow<false>(ctx, A, s, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits);
     if ((rc = launch_check(ctx, "k_primary_shadow"))) return rc;
+    return end_timed(ctx);
+}
+
+int vf_trace_sun_cone(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal,
+                      float *shadow, vf_hit *hits) {
+    int rc = check_ready(ctx, "vf_trace_sun_cone");
+    if (rc) return rc;
+    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_sun_cone: no scene (vf_set_scene)");
+    if (!p || !gb_pos || !gb_normal || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_sun_cone: null pass or buffer");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_sun_cone: bad size %dx%d", w, h);
+    TileArgs A{w, h, (w + 15) / 16, 0, p->look_p[3]};
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    const float *s = ctx->S.sun_dir;
+    const float ang = ctx->scene.sun_angle;
+    if ((rc = begin_timed(ctx))) return rc;
+    if (ctx->counting)
+        hipLaunchKernelGGL(k_sun_cone<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], ang,
+                           p->frame_idx, reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    else
+        hipLaunchKernelGGL(k_sun_cone<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], ang,
+                           p->frame_idx, reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    if ((rc = launch_check(ctx, "k_sun_cone"))) return rc;
     return end_timed(ctx);
 }
 

@@ -58,7 +58,8 @@ _SIGS = {
     "vf_read_skytop": (C.c_int, [_P, _P]),
     "vf_set_scene": (C.c_int, [_P, C.POINTER(SceneUbo)]),
     "vf_trace_primary": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P]),
-    "vf_trace_shadow": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    "vf_trace_shadow": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    "vf_trace_sun_cone": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P]),
     "vf_trace_primary_shadow": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "vf_trace_gi": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
                               _P, _P]),
@@ -160,9 +161,11 @@ def sun_dir(az_deg, el_deg):
     return (s / np.float32(np.linalg.norm(s))).astype(np.float32)
 
 
-def make_pass(pos, right, up, fwd, fov_deg=70.0, frame_idx=0, jitter=(0.0, 0.0)):
-    """The camera part of PassUbo (set_frame_uniforms, src/gfx/renderer/frame.rs:549-566)."""
+def make_pass(pos, right, up, fwd, fov_deg=70.0, frame_idx=0, jitter=(0.0, 0.0), leaf_light=0.6):
+    """The camera part of PassUbo (set_frame_uniforms, src/gfx/renderer/frame.rs:549-566) and look_p[3] =
+    r.foliage.light (frame.rs:606; default 0.6, cvars.rs:143): the sunlight each leaf lets through."""
     p = PassUbo()
+    p.look_p[3] = float(leaf_light)
     thf = np.float32(math.tan(math.radians(fov_deg * 0.5)))
     for i in range(3):
         p.cam_pos[i] = float(pos[i]); p.cam_right[i] = float(right[i])
@@ -174,6 +177,10 @@ def make_pass(pos, right, up, fwd, fov_deg=70.0, frame_idx=0, jitter=(0.0, 0.0))
     p.frame_idx = int(frame_idx)
     p.jitter[0], p.jitter[1] = float(jitter[0]), float(jitter[1])
     return p
+
+
+_DEFAULT_PASS = PassUbo()
+_DEFAULT_PASS.look_p[3] = 0.6
 
 
 def default_scene(world_bricks, world_win=(0, 0, 0, 0), sun=(32.0, 46.0), sky_model=0):
@@ -192,6 +199,7 @@ def default_scene(world_bricks, world_win=(0, 0, 0, 0), sun=(32.0, 46.0), sky_mo
     s.sky_horizon[0], s.sky_horizon[1], s.sky_horizon[2] = 0.75, 0.82, 0.90
     s.sunset_rgb[0], s.sunset_rgb[1], s.sunset_rgb[2] = 0.3, 0.15, 0.05
     s.sky_scale = 1.0
+    s.sun_angle = 0.028   # 0.028 * r.sun.softness (1) * (1 + 6 cloudy^2), clear sky (frame.rs:230)
     s.gi_scale = 1.0
     s.sky_model = int(sky_model)
     s.num_objects = 0
@@ -373,9 +381,16 @@ class Context:
         self._ck(lib().vf_trace_primary(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
                                         _ptr(gb_pos), _ptr(gb_normal)))
 
-    def trace_shadow(self, width, height, gb_pos, gb_normal, shadow_raw, hits=None):
-        self._ck(lib().vf_trace_shadow(self._h, int(width), int(height), _ptr(gb_pos), _ptr(gb_normal),
+    def trace_shadow(self, width, height, gb_pos, gb_normal, shadow_raw, hits=None, pass_ubo=None):
+        """shadow_trace.comp; the pass supplies look_p[3] (r.foliage.light, the reference default 0.6)."""
+        p = pass_ubo if pass_ubo is not None else _DEFAULT_PASS
+        self._ck(lib().vf_trace_shadow(self._h, C.byref(p), int(width), int(height), _ptr(gb_pos), _ptr(gb_normal),
                                        _ptr(shadow_raw), _ptr(hits)))
+
+    def trace_sun_cone(self, pass_ubo, width, height, gb_pos, gb_normal, shadow, hits=None):
+        """lighting.comp's cone-jittered sun ray (float per pixel)."""
+        self._ck(lib().vf_trace_sun_cone(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
+                                         _ptr(gb_normal), _ptr(shadow), _ptr(hits)))
 
     def trace_primary_shadow(self, pass_ubo, width, height, hits, gb_pos, gb_normal, shadow_raw, shadow_hits=None):
         self._ck(lib().vf_trace_primary_shadow(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
