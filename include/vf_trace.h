@@ -109,11 +109,13 @@ typedef struct vf_ray { float o[3]; float d[3]; } vf_ray;
 
 /* Hit record, 20 B (common.glsl:454-461 Hit, without the body index).
  * face: 0 = none (the ray started inside the world box), 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z
- * (the outward normal of the face the ray entered through).  Miss: hit = 0, t = -1, rest 0. */
+ * (the outward normal of the face the ray entered through).  Miss: hit = 0, t = -1, rest 0.
+ * body = 1: the hit is on a body (Hit.obj >= 0); cell and face are then in the body's own voxel
+ * space, and the body index comes with vf_trace_gbuffer's d_body. */
 typedef struct vf_hit {
-    int32_t cell[3];   /* absolute voxel coordinates */
+    int32_t cell[3];   /* absolute voxel coordinates (world), or body voxel coordinates */
     float t;
-    uint8_t mat, face, hit, pad;
+    uint8_t mat, face, hit, body;
 } vf_hit;
 
 /* The world as GpuWorld::upload (src/gfx/gpu_world.rs:228-437) packs it. */
@@ -172,6 +174,47 @@ int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry);
 /* Read back the sky-top map (R16UI, (wbx*8) x (wbz*8)), host. */
 int vf_read_skytop(vf_ctx *ctx, uint16_t *skytop);
 
+/* ---- bodies (dynamic voxel objects) -------------------------------------------------------- */
+/* Body header, std430, 240 B (gpu_world.rs:34-47 ObjHeader; common.glsl:65-72).  Matrices are
+ * column-major (GLSL mat4; math.rs:30 Transform::to_mat4).  prev_obj_to_world and the .w of the boxes
+ * (glow integrity) are not read by the traversal. */
+typedef struct vf_obj_header {
+    float world_to_obj[16];
+    float obj_to_world[16];
+    float prev_obj_to_world[16];
+    int32_t dims_bricks[4];   /* bricks x, y, z; offset of the body's grid in the body grid array */
+    float aabb_min[4];        /* the body's world box */
+    float aabb_max[4];
+} vf_obj_header;
+
+/* BVH node, 32 B (common.glsl:77-81 BvhNode, as VkGpuWorld::writeBvh / gpu_world.rs:556-558 writes it):
+ * count 0 = interior node with children first and first + 1, else a leaf of prims [first, first+count). */
+typedef struct vf_bvh_node {
+    float min[3]; uint32_t first;
+    float max[3]; uint32_t count;
+} vf_bvh_node;
+
+/* GpuWorld::init_objects (gpu_world.rs:661-707): n bodies with their grids and headers, replacing any
+ * before.  grid: every body's brick grid, body i's brick (x, y, z) at dims_bricks[3] + (z*by + y)*bx + x,
+ * encoded like the world grid (0 empty, 1-255 uniform, 256 + slot); body bricks live in the world's
+ * atlas (slots uploaded with vf_world_upload / rewritten with vf_world_update, as the reference numbers
+ * them into the same atlas, gpu_world.rs:274-293).  The BVH over the bodies' world boxes is built here
+ * (build_bvh, gpu_world.rs:550-600).  n = 0 removes the bodies.  The traces include the bodies when
+ * the scene's num_objects is non-zero (it must then equal n). */
+int vf_objects_upload(vf_ctx *ctx, const vf_obj_header *headers, int32_t n, const uint32_t *grid, int64_t grid_len);
+/* GpuWorld::update_objects (gpu_world.rs:650-656): new headers (poses, boxes) for the same n bodies and
+ * a rebuilt BVH; written in place, so recorded frame graphs see them.  Not during a graph capture. */
+int vf_objects_set_headers(vf_ctx *ctx, const vf_obj_header *headers, int32_t n);
+/* The caller's own BVH (the layout above) instead of the one built here: checked to be a tree over
+ * the nodes with prim ranges and indices in range and at most 24 pending subtrees on the reference's
+ * traversal stack (common.glsl:697).  Replaced again by the next upload / set_headers. */
+int vf_objects_set_bvh(vf_ctx *ctx, const vf_bvh_node *nodes, int32_t n_nodes, const uint32_t *prims, int32_t n_prims);
+/* GpuWorld::sync_objects (gpu_world.rs:710-745): body grid entries idx[k] <- val[k]. */
+int vf_objects_update_grid(vf_ctx *ctx, const uint32_t *idx, const uint32_t *val, int32_t n);
+/* The BVH in use, host copies (counts always written; arrays when large enough, else VF_E_INVALID). */
+int vf_objects_read_bvh(vf_ctx *ctx, vf_bvh_node *nodes, int32_t max_nodes, int32_t *n_nodes, uint32_t *prims,
+                        int32_t max_prims, int32_t *n_prims);
+
 /* ---- per-frame uniforms -------------------------------------------------------------------- */
 /* Renderer::set_frame_uniforms (frame.rs:549): the scene block for the following traces. */
 int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
@@ -186,6 +229,14 @@ int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
  * camera's voxel. */
 int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
                      float *d_gb_pos, uint32_t *d_gb_normal);
+/* P::Gbuf (+ P::ShTrace) over the scene (traceScenePrimary = traceSceneS, common.glsl:787-793, 955): the
+ * world, then the bodies closer than its hit, with every per-pixel output: as vf_trace_primary_shadow,
+ * plus d_body (nullable): the hit body's index per pixel (gbuffer.comp h.obj; -1 = world or sky).
+ * d_shadow_raw NULL: no sun ray (vf_trace_primary's outputs).  A body hit's gbNormal is
+ * normalize(mat3(objToWorld) * n) (common.glsl:678) in RGBA8 (NaN -> 0). */
+int vf_trace_gbuffer(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
+                     float *d_gb_pos, uint32_t *d_gb_normal, int32_t *d_body, uint32_t *d_shadow_raw,
+                     vf_hit *d_shadow_hits);
 /* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map, grass and under-water
  * refraction): one sun ray per G-buffer hit from P + N*0.03 through the dry field (water clear), with
  * FOLIAGE_LIGHT: sunVisibilityT (common.glsl:1964-1981, deterministic) lets pass->look_p[3]

@@ -14,19 +14,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
-                      ("pad", "u1")])
+                      ("body", "u1")])
 
 
 class VfoWorld(C.Structure):
     _fields_ = [("wb", C.c_int32 * 3), ("win", C.c_int32 * 4), ("grid", C.c_void_p), ("atlas", C.c_void_p),
                 ("n_slots", C.c_uint32), ("dist", C.c_void_p), ("dry", C.c_void_p), ("coarse1", C.c_void_p),
                 ("coarse2", C.c_void_p), ("c1", C.c_int32 * 3), ("c2", C.c_int32 * 3), ("use_dist", C.c_int32),
-                ("coarse_skip", C.c_int32)]
+                ("coarse_skip", C.c_int32), ("n_bodies", C.c_int32), ("bodies", C.c_void_p), ("obj_grid", C.c_void_p),
+                ("bvh", C.c_void_p), ("n_nodes", C.c_int32), ("bvh_prims", C.c_void_p), ("use_bvh", C.c_int32)]
 
 
 class VfoHit(C.Structure):
     _fields_ = [("hit", C.c_int32), ("t", C.c_float), ("n", C.c_int32 * 3), ("mat", C.c_uint32),
-                ("cell", C.c_int32 * 3)]
+                ("cell", C.c_int32 * 3), ("body", C.c_int32)]
 
 
 class VfoCount(C.Structure):
@@ -45,6 +46,11 @@ class VfoTail(C.Structure):
     _fields_ = [("valid", C.c_int32), ("p", C.c_float * 3), ("n", C.c_float * 3), ("dir", C.c_float * 3),
                 ("thr", C.c_float * 3)]
 
+
+# 240-byte body header (ObjHeader, common.glsl:65-72; the C ABI's vf_obj_header)
+OBJ_HEADER_DTYPE = np.dtype([("world_to_obj", "<f4", (16,)), ("obj_to_world", "<f4", (16,)),
+                             ("prev_obj_to_world", "<f4", (16,)), ("dims_bricks", "<i4", (4,)),
+                             ("aabb_min", "<f4", (4,)), ("aabb_max", "<f4", (4,))])
 
 # 64-byte query / training record (vfo_record = the C ABI's vf_gi_record)
 RECORD_DTYPE = np.dtype([("pos", "<f4", (3,)), ("tag", "<u4"), ("n", "<f4", (3,)), ("pad0", "<f4"),
@@ -93,7 +99,12 @@ def lib():
             "vfo_cos": (C.c_float, [C.c_float]),
             "vfo_primary_dir": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P]),
             "vfo_probe_dir": (None, [C.c_int32, C.c_int32, C.c_uint32, P]),
-            "vfo_pass_primary": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, C.c_int]),
+            "vfo_pass_primary": (None, [P, P, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
+            "vfo_build_bvh": (None, [P, C.c_int32, P, P, P]),
+            "vfo_trace_objects": (None, [P, P, P, P, C.c_float, C.c_int32, P]),
+            "vfo_trace_scene": (None, [P, P, P, C.c_float, C.c_uint32, P, P]),
+            "vfo_trace_scene_batch": (None, [P, P, C.c_int64, C.c_float, C.c_uint32, P, P, C.c_int]),
+            "vfo_hit_normal": (None, [P, P, P]),
             "vfo_pass_primary_rows": (None, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
                                               C.c_int]),
             "vfo_pass_shadow": (None, [P, P, P, C.c_float, C.c_int32, C.c_int32, P, P, P, P, P, C.c_int]),
@@ -177,6 +188,65 @@ class OracleWorld:
     def ring(self):
         return self.win[3] != 0
 
+    # bodies (common.glsl:65-82, 664-793)
+    def set_bodies(self, headers, grid, use_bvh=True, bvh=None):
+        """headers: OBJ_HEADER_DTYPE records (or float32 [n, 60]); grid: the body grids (u32).  The BVH is
+        built by the oracle's build_bvh unless bvh = (nodes float32 [m, 8], prims u32) is given."""
+        h = np.ascontiguousarray(headers)
+        self.bodies = np.ascontiguousarray(h.view(np.float32).reshape(-1, 60))
+        n = self.bodies.shape[0]
+        self.obj_grid = np.ascontiguousarray(grid, dtype=np.uint32)
+        if self.obj_grid.size == 0:
+            self.obj_grid = np.zeros(1, np.uint32)
+        if bvh is None:
+            nodes = np.zeros((max(2 * n - 1, 1), 8), np.float32)
+            prims = np.zeros(max(n, 1), np.uint32)
+            nn = C.c_int32()
+            lib().vfo_build_bvh(_p(self.bodies), int(n), _p(nodes), C.byref(nn), _p(prims))
+            self.bvh_nodes, self.bvh_prims = nodes[:nn.value].copy(), prims[:n].copy()
+        else:
+            self.bvh_nodes = np.ascontiguousarray(bvh[0], dtype=np.float32).reshape(-1, 8)
+            self.bvh_prims = np.ascontiguousarray(bvh[1], dtype=np.uint32)
+        self.s.n_bodies = int(n)
+        self.s.bodies = self.bodies.ctypes.data
+        self.s.obj_grid = self.obj_grid.ctypes.data
+        self.s.bvh = self.bvh_nodes.ctypes.data if self.bvh_nodes.size else None
+        self.s.n_nodes = int(self.bvh_nodes.shape[0])
+        self.s.bvh_prims = self.bvh_prims.ctypes.data if self.bvh_prims.size else None
+        self.s.use_bvh = int(bool(use_bvh))
+
+    def use_bvh(self, on):
+        self.s.use_bvh = int(bool(on))
+
+    def trace_scene_one(self, ro, rd, t_cap=1e30, skip_mat=0):
+        """traceSceneS for one ray -> VfoHit (body = index + 1, 0 for the world)."""
+        h = VfoHit()
+        ro32, rd32 = f32(ro), f32(rd)
+        lib().vfo_trace_scene(C.byref(self.s), _p(ro32), _p(rd32), float(t_cap), int(skip_mat), C.byref(h), None)
+        return h
+
+    def trace_scene_rays(self, rays, t_cap=1e30, skip_mat=0, threads=0):
+        """traceSceneS per ray: (hits, body index per ray, -1 = world or miss)."""
+        r = f32(rays).reshape(-1, 6)
+        out = np.zeros(r.shape[0], HIT_DTYPE)
+        body = np.zeros(r.shape[0], np.int32)
+        lib().vfo_trace_scene_batch(C.byref(self.s), _p(r), r.shape[0], float(t_cap), int(skip_mat), _p(out), _p(body),
+                                    int(threads))
+        return out, body
+
+    def objects_only(self, ro, rd, limit, any_hit=False):
+        """traceObjectsA from an empty hit: VfoHit."""
+        h = VfoHit()
+        h.t = -1.0
+        ro32, rd32 = f32(ro), f32(rd)
+        lib().vfo_trace_objects(C.byref(self.s), _p(ro32), _p(rd32), C.byref(h), float(limit), int(bool(any_hit)), None)
+        return h
+
+    def hit_normal(self, h):
+        out = (C.c_float * 3)()
+        lib().vfo_hit_normal(C.byref(self.s), C.byref(h), out)
+        return np.array(out[:], np.float32)
+
     def full_box(self):
         lo = [self.win[0] if self.ring() else 0, 0, self.win[2] if self.ring() else 0]
         hi = [lo[0] + self.wb[0] - 1, self.wb[1] - 1, lo[2] + self.wb[2] - 1]
@@ -242,8 +312,10 @@ class OracleWorld:
         pos = np.zeros((n, 4), np.float32)
         nrm = np.zeros(n, np.uint32)
         cnt = VfoCount()
+        body = np.zeros(n, np.int32)
         lib().vfo_pass_primary(C.byref(self.s), C.byref(c), int(width), int(height), _p(hits), _p(pos), _p(nrm),
-                               C.byref(cnt), int(threads))
+                               _p(body), C.byref(cnt), int(threads))
+        self.last_body = body
         return hits, pos, nrm, cnt.as_dict()
 
     def pass_primary_rows(self, cam, width, height, y0, ystep, nrows, threads=0):
@@ -377,6 +449,14 @@ class OracleWorld:
         """exitSolid (common.glsl:990-1016)."""
         ro32, rd32 = f32(ro), f32(rd)
         return float(lib().vfo_exit_solid(C.byref(self.s), _p(ro32), _p(rd32), int(max_steps)))
+
+
+def face_code(n):
+    """Hit face code of an integer normal (vf_hit.face): 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z, 0 none."""
+    for a in range(3):
+        if n[a]:
+            return 2 * a + (1 if n[a] > 0 else 2)
+    return 0
 
 
 def pcg(v):

@@ -241,6 +241,231 @@ void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float t
     }
 }
 
+/* ---- bodies: traceObject / traceObjectsA / traceSceneS (common.glsl:664-720, 787-793) ---- */
+static inline int32_t fbits(float f) { int32_t v; memcpy(&v, &f, 4); return v; }
+static inline float ibits(int32_t v) { float f; memcpy(&f, &v, 4); return f; }
+/* ObjHeader as 60 floats: worldToObj [0..15], objToWorld [16..31], prevObjToWorld [32..47] (column-major,
+ * m[c][r] = a[4c + r]), dimsBricks [48..51] (int bits), aabbMin [52..55], aabbMax [56..59]. */
+static inline const float *obj_header(const vfo_world *w, int32_t oi) { return w->bodies + (size_t)oi * 60; }
+
+/* traceVolumeS with useTex false (common.glsl:541-658): the box [0, dims*8), the body's grid at its offset
+ * in objGrid, skipMat kSkipMat = 5, no water test, no distance-field or coarse skips. */
+static void trace_body(const vfo_world *w, int32_t oi, const float ro[3], const float rd[3], float tcap, vfo_hit *h,
+                       vfo_count *cnt) {
+    const float *b = obj_header(w, oi);
+    const int32_t db[3] = {fbits(b[48]), fbits(b[49]), fbits(b[50])};
+    const uint32_t goff = (uint32_t)fbits(b[51]);
+    const uint32_t skip = 5u;
+    memset(h, 0, sizeof(*h));
+    h->t = -1.0f;
+    float sd[3], inv[3], t0[3], t1[3], tmin3[3], tmax3[3], up[3];
+    int32_t st[3];
+    for (int a = 0; a < 3; ++a) {
+        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
+        inv[a] = 1.0f / sd[a];
+        float lo = 0.0f;
+        float dims = (float)(db[a] * 8);
+        t0[a] = (lo - ro[a]) * inv[a];
+        t1[a] = ((lo + dims) - ro[a]) * inv[a];
+        tmin3[a] = gmin(t0[a], t1[a]);
+        tmax3[a] = gmax(t0[a], t1[a]);
+        st[a] = sd[a] > 0.0f ? 1 : -1;
+        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
+    }
+    float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
+    float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
+    if (tExit < gmax(tEnter, 0.0f) || tEnter > tcap) return;
+    float t = gmax(tEnter, 0.0f);
+    int32_t bc[3], normal[3] = {0, 0, 0};
+    float tD8[3], tM8[3];
+    {
+        float e = t + 1e-3f;
+        for (int a = 0; a < 3; ++a) {
+            float entry = ro[a] + sd[a] * e;
+            bc[a] = iclamp(f2i(floorf(entry * 0.125f)), 0, db[a] - 1);
+            tD8[a] = fabsf(inv[a]) * 8.0f;
+            tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
+        }
+    }
+    if (tEnter > 0.0f) {
+        if (tEnter == tmin3[0]) normal[0] = -st[0];
+        else if (tEnter == tmin3[1]) normal[1] = -st[1];
+        else normal[2] = -st[2];
+    }
+    const int maxOuter = 3 * (db[0] + db[1] + db[2]) + 4;
+    for (int i = 0; i < maxOuter; ++i) {
+        if (t > tcap) return;
+        uint32_t g = w->obj_grid[goff + (uint32_t)((bc[2] * db[1] + bc[1]) * db[0] + bc[0])];
+        if (cnt) { cnt->outer++; cnt->bytes += 4; }
+        if (g >= 256u) {
+            if (brick_voxels(w, ro, sd, inv, st, up, bc, g, t, normal, skip, h, cnt)) return;
+        } else if (g != 0u && g != skip) {
+            float e = t + 1e-3f;
+            h->hit = 1; h->t = t; h->mat = g;
+            for (int a = 0; a < 3; ++a) { h->n[a] = normal[a]; h->cell[a] = f2i(floorf(ro[a] + sd[a] * e)); }
+            return;
+        }
+        int ax;
+        if (tM8[0] < tM8[1] && tM8[0] < tM8[2]) ax = 0;
+        else if (tM8[1] < tM8[2]) ax = 1;
+        else ax = 2;
+        bc[ax] += st[ax];
+        t = tM8[ax];
+        tM8[ax] += tD8[ax];
+        normal[0] = normal[1] = normal[2] = 0;
+        normal[ax] = -st[ax];
+        for (int a = 0; a < 3; ++a)
+            if (bc[a] < 0 || bc[a] >= db[a]) return;
+    }
+}
+
+static void box_span(const float lo[3], const float hi[3], const float ro[3], const float inv[3], float *tEnter,
+                     float *tExit) {
+    float tmin3[3], tmax3[3];
+    for (int a = 0; a < 3; ++a) {
+        float t0 = (lo[a] - ro[a]) * inv[a], t1 = (hi[a] - ro[a]) * inv[a];
+        tmin3[a] = gmin(t0, t1);
+        tmax3[a] = gmax(t0, t1);
+    }
+    *tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
+    *tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
+}
+
+/* traceObject (common.glsl:665-683) */
+static void trace_object(const vfo_world *w, int32_t oi, const float ro[3], const float rd[3], const float inv[3],
+                         vfo_hit *best, float *limit, vfo_count *cnt) {
+    const float *b = obj_header(w, oi);
+    float tEnter, tExit;
+    box_span(b + 52, b + 56, ro, inv, &tEnter, &tExit);
+    if (tExit < gmax(tEnter, 0.0f) || tEnter > *limit) return;
+    const float *m = b;   /* worldToObj, m[c][r] = m[4c + r] */
+    float roO[3], rdO[3];
+    for (int r = 0; r < 3; ++r) {
+        roO[r] = ((m[r] * ro[0] + m[4 + r] * ro[1]) + m[8 + r] * ro[2]) + m[12 + r];
+        rdO[r] = (m[r] * rd[0] + m[4 + r] * rd[1]) + m[8 + r] * rd[2];
+    }
+    vfo_hit oh;
+    trace_body(w, oi, roO, rdO, *limit, &oh, cnt);
+    if (oh.hit && oh.t < *limit) {
+        *best = oh;
+        best->body = oi + 1;
+        *limit = oh.t;
+    }
+}
+
+void vfo_trace_objects(const vfo_world *w, const float ro[3], const float rd[3], vfo_hit *best, float limit,
+                       int32_t any_hit, vfo_count *cnt) {
+    if (w->n_bodies == 0) return;
+    float inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / (fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a]);
+    if (!w->use_bvh) {   /* common.glsl:719 */
+        for (int32_t oi = 0; oi < w->n_bodies && !(any_hit && best->hit); ++oi) trace_object(w, oi, ro, rd, inv, best, &limit, cnt);
+        return;
+    }
+    /* common.glsl:697-716: near child first; subtrees beyond the current limit are culled */
+    int32_t stack[24];
+    int sp = 0, node = 0;
+    for (int it = 0; it < 512; ++it) {
+        const float *nd = w->bvh + (size_t)node * 8;
+        float tEnter, tExit;
+        box_span(nd, nd + 4, ro, inv, &tEnter, &tExit);
+        int visit = tExit >= gmax(tEnter, 0.0f) && tEnter <= limit;
+        uint32_t first = (uint32_t)fbits(nd[3]), count = (uint32_t)fbits(nd[7]);
+        if (visit && count == 0u) {
+            if (sp < 24) stack[sp++] = (int32_t)first + 1;
+            node = (int32_t)first;
+            continue;
+        }
+        if (visit)
+            for (uint32_t k = 0; k < count; ++k) trace_object(w, (int32_t)w->bvh_prims[first + k], ro, rd, inv, best, &limit, cnt);
+        if (any_hit && best->hit) return;
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+void vfo_trace_scene(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip, vfo_hit *h,
+                     vfo_count *cnt) {
+    vfo_trace(w, ro, rd, tcap, skip, h, cnt);
+    h->body = 0;
+    vfo_trace_objects(w, ro, rd, h, h->hit ? h->t : tcap, 0, cnt);
+}
+
+void vfo_hit_normal(const vfo_world *w, const vfo_hit *h, float out[3]) {
+    const float n[3] = {(float)h->n[0], (float)h->n[1], (float)h->n[2]};
+    if (h->body == 0) {
+        out[0] = n[0]; out[1] = n[1]; out[2] = n[2];
+        return;
+    }
+    const float *o = obj_header(w, h->body - 1) + 16;   /* objToWorld */
+    for (int r = 0; r < 3; ++r) out[r] = (o[r] * n[0] + o[4 + r] * n[1]) + o[8 + r] * n[2];
+    normalize3(out);
+}
+
+/* build_bvh (gpu_world.rs:550-600): box of the node's bodies; <= 2 bodies: a leaf; else the median of
+ * the box centroids (min + max) along the axis of their widest extent (strictly wider than a lower axis
+ * to win), children appended as a pair, the left subtree built first.  The median is taken after
+ * ordering the range by (centroid, body index), one of the partitions select_nth_unstable_by may give. */
+static void bvh_set(float *nd, const float mn[3], const float mx[3], uint32_t a, uint32_t cnt) {
+    nd[0] = mn[0]; nd[1] = mn[1]; nd[2] = mn[2]; nd[3] = ibits((int32_t)a);
+    nd[4] = mx[0]; nd[5] = mx[1]; nd[6] = mx[2]; nd[7] = ibits((int32_t)cnt);
+}
+static void bvh_build(const float *bodies, uint32_t *prims, uint32_t count, float *nodes, int32_t *n_nodes, int32_t n,
+                      uint32_t first) {
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t i = 0; i < count; ++i) {
+        const float *b = bodies + (size_t)prims[i] * 60;
+        for (int k = 0; k < 3; ++k) {
+            if (b[52 + k] < mn[k]) mn[k] = b[52 + k];
+            if (b[56 + k] > mx[k]) mx[k] = b[56 + k];
+        }
+    }
+    if (count <= 2) {
+        bvh_set(nodes + (size_t)n * 8, mn, mx, first, count);
+        return;
+    }
+    float cmn[3] = {1e30f, 1e30f, 1e30f}, cmx[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t i = 0; i < count; ++i) {
+        const float *b = bodies + (size_t)prims[i] * 60;
+        for (int k = 0; k < 3; ++k) {
+            float c = b[52 + k] + b[56 + k];
+            if (c < cmn[k]) cmn[k] = c;
+            if (c > cmx[k]) cmx[k] = c;
+        }
+    }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+        if (cmx[k] - cmn[k] > cmx[ax] - cmn[ax]) ax = k;
+    /* insertion sort of the range by (centroid on ax, index) */
+    for (uint32_t i = 1; i < count; ++i) {
+        uint32_t p = prims[i];
+        float cp = bodies[(size_t)p * 60 + 52 + ax] + bodies[(size_t)p * 60 + 56 + ax];
+        uint32_t j = i;
+        while (j > 0) {
+            uint32_t q = prims[j - 1];
+            float cq = bodies[(size_t)q * 60 + 52 + ax] + bodies[(size_t)q * 60 + 56 + ax];
+            if (cq < cp || (cq == cp && q < p)) break;
+            prims[j] = q;
+            --j;
+        }
+        prims[j] = p;
+    }
+    uint32_t mid = count / 2;
+    int32_t left = *n_nodes;
+    *n_nodes += 2;
+    bvh_set(nodes + (size_t)n * 8, mn, mx, (uint32_t)left, 0u);
+    bvh_build(bodies, prims, mid, nodes, n_nodes, left, first);
+    bvh_build(bodies, prims + mid, count - mid, nodes, n_nodes, left + 1, first + mid);
+}
+
+void vfo_build_bvh(const float *bodies, int32_t n, float *nodes, int32_t *n_nodes, uint32_t *prims) {
+    *n_nodes = 0;
+    if (n <= 0) return;
+    for (int32_t i = 0; i < n; ++i) prims[i] = (uint32_t)i;
+    *n_nodes = 1;
+    bvh_build(bodies, prims, (uint32_t)n, nodes, n_nodes, 0, 0u);
+}
+
 static void to_rec(const vfo_hit *h, vfo_hitrec *r) {
     memset(r, 0, sizeof(*r));
     if (!h->hit) { r->t = -1.0f; return; }
@@ -249,6 +474,7 @@ static void to_rec(const vfo_hit *h, vfo_hitrec *r) {
     r->mat = (uint8_t)h->mat;
     r->face = (uint8_t)vfo_face_code(h->n);
     r->hit = 1;
+    r->body = h->body != 0;
 }
 
 static int nthreads(int threads) {
@@ -280,6 +506,18 @@ void vfo_trace_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float t
         add_cnt(&total, &c);
     }
     if (cnt) add_cnt(cnt, &total);
+}
+
+/* traceSceneS per ray (world + bodies), with the body index per ray (-1 = world or miss; nullable). */
+void vfo_trace_scene_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip,
+                           vfo_hitrec *out, int32_t *body, int threads) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads(threads))
+    for (int64_t i = 0; i < n; ++i) {
+        vfo_hit h;
+        vfo_trace_scene(w, rays[i].o, rays[i].d, tcap, skip, &h, NULL);
+        to_rec(&h, &out[i]);
+        if (body) body[i] = h.hit ? h.body - 1 : -1;
+    }
 }
 
 /* ---- FP64 plain voxel march: an independent formulation over the same voxels ----
@@ -687,7 +925,8 @@ float vfo_exit_solid(const vfo_world *w, const float ro[3], const float rd[3], i
     return -1.0f;
 }
 
-/* gbuffer.comp:233-266: the primary trace with the camera-inside-solid pre-step. */
+/* gbuffer.comp:233-266: the primary trace (traceScenePrimary: world, then bodies) with the
+ * camera-inside-solid pre-step. */
 static void primary_trace(const vfo_world *w, const float cam[3], uint32_t cam_mat, const float rd[3], vfo_hit *h,
                           vfo_count *cnt) {
     float ro[3] = {cam[0], cam[1], cam[2]}, tskip = 0.0f;
@@ -698,7 +937,7 @@ static void primary_trace(const vfo_world *w, const float cam[3], uint32_t cam_m
         tskip = gmax(tskip, 0.0f);
         for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
     }
-    vfo_trace(w, ro, rd, 1e30f, cam_mat == 5u ? 5u : 0u, h, cnt);
+    vfo_trace_scene(w, ro, rd, 1e30f, cam_mat == 5u ? 5u : 0u, h, cnt);
     h->t = h->t + tskip;
     if (void_ray) h->hit = 0;
 }
@@ -706,9 +945,21 @@ static void primary_trace(const vfo_world *w, const float cam[3], uint32_t cam_m
 /* ---- passes ---- */
 static inline uint32_t unorm8(float v) { return (uint32_t)floorf(fclamp(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
 static inline float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
+/* gbNormal RGBA8 of the hit's world normal (n * 0.5 + 0.5; a NaN component stores 0, as Vulkan's
+ * float -> unorm conversion does). */
+static uint32_t pack_normal(const vfo_world *w, const vfo_hit *h) {
+    float n[3];
+    vfo_hit_normal(w, h, n);
+    uint32_t v = 255u << 24;
+    for (int a = 0; a < 3; ++a) {
+        float x = n[a] * 0.5f + 0.5f;
+        v |= (x == x ? unorm8(x) : 0u) << (8 * a);
+    }
+    return v;
+}
 
 void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H, vfo_hitrec *hits,
-                      float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads) {
+                      float *gb_pos, uint32_t *gb_normal, int32_t *body, vfo_count *cnt, int threads) {
     vfo_count total = {0, 0, 0, 0};
     const uint32_t cam_mat = vfo_voxel_mat_at(w, c->pos);
 #pragma omp parallel num_threads(nthreads(threads))
@@ -731,14 +982,8 @@ void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t W, int32_t H
                     gb_pos[i * 4 + 3] = -1.0f;
                 }
             }
-            if (gb_normal) {
-                uint32_t v = 0;
-                if (h.hit) {
-                    v = unorm8((float)h.n[0] * 0.5f + 0.5f) | (unorm8((float)h.n[1] * 0.5f + 0.5f) << 8) |
-                        (unorm8((float)h.n[2] * 0.5f + 0.5f) << 16) | (255u << 24);
-                }
-                gb_normal[i] = v;
-            }
+            if (gb_normal) gb_normal[i] = h.hit ? pack_normal(w, &h) : 0u;
+            if (body) body[i] = h.hit ? h.body - 1 : -1;
         }
 #pragma omp critical
         add_cnt(&total, &cc);
@@ -767,8 +1012,7 @@ void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int3
             if (h.hit) {
                 for (int a = 0; a < 3; ++a) gb_pos[i * 4 + a] = c->pos[a] + rd[a] * h.t;
                 gb_pos[i * 4 + 3] = h.t;
-                gb_normal[i] = unorm8((float)h.n[0] * 0.5f + 0.5f) | (unorm8((float)h.n[1] * 0.5f + 0.5f) << 8) |
-                               (unorm8((float)h.n[2] * 0.5f + 0.5f) << 16) | (255u << 24);
+                gb_normal[i] = pack_normal(w, &h);
             } else {
                 gb_pos[i * 4 + 0] = gb_pos[i * 4 + 1] = gb_pos[i * 4 + 2] = 0.0f;
                 gb_pos[i * 4 + 3] = -1.0f;
@@ -907,7 +1151,7 @@ void vfo_cosine_hemisphere(const float n[3], uint32_t *state, float out[3]) {
     vfo_cosine_hemisphere_u(n, u, out);
 }
 
-/* sunVisibilityT (common.glsl:1964-1981) over the world: leaves (palette kind 3) let leaf_light of the
+/* sunVisibilityT (common.glsl:1964-1981) over the scene (a body always blocks): leaves (palette kind 3) let leaf_light of the
  * light through (stochastic: with that probability), up to 6; the far-horizon and cloud factors are 1
  * (neither is traced here).  first: the first segment's hit; the ray counter counts the ray once. */
 float vfo_sun_visibility_t(const vfo_world *w, const float *palette, const float o0[3], const float d[3],
@@ -917,14 +1161,14 @@ float vfo_sun_visibility_t(const vfo_world *w, const float *palette, const float
     *tfirst = 0.0f;
     for (int i = 0; i < 6; ++i) {
         vfo_hit h;
-        vfo_trace(w, o, d, 1e30f, 5u, &h, cnt);
+        vfo_trace_scene(w, o, d, 1e30f, 5u, &h, cnt);   /* traceSceneShadow */
         if (cnt && i > 0) cnt->rays--;
         if (i == 0 && first) *first = h;
         if (!h.hit) return T;
         float p[3];
         for (int a = 0; a < 3; ++a) p[a] = o[a] + d[a] * h.t;
         if (i == 0) *tfirst = h.t;
-        if (palette[(size_t)h.mat * 12 + 6] != 3.0f || leaf_light <= 0.0f) return 0.0f;
+        if (h.body != 0 || palette[(size_t)h.mat * 12 + 6] != 3.0f || leaf_light <= 0.0f) return 0.0f;
         if (stochastic) {
             if (rnd(seed) >= leaf_light) return 0.0f;
         } else {
@@ -984,7 +1228,7 @@ void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun
     }
 }
 
-/* giPathImpl (common.glsl:2250-2366) without bodies, NEE lamps or the hash cache, for lastStage 1 (one
+/* giPathImpl (common.glsl:2250-2366) with GI_OBJECTS, without NEE lamps or the hash cache, for lastStage 1 (one
  * bounce) or 3 (two bounces); traceOn = the rising-ray trace-on past the reach (common.glsl:2309-2320).
  * seed: the path's RNG state (bounce 2's direction, cosineHemisphere(hitN, seed)).  With tail non-null
  * the path stops where bounce 2 would start (after `thr *= hitAlb * giScale`, common.glsl:2277 and
@@ -995,7 +1239,7 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
     float thr[3] = {1.0f, 1.0f, 1.0f}, ro[3], rdir[3], hitAlb[3] = {0, 0, 0}, hitP[3] = {0, 0, 0};
     float hitN[3] = {0.0f, 1.0f, 0.0f}, hitD[3] = {0, 0, 0};
     float hitNdl = 0.0f, reach = s->gi_reach;
-    int stage = 0, extended = 0;
+    int stage = 0, extended = 0, hit_body = 0;
     L[0] = L[1] = L[2] = 0.0f;
     if (bounce) { memset(bounce, 0, sizeof *bounce); bounce->t = -1.0f; }
     if (tail) memset(tail, 0, sizeof *tail);
@@ -1004,6 +1248,10 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
         int sunRay = (stage & 1) == 1;
         vfo_hit h;
         vfo_trace(w, ro, rdir, sunRay ? 1e30f : reach, 0u, &h, cnt);
+        if (!(sunRay && h.hit)) {   /* GI_OBJECTS (common.glsl:2267-2271) */
+            if (!sunRay) h.body = 0;
+            vfo_trace_objects(w, ro, rdir, &h, sunRay ? 1e30f : gmin(h.hit ? h.t : reach, 16.0f), sunRay, cnt);
+        }
         int next_bounce = 0;
         if (sunRay) {
             if (!h.hit)
@@ -1039,8 +1287,9 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
             const float *mat = s->palette + (size_t)h.mat * 12;
             float emit = mat[3] * 8.0f;
             if (s->lights_nee && mat[3] >= 0.5f) emit = 0.0f;
+            vfo_hit_normal(w, &h, hitN);
+            if (stage == 0) hit_body = h.body != 0;   /* common.glsl:2327 */
             for (int a = 0; a < 3; ++a) {
-                hitN[a] = (float)h.n[a];
                 L[a] += thr[a] * mat[a] * emit;
                 hitP[a] = ro[a] + rdir[a] * h.t;
                 hitD[a] = rdir[a];
@@ -1059,7 +1308,7 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
         }
         if (next_bounce) {   /* bounce 2 from the first hit */
             for (int a = 0; a < 3; ++a) thr[a] = thr[a] * (hitAlb[a] * s->gi_scale);
-            if (tail) {
+            if (tail && !hit_body) {   /* a body's bounce-1 hit is never a cache query (common.glsl:2283) */
                 tail->valid = 1;
                 for (int a = 0; a < 3; ++a) {
                     tail->p[a] = hitP[a];

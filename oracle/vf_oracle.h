@@ -48,6 +48,16 @@ typedef struct vfo_world {
     int32_t c1[3], c2[3];
     int32_t use_dist;         /* USE_DIST_FIELD (default on, cvars.rs:110) */
     int32_t coarse_skip;      /* kCoarseSkip (common.glsl:20); 0 = plain brick-by-brick DDA */
+    /* bodies (common.glsl:65-82): numObjects, the ObjHeaders (240 B = 60 floats each: worldToObj,
+     * objToWorld, prevObjToWorld column-major, dimsBricks as int bits, aabbMin, aabbMax), objGrid, and the
+     * BVH (8 floats per node: min.xyz, first bits, max.xyz, count bits) with its prims. */
+    int32_t n_bodies;
+    const float *bodies;
+    const uint32_t *obj_grid;
+    const float *bvh;
+    int32_t n_nodes;
+    const uint32_t *bvh_prims;
+    int32_t use_bvh;          /* USE_OBJ_BVH (default on, renderer/mod.rs:96); 0 = the loop over every body */
 } vfo_world;
 
 typedef struct vfo_hit {
@@ -55,7 +65,8 @@ typedef struct vfo_hit {
     float t;
     int32_t n[3];             /* face normal, one axis +-1, or all 0 (ray started inside the box) */
     uint32_t mat;
-    int32_t cell[3];          /* absolute voxel coordinates */
+    int32_t cell[3];          /* absolute voxel coordinates (a body's: its own voxel coordinates) */
+    int32_t body;             /* 0 = the world, else the body index + 1 (Hit.obj + 1) */
 } vfo_hit;
 
 /* Fetch counters of the algorithmic byte model: grid 4 B per outer iteration, 1 B per
@@ -66,13 +77,26 @@ typedef struct vfo_count {
 
 /* Explicit ray record (24 B) and hit record (20 B), the same as the C-ABI's vf_ray / vf_hit. */
 typedef struct vfo_ray { float o[3]; float d[3]; } vfo_ray;
-typedef struct vfo_hitrec { int32_t cell[3]; float t; uint8_t mat, face, hit, pad; } vfo_hitrec;
+typedef struct vfo_hitrec { int32_t cell[3]; float t; uint8_t mat, face, hit, body; } vfo_hitrec;
 
 /* --- traversal (common.glsl:474-658) --- */
 void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip_mat,
                vfo_hit *out, vfo_count *cnt);
 void vfo_trace_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip_mat,
                      vfo_hitrec *out, vfo_count *cnt, int threads);
+/* --- bodies (common.glsl:664-720, 787-793; gpu_world.rs:550-600) --- */
+/* build_bvh over the bodies' world boxes: nodes (8 floats each, at most 2n - 1) and prims (n). */
+void vfo_build_bvh(const float *bodies, int32_t n, float *nodes, int32_t *n_nodes, uint32_t *prims);
+/* traceObjectsA: the nearest body hit closer than limit merged into *best (any_hit: the first found). */
+void vfo_trace_objects(const vfo_world *w, const float ro[3], const float rd[3], vfo_hit *best, float limit,
+                       int32_t any_hit, vfo_count *cnt);
+/* traceSceneS: the world, then the bodies closer than its hit. */
+void vfo_trace_scene(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip_mat,
+                     vfo_hit *out, vfo_count *cnt);
+void vfo_trace_scene_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip_mat,
+                           vfo_hitrec *out, int32_t *body, int threads);
+/* The hit's world normal: the face normal, or normalize(mat3(objToWorld) * n) for a body. */
+void vfo_hit_normal(const vfo_world *w, const vfo_hit *h, float out[3]);
 /* FP64 plain voxel march over the same voxels (independent pin for t). */
 void vfo_march_f64(const vfo_world *w, const double ro[3], const double rd[3], double tcap, uint32_t skip_mat,
                    int32_t *hit, double *t, int32_t cell[3], int32_t n[3], uint32_t *mat);
@@ -136,10 +160,10 @@ typedef struct vfo_sky {
     const uint16_t *skytop;   /* (wbx*8) x (wbz*8), index z*(wbx*8) + x */
 } vfo_sky;
 
-/* gbuffer.comp primary pass (world only; a camera inside solid terrain or water as gbuffer.comp:233-266):
- * hits + gbPos (xyz, t) + gbNormal (RGBA8). */
+/* gbuffer.comp primary pass over the scene (traceScenePrimary; a camera inside solid terrain or water as
+ * gbuffer.comp:233-266): hits + gbPos (xyz, t) + gbNormal (RGBA8) + the body index (-1 world / sky). */
 void vfo_pass_primary(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height,
-                      vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, vfo_count *cnt, int threads);
+                      vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal, int32_t *body, vfo_count *cnt, int threads);
 void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t width, int32_t height, int32_t y0,
                            int32_t ystep, int32_t nrows, vfo_hitrec *hits, float *gb_pos, uint32_t *gb_normal,
                            vfo_count *cnt, int threads);

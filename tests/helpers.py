@@ -55,3 +55,37 @@ def hit_diff(a, b):
     ab = a.view(np.uint8).reshape(-1, 20)
     bb = b.view(np.uint8).reshape(-1, 20)
     return np.nonzero((ab != bb).any(1))[0]
+
+
+def random_bodies(rng, n, lo, hi, max_size=20):
+    """n small bodies (crates, spheres, pillars, and blocks of random voxels) with random poses whose
+    centres lie in the box [lo, hi)."""
+    from vftrace import bodies as B
+    vols, poses = [], []
+    for k in range(n):
+        kind = k % 4
+        if kind == 0:
+            v = B.crate(int(rng.integers(6, max_size)))
+        elif kind == 1:
+            v = B.sphere(int(rng.integers(3, max_size // 2)), int(rng.choice([17, 19, 21])))
+        elif kind == 2:
+            v = B.pillar(int(rng.integers(3, 10)), int(rng.integers(8, max_size + 8)), 8)
+        else:
+            s = int(rng.integers(1, 3)) * 8
+            v = np.where(rng.random((s, s, s)) < 0.3, rng.integers(1, 30, (s, s, s)), 0).astype(np.uint8)
+            v[v == 5] = 6   # no water: bodies trace with water clear
+        c = lo + rng.random(3) * (np.asarray(hi) - np.asarray(lo))
+        poses.append(B.pose(c, v.shape, float(rng.uniform(0, 6.283)), float(rng.uniform(-1.0, 1.0))))
+        vols.append(v)
+    return vols, poses
+
+
+def body_world(vox, vols, poses, palette=None, **oracle_kw):
+    """A dense world with bodies: (packed arrays incl. the body slots, headers, body grid, oracle world)."""
+    from vftrace import bodies as B
+    w = vf.GenWorld.from_dense(vox, (0, 0, 0, 0), palette)
+    a = w.arrays()
+    a2, hd, g = B.attach(a, vols, poses)
+    ow = O.OracleWorld(a2, **oracle_kw)
+    ow.set_bodies(hd, g)
+    return a2, hd, g, ow

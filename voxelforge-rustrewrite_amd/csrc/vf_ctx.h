@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "vf_bodies.h"
 #include "vf_device.h"
 #include "vf_internal.h"
 
@@ -52,6 +53,17 @@ struct vf_ctx {
     unsigned long long *q_ctr = nullptr, *q_stamps = nullptr;
     float4 *q_rays = nullptr, *q_sun = nullptr;
     int64_t q_cap = 0;
+    // bodies (vf_bodies.hip): repacked headers, grids, BVH; grown, never shrunk, so frame graphs that
+    // recorded them stay valid while the body count is unchanged
+    int32_t n_bodies = 0;
+    float4 *b_hdr = nullptr, *b_nodes = nullptr;
+    int32_t *b_esc = nullptr;
+    uint32_t *b_grid = nullptr, *b_prims = nullptr;
+    size_t b_hdr_cap = 0, b_grid_cap = 0, b_node_cap = 0, b_esc_cap = 0, b_prim_cap = 0;   // elements
+    uint32_t b_slot_end = 0;                  // one past the highest atlas slot the body grids name
+    std::vector<uint32_t> b_host_grid;        // host copy of the body grids
+    std::vector<vf_bvh_node> b_host_nodes;    // the BVH in use
+    std::vector<uint32_t> b_host_prims;
 };
 
 struct vf_graph {
@@ -70,6 +82,11 @@ int check_ready(vf_ctx *ctx, const char *what);
 int no_capture(vf_ctx *ctx, const char *what);
 void free_queue(vf_ctx *ctx);
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+// bodies (vf_bodies.hip): the view the kernels get (n = 0 unless the scene's num_objects is set), and the
+// launch-time check that the scene's num_objects matches the uploaded bodies and their bricks exist
+DevBodies dev_bodies(const vf_ctx *ctx);
+int check_bodies(vf_ctx *ctx, const char *what);
+void free_bodies(vf_ctx *ctx);
 void slot_occupancy(const uint8_t *vox, uint64_t out[8]);
 void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n_slots);
 }  // namespace vf

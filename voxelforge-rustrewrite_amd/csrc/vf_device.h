@@ -43,6 +43,7 @@ struct Hit {
     int32_t face;
     uint32_t mat;
     int32_t cell[3];
+    int32_t obj;   // the body hit (-1 = the world); set and read only by the body-aware paths
 };
 
 // GLSL built-ins with their specified definitions (signed zeros matter for bit equality).
@@ -302,12 +303,12 @@ __device__ __forceinline__ int32_t xcd_remap(int32_t i, int32_t n) {
     return x * q + imin(x, r) + (i >> 3);
 }
 
-__device__ __forceinline__ void store_hit(vf_hit *out, const Hit &h) {
+__device__ __forceinline__ void store_hit(vf_hit *out, const Hit &h, bool body = false) {
     uint32_t *o = reinterpret_cast<uint32_t *>(out);
     if (h.hit) {
         o[0] = (uint32_t)h.cell[0]; o[1] = (uint32_t)h.cell[1]; o[2] = (uint32_t)h.cell[2];
         o[3] = __float_as_uint(h.t);
-        o[4] = (h.mat & 255u) | ((uint32_t)h.face << 8) | (1u << 16);
+        o[4] = (h.mat & 255u) | ((uint32_t)h.face << 8) | (1u << 16) | ((uint32_t)body << 24);
     } else {
         o[0] = 0u; o[1] = 0u; o[2] = 0u;
         o[3] = __float_as_uint(-1.0f);

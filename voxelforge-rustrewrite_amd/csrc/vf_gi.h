@@ -2,6 +2,7 @@
 // skyOpen, giEscape, skyColor, the PCG streams and giPathImpl.
 #pragma once
 
+#include "vf_bodies.h"
 #include "vf_device.h"
 
 // skyOpen with colLoaded (common.glsl:232-258); a bounded world has no horizon (outside -> 1).
@@ -117,19 +118,22 @@ struct GiPathCfg {
     int32_t lights_nee; // LIGHTS_NEE
 };
 
-// giPathImpl (common.glsl:2250-2366) without bodies, lamps' NEE or the hash cache.  LAST_STAGE 1: one
+// giPathImpl (common.glsl:2250-2366) without lamps' NEE or the hash cache.  LAST_STAGE 1: one
 // bounce (+ its sun ray); 3: two bounces.  trace_on: the rising-ray trace-on past the reach
-// (common.glsl:2309-2320).  TAIL: stop where bounce 2 would start and report it in tail.
-template <int LAST_STAGE, bool TAIL, bool COUNT>
+// (common.glsl:2309-2320).  TAIL: stop where bounce 2 would start and report it in tail, unless the
+// bounce-1 hit is a body (a moving surface is never answered by a cache, common.glsl:2327, 2283): that
+// path is traced to its end.  BODIES (GI_OBJECTS, common.glsl:2267-2271): a bounce ray also meets the
+// bodies nearer than min(its world hit or reach, 16), a sun ray that missed the world any body.
+template <int LAST_STAGE, bool TAIL, bool COUNT, bool BODIES = false>
 __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &G, const float P[3], const float N[3],
                         const float dir0[3], uint32_t &seed, bool trace_on, float L[3], Hit *bounce, GiTail *tail,
-                        Cnt &c) {
+                        Cnt &c, const DevBodies *B = nullptr) {
     float thr[3] = {1.0f, 1.0f, 1.0f}, ro[3], rdir[3];
     float hitAlb[3] = {0.0f, 0.0f, 0.0f}, hitP[3] = {0.0f, 0.0f, 0.0f}, hitN[3] = {0.0f, 1.0f, 0.0f};
     float hitD[3] = {0.0f, 0.0f, 0.0f};
     float hitNdl = 0.0f, reach = G.reach;
     int stage = 0;
-    bool extended = false;
+    bool extended = false, hit_body = false;
     L[0] = L[1] = L[2] = 0.0f;
     if (TAIL) tail->valid = false;
 #pragma unroll
@@ -138,6 +142,11 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
         const bool sunRay = (stage & 1) == 1;
         Hit h;
         trace_world<0, COUNT>(W, ro, rdir, sunRay ? 1e30f : reach, h, c);
+        if (BODIES) {
+            h.obj = -1;
+            if (!(sunRay && h.hit))
+                trace_objects<COUNT>(W, *B, ro, rdir, h, sunRay ? 1e30f : gmin(h.hit ? h.t : reach, 16.0f), sunRay, c);
+        }
         if (sunRay) {
             if (!h.hit) {
 #pragma unroll
@@ -177,7 +186,8 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
             float emit = alb4.w * 8.0f;
             if (G.lights_nee && alb4.w >= 0.5f) emit = 0.0f;
             const float alb[3] = {alb4.x, alb4.y, alb4.z};
-            face_normal(h.face, hitN);
+            hit_normal<BODIES>(B, h, hitN);
+            if (BODIES && stage == 0) hit_body = h.obj >= 0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 L[a] += thr[a] * alb[a] * emit;
@@ -199,7 +209,7 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
         // bounce 2 from the first hit
 #pragma unroll
         for (int a = 0; a < 3; ++a) thr[a] = thr[a] * (hitAlb[a] * S.gi_scale);
-        if (TAIL) {
+        if (TAIL && !hit_body) {
             tail->valid = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -235,24 +245,26 @@ __device__ __forceinline__ void sample_cone(const float dir[3], float ang, uint3
     normalize3(out);
 }
 
-// sunVisibilityT (common.glsl:1964-1981) over the world: a leaf voxel (palette kind 3) lets the light
+// sunVisibilityT (common.glsl:1964-1981) over the world (and the bodies: traceSceneShadow): a leaf voxel (palette kind 3) lets the light
 // through with probability (STOCHASTIC) or fraction leaf_light, and the ray goes on from where it
 // leaves that voxel; up to 6 leaves.  tfirst = distance to the first blocker (0 = none).  The far
 // horizon's and the clouds' factors (horizonSunVisibility, cloudShadow) are 1 here: neither the
 // horizon heightfield nor clouds are traced by this library.  The first
 // segment's hit record at first_base[idx] (first_base nullable); the ray counter counts the whole
 // visibility ray once.
-template <bool STOCHASTIC, bool COUNT>
+template <bool STOCHASTIC, bool COUNT, bool BODIES = false>
 __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float o0[3], const float d[3], float leaf_light,
-                                                  uint32_t &seed, float &tfirst, vf_hit *first_base, size_t idx, Cnt &c) {
+                                                  uint32_t &seed, float &tfirst, vf_hit *first_base, size_t idx, Cnt &c,
+                                                  const DevBodies *B = nullptr) {
     // the first segment: the plain sun ray (most rays end here: no hit, or a hit that is not a leaf)
     Hit h;
     trace_world<5, COUNT>(W, o0, d, 1e30f, h, c);
-    if (first_base) store_hit(first_base + idx, h);
+    if (BODIES) add_bodies<COUNT>(W, *B, o0, d, 1e30f, h, c);   // traceSceneShadow: a body always blocks
+    if (first_base) store_hit(first_base + idx, h, BODIES && h.obj >= 0);
     tfirst = 0.0f;
     if (!h.hit) return 1.0f;
     tfirst = h.t;
-    if (W.palette[h.mat * 3u + 1u].z != 3.0f || leaf_light <= 0.0f) return 0.0f;
+    if ((BODIES && h.obj >= 0) || W.palette[h.mat * 3u + 1u].z != 3.0f || leaf_light <= 0.0f) return 0.0f;
     // through leaves: leave each leaf voxel across its exit face along d and trace on
     float T = 1.0f;
     float o[3] = {o0[0], o0[1], o0[2]};
@@ -276,9 +288,10 @@ __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float
 #pragma unroll
         for (int a = 0; a < 3; ++a) o[a] = p[a] + d[a] * tm;
         trace_world<5, COUNT>(W, o, d, 1e30f, h, c);
+        if (BODIES) add_bodies<COUNT>(W, *B, o, d, 1e30f, h, c);
         if (COUNT) c.rays--;
         if (!h.hit) return T;
-        if (W.palette[h.mat * 3u + 1u].z != 3.0f) return 0.0f;
+        if ((BODIES && h.obj >= 0) || W.palette[h.mat * 3u + 1u].z != 3.0f) return 0.0f;
     }
     return 0.0f;
 }

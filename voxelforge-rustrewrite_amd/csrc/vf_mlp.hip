@@ -559,7 +559,8 @@ __device__ __forceinline__ void sample_start(const float N[3], int32_t px, int32
 }
 
 // Plain traced two-bounce GI: the mean of spp full paths per pixel.
-__device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &S, const PathArgs &A,
+template <bool BODIES>
+__device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &S, const DevBodies &B, const PathArgs &A,
                                             const float4 *gbpos, const uint32_t *gbn, int32_t px, int32_t py,
                                             float4 *out) {
     const size_t i = (size_t)py * A.w + px;
@@ -573,7 +574,7 @@ __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, false, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c);
+            gi_path<3, false, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -582,11 +583,12 @@ __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &
     out[i] = sky ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(sum[0] / fs, sum[1] / fs, sum[2] / fs, 1.0f);
 }
 
-__global__ void __launch_bounds__(256) k_gi_plain(DevWorld W, SkyParams S, PathArgs A, const float4 *gbpos,
+template <bool BODIES>
+__global__ void __launch_bounds__(256) k_gi_plain(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
                                                   const uint32_t *gbn, float4 *out) {
     int32_t px, py;
     if (!tile_pixel(A, px, py)) return;
-    plain_pixel(W, S, A, gbpos, gbn, px, py, out);
+    plain_pixel<BODIES>(W, S, B, A, gbpos, gbn, px, py, out);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
@@ -609,13 +611,14 @@ __device__ __forceinline__ void push_query(bool want, unsigned *qcount, float4 *
 // Cached render, traced part.  Fallback frames: the plain path, written to out.  Otherwise terminate 1:
 // per sample the path up to its tail (pre += its light; a query at the tail, or tailv[slot] = 0);
 // terminate 0: nothing (k_mlp_infer<2> answers at the G-buffer points).
-__global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, PathArgs A, const float4 *gbpos,
+template <bool BODIES>
+__global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
                                                    const uint32_t *gbn, const DevStats *st, float4 *out, float4 *pre,
                                                    float4 *tailv, float4 *queries, unsigned *qcount) {
     int32_t px, py;
     const bool inside = tile_pixel(A, px, py);
     if (st->fallback) {
-        if (inside) plain_pixel(W, S, A, gbpos, gbn, px, py, out);
+        if (inside) plain_pixel<BODIES>(W, S, B, A, gbpos, gbn, px, py, out);
         return;
     }
     if (A.terminate == 0) return;   // answered at the G-buffer point by k_mlp_infer<2>
@@ -633,7 +636,7 @@ __global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, Path
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, true, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c);
+            gi_path<3, true, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -674,7 +677,8 @@ struct SampleArgs {
 
 // Training samples: sample i takes pixel floor((i + r) * W*H / n) (r from its own stream
 // pcg(i + pcg(frame ^ salt))) and a cosine-hemisphere bounce-1 direction from the same stream.
-__global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, SampleArgs A, const float4 *gbpos,
+template <bool BODIES>
+__global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, DevBodies B, SampleArgs A, const float4 *gbpos,
                                                     const uint32_t *gbn, float4 *out) {
     const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
     if (i >= A.n) return;
@@ -695,7 +699,7 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Sam
     cosine_hemisphere(N, seed, dir);
     Cnt c = {0, 0, 0, 0};
     if (A.terminate == 0) {
-        gi_path<3, false, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c);
+        gi_path<3, false, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
         float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
         normalize3(v);
         r[0] = make_float4(P[0], P[1], P[2], __uint_as_float(1u));
@@ -705,14 +709,14 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Sam
         return;
     }
     GiTail t;
-    gi_path<3, true, false>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c);
+    gi_path<3, true, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
     if (!t.valid) {
         r[0] = z; r[1] = z; r[2] = z; r[3] = z;
         return;
     }
     float d2[3], L2[3];
     cosine_hemisphere(t.n, seed, d2);
-    gi_path<1, false, false>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c);
+    gi_path<1, false, false, BODIES>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
     r[0] = make_float4(t.p[0], t.p[1], t.p[2], __uint_as_float(1u));
     r[1] = make_float4(t.n[0], t.n[1], t.n[2], 0.0f);
     r[2] = make_float4(-t.dir[0], -t.dir[1], -t.dir[2], 0.0f);
@@ -1072,7 +1076,7 @@ int check_gbuffer(vf_ctx *ctx, const char *what, const vf_pass_ubo *pass, int32_
     if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "%s: no scene (vf_set_scene)", what);
     if (!pass || !gb_pos || !gb_normal) return gc_fail_null(ctx, what);
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 26)) return vf::fail(ctx, VF_E_INVALID, "%s: bad size %dx%d", what, w, h);
-    return VF_OK;
+    return vf::check_bodies(ctx, what);
 }
 
 PathArgs path_args(const vf_pass_ubo *pass, int32_t w, int32_t h, uint32_t frame, int32_t spp, uint32_t salt, float reach,
@@ -1102,8 +1106,13 @@ int launch_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t 
     A.G.reach = c->p.gi_reach;
     A.G.lights_nee = c->p.lights_nee ? 1 : 0;
     for (int a = 0; a < 3; ++a) A.cam[a] = pass->cam_pos[a];
-    hipLaunchKernelGGL(k_gc_samples, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
-                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
+    const DevBodies B = vf::dev_bodies(ctx);
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_gc_samples<true>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
+    else
+        hipLaunchKernelGGL(k_gc_samples<false>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
     return launch_check(ctx, "k_gc_samples");
 }
 
@@ -1211,8 +1220,13 @@ int vf_trace_gi_path(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h,
     if (!(gi_reach > 0.0f) || gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: gi_reach out of range");
     const PathArgs A = path_args(pass, w, h, frame, spp, salt, gi_reach, lights_nee, 1);
     if ((rc = vf::begin_timed(ctx))) return rc;
-    hipLaunchKernelGGL(k_gi_plain, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
-                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
+    const DevBodies B = vf::dev_bodies(ctx);
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_gi_plain<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
+    else
+        hipLaunchKernelGGL(k_gi_plain<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
     if ((rc = launch_check(ctx, "k_gi_plain"))) return rc;
     return vf::end_timed(ctx);
 }
@@ -1260,9 +1274,15 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
     // 3. render: traced part + queries, the network on the queries, resolve
     const int32_t spp = c->p.spp;
     const PathArgs R = path_args(pass, w, h, frame, spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate);
-    hipLaunchKernelGGL(k_gc_render, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, R,
-                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
-                       c->pre, c->tailv, c->queries, c->qcount);
+    const DevBodies B = vf::dev_bodies(ctx);
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_gc_render<true>, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, R,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
+                           c->pre, c->tailv, c->queries, c->qcount);
+    else
+        hipLaunchKernelGGL(k_gc_render<false>, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, R,
+                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
+                           c->pre, c->tailv, c->queries, c->qcount);
     if ((rc = launch_check(ctx, "k_gc_render"))) return rc;
     InferArgs I;
     std::memset(&I, 0, sizeof I);

@@ -665,6 +665,8 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
                       const uint32_t *gb_normal, vf_hit *gi_hits, vf_hit *sun_hits, const vf_queue_cfg *cfg_in) {
     int rc = vf::check_ready(ctx, "vf_trace_gi_batch");
     if (rc) return rc;
+    if (ctx->scene.num_objects != 0)
+        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_trace_gi_batch: the persistent queue traces the world only (the scene has bodies)");
     if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: no scene (vf_set_scene)");
     if (!pass || !gb_pos || !gb_normal || !gi_hits || !sun_hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: null buffer");
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: bad size %dx%d", w, h);
@@ -723,6 +725,8 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
 int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in) {
     int rc = vf::check_ready(ctx, "vf_trace_batch");
     if (rc) return rc;
+    if (ctx->scene.num_objects != 0)
+        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_trace_batch: the persistent queue traces the world only (the scene has bodies)");
     if (!reqs || n_req <= 0 || n_req > kMaxReq)
         return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: 1..%d requests required", kMaxReq);
     if ((rc = check_cfg(ctx, cfg_in, "vf_trace_batch"))) return rc;

@@ -148,9 +148,9 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
 // deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
 // to the first blocker.
-template <bool COUNT>
+template <bool COUNT, bool BODIES = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
-                                                 vf_hit *hits, size_t i, Cnt &c) {
+                                                 vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr) {
     float vis = 0.0f, tb = 0.0f;
     if (p.w < 0.0f) {
         vis = 1.0f;
@@ -162,7 +162,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         } else {
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT>(W, so, sun, leaf, seed, tb, hits, i, c);
+            vis = sun_visibility_t<false, COUNT, BODIES>(W, so, sun, leaf, seed, tb, hits, i, c, B);
         }
     }
     const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
@@ -171,9 +171,9 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
 }
 
 // shadow_trace.comp (shadow_trace.comp:17-34, 87, 96-105) for a 16x16 tile per workgroup.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx, float sy, float sz, const float4 *gbpos,
-                                                const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArgs A, float sx, float sy, float sz,
+                                                const float4 *gbpos, const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
                                                 unsigned long long *cnt) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, TileArgs A, float sx
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     const float sun[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
-    shadow[i] = shadow_texel<COUNT>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c);
+    shadow[i] = shadow_texel<COUNT, BODIES>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -243,6 +243,45 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
     if (camera_special(cam_mat) == SPECIAL) flush_count<COUNT>(c, cnt);   // only the variant whose results stand
 }
 
+// gbuffer.comp over the scene (traceScenePrimary = traceSceneS, common.glsl:787-793: the world, then the
+// bodies closer than its hit; a camera in solid terrain or water as gbuffer.comp:233-266), optionally
+// followed by shadow_trace.comp on the values just written (SHADOW), for a 16x16 tile per workgroup.
+// body (nullable): h.obj per pixel, -1 for the world and the sky.
+template <bool COUNT, bool SHADOW>
+__global__ void __launch_bounds__(256) k_scene(DevWorld W, DevBodies B, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
+                                               float4 *gbpos, uint32_t *gbn, int32_t *body, uint32_t *shadow, vf_hit *shits,
+                                               unsigned long long *cnt) {
+    const uint32_t cam_mat = A.cam_air ? 0u : voxel_mat_at(W, A.pos);
+    int32_t px, py;
+    tile_pixel_of(xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x), A.tiles_x, px, py);
+    if (px >= A.w || py >= A.h) return;
+    Cnt c = {0, 0, 0, 0};
+    float rd[3];
+    primary_dir(A, px, py, rd);
+    Hit h;
+    scene_primary_trace<COUNT>(W, B, A.pos, cam_mat, rd, h, c);
+    const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
+    if (hits) store_hit(hits + i, h, h.hit && h.obj >= 0);
+    float4 p;
+    uint32_t nq = 0u;
+    if (h.hit) {
+        p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
+        float n[3];
+        hit_normal<true>(&B, h, n);
+        nq = pack_normal(n);
+    } else {
+        p = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    }
+    if (gbpos) gbpos[i] = p;
+    if (gbn) gbn[i] = nq;
+    if (body) body[i] = h.hit ? h.obj : -1;
+    if (SHADOW) {
+        const float sun[3] = {sx, sy, sz};
+        shadow[i] = shadow_texel<COUNT, true>(W, p, nq, sun, A.leaf, shits, i, c, &B);
+    }
+    flush_count<COUNT>(c, cnt);
+}
+
 struct GiArgs {
     int32_t w, h, ow, oh, tiles_x, ntiles, half_res, lights_nee;
     uint32_t frame;
@@ -250,9 +289,9 @@ struct GiArgs {
 };
 
 // gi.comp (gi.comp:34-84) + giPathImpl with one bounce (common.glsl:2250-2366).
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, const float4 *gbpos, const uint32_t *gbn,
-                                            float4 *radiance, vf_hit *bounce, unsigned long long *cnt) {
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, DevBodies B, GiArgs A, const float4 *gbpos,
+                                            const uint32_t *gbn, float4 *radiance, vf_hit *bounce, unsigned long long *cnt) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -278,7 +317,7 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, c
     }
     Cnt c = {0, 0, 0, 0};
     Hit bh;
-    bh.hit = 0; bh.t = -1.0f; bh.face = 0; bh.mat = 0; bh.cell[0] = bh.cell[1] = bh.cell[2] = 0;
+    bh.hit = 0; bh.t = -1.0f; bh.face = 0; bh.mat = 0; bh.cell[0] = bh.cell[1] = bh.cell[2] = 0; bh.obj = -1;
     float Lr[3] = {0.0f, 0.0f, 0.0f};
     if (!sky) {
         const size_t fi = (size_t)fy * A.w + fx;
@@ -292,18 +331,18 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, GiArgs A, c
         const float P[3] = {P4.x, P4.y, P4.z};
         const GiPathCfg G = {A.reach, A.lights_nee};
         uint32_t seed = 0u;
-        gi_path<1, false, COUNT>(W, S, G, P, N, rdir, seed, true, Lr, &bh, nullptr, c);
+        gi_path<1, false, COUNT, BODIES>(W, S, G, P, N, rdir, seed, true, Lr, &bh, nullptr, c, &B);
     }
     radiance[oi] = make_float4(Lr[0], Lr[1], Lr[2], sky ? 0.0f : 1.0f);
-    if (bounce) store_hit(bounce + oi, bh);
+    if (bounce) store_hit(bounce + oi, bh, BODIES && bh.hit && bh.obj >= 0);
     flush_count<COUNT>(c, cnt);
 }
 
 // lighting.comp's sun ray (SVGF, lighting.comp:44-84): per pixel seed pixelSeed(pix, frameIdx, 0), a
 // lit texel (N.L > 0) traces sunVisibility (FOLIAGE_LIGHT, stochastic) from P + N*0.03 along
 // sampleCone(sunDir, sunAngle); the raw sample is 1 for sky, 0 for a texel facing away, else 0 or 1.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, TileArgs A, float sx, float sy, float sz, float ang,
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, DevBodies B, TileArgs A, float sx, float sy, float sz, float ang,
                                                   uint32_t frame, const float4 *gbpos, const uint32_t *gbn, float *out,
                                                   vf_hit *hits, unsigned long long *cnt) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
@@ -327,7 +366,7 @@ __global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, TileArgs A, float 
             float sd[3], tf;
             sample_cone(sun, ang, seed, sd);
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
-            v = sun_visibility_t<true, COUNT>(W, so, sd, A.leaf, seed, tf, hits, i, c);
+            v = sun_visibility_t<true, COUNT, BODIES>(W, so, sd, A.leaf, seed, tf, hits, i, c, &B);
             stored = true;
         }
     }
@@ -363,8 +402,9 @@ struct ProbeArgs {
 };
 
 // probe_trace.comp ray source (probes.glsl:155-170) traced with skipMat 5 (kSkipMat).
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_probes(DevWorld W, ProbeArgs A, vf_hit *hits, unsigned long long *cnt) {
+// BODIES (GI_OBJECTS, probe_trace.comp:47): bodies nearer than min(the world hit or the reach, 16).
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_probes(DevWorld W, DevBodies B, ProbeArgs A, vf_hit *hits, unsigned long long *cnt) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const long long idx = (long long)L * 256 + threadIdx.x;
     if (idx >= A.n) return;
@@ -401,7 +441,11 @@ __global__ void __launch_bounds__(256) k_probes(DevWorld W, ProbeArgs A, vf_hit 
     Cnt cc = {0, 0, 0, 0};
     Hit h;
     trace_world<5, COUNT>(W, ro, rd, A.reach, h, cc);
-    store_hit(hits + idx, h);
+    if (BODIES) {
+        h.obj = -1;
+        trace_objects<COUNT>(W, B, ro, rd, h, gmin(h.hit ? h.t : A.reach, 16.0f), false, cc);
+    }
+    store_hit(hits + idx, h, BODIES && h.hit && h.obj >= 0);
     flush_count<COUNT>(cc, cnt);
 }
 
@@ -796,6 +840,77 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
         hipLaunchKernelGGL((k_primary_shadow<COUNT, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
                            hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
 }
+// The scene variant (bodies present): one tile grid for every camera case.
+template <bool COUNT, bool SHADOW>
+void launch_scene(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits, float4 *gbpos, uint32_t *gbn, int32_t *body,
+                  uint32_t *shadow, vf_hit *shits) {
+    A.cam_air = camera_class(ctx, A.pos) == 0 ? 1 : 0;
+    hipLaunchKernelGGL((k_scene<COUNT, SHADOW>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, vf::dev_bodies(ctx), A,
+                       s[0], s[1], s[2], hits, gbpos, gbn, body, shadow, shits, ctx->d_cnt);
+}
+
+PrimArgs prim_args(const vf_pass_ubo *p, int32_t w, int32_t h) {
+    PrimArgs A;
+    for (int a = 0; a < 3; ++a) {
+        A.pos[a] = p->cam_pos[a]; A.right[a] = p->cam_right[a]; A.up[a] = p->cam_up[a]; A.fwd[a] = p->cam_fwd[a];
+    }
+    A.thf = p->tan_half_fov;
+    A.jit[0] = p->jitter[0];
+    A.jit[1] = p->jitter[1];
+    A.w = w; A.h = h;
+    A.tiles_x = (w + 15) / 16;
+    A.ntiles = A.tiles_x * ((h + 15) / 16);
+    A.cam_air = 0;
+    A.leaf = p->look_p[3];
+    return A;
+}
+
+__global__ void k_fill_i32(int32_t *p, long long n, int32_t v) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// P::Gbuf (+ P::ShTrace) with every output: the world-only kernels when there are no bodies, else the
+// scene kernel.
+int trace_gbuffer(vf_ctx *ctx, const char *what, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos,
+                  uint32_t *gb_normal, int32_t *body, uint32_t *shadow, vf_hit *shadow_hits) {
+    int rc = check_ready(ctx, what);
+    if (rc) return rc;
+    if (shadow && !ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "%s: no scene (vf_set_scene)", what);
+    if (!p) return vf::fail(ctx, VF_E_INVALID, "%s: null pass", what);
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "%s: bad size %dx%d", what, w, h);
+    if ((rc = vf::check_bodies(ctx, what))) return rc;
+    const PrimArgs A = prim_args(p, w, h);
+    const float *s = ctx->S.sun_dir;
+    float4 *gp = reinterpret_cast<float4 *>(gb_pos);
+    if ((rc = begin_timed(ctx))) return rc;
+    if (vf::dev_bodies(ctx).n > 0) {
+        if (shadow) {
+            if (ctx->counting) launch_scene<true, true>(ctx, A, s, hits, gp, gb_normal, body, shadow, shadow_hits);
+            else launch_scene<false, true>(ctx, A, s, hits, gp, gb_normal, body, shadow, shadow_hits);
+        } else {
+            if (ctx->counting) launch_scene<true, false>(ctx, A, s, hits, gp, gb_normal, body, nullptr, nullptr);
+            else launch_scene<false, false>(ctx, A, s, hits, gp, gb_normal, body, nullptr, nullptr);
+        }
+        if ((rc = launch_check(ctx, "k_scene"))) return rc;
+        return end_timed(ctx);
+    }
+    if (shadow) {
+        if (ctx->counting) launch_primary_shadow<true>(ctx, A, s, hits, gp, gb_normal, shadow, shadow_hits);
+        else launch_primary_shadow<false>(ctx, A, s, hits, gp, gb_normal, shadow, shadow_hits);
+        if ((rc = launch_check(ctx, "k_primary_shadow"))) return rc;
+    } else {
+        if (ctx->counting) launch_primary<true>(ctx, A, hits, gp, gb_normal);
+        else launch_primary<false>(ctx, A, hits, gp, gb_normal);
+        if ((rc = launch_check(ctx, "k_primary"))) return rc;
+    }
+    if (body) {
+        const long long n = (long long)w * h;
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, body, n, -1);
+        if ((rc = launch_check(ctx, "body index fill"))) return rc;
+    }
+    return end_timed(ctx);
+}
 }  // namespace
 
 extern "C" {
@@ -852,6 +967,7 @@ void vf_ctx_destroy(vf_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_world(ctx);
     vf::free_queue(ctx);
+    vf::free_bodies(ctx);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1185,9 +1301,7 @@ int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *s) {
             return vf::fail(ctx, VF_E_INVALID, "vf_set_scene: world_win.w (ring mode) differs from the uploaded world");
         if (ctx->W.ring) { ctx->W.minb[0] = s->world_win[0]; ctx->W.minb[2] = s->world_win[2]; }
     }
-    if (s->num_objects != 0)
-        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_set_scene: num_objects = %d: body traversal is not part of this library",
-                        s->num_objects);
+    if (s->num_objects < 0) return vf::fail(ctx, VF_E_INVALID, "vf_set_scene: num_objects = %d", s->num_objects);
     ctx->scene = *s;
     SkyParams &S = ctx->S;
     for (int a = 0; a < 3; ++a) {
@@ -1236,28 +1350,14 @@ int vf_last_kernel_ms(vf_ctx *ctx, float *ms) {
 }
 
 int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos, uint32_t *gb_normal) {
-    int rc = check_ready(ctx, "vf_trace_primary");
-    if (rc) return rc;
-    if (!p || !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary: null pass or hits");
-    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary: bad size %dx%d", w, h);
-    PrimArgs A;
-    for (int a = 0; a < 3; ++a) {
-        A.pos[a] = p->cam_pos[a]; A.right[a] = p->cam_right[a]; A.up[a] = p->cam_up[a]; A.fwd[a] = p->cam_fwd[a];
-    }
-    A.thf = p->tan_half_fov;
-    A.jit[0] = p->jitter[0];
-    A.jit[1] = p->jitter[1];
-    A.w = w; A.h = h;
-    A.tiles_x = (w + 15) / 16;
-    A.ntiles = A.tiles_x * ((h + 15) / 16);
-    A.leaf = p->look_p[3];
-    if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        launch_primary<true>(ctx, A, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal);
-    else
-        launch_primary<false>(ctx, A, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal);
-    if ((rc = launch_check(ctx, "k_primary"))) return rc;
-    return end_timed(ctx);
+    if (ctx && !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary: null hits");
+    return trace_gbuffer(ctx, "vf_trace_primary", p, w, h, hits, gb_pos, gb_normal, nullptr, nullptr, nullptr);
+}
+
+int vf_trace_gbuffer(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos, uint32_t *gb_normal,
+                     int32_t *body, uint32_t *shadow, vf_hit *shadow_hits) {
+    if (ctx && !hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gbuffer: null hits");
+    return trace_gbuffer(ctx, "vf_trace_gbuffer", p, w, h, hits, gb_pos, gb_normal, body, shadow, shadow_hits);
 }
 
 int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal,
@@ -1267,47 +1367,35 @@ int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, con
     if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: no scene (vf_set_scene)");
     if (!p || !gb_pos || !gb_normal || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: null pass or buffer");
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_shadow: bad size %dx%d", w, h);
+    if ((rc = vf::check_bodies(ctx, "vf_trace_shadow"))) return rc;
     TileArgs A{w, h, (w + 15) / 16, 0, p->look_p[3]};
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     const float *s = ctx->S.sun_dir;
+    const DevBodies B = vf::dev_bodies(ctx);
+    const float4 *gp = reinterpret_cast<const float4 *>(gb_pos);
     if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
-    else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    if (B.n > 0) {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_shadow<true, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
+                               gp, gb_normal, shadow, hits, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_shadow<false, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
+                               gp, gb_normal, shadow, hits, ctx->d_cnt);
+    } else if (ctx->counting) {
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
+                           gb_normal, shadow, hits, ctx->d_cnt);
+    } else {
+        hipLaunchKernelGGL((k_shadow<false, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
+                           gb_normal, shadow, hits, ctx->d_cnt);
+    }
     if ((rc = launch_check(ctx, "k_shadow"))) return rc;
     return end_timed(ctx);
 }
 
 int vf_trace_primary_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, vf_hit *hits, float *gb_pos,
                             uint32_t *gb_normal, uint32_t *shadow, vf_hit *shadow_hits) {
-    int rc = check_ready(ctx, "vf_trace_primary_shadow");
-    if (rc) return rc;
-    if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: no scene (vf_set_scene)");
-    if (!p || !hits || !shadow) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: null pass, hits or shadow");
-    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30))
-        return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: bad size %dx%d", w, h);
-    PrimArgs A;
-    for (int a = 0; a < 3; ++a) {
-        A.pos[a] = p->cam_pos[a]; A.right[a] = p->cam_right[a]; A.up[a] = p->cam_up[a]; A.fwd[a] = p->cam_fwd[a];
-    }
-    A.thf = p->tan_half_fov;
-    A.jit[0] = p->jitter[0];
-    A.jit[1] = p->jitter[1];
-    A.w = w; A.h = h;
-    A.tiles_x = (w + 15) / 16;
-    A.ntiles = A.tiles_x * ((h + 15) / 16);
-    A.leaf = p->look_p[3];
-    const float *s = ctx->S.sun_dir;
-    if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        launch_primary_shadow<true>(ctx, A, s, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits);
-    else
-        launch_primary_shadow<false>(ctx, A, s, hits, reinterpret_cast<float4 *>(gb_pos), gb_normal, shadow, shadow_hits);
-    if ((rc = launch_check(ctx, "k_primary_shadow"))) return rc;
-    return end_timed(ctx);
+    if (ctx && (!hits || !shadow)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_primary_shadow: null hits or shadow");
+    return trace_gbuffer(ctx, "vf_trace_primary_shadow", p, w, h, hits, gb_pos, gb_normal, nullptr, shadow, shadow_hits);
 }
 
 int vf_trace_sun_cone(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, const float *gb_pos, const uint32_t *gb_normal,
@@ -1321,13 +1409,24 @@ int vf_trace_sun_cone(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, c
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     const float *s = ctx->S.sun_dir;
     const float ang = ctx->scene.sun_angle;
+    if ((rc = vf::check_bodies(ctx, "vf_trace_sun_cone"))) return rc;
+    const DevBodies B = vf::dev_bodies(ctx);
+    const float4 *gp = reinterpret_cast<const float4 *>(gb_pos);
     if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        hipLaunchKernelGGL(k_sun_cone<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], ang,
-                           p->frame_idx, reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
-    else
-        hipLaunchKernelGGL(k_sun_cone<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2], ang,
-                           p->frame_idx, reinterpret_cast<const float4 *>(gb_pos), gb_normal, shadow, hits, ctx->d_cnt);
+    if (B.n > 0) {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_sun_cone<true, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
+                               ang, p->frame_idx, gp, gb_normal, shadow, hits, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_sun_cone<false, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
+                               ang, p->frame_idx, gp, gb_normal, shadow, hits, ctx->d_cnt);
+    } else if (ctx->counting) {
+        hipLaunchKernelGGL((k_sun_cone<true, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], ang,
+                           p->frame_idx, gp, gb_normal, shadow, hits, ctx->d_cnt);
+    } else {
+        hipLaunchKernelGGL((k_sun_cone<false, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
+                           ang, p->frame_idx, gp, gb_normal, shadow, hits, ctx->d_cnt);
+    }
     if ((rc = launch_check(ctx, "k_sun_cone"))) return rc;
     return end_timed(ctx);
 }
@@ -1350,15 +1449,25 @@ int vf_trace_gi(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, int32_t
     A.lights_nee = lights_nee ? 1 : 0;
     A.frame = p->frame_idx;
     A.reach = gi_reach;
+    if ((rc = vf::check_bodies(ctx, "vf_trace_gi"))) return rc;
+    const DevBodies B = vf::dev_bodies(ctx);
+    const float4 *gp = reinterpret_cast<const float4 *>(gb_pos);
+    float4 *rad = reinterpret_cast<float4 *>(radiance);
     if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        hipLaunchKernelGGL(k_gi<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(radiance), bounce,
-                           ctx->d_cnt);
-    else
-        hipLaunchKernelGGL(k_gi<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(radiance), bounce,
-                           ctx->d_cnt);
+    if (B.n > 0) {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_gi<true, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A, gp, gb_normal,
+                               rad, bounce, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_gi<false, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A, gp, gb_normal,
+                               rad, bounce, ctx->d_cnt);
+    } else if (ctx->counting) {
+        hipLaunchKernelGGL((k_gi<true, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A, gp, gb_normal, rad,
+                           bounce, ctx->d_cnt);
+    } else {
+        hipLaunchKernelGGL((k_gi<false, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A, gp, gb_normal,
+                           rad, bounce, ctx->d_cnt);
+    }
     if ((rc = launch_check(ctx, "k_gi"))) return rc;
     return end_timed(ctx);
 }
@@ -1402,11 +1511,20 @@ int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int
     A.reach = reach;
     A.n = (long long)dims[0] * dims[1] * dims[2] * n_rays;
     if (A.n > (1ll << 31) * 64) return vf::fail(ctx, VF_E_INVALID, "vf_trace_probes: too many rays");
+    if ((rc = vf::check_bodies(ctx, "vf_trace_probes"))) return rc;
+    const DevBodies B = vf::dev_bodies(ctx);
+    const unsigned nb = blocks_for(A.n);
     if ((rc = begin_timed(ctx))) return rc;
-    if (ctx->counting)
-        hipLaunchKernelGGL(k_probes<true>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, A, hits, ctx->d_cnt);
-    else
-        hipLaunchKernelGGL(k_probes<false>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, A, hits, ctx->d_cnt);
+    if (B.n > 0) {
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_probes<true, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, B, A, hits, ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_probes<false, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, B, A, hits, ctx->d_cnt);
+    } else if (ctx->counting) {
+        hipLaunchKernelGGL((k_probes<true, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, B, A, hits, ctx->d_cnt);
+    } else {
+        hipLaunchKernelGGL((k_probes<false, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, B, A, hits, ctx->d_cnt);
+    }
     if ((rc = launch_check(ctx, "k_probes"))) return rc;
     return end_timed(ctx);
 }

@@ -13,13 +13,14 @@ import os
 
 import numpy as np
 
-from .abi import (GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams, GiCacheStats, Hit,
-                  HIT_DTYPE, Material, PassUbo, QueueCfg, Ray, RAY_DTYPE, Request, SceneUbo, TuneBatchArgs, TuneGiArgs,
-                  TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH, VF_TUNE_GI_BATCH, WorldDesc)
+from .abi import (BVH_NODE_DTYPE, BvhNode, GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams,
+                  GiCacheStats, Hit, HIT_DTYPE, Material, OBJ_HEADER_DTYPE, ObjHeader, PassUbo, QueueCfg, Ray, RAY_DTYPE,
+                  Request, SceneUbo, TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH,
+                  VF_TUNE_GI_BATCH, WorldDesc)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
-           "RAY_DTYPE", "GI_RECORD_DTYPE", "GiCache", "fly_camera", "sun_dir", "make_pass", "default_scene",
-           "library_path"]
+           "RAY_DTYPE", "GI_RECORD_DTYPE", "OBJ_HEADER_DTYPE", "BVH_NODE_DTYPE", "GiCache", "fly_camera", "sun_dir",
+           "make_pass", "default_scene", "library_path"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -58,6 +59,12 @@ _SIGS = {
     "vf_read_skytop": (C.c_int, [_P, _P]),
     "vf_set_scene": (C.c_int, [_P, C.POINTER(SceneUbo)]),
     "vf_trace_primary": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P]),
+    "vf_trace_gbuffer": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P]),
+    "vf_objects_upload": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int64]),
+    "vf_objects_set_headers": (C.c_int, [_P, _P, C.c_int32]),
+    "vf_objects_set_bvh": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32]),
+    "vf_objects_update_grid": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "vf_objects_read_bvh": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), _P, C.c_int32, C.POINTER(C.c_int32)]),
     "vf_trace_shadow": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P]),
     "vf_trace_sun_cone": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P]),
     "vf_trace_primary_shadow": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
@@ -376,10 +383,74 @@ class Context:
     def set_scene(self, scene):
         self._ck(lib().vf_set_scene(self._h, C.byref(scene)))
 
+    def upload_arrays(self, arrays):
+        """vf_world_upload from numpy arrays (GenWorld.arrays() layout; masks / coarse pyramid optional)."""
+        wb = tuple(int(x) for x in arrays["world_bricks"])
+        grid = np.ascontiguousarray(arrays["grid"], dtype=np.uint32)
+        atlas = np.ascontiguousarray(arrays["atlas"], dtype=np.uint8)
+        pal = np.ascontiguousarray(arrays["palette"], dtype=np.float32).reshape(256, 12)
+        d = WorldDesc()
+        d.world_bricks[:] = wb
+        d.world_win[:] = tuple(int(x) for x in arrays["world_win"])
+        d.grid = grid.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.atlas = atlas.ctypes.data_as(C.POINTER(C.c_uint8)) if atlas.size else None
+        d.n_slots = atlas.size // 512
+        keep = [grid, atlas, pal]
+        for k, typ in (("masks", C.c_uint32), ("coarse1", C.c_uint8), ("coarse2", C.c_uint8)):
+            a = arrays.get(k)
+            if a is not None and np.asarray(a).size:
+                a = np.ascontiguousarray(a)
+                keep.append(a)
+                setattr(d, k, a.ctypes.data_as(C.POINTER(typ)))
+        d.palette = pal.ctypes.data_as(C.POINTER(Material))
+        self._ck(lib().vf_world_upload(self._h, C.byref(d)))
+        self.world_bricks = wb
+        del keep
+
+    # bodies (GpuWorld::init_objects / update_objects / sync_objects)
+    def objects_upload(self, headers, grid):
+        h = np.ascontiguousarray(headers, dtype=OBJ_HEADER_DTYPE)
+        g = np.ascontiguousarray(grid, dtype=np.uint32)
+        self._ck(lib().vf_objects_upload(self._h, h.ctypes.data if h.size else None, int(h.size),
+                                         g.ctypes.data if g.size else None, int(g.size)))
+
+    def objects_set_headers(self, headers):
+        h = np.ascontiguousarray(headers, dtype=OBJ_HEADER_DTYPE)
+        self._ck(lib().vf_objects_set_headers(self._h, h.ctypes.data if h.size else None, int(h.size)))
+
+    def objects_set_bvh(self, nodes, prims):
+        nd = np.ascontiguousarray(nodes, dtype=BVH_NODE_DTYPE)
+        pr = np.ascontiguousarray(prims, dtype=np.uint32)
+        self._ck(lib().vf_objects_set_bvh(self._h, nd.ctypes.data if nd.size else None, int(nd.size),
+                                          pr.ctypes.data if pr.size else None, int(pr.size)))
+
+    def objects_update_grid(self, idx, val):
+        i = np.ascontiguousarray(idx, dtype=np.uint32)
+        v = np.ascontiguousarray(val, dtype=np.uint32)
+        if i.size != v.size:
+            raise ValueError("objects_update_grid: mismatched arrays")
+        self._ck(lib().vf_objects_update_grid(self._h, i.ctypes.data if i.size else None,
+                                              v.ctypes.data if v.size else None, int(i.size)))
+
+    def objects_read_bvh(self):
+        nn, npr = C.c_int32(), C.c_int32()
+        self._ck(lib().vf_objects_read_bvh(self._h, None, 0, C.byref(nn), None, 0, C.byref(npr)))
+        nodes = np.zeros(nn.value, BVH_NODE_DTYPE)
+        prims = np.zeros(npr.value, np.uint32)
+        self._ck(lib().vf_objects_read_bvh(self._h, nodes.ctypes.data if nodes.size else None, nn.value, C.byref(nn),
+                                           prims.ctypes.data if prims.size else None, npr.value, C.byref(npr)))
+        return nodes, prims
+
     # passes (device buffers: torch tensors)
     def trace_primary(self, pass_ubo, width, height, hits, gb_pos=None, gb_normal=None):
         self._ck(lib().vf_trace_primary(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
                                         _ptr(gb_pos), _ptr(gb_normal)))
+
+    def trace_gbuffer(self, pass_ubo, width, height, hits, gb_pos=None, gb_normal=None, body=None, shadow_raw=None,
+                      shadow_hits=None):
+        """P::Gbuf over the scene (+ P::ShTrace when shadow_raw is given) with the body index per pixel."""
+        self._ck(lib().vf_trace_gbuffer(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits), _ptr(gb_pos),
+                                        _ptr(gb_normal), _ptr(body), _ptr(shadow_raw), _ptr(shadow_hits)))
 
     def trace_shadow(self, width, height, gb_pos, gb_normal, shadow_raw, hits=None, pass_ubo=None):
         """shadow_trace.comp; the pass supplies look_p[3] (r.foliage.light, the reference default 0.6)."""

@@ -75,7 +75,18 @@ class Ray(C.Structure):
 
 class Hit(C.Structure):
     _fields_ = [("cell", i3), ("t", C.c_float), ("mat", C.c_uint8), ("face", C.c_uint8),
-                ("hit", C.c_uint8), ("pad", C.c_uint8)]
+                ("hit", C.c_uint8), ("body", C.c_uint8)]
+
+
+class ObjHeader(C.Structure):
+    """std430 ObjHeader, 240 B (src/gfx/gpu_world.rs:34-47)."""
+    _fields_ = [("world_to_obj", C.c_float * 16), ("obj_to_world", C.c_float * 16),
+                ("prev_obj_to_world", C.c_float * 16), ("dims_bricks", i4), ("aabb_min", f4), ("aabb_max", f4)]
+
+
+class BvhNode(C.Structure):
+    """BvhNode, 32 B (shaders/common.glsl:77-81)."""
+    _fields_ = [("min", f3), ("first", C.c_uint32), ("max", f3), ("count", C.c_uint32)]
 
 
 class WorldDesc(C.Structure):
@@ -144,7 +155,12 @@ VF_TUNE_GI_BATCH, VF_TUNE_BATCH = 0, 1
 # numpy views of the 24-byte ray and 20-byte hit records
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("d", "<f4", (3,))])
 HIT_DTYPE = np.dtype([("cell", "<i4", (3,)), ("t", "<f4"), ("mat", "u1"), ("face", "u1"), ("hit", "u1"),
-                      ("pad", "u1")])
+                      ("body", "u1")])
+# 240-byte body header and 32-byte BVH node
+OBJ_HEADER_DTYPE = np.dtype([("world_to_obj", "<f4", (16,)), ("obj_to_world", "<f4", (16,)),
+                             ("prev_obj_to_world", "<f4", (16,)), ("dims_bricks", "<i4", (4,)),
+                             ("aabb_min", "<f4", (4,)), ("aabb_max", "<f4", (4,))])
+BVH_NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("first", "<u4"), ("max", "<f4", (3,)), ("count", "<u4")])
 # 64-byte query / training record (vf_gi_record)
 GI_RECORD_DTYPE = np.dtype([("pos", "<f4", (3,)), ("tag", "<u4"), ("n", "<f4", (3,)), ("pad0", "<f4"),
                             ("view", "<f4", (3,)), ("pad1", "<f4"), ("rgb", "<f4", (3,)), ("pad2", "<f4")])
@@ -170,3 +186,6 @@ assert C.sizeof(Ray) == 24 and RAY_DTYPE.itemsize == 24
 assert C.sizeof(Hit) == 20 and HIT_DTYPE.itemsize == 20
 assert GI_RECORD_DTYPE.itemsize == 64
 assert C.sizeof(GiCacheParams) == 64 and C.sizeof(GiCacheStats) == 40
+
+assert C.sizeof(ObjHeader) == 240 and OBJ_HEADER_DTYPE.itemsize == 240
+assert C.sizeof(BvhNode) == 32 and BVH_NODE_DTYPE.itemsize == 32
