@@ -33,7 +33,7 @@ enum {
     VF_E_DEVICE = -2,      /* HIP runtime error */
     VF_E_NO_WORLD = -3,    /* a trace before vf_world_upload */
     VF_E_OOM = -4,         /* device allocation failed */
-    VF_E_UNSUPPORTED = -5  /* a feature outside this library (e.g. bodies) */
+    VF_E_UNSUPPORTED = -5  /* a feature outside this library (e.g. bodies in the persistent-queue batches) */
 };
 
 /* ---- layouts shared with the reference host ------------------------------------------------ */
@@ -220,8 +220,8 @@ int vf_objects_read_bvh(vf_ctx *ctx, vf_bvh_node *nodes, int32_t max_nodes, int3
 int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
 
 /* ---- traced passes ------------------------------------------------------------------------- */
-/* P::Gbuf (record.rs:327; gbuffer.comp:189-266, 470-480): one pinhole primary ray per pixel, world
- * only.  A camera inside solid terrain starts each ray where it leaves the solid (exitSolid,
+/* P::Gbuf (record.rs:327; gbuffer.comp:189-266, 470-480): one pinhole primary ray per pixel over the
+ * scene (the world, and the bodies when the scene's num_objects is set).  A camera inside solid terrain starts each ray where it leaves the solid (exitSolid,
  * common.glsl:990-1016; t counts from the camera, rays that only leave the world are misses); a
  * camera in water traces with skipMat 5.  Writes hits [w*h], and when non-NULL gbPos (RGBA32F: P, t;
  * sky (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5, a=255; sky 0).  Row 0 is the image bottom.
@@ -291,6 +291,7 @@ int vf_queue_defaults(vf_queue_cfg *cfg);
  * sunDir, no cap) per GI hit that is not water and faces the sun.  Both go through the persistent
  * queue kernel.  gi_hits / sun_hits: one record per pixel (miss where no ray).  Not a reference
  * entry point: it batches what P::Gi traces, for incoherent-ray throughput. */
+/* The persistent-queue entry points trace the world only: a scene with bodies is VF_E_UNSUPPORTED. */
 int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, float reach,
                       const float *d_gb_pos, const uint32_t *d_gb_normal, vf_hit *d_gi_hits, vf_hit *d_sun_hits,
                       const vf_queue_cfg *cfg);
