@@ -78,7 +78,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="minimum CPU-baseline time (repeats the frame)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the fixed-world, GI-batch and uneven-batch lines")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the open-forest, fixed-world, GI-batch, learned-cache and uneven-batch lines")
     ap.add_argument("--no-graph", action="store_true", help="launch the passes eagerly instead of replaying a graph")
     ap.add_argument("--cache-frames", type=int, default=256, help="online-training frames before the GI cache is judged")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -257,8 +258,8 @@ def main():
                                               hits, args.cpu_seconds)
     for g in (g_step, g_two, g_prim, g_shad):
         g.close()
-    result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
     if not args.no_extra:
+        result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
         result["other_configs"] = other_configs(ctx, args)
     ctx.close()
     print(json.dumps(result), flush=True)
