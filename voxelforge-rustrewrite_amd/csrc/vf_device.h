@@ -91,9 +91,17 @@ __device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, con
     hi = (lz & 1) ? q.w : q.y;
 }
 
+// Per-lane copy of the current brick's 512-bit mask for the LDS variant of the voxel DDA: word k of the
+// lane with thread id t at vf_vmask[k * 256 + t] (consecutive lanes in consecutive banks).  Only
+// kernels of 256 threads that instantiate brick_voxels<., true> reference it (16 KB per workgroup).
+static __shared__ uint32_t vf_vmask[16 * 256];
+
 // brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
 // arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
-template <bool COUNT>
+// LDSM: the mask goes to the lane's LDS slots and a step along z reads the new 64-bit layer from
+// there (one LDS read instead of a 16-to-2 word select in registers, which a wave executes on almost
+// every step since some lane steps along z).
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
                                              const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c,
@@ -101,6 +109,13 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     const uint32_t slot = g - 256u;
     const uint4 *mp = (skip == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
     const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
+    uint32_t *sm = vf_vmask + threadIdx.x;
+    if (LDSM) {
+        sm[0 * 256] = m0.x; sm[1 * 256] = m0.y; sm[2 * 256] = m0.z; sm[3 * 256] = m0.w;
+        sm[4 * 256] = m1.x; sm[5 * 256] = m1.y; sm[6 * 256] = m1.z; sm[7 * 256] = m1.w;
+        sm[8 * 256] = m2.x; sm[9 * 256] = m2.y; sm[10 * 256] = m2.z; sm[11 * 256] = m2.w;
+        sm[12 * 256] = m3.x; sm[13 * 256] = m3.y; sm[14 * 256] = m3.z; sm[15 * 256] = m3.w;
+    }
     int32_t bmin[3], vc[3];
     float tM[3], tD[3];
     const float e = t + 1e-3f;
@@ -114,7 +129,12 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     }
     int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
     uint32_t wlo, whi;
-    mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+    if (LDSM) {
+        wlo = sm[(lz * 2) * 256];
+        whi = sm[(lz * 2 + 1) * 256];
+    } else {
+        mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+    }
     float tv = t;
     int32_t f2 = face;
     bool hit = false;
@@ -142,7 +162,14 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         ly += a1 ? s1 : 0;
         lz += a2 ? s2 : 0;
         if (((uint32_t)lx | (uint32_t)ly | (uint32_t)lz) > 7u) break;
-        if (a2) mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+        if (a2) {
+            if (LDSM) {
+                wlo = sm[(lz * 2) * 256];
+                whi = sm[(lz * 2 + 1) * 256];
+            } else {
+                mask_layer(m0, m1, m2, m3, lz, wlo, whi);
+            }
+        }
     }
     if (hit) {
         h.hit = 1;
@@ -157,7 +184,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
 
 // traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5), or -1 to
 // take it from skip_rt (one inlined copy serving both).
-template <int SKIP, bool COUNT>
+template <int SKIP, bool COUNT, bool LDSM = false>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
                                             Cnt &c, const int skip_rt = 0) {
     const int sk = SKIP >= 0 ? SKIP : skip_rt;
@@ -216,7 +243,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             }
         }
         if (g >= 256u) {
-            if (brick_voxels<COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
+            if (brick_voxels<COUNT, LDSM>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
         } else if (g != 0u && g != (uint32_t)sk) {
             const float e = t + 1e-3f;
             h.hit = 1; h.t = t; h.face = face; h.mat = g;
@@ -612,7 +639,7 @@ __device__ float exit_solid(const DevWorld &W, const float ro[3], const float rd
 // The primary ray of gbuffer.comp (233-266): a camera inside solid terrain (cam_mat = voxelMatAt(camPos)
 // neither air nor water) starts each ray where it leaves that solid (tSkip, added to h.t), and a ray that
 // only leaves it out of the world is a miss; a camera in water traces with skipMat 5 (WATER_CLEAR).
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ void primary_trace(const DevWorld &W, const float cam[3], uint32_t cam_mat,
                                               const float rd[3], Hit &h, Cnt &c) {
     float ro[3] = {cam[0], cam[1], cam[2]};
@@ -625,7 +652,7 @@ __device__ __forceinline__ void primary_trace(const DevWorld &W, const float cam
 #pragma unroll
         for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
     }
-    trace_world<-1, COUNT>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
+    trace_world<-1, COUNT, LDSM>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
     h.t = h.t + tskip;
     if (void_ray) h.hit = 0;
 }

@@ -70,8 +70,8 @@ __device__ __forceinline__ void primary_dir(const PrimArgs &A, int32_t px, int32
 template <bool COUNT, bool SPECIAL>
 __device__ __forceinline__ void primary_hit(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, const float rd[3],
                                             Hit &h, Cnt &c) {
-    if (SPECIAL) primary_trace<COUNT>(W, A.pos, cam_mat, rd, h, c);
-    else trace_world<0, COUNT>(W, A.pos, rd, 1e30f, h, c);
+    if (SPECIAL) primary_trace<COUNT, true>(W, A.pos, cam_mat, rd, h, c);
+    else trace_world<0, COUNT, true>(W, A.pos, rd, 1e30f, h, c);
 }
 
 // Tile L's pixel for this thread (8x8 per wave, 16x16 per workgroup).
@@ -148,7 +148,7 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
 // deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
 // to the first blocker.
-template <bool COUNT, bool BODIES = false>
+template <bool COUNT, bool BODIES = false, bool LDSM = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr) {
     float vis = 0.0f, tb = 0.0f;
@@ -162,7 +162,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         } else {
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT, BODIES>(W, so, sun, leaf, seed, tb, hits, i, c, B);
+            vis = sun_visibility_t<false, COUNT, BODIES, LDSM>(W, so, sun, leaf, seed, tb, hits, i, c, B);
         }
     }
     const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     const float sun[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
-    shadow[i] = shadow_texel<COUNT, BODIES>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B);
+    shadow[i] = shadow_texel<COUNT, BODIES, true>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -216,7 +216,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT>(W, p, nq, sun, A.leaf, shits, i, c);
+    shadow[i] = shadow_texel<COUNT, false, true>(W, p, nq, sun, A.leaf, shits, i, c);
 }
 
 template <bool COUNT, bool SPECIAL>
