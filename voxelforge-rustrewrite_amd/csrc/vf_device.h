@@ -147,11 +147,13 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     const int32_t s0 = st[0], s1 = st[1], s2 = st[2];
     for (int j = 0; j < 26; ++j) {
         if (COUNT) { c.voxel++; c.bytes += 1; }
-        const uint32_t wv = (ly & 4) ? whi : wlo;
-        if ((wv >> (((ly & 3) << 3) | lx)) & 1u) { hit = true; break; }
+        const uint32_t idx = ((uint32_t)ly << 3) | (uint32_t)lx;   // bit of (lx, ly) in the 64-bit layer
+        const uint64_t layer = ((uint64_t)whi << 32) | (uint64_t)wlo;
+        if ((layer >> idx) & 1ull) { hit = true; break; }
+        const bool lt_yz = tMy < tMz;
         const bool a0 = tMx < tMy && tMx < tMz;
-        const bool a1 = !a0 && tMy < tMz;
-        const bool a2 = !a0 && !a1;
+        const bool a1 = !a0 && lt_yz;
+        const bool a2 = !a0 && !lt_yz;
         tv = a0 ? tMx : (a1 ? tMy : tMz);
         f2 = a0 ? fA0 : (a1 ? fA1 : fA2);
         const float nx = tMx + tDx, ny = tMy + tDy, nz = tMz + tDz;
