@@ -211,7 +211,7 @@ __device__ __forceinline__ void add_bodies(const DevWorld &W, const DevBodies &B
 // a camera inside solid terrain starts the ray where it leaves the solid, the bodies are traced from the
 // same origin (closer than the world hit), t counts from the camera, and a ray that only leaves the solid
 // out of the world is a miss; a camera in water traces the world with skipMat 5.
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ void scene_primary_trace(const DevWorld &W, const DevBodies &B, const float cam[3],
                                                     uint32_t cam_mat, const float rd[3], Hit &h, Cnt &c) {
     float ro[3] = {cam[0], cam[1], cam[2]};
@@ -224,7 +224,7 @@ __device__ __forceinline__ void scene_primary_trace(const DevWorld &W, const Dev
 #pragma unroll
         for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
     }
-    trace_world<-1, COUNT>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
+    trace_world<-1, COUNT, LDSM>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
     add_bodies<COUNT>(W, B, ro, rd, 1e30f, h, c);
     h.t = h.t + tskip;
     if (void_ray) h.hit = 0;

@@ -472,7 +472,7 @@ __device__ __forceinline__ int32_t ray_begin(const DevWorld &W, const float ro[3
     return RAY_RUNNING;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ int32_t ray_outer(const DevWorld &W, RayState &r, Hit &h, Cnt &c, const int skip) {
     if (r.iter >= W.max_outer) return RAY_MISS;
     r.iter++;
@@ -497,7 +497,7 @@ __device__ __forceinline__ int32_t ray_outer(const DevWorld &W, RayState &r, Hit
         }
     }
     if (g >= 256u) {
-        if (brick_voxels<COUNT>(W, r.ro, r.sd, r.inv, st, up, faceA, r.bc, g, r.t, r.face, h, c, skip)) return RAY_HIT;
+        if (brick_voxels<COUNT, LDSM>(W, r.ro, r.sd, r.inv, st, up, faceA, r.bc, g, r.t, r.face, h, c, skip)) return RAY_HIT;
     } else if (g != 0u && g != (uint32_t)skip) {
         const float e = r.t + 1e-3f;
         h.hit = 1; h.t = r.t; h.face = r.face; h.mat = g;

@@ -124,7 +124,7 @@ struct GiPathCfg {
 // bounce-1 hit is a body (a moving surface is never answered by a cache, common.glsl:2327, 2283): that
 // path is traced to its end.  BODIES (GI_OBJECTS, common.glsl:2267-2271): a bounce ray also meets the
 // bodies nearer than min(its world hit or reach, 16), a sun ray that missed the world any body.
-template <int LAST_STAGE, bool TAIL, bool COUNT, bool BODIES = false>
+template <int LAST_STAGE, bool TAIL, bool COUNT, bool BODIES = false, bool LDSM = false>
 __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &G, const float P[3], const float N[3],
                         const float dir0[3], uint32_t &seed, bool trace_on, float L[3], Hit *bounce, GiTail *tail,
                         Cnt &c, const DevBodies *B = nullptr) {
@@ -141,7 +141,7 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
     for (int it = 0; it < 6; ++it) {
         const bool sunRay = (stage & 1) == 1;
         Hit h;
-        trace_world<0, COUNT>(W, ro, rdir, sunRay ? 1e30f : reach, h, c);
+        trace_world<0, COUNT, LDSM>(W, ro, rdir, sunRay ? 1e30f : reach, h, c);
         if (BODIES) {
             h.obj = -1;
             if (!(sunRay && h.hit))

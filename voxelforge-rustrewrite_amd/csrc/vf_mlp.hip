@@ -574,7 +574,7 @@ __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, false, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
+            gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -636,7 +636,7 @@ __global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, DevB
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, true, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
+            gi_path<3, true, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Dev
     cosine_hemisphere(N, seed, dir);
     Cnt c = {0, 0, 0, 0};
     if (A.terminate == 0) {
-        gi_path<3, false, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
+        gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
         float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
         normalize3(v);
         r[0] = make_float4(P[0], P[1], P[2], __uint_as_float(1u));
@@ -709,14 +709,14 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Dev
         return;
     }
     GiTail t;
-    gi_path<3, true, false, BODIES>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
+    gi_path<3, true, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
     if (!t.valid) {
         r[0] = z; r[1] = z; r[2] = z; r[3] = z;
         return;
     }
     float d2[3], L2[3];
     cosine_hemisphere(t.n, seed, d2);
-    gi_path<1, false, false, BODIES>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
+    gi_path<1, false, false, BODIES, true>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
     r[0] = make_float4(t.p[0], t.p[1], t.p[2], __uint_as_float(1u));
     r[1] = make_float4(t.n[0], t.n[1], t.n[2], 0.0f);
     r[2] = make_float4(-t.dir[0], -t.dir[1], -t.dir[2], 0.0f);

@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
             idle = __ballot(!active && !finished);
         }
         if (active) {
-            const int32_t st = ray_outer<COUNT>(W, r, h, c, 0);
+            const int32_t st = ray_outer<COUNT, true>(W, r, h, c, 0);
             if (st != RAY_RUNNING) {
                 if (st == RAY_MISS) miss(h);
                 active = false;
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
             idle = __ballot(!active && !finished);
         }
         if (active) {
-            const int32_t st = ray_outer<COUNT>(W, r, h, c, skip);
+            const int32_t st = ray_outer<COUNT, true>(W, r, h, c, skip);
             if (st != RAY_RUNNING) {
                 if (st == RAY_MISS) miss(h);
                 active = false;
@@ -500,7 +500,7 @@ __global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, const float4 *
                 const float ro[3] = {a.x, a.y, a.z}, rd[3] = {b.x, b.y, b.z};
                 tag = __float_as_uint(a.w);
                 Hit h;
-                trace_world<0, COUNT>(W, ro, rd, b.w, h, c);
+                trace_world<0, COUNT, true>(W, ro, rd, b.w, h, c);
                 store_hit(hits + tag, h);
                 if (SUN) {
                     if (h.hit && h.mat != 5u) {
@@ -567,8 +567,8 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, 
                 batch_ray(W, q, id - q.first, ro, rd, tcap, out, skip, tskip, void_ray);
                 if (out != 0xFFFFFFFFu) {
                     Hit h;
-                    if (skip == 0) trace_world<0, COUNT>(W, ro, rd, tcap, h, c);
-                    else trace_world<5, COUNT>(W, ro, rd, tcap, h, c);
+                    if (skip == 0) trace_world<0, COUNT, true>(W, ro, rd, tcap, h, c);
+                    else trace_world<5, COUNT, true>(W, ro, rd, tcap, h, c);
                     h.t = h.t + tskip;
                     if (void_ray) h.hit = 0;
                     store_hit(q.hits + out, h);

@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_scene(DevWorld W, DevBodies B, PrimArgs
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
-    scene_primary_trace<COUNT>(W, B, A.pos, cam_mat, rd, h, c);
+    scene_primary_trace<COUNT, true>(W, B, A.pos, cam_mat, rd, h, c);
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     if (hits) store_hit(hits + i, h, h.hit && h.obj >= 0);
     float4 p;
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, DevBodies B
         const float P[3] = {P4.x, P4.y, P4.z};
         const GiPathCfg G = {A.reach, A.lights_nee};
         uint32_t seed = 0u;
-        gi_path<1, false, COUNT, BODIES>(W, S, G, P, N, rdir, seed, true, Lr, &bh, nullptr, c, &B);
+        gi_path<1, false, COUNT, BODIES, true>(W, S, G, P, N, rdir, seed, true, Lr, &bh, nullptr, c, &B);
     }
     radiance[oi] = make_float4(Lr[0], Lr[1], Lr[2], sky ? 0.0f : 1.0f);
     if (bounce) store_hit(bounce + oi, bh, BODIES && bh.hit && bh.obj >= 0);
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, DevBodies B, TileA
             float sd[3], tf;
             sample_cone(sun, ang, seed, sd);
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
-            v = sun_visibility_t<true, COUNT, BODIES>(W, so, sd, A.leaf, seed, tf, hits, i, c, &B);
+            v = sun_visibility_t<true, COUNT, BODIES, true>(W, so, sd, A.leaf, seed, tf, hits, i, c, &B);
             stored = true;
         }
     }
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(256) k_rays(DevWorld W, const vf_ray *rays, lo
     const float ro[3] = {r[0], r[1], r[2]}, rd[3] = {r[3], r[4], r[5]};
     Cnt c = {0, 0, 0, 0};
     Hit h;
-    trace_world<SKIP, COUNT>(W, ro, rd, tcap, h, c);
+    trace_world<SKIP, COUNT, true>(W, ro, rd, tcap, h, c);
     store_hit(hits + i, h);
     flush_count<COUNT>(c, cnt);
 }
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(256) k_probes(DevWorld W, DevBodies B, ProbeAr
     for (int k = 0; k < 3; ++k) rd[k] = d[k] + 2.0f * c2v[k];
     Cnt cc = {0, 0, 0, 0};
     Hit h;
-    trace_world<5, COUNT>(W, ro, rd, A.reach, h, cc);
+    trace_world<5, COUNT, true>(W, ro, rd, A.reach, h, cc);
     if (BODIES) {
         h.obj = -1;
         trace_objects<COUNT>(W, B, ro, rd, h, gmin(h.hit ? h.t : A.reach, 16.0f), false, cc);
