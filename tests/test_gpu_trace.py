@@ -465,3 +465,21 @@ def test_sun_cone_matches_oracle(ctx):
         assert np.array_equal(out.cpu().numpy(), o)
         assert hit_diff(vf.hits_to_numpy(hh), oh).size == 0
         assert 0.0 < (o == 0).mean() < 1.0
+
+
+def test_world_loaded_from_a_save_traces_identically(ctx, tmp_path):
+    """The config-1 world written as a .vfworld save (the reference's format), read back and uploaded:
+    the primary pass equals the oracle's on the original world."""
+    from vftrace import vfworld as V
+    w, a, ow = world(1)
+    path = tmp_path / "cfg1.vfworld"
+    V.write_vfworld(path, a)
+    back = V.arrays_from_save(V.read_vfworld(path))
+    ctx.upload_arrays(back)
+    ctx.set_scene(vf.default_scene(back["world_bricks"], back["world_win"]))
+    pu, cam = config1_camera()
+    W, H = 320, 180
+    hits, gbp, gbn = primary_gpu(ctx, pu, W, H)
+    oh, opos, onrm, _ = ow.pass_primary(cam, W, H)
+    assert hit_diff(vf.hits_to_numpy(hits), oh).size == 0
+    assert np.array_equal(gbp.cpu().numpy().reshape(-1, 4).view(np.uint32), opos.view(np.uint32))
