@@ -126,6 +126,17 @@ def lib():
                                            C.c_int]),
             "vfo_gicache_samples": (None, [P, P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_int32,
                                            C.c_uint32, P, C.c_int]),
+            "vfo_exp": (C.c_float, [C.c_float]),
+            "vfo_pow16": (C.c_float, [C.c_float]),
+            "vfo_pow64": (C.c_float, [C.c_float]),
+            "vfo_f16_to_f32": (C.c_float, [C.c_uint16]),
+            "vfo_f32_to_f16": (C.c_uint16, [C.c_float]),
+            "vfo_gbuffer_motion": (None, [C.c_int32, C.c_int32, P, P, P, C.c_float, P, P, P, P, P, C.c_float, P, P, P,
+                                          C.c_int32, P, P]),
+            "vfo_gi_accumulate": (None, [C.c_int32, C.c_int32, C.c_uint32, P, C.c_int32] + [P] * 12),
+            "vfo_light_accumulate": (None, [C.c_int32, C.c_int32, P] + [P] * 9),
+            "vfo_gi_atrous_level": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32] + [P] * 7),
+            "vfo_gi_atrous": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32] + [P] * 7),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -554,3 +565,90 @@ def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_cou
     if return_counts:
         return gi, sh, cnt
     return gi, sh
+
+
+# ---- temporal consumers (vfo_temporal.c) -------------------------------------------------------------
+def half_dims(w, h):
+    """The half-resolution GI targets' size: w.div_ceil(2) x h.div_ceil(2) (renderer/mod.rs:525)."""
+    return (w + 1) // 2, (h + 1) // 2
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def gi_accumulate(w, h, frame_idx, shift, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion, gb_delta,
+                  indirect, hist_prev, mom_prev, bicubic=True):
+    """gi.comp's accumulation (gi.comp:37-63, 93-184).  Inputs as numpy arrays: gb_pos f32 [h, w, 4],
+    gb_normal u32 [h, w], gb_motion u32 [h, w] (RG16F bits), gb_delta u16 [h, w, 4] (RGBA16F bits),
+    indirect f32 [hh, hw, 4], hist_prev u16 [hh, hw, 4] (RGBA16F bits), mom_prev f32 [hh, hw, 4].
+    Returns (hist_cur u16, mom_cur f32, guide u32 [hh, hw, 4])."""
+    hw, hh = half_dims(w, h)
+    ins = [_c(gb_pos, np.float32), _c(gb_normal, np.uint32), _c(gb_pos_prev, np.float32),
+           _c(gb_normal_prev, np.uint32), _c(gb_motion, np.uint32), _c(gb_delta, np.uint16), _c(indirect, np.float32),
+           _c(hist_prev, np.uint16), _c(mom_prev, np.float32)]
+    hist = np.zeros((hh, hw, 4), np.uint16)
+    mom = np.zeros((hh, hw, 4), np.float32)
+    guide = np.zeros((hh, hw, 4), np.uint32)
+    sh = (C.c_float * 3)(*[float(x) for x in shift])
+    lib().vfo_gi_accumulate(w, h, frame_idx, sh, int(bool(bicubic)), *[a.ctypes.data for a in ins],
+                            hist.ctypes.data, mom.ctypes.data, guide.ctypes.data)
+    return hist, mom, guide
+
+
+def light_accumulate(w, h, shift, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion, gb_delta, shadow,
+                     hist_prev):
+    """lighting.comp's shadow history (lighting.comp:36-42, 92-139).  shadow f32 [h, w]; hist_prev f32
+    [h, w, 4].  Returns hist_cur f32 [h, w, 4] = (shadow mean, 2nd moment, count, fast)."""
+    ins = [_c(gb_pos, np.float32), _c(gb_normal, np.uint32), _c(gb_pos_prev, np.float32),
+           _c(gb_normal_prev, np.uint32), _c(gb_motion, np.uint32), _c(gb_delta, np.uint16), _c(shadow, np.float32),
+           _c(hist_prev, np.float32)]
+    out = np.zeros((h, w, 4), np.float32)
+    sh = (C.c_float * 3)(*[float(x) for x in shift])
+    lib().vfo_light_accumulate(w, h, sh, *[a.ctypes.data for a in ins], out.ctypes.data)
+    return out
+
+
+def gi_atrous_level(w, h, it, step, gb_pos, gb_normal, gi_mom, gi_hist, guide, in_cv=None, histfix=True):
+    """One a-trous level (atrous.comp); returns outCV u16 [hh, hw, 4] (RGBA16F bits)."""
+    hw, hh = half_dims(w, h)
+    ins = [_c(gb_pos, np.float32), _c(gb_normal, np.uint32), _c(gi_mom, np.float32), _c(gi_hist, np.uint16),
+           _c(guide, np.uint32)]
+    icv = _c(in_cv, np.uint16) if in_cv is not None else np.zeros((hh, hw, 4), np.uint16)
+    out = np.zeros((hh, hw, 4), np.uint16)
+    lib().vfo_gi_atrous_level(w, h, it, step, int(bool(histfix)), *[a.ctypes.data for a in ins], icv.ctypes.data,
+                              out.ctypes.data)
+    return out
+
+
+def gi_atrous(w, h, levels, gb_pos, gb_normal, gi_mom, gi_hist, guide, histfix=True):
+    """`levels` a-trous levels with the reference's ping-pong (level i writes filt[i & 1]); returns
+    (filt0, filt1)."""
+    hw, hh = half_dims(w, h)
+    ins = [_c(gb_pos, np.float32), _c(gb_normal, np.uint32), _c(gi_mom, np.float32), _c(gi_hist, np.uint16),
+           _c(guide, np.uint32)]
+    f0 = np.zeros((hh, hw, 4), np.uint16)
+    f1 = np.zeros((hh, hw, 4), np.uint16)
+    lib().vfo_gi_atrous(w, h, levels, int(bool(histfix)), *[a.ctypes.data for a in ins], f0.ctypes.data,
+                        f1.ctypes.data)
+    return f0, f1
+
+
+def gbuffer_motion(w, h, pass_ubo, gb_pos, body=None, bodies=None):
+    """gbuffer.comp's gbMotion (u32 [h, w], RG16F bits) and gbDelta (u16 [h, w, 4], RGBA16F bits) from the
+    written G-buffer, the pass's current and previous camera, and optional per-pixel body indices with
+    the bodies' headers (OBJ_HEADER_DTYPE records)."""
+    pu = pass_ubo
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+    pos = _c(gb_pos, np.float32)
+    bd = _c(body, np.int32) if body is not None else None
+    hb = np.ascontiguousarray(np.asarray(bodies).view(np.float32).reshape(-1, 60)) if bodies is not None else None
+    nb = 0 if hb is None else hb.shape[0]
+    mot = np.zeros((h, w), np.uint32)
+    dl = np.zeros((h, w, 4), np.uint16)
+    lib().vfo_gbuffer_motion(w, h, f3(pu.cam_right), f3(pu.cam_up), f3(pu.cam_fwd), float(pu.tan_half_fov),
+                             (C.c_float * 2)(*[float(x) for x in pu.jitter]), f3(pu.prev_cam_pos), f3(pu.prev_cam_right),
+                             f3(pu.prev_cam_up), f3(pu.prev_cam_fwd), float(pu.prev_tan_half_fov), pos.ctypes.data,
+                             None if bd is None else bd.ctypes.data, None if hb is None else hb.ctypes.data, nb,
+                             mot.ctypes.data, dl.ctypes.data)
+    return mot, dl

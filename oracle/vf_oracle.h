@@ -217,6 +217,36 @@ float vfo_sky_open(const vfo_world *w, const vfo_sky *s, const float p[3]);
 void vfo_gi_rays(int32_t width, int32_t height, uint32_t frame, const float *gb_pos, const uint32_t *gb_normal,
                  vfo_ray *rays, uint8_t *valid);
 
+/* ---- temporal consumers (vfo_temporal.c): gi.comp accumulation, lighting.comp history, a-trous ----
+ * Layouts: gb_pos RGBA32F and gb_normal RGBA8 per full-res pixel; gb_motion RG16F (x in the low
+ * half); gb_delta RGBA16F; GI history RGBA16F, GI moments RGBA32F, guide RGBA32UI per half-res
+ * texel ((w+1)/2 x (h+1)/2); indirect RGBA32F per half-res texel (vf_trace_gi's radiance); shadow
+ * float and shadow history RGBA32F per full-res pixel. */
+float vfo_exp(float x);
+float vfo_pow16(float x);
+float vfo_pow64(float x);
+float vfo_f16_to_f32(uint16_t h);
+uint16_t vfo_f32_to_f16(float f);
+/* gbuffer.comp's gbMotion / gbDelta from the written G-buffer (body: index per pixel or -1, nullable;
+ * bodies: 60 floats per body as vf_obj_header). */
+void vfo_gbuffer_motion(int32_t w, int32_t h, const float cam_right[3], const float cam_up[3], const float cam_fwd[3],
+                        float thf, const float jit[2], const float prev_pos[3], const float prev_right[3],
+                        const float prev_up[3], const float prev_fwd[3], float prev_thf, const float *gb_pos,
+                        const int32_t *body, const float *bodies, int32_t n_bodies, uint32_t *motion, uint16_t *delta);
+void vfo_gi_accumulate(int32_t w, int32_t h, uint32_t frame_idx, const float shift[3], int32_t bicubic,
+                       const float *gb_pos, const uint32_t *gb_normal, const float *gb_pos_prev,
+                       const uint32_t *gb_normal_prev, const uint32_t *gb_motion, const uint16_t *gb_delta,
+                       const float *indirect, const uint16_t *hist_prev, const float *mom_prev, uint16_t *hist_cur,
+                       float *mom_cur, uint32_t *guide);
+void vfo_light_accumulate(int32_t w, int32_t h, const float shift[3], const float *gb_pos, const uint32_t *gb_normal,
+                          const float *gb_pos_prev, const uint32_t *gb_normal_prev, const uint32_t *gb_motion,
+                          const uint16_t *gb_delta, const float *shadow, const float *hist_prev, float *hist_cur);
+void vfo_gi_atrous_level(int32_t w, int32_t h, int32_t it, int32_t step, int32_t histfix, const float *gb_pos,
+                         const uint32_t *gb_normal, const float *gi_mom, const uint16_t *gi_hist, const uint32_t *guide,
+                         const uint16_t *in_cv, uint16_t *out_cv);
+void vfo_gi_atrous(int32_t w, int32_t h, int32_t levels, int32_t histfix, const float *gb_pos, const uint32_t *gb_normal,
+                   const float *gi_mom, const uint16_t *gi_hist, const uint32_t *guide, uint16_t *filt0, uint16_t *filt1);
+
 #ifdef __cplusplus
 }
 #endif

@@ -209,10 +209,20 @@ int write_bvh(vf_ctx *ctx, const char *what, std::vector<vf_bvh_node> nodes, std
 int write_headers(vf_ctx *ctx, const char *what, const vf_obj_header *hd, int32_t n) {
     std::vector<float4> packed((size_t)n * kBodyWords);
     for (int32_t i = 0; i < n; ++i) body_header(hd[i], &packed[(size_t)i * kBodyWords]);
+    // rows of prevObjToWorld's top 3x4 (the motion vectors' previous pose, gbuffer.comp:84-88)
+    std::vector<float4> prev((size_t)n * 3);
+    for (int32_t i = 0; i < n; ++i)
+        for (int r = 0; r < 3; ++r) {
+            const float *m = hd[i].prev_obj_to_world;
+            prev[(size_t)i * 3 + r] = make_float4(m[r], m[4 + r], m[8 + r], m[12 + r]);
+        }
     int rc;
     if ((rc = grow(ctx, what, ctx->b_hdr, ctx->b_hdr_cap, packed.size()))) return rc;
+    if ((rc = grow(ctx, what, ctx->b_prev, ctx->b_prev_cap, prev.size()))) return rc;
     if (!packed.empty())
         VF_HIP(ctx, hipMemcpyAsync(ctx->b_hdr, packed.data(), packed.size() * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    if (!prev.empty())
+        VF_HIP(ctx, hipMemcpyAsync(ctx->b_prev, prev.data(), prev.size() * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return VF_OK;
 }
@@ -249,9 +259,11 @@ int check_bodies(vf_ctx *ctx, const char *what) {
 }
 
 void free_bodies(vf_ctx *ctx) {
-    for (void *p : {(void *)ctx->b_hdr, (void *)ctx->b_nodes, (void *)ctx->b_esc, (void *)ctx->b_grid, (void *)ctx->b_prims})
+    for (void *p : {(void *)ctx->b_hdr, (void *)ctx->b_nodes, (void *)ctx->b_esc, (void *)ctx->b_grid, (void *)ctx->b_prims,
+                    (void *)ctx->b_prev})
         if (p) (void)hipFree(p);
-    ctx->b_hdr = ctx->b_nodes = nullptr;
+    ctx->b_hdr = ctx->b_nodes = ctx->b_prev = nullptr;
+    ctx->b_prev_cap = 0;
     ctx->b_esc = nullptr;
     ctx->b_grid = ctx->b_prims = nullptr;
     ctx->b_hdr_cap = ctx->b_grid_cap = ctx->b_node_cap = ctx->b_esc_cap = ctx->b_prim_cap = 0;

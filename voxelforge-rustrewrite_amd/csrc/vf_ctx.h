@@ -60,6 +60,8 @@ struct vf_ctx {
     int32_t *b_esc = nullptr;
     uint32_t *b_grid = nullptr, *b_prims = nullptr;
     size_t b_hdr_cap = 0, b_grid_cap = 0, b_node_cap = 0, b_esc_cap = 0, b_prim_cap = 0;   // elements
+    float4 *b_prev = nullptr;                 // 3 float4 per body: rows of prevObjToWorld (motion vectors)
+    size_t b_prev_cap = 0;
     uint32_t b_slot_end = 0;                  // one past the highest atlas slot the body grids name
     std::vector<uint32_t> b_host_grid;        // host copy of the body grids
     std::vector<vf_bvh_node> b_host_nodes;    // the BVH in use

@@ -103,6 +103,10 @@ _SIGS = {
     "vf_gicache_grad": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
     "vf_gicache_adam": (C.c_int, [_P, _P, _P]),
     "vf_gicache_train": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "vf_gbuffer_motion": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    "vf_gi_accumulate": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_int32] + [_P] * 12),
+    "vf_light_accumulate": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32] + [_P] * 9),
+    "vf_gi_atrous": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32] + [_P] * 7),
     "vf_gen_defaults": (C.c_int, [C.c_int32, C.POINTER(GenParams)]),
     "vf_gen_world_create": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
     "vf_gen_world_from_dense": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, C.POINTER(_P)]),
@@ -183,6 +187,16 @@ def make_pass(pos, right, up, fwd, fov_deg=70.0, frame_idx=0, jitter=(0.0, 0.0),
     p.prev_tan_half_fov = float(thf)
     p.frame_idx = int(frame_idx)
     p.jitter[0], p.jitter[1] = float(jitter[0]), float(jitter[1])
+    return p
+
+
+def set_prev_camera(p, prev):
+    """Copies last frame's camera (pass `prev`'s cam_*, tan_half_fov) into p's prev_cam_* fields, as
+    set_frame_uniforms does each frame (frame.rs:549-566); returns p."""
+    for i in range(3):
+        p.prev_cam_pos[i] = prev.cam_pos[i]; p.prev_cam_right[i] = prev.cam_right[i]
+        p.prev_cam_up[i] = prev.cam_up[i]; p.prev_cam_fwd[i] = prev.cam_fwd[i]
+    p.prev_tan_half_fov = prev.tan_half_fov
     return p
 
 
@@ -472,6 +486,34 @@ class Context:
         self._ck(lib().vf_trace_gi(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(half_res)),
                                    float(gi_reach), int(bool(lights_nee)), _ptr(gb_pos), _ptr(gb_normal),
                                    _ptr(radiance), _ptr(bounce)))
+
+    # temporal consumers (include/vf_temporal.h)
+    def gbuffer_motion(self, pass_ubo, width, height, gb_pos, gb_motion, gb_delta, body=None):
+        """gbuffer.comp's gbMotion (RG16F bits per pixel, int32 tensor) and gbDelta (RGBA16F bits)."""
+        self._ck(lib().vf_gbuffer_motion(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
+                                         _ptr(body), _ptr(gb_motion), _ptr(gb_delta)))
+
+    def gi_accumulate(self, pass_ubo, width, height, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
+                      gb_delta, indirect, hist_prev, mom_prev, hist_cur, mom_cur, guide, bicubic=True):
+        """gi.comp's reprojected accumulation of the traced half-res GI (RGBA16F history as int16/uint16
+        tensors of half-float bits, RGBA32F moments, RGBA32UI guide)."""
+        self._ck(lib().vf_gi_accumulate(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(bicubic)),
+                                        *[_ptr(t) for t in (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
+                                                            gb_delta, indirect, hist_prev, mom_prev, hist_cur, mom_cur,
+                                                            guide)]))
+
+    def light_accumulate(self, pass_ubo, width, height, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
+                         gb_delta, shadow, hist_prev, hist_cur):
+        """lighting.comp's sun-shadow history (RGBA32F per pixel)."""
+        self._ck(lib().vf_light_accumulate(self._h, C.byref(pass_ubo), int(width), int(height),
+                                           *[_ptr(t) for t in (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev,
+                                                               gb_motion, gb_delta, shadow, hist_prev, hist_cur)]))
+
+    def gi_atrous(self, width, height, levels, gb_pos, gb_normal, gi_mom, gi_hist, guide, filt0, filt1,
+                  histfix=True):
+        """`levels` a-trous levels; level i writes filt[i & 1] (odd counts end in filt0)."""
+        self._ck(lib().vf_gi_atrous(self._h, int(width), int(height), int(levels), int(bool(histfix)),
+                                    *[_ptr(t) for t in (gb_pos, gb_normal, gi_mom, gi_hist, guide, filt0, filt1)]))
 
     def trace_rays(self, rays, n, hits, t_cap=1e30, skip_mat=0):
         self._ck(lib().vf_trace_rays(self._h, _ptr(rays), int(n), float(t_cap), int(skip_mat), _ptr(hits)))
