@@ -10,8 +10,10 @@ steps, in Grays/s.
 Also printed in the same JSON line: the roofline of the dominant kernel (algorithmic bytes of the
 reference visit sequence, counted by the kernels' counting variant in an untimed pass, over the
 kernel time from HIP events on the launch stream) and the CPU baseline (the project's C oracle with
-OpenMP on the same frame, which also checks the frame's hits against the GPU's).  The frame is
-recorded once as a hipGraph (vf_graph_begin / vf_graph_end) and replayed per step.
+OpenMP on the same frame, which also checks the frame's hits against the GPU's).  Each step launches
+the frame's pass eagerly on the context's stream (--graph-steps: replays one recorded hipGraph per
+step instead).  The temporal passes that consume the traced output (motion vectors, GI accumulation,
+shadow history, a-trous filter) are timed at 1920 x 1080 in the same run.
 """
 import argparse
 import json
@@ -80,7 +82,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the open-forest, fixed-world, GI-batch, learned-cache and uneven-batch lines")
-    ap.add_argument("--no-graph", action="store_true", help="launch the passes eagerly instead of replaying a graph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="record no graphs at all (also the kernel-duration loop launches eagerly)")
+    ap.add_argument("--graph-steps", action="store_true",
+                    help="replay one recorded graph per step instead of launching the pass eagerly")
     ap.add_argument("--cache-frames", type=int, default=256, help="online-training frames before the GI cache is judged")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (calibrates rocprof FETCH/WRITE_SIZE)")
@@ -141,17 +146,20 @@ def main():
     log(f"per step: {cp['rays']} primary + {cs['rays']} sun rays; fetch bytes/ray primary "
         f"{cp['bytes'] / max(cp['rays'], 1):.1f}, sun {cs['bytes'] / max(cs['rays'], 1):.1f}")
 
-    # the frame (primary pass, then the sun pass) recorded once as a graph and replayed per step;
-    # one-kernel graphs time each kernel alone
-    class Eager:   # --no-graph: the same calls, launched directly each time
+    # each step launches the frame's pass (primary and sun rays, one fused kernel); the comparison lines
+    # launch the two passes, or each kernel alone, the same way
+    class Eager:   # the same calls, launched directly each time
         def __init__(self, fn):
             self.fn = fn
 
         def close(self):
             pass
 
-    def record(fn):
-        if args.no_graph:
+    def record(fn, graph=None):
+        # steps are launched eagerly (a renderer launches each frame's pass once; consecutive eager
+        # launches queue back to back, a one-kernel graph replay adds ~8 us per step); the
+        # kernel-duration loop below records its back-to-back launches as one graph
+        if args.no_graph or not (args.graph_steps if graph is None else graph):
             return Eager(fn)
         ctx.graph_begin()
         fn()
@@ -201,7 +209,7 @@ def main():
     fused_bytes = cp["bytes"] + cs["bytes"] + n * (HIT_BYTES + GBPOS_BYTES + GBN_BYTES + SHADOW_BYTES)
     dominant = "k_primary_shadow"
     reps = 20
-    g_rep = record(lambda: [ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh) for _ in range(reps)])
+    g_rep = record(lambda: [ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh) for _ in range(reps)], graph=True)
     _, rep_ms = timed_loop(g_rep, max(1, args.steps // reps))
     kernel_ms = rep_ms / reps
     g_rep.close()
@@ -259,6 +267,7 @@ def main():
     for g in (g_step, g_two, g_prim, g_shad):
         g.close()
     if not args.no_extra:
+        result["temporal_1080p"] = temporal_config(ctx, stream, args, pu)
         result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
         result["other_configs"] = other_configs(ctx, args)
     ctx.close()
@@ -311,6 +320,76 @@ def open_forest_config(ctx, stream, args, hits, gbp, gbn, sh):
     scene.num_objects = 0
     ctx.set_scene(scene)
     ctx.objects_upload(hd[:0], grid[:0])
+    return out
+
+
+# Algorithmic bytes per unit of the temporal passes (each input texel the pass needs read once, every
+# output written once): gbuffer motion per pixel = gbPos 16 + gbMotion 4 + gbDelta 8; gi.comp
+# accumulation per half-res texel = this frame's gbPos 2x16 + gbNormal 2x4 (representative and rotating
+# sample) + motion 4 + delta 8 + traced radiance 16 + last frame's gbPos 16 + gbNormal 4 + history 8 +
+# moments 16 + outputs 8 + 16 + 16; lighting.comp history per pixel = gbPos 16 + gbNormal 4 + motion 4 +
+# delta 8 + sample 4 + last frame's gbPos 16 + gbNormal 4 + history 16 + output 16; an a-trous level per
+# half-res texel = gbPos 16 + gbNormal 4 + guide 16 + its input (history 8 + moments 16 at level 0,
+# the previous level 8 after) + output 8.
+TEMPORAL_BYTES = {"k_motion": 28, "k_gi_accumulate": 152, "k_light_accumulate": 88, "k_atrous_3": 68 + 52 * 2}
+
+
+def temporal_config(ctx, stream, args, pu_prev):
+    """The temporal passes of one 1920x1080 frame of the endless world after the camera moved 0.3 voxels
+    and turned 0.01 rad: motion vectors, gi.comp accumulation (Catmull-Rom on), lighting.comp history,
+    3 a-trous levels, on inputs traced by the library (G-buffer of both frames, half-res GI, cone sun
+    ray).  Each kernel: 20 launches recorded back to back in a graph, timed with HIP events."""
+    dev = torch.device("cuda:0")
+    n = W * H
+    hw, hh = (W + 1) // 2, (H + 1) // 2
+    pos = [pu_prev.cam_pos[i] + 0.3 * pu_prev.cam_fwd[i] for i in range(3)]
+    yaw = math.atan2(pu_prev.cam_fwd[0], pu_prev.cam_fwd[2]) + 0.01
+    pitch = math.asin(max(-1.0, min(1.0, pu_prev.cam_fwd[1])))
+    cam_pos, right, up, fwd = vf.fly_camera(pos, yaw, pitch)
+    pu = vf.set_prev_camera(vf.make_pass(cam_pos, right, up, fwd, 70.0, 1), pu_prev)
+    z = lambda k, dt: torch.zeros(k, dtype=dt, device=dev)
+    hits = z(n * HIT_BYTES, torch.uint8)
+    gpos, gn = [z(n * 4, torch.float32) for _ in range(2)], [z(n, torch.int32) for _ in range(2)]
+    mot, dl = z(n, torch.int32), z(n * 4, torch.int16)
+    rad, sun = z(hw * hh * 4, torch.float32), z(n, torch.float32)
+    hist = [z(hw * hh * 4, torch.int16) for _ in range(2)]
+    mom = [z(hw * hh * 4, torch.float32) for _ in range(2)]
+    lh = [z(n * 4, torch.float32) for _ in range(2)]
+    guide, f0, f1 = z(hw * hh * 4, torch.int32), z(hw * hh * 4, torch.int16), z(hw * hh * 4, torch.int16)
+    ctx.trace_gbuffer(pu_prev, W, H, hits, gpos[1], gn[1], None)
+    ctx.trace_gbuffer(pu, W, H, hits, gpos[0], gn[0], None)
+    ctx.gbuffer_motion(pu, W, H, gpos[0], mot, dl)
+    ctx.trace_gi(pu, W, H, gpos[0], gn[0], rad, None, half_res=True, gi_reach=48.0)
+    ctx.trace_sun_cone(pu, W, H, gpos[0], gn[0], sun)
+    # a converged history: accumulate a few frames first (same inputs)
+    for _ in range(6):
+        ctx.gi_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, rad, hist[1], mom[1], hist[0], mom[0], guide)
+        ctx.gi_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, rad, hist[0], mom[0], hist[1], mom[1], guide)
+        ctx.light_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, sun, lh[1], lh[0])
+        ctx.light_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, sun, lh[0], lh[1])
+    torch.cuda.synchronize()
+    calls = {
+        "k_motion": (lambda: ctx.gbuffer_motion(pu, W, H, gpos[0], mot, dl), n),
+        "k_gi_accumulate": (lambda: ctx.gi_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, rad, hist[1],
+                                                      mom[1], hist[0], mom[0], guide), hw * hh),
+        "k_light_accumulate": (lambda: ctx.light_accumulate(pu, W, H, gpos[0], gn[0], gpos[1], gn[1], mot, dl, sun,
+                                                            lh[1], lh[0]), n),
+        "k_atrous_3": (lambda: ctx.gi_atrous(W, H, 3, gpos[0], gn[0], mom[0], hist[0], guide, f0, f1), hw * hh),
+    }
+    reps = 20
+    out = {}
+    total = 0.0
+    for name, (fn, units) in calls.items():
+        ms = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
+        nbytes = TEMPORAL_BYTES[name] * units
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        total += ms
+        out[name] = {"us": round(ms * 1e3, 3), "algorithmic_bytes": int(nbytes), "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["all_us"] = round(total * 1e3, 3)
+    cnt = hist[0].view(torch.float16).view(-1, 4)[:, 3].float()
+    out["history_reused"] = round(float((cnt > 1).float().mean().item()), 4)
+    log(f"temporal passes 1080p: {out}")
     return out
 
 

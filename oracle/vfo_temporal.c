@@ -16,7 +16,9 @@
  *     degree-7 Taylor polynomial of e^r, |r| <= ln2/2, scaled by 2^k; x < -86 gives 0, so no
  *     result is subnormal.  Error < 2 ulp (GLSL allows 3 + 2|x| ulp).
  *   - pow(x, 16) and pow(x, 64) (the a-trous normal weights): 4 and 6 squarings.
- * RGBA16F stores round to nearest even; RGBA8 normals decode as c/255*2-1.
+ * RGBA16F stores round to nearest even; RGBA8 normals decode as c/255*2-1.  In the a-trous weights
+ * a division by the centre's depth scale or luminance scale is a multiplication by its reciprocal
+ * (computed once per centre), which GLSL's a/b precision (2.5 ulp) allows.
  *
  * Pinning (tests/test_temporal_oracle.py): exp against float64 within 2 ulp; the half
  * conversions against numpy's; closed forms for a still camera (running mean, history count
@@ -342,7 +344,7 @@ void vfo_gi_atrous_level(int32_t w, int32_t h, int32_t it, int32_t step, int32_t
                 if (cLen < 4.0f) {
                     float acc[3] = {cCol[0] * cLen, cCol[1] * cLen, cCol[2] * cLen};
                     float ws = cLen;
-                    const float dscale = 1.0f + posT[3] * 0.01f;
+                    const float dinv = 1.0f / (1.0f + posT[3] * 0.01f);
                     for (int dy = -3; dy <= 3; ++dy)
                         for (int dx = -3; dx <= 3; ++dx) {
                             if (dx == 0 && dy == 0) continue;
@@ -356,7 +358,7 @@ void vfo_gi_atrous_level(int32_t w, int32_t h, int32_t it, int32_t step, int32_t
                             dec_normal(g[3], gN);
                             const float d[3] = {gP[0] - P[0], gP[1] - P[1], gP[2] - P[2]};
                             const float ha = vfo_f16_to_f32(gi_hist[q * 4 + 3]);
-                            const float wt = vfo_pow16(tmax(dot3(N, gN), 0.0f)) * vfo_exp(-fabsf(dot3(d, N)) / dscale) * ha;
+                            const float wt = vfo_pow16(tmax(dot3(N, gN), 0.0f)) * vfo_exp(-fabsf(dot3(d, N)) * dinv) * ha;
                             for (int c = 0; c < 3; ++c) acc[c] = acc[c] + vfo_f16_to_f32(gi_hist[q * 4 + c]) * wt;
                             ws = ws + wt;
                         }
@@ -397,6 +399,7 @@ void vfo_gi_atrous_level(int32_t w, int32_t h, int32_t it, int32_t step, int32_t
             float vAcc = cVar * k00 * k00;
             float wSum = k00;
             const float depthScale = 1.0f * (1.0f + posT[3] * 0.01f);
+            const float zinv = 1.0f / depthScale, linv = 1.0f / lumPhi;
             for (int dy = -2; dy <= 2; ++dy)                                       /* atrous.comp:132-151 */
                 for (int dx = -2; dx <= 2; ++dx) {
                     if (dx == 0 && dy == 0) continue;
@@ -415,8 +418,8 @@ void vfo_gi_atrous_level(int32_t w, int32_t h, int32_t it, int32_t step, int32_t
                     const float kern = h5[dx < 0 ? -dx : dx] * h5[dy < 0 ? -dy : dy];
                     const float wN = vfo_pow64(tmax(dot3(N, qn), 0.0f));
                     const float d[3] = {qp[0] - P[0], qp[1] - P[1], qp[2] - P[2]};
-                    const float wZ = vfo_exp(-fabsf(dot3(d, N)) / depthScale);
-                    const float wL = vfo_exp(-fabsf(cLum - qLum) / lumPhi);
+                    const float wZ = vfo_exp(-fabsf(dot3(d, N)) * zinv);
+                    const float wL = vfo_exp(-fabsf(cLum - qLum) * linv);
                     const float wt = kern * wN * wZ * wL;
                     for (int c = 0; c < 3; ++c) cAcc[c] = cAcc[c] + wt * qc[c];
                     vAcc = vAcc + wt * wt * qv;

@@ -18,13 +18,14 @@ namespace {
 
 // exp(x): Cody-Waite reduction x = k ln2 + r with ln2 split into a 16-bit head and a tail, the
 // degree-7 Taylor polynomial of e^r (|r| <= ln2/2, truncation < 6e-9 relative), times 2^k built in
-// the exponent field.  x < -86 gives 0 (no subnormal results); error < 2 ulp.
+// the exponent field.  x < -86 gives 0 (no subnormal results); error < 2 ulp.  The filters only take
+// exp of a non-positive argument, so the polynomial runs without a branch on x clamped to [-86, 0]
+// (GLSL-style min / max: a NaN passes through and comes out NaN) and is multiplied by 0 below -86
+// (x > 0 is outside this function's domain).
 __device__ __forceinline__ float t_exp(float x) {
-    if (x != x) return x;
-    if (x < -86.0f) return 0.0f;
-    if (x > 88.0f) return __int_as_float(0x7f800000);
-    const float k = __builtin_rintf(x * 1.44269504f);
-    const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-6f;
+    const float xc = gmin(gmax(x, -86.0f), 0.0f);
+    const float k = __builtin_rintf(xc * 1.44269504f);
+    const float r = (xc - k * 0.693145751953125f) - k * 1.42860677e-6f;
     float p = 1.0f / 5040.0f;
     p = p * r + 1.0f / 720.0f;
     p = p * r + 1.0f / 120.0f;
@@ -33,7 +34,8 @@ __device__ __forceinline__ float t_exp(float x) {
     p = p * r + 0.5f;
     p = p * r + 1.0f;
     p = p * r + 1.0f;
-    return p * __int_as_float(((int32_t)k + 127) << 23);
+    const float e = p * __int_as_float(((int32_t)k + 127) << 23);
+    return e * (x < -86.0f ? 0.0f : 1.0f);
 }
 // pow(x, 16) and pow(x, 64) by squaring.
 __device__ __forceinline__ float t_pow16(float x) { float y = x * x; y = y * y; y = y * y; return y * y; }
@@ -316,15 +318,100 @@ __device__ __forceinline__ void read_input(size_t q, const uint2 *hist, const fl
     col[0] = v[0]; col[1] = v[1]; col[2] = v[2];
 }
 
-// One a-trous level (atrous.comp:73-154) for one half-res texel.  ITER0: the first level (reads the
-// history and moments, spatial variance for fresh texels, history fix).
+// GI_HISTFIX (atrous.comp:87-112): a texel with fewer than 4 frames of history takes the history-length
+// weighted mean of converged texels on its surface, 7x7 taps 3 texels apart.
+__device__ __forceinline__ void history_fix(const TArgs &A, int32_t px, int32_t py, float4 posT, const float P[3],
+                                            const float N[3], const uint2 *hist, const uint4 *guide, float cLen,
+                                            float cCol[3]) {
+    float acc[3] = {cCol[0] * cLen, cCol[1] * cLen, cCol[2] * cLen};
+    float ws = cLen;
+    const float dinv = 1.0f / (1.0f + posT.w * 0.01f);
+    for (int dy = -3; dy <= 3; ++dy) {
+        const int32_t qy = py + dy * 3;
+        if (qy < 0 || qy >= A.ry) continue;
+        for (int dx = -3; dx <= 3; ++dx) {
+            const int32_t qx = px + dx * 3;
+            if ((dx == 0 && dy == 0) || qx < 0 || qx >= A.rx) continue;
+            const size_t q = (size_t)qy * A.rx + qx;
+            const uint4 g = guide[q];
+            if (g.w == 0u) continue;
+            float gN[3];
+            dec_n(g.w, gN);
+            const float d[3] = {__uint_as_float(g.x) - P[0], __uint_as_float(g.y) - P[1], __uint_as_float(g.z) - P[2]};
+            float hv[4];
+            h4(hist[q], hv);
+            const float wt = t_pow16(gmax(dot3(N, gN), 0.0f)) * t_exp(-fabsf(dot3(d, N)) * dinv) * hv[3];
+            acc[0] = acc[0] + hv[0] * wt; acc[1] = acc[1] + hv[1] * wt; acc[2] = acc[2] + hv[2] * wt;
+            ws = ws + wt;
+        }
+    }
+    if (ws > 0.0f) { cCol[0] = acc[0] / ws; cCol[1] = acc[1] / ws; cCol[2] = acc[2] / ws; }
+}
+
+// One a-trous level (atrous.comp:73-154) per 16x16 lattice of texels: a workgroup takes the texels
+// (X0 + i*step, Y0 + j*step), i, j < 16, of one phase (X0, Y0) of a 16*step-texel super-tile, so every
+// tap of every centre lies on the same lattice and the 20x20 lattice points around the tile (2 on each
+// side) hold all of them, whatever the level's tap spacing.  Those are staged in LDS first, per texel:
+// the guide's position and decoded normal, the level's input colour, variance, luminance, history
+// length (level 0) and a sky flag, computed once instead of by each of the 24 centres that tap it.
+// A tap's LDS address is the centre's plus a constant.
+template <bool ITER0> constexpr int stage_fields() { return ITER0 ? 13 : 12; }
+constexpr int32_t kStageW = 20;   // lattice points per staged row (16 + 2 + 2)
+// Row pitch of the staged fields: an odd multiple of 8 floats, so the 8 rows of a wavefront's 8x8
+// texels fall in 8 different groups of 8 banks.
+constexpr int32_t kStageP = 24;
+constexpr int32_t kStageT = kStageP * kStageW;   // floats per staged field
+
+struct Lattice { int32_t stx, phases, nwg; };
+Lattice lattice_of(int32_t rx, int32_t ry, int32_t step) {
+    Lattice g;
+    const int32_t span = 16 * step;
+    g.stx = (rx + span - 1) / span;
+    g.phases = step * step;
+    g.nwg = g.stx * ((ry + span - 1) / span) * g.phases;
+    return g;
+}
+
 template <bool ITER0>
-__global__ void __launch_bounds__(256) k_atrous(TArgs A, int32_t histfix, const float4 *__restrict__ gb_pos,
+__global__ void __launch_bounds__(256) k_atrous(TArgs A, int32_t stx, int32_t histfix, const float4 *__restrict__ gb_pos,
                                                 const uint32_t *__restrict__ gb_n, const float4 *__restrict__ mom,
                                                 const uint2 *__restrict__ hist, const uint4 *__restrict__ guide,
                                                 const uint2 *__restrict__ in_cv, uint2 *__restrict__ out_cv) {
-    int32_t px, py;
-    if (!texel_of(A, px, py)) return;
+    constexpr int F = stage_fields<ITER0>();
+    constexpr int32_t T = kStageT;
+    __shared__ float st[F * kStageT];
+    const int32_t step = A.step, phases = step * step;
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t ph = L % phases, stile = L / phases;
+    const int32_t X0 = (stile % stx) * 16 * step + ph % step, Y0 = (stile / stx) * 16 * step + ph / step;
+    for (int32_t j = (int32_t)threadIdx.x; j < kStageW * kStageW; j += 256) {
+        const int32_t cx = j % kStageW, cy = j / kStageW, i = cy * kStageP + cx;
+        const int32_t qx = X0 + (cx - 2) * step, qy = Y0 + (cy - 2) * step;
+        float v[F];
+#pragma unroll
+        for (int k = 0; k < F; ++k) v[k] = 0.0f;
+        v[11] = 1.0f;   // outside the image: like sky, never a tap
+        if (qx >= 0 && qy >= 0 && qx < A.rx && qy < A.ry) {
+            const size_t q = (size_t)qy * A.rx + qx;
+            const uint4 g = guide[q];
+            float col[3], var, len;
+            read_input<ITER0>(q, hist, mom, in_cv, col, var, len);
+            v[0] = __uint_as_float(g.x); v[1] = __uint_as_float(g.y); v[2] = __uint_as_float(g.z);
+            dec_n(g.w, &v[3]);
+            v[6] = col[0]; v[7] = col[1]; v[8] = col[2];
+            v[9] = var;
+            v[10] = lumi(col[0], col[1], col[2]);
+            v[11] = g.w == 0u ? 1.0f : 0.0f;
+            if (ITER0) v[F - 1] = len;
+        }
+#pragma unroll
+        for (int k = 0; k < F; ++k) st[k * T + i] = v[k];
+    }
+    __syncthreads();
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t lx = (wv & 1) * 8 + (lane & 7), ly = (wv >> 1) * 8 + (lane >> 3);
+    const int32_t px = X0 + lx * step, py = Y0 + ly * step;
+    if (px >= A.rx || py >= A.ry) return;
     const size_t o = (size_t)py * A.rx + px;
     int32_t fx, fy;
     const float4 posT = rep_texel(gb_pos, A.w, A.h, px, py, fx, fy);
@@ -335,48 +422,22 @@ __global__ void __launch_bounds__(256) k_atrous(TArgs A, int32_t histfix, const 
     const float P[3] = {posT.x, posT.y, posT.z};
     float N[3];
     dec_n(gb_n[(size_t)fy * A.w + fx], N);
-    float cCol[3], cVar, cLen;
-    read_input<ITER0>(o, hist, mom, in_cv, cCol, cVar, cLen);
-    if (ITER0 && histfix && cLen < 4.0f) {   // atrous.comp:87-112
-        float acc[3] = {cCol[0] * cLen, cCol[1] * cLen, cCol[2] * cLen};
-        float ws = cLen;
-        const float dscale = 1.0f + posT.w * 0.01f;
-        for (int dy = -3; dy <= 3; ++dy) {
-            const int32_t qy = py + dy * 3;
-            if (qy < 0 || qy >= A.ry) continue;
-            for (int dx = -3; dx <= 3; ++dx) {
-                const int32_t qx = px + dx * 3;
-                if ((dx == 0 && dy == 0) || qx < 0 || qx >= A.rx) continue;
-                const size_t q = (size_t)qy * A.rx + qx;
-                const uint4 g = guide[q];
-                if (g.w == 0u) continue;
-                float gN[3];
-                dec_n(g.w, gN);
-                const float d[3] = {__uint_as_float(g.x) - P[0], __uint_as_float(g.y) - P[1], __uint_as_float(g.z) - P[2]};
-                float hv[4];
-                h4(hist[q], hv);
-                const float wt = t_pow16(gmax(dot3(N, gN), 0.0f)) * t_exp(-fabsf(dot3(d, N)) / dscale) * hv[3];
-                acc[0] = acc[0] + hv[0] * wt; acc[1] = acc[1] + hv[1] * wt; acc[2] = acc[2] + hv[2] * wt;
-                ws = ws + wt;
-            }
-        }
-        if (ws > 0.0f) { cCol[0] = acc[0] / ws; cCol[1] = acc[1] / ws; cCol[2] = acc[2] / ws; }
-    }
+    const int32_t ci = (ly + 2) * kStageP + (lx + 2);
+    float cCol[3] = {st[6 * T + ci], st[7 * T + ci], st[8 * T + ci]};
+    float cVar = st[9 * T + ci];
+    const float cLen = ITER0 ? st[12 * T + ci] : 0.0f;
+    if (ITER0 && histfix && cLen < 4.0f) history_fix(A, px, py, posT, P, N, hist, guide, cLen, cCol);
     const float cLum = lumi(cCol[0], cCol[1], cCol[2]);
-    float vg;   // centreVariance (atrous.comp:51-71)
-    if (!ITER0) {
-        vg = cVar;
-    } else if (cLen >= 4.0f) {
+    float vg;   // centreVariance (atrous.comp:51-71); level 0 has step 1, so the 3x3 texels are staged
+    if (!ITER0 || cLen >= 4.0f) {
         vg = cVar;
     } else {
         float s = 0.0f, s2 = 0.0f, n = 0.0f;
         for (int dy = -1; dy <= 1; ++dy) {
-            const int32_t qy = imin(imax(py + dy, 0), A.ry - 1);
+            const int32_t qy = imin(imax(py + dy, 0), A.ry - 1) - Y0 + 2;
             for (int dx = -1; dx <= 1; ++dx) {
-                const int32_t qx = imin(imax(px + dx, 0), A.rx - 1);
-                float hv[4];
-                h4(hist[(size_t)qy * A.rx + qx], hv);
-                const float l = lumi(hv[0], hv[1], hv[2]);
+                const int32_t qx = imin(imax(px + dx, 0), A.rx - 1) - X0 + 2;
+                const float l = st[10 * T + qy * kStageP + qx];
                 s = s + l;
                 s2 = s2 + l * l;
                 n = n + 1.0f;
@@ -392,29 +453,25 @@ __global__ void __launch_bounds__(256) k_atrous(TArgs A, int32_t histfix, const 
     float vAcc = cVar * k00 * k00;
     float wSum = k00;
     const float depthScale = 1.0f * (1.0f + posT.w * 0.01f);
+    const float zinv = 1.0f / depthScale, linv = 1.0f / lumPhi;
     const float h5[3] = {0.375f, 0.25f, 0.0625f};
 #pragma unroll
     for (int dy = -2; dy <= 2; ++dy) {
-        const int32_t qy = py + dy * A.step;
-        if (qy < 0 || qy >= A.ry) continue;
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
-            const int32_t qx = px + dx * A.step;
-            if ((dx == 0 && dy == 0) || qx < 0 || qx >= A.rx) continue;
-            const size_t q = (size_t)qy * A.rx + qx;
-            const uint4 g = guide[q];
-            if (g.w == 0u) continue;
-            float qn[3];
-            dec_n(g.w, qn);
-            float qc[3], qv, qlen;
-            read_input<ITER0>(q, hist, mom, in_cv, qc, qv, qlen);
-            if (ITER0 && qlen < 4.0f) qv = gmax(qv, vg);
-            const float qLum = lumi(qc[0], qc[1], qc[2]);
+            if (dx == 0 && dy == 0) continue;
+            const int32_t li = ci + dy * kStageP + dx;
+            if (st[11 * T + li] != 0.0f) continue;   // outside the image or a sky guide
+            const float qn[3] = {st[3 * T + li], st[4 * T + li], st[5 * T + li]};
+            const float qc[3] = {st[6 * T + li], st[7 * T + li], st[8 * T + li]};
+            float qv = st[9 * T + li];
+            if (ITER0 && st[12 * T + li] < 4.0f) qv = gmax(qv, vg);
+            const float qLum = st[10 * T + li];
             const float kern = h5[dx < 0 ? -dx : dx] * h5[dy < 0 ? -dy : dy];
             const float wN = t_pow64(gmax(dot3(N, qn), 0.0f));
-            const float d[3] = {__uint_as_float(g.x) - P[0], __uint_as_float(g.y) - P[1], __uint_as_float(g.z) - P[2]};
-            const float wZ = t_exp(-fabsf(dot3(d, N)) / depthScale);
-            const float wL = t_exp(-fabsf(cLum - qLum) / lumPhi);
+            const float d[3] = {st[li] - P[0], st[T + li] - P[1], st[2 * T + li] - P[2]};
+            const float wZ = t_exp(-fabsf(dot3(d, N)) * zinv);
+            const float wL = t_exp(-fabsf(cLum - qLum) * linv);
             const float wt = kern * wN * wZ * wL;
             cAcc[0] = cAcc[0] + wt * qc[0]; cAcc[1] = cAcc[1] + wt * qc[1]; cAcc[2] = cAcc[2] + wt * qc[2];
             vAcc = vAcc + wt * wt * qv;
@@ -608,14 +665,17 @@ extern "C" int vf_gi_atrous(vf_ctx *ctx, int32_t w, int32_t h, int32_t levels, i
         A.step = 1 << i;
         uint2 *dst = (uint2 *)((i & 1) ? filt1 : filt0);
         const uint2 *src = (const uint2 *)((i & 1) ? filt0 : filt1);
+        const Lattice g = lattice_of(A.rx, A.ry, A.step);
+        const int32_t hf = histfix ? 1 : 0;
+        const float4 *gp = (const float4 *)gb_pos, *gm = (const float4 *)gi_mom;
+        const uint2 *gh = (const uint2 *)gi_hist;
+        const uint4 *gg = (const uint4 *)guide;
         if (i == 0)
-            hipLaunchKernelGGL(k_atrous<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, A, histfix ? 1 : 0,
-                               (const float4 *)gb_pos, gb_normal, (const float4 *)gi_mom, (const uint2 *)gi_hist,
-                               (const uint4 *)guide, (const uint2 *)nullptr, dst);
+            hipLaunchKernelGGL(k_atrous<true>, dim3(g.nwg), dim3(256), 0, ctx->stream, A, g.stx, hf, gp, gb_normal, gm, gh, gg,
+                               (const uint2 *)nullptr, dst);
         else
-            hipLaunchKernelGGL(k_atrous<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, A, histfix ? 1 : 0,
-                               (const float4 *)gb_pos, gb_normal, (const float4 *)gi_mom, (const uint2 *)gi_hist,
-                               (const uint4 *)guide, src, dst);
+            hipLaunchKernelGGL(k_atrous<false>, dim3(g.nwg), dim3(256), 0, ctx->stream, A, g.stx, hf, gp, gb_normal, gm, gh, gg,
+                               src, dst);
         if ((rc = vf::launch_check(ctx, "k_atrous"))) return rc;
     }
     return vf::end_timed(ctx);
