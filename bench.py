@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the open-forest, fixed-world, GI-batch, learned-cache and uneven-batch lines")
+    ap.add_argument("--extras", default="temporal,forest,other",
+                    help="comma list of the extra measurements: temporal (the passes after the trace), forest (open "
+                         "forest with bodies), other (fixed world, GI batch, GI cache, probes, uneven batch)")
     ap.add_argument("--no-graph", action="store_true",
                     help="record no graphs at all (also the kernel-duration loop launches eagerly)")
     ap.add_argument("--graph-steps", action="store_true",
@@ -267,9 +270,13 @@ def main():
     for g in (g_step, g_two, g_prim, g_shad):
         g.close()
     if not args.no_extra:
-        result["temporal_1080p"] = temporal_config(ctx, stream, args, pu)
-        result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
-        result["other_configs"] = other_configs(ctx, args)
+        extras = set(args.extras.split(","))
+        if "temporal" in extras:
+            result["temporal_1080p"] = temporal_config(ctx, stream, args, pu)
+        if "forest" in extras:
+            result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
+        if "other" in extras:
+            result["other_configs"] = other_configs(ctx, args)
     ctx.close()
     print(json.dumps(result), flush=True)
 
@@ -380,7 +387,10 @@ def temporal_config(ctx, stream, args, pu_prev):
     out = {}
     total = 0.0
     for name, (fn, units) in calls.items():
-        ms = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
+        if args.no_graph:   # counter runs: plain launches
+            ms = eager_ms(stream, fn, reps * max(1, args.steps // reps))
+        else:
+            ms = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
         nbytes = TEMPORAL_BYTES[name] * units
         gbs = nbytes / (ms * 1e-3) / 1e9
         total += ms
@@ -391,6 +401,20 @@ def temporal_config(ctx, stream, args, pu_prev):
     out["history_reused"] = round(float((cnt > 1).float().mean().item()), 4)
     log(f"temporal passes 1080p: {out}")
     return out
+
+
+def eager_ms(stream, fn, k):
+    """Average time of fn over k back-to-back eager calls (HIP events on the launch stream)."""
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / k
 
 
 def graph_ms(ctx, stream, fn, k):

@@ -319,13 +319,15 @@ __device__ __forceinline__ void read_input(size_t q, const uint2 *hist, const fl
 }
 
 // GI_HISTFIX (atrous.comp:87-112): a texel with fewer than 4 frames of history takes the history-length
-// weighted mean of converged texels on its surface, 7x7 taps 3 texels apart.
-__device__ __forceinline__ void history_fix(const TArgs &A, int32_t px, int32_t py, float4 posT, const float P[3],
+// weighted mean of converged texels on its surface, 7x7 taps 3 texels apart.  Rare (disocclusions), but
+// a wavefront holding one such texel runs its 48 taps (~5000 instructions and their loads) on top of
+// the filter; such wavefronts set the first level's tail.
+__device__ __forceinline__ void history_fix(const TArgs &A, int32_t px, int32_t py, float posTw, const float P[3],
                                             const float N[3], const uint2 *hist, const uint4 *guide, float cLen,
                                             float cCol[3]) {
     float acc[3] = {cCol[0] * cLen, cCol[1] * cLen, cCol[2] * cLen};
     float ws = cLen;
-    const float dinv = 1.0f / (1.0f + posT.w * 0.01f);
+    const float dinv = 1.0f / (1.0f + posTw * 0.01f);
     for (int dy = -3; dy <= 3; ++dy) {
         const int32_t qy = py + dy * 3;
         if (qy < 0 || qy >= A.ry) continue;
@@ -411,22 +413,24 @@ __global__ void __launch_bounds__(256) k_atrous(TArgs A, int32_t stx, int32_t hi
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t lx = (wv & 1) * 8 + (lane & 7), ly = (wv >> 1) * 8 + (lane >> 3);
     const int32_t px = X0 + lx * step, py = Y0 + ly * step;
-    if (px >= A.rx || py >= A.ry) return;
+    const bool inside = px < A.rx && py < A.ry;
     const size_t o = (size_t)py * A.rx + px;
-    int32_t fx, fy;
-    const float4 posT = rep_texel(gb_pos, A.w, A.h, px, py, fx, fy);
-    if (posT.w < 0.0f) {
-        out_cv[o] = make_uint2(0u, 0u);
-        return;
-    }
+    int32_t fx = 0, fy = 0;
+    float4 posT = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    if (inside) posT = rep_texel(gb_pos, A.w, A.h, px, py, fx, fy);
+    const bool live = inside && posT.w >= 0.0f;
     const float P[3] = {posT.x, posT.y, posT.z};
-    float N[3];
-    dec_n(gb_n[(size_t)fy * A.w + fx], N);
+    float N[3] = {0.0f, 0.0f, 0.0f};
+    if (live) dec_n(gb_n[(size_t)fy * A.w + fx], N);
     const int32_t ci = (ly + 2) * kStageP + (lx + 2);
     float cCol[3] = {st[6 * T + ci], st[7 * T + ci], st[8 * T + ci]};
     float cVar = st[9 * T + ci];
     const float cLen = ITER0 ? st[12 * T + ci] : 0.0f;
-    if (ITER0 && histfix && cLen < 4.0f) history_fix(A, px, py, posT, P, N, hist, guide, cLen, cCol);
+    if (ITER0 && histfix && live && cLen < 4.0f) history_fix(A, px, py, posT.w, P, N, hist, guide, cLen, cCol);
+    if (!live) {
+        if (inside) out_cv[o] = make_uint2(0u, 0u);
+        return;
+    }
     const float cLum = lumi(cCol[0], cCol[1], cCol[2]);
     float vg;   // centreVariance (atrous.comp:51-71); level 0 has step 1, so the 3x3 texels are staged
     if (!ITER0 || cLen >= 4.0f) {
