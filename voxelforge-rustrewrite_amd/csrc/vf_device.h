@@ -145,32 +145,34 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     const float tDx = tD[0], tDy = tD[1], tDz = tD[2];
     const int32_t fA0 = faceA[0], fA1 = faceA[1], fA2 = faceA[2];
     const int32_t s0 = st[0], s1 = st[1], s2 = st[2];
+    // One exit per trip: the next step is computed first and committed only when the ray neither hits
+    // the current voxel nor leaves the brick with it (a step out of the brick is not used: the caller's
+    // brick DDA continues from its own state).
+#pragma unroll 1
     for (int j = 0; j < 26; ++j) {
         if (COUNT) { c.voxel++; c.bytes += 1; }
         const uint32_t idx = ((uint32_t)ly << 3) | (uint32_t)lx;   // bit of (lx, ly) in the 64-bit layer
         const uint64_t layer = ((uint64_t)whi << 32) | (uint64_t)wlo;
-        if ((layer >> idx) & 1ull) { hit = true; break; }
+        hit = ((layer >> idx) & 1ull) != 0ull;
         const bool lt_yz = tMy < tMz;
         const bool a0 = tMx < tMy && tMx < tMz;
         const bool a1 = !a0 && lt_yz;
         const bool a2 = !a0 && !lt_yz;
+        const int32_t nlx = lx + (a0 ? s0 : 0), nly = ly + (a1 ? s1 : 0), nlz = lz + (a2 ? s2 : 0);
+        const bool out = ((uint32_t)nlx | (uint32_t)nly | (uint32_t)nlz) > 7u;
+        if (hit | out) break;
         tv = a0 ? tMx : (a1 ? tMy : tMz);
         f2 = a0 ? fA0 : (a1 ? fA1 : fA2);
         const float nx = tMx + tDx, ny = tMy + tDy, nz = tMz + tDz;
         tMx = a0 ? nx : tMx;
         tMy = a1 ? ny : tMy;
         tMz = a2 ? nz : tMz;
-        lx += a0 ? s0 : 0;
-        ly += a1 ? s1 : 0;
-        lz += a2 ? s2 : 0;
-        if (((uint32_t)lx | (uint32_t)ly | (uint32_t)lz) > 7u) break;
-        if (a2) {
-            if (LDSM) {
-                wlo = sm[(lz * 2) * 256];
-                whi = sm[(lz * 2 + 1) * 256];
-            } else {
-                mask_layer(m0, m1, m2, m3, lz, wlo, whi);
-            }
+        lx = nlx; ly = nly; lz = nlz;
+        if (LDSM) {   // the layer of the new voxel, read every trip (no branch; unchanged unless a z step)
+            wlo = sm[(lz * 2) * 256];
+            whi = sm[(lz * 2 + 1) * 256];
+        } else if (a2) {
+            mask_layer(m0, m1, m2, m3, lz, wlo, whi);
         }
     }
     if (hit) {
