@@ -82,9 +82,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the open-forest, fixed-world, GI-batch, learned-cache and uneven-batch lines")
-    ap.add_argument("--extras", default="temporal,forest,other",
-                    help="comma list of the extra measurements: temporal (the passes after the trace), forest (open "
-                         "forest with bodies), other (fixed world, GI batch, GI cache, probes, uneven batch)")
+    ap.add_argument("--extras", default="temporal,pack,forest,other",
+                    help="comma list of the extra measurements: temporal (the passes after the trace), pack (the "
+                         "world packed on the device), forest (open forest with bodies), other (fixed world, GI "
+                         "batch, GI cache, probes, uneven batch)")
     ap.add_argument("--no-graph", action="store_true",
                     help="record no graphs at all (also the kernel-duration loop launches eagerly)")
     ap.add_argument("--graph-steps", action="store_true",
@@ -273,12 +274,42 @@ def main():
         extras = set(args.extras.split(","))
         if "temporal" in extras:
             result["temporal_1080p"] = temporal_config(ctx, stream, args, pu)
+        if "pack" in extras:
+            result["world_pack"] = pack_config(ctx, world, scene)
         if "forest" in extras:
             result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
         if "other" in extras:
             result["other_configs"] = other_configs(ctx, args)
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def pack_config(ctx, world, scene):
+    """The headline world uploaded twice: its host-packed arrays (vf_world_upload: grid and atlas as the
+    reference's CPU pack leaves them), then the reference's CPU-side form (grid of brick references, the
+    bricks in another order) packed on the device (vf_world_pack); the two device worlds must be equal.
+    Wall times include the host-to-device copies and the whole derived-data build."""
+    a = world.arrays()
+    grid, atlas = np.asarray(a["grid"], np.uint32), np.asarray(a["atlas"], np.uint8).reshape(-1, 512)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(atlas.shape[0])
+    src = grid.copy()
+    det = grid >= 256
+    src[det] = 256 + np.argsort(perm)[grid[det] - 256]
+    bricks = np.ascontiguousarray(atlas[perm])
+    out = {"bricks": int(atlas.shape[0]), "atlas_mb": round(atlas.nbytes / 1e6, 1)}
+    t0 = time.perf_counter()
+    ctx.upload_arrays(a)
+    out["upload_host_packed_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    g_up = ctx.world_read(0)[0]
+    t0 = time.perf_counter()
+    slot = ctx.pack_world(a["world_bricks"], a["world_win"], src, bricks, a["palette"])
+    out["pack_on_device_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    g_pk = ctx.world_read(0)[0]
+    out["same_grid"] = bool(np.array_equal(g_up, g_pk) and np.array_equal(slot, perm))
+    ctx.set_scene(scene)
+    log(f"world pack: {out}")
+    return out
 
 
 def open_forest_config(ctx, stream, args, hits, gbp, gbn, sh):

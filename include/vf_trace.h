@@ -159,6 +159,27 @@ int vf_memset_device(vf_ctx *ctx, void *d_dst, int value, size_t bytes);
 /* GpuWorld::upload (gpu_world.rs:228): grid, atlas, masks, coarse pyramid and palette, then the full
  * brick distance field (gpu_world.rs:422) and the sky-top map (grasstop.comp skyTop). */
 int vf_world_upload(vf_ctx *ctx, const vf_world_desc *desc);
+/* The world as the reference's CPU side holds it (World::grid and SparseVolume::bricks): grid cells
+ * 0 empty, 1-255 uniform, 256 + i = detailed brick i of `bricks` (512 voxels each, Brick::vox order). */
+typedef struct vf_world_src {
+    int32_t world_bricks[3];
+    int32_t world_win[4];
+    const uint32_t *grid;        /* [wbx*wby*wbz], (z*wby + y)*wbx + x */
+    const uint8_t *bricks;       /* [n_bricks*512] */
+    uint32_t n_bricks;
+    uint32_t atlas_slots;        /* atlas capacity; 0 = as many slots as the grid names distinct bricks */
+    const vf_material *palette;  /* [256] */
+} vf_world_src;
+/* GpuWorld::upload's pack (gpu_world.rs:228-330 with encode_brick :191-214 and gpu_voxels :58-67) on the
+ * device, then what vf_world_upload builds: each detailed brick gets the next atlas slot at its first
+ * cell in grid order (so a brick named by several cells shares one slot), its voxels go to the slot with
+ * TORCH_FLAME as air, `masks` of the slot = brick_mask of its voxels (sparse_volume.rs:48-63); past the
+ * capacity a brick's cells take its majority material (gpu_world.rs:202-213).  The coarse pyramid
+ * comes from the source grid.  slot_of_brick (nullable, [n_bricks]): each brick's slot (the
+ * reference's Brick::gpu_slot), -1 for a brick the grid does not name or that did not fit. */
+int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot_of_brick);
+/* Copies the device world's encoded grid ([wbx*wby*wbz]) and the first n_slots atlas slots back. */
+int vf_world_read(vf_ctx *ctx, uint32_t *grid, uint8_t *atlas, uint32_t n_slots, uint32_t *masks);
 /* GpuWorld::sync (gpu_world.rs:985) minus the Vulkan staging: set grid cells and (re)write atlas
  * slots.  cells: grid indices + new values; slots: slot ids + 512 B each.  The caller then rebuilds
  * the distance field over the dirty box with vf_build_brickdist, as the reference does. */

@@ -652,3 +652,70 @@ def gbuffer_motion(w, h, pass_ubo, gb_pos, body=None, bodies=None):
                              None if bd is None else bd.ctypes.data, None if hb is None else hb.ctypes.data, nb,
                              mot.ctypes.data, dl.ctypes.data)
     return mot, dl
+
+
+# ---- world pack (GpuWorld::upload's CPU pack) ----------------------------------------------------------
+TORCH_FLAME = 254
+
+
+def majority_material(vox):
+    """encode_brick's full-atlas value (gpu_world.rs:202-213): the most frequent of materials 1..255 other
+    than TORCH_FLAME (strictly more to replace, starting from 1), if it fills half the brick, else 0."""
+    count = np.bincount(np.asarray(vox, np.uint8).reshape(-1), minlength=256)
+    best = 1
+    for m in range(2, 256):
+        if count[m] > count[best] and m != TORCH_FLAME:
+            best = m
+    return best if count[best] * 2 >= 512 else 0
+
+
+def pack_world_loop(grid_src, bricks, atlas_slots=0):
+    """The reference's loop (gpu_world.rs:245-263 + encode_brick :191-214) transcribed for small inputs:
+    cells in order; a detailed cell's brick takes the next slot unless it has one; none left: majority."""
+    grid_src = np.asarray(grid_src, np.uint32)
+    bricks = np.asarray(bricks, np.uint8).reshape(-1, 512)
+    need = len({int(v) - 256 for v in grid_src if v >= 256})
+    cap = atlas_slots or need
+    gpu_slot = [-1] * bricks.shape[0]
+    n_used = 0
+    out = np.zeros_like(grid_src)
+    for gi, v in enumerate(grid_src):
+        if v < 256:
+            out[gi] = v
+            continue
+        b = int(v) - 256
+        if gpu_slot[b] < 0 and n_used < cap:
+            gpu_slot[b] = n_used
+            n_used += 1
+        out[gi] = 256 + gpu_slot[b] if gpu_slot[b] >= 0 else majority_material(bricks[b])
+    return out, np.asarray(gpu_slot, np.int64), cap
+
+
+def pack_world(grid_src, bricks, atlas_slots=0):
+    """GpuWorld::upload's pack, vectorised: (encoded grid u32, atlas [cap, 512] with TORCH_FLAME as air,
+    brick masks [cap, 16] of the bricks' own voxels, slot per brick (-1 none))."""
+    grid_src = np.asarray(grid_src, np.uint32)
+    bricks = np.asarray(bricks, np.uint8).reshape(-1, 512)
+    det = np.nonzero(grid_src >= 256)[0]
+    ref = grid_src[det].astype(np.int64) - 256
+    uniq, first = np.unique(ref, return_index=True)
+    order = np.argsort(first, kind="stable")          # bricks by their first cell in grid order
+    named = uniq[order]
+    cap = atlas_slots or len(named)
+    slot = np.full(bricks.shape[0], -1, np.int64)
+    fit = named[:cap]
+    slot[fit] = np.arange(len(fit))
+    out = grid_src.copy()
+    s = slot[ref]
+    enc = np.where(s >= 0, 256 + s, 0).astype(np.uint32)
+    for k in np.nonzero(s < 0)[0]:
+        enc[k] = majority_material(bricks[ref[k]])
+    out[det] = enc
+    atlas = np.zeros((cap, 512), np.uint8)
+    masks = np.zeros((cap, 16), np.uint32)
+    if len(fit):
+        atlas[:len(fit)] = np.where(bricks[fit] == TORCH_FLAME, 0, bricks[fit])
+        for i, b in enumerate(fit):
+            masks[i] = brick_mask(bricks[b])
+    return out, atlas, masks, slot
+

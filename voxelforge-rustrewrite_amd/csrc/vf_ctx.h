@@ -91,4 +91,8 @@ int check_bodies(vf_ctx *ctx, const char *what);
 void free_bodies(vf_ctx *ctx);
 void slot_occupancy(const uint8_t *vox, uint64_t out[8]);
 void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n_slots);
+// the world's buffers and derived data (vf_trace.hip), shared by vf_world_upload and vf_world_pack
+int world_prepare(vf_ctx *ctx, const int32_t wb[3], const int32_t win[4], uint32_t n_slots, const char *what);
+int world_finish(vf_ctx *ctx, bool coarse_given);
+int build_coarse(vf_ctx *ctx, const uint32_t *d_grid);
 }  // namespace vf

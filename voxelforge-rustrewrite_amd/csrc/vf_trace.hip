@@ -913,6 +913,116 @@ int trace_gbuffer(vf_ctx *ctx, const char *what, const vf_pass_ubo *p, int32_t w
 }
 }  // namespace
 
+namespace vf {
+
+// The device world's shape and buffers for a world of wb bricks with n_slots atlas slots: checks, frees
+// the previous world, fills the DevWorld description and allocates every buffer (vf_world_upload and
+// vf_world_pack share it).
+int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uint32_t n_slots, const char *what) {
+    for (int a = 0; a < 3; ++a)
+        if (wb_in[a] <= 0 || wb_in[a] > 4096)
+            return vf::fail(ctx, VF_E_INVALID, "%s: world_bricks[%d] = %d out of range", what, a, wb_in[a]);
+    const int ring = win[3] != 0;
+    if (ring) {
+        const int32_t x = wb_in[0], z = wb_in[2];
+        if ((x & (x - 1)) || (z & (z - 1))) return vf::fail(ctx, VF_E_INVALID, "%s: ring mode needs power-of-two x/z brick dims", what);
+    }
+    if (n_slots > (1u << 24)) return vf::fail(ctx, VF_E_INVALID, "%s: %u atlas slots > 2^24", what, n_slots);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_world(ctx);
+    DevWorld &W = ctx->W;
+    W = DevWorld{};
+    for (int a = 0; a < 3; ++a) {
+        W.wb[a] = wb_in[a];
+        W.tdim[a] = (W.wb[a] + 3) / 4;
+        ctx->c1d[a] = (W.wb[a] + 3) / 4;
+        ctx->c2d[a] = (W.wb[a] + 15) / 16;
+    }
+    W.ring = ring;
+    W.minb[0] = ring ? win[0] : 0;
+    W.minb[1] = 0;
+    W.minb[2] = ring ? win[2] : 0;
+    W.use_dist = 1;
+    W.max_outer = 3 * (W.wb[0] + W.wb[1] + W.wb[2]) + 4;
+    const size_t n = (size_t)W.wb[0] * W.wb[1] * W.wb[2];
+    ctx->n_cells = n;
+    ctx->n_cells_tiled = (size_t)W.tdim[0] * W.tdim[1] * W.tdim[2] * 64;
+    ctx->n_slots = n_slots;
+    const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
+    const size_t slots = std::max<size_t>(n_slots, 1);
+    const size_t nsky = (size_t)W.wb[0] * 8 * W.wb[2] * 8;
+#define VF_ALLOC(ptr, bytes)                                                                                     \
+    do {                                                                                                         \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                                                     \
+        if (e_ != hipSuccess) {                                                                                  \
+            free_world(ctx);                                                                                     \
+            return vf::fail(ctx, VF_E_OOM, "%s: hipMalloc(%zu): %s", what, (size_t)(bytes), hipGetErrorString(e_)); \
+        }                                                                                                        \
+    } while (0)
+    VF_ALLOC(ctx->grid, n * 4);
+    VF_ALLOC(ctx->cells, ctx->n_cells_tiled * 8);
+    VF_ALLOC(ctx->atlas, slots * 512);
+    VF_ALLOC(ctx->occ, slots * 64);
+    VF_ALLOC(ctx->occdry, slots * 64);
+    VF_ALLOC(ctx->masks_ref, slots * 64);
+    for (auto &p : ctx->dist) VF_ALLOC(p, n);
+    VF_ALLOC(ctx->c1, n1);
+    VF_ALLOC(ctx->c2, n2);
+    VF_ALLOC(ctx->skytop, nsky * 2);
+    VF_ALLOC(ctx->palette, 256 * 48);
+#undef VF_ALLOC
+    VF_HIP(ctx, hipMemsetAsync(ctx->cells, 0, ctx->n_cells_tiled * 8, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->atlas, 0, slots * 512, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->masks_ref, 0, slots * 64, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->occ, 0, slots * 64, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->occdry, 0, slots * 64, ctx->stream));
+    return VF_OK;
+}
+
+// Everything derived from the grid and the atlas once both are on the device: occupancy masks, the
+// coarse pyramid (unless the caller's was copied), the distance fields, the cell words, the sky-top map.
+int world_finish(vf_ctx *ctx, bool coarse_given) {
+    int rc;
+    DevWorld &W = ctx->W;
+    if (ctx->n_slots) {
+        hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)ctx->n_slots * 16)), dim3(256), 0, ctx->stream, ctx->atlas, nullptr,
+                           0u, ctx->n_slots, ctx->occ, ctx->occdry);
+        if ((rc = launch_check(ctx, "occupancy masks"))) return rc;
+    }
+    if (!coarse_given && (rc = rebuild_coarse(ctx))) return rc;
+    W.cells = ctx->cells;
+    W.atlas = ctx->atlas;
+    W.occ = reinterpret_cast<const uint4 *>(ctx->occ);
+    W.occdry = reinterpret_cast<const uint4 *>(ctx->occdry);
+    W.skytop = ctx->skytop;
+    W.palette = ctx->palette;
+    const int32_t lo[3] = {W.minb[0], 0, W.minb[2]};
+    const int32_t hi[3] = {W.minb[0] + W.wb[0] - 1, W.wb[1] - 1, W.minb[2] + W.wb[2] - 1};
+    if ((rc = distance_field(ctx, lo, hi, lo, hi))) return rc;
+    if ((rc = repack(ctx))) return rc;
+    if ((rc = rebuild_skytop(ctx))) return rc;
+    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->has_world = true;
+    return VF_OK;
+}
+
+// The coarse pyramid of a grid that is not (yet) the world's: coarse1 = any non-empty brick, coarse2 =
+// any coarse1 cell (gpu_world.rs:300-313).
+int build_coarse(vf_ctx *ctx, const uint32_t *d_grid) {
+    const int32_t *wb = ctx->W.wb;
+    const int64_t n1 = (int64_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2];
+    const int64_t n2 = (int64_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
+    hipLaunchKernelGGL(k_coarse1, dim3(blocks_for(n1)), dim3(256), 0, ctx->stream, d_grid, wb[0], wb[1], wb[2], ctx->c1d[0],
+                       ctx->c1d[1], ctx->c1d[2], ctx->c1);
+    hipLaunchKernelGGL(k_coarse2, dim3(blocks_for(n2)), dim3(256), 0, ctx->stream, ctx->c1, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2],
+                       ctx->c2d[0], ctx->c2d[1], ctx->c2d[2], ctx->c2);
+    return launch_check(ctx, "coarse pyramid");
+}
+
+}  // namespace vf
+
 extern "C" {
 
 int vf_abi_version(void) { return VF_ABI_VERSION; }
@@ -1042,70 +1152,19 @@ int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
     if (!ctx || !d) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: null argument");
     if (int rc = no_capture(ctx, "vf_world_upload")) return rc;
     if (!d->grid || !d->palette) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: grid and palette are required");
+    if (d->n_slots && !d->atlas) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: n_slots > 0 without an atlas");
     for (int a = 0; a < 3; ++a)
         if (d->world_bricks[a] <= 0 || d->world_bricks[a] > 4096)
             return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: world_bricks[%d] = %d out of range", a, d->world_bricks[a]);
-    const int ring = d->world_win[3] != 0;
-    if (ring) {
-        const int32_t x = d->world_bricks[0], z = d->world_bricks[2];
-        if ((x & (x - 1)) || (z & (z - 1)))
-            return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: ring mode needs power-of-two x/z brick dims");
-    }
-    if (d->n_slots && !d->atlas) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: n_slots > 0 without an atlas");
-    if (d->n_slots > (1u << 24)) return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: n_slots %u > 2^24", d->n_slots);
     const size_t n = (size_t)d->world_bricks[0] * d->world_bricks[1] * d->world_bricks[2];
     for (size_t i = 0; i < n; ++i)
         if (d->grid[i] >= 256u && d->grid[i] - 256u >= d->n_slots)
             return vf::fail(ctx, VF_E_INVALID, "vf_world_upload: grid[%zu] = %u names slot %u of %u", i, d->grid[i],
                             d->grid[i] - 256u, d->n_slots);
-    int rc = set_device(ctx);
+    int rc = vf::world_prepare(ctx, d->world_bricks, d->world_win, d->n_slots, "vf_world_upload");
     if (rc) return rc;
-    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    free_world(ctx);
-    DevWorld &W = ctx->W;
-    W = DevWorld{};
-    for (int a = 0; a < 3; ++a) {
-        W.wb[a] = d->world_bricks[a];
-        W.tdim[a] = (W.wb[a] + 3) / 4;
-        ctx->c1d[a] = (W.wb[a] + 3) / 4;
-        ctx->c2d[a] = (W.wb[a] + 15) / 16;
-    }
-    W.ring = ring;
-    W.minb[0] = ring ? d->world_win[0] : 0;
-    W.minb[1] = 0;
-    W.minb[2] = ring ? d->world_win[2] : 0;
-    W.use_dist = 1;
-    W.max_outer = 3 * (W.wb[0] + W.wb[1] + W.wb[2]) + 4;
-    ctx->n_cells = n;
     ctx->host_grid.assign(d->grid, d->grid + n);
     vf::host_occupancy(ctx->host_occ, d->atlas, d->n_slots);
-    ctx->n_cells_tiled = (size_t)W.tdim[0] * W.tdim[1] * W.tdim[2] * 64;
-    ctx->n_slots = d->n_slots;
-    const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
-    const size_t slots = std::max<size_t>(d->n_slots, 1);
-    const size_t nsky = (size_t)W.wb[0] * 8 * W.wb[2] * 8;
-#define VF_ALLOC(ptr, bytes)                                                     \
-    do {                                                                         \
-        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                     \
-        if (e_ != hipSuccess) {                                                  \
-            free_world(ctx);                                                     \
-            return vf::fail(ctx, VF_E_OOM, "vf_world_upload: hipMalloc(%zu): %s", \
-                            (size_t)(bytes), hipGetErrorString(e_));             \
-        }                                                                        \
-    } while (0)
-    VF_ALLOC(ctx->grid, n * 4);
-    VF_ALLOC(ctx->cells, ctx->n_cells_tiled * 8);
-    VF_ALLOC(ctx->atlas, slots * 512);
-    VF_ALLOC(ctx->occ, slots * 64);
-    VF_ALLOC(ctx->occdry, slots * 64);
-    VF_ALLOC(ctx->masks_ref, slots * 64);
-    for (auto &p : ctx->dist) VF_ALLOC(p, n);
-    VF_ALLOC(ctx->c1, n1);
-    VF_ALLOC(ctx->c2, n2);
-    VF_ALLOC(ctx->skytop, nsky * 2);
-    VF_ALLOC(ctx->palette, 256 * 48);
-#undef VF_ALLOC
-    VF_HIP(ctx, hipMemsetAsync(ctx->cells, 0, ctx->n_cells_tiled * 8, ctx->stream));
     VF_HIP(ctx, hipMemcpyAsync(ctx->grid, d->grid, n * 4, hipMemcpyHostToDevice, ctx->stream));
     if (d->n_slots) {
         // TORCH_FLAME -> air as the GPU atlas sees it (gpu_world.rs:58-67)
@@ -1116,37 +1175,15 @@ int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
         VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (d->masks)
             VF_HIP(ctx, hipMemcpyAsync(ctx->masks_ref, d->masks, (size_t)d->n_slots * 64, hipMemcpyHostToDevice, ctx->stream));
-    } else {
-        VF_HIP(ctx, hipMemsetAsync(ctx->atlas, 0, 512, ctx->stream));
     }
-    VF_HIP(ctx, hipMemsetAsync(ctx->occ, 0, slots * 64, ctx->stream));
-    VF_HIP(ctx, hipMemsetAsync(ctx->occdry, 0, slots * 64, ctx->stream));
     VF_HIP(ctx, hipMemcpyAsync(ctx->palette, d->palette, 256 * 48, hipMemcpyHostToDevice, ctx->stream));
-    if (d->n_slots) {
-        hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)d->n_slots * 16)), dim3(256), 0, ctx->stream, ctx->atlas, nullptr, 0u,
-                           d->n_slots, ctx->occ, ctx->occdry);
-        if ((rc = launch_check(ctx, "occupancy masks"))) return rc;
-    }
-    if (d->coarse1 && d->coarse2) {
+    const bool coarse = d->coarse1 && d->coarse2;
+    if (coarse) {
+        const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
         VF_HIP(ctx, hipMemcpyAsync(ctx->c1, d->coarse1, n1, hipMemcpyHostToDevice, ctx->stream));
         VF_HIP(ctx, hipMemcpyAsync(ctx->c2, d->coarse2, n2, hipMemcpyHostToDevice, ctx->stream));
-    } else if ((rc = rebuild_coarse(ctx))) {
-        return rc;
     }
-    W.cells = ctx->cells;
-    W.atlas = ctx->atlas;
-    W.occ = reinterpret_cast<const uint4 *>(ctx->occ);
-    W.occdry = reinterpret_cast<const uint4 *>(ctx->occdry);
-    W.skytop = ctx->skytop;
-    W.palette = ctx->palette;
-    const int32_t lo[3] = {W.minb[0], 0, W.minb[2]};
-    const int32_t hi[3] = {W.minb[0] + W.wb[0] - 1, W.wb[1] - 1, W.minb[2] + W.wb[2] - 1};
-    if ((rc = distance_field(ctx, lo, hi, lo, hi))) return rc;
-    if ((rc = repack(ctx))) return rc;
-    if ((rc = rebuild_skytop(ctx))) return rc;
-    VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->has_world = true;
-    return VF_OK;
+    return vf::world_finish(ctx, coarse);
 }
 
 int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_val, uint32_t n_cells, const uint32_t *slot_idx,

@@ -16,7 +16,7 @@ import numpy as np
 from .abi import (BVH_NODE_DTYPE, BvhNode, GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams,
                   GiCacheStats, Hit, HIT_DTYPE, Material, OBJ_HEADER_DTYPE, ObjHeader, PassUbo, QueueCfg, Ray, RAY_DTYPE,
                   Request, SceneUbo, TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH,
-                  VF_TUNE_GI_BATCH, WorldDesc)
+                  VF_TUNE_GI_BATCH, WorldDesc, WorldSrc)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
            "RAY_DTYPE", "GI_RECORD_DTYPE", "OBJ_HEADER_DTYPE", "BVH_NODE_DTYPE", "GiCache", "fly_camera", "sun_dir",
@@ -52,6 +52,8 @@ _SIGS = {
     "vf_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "vf_memset_device": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
     "vf_world_upload": (C.c_int, [_P, C.POINTER(WorldDesc)]),
+    "vf_world_pack": (C.c_int, [_P, C.POINTER(WorldSrc), _P]),
+    "vf_world_read": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
     "vf_world_update": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32]),
     "vf_build_brickdist": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32)]),
@@ -420,6 +422,35 @@ class Context:
         self._ck(lib().vf_world_upload(self._h, C.byref(d)))
         self.world_bricks = wb
         del keep
+
+    def pack_world(self, world_bricks, world_win, grid, bricks, palette, atlas_slots=0):
+        """vf_world_pack: the world from the CPU side's grid (0 / 1-255 / 256 + brick index) and bricks
+        ([n, 512] u8), packed on the device.  Returns each brick's slot (-1: unnamed or did not fit)."""
+        wb = tuple(int(x) for x in world_bricks)
+        g = np.ascontiguousarray(grid, dtype=np.uint32)
+        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1)
+        pal = np.ascontiguousarray(palette, dtype=np.float32).reshape(256, 12)
+        d = WorldSrc()
+        d.world_bricks[:] = wb
+        d.world_win[:] = tuple(int(x) for x in world_win)
+        d.grid = g.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.bricks = b.ctypes.data_as(C.POINTER(C.c_uint8)) if b.size else None
+        d.n_bricks = b.size // 512
+        d.atlas_slots = int(atlas_slots)
+        d.palette = pal.ctypes.data_as(C.POINTER(Material))
+        slots = np.zeros(max(d.n_bricks, 1), np.int32)
+        self._ck(lib().vf_world_pack(self._h, C.byref(d), slots.ctypes.data))
+        self.world_bricks = wb
+        return slots[:d.n_bricks]
+
+    def world_read(self, n_slots):
+        """The device world's encoded grid, its first n_slots atlas slots and their brick masks."""
+        wb = self.world_bricks
+        grid = np.zeros(wb[0] * wb[1] * wb[2], np.uint32)
+        atlas = np.zeros((max(n_slots, 1), 512), np.uint8)
+        masks = np.zeros((max(n_slots, 1), 16), np.uint32)
+        self._ck(lib().vf_world_read(self._h, grid.ctypes.data, atlas.ctypes.data, int(n_slots), masks.ctypes.data))
+        return grid, atlas[:n_slots], masks[:n_slots]
 
     # bodies (GpuWorld::init_objects / update_objects / sync_objects)
     def objects_upload(self, headers, grid):

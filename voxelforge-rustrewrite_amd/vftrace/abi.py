@@ -98,6 +98,15 @@ class WorldDesc(C.Structure):
     ]
 
 
+class WorldSrc(C.Structure):
+    """vf_world_src: the world as the reference's CPU side holds it (grid with brick references, bricks)."""
+    _fields_ = [
+        ("world_bricks", i3), ("world_win", i4),
+        ("grid", C.POINTER(C.c_uint32)), ("bricks", C.POINTER(C.c_uint8)), ("n_bricks", C.c_uint32),
+        ("atlas_slots", C.c_uint32), ("palette", C.POINTER(Material)),
+    ]
+
+
 class GenParams(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("seed", C.c_uint32), ("world_bricks", i3), ("chunks", C.c_int32),
