@@ -178,6 +178,24 @@ typedef struct vf_world_src {
  * comes from the source grid.  slot_of_brick (nullable, [n_bricks]): each brick's slot (the
  * reference's Brick::gpu_slot), -1 for a brick the grid does not name or that did not fit. */
 int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot_of_brick);
+/* One chunk of GpuWorld::sync_chunks (gpu_world.rs:1194-1376): 16 x wby x 16 bricks of a ring world. */
+typedef struct vf_chunk {
+    int32_t cx, cz;              /* chunk coordinates (16 bricks a chunk, absolute) */
+    int32_t loaded;              /* World::chunk_loaded: 0 = evicted and not refilled (its cells are cleared) */
+    const uint32_t *cells;       /* loaded: [16*wby*16], (z*wby + y)*16 + x: 0 / 1-255 / 256 + i = brick i of `bricks` */
+} vf_chunk;
+/* GpuWorld::sync_chunks on the device (ring worlds).  `released` (World::released_gpu_slots) slots that
+ * hold world bricks go back to the free list, highest first, so pops hand out ascending slots.  Chunks
+ * are taken in order, once per ring slot.  A loaded chunk's cells are encoded in (z, y, x) order: a brick
+ * with gpu_slot[i] < 0 takes a slot from the free list, else the next unused one, else its cells take
+ * its majority material (encode_brick); every brick it names is written to its slot (TORCH_FLAME as air)
+ * with its brick masks; its coarse cells come from its cells.  An unloaded chunk's cells and coarse cells
+ * are cleared.  Then the distance field over each chunk's box +-16 bricks, the cell words and the
+ * sky-top map.  gpu_slot: [n_bricks], in and out (Brick::gpu_slot). */
+int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t n_chunks, const uint8_t *bricks, uint32_t n_bricks,
+                         int32_t *gpu_slot, const int32_t *released, int32_t n_released);
+/* The atlas bookkeeping: slots handed out, the free list's length, capacity. */
+int vf_world_atlas_state(vf_ctx *ctx, uint32_t *used, uint32_t *n_free, uint32_t *capacity);
 /* Copies the device world's encoded grid ([wbx*wby*wbz]) and the first n_slots atlas slots back. */
 int vf_world_read(vf_ctx *ctx, uint32_t *grid, uint8_t *atlas, uint32_t n_slots, uint32_t *masks);
 /* GpuWorld::sync (gpu_world.rs:985) minus the Vulkan staging: set grid cells and (re)write atlas
@@ -194,6 +212,8 @@ int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], co
 int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry);
 /* Read back the sky-top map (R16UI, (wbx*8) x (wbz*8)), host. */
 int vf_read_skytop(vf_ctx *ctx, uint16_t *skytop);
+/* Read back the coarse occupancy pyramid (coarse1: 4^3 bricks a cell, coarse2: 16^3), host. */
+int vf_read_coarse(vf_ctx *ctx, uint8_t *coarse1, uint8_t *coarse2);
 
 /* ---- bodies (dynamic voxel objects) -------------------------------------------------------- */
 /* Body header, std430, 240 B (gpu_world.rs:34-47 ObjHeader; common.glsl:65-72).  Matrices are

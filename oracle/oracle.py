@@ -719,3 +719,123 @@ def pack_world(grid_src, bricks, atlas_slots=0):
             masks[i] = brick_mask(bricks[b])
     return out, atlas, masks, slot
 
+
+
+# ---- chunk sync (GpuWorld::sync_chunks) ------------------------------------------------------------------
+RING_CHUNK_B = 16
+
+
+class RingAtlas:
+    """GpuWorld's ring-world state that sync_chunks reads and writes (gpu_world.rs:140-170): the World's
+    source grid (0 / 1-255 / 256 + brick index), the encoded grid, the atlas with its brick masks, the
+    coarse pyramid, the free list, the slots handed out and which slots hold world bricks."""
+
+    def __init__(self, wb, src, bricks, capacity):
+        """The state right after GpuWorld::upload packs `src` (pack_world) into an atlas of `capacity`."""
+        self.wb = tuple(int(x) for x in wb)
+        self.src = np.array(src, np.uint32).reshape(-1)
+        out, atlas, masks, slot = pack_world(self.src, bricks, capacity)
+        self.cap = int(capacity)
+        self.grid = out
+        self.atlas = np.zeros((self.cap, 512), np.uint8)
+        self.masks = np.zeros((self.cap, 16), np.uint32)
+        self.used = int((slot >= 0).sum())
+        self.atlas[:self.used], self.masks[:self.used] = atlas[:self.used], masks[:self.used]
+        self.free = []
+        self.world_slot = np.zeros(self.cap, np.uint8)
+        self.world_slot[:self.used] = 1
+        self.slot0 = slot.astype(np.int32)
+        self.c1d = tuple((x + 3) // 4 for x in self.wb)
+        self.c2d = tuple((x + 15) // 16 for x in self.wb)
+        self.c1 = np.zeros(int(np.prod(self.c1d)), np.uint8)
+        self.c2 = np.zeros(int(np.prod(self.c2d)), np.uint8)
+        lib().vfo_coarse(_p(self.src), (C.c_int32 * 3)(*self.wb), _p(self.c1), _p(self.c2))
+
+    def alloc_slot(self):
+        """gpu_world.rs:178-186: the free list's back, else the next unused slot, else -1."""
+        if self.free:
+            return self.free.pop()
+        if self.used < self.cap:
+            self.used += 1
+            return self.used - 1
+        return -1
+
+    def coarse_cell1(self, ci):
+        """gpu_world.rs:944-957 over the source grid."""
+        c = self.c1d
+        wbx, wby, wbz = self.wb
+        cx, cy, cz = ci % c[0], (ci // c[0]) % c[1], ci // (c[0] * c[1])
+        g = self.src.reshape(wbz, wby, wbx)
+        return np.uint8(g[cz * 4:min((cz + 1) * 4, wbz), cy * 4:min((cy + 1) * 4, wby),
+                          cx * 4:min((cx + 1) * 4, wbx)].any())
+
+    def coarse_cell2(self, ci):
+        """gpu_world.rs:959-973 over coarse1."""
+        c, d = self.c2d, self.c1d
+        cx, cy, cz = ci % c[0], (ci // c[0]) % c[1], ci // (c[0] * c[1])
+        g = self.c1.reshape(d[2], d[1], d[0])
+        return np.uint8(g[cz * 4:min((cz + 1) * 4, d[2]), cy * 4:min((cy + 1) * 4, d[1]),
+                          cx * 4:min((cx + 1) * 4, d[0])].any())
+
+
+def sync_chunks(st, chunks, bricks, gpu_slot, released=()):
+    """GpuWorld::sync_chunks (gpu_world.rs:1194-1376) on a RingAtlas, transcribed: chunks = [(cx, cz, cells or
+    None)] (None: evicted and not refilled), cells [16*wby*16] in (z, y, x) order.  Returns (gpu_slot, boxes):
+    the bricks' slots after the sync and the brick boxes whose distance fields it rebuilds, in order."""
+    wbx, wby, wbz = st.wb
+    cb = RING_CHUNK_B
+    bricks = np.asarray(bricks, np.uint8).reshape(-1, 512)
+    gpu_slot = np.array(gpu_slot, np.int64).reshape(-1)
+    for sl in sorted((int(x) for x in released), reverse=True):          # :1199-1207
+        if 0 <= sl < st.world_slot.size and st.world_slot[sl]:
+            st.world_slot[sl] = 0
+            st.free.append(sl)
+    ring_w = wbx // cb
+    done = set()
+    up, boxes = [], []
+    grid3 = st.grid.reshape(wbz, wby, wbx)
+    src3 = st.src.reshape(wbz, wby, wbx)
+    c1_3 = st.c1.reshape(st.c1d[2], st.c1d[1], st.c1d[0])
+    c2_3 = st.c2.reshape(st.c2d[2], st.c2d[1], st.c2d[0])
+    for cx, cz, cells in chunks:
+        slot = (cz & (ring_w - 1)) * ring_w + (cx & (ring_w - 1))
+        if slot in done:
+            continue
+        done.add(slot)
+        tx, tz = (cx * cb) & (wbx - 1), (cz * cb) & (wbz - 1)
+        boxes.append(([cx * cb, 0, cz * cb], [cx * cb + cb - 1, wby - 1, cz * cb + cb - 1]))
+        if cells is None:                                                # :1228-1253
+            grid3[tz:tz + cb, :, tx:tx + cb] = 0
+            src3[tz:tz + cb, :, tx:tx + cb] = 0
+            c1_3[tz // 4:(tz + cb) // 4, :, tx // 4:(tx + cb) // 4] = 0
+            c2_3[tz // 16:(tz + cb) // 16, :, tx // 16:(tx + cb) // 16] = 0
+            continue
+        cells = np.asarray(cells, np.uint32).reshape(cb, wby, cb)
+        src3[tz:tz + cb, :, tx:tx + cb] = cells
+        enc = cells.copy()
+        flat = enc.reshape(-1)
+        for k in np.nonzero(flat >= 256)[0]:                             # :1255-1273, encode_brick :191-214
+            b = int(flat[k]) - 256
+            if gpu_slot[b] < 0:
+                gpu_slot[b] = st.alloc_slot()
+            if gpu_slot[b] >= 0:
+                st.world_slot[gpu_slot[b]] = 1
+                flat[k] = 256 + gpu_slot[b]
+                up.append(b)
+            else:
+                flat[k] = majority_material(bricks[b])
+        grid3[tz:tz + cb, :, tx:tx + cb] = enc
+        for z in range(tz // 4, (tz + cb) // 4):                          # :1274-1284
+            for y in range(st.c1d[1]):
+                for x in range(tx // 4, (tx + cb) // 4):
+                    ci = (z * st.c1d[1] + y) * st.c1d[0] + x
+                    st.c1[ci] = st.coarse_cell1(ci)
+        for z in range(tz // 16, (tz + cb) // 16):                        # :1285-1295
+            for y in range(st.c2d[1]):
+                for x in range(tx // 16, (tx + cb) // 16):
+                    ci = (z * st.c2d[1] + y) * st.c2d[0] + x
+                    st.c2[ci] = st.coarse_cell2(ci)
+    for b in up:                                                         # :1300-1333, gpu_voxels, masks_of
+        st.atlas[gpu_slot[b]] = np.where(bricks[b] == TORCH_FLAME, 0, bricks[b])
+        st.masks[gpu_slot[b]] = brick_mask(bricks[b])
+    return gpu_slot.astype(np.int32), boxes

@@ -1,0 +1,148 @@
+"""GPU parity of the device chunk sync (vf_world_sync_chunks: GpuWorld::sync_chunks, gpu_world.rs:1194-1376)
+against the oracle's transcription, through the C ABI, on the endless-world shape (a ring world with negative
+absolute bricks): after each sync the encoded grid, the atlas slots handed out, their brick masks, the coarse
+pyramid, every brick's slot and the free list bit-identical; the distance fields equal to the oracle's
+record_distance_field over the same boxes; the sky-top map and traced rays equal to the oracle's world.
+Covers new and reused bricks, bricks named twice, a chunk named twice in one call (one sync per ring slot),
+evicted chunks, released slots (non-world, repeated and negative ones ignored) and a full atlas."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import vftrace as vf
+from helpers import hit_diff, random_rays
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vf.Context(0)
+    yield c
+    c.close()
+
+
+def _endless(extra):
+    w = vf.GenWorld.generate(2, world_bricks=(64, 32, 64), chunks=3, trees_per_chunk=30)
+    a = w.arrays()
+    grid = np.asarray(a["grid"], np.uint32)
+    bricks = np.asarray(a["atlas"], np.uint8).reshape(-1, 512)
+    return a, grid, bricks, bricks.shape[0] + extra
+
+
+def _cells(rng, wby, pool):
+    n = 16 * wby * 16
+    c = np.where(rng.random(n) < 0.15, rng.integers(1, 40, n), 0).astype(np.uint32)
+    det = rng.random(n) < 0.1
+    c[det] = 256 + rng.choice(pool, int(det.sum()))
+    return c
+
+
+def _check(ctx, st, gslot, dslot):
+    used, n_free, cap = ctx.atlas_state()
+    assert (used, n_free, cap) == (st.used, len(st.free), st.cap)
+    assert np.array_equal(dslot, gslot)
+    g, at, m = ctx.world_read(st.cap)
+    assert np.array_equal(g, st.grid), f"{int((g != st.grid).sum())} grid cells differ"
+    assert np.array_equal(at[:st.used], st.atlas[:st.used]) and np.array_equal(m[:st.used], st.masks[:st.used])
+    c1, c2 = ctx.read_coarse()
+    assert np.array_equal(c1, st.c1) and np.array_equal(c2, st.c2)
+
+
+def _fields_and_traces(ctx, a, ow, st, boxes, seed):
+    ow.grid[:] = st.grid
+    ow.atlas[:] = st.atlas.reshape(-1)
+    ow.c1[:] = st.c1
+    ow.c2[:] = st.c2
+    for lo, hi in boxes:
+        ow.rebuild_fields([lo[0] - 16, lo[1] - 16, lo[2] - 16], [hi[0] + 16, hi[1] + 16, hi[2] + 16], lo, hi)
+    d, r = ctx.read_brickdist()
+    assert np.array_equal(d, ow.d[0]) and np.array_equal(r, ow.d[2])
+    kinds = np.ascontiguousarray(ow.palette[:, 6])
+    O.lib().vfo_skytop(C.byref(ow.s), O._p(kinds), O._p(ow.skytop))
+    sky = np.zeros_like(ow.skytop)
+    vf.lib().vf_read_skytop(ctx._h, sky.ctypes.data)
+    assert np.array_equal(sky, ow.skytop)
+    win = a["world_win"]
+    rays = random_rays(60_000, [win[0] * 8 - 30, -10, win[2] * 8 - 30], [(win[0] + 64) * 8 + 30, 266,
+                                                                        (win[2] + 64) * 8 + 30], seed=seed)
+    rt = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(DEV)
+    h = torch.zeros(len(rays) * 20, dtype=torch.uint8, device=DEV)
+    ctx.trace_rays(rt, len(rays), h)
+    ctx.sync()
+    g = vf.hits_to_numpy(h)[:len(rays)]
+    assert hit_diff(g, ow.trace_rays(rays)[0]).size == 0
+
+
+@pytest.mark.parametrize("extra", [3000, 40])
+def test_sync_chunks_matches_oracle(ctx, extra):
+    a, grid, bricks0, cap = _endless(extra)
+    wb, win = a["world_bricks"], a["world_win"]
+    assert win[3] == 1 and win[0] < 0
+    dslot = ctx.pack_world(wb, win, grid, bricks0, a["palette"], atlas_slots=cap)
+    st = O.RingAtlas(wb, grid, bricks0, cap)
+    assert np.array_equal(dslot, st.slot0)
+    ctx.set_scene(vf.default_scene(wb, win))
+    ow = O.OracleWorld({"world_bricks": wb, "world_win": win, "grid": st.grid, "atlas": st.atlas.reshape(-1),
+                        "palette": a["palette"]})
+    rng = np.random.default_rng(extra)
+    n0 = bricks0.shape[0]
+    new = rng.integers(0, 60, (5000, 512)).astype(np.uint8)
+    new[:, :200] = 0
+    new[rng.random(5000) < 0.1] = 254
+    new[rng.random(5000) < 0.3, :320] = rng.integers(1, 60)          # majority candidates
+    bricks = np.concatenate([bricks0, new])
+    gslot = np.concatenate([st.slot0, np.full(5000, -1, np.int32)]).astype(np.int32)
+    cx0, cz0 = win[0] // 16, win[2] // 16
+    src3 = st.src.reshape(wb[2], wb[1], wb[0])
+    nxt = n0
+    for it in range(3):
+        chunks = []
+        for j in range(3):
+            cx, cz = cx0 + int(rng.integers(0, 4)), cz0 + int(rng.integers(0, 4))
+            if it == 1 and j == 0:
+                chunks.append((cx, cz, None))                         # evicted, not refilled
+                continue
+            live = np.nonzero(gslot >= 0)[0]
+            pool = np.concatenate([np.arange(nxt, nxt + 400), rng.choice(live, 100)])
+            nxt += 400
+            chunks.append((cx, cz, _cells(rng, wb[1], pool)))
+        chunks.append((chunks[-1][0] + 4, chunks[-1][1], _cells(rng, wb[1], np.arange(nxt, nxt + 10))))   # same ring slot
+        # bricks of the chunks about to be replaced or cleared, named nowhere else: the world drops them
+        replaced = np.zeros(src3.shape, bool)
+        for cx, cz, _ in chunks:
+            tx, tz = (cx * 16) & (wb[0] - 1), (cz * 16) & (wb[2] - 1)
+            replaced[tz:tz + 16, :, tx:tx + 16] = True
+        gone = set((src3[replaced & (src3 >= 256)] - 256).tolist()) - set((src3[~replaced & (src3 >= 256)] - 256).tolist())
+        gone = np.array(sorted(b for b in gone if gslot[b] >= 0), np.int64)
+        back = gone if it == 2 else gone[:300] if extra > 100 else gone[:50]   # pops, then fresh slots
+        released = gslot[back].tolist() + [-1, cap + 3, int(gslot[gone[0]]) if len(gone) else 0]
+        gslot[gone] = -1
+        gslot_ref, boxes = O.sync_chunks(st, chunks, bricks, gslot, released)
+        dslot = ctx.sync_chunks(chunks, bricks, gslot, released)
+        _check(ctx, st, gslot_ref, dslot)
+        gslot = gslot_ref
+        _fields_and_traces(ctx, a, ow, st, boxes, seed=it)
+    if extra == 40:                                                   # the atlas filled: majority cells
+        named = np.unique(st.src[st.src >= 256] - 256)
+        assert st.used == cap and (gslot[named] < 0).any()
+
+
+def test_sync_chunks_argument_errors(ctx):
+    a, grid, bricks0, cap = _endless(10)
+    wb, win = a["world_bricks"], a["world_win"]
+    ctx.pack_world(wb, win, grid, bricks0, a["palette"], atlas_slots=cap)
+    before = ctx.atlas_state()
+    cells = np.full(16 * wb[1] * 16, 256 + bricks0.shape[0], np.uint32)   # names a brick beyond the list
+    with pytest.raises(vf.VfError):
+        ctx.sync_chunks([(win[0] // 16, win[2] // 16, cells)], bricks0, np.arange(bricks0.shape[0], dtype=np.int32),
+                        released=[0, 1, 2])
+    assert ctx.atlas_state() == before                                   # nothing released on failure
+    flat = vf.GenWorld.generate(1, world_bricks=(16, 8, 16)).arrays()
+    ctx.upload_arrays(flat)                                               # not a ring world
+    with pytest.raises(vf.VfError):
+        ctx.sync_chunks([(0, 0, None)], np.zeros((0, 512), np.uint8), np.zeros(0, np.int32))

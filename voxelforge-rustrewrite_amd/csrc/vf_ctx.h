@@ -42,6 +42,11 @@ struct vf_ctx {
     std::vector<uint32_t> host_grid;   // host copies of the grid and of the slots' occupancy bits
     std::vector<uint64_t> host_occ;    // (8 words per slot): classify the camera voxel without a device read
     size_t n_cells = 0, n_cells_tiled = 0;
+    // the atlas bookkeeping of GpuWorld (gpu_world.rs:146-186): slots handed out so far, the free list
+    // (popped from the back), which slots hold world bricks (only those are freed by chunk syncs)
+    uint32_t atlas_used = 0;
+    std::vector<uint32_t> free_slots;
+    std::vector<uint8_t> world_slot;
     bool counting = false;
     unsigned long long *d_cnt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -95,4 +100,7 @@ void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n
 int world_prepare(vf_ctx *ctx, const int32_t wb[3], const int32_t win[4], uint32_t n_slots, const char *what);
 int world_finish(vf_ctx *ctx, bool coarse_given);
 int build_coarse(vf_ctx *ctx, const uint32_t *d_grid);
+int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n);
+int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]);
+int refresh_world(vf_ctx *ctx);
 }  // namespace vf

@@ -1021,6 +1021,25 @@ int build_coarse(vf_ctx *ctx, const uint32_t *d_grid) {
     return launch_check(ctx, "coarse pyramid");
 }
 
+// Occupancy masks of the listed slots (device list), after their atlas bricks changed.
+int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n) {
+    if (!n) return VF_OK;
+    hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)n * 16)), dim3(256), 0, ctx->stream, ctx->atlas, d_slots, n, ctx->n_slots,
+                       ctx->occ, ctx->occdry);
+    return launch_check(ctx, "occupancy masks");
+}
+
+// The brick distance field over one box (record_distance_field's box semantics).
+int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]) {
+    return distance_field(ctx, lo, hi, dlo, dhi);
+}
+
+// Cell words and sky-top map after grid, field or atlas changes.
+int refresh_world(vf_ctx *ctx) {
+    int rc = repack(ctx);
+    return rc ? rc : rebuild_skytop(ctx);
+}
+
 }  // namespace vf
 
 extern "C" {
@@ -1164,6 +1183,11 @@ int vf_world_upload(vf_ctx *ctx, const vf_world_desc *d) {
     int rc = vf::world_prepare(ctx, d->world_bricks, d->world_win, d->n_slots, "vf_world_upload");
     if (rc) return rc;
     ctx->host_grid.assign(d->grid, d->grid + n);
+    ctx->atlas_used = d->n_slots;
+    ctx->free_slots.clear();
+    ctx->world_slot.assign(d->n_slots, 0);
+    for (size_t i = 0; i < n; ++i)
+        if (d->grid[i] >= 256u) ctx->world_slot[d->grid[i] - 256u] = 1;
     vf::host_occupancy(ctx->host_occ, d->atlas, d->n_slots);
     VF_HIP(ctx, hipMemcpyAsync(ctx->grid, d->grid, n * 4, hipMemcpyHostToDevice, ctx->stream));
     if (d->n_slots) {

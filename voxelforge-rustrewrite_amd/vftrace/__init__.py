@@ -16,7 +16,7 @@ import numpy as np
 from .abi import (BVH_NODE_DTYPE, BvhNode, GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams,
                   GiCacheStats, Hit, HIT_DTYPE, Material, OBJ_HEADER_DTYPE, ObjHeader, PassUbo, QueueCfg, Ray, RAY_DTYPE,
                   Request, SceneUbo, TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH,
-                  VF_TUNE_GI_BATCH, WorldDesc, WorldSrc)
+                  VF_TUNE_GI_BATCH, WorldDesc, WorldSrc, Chunk)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
            "RAY_DTYPE", "GI_RECORD_DTYPE", "OBJ_HEADER_DTYPE", "BVH_NODE_DTYPE", "GiCache", "fly_camera", "sun_dir",
@@ -54,6 +54,9 @@ _SIGS = {
     "vf_world_upload": (C.c_int, [_P, C.POINTER(WorldDesc)]),
     "vf_world_pack": (C.c_int, [_P, C.POINTER(WorldSrc), _P]),
     "vf_world_read": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "vf_world_sync_chunks": (C.c_int, [_P, C.POINTER(Chunk), C.c_int32, _P, C.c_uint32, _P, _P, C.c_int32]),
+    "vf_world_atlas_state": (C.c_int, [_P, _P, _P, _P]),
+    "vf_read_coarse": (C.c_int, [_P, _P, _P]),
     "vf_world_update": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32]),
     "vf_build_brickdist": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32)]),
@@ -451,6 +454,45 @@ class Context:
         masks = np.zeros((max(n_slots, 1), 16), np.uint32)
         self._ck(lib().vf_world_read(self._h, grid.ctypes.data, atlas.ctypes.data, int(n_slots), masks.ctypes.data))
         return grid, atlas[:n_slots], masks[:n_slots]
+
+    def sync_chunks(self, chunks, bricks, gpu_slot, released=()):
+        """vf_world_sync_chunks (GpuWorld::sync_chunks): chunks = [(cx, cz, cells or None)], cells [16*wby*16] u32
+        in (z, y, x) order (None: evicted and not refilled); bricks [n, 512] u8; gpu_slot [n] i32 (-1: none).
+        Returns the updated gpu_slot."""
+        keep = []
+        arr = (Chunk * max(len(chunks), 1))()
+        for i, (cx, cz, cells) in enumerate(chunks):
+            arr[i].cx, arr[i].cz = int(cx), int(cz)
+            if cells is None:
+                arr[i].loaded, arr[i].cells = 0, None
+            else:
+                c = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
+                keep.append(c)
+                arr[i].loaded, arr[i].cells = 1, c.ctypes.data_as(C.POINTER(C.c_uint32))
+        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1, 512)
+        slot = np.array(gpu_slot, dtype=np.int32).reshape(-1)
+        if slot.size != b.shape[0]:
+            raise ValueError(f"gpu_slot has {slot.size} entries for {b.shape[0]} bricks")
+        rel = np.ascontiguousarray(released, dtype=np.int32).reshape(-1)
+        self._ck(lib().vf_world_sync_chunks(self._h, arr, len(chunks), b.ctypes.data if b.size else None, b.shape[0],
+                                            slot.ctypes.data if slot.size else None, rel.ctypes.data if rel.size else None,
+                                            rel.size))
+        del keep
+        return slot
+
+    def atlas_state(self):
+        """(slots handed out, free-list length, capacity) of the device world's atlas."""
+        v = (C.c_uint32 * 3)()
+        self._ck(lib().vf_world_atlas_state(self._h, C.byref(v, 0), C.byref(v, 4), C.byref(v, 8)))
+        return int(v[0]), int(v[1]), int(v[2])
+
+    def read_coarse(self):
+        """The device world's coarse occupancy pyramid: (coarse1, coarse2), flat u8."""
+        wb = self.world_bricks
+        c1 = np.zeros(int(np.prod([(x + 3) // 4 for x in wb])), np.uint8)
+        c2 = np.zeros(int(np.prod([(x + 15) // 16 for x in wb])), np.uint8)
+        self._ck(lib().vf_read_coarse(self._h, c1.ctypes.data, c2.ctypes.data))
+        return c1, c2
 
     # bodies (GpuWorld::init_objects / update_objects / sync_objects)
     def objects_upload(self, headers, grid):

@@ -107,6 +107,11 @@ class WorldSrc(C.Structure):
     ]
 
 
+class Chunk(C.Structure):
+    """vf_chunk: one 16 x wby x 16-brick chunk of a ring world for vf_world_sync_chunks."""
+    _fields_ = [("cx", C.c_int32), ("cz", C.c_int32), ("loaded", C.c_int32), ("cells", C.POINTER(C.c_uint32))]
+
+
 class GenParams(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("seed", C.c_uint32), ("world_bricks", i3), ("chunks", C.c_int32),
