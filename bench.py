@@ -303,13 +303,71 @@ def pack_config(ctx, world, scene):
     out["upload_host_packed_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     g_up = ctx.world_read(0)[0]
     t0 = time.perf_counter()
-    slot = ctx.pack_world(a["world_bricks"], a["world_win"], src, bricks, a["palette"])
+    slot = ctx.pack_world(a["world_bricks"], a["world_win"], src, bricks, a["palette"], atlas_slots=atlas.shape[0] + 4096)
     out["pack_on_device_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     g_pk = ctx.world_read(0)[0]
     out["same_grid"] = bool(np.array_equal(g_up, g_pk) and np.array_equal(slot, perm))
+    wb, win = tuple(a["world_bricks"]), tuple(a["world_win"])
+    if win[3]:   # ring world: two streaming steps over the four fullest chunk rows (the first grows the buffers)
+        ring_w = wb[0] // 16
+        order = np.argsort(-(src.reshape(wb[2], wb[1], wb[0]) >= 256).reshape(ring_w, -1).sum(1), kind="stable")
+        first, src1, b1, s1 = sync_step(ctx, a, src, bricks, slot, (int(order[0]), int(order[1])))
+        out["chunk_sync"], _, _, _ = sync_step(ctx, a, src1, b1, s1, (int(order[2]), int(order[3])))
+        out["chunk_sync"]["first_call_ms"] = first["sync_ms"]
+        out["chunk_sync"]["consistent"] &= first["consistent"]
     ctx.set_scene(scene)
     log(f"world pack: {out}")
     return out
+
+
+def sync_step(ctx, a, src, bricks, slot, rows):
+    """One streaming step of the endless world (vf_world_sync_chunks: GpuWorld::sync_chunks): the ring's
+    row of chunks rows[0] is evicted, the slots of the bricks only it names released, and refilled with
+    the content of row rows[1] as new bricks (copies, no slot yet).  Wall time of the call: the chunk
+    cells and the named bricks to the device, slot allocation, grid/atlas/mask/coarse writes, the
+    distance field over each chunk's box +-16 bricks, cell words and sky-top, the host bookkeeping.
+    Returns the report and the world after the step (source grid, bricks, slots)."""
+    wb, win = tuple(a["world_bricks"]), tuple(a["world_win"])
+    cb, ring_w = 16, wb[0] // 16
+    g3 = src.reshape(wb[2], wb[1], wb[0])
+    tz_dst, tz_src = rows[0] * cb, rows[1] * cb
+    cz_dst = next(c for c in range(win[2] // cb, win[2] // cb + ring_w) if (c * cb) & (wb[2] - 1) == tz_dst)
+    dst_rows = g3[tz_dst:tz_dst + cb]
+    rest = np.concatenate([g3[:tz_dst].reshape(-1), g3[tz_dst + cb:].reshape(-1)])
+    gone = np.setdiff1d(np.unique(dst_rows[dst_rows >= 256]), rest[rest >= 256]) - 256   # named only there
+    n_b = bricks.shape[0]
+    src_rows = g3[tz_src:tz_src + cb]
+    named = np.unique(src_rows[src_rows >= 256]) - 256
+    new_id = np.full(n_b, -1, np.int64)
+    new_id[named] = n_b + np.arange(named.size)
+    all_bricks = np.concatenate([bricks, bricks[named]])
+    gslot = np.concatenate([slot, np.full(named.size, -1, np.int32)]).astype(np.int32)
+    released = gslot[gone].copy()
+    gslot[gone] = -1
+    new_rows = src_rows.copy()
+    det = new_rows >= 256
+    new_rows[det] = 256 + new_id[new_rows[det] - 256]
+    chunks = []
+    for i in range(ring_w):
+        cx = win[0] // cb + i
+        tx = (cx * cb) & (wb[0] - 1)
+        chunks.append((cx, cz_dst, np.ascontiguousarray(new_rows[:, :, tx:tx + cb]).reshape(-1)))
+    used0, free0, _ = ctx.atlas_state()
+    ctx.sync()
+    t0 = time.perf_counter()
+    out_slot = ctx.sync_chunks(chunks, all_bricks, gslot, released)
+    ms = (time.perf_counter() - t0) * 1e3
+    used, n_free, _ = ctx.atlas_state()
+    fresh = out_slot[n_b:]
+    ok = bool((fresh >= 0).all() and len(set(fresh.tolist())) == fresh.size and
+              used == used0 + max(0, named.size - gone.size - free0) and n_free == max(0, free0 + gone.size - named.size))
+    cells = ring_w * cb * wb[1] * cb
+    src2 = src.copy()
+    src2.reshape(g3.shape)[tz_dst:tz_dst + cb] = new_rows
+    rep = {"chunks": ring_w, "cells": cells, "bricks_in": int(named.size), "slots_released": int(gone.size),
+           "sync_ms": round(ms, 2), "consistent": ok,
+           "host_to_device_mb": round((cells * 4 + named.size * 516) / 1e6, 2)}
+    return rep, src2, all_bricks, out_slot
 
 
 def open_forest_config(ctx, stream, args, hits, gbp, gbn, sh):

@@ -194,6 +194,16 @@ typedef struct vf_chunk {
  * sky-top map.  gpu_slot: [n_bricks], in and out (Brick::gpu_slot). */
 int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t n_chunks, const uint8_t *bricks, uint32_t n_bricks,
                          int32_t *gpu_slot, const int32_t *released, int32_t n_released);
+/* GpuWorld::sync (gpu_world.rs:985-1142) on the device: the world's edited cells.  cells[i] is a grid index,
+ * values[i] the world's value there (0 / 1-255 / 256 + i = brick i of `bricks`); a cell listed twice takes
+ * its last value.  In grid order, once per cell: a brick that became uniform turns back into a uniform cell
+ * and frees its slot (SparseVolume::compact_brick; values[] returns the new value, gpu_slot -1), then the
+ * cell is encoded as vf_world_sync_chunks encodes.  The touched coarse cells are rebuilt from the world's
+ * values; the dirty bricks are clustered (boxes within 16 bricks grow, boxes within 32 merge) and each
+ * cluster gets its sky-top columns and its distance field over the box +-16 bricks.  gpu_slot: [n_bricks],
+ * in and out. */
+int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t *values, uint32_t n, const uint8_t *bricks,
+                        uint32_t n_bricks, int32_t *gpu_slot);
 /* The atlas bookkeeping: slots handed out, the free list's length, capacity. */
 int vf_world_atlas_state(vf_ctx *ctx, uint32_t *used, uint32_t *n_free, uint32_t *capacity);
 /* Copies the device world's encoded grid ([wbx*wby*wbz]) and the first n_slots atlas slots back. */

@@ -725,14 +725,16 @@ def pack_world(grid_src, bricks, atlas_slots=0):
 RING_CHUNK_B = 16
 
 
-class RingAtlas:
-    """GpuWorld's ring-world state that sync_chunks reads and writes (gpu_world.rs:140-170): the World's
+class GpuWorldState:
+    """GpuWorld's state that sync and sync_chunks read and write (gpu_world.rs:140-170): the World's
     source grid (0 / 1-255 / 256 + brick index), the encoded grid, the atlas with its brick masks, the
     coarse pyramid, the free list, the slots handed out and which slots hold world bricks."""
 
-    def __init__(self, wb, src, bricks, capacity):
-        """The state right after GpuWorld::upload packs `src` (pack_world) into an atlas of `capacity`."""
+    def __init__(self, wb, src, bricks, capacity, win=(0, 0, 0, 0)):
+        """The state right after GpuWorld::upload packs `src` (pack_world) into an atlas of `capacity`;
+        win: the ring window (x, y, z origin in bricks, ring flag)."""
         self.wb = tuple(int(x) for x in wb)
+        self.win = tuple(int(x) for x in win)
         self.src = np.array(src, np.uint32).reshape(-1)
         out, atlas, masks, slot = pack_world(self.src, bricks, capacity)
         self.cap = int(capacity)
@@ -779,7 +781,7 @@ class RingAtlas:
 
 
 def sync_chunks(st, chunks, bricks, gpu_slot, released=()):
-    """GpuWorld::sync_chunks (gpu_world.rs:1194-1376) on a RingAtlas, transcribed: chunks = [(cx, cz, cells or
+    """GpuWorld::sync_chunks (gpu_world.rs:1194-1376) on a GpuWorldState, transcribed: chunks = [(cx, cz, cells or
     None)] (None: evicted and not refilled), cells [16*wby*16] in (z, y, x) order.  Returns (gpu_slot, boxes):
     the bricks' slots after the sync and the brick boxes whose distance fields it rebuilds, in order."""
     wbx, wby, wbz = st.wb
@@ -839,3 +841,97 @@ def sync_chunks(st, chunks, bricks, gpu_slot, released=()):
         st.atlas[gpu_slot[b]] = np.where(bricks[b] == TORCH_FLAME, 0, bricks[b])
         st.masks[gpu_slot[b]] = brick_mask(bricks[b])
     return gpu_slot.astype(np.int32), boxes
+
+
+DIST_CAP = 16
+
+
+def brick_coords(st, gi):
+    """World::brick_coords (sparse_volume.rs:206-215): grid index -> brick coordinates, absolute in a ring
+    world (the window's chunk in each ring slot)."""
+    wbx, wby, wbz = st.wb
+    b = [gi % wbx, (gi // wbx) % wby, gi // (wbx * wby)]
+    if st.win[3]:
+        b[0] = st.win[0] + (b[0] - st.win[0]) % wbx
+        b[2] = st.win[2] + (b[2] - st.win[2]) % wbz
+    return b
+
+
+def edit_boxes(st, dirty):
+    """GpuWorld::sync's clusters (gpu_world.rs:1103-1130): a dirty brick grows the first box within DIST_CAP
+    of it, else starts one; then boxes within 2 DIST_CAP of each other merge, first pair first, until none."""
+    boxes = []
+    for gi in dirty:
+        b = brick_coords(st, gi)
+        for x in boxes:
+            if all(x[0][k] - DIST_CAP <= b[k] <= x[1][k] + DIST_CAP for k in range(3)):
+                x[0] = [min(x[0][k], b[k]) for k in range(3)]
+                x[1] = [max(x[1][k], b[k]) for k in range(3)]
+                break
+        else:
+            boxes.append([list(b), list(b)])
+    merged = True
+    while merged:
+        merged = False
+        for i in range(len(boxes)):
+            for j in range(i + 1, len(boxes)):
+                if all(boxes[j][0][k] <= boxes[i][1][k] + 2 * DIST_CAP and boxes[j][1][k] >= boxes[i][0][k] - 2 * DIST_CAP
+                       for k in range(3)):
+                    bj = boxes.pop(j)
+                    boxes[i][0] = [min(boxes[i][0][k], bj[0][k]) for k in range(3)]
+                    boxes[i][1] = [max(boxes[i][1][k], bj[1][k]) for k in range(3)]
+                    merged = True
+                    break
+            if merged:
+                break
+    return [(x[0], x[1]) for x in boxes]
+
+
+def sync_edits(st, cells, values, bricks, gpu_slot):
+    """GpuWorld::sync (gpu_world.rs:985-1142) on a GpuWorldState, transcribed: the World's edited cells take
+    their values (a cell listed twice: its last value), then in grid order, once per cell, compact_brick
+    (sparse_volume.rs:354-366: a uniform brick becomes its material and frees its slot) and encode_brick;
+    the touched coarse cells; the clusters.  Returns (values after compaction per input entry, gpu_slot,
+    boxes)."""
+    bricks = np.asarray(bricks, np.uint8).reshape(-1, 512)
+    gpu_slot = np.array(gpu_slot, np.int64).reshape(-1)
+    cells = [int(c) for c in cells]
+    for gi, v in zip(cells, values):
+        st.src[gi] = int(v)
+    dirty = sorted(set(cells))
+    up = []
+    for gi in dirty:
+        v = int(st.src[gi])
+        if v >= 256:
+            b = v - 256
+            if (bricks[b] == bricks[b][0]).all():                       # compact_brick
+                freed = int(gpu_slot[b])
+                gpu_slot[b] = -1
+                st.src[gi] = bricks[b][0]
+                if freed >= 0:
+                    st.world_slot[freed] = 0
+                    st.free.append(freed)
+        v = int(st.src[gi])
+        if v < 256:
+            st.grid[gi] = v
+            continue
+        b = v - 256
+        if gpu_slot[b] < 0:
+            gpu_slot[b] = st.alloc_slot()
+        if gpu_slot[b] >= 0:
+            st.world_slot[gpu_slot[b]] = 1
+            st.grid[gi] = 256 + gpu_slot[b]
+            up.append(b)
+        else:
+            st.grid[gi] = majority_material(bricks[b])
+    for b in up:
+        st.atlas[gpu_slot[b]] = np.where(bricks[b] == TORCH_FLAME, 0, bricks[b])
+        st.masks[gpu_slot[b]] = brick_mask(bricks[b])
+    wbx, wby, _ = st.wb
+    c1 = sorted({((gi // (wbx * wby)) // 4 * st.c1d[1] + (gi // wbx) % wby // 4) * st.c1d[0] + gi % wbx // 4 for gi in dirty})
+    c2 = sorted({((gi // (wbx * wby)) // 16 * st.c2d[1] + (gi // wbx) % wby // 16) * st.c2d[0] + gi % wbx // 16 for gi in dirty})
+    for ci in c1:
+        st.c1[ci] = st.coarse_cell1(ci)
+    for ci in c2:
+        st.c2[ci] = st.coarse_cell2(ci)
+    return (np.array([st.src[gi] for gi in cells], np.uint32), gpu_slot.astype(np.int32), edit_boxes(st, dirty))

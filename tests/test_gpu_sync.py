@@ -1,10 +1,13 @@
-"""GPU parity of the device chunk sync (vf_world_sync_chunks: GpuWorld::sync_chunks, gpu_world.rs:1194-1376)
-against the oracle's transcription, through the C ABI, on the endless-world shape (a ring world with negative
-absolute bricks): after each sync the encoded grid, the atlas slots handed out, their brick masks, the coarse
-pyramid, every brick's slot and the free list bit-identical; the distance fields equal to the oracle's
-record_distance_field over the same boxes; the sky-top map and traced rays equal to the oracle's world.
-Covers new and reused bricks, bricks named twice, a chunk named twice in one call (one sync per ring slot),
-evicted chunks, released slots (non-world, repeated and negative ones ignored) and a full atlas."""
+"""GPU parity of the device world syncs against the oracle's transcriptions, through the C ABI: the chunk sync
+(vf_world_sync_chunks: GpuWorld::sync_chunks, gpu_world.rs:1194-1376) on the endless-world shape (a ring
+world with negative absolute bricks) and the edit sync (vf_world_sync_edits: GpuWorld::sync, :985-1142) on
+it and on the fixed world.  After each sync the encoded grid, the atlas slots handed out, their brick masks,
+the coarse pyramid, every brick's slot and the free list bit-identical; the distance fields equal to the
+oracle's record_distance_field over the same boxes; the sky-top map and traced rays equal to the oracle's
+world.  Chunks: new and reused bricks, bricks named twice, a chunk named twice in one call (one sync per
+ring slot), evicted chunks, released slots (non-world, repeated and negative ones ignored), a full atlas.
+Edits: clustered and scattered cells, cells listed twice, bricks edited in place, bricks that became
+uniform (compacted, their slots freed and reused), new bricks past a full atlas."""
 import ctypes as C
 
 import numpy as np
@@ -84,7 +87,7 @@ def test_sync_chunks_matches_oracle(ctx, extra):
     wb, win = a["world_bricks"], a["world_win"]
     assert win[3] == 1 and win[0] < 0
     dslot = ctx.pack_world(wb, win, grid, bricks0, a["palette"], atlas_slots=cap)
-    st = O.RingAtlas(wb, grid, bricks0, cap)
+    st = O.GpuWorldState(wb, grid, bricks0, cap, win)
     assert np.array_equal(dslot, st.slot0)
     ctx.set_scene(vf.default_scene(wb, win))
     ow = O.OracleWorld({"world_bricks": wb, "world_win": win, "grid": st.grid, "atlas": st.atlas.reshape(-1),
@@ -130,6 +133,67 @@ def test_sync_chunks_matches_oracle(ctx, extra):
     if extra == 40:                                                   # the atlas filled: majority cells
         named = np.unique(st.src[st.src >= 256] - 256)
         assert st.used == cap and (gslot[named] < 0).any()
+
+
+def _edits(rng, st, wb, pool_new, bricks):
+    """A cluster of cells around a random brick and scattered single cells: air, materials, new bricks,
+    bricks edited in place and bricks edited into one material."""
+    n = int(np.prod(wb))
+    c = rng.integers(0, [wb[0] - 6, wb[1] - 6, wb[2] - 6])
+    z, y, x = np.meshgrid(np.arange(6) + c[2], np.arange(6) + c[1], np.arange(6) + c[0], indexing="ij")
+    cells = np.concatenate([((z * wb[1] + y) * wb[0] + x).reshape(-1), rng.integers(0, n, 6)])
+    cells = np.concatenate([cells, cells[:10]])                            # listed twice
+    k = cells.size
+    vals = np.where(rng.random(k) < 0.4, rng.integers(0, 40, k), 0).astype(np.uint32)
+    new = rng.random(k) < 0.35
+    vals[new] = 256 + rng.choice(pool_new, int(new.sum()), replace=False)
+    for i in np.nonzero(rng.random(k) < 0.25)[0]:
+        v = int(st.src[cells[i]])
+        if v >= 256:                                                     # the world's brick there, edited
+            if rng.random() < 0.4:
+                bricks[v - 256] = rng.integers(0, 40)                    # into one material: compacted
+            else:
+                bricks[v - 256, rng.integers(0, 512, 20)] = rng.integers(1, 40)
+            vals[i] = v
+    return cells, vals
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_sync_edits_matches_oracle(ctx, kind):
+    if kind == 2:
+        a, grid, bricks0, cap = _endless(150)
+    else:
+        a = vf.GenWorld.generate(1, world_bricks=(192, 16, 96)).arrays()
+        grid = np.asarray(a["grid"], np.uint32)
+        bricks0 = np.asarray(a["atlas"], np.uint8).reshape(-1, 512)
+        cap = bricks0.shape[0] + 150
+    wb, win = a["world_bricks"], a["world_win"]
+    dslot = ctx.pack_world(wb, win, grid, bricks0, a["palette"], atlas_slots=cap)
+    st = O.GpuWorldState(wb, grid, bricks0, cap, win)
+    assert np.array_equal(dslot, st.slot0)
+    ctx.set_scene(vf.default_scene(wb, win))
+    ow = O.OracleWorld({"world_bricks": wb, "world_win": win, "grid": st.grid.copy(), "atlas": st.atlas.reshape(-1),
+                        "palette": a["palette"]})
+    rng = np.random.default_rng(10 + kind)
+    n0 = bricks0.shape[0]
+    new = rng.integers(0, 40, (3000, 512)).astype(np.uint8)
+    new[:, :150] = 0
+    new[rng.random(3000) < 0.3, :320] = rng.integers(1, 40)               # majority candidates
+    bricks = np.concatenate([bricks0, new])
+    gslot = np.concatenate([st.slot0, np.full(3000, -1, np.int32)]).astype(np.int32)
+    pool = np.arange(n0, n0 + 3000)
+    n_compacted = 0
+    for it in range(4):
+        cells, vals = _edits(rng, st, wb, pool[it * 700:(it + 1) * 700], bricks)
+        vref, gref, boxes = O.sync_edits(st, cells, vals, bricks, gslot)
+        vdev, gdev = ctx.sync_edits(cells, vals, bricks, gslot)
+        assert np.array_equal(vdev, vref)
+        _check(ctx, st, gref, gdev)
+        n_compacted += int(((vals >= 256) & (vref < 256)).sum())
+        gslot = gref
+        _fields_and_traces(ctx, a, ow, st, boxes, seed=20 + it)
+    named = np.unique(st.src[st.src >= 256] - 256)
+    assert n_compacted > 0 and (gslot[named] < 0).any()                   # compaction and a full atlas ran
 
 
 def test_sync_chunks_argument_errors(ctx):

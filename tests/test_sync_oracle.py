@@ -27,7 +27,7 @@ def _state(wb, src_cells, bricks, cap):
     g = src.reshape(wb[2], wb[1], wb[0])
     for (x, y, z), v in src_cells.items():
         g[z, y, x] = v
-    return O.RingAtlas(wb, src, bricks, cap)
+    return O.GpuWorldState(wb, src, bricks, cap)
 
 
 def test_free_list_release_and_slot_order():
@@ -81,7 +81,7 @@ def test_one_chunk_world_synced_from_empty_equals_the_pack(seed):
     det = rng.random(cells.size) < 0.4
     cells[det] = 256 + rng.integers(0, n_b, int(det.sum()))
     for cap in (1000, 150):
-        st = O.RingAtlas(wb, np.zeros(cells.size, np.uint32), bricks, cap)
+        st = O.GpuWorldState(wb, np.zeros(cells.size, np.uint32), bricks, cap)
         slot, _ = O.sync_chunks(st, [(0, 0, cells)], bricks, np.full(n_b, -1, np.int32))
         out, atlas, masks, pslot = O.pack_world(cells, bricks, cap)
         assert np.array_equal(st.grid, out) and np.array_equal(slot, pslot)
@@ -106,7 +106,7 @@ def test_random_sync_sequences_keep_the_invariants(seed, cap):
     bricks = rng.integers(0, 20, (n_b, 512)).astype(np.uint8)
     bricks[rng.random(n_b) < 0.1] = 254
     src = np.zeros(int(np.prod(wb)), np.uint32)
-    st = O.RingAtlas(wb, src, bricks, cap)
+    st = O.GpuWorldState(wb, src, bricks, cap)
     gslot = np.full(n_b, -1, np.int32)
     nxt = 0
     for it in range(6):
@@ -138,3 +138,85 @@ def test_random_sync_sequences_keep_the_invariants(seed, cap):
         assert len(b) >= 1
         assert np.array_equal(st.atlas[v - 256], np.where(bricks[b[0]] == 254, 0, bricks[b[0]]))
         assert np.array_equal(st.masks[v - 256], O.brick_mask(bricks[b[0]]))
+
+
+# ---- edit sync (GpuWorld::sync) ------------------------------------------------------------------------
+
+def test_edit_compaction_frees_a_slot_the_next_cell_pops():
+    wb = (16, 4, 16)
+    b_uni = _solid(5)
+    bricks = np.stack([_solid(1), b_uni, _solid(2), _solid(3)])
+    bricks[0, 7] = 9                                                 # brick 0 detailed
+    bricks[2, 3] = 8
+    bricks[3, 1] = 7
+    st = _state(wb, {(0, 0, 0): 256 + 0, (1, 0, 0): 256 + 1}, bricks, cap=3)
+    assert st.used == 2
+    gslot = st.slot0.copy()                                          # [0, 1, -1, -1]
+    # cell 1 (brick 1, now uniform 5) -> 5, slot 1 freed; cell 30 names new brick 2 -> pops slot 1; cell 40 names
+    # new brick 3 -> fresh slot 2; cell 0 listed twice: its last value (air)
+    vals, out, boxes = O.sync_edits(st, [40, 1, 0, 30, 0], [256 + 3, 256 + 1, 256 + 0, 256 + 2, 0], bricks, gslot)
+    assert vals.tolist() == [256 + 3, 5, 0, 256 + 2, 0]
+    assert out.tolist() == [0, -1, 1, 2]
+    assert st.grid[[0, 1, 30, 40]].tolist() == [0, 5, 256 + 1, 256 + 2]
+    assert st.free == [] and st.used == 3 and st.world_slot.tolist() == [1, 1, 1]
+    assert np.array_equal(st.atlas[1], bricks[2]) and np.array_equal(st.atlas[2], bricks[3])
+    assert boxes == [([0, 0, 0], [14, 2, 0])]   # cells 0, 1, 30 (x 14, y 1), 40 (x 8, y 2): one box
+
+
+def test_edit_clusters_grow_then_merge():
+    wb = (128, 2, 16)
+    st = O.GpuWorldState(wb, np.zeros(128 * 2 * 16, np.uint32), np.zeros((1, 512), np.uint8), 4)
+    assert O.edit_boxes(st, [0, 10, 45, 60]) == [([0, 0, 0], [10, 0, 0]), ([45, 0, 0], [60, 0, 0])]
+    assert O.edit_boxes(st, [0, 10, 40, 45, 60]) == [([0, 0, 0], [60, 0, 0])]       # 40 <= 10 + 32: merged
+    assert O.edit_boxes(st, [5, 128 * 2 * 15 + 5]) == [([5, 0, 0], [5, 0, 15])]      # z 15 within 16
+    ring = O.GpuWorldState((32, 2, 32), np.zeros(32 * 2 * 32, np.uint32), np.zeros((1, 512), np.uint8), 4,
+                           win=(-16, 0, -32, 1))
+    # grid x 20 is absolute -12 (the window starts at -16: x in [-16, 16)), grid x 3 is absolute 3
+    assert O.brick_coords(ring, 20) == [-12, 0, -32] and O.brick_coords(ring, 3) == [3, 0, -32]
+
+
+@pytest.mark.parametrize("seed,cap", [(0, 3000), (1, 520)])
+def test_random_edits_render_the_source_world(seed, cap):
+    """After any edits, every cell renders what the World holds: uniform cells as given, detailed cells from
+    their brick's atlas slot (TORCH_FLAME as air) or, with no slot, as its majority material; the coarse
+    pyramid equals a full rebuild; slots in use and the free list partition the slots handed out."""
+    rng = np.random.default_rng(seed)
+    wb = (32, 8, 32)
+    n = 32 * 8 * 32
+    n_b = 4000
+    bricks = rng.integers(0, 6, (n_b, 512)).astype(np.uint8)
+    bricks[rng.random(n_b) < 0.05] = 254
+    src = np.where(rng.random(n) < 0.2, rng.integers(1, 30, n), 0).astype(np.uint32)
+    det = rng.random(n) < 0.05
+    src[det] = 256 + np.arange(int(det.sum()))
+    nxt = int(det.sum())
+    st = O.GpuWorldState(wb, src, bricks, cap)
+    gslot = np.concatenate([st.slot0, np.full(n_b - st.slot0.size, -1, np.int32)])[:n_b].astype(np.int32)
+    for it in range(5):
+        k = 300
+        cells = rng.integers(0, n, k)
+        vals = np.where(rng.random(k) < 0.5, rng.integers(0, 30, k), 0).astype(np.uint32)
+        new = rng.random(k) < 0.3
+        vals[new] = 256 + np.arange(nxt, nxt + int(new.sum()))
+        nxt += int(new.sum())
+        uni = rng.random(k) < 0.1                                      # bricks edited into one material
+        for c in np.nonzero(uni)[0]:
+            v = int(st.src[cells[c]])
+            if v >= 256:
+                bricks[v - 256] = rng.integers(0, 30)
+                vals[c] = v
+        out_vals, gslot, _ = O.sync_edits(st, cells, vals, bricks, gslot)
+        ws = set(np.nonzero(st.world_slot)[0].tolist())
+        assert not ws & set(st.free) and ws | set(st.free) == set(range(st.used))
+    g, s = st.grid, st.src
+    assert np.array_equal(g[s < 256], s[s < 256])
+    for gi in np.nonzero(s >= 256)[0]:
+        b = int(s[gi]) - 256
+        if g[gi] >= 256:
+            assert gslot[b] == g[gi] - 256
+            assert np.array_equal(st.atlas[g[gi] - 256], np.where(bricks[b] == 254, 0, bricks[b]))
+        else:
+            assert gslot[b] < 0 and g[gi] == O.majority_material(bricks[b])
+    c1, c2 = np.zeros_like(st.c1), np.zeros_like(st.c2)
+    O.lib().vfo_coarse(O._p(st.src), (O.C.c_int32 * 3)(*wb), O._p(c1), O._p(c2))
+    assert np.array_equal(st.c1, c1) and np.array_equal(st.c2, c2)

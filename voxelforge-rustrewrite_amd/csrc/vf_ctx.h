@@ -35,6 +35,7 @@ struct vf_ctx {
     uint32_t *occ = nullptr, *occdry = nullptr, *masks_ref = nullptr;
     uint8_t *dist[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t *c1 = nullptr, *c2 = nullptr;
+    uint8_t *srcnz = nullptr;   // per cell: the caller's (source) cell is not air; the coarse pyramid's input
     uint16_t *skytop = nullptr;
     float4 *palette = nullptr;
     int32_t c1d[3] = {0, 0, 0}, c2d[3] = {0, 0, 0};
@@ -47,6 +48,10 @@ struct vf_ctx {
     uint32_t atlas_used = 0;
     std::vector<uint32_t> free_slots;
     std::vector<uint8_t> world_slot;
+    uint8_t *pin = nullptr;     // pinned host staging for chunk syncs (hipHostMalloc), grown on demand
+    size_t pin_cap = 0;
+    uint8_t *sync_ws = nullptr;  // device workspace of chunk syncs, grown on demand
+    size_t sync_ws_cap = 0;
     bool counting = false;
     unsigned long long *d_cnt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -98,9 +103,12 @@ void slot_occupancy(const uint8_t *vox, uint64_t out[8]);
 void host_occupancy(std::vector<uint64_t> &occ, const uint8_t *atlas, uint32_t n_slots);
 // the world's buffers and derived data (vf_trace.hip), shared by vf_world_upload and vf_world_pack
 int world_prepare(vf_ctx *ctx, const int32_t wb[3], const int32_t win[4], uint32_t n_slots, const char *what);
-int world_finish(vf_ctx *ctx, bool coarse_given);
+int world_finish(vf_ctx *ctx, bool coarse_given, bool source_given = false);
 int build_coarse(vf_ctx *ctx, const uint32_t *d_grid);
 int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n);
 int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]);
 int refresh_world(vf_ctx *ctx);
+int skytop_columns(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t nx, int32_t nz);
+int cell_words(vf_ctx *ctx);
+int skytop_box(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t x1, int32_t z1);
 }  // namespace vf

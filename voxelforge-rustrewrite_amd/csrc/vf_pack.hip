@@ -14,6 +14,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/vf_trace.h"
@@ -209,7 +212,7 @@ extern "C" int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot
     if ((rc = vf::launch_check(ctx, "world pack"))) return rc;
     VF_HIP(ctx, hipMemcpyAsync(ctx->palette, src->palette, 256 * 48, hipMemcpyHostToDevice, st));
     if ((rc = vf::build_coarse(ctx, d_src))) return rc;   // from the source grid, as the reference's pack
-    if ((rc = vf::world_finish(ctx, true))) return rc;
+    if ((rc = vf::world_finish(ctx, true, true))) return rc;
     ctx->atlas_used = std::min(total, cap);
     ctx->free_slots.clear();
     ctx->world_slot.assign(cap, 0);
@@ -273,13 +276,11 @@ __device__ __forceinline__ size_t sync_cell_pos(const SyncArgs &A, const int2 *o
     return ((size_t)(o.y + z) * A.wby + y) * A.wbx + (size_t)(o.x + x);
 }
 
-__global__ void k_sync_first(const uint32_t *cells, int64_t n, const int32_t *gslot, uint32_t *first, uint8_t *ref) {
+__global__ void k_sync_first(const uint32_t *cells, int64_t n, const int32_t *gslot, uint32_t *first) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t v = cells[k];
-    if (v < 256u) return;
-    ref[v - 256u] = 1u;
-    if (gslot[v - 256u] < 0) atomicMin(&first[v - 256u], (uint32_t)k);
+    if (v >= 256u && gslot[v - 256u] < 0) atomicMin(&first[v - 256u], (uint32_t)k);
 }
 
 __global__ void k_sync_flags(const uint32_t *cells, int64_t n, const int32_t *gslot, const uint32_t *first, uint32_t *flag) {
@@ -303,7 +304,7 @@ __global__ void k_sync_assign(const SyncArgs A, const uint32_t *cells, int64_t n
 }
 
 __global__ void k_sync_encode(const SyncArgs A, const int2 *org, const uint32_t *cells, int64_t n, const int32_t *slot,
-                              const uint32_t *major, uint32_t *grid) {
+                              const uint32_t *major, uint32_t *grid, uint8_t *srcnz, uint32_t *enc) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t v = cells[k];
@@ -312,21 +313,34 @@ __global__ void k_sync_encode(const SyncArgs A, const int2 *org, const uint32_t 
         const int32_t s = slot[v - 256u];
         e = s >= 0 ? 256u + (uint32_t)s : major[v - 256u];
     }
-    grid[sync_cell_pos(A, org, k)] = e;
+    const size_t pos = sync_cell_pos(A, org, k);
+    grid[pos] = e;
+    srcnz[pos] = v != 0u ? 1u : 0u;
+    enc[k] = e;
 }
 
-// the slots whose bricks are written (named by the loaded chunks and placed), or -1
-__global__ void k_sync_written(const int32_t *slot, const uint8_t *ref, int64_t nb, int32_t *written) {
+// the slots the named bricks are written to, or -1 (no slot: majority material)
+__global__ void k_sync_written(const int32_t *slot, int64_t nb, int32_t *written) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nb) written[b] = (ref[b] && slot[b] >= 0) ? slot[b] : -1;
+    if (b < nb) written[b] = slot[b] >= 0 ? slot[b] : -1;
+}
+
+// the occupancy rows of the slots the named bricks went to (-1: none), for the host copy
+__global__ void k_sync_occ(const int32_t *written, int64_t n_words, const uint32_t *occ, uint32_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_words) return;
+    const int32_t s = written[i >> 4];
+    out[i] = s >= 0 ? occ[(size_t)s * 16u + (size_t)(i & 15)] : 0u;
 }
 
 // an unloaded chunk's grid cells -> 0
-__global__ void k_sync_clear(const SyncArgs A, int2 o, uint32_t *grid) {
+__global__ void k_sync_clear(const SyncArgs A, int2 o, uint32_t *grid, uint8_t *srcnz) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.cells_per_chunk) return;
     const int32_t x = (int32_t)(k % kChunkB), y = (int32_t)((k / kChunkB) % A.wby), z = (int32_t)(k / ((int64_t)kChunkB * A.wby));
-    grid[((size_t)(o.y + z) * A.wby + y) * A.wbx + (size_t)(o.x + x)] = 0u;
+    const size_t pos = ((size_t)(o.y + z) * A.wby + y) * A.wbx + (size_t)(o.x + x);
+    grid[pos] = 0u;
+    srcnz[pos] = 0u;
 }
 
 // coarse_cell1 (gpu_world.rs:944-957) of the chunk's 4^3-brick cells from its own cells (cells == NULL:
@@ -385,11 +399,14 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: null or negative argument");
     const int32_t ring_w = W.wb[0] / kChunkB, ring_d = W.wb[2] / kChunkB;
     const int64_t per = (int64_t)kChunkB * W.wb[1] * kChunkB;
-    // chunks once per ring slot, in order; loaded ones concatenated
+    // chunks once per ring slot, in order; loaded ones concatenated with their bricks renumbered by first
+    // mention (only the bricks the chunks name travel to the device)
     std::vector<uint8_t> done((size_t)ring_w * ring_d, 0);
     std::vector<const vf_chunk *> list;
     std::vector<int2> org_loaded;
     std::vector<uint32_t> cat;
+    std::vector<int32_t> loc(n_bricks, -1);
+    std::vector<uint32_t> named;                     // local brick -> caller's brick
     for (int32_t i = 0; i < n_chunks; ++i) {
         const vf_chunk &c = chunks[i];
         if (c.cx < -(1 << 26) || c.cx >= (1 << 26) || c.cz < -(1 << 26) || c.cz >= (1 << 26))
@@ -398,19 +415,56 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         if (done[slot]) continue;
         done[slot] = 1;
         list.push_back(&c);
-        if (c.loaded) {
-            if (!c.cells) return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: loaded chunk (%d, %d) without cells", c.cx, c.cz);
-            for (int64_t k = 0; k < per; ++k)
-                if (c.cells[k] >= 256u && c.cells[k] - 256u >= n_bricks)
-                    return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: chunk (%d, %d) names brick %u of %u", c.cx, c.cz,
-                                    c.cells[k] - 256u, n_bricks);
-            org_loaded.push_back(make_int2((c.cx * kChunkB) & (W.wb[0] - 1), (c.cz * kChunkB) & (W.wb[2] - 1)));
-            cat.insert(cat.end(), c.cells, c.cells + per);
+        if (!c.loaded) continue;
+        if (!c.cells) return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: loaded chunk (%d, %d) without cells", c.cx, c.cz);
+        if ((int64_t)cat.size() + per > (int64_t)INT32_MAX)
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: more than 2^31 cells in one call");
+        org_loaded.push_back(make_int2((c.cx * kChunkB) & (W.wb[0] - 1), (c.cz * kChunkB) & (W.wb[2] - 1)));
+        const size_t base = cat.size();
+        cat.insert(cat.end(), c.cells, c.cells + per);
+        for (size_t k = base; k < cat.size(); ++k) {
+            const uint32_t v = cat[k];
+            if (v < 256u) continue;
+            if (v - 256u >= n_bricks)
+                return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: chunk (%d, %d) names brick %u of %u", c.cx, c.cz, v - 256u,
+                                n_bricks);
+            int32_t &l = loc[v - 256u];
+            if (l < 0) {
+                l = (int32_t)named.size();
+                named.push_back(v - 256u);
+            }
+            cat[k] = 256u + (uint32_t)l;
         }
     }
     const int64_t n = (int64_t)cat.size();
-    if (n > (int64_t)INT32_MAX) return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: %lld cells in one call", (long long)n);
-    const int64_t nb = std::max<int64_t>(n_bricks, 1);
+    const uint32_t nl = (uint32_t)named.size();
+    const int64_t nb = std::max<int64_t>(nl, 1);
+    // one pinned staging block: the named bricks, their slots, the cells (one DMA each way)
+    const size_t off_slot = (size_t)nl * 512u, off_cells = off_slot + (size_t)nb * 4u, stage = off_cells + (size_t)n * 4u;
+    if (stage > ctx->pin_cap) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_cap = 0;
+        const size_t want = std::max<size_t>(stage + stage / 4, (size_t)1 << 20);
+        if (hipHostMalloc((void **)&ctx->pin, want, hipHostMallocDefault) != hipSuccess)
+            return vf::fail(ctx, VF_E_OOM, "vf_world_sync_chunks: %zu bytes of pinned staging", want);
+        ctx->pin_cap = want;
+    }
+    uint8_t *pb = ctx->pin;
+    int32_t *pslot = reinterpret_cast<int32_t *>(pb + off_slot);
+    uint32_t *pcells = reinterpret_cast<uint32_t *>(pb + off_cells);
+    {
+        const unsigned nt = nl >= 4096u ? std::max(1u, std::min(8u, std::thread::hardware_concurrency())) : 1u;
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t l = t; l < nl; l += nt) std::memcpy(pb + l * 512u, bricks + (size_t)named[l] * 512u, 512);
+            });
+        for (size_t l = 0; l < nl; l += nt) std::memcpy(pb + l * 512u, bricks + (size_t)named[l] * 512u, 512);
+        for (auto &x : th) x.join();
+    }
+    for (uint32_t l = 0; l < nl; ++l) pslot[l] = gpu_slot[named[l]];
+    if (n) std::memcpy(pcells, cat.data(), (size_t)n * 4u);
     // released slots that hold world bricks, highest first (gpu_world.rs:1199-1207); taken below, once
     // nothing can fail before the world changes
     std::vector<int32_t> rel(released, released + n_released);
@@ -420,23 +474,36 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
                              [&](int32_t sl) { return sl < 0 || (size_t)sl >= ctx->world_slot.size() || !ctx->world_slot[(size_t)sl]; }),
               rel.end());
     hipStream_t st = ctx->stream;
-    DevTemp t;
-    uint32_t *d_cells = nullptr, *first = nullptr, *flag = nullptr, *rank = nullptr, *major = nullptr, *d_free = nullptr;
-    int32_t *slot = nullptr, *written = nullptr;
-    uint8_t *ref = nullptr, *d_bricks = nullptr;
-    int2 *d_org = nullptr;
-    hipError_t e;
     const size_t nf = ctx->free_slots.size() + rel.size();
+    const int64_t nc = std::max<int64_t>(n, 1);
     size_t scan_bytes = 0;
-    uint8_t *scan_tmp = nullptr;
-    if (n) VF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, rank, (int)n, st));
-    if ((e = t.alloc(scan_tmp, std::max<size_t>(scan_bytes, 1))) != hipSuccess ||
-        (e = t.alloc(d_cells, std::max<int64_t>(n, 1))) != hipSuccess || (e = t.alloc(first, nb)) != hipSuccess ||
-        (e = t.alloc(flag, std::max<int64_t>(n, 1))) != hipSuccess || (e = t.alloc(rank, std::max<int64_t>(n, 1))) != hipSuccess ||
-        (e = t.alloc(major, nb)) != hipSuccess || (e = t.alloc(d_free, std::max<size_t>(nf, 1))) != hipSuccess ||
-        (e = t.alloc(slot, nb)) != hipSuccess || (e = t.alloc(written, nb)) != hipSuccess || (e = t.alloc(ref, nb)) != hipSuccess ||
-        (e = t.alloc(d_bricks, (size_t)nb * 512u)) != hipSuccess || (e = t.alloc(d_org, std::max<size_t>(org_loaded.size(), 1))) != hipSuccess)
-        return vf::fail(ctx, VF_E_OOM, "vf_world_sync_chunks: hipMalloc: %s", hipGetErrorString(e));
+    if (n) VF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, st));
+    // device workspace, carved in 256-byte aligned pieces; kept by the context and grown on demand
+    size_t ws = 0;
+    const auto take = [&ws](size_t bytes) { const size_t o = ws; ws += (std::max<size_t>(bytes, 1) + 255u) & ~(size_t)255u; return o; };
+    const size_t o_stage = take(stage), o_scan = take(scan_bytes), o_enc = take((size_t)nc * 4u), o_flag = take((size_t)nc * 4u),
+                 o_rank = take((size_t)nc * 4u), o_first = take((size_t)nb * 4u), o_major = take((size_t)nb * 4u),
+                 o_free = take(std::max<size_t>(nf, 1) * 4u), o_written = take((size_t)nb * 4u), o_gocc = take((size_t)nb * 64u),
+                 o_org = take(std::max<size_t>(org_loaded.size(), 1) * sizeof(int2));
+    if (ws > ctx->sync_ws_cap) {
+        VF_HIP(ctx, hipStreamSynchronize(st));
+        if (ctx->sync_ws) (void)hipFree(ctx->sync_ws);
+        ctx->sync_ws = nullptr;
+        ctx->sync_ws_cap = 0;
+        const size_t want = ws + ws / 4;
+        hipError_t e = hipMalloc((void **)&ctx->sync_ws, want);
+        if (e != hipSuccess) return vf::fail(ctx, VF_E_OOM, "vf_world_sync_chunks: hipMalloc(%zu): %s", want, hipGetErrorString(e));
+        ctx->sync_ws_cap = want;
+    }
+    uint8_t *wsb = ctx->sync_ws;
+    uint8_t *d_stage = wsb + o_stage, *scan_tmp = wsb + o_scan, *d_bricks = d_stage;
+    uint32_t *enc = (uint32_t *)(wsb + o_enc), *flag = (uint32_t *)(wsb + o_flag), *rank = (uint32_t *)(wsb + o_rank),
+             *first = (uint32_t *)(wsb + o_first), *major = (uint32_t *)(wsb + o_major), *d_free = (uint32_t *)(wsb + o_free),
+             *gocc = (uint32_t *)(wsb + o_gocc);
+    int32_t *written = (int32_t *)(wsb + o_written);
+    int2 *d_org = (int2 *)(wsb + o_org);
+    int32_t *slot = reinterpret_cast<int32_t *>(d_stage + off_slot);
+    uint32_t *d_cells = reinterpret_cast<uint32_t *>(d_stage + off_cells);
     for (int32_t sl : rel) {
         ctx->world_slot[(size_t)sl] = 0;
         ctx->free_slots.push_back((uint32_t)sl);
@@ -450,17 +517,12 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
     A.cap = ctx->n_slots;
     uint32_t total_new = 0;
     if (n) {
-        VF_HIP(ctx, hipMemcpyAsync(d_cells, cat.data(), (size_t)n * 4u, hipMemcpyHostToDevice, st));
+        VF_HIP(ctx, hipMemcpyAsync(d_stage, pb, stage, hipMemcpyHostToDevice, st));
         VF_HIP(ctx, hipMemcpyAsync(d_org, org_loaded.data(), org_loaded.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-        if (n_bricks) {
-            VF_HIP(ctx, hipMemcpyAsync(d_bricks, bricks, (size_t)n_bricks * 512u, hipMemcpyHostToDevice, st));
-            VF_HIP(ctx, hipMemcpyAsync(slot, gpu_slot, (size_t)n_bricks * 4u, hipMemcpyHostToDevice, st));
-        }
         if (nf) VF_HIP(ctx, hipMemcpyAsync(d_free, ctx->free_slots.data(), nf * 4u, hipMemcpyHostToDevice, st));
-        VF_HIP(ctx, hipMemsetAsync(ref, 0, (size_t)nb, st));
         hipLaunchKernelGGL(k_fill_i32, dim3(vf::blocks_for(nb)), dim3(256), 0, st, (int32_t *)first, -1, nb);
         hipLaunchKernelGGL(k_sync_first, dim3(vf::blocks_for(n)), dim3(256), 0, st, (const uint32_t *)d_cells, n,
-                           (const int32_t *)slot, first, ref);
+                           (const int32_t *)slot, first);
         hipLaunchKernelGGL(k_sync_flags, dim3(vf::blocks_for(n)), dim3(256), 0, st, (const uint32_t *)d_cells, n,
                            (const int32_t *)slot, (const uint32_t *)first, flag);
         if ((rc = vf::launch_check(ctx, "chunk sync"))) return rc;
@@ -472,19 +534,21 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         total_new = last[0] + last[1];
         hipLaunchKernelGGL(k_sync_assign, dim3(vf::blocks_for(n)), dim3(256), 0, st, A, (const uint32_t *)d_cells, n,
                            (const uint32_t *)flag, (const uint32_t *)rank, (const uint32_t *)d_free, slot);
-        if (n_bricks && total_new > nf && (uint64_t)A.used + (total_new - nf) > A.cap)
-            hipLaunchKernelGGL(k_majority, dim3(n_bricks), dim3(256), 0, st, (const uint8_t *)d_bricks, (const int32_t *)slot, major);
+        if (nl && total_new > nf && (uint64_t)A.used + (total_new - nf) > A.cap)
+            hipLaunchKernelGGL(k_majority, dim3(nl), dim3(256), 0, st, (const uint8_t *)d_bricks, (const int32_t *)slot, major);
         hipLaunchKernelGGL(k_sync_encode, dim3(vf::blocks_for(n)), dim3(256), 0, st, A, (const int2 *)d_org, (const uint32_t *)d_cells,
-                           n, (const int32_t *)slot, (const uint32_t *)major, ctx->grid);
-        if (n_bricks) {
-            hipLaunchKernelGGL(k_sync_written, dim3(vf::blocks_for(nb)), dim3(256), 0, st, (const int32_t *)slot, (const uint8_t *)ref,
-                               nb, written);
-            const int64_t pieces = (int64_t)n_bricks * 32, words = (int64_t)n_bricks * 16;
+                           n, (const int32_t *)slot, (const uint32_t *)major, ctx->grid, ctx->srcnz, enc);
+        if (nl) {
+            // every named brick with a slot is written (encode_brick's `needs upload`)
+            hipLaunchKernelGGL(k_sync_written, dim3(vf::blocks_for(nb)), dim3(256), 0, st, (const int32_t *)slot, nb, written);
+            const int64_t pieces = (int64_t)nl * 32, words = (int64_t)nl * 16;
             hipLaunchKernelGGL(k_scatter, dim3(vf::blocks_for(pieces)), dim3(256), 0, st, (const uint4 *)d_bricks, pieces,
                                (const int32_t *)written, (uint4 *)ctx->atlas);
             hipLaunchKernelGGL(k_brick_mask, dim3(vf::blocks_for(words)), dim3(256), 0, st, (const uint8_t *)d_bricks, words,
                                (const int32_t *)written, ctx->masks_ref);
-            if ((rc = vf::masks_for(ctx, (const uint32_t *)written, n_bricks))) return rc;   // -1 entries are skipped
+            if ((rc = vf::masks_for(ctx, (const uint32_t *)written, nl))) return rc;   // -1 entries are skipped
+            hipLaunchKernelGGL(k_sync_occ, dim3(vf::blocks_for(words)), dim3(256), 0, st, (const int32_t *)written, words,
+                               (const uint32_t *)ctx->occ, gocc);
         }
     }
     // cleared chunks, coarse cells of every synced chunk
@@ -493,7 +557,7 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         const int2 o = make_int2((c->cx * kChunkB) & (W.wb[0] - 1), (c->cz * kChunkB) & (W.wb[2] - 1));
         const uint32_t *cells = nullptr;
         if (c->loaded) cells = d_cells + (li++) * per;
-        else hipLaunchKernelGGL(k_sync_clear, dim3(vf::blocks_for(per)), dim3(256), 0, st, A, o, ctx->grid);
+        else hipLaunchKernelGGL(k_sync_clear, dim3(vf::blocks_for(per)), dim3(256), 0, st, A, o, ctx->grid, ctx->srcnz);
         const int32_t c1per = (kChunkB / 4) * ctx->c1d[1] * (kChunkB / 4);
         hipLaunchKernelGGL(k_sync_coarse1, dim3(vf::blocks_for(c1per)), dim3(256), 0, st, A, o, cells, ctx->c1d[0], ctx->c1d[1], ctx->c1);
         hipLaunchKernelGGL(k_sync_coarse2, dim3(vf::blocks_for(ctx->c2d[1])), dim3(256), 0, st, o, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2],
@@ -507,31 +571,317 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         const int32_t flo[3] = {lo[0] - 16, lo[1] - 16, lo[2] - 16}, fhi[3] = {hi[0] + 16, hi[1] + 16, hi[2] + 16};
         if ((rc = vf::distance_box(ctx, flo, fhi, lo, hi))) return rc;
     }
-    if ((rc = vf::refresh_world(ctx))) return rc;
-    // bookkeeping: the slots popped and handed out, world-owned slots, the callers' gpu_slot
-    std::vector<int32_t> out((size_t)nb, -1);
-    if (n && n_bricks) {
-        VF_HIP(ctx, hipMemcpyAsync(out.data(), slot, (size_t)n_bricks * 4u, hipMemcpyDeviceToHost, st));
-        std::vector<int32_t> wr((size_t)n_bricks);
-        VF_HIP(ctx, hipMemcpyAsync(wr.data(), written, (size_t)n_bricks * 4u, hipMemcpyDeviceToHost, st));
-        VF_HIP(ctx, hipMemcpyAsync(ctx->host_grid.data(), ctx->grid, ctx->n_cells * 4u, hipMemcpyDeviceToHost, st));
-        VF_HIP(ctx, hipStreamSynchronize(st));
-        const uint32_t pops = std::min<uint32_t>(total_new, (uint32_t)nf);
-        ctx->free_slots.resize(nf - pops);
-        const uint32_t fresh = total_new > nf ? std::min<uint32_t>(total_new - (uint32_t)nf, A.cap - A.used) : 0u;
-        ctx->atlas_used += fresh;
-        for (uint32_t b = 0; b < n_bricks; ++b) {
-            gpu_slot[b] = out[b] >= 0 ? out[b] : -1;
-            if (wr[b] >= 0) {
-                ctx->world_slot[(size_t)wr[b]] = 1;
-                uint8_t vox[512];
-                for (int k = 0; k < 512; ++k) vox[k] = bricks[(size_t)b * 512u + k] == kTorchFlame ? 0 : bricks[(size_t)b * 512u + k];
-                vf::slot_occupancy(vox, &ctx->host_occ[(size_t)wr[b] * 8u]);
+    if ((rc = vf::cell_words(ctx))) return rc;
+    for (const vf_chunk *c : list) {   // record_grass_top over each chunk's columns (gpu_world.rs:1358)
+        const int32_t tx = (c->cx * kChunkB) & (W.wb[0] - 1), tz = (c->cz * kChunkB) & (W.wb[2] - 1);
+        if ((rc = vf::skytop_columns(ctx, tx * 8, tz * 8, kChunkB * 8, kChunkB * 8))) return rc;
+    }
+    // bookkeeping: the host grid copy, the slots popped and handed out, world-owned slots, the caller's gpu_slot
+    // back through the staging block: slots over the brick voxels' place, occupancy rows after them
+    int32_t *out = pslot;
+    uint32_t *hocc = reinterpret_cast<uint32_t *>(pb);
+    if (n) {
+        VF_HIP(ctx, hipMemcpyAsync(pcells, enc, (size_t)n * 4u, hipMemcpyDeviceToHost, st));
+        if (nl) {
+            VF_HIP(ctx, hipMemcpyAsync(pslot, slot, (size_t)nl * 4u, hipMemcpyDeviceToHost, st));
+            VF_HIP(ctx, hipMemcpyAsync(hocc, gocc, (size_t)nl * 64u, hipMemcpyDeviceToHost, st));
+        }
+    }
+    VF_HIP(ctx, hipStreamSynchronize(st));
+    li = 0;
+    for (const vf_chunk *c : list) {
+        const int2 o = make_int2((c->cx * kChunkB) & (W.wb[0] - 1), (c->cz * kChunkB) & (W.wb[2] - 1));
+        const uint32_t *src = c->loaded ? pcells + (li++) * per : nullptr;
+        for (int32_t z = 0; z < kChunkB; ++z)
+            for (int32_t y = 0; y < W.wb[1]; ++y) {
+                uint32_t *row = &ctx->host_grid[((size_t)(o.y + z) * W.wb[1] + y) * W.wb[0] + o.x];
+                if (src) std::memcpy(row, src + ((size_t)z * W.wb[1] + y) * kChunkB, kChunkB * 4u);
+                else std::memset(row, 0, kChunkB * 4u);
+            }
+    }
+    const uint32_t pops = std::min<uint32_t>(total_new, (uint32_t)nf);
+    ctx->free_slots.resize(nf - pops);
+    ctx->atlas_used += total_new > nf ? std::min<uint32_t>(total_new - (uint32_t)nf, A.cap - A.used) : 0u;
+    for (uint32_t l = 0; l < nl; ++l) {
+        gpu_slot[named[l]] = out[l] >= 0 ? out[l] : -1;
+        if (out[l] < 0) continue;
+        ctx->world_slot[(size_t)out[l]] = 1;
+        std::memcpy(&ctx->host_occ[(size_t)out[l] * 8u], hocc + (size_t)l * 16u, 64);
+    }
+    return VF_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Edit sync (GpuWorld::sync, gpu_world.rs:985-1142)
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int32_t kDistCap = 16;   // DIST_CAP (gpu_world.rs:31)
+
+// the dirty cells' encoded and source values
+__global__ void k_edit_cells(const uint32_t *idx, const uint32_t *enc, const uint32_t *src, int64_t n, uint32_t *grid,
+                             uint8_t *srcnz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    grid[idx[i]] = enc[i];
+    srcnz[idx[i]] = src[i] != 0u ? 1u : 0u;
+}
+
+// coarse_cell1 (gpu_world.rs:944-957) of the listed cells, over the source cells
+__global__ void k_coarse1_list(const uint32_t *list, int64_t n, const uint8_t *srcnz, int32_t wbx, int32_t wby, int32_t wbz,
+                               int32_t c1x, int32_t c1y, uint8_t *c1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t ci = list[i];
+    const int32_t cx = (int32_t)(ci % (uint32_t)c1x), cy = (int32_t)((ci / (uint32_t)c1x) % (uint32_t)c1y),
+                  cz = (int32_t)(ci / ((uint32_t)c1x * (uint32_t)c1y));
+    uint8_t occ = 0u;
+    for (int32_t z = cz * 4; z < min(cz * 4 + 4, wbz) && !occ; ++z)
+        for (int32_t y = cy * 4; y < min(cy * 4 + 4, wby) && !occ; ++y)
+            for (int32_t x = cx * 4; x < min(cx * 4 + 4, wbx); ++x)
+                if (srcnz[((size_t)z * wby + y) * wbx + x]) { occ = 1u; break; }
+    c1[ci] = occ;
+}
+
+// coarse_cell2 (gpu_world.rs:959-973) of the listed cells, over coarse1
+__global__ void k_coarse2_list(const uint32_t *list, int64_t n, const uint8_t *c1, int32_t c1x, int32_t c1y, int32_t c1z,
+                               int32_t c2x, int32_t c2y, uint8_t *c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t ci = list[i];
+    const int32_t cx = (int32_t)(ci % (uint32_t)c2x), cy = (int32_t)((ci / (uint32_t)c2x) % (uint32_t)c2y),
+                  cz = (int32_t)(ci / ((uint32_t)c2x * (uint32_t)c2y));
+    uint8_t occ = 0u;
+    for (int32_t z = cz * 4; z < min(cz * 4 + 4, c1z) && !occ; ++z)
+        for (int32_t y = cy * 4; y < min(cy * 4 + 4, c1y) && !occ; ++y)
+            for (int32_t x = cx * 4; x < min(cx * 4 + 4, c1x); ++x)
+                if (c1[((size_t)z * c1y + y) * c1x + x]) { occ = 1u; break; }
+    c2[ci] = occ;
+}
+
+// encode_brick's full-atlas value (gpu_world.rs:202-213) on the host
+uint32_t majority_host(const uint8_t *vox) {
+    uint32_t count[256] = {0};
+    for (int k = 0; k < 512; ++k) count[vox[k]]++;
+    uint32_t best = 1;
+    for (uint32_t m = 2; m < 256; ++m)
+        if (count[m] > count[best] && m != kTorchFlame) best = m;
+    return count[best] * 2u >= 512u ? best : 0u;
+}
+
+struct BrickBox {
+    int32_t lo[3], hi[3];
+};
+
+}  // namespace
+
+extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t *values, uint32_t n, const uint8_t *bricks,
+                                   uint32_t n_bricks, int32_t *gpu_slot) {
+    int rc = vf::check_ready(ctx, "vf_world_sync_edits");
+    if (rc) return rc;
+    if ((rc = vf::no_capture(ctx, "vf_world_sync_edits"))) return rc;
+    if (!n) return VF_OK;
+    if (!cells || !values || (n_bricks && (!bricks || !gpu_slot)))
+        return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_edits: null argument");
+    const DevWorld &W = ctx->W;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (cells[i] >= ctx->n_cells) return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_edits: cell %u of %zu", cells[i], ctx->n_cells);
+        if (values[i] >= 256u && values[i] - 256u >= n_bricks)
+            return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_edits: cell %u names brick %u of %u", cells[i], values[i] - 256u, n_bricks);
+    }
+    // the dirty cells sorted, once each (a cell listed twice takes its last value)
+    std::vector<uint32_t> ord(n);
+    for (uint32_t i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return cells[a] < cells[b]; });
+    std::vector<uint32_t> dirty, val;
+    std::vector<uint32_t> pos_of(n);   // input entry -> dirty index
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = ord[k];
+        if (dirty.empty() || dirty.back() != cells[i]) {
+            dirty.push_back(cells[i]);
+            val.push_back(values[i]);
+        } else {
+            val.back() = values[i];
+        }
+        pos_of[i] = (uint32_t)dirty.size() - 1u;
+    }
+    const size_t nd = dirty.size();
+    // the sequential part (gpu_world.rs:997-1021), on copies of the bookkeeping: compaction of bricks
+    // that became uniform (SparseVolume::compact_brick), then encode_brick with alloc_slot
+    std::vector<uint32_t> fl = ctx->free_slots;
+    uint32_t used = ctx->atlas_used;
+    std::unordered_map<uint32_t, int32_t> slot_of;   // bricks whose slot changes
+    const auto cur_slot = [&](uint32_t b) {
+        auto it = slot_of.find(b);
+        return it != slot_of.end() ? it->second : gpu_slot[b];
+    };
+    std::vector<std::pair<uint32_t, uint8_t>> ws_ops;   // world_slot writes in order
+    std::vector<uint32_t> enc(nd), up;
+    for (size_t j = 0; j < nd; ++j) {
+        if (val[j] >= 256u) {
+            const uint32_t b = val[j] - 256u;
+            const uint8_t *v = bricks + (size_t)b * 512u;
+            bool uni = true;
+            for (int k = 1; k < 512 && uni; ++k) uni = v[k] == v[0];
+            if (uni) {
+                const int32_t s = cur_slot(b);
+                slot_of[b] = -1;
+                val[j] = v[0];
+                if (s >= 0) {
+                    ws_ops.emplace_back((uint32_t)s, 0);
+                    fl.push_back((uint32_t)s);
+                }
             }
         }
-    } else {
-        VF_HIP(ctx, hipMemcpyAsync(ctx->host_grid.data(), ctx->grid, ctx->n_cells * 4u, hipMemcpyDeviceToHost, st));
-        VF_HIP(ctx, hipStreamSynchronize(st));
+        if (val[j] < 256u) {
+            enc[j] = val[j];
+            continue;
+        }
+        const uint32_t b = val[j] - 256u;
+        int32_t s = cur_slot(b);
+        if (s < 0) {
+            if (!fl.empty()) {
+                s = (int32_t)fl.back();
+                fl.pop_back();
+            } else if (used < ctx->n_slots) {
+                s = (int32_t)used++;
+            } else {
+                s = -1;
+            }
+            slot_of[b] = s;
+        }
+        if (s >= 0) {
+            ws_ops.emplace_back((uint32_t)s, 1);
+            enc[j] = 256u + (uint32_t)s;
+            up.push_back(b);
+        } else {
+            enc[j] = majority_host(bricks + (size_t)b * 512u);
+        }
     }
+    std::sort(up.begin(), up.end());
+    up.erase(std::unique(up.begin(), up.end()), up.end());
+    // coarse cells touched (gpu_world.rs:1033-1065)
+    std::vector<uint32_t> c1l(nd), c2l(nd);
+    const int32_t wbx = W.wb[0], wby = W.wb[1], wbz = W.wb[2];
+    for (size_t j = 0; j < nd; ++j) {
+        const int32_t g = (int32_t)dirty[j];
+        const int32_t bx = g % wbx, by = (g / wbx) % wby, bz = g / (wbx * wby);
+        c1l[j] = (uint32_t)(((bz / 4) * ctx->c1d[1] + by / 4) * ctx->c1d[0] + bx / 4);
+        c2l[j] = (uint32_t)(((bz / 16) * ctx->c2d[1] + by / 16) * ctx->c2d[0] + bx / 16);
+    }
+    for (auto *l : {&c1l, &c2l}) {
+        std::sort(l->begin(), l->end());
+        l->erase(std::unique(l->begin(), l->end()), l->end());
+    }
+    // clusters of dirty bricks, absolute coordinates (gpu_world.rs:1103-1130)
+    std::vector<BrickBox> boxes;
+    for (size_t j = 0; j < nd; ++j) {
+        const int32_t g = (int32_t)dirty[j];
+        int32_t b[3] = {g % wbx, (g / wbx) % wby, g / (wbx * wby)};
+        if (W.ring) {   // World::brick_coords: the ring slot's chunk inside the window
+            b[0] = W.minb[0] + (((b[0] - W.minb[0]) % wbx) + wbx) % wbx;
+            b[2] = W.minb[2] + (((b[2] - W.minb[2]) % wbz) + wbz) % wbz;
+        }
+        BrickBox *hit = nullptr;
+        for (auto &x : boxes) {
+            bool in = true;
+            for (int k = 0; k < 3; ++k) in = in && b[k] >= x.lo[k] - kDistCap && b[k] <= x.hi[k] + kDistCap;
+            if (in) { hit = &x; break; }
+        }
+        if (hit) {
+            for (int k = 0; k < 3; ++k) { hit->lo[k] = std::min(hit->lo[k], b[k]); hit->hi[k] = std::max(hit->hi[k], b[k]); }
+        } else {
+            boxes.push_back(BrickBox{{b[0], b[1], b[2]}, {b[0], b[1], b[2]}});
+        }
+    }
+    for (bool merged = true; merged;) {
+        merged = false;
+        for (size_t i = 0; i < boxes.size() && !merged; ++i)
+            for (size_t j = i + 1; j < boxes.size(); ++j) {
+                bool near = true;
+                for (int k = 0; k < 3; ++k)
+                    near = near && boxes[j].lo[k] <= boxes[i].hi[k] + 2 * kDistCap && boxes[j].hi[k] >= boxes[i].lo[k] - 2 * kDistCap;
+                if (near) {
+                    const BrickBox bj = boxes[j];
+                    boxes.erase(boxes.begin() + (ptrdiff_t)j);
+                    for (int k = 0; k < 3; ++k) { boxes[i].lo[k] = std::min(boxes[i].lo[k], bj.lo[k]); boxes[i].hi[k] = std::max(boxes[i].hi[k], bj.hi[k]); }
+                    merged = true;
+                    break;
+                }
+            }
+    }
+    // staging: dirty cells, their encoded and source values, the uploaded bricks and their slots, coarse lists
+    const size_t nu = up.size(), nbk = std::max<size_t>(nu, 1);
+    size_t hs = 0;
+    const auto htake = [&hs](size_t bytes) { const size_t o = hs; hs += (std::max<size_t>(bytes, 1) + 255u) & ~(size_t)255u; return o; };
+    const size_t h_br = htake(nu * 512u), h_sl = htake(nbk * 4u), h_idx = htake(nd * 4u), h_enc = htake(nd * 4u), h_src = htake(nd * 4u),
+                 h_c1 = htake(c1l.size() * 4u), h_c2 = htake(c2l.size() * 4u), h_occ = htake(nbk * 64u);
+    if (hs > ctx->pin_cap) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_cap = 0;
+        const size_t want = std::max<size_t>(hs + hs / 4, (size_t)1 << 20);
+        if (hipHostMalloc((void **)&ctx->pin, want, hipHostMallocDefault) != hipSuccess)
+            return vf::fail(ctx, VF_E_OOM, "vf_world_sync_edits: %zu bytes of pinned staging", want);
+        ctx->pin_cap = want;
+    }
+    if (hs > ctx->sync_ws_cap) {
+        VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->sync_ws) (void)hipFree(ctx->sync_ws);
+        ctx->sync_ws = nullptr;
+        ctx->sync_ws_cap = 0;
+        const size_t want = hs + hs / 4;
+        hipError_t e = hipMalloc((void **)&ctx->sync_ws, want);
+        if (e != hipSuccess) return vf::fail(ctx, VF_E_OOM, "vf_world_sync_edits: hipMalloc(%zu): %s", want, hipGetErrorString(e));
+        ctx->sync_ws_cap = want;
+    }
+    uint8_t *pb = ctx->pin, *db = ctx->sync_ws;
+    int32_t *pslot = (int32_t *)(pb + h_sl);
+    for (size_t u = 0; u < nu; ++u) {
+        std::memcpy(pb + h_br + u * 512u, bricks + (size_t)up[u] * 512u, 512);
+        pslot[u] = cur_slot(up[u]);
+    }
+    std::memcpy(pb + h_idx, dirty.data(), nd * 4u);
+    std::memcpy(pb + h_enc, enc.data(), nd * 4u);
+    std::memcpy(pb + h_src, val.data(), nd * 4u);
+    std::memcpy(pb + h_c1, c1l.data(), c1l.size() * 4u);
+    std::memcpy(pb + h_c2, c2l.data(), c2l.size() * 4u);
+    hipStream_t st = ctx->stream;
+    VF_HIP(ctx, hipMemcpyAsync(db, pb, h_occ, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_edit_cells, dim3(vf::blocks_for((int64_t)nd)), dim3(256), 0, st, (const uint32_t *)(db + h_idx),
+                       (const uint32_t *)(db + h_enc), (const uint32_t *)(db + h_src), (int64_t)nd, ctx->grid, ctx->srcnz);
+    if (nu) {
+        const int32_t *dsl = (const int32_t *)(db + h_sl);
+        hipLaunchKernelGGL(k_scatter, dim3(vf::blocks_for((int64_t)nu * 32)), dim3(256), 0, st, (const uint4 *)(db + h_br),
+                           (int64_t)nu * 32, dsl, (uint4 *)ctx->atlas);
+        hipLaunchKernelGGL(k_brick_mask, dim3(vf::blocks_for((int64_t)nu * 16)), dim3(256), 0, st, (const uint8_t *)(db + h_br),
+                           (int64_t)nu * 16, dsl, ctx->masks_ref);
+        if ((rc = vf::masks_for(ctx, (const uint32_t *)dsl, (uint32_t)nu))) return rc;
+        hipLaunchKernelGGL(k_sync_occ, dim3(vf::blocks_for((int64_t)nu * 16)), dim3(256), 0, st, dsl, (int64_t)nu * 16,
+                           (const uint32_t *)ctx->occ, (uint32_t *)(db + h_occ));
+    }
+    hipLaunchKernelGGL(k_coarse1_list, dim3(vf::blocks_for((int64_t)c1l.size())), dim3(256), 0, st, (const uint32_t *)(db + h_c1),
+                       (int64_t)c1l.size(), (const uint8_t *)ctx->srcnz, wbx, wby, wbz, ctx->c1d[0], ctx->c1d[1], ctx->c1);
+    hipLaunchKernelGGL(k_coarse2_list, dim3(vf::blocks_for((int64_t)c2l.size())), dim3(256), 0, st, (const uint32_t *)(db + h_c2),
+                       (int64_t)c2l.size(), (const uint8_t *)ctx->c1, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2], ctx->c2d[0], ctx->c2d[1],
+                       ctx->c2);
+    if ((rc = vf::launch_check(ctx, "edit sync"))) return rc;
+    for (const BrickBox &x : boxes) {   // grass tops, then the distance field, per cluster (gpu_world.rs:1133-1141)
+        if ((rc = vf::skytop_box(ctx, x.lo[0] * 8, x.lo[2] * 8, x.hi[0] * 8 + 7, x.hi[2] * 8 + 7))) return rc;
+        const int32_t lo[3] = {x.lo[0] - kDistCap, x.lo[1] - kDistCap, x.lo[2] - kDistCap};
+        const int32_t hi[3] = {x.hi[0] + kDistCap, x.hi[1] + kDistCap, x.hi[2] + kDistCap};
+        if ((rc = vf::distance_box(ctx, lo, hi, x.lo, x.hi))) return rc;
+    }
+    if ((rc = vf::cell_words(ctx))) return rc;
+    if (nu) VF_HIP(ctx, hipMemcpyAsync(pb + h_occ, db + h_occ, nu * 64u, hipMemcpyDeviceToHost, st));
+    VF_HIP(ctx, hipStreamSynchronize(st));
+    // commit the bookkeeping
+    ctx->free_slots.swap(fl);
+    ctx->atlas_used = used;
+    for (const auto &op : ws_ops) ctx->world_slot[op.first] = op.second;
+    for (size_t j = 0; j < nd; ++j) ctx->host_grid[dirty[j]] = enc[j];
+    for (size_t u = 0; u < nu; ++u) std::memcpy(&ctx->host_occ[(size_t)pslot[u] * 8u], pb + h_occ + u * 64u, 64);
+    for (const auto &kv : slot_of) gpu_slot[kv.first] = kv.second;
+    for (uint32_t i = 0; i < n; ++i) values[i] = val[pos_of[i]];
     return VF_OK;
 }

@@ -572,12 +572,16 @@ __global__ void __launch_bounds__(256) k_pack(const uint32_t *grid, const uint8_
 
 // Sky-top map (grasstop.comp:121-146 skyTop): 1 + the highest voxel of the column whose material
 // kind is neither water (1) nor foliage (3); 0 if none.  Columns in grid (wrapped) coordinates.
+// One thread per column of the window [x0, x0 + nx) x [z0, z0 + nz) (voxels).
 __global__ void __launch_bounds__(256) k_skytop(const uint32_t *grid, const uint8_t *atlas, const float4 *palette,
-                                                int32_t wbx, int32_t wby, int32_t wbz, uint16_t *skytop) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int32_t sx = wbx * 8;
-    if (i >= (int64_t)sx * wbz * 8) return;
-    const int32_t x = (int32_t)(i % sx), z = (int32_t)(i / sx);
+                                                int32_t wbx, int32_t wby, int32_t wbz, int32_t x0, int32_t z0, int32_t nx,
+                                                int32_t nz, uint16_t *skytop) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= (int64_t)nx * nz) return;
+    // grid (wrapped) column: a ring world's window may start at negative absolute voxels and cross its edge
+    const int32_t sx = wbx * 8, sz = wbz * 8;
+    const int32_t x = ((x0 + (int32_t)(j % nx)) % sx + sx) % sx, z = ((z0 + (int32_t)(j / nx)) % sz + sz) % sz;
+    const int64_t i = (int64_t)z * sx + x;
     uint32_t sky = 0u;
     for (int32_t by = wby - 1; by >= 0 && sky == 0u; --by) {
         const uint32_t g = grid[((size_t)(z >> 3) * wby + by) * wbx + (x >> 3)];
@@ -592,9 +596,15 @@ __global__ void __launch_bounds__(256) k_skytop(const uint32_t *grid, const uint
     skytop[i] = (uint16_t)sky;
 }
 
-__global__ void k_update_cells(uint32_t *grid, const uint32_t *idx, const uint32_t *val, uint32_t n) {
+__global__ void k_update_cells(uint32_t *grid, uint8_t *srcnz, const uint32_t *idx, const uint32_t *val, uint32_t n) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) grid[idx[i]] = val[i];
+    if (i >= n) return;
+    grid[idx[i]] = val[i];
+    srcnz[idx[i]] = val[i] != 0u ? 1u : 0u;
+}
+__global__ void k_nonzero(const uint32_t *grid, int64_t n, uint8_t *nz) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) nz[i] = grid[i] != 0u ? 1u : 0u;
 }
 __global__ void k_update_slots(uint8_t *atlas, const uint32_t *slots, const uint8_t *vox, uint32_t n) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;   // one thread per 4 bytes
@@ -676,13 +686,14 @@ namespace {
 
 void free_world(vf_ctx *ctx) {
     void *ptrs[] = {ctx->grid, ctx->cells, ctx->atlas, ctx->occ, ctx->occdry, ctx->masks_ref, ctx->dist[0], ctx->dist[1],
-                    ctx->dist[2], ctx->dist[3], ctx->c1, ctx->c2, ctx->skytop, ctx->palette};
+                    ctx->dist[2], ctx->dist[3], ctx->c1, ctx->c2, ctx->skytop, ctx->palette, ctx->srcnz};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ctx->grid = nullptr; ctx->cells = nullptr; ctx->atlas = nullptr; ctx->occ = nullptr; ctx->occdry = nullptr;
     ctx->masks_ref = nullptr;
     for (auto &d : ctx->dist) d = nullptr;
     ctx->c1 = ctx->c2 = nullptr;
+    ctx->srcnz = nullptr;
     ctx->skytop = nullptr;
     ctx->palette = nullptr;
     ctx->has_world = false;
@@ -709,13 +720,16 @@ int repack(vf_ctx *ctx) {
     return launch_check(ctx, "cell words");
 }
 
-int rebuild_skytop(vf_ctx *ctx) {
+int skytop_window(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t nx, int32_t nz) {
     const int32_t *wb = ctx->W.wb;
-    const int64_t n = (int64_t)wb[0] * 8 * wb[2] * 8;
+    const int64_t n = (int64_t)nx * nz;
+    if (n <= 0) return VF_OK;
     hipLaunchKernelGGL(k_skytop, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, ctx->grid, ctx->atlas, ctx->palette, wb[0],
-                       wb[1], wb[2], ctx->skytop);
+                       wb[1], wb[2], x0, z0, nx, nz, ctx->skytop);
     return launch_check(ctx, "sky-top");
 }
+
+int rebuild_skytop(vf_ctx *ctx) { return skytop_window(ctx, 0, 0, ctx->W.wb[0] * 8, ctx->W.wb[2] * 8); }
 
 int dist_pass(vf_ctx *ctx, const int32_t bmin[3], const int32_t bmax[3], int axis, int fp, const uint8_t *src, uint8_t *dst) {
     DistArgs A;
@@ -970,6 +984,7 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     for (auto &p : ctx->dist) VF_ALLOC(p, n);
     VF_ALLOC(ctx->c1, n1);
     VF_ALLOC(ctx->c2, n2);
+    VF_ALLOC(ctx->srcnz, n);
     VF_ALLOC(ctx->skytop, nsky * 2);
     VF_ALLOC(ctx->palette, 256 * 48);
 #undef VF_ALLOC
@@ -983,9 +998,14 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
 
 // Everything derived from the grid and the atlas once both are on the device: occupancy masks, the
 // coarse pyramid (unless the caller's was copied), the distance fields, the cell words, the sky-top map.
-int world_finish(vf_ctx *ctx, bool coarse_given) {
+int world_finish(vf_ctx *ctx, bool coarse_given, bool source_given) {
     int rc;
     DevWorld &W = ctx->W;
+    if (!source_given) {   // the source grid is not known: its non-air cells are the encoded grid's
+        hipLaunchKernelGGL(k_nonzero, dim3(blocks_for((int64_t)ctx->n_cells)), dim3(256), 0, ctx->stream,
+                           (const uint32_t *)ctx->grid, (int64_t)ctx->n_cells, ctx->srcnz);
+        if ((rc = launch_check(ctx, "source cells"))) return rc;
+    }
     if (ctx->n_slots) {
         hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)ctx->n_slots * 16)), dim3(256), 0, ctx->stream, ctx->atlas, nullptr,
                            0u, ctx->n_slots, ctx->occ, ctx->occdry);
@@ -1012,6 +1032,8 @@ int world_finish(vf_ctx *ctx, bool coarse_given) {
 // any coarse1 cell (gpu_world.rs:300-313).
 int build_coarse(vf_ctx *ctx, const uint32_t *d_grid) {
     const int32_t *wb = ctx->W.wb;
+    hipLaunchKernelGGL(k_nonzero, dim3(blocks_for((int64_t)ctx->n_cells)), dim3(256), 0, ctx->stream, d_grid,
+                       (int64_t)ctx->n_cells, ctx->srcnz);
     const int64_t n1 = (int64_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2];
     const int64_t n2 = (int64_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
     hipLaunchKernelGGL(k_coarse1, dim3(blocks_for(n1)), dim3(256), 0, ctx->stream, d_grid, wb[0], wb[1], wb[2], ctx->c1d[0],
@@ -1032,6 +1054,27 @@ int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n) {
 // The brick distance field over one box (record_distance_field's box semantics).
 int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]) {
     return distance_field(ctx, lo, hi, dlo, dhi);
+}
+
+// The sky-top map over the voxel columns [x0, x0 + nx) x [z0, z0 + nz) (grid coordinates).
+int skytop_columns(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t nx, int32_t nz) { return skytop_window(ctx, x0, z0, nx, nz); }
+
+// The cell words (grid value, distance fields, coarse flags) of the whole grid.
+int cell_words(vf_ctx *ctx) { return repack(ctx); }
+
+// record_grass_top's window (gpu_world.rs:1431-1441): voxel columns [x0, x1] x [z0, z1], absolute in a ring
+// world (at most one ring wide), clamped to the grid otherwise.
+int skytop_box(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t x1, int32_t z1) {
+    const int32_t nx = ctx->W.wb[0] * 8, nz = ctx->W.wb[2] * 8;
+    if (ctx->W.ring) {
+        x1 = std::min(x1, x0 + nx - 1);
+        z1 = std::min(z1, z0 + nz - 1);
+    } else {
+        x0 = std::clamp(x0, 0, nx - 1); x1 = std::clamp(x1, 0, nx - 1);
+        z0 = std::clamp(z0, 0, nz - 1); z1 = std::clamp(z1, 0, nz - 1);
+    }
+    if (x1 < x0 || z1 < z0) return VF_OK;
+    return skytop_window(ctx, x0, z0, x1 - x0 + 1, z1 - z0 + 1);
 }
 
 // Cell words and sky-top map after grid, field or atlas changes.
@@ -1098,6 +1141,8 @@ void vf_ctx_destroy(vf_ctx *ctx) {
     vf::free_queue(ctx);
     vf::free_bodies(ctx);
     if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->sync_ws) (void)hipFree(ctx->sync_ws);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -1239,7 +1284,7 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
             cleanup();
             return vf::fail(ctx, VF_E_DEVICE, "vf_world_update: %s", hipGetErrorString(e));
         }
-        hipLaunchKernelGGL(k_update_cells, dim3(blocks_for(n_cells)), dim3(256), 0, ctx->stream, ctx->grid, (const uint32_t *)d_ci,
+        hipLaunchKernelGGL(k_update_cells, dim3(blocks_for(n_cells)), dim3(256), 0, ctx->stream, ctx->grid, ctx->srcnz, (const uint32_t *)d_ci,
                            (const uint32_t *)d_cv, n_cells);
     }
     for (uint32_t i = 0; i < n_cells; ++i) ctx->host_grid[cell_idx[i]] = cell_val[i];

@@ -56,6 +56,7 @@ _SIGS = {
     "vf_world_read": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
     "vf_world_sync_chunks": (C.c_int, [_P, C.POINTER(Chunk), C.c_int32, _P, C.c_uint32, _P, _P, C.c_int32]),
     "vf_world_atlas_state": (C.c_int, [_P, _P, _P, _P]),
+    "vf_world_sync_edits": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "vf_read_coarse": (C.c_int, [_P, _P, _P]),
     "vf_world_update": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32]),
     "vf_build_brickdist": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -479,6 +480,23 @@ class Context:
                                             rel.size))
         del keep
         return slot
+
+    def sync_edits(self, cells, values, bricks, gpu_slot):
+        """vf_world_sync_edits (GpuWorld::sync): the world's edited cells (grid indices) and their values
+        (0 / 1-255 / 256 + brick index); bricks [n, 512] u8; gpu_slot [n] i32.  Returns (values after the
+        compaction of bricks that became uniform, gpu_slot)."""
+        c = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
+        v = np.array(values, dtype=np.uint32).reshape(-1)
+        if v.size != c.size:
+            raise ValueError(f"{v.size} values for {c.size} cells")
+        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1, 512)
+        slot = np.array(gpu_slot, dtype=np.int32).reshape(-1)
+        if slot.size != b.shape[0]:
+            raise ValueError(f"gpu_slot has {slot.size} entries for {b.shape[0]} bricks")
+        self._ck(lib().vf_world_sync_edits(self._h, c.ctypes.data if c.size else None, v.ctypes.data if v.size else None,
+                                           c.size, b.ctypes.data if b.size else None, b.shape[0],
+                                           slot.ctypes.data if slot.size else None))
+        return v, slot
 
     def atlas_state(self):
         """(slots handed out, free-list length, capacity) of the device world's atlas."""
