@@ -169,6 +169,8 @@ typedef struct vf_world_src {
     uint32_t n_bricks;
     uint32_t atlas_slots;        /* atlas capacity; 0 = as many slots as the grid names distinct bricks */
     const vf_material *palette;  /* [256] */
+    uint32_t brick_stride;       /* bytes from one brick's voxels to the next's; 0 = 512 (packed) */
+    uint32_t slot_stride;        /* bytes between slot_of_brick entries (multiple of 4); 0 = 4 */
 } vf_world_src;
 /* GpuWorld::upload's pack (gpu_world.rs:228-330 with encode_brick :191-214 and gpu_voxels :58-67) on the
  * device, then what vf_world_upload builds: each detailed brick gets the next atlas slot at its first
@@ -176,7 +178,9 @@ typedef struct vf_world_src {
  * TORCH_FLAME as air, `masks` of the slot = brick_mask of its voxels (sparse_volume.rs:48-63); past the
  * capacity a brick's cells take its majority material (gpu_world.rs:202-213).  The coarse pyramid
  * comes from the source grid.  slot_of_brick (nullable, [n_bricks]): each brick's slot (the
- * reference's Brick::gpu_slot), -1 for a brick the grid does not name or that did not fit. */
+ * reference's Brick::gpu_slot), -1 for a brick the grid does not name or that did not fit.  With the
+ * strides, `bricks` and `slot_of_brick` may point at the `vox` and `gpu_slot` fields of an array of
+ * the caller's brick records (World::bricks) and are read and written in place. */
 int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot_of_brick);
 /* One chunk of GpuWorld::sync_chunks (gpu_world.rs:1194-1376): 16 x wby x 16 bricks of a ring world. */
 typedef struct vf_chunk {
@@ -191,9 +195,12 @@ typedef struct vf_chunk {
  * its majority material (encode_brick); every brick it names is written to its slot (TORCH_FLAME as air)
  * with its brick masks; its coarse cells come from its cells.  An unloaded chunk's cells and coarse cells
  * are cleared.  Then the distance field over each chunk's box +-16 bricks, the cell words and the
- * sky-top map.  gpu_slot: [n_bricks], in and out (Brick::gpu_slot). */
+ * sky-top map.  gpu_slot: [n_bricks], in and out (Brick::gpu_slot).  brick_stride / slot_stride: bytes
+ * between consecutive bricks' voxels / slots (0 = packed: 512 / 4), so both may point into the caller's
+ * brick records. */
 int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t n_chunks, const uint8_t *bricks, uint32_t n_bricks,
-                         int32_t *gpu_slot, const int32_t *released, int32_t n_released);
+                         uint32_t brick_stride, int32_t *gpu_slot, uint32_t slot_stride, const int32_t *released,
+                         int32_t n_released);
 /* GpuWorld::sync (gpu_world.rs:985-1142) on the device: the world's edited cells.  cells[i] is a grid index,
  * values[i] the world's value there (0 / 1-255 / 256 + i = brick i of `bricks`); a cell listed twice takes
  * its last value.  In grid order, once per cell: a brick that became uniform turns back into a uniform cell
@@ -201,9 +208,9 @@ int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t n_chunks, 
  * cell is encoded as vf_world_sync_chunks encodes.  The touched coarse cells are rebuilt from the world's
  * values; the dirty bricks are clustered (boxes within 16 bricks grow, boxes within 32 merge) and each
  * cluster gets its sky-top columns and its distance field over the box +-16 bricks.  gpu_slot: [n_bricks],
- * in and out. */
+ * in and out; the strides as for vf_world_sync_chunks. */
 int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t *values, uint32_t n, const uint8_t *bricks,
-                        uint32_t n_bricks, int32_t *gpu_slot);
+                        uint32_t n_bricks, uint32_t brick_stride, int32_t *gpu_slot, uint32_t slot_stride);
 /* The atlas bookkeeping: slots handed out, the free list's length, capacity. */
 int vf_world_atlas_state(vf_ctx *ctx, uint32_t *used, uint32_t *n_free, uint32_t *capacity);
 /* Copies the device world's encoded grid ([wbx*wby*wbz]) and the first n_slots atlas slots back. */

@@ -210,3 +210,45 @@ def test_sync_chunks_argument_errors(ctx):
     ctx.upload_arrays(flat)                                               # not a ring world
     with pytest.raises(vf.VfError):
         ctx.sync_chunks([(0, 0, None)], np.zeros((0, 512), np.uint8), np.zeros(0, np.int32))
+
+
+BRICK_RECORD = np.dtype([("vox", np.uint8, 512), ("gpu_slot", np.int32), ("mask_ok", np.uint8), ("mask", np.uint32, 16)],
+                        align=True)   # a brick record as the caller's world holds it (voxels, slot, cached masks)
+
+
+def test_brick_records_read_and_written_in_place(ctx):
+    """The pack, the chunk sync and the edit sync given brick records (voxels and slot fields at the record
+    stride) leave the same device world and slots as given packed arrays."""
+    a, grid, bricks0, cap = _endless(2000)
+    wb, win = a["world_bricks"], a["world_win"]
+    rng = np.random.default_rng(7)
+    n0 = bricks0.shape[0]
+    bricks = np.concatenate([bricks0, rng.integers(0, 30, (1500, 512)).astype(np.uint8)])
+    rec = np.zeros(bricks.shape[0], BRICK_RECORD)
+    rec["vox"] = bricks
+    rec["gpu_slot"] = -7
+    cx0, cz0 = win[0] // 16, win[2] // 16
+    chunks = [(cx0 + 1, cz0 + 2, _cells(rng, wb[1], np.arange(n0, n0 + 800))), (cx0 + 3, cz0, None)]
+    cells = rng.integers(0, int(np.prod(wb)), 200)
+    vals = np.where(rng.random(200) < 0.5, 256 + rng.integers(n0 + 800, n0 + 1500, 200), rng.integers(0, 30, 200)).astype(np.uint32)
+    results = []
+    for mode in ("packed", "records"):
+        if mode == "packed":
+            slot = ctx.pack_world(wb, win, grid, bricks, a["palette"], atlas_slots=cap)
+            slot = ctx.sync_chunks(chunks, bricks, slot, released=[5, 9])
+            v, slot = ctx.sync_edits(cells, vals, bricks, slot)
+        else:
+            r = rec.copy()
+            slot = ctx.pack_world(wb, win, grid, r, a["palette"], atlas_slots=cap)
+            assert np.array_equal(slot, r["gpu_slot"])
+            ctx.sync_chunks(chunks, r, None, released=[5, 9])
+            v, _ = ctx.sync_edits(cells, vals, r, None)
+            slot = r["gpu_slot"].copy()
+            assert np.array_equal(r["vox"], bricks) and not r["mask_ok"].any()   # only gpu_slot written
+        g, at, m = ctx.world_read(cap)
+        c1, c2 = ctx.read_coarse()
+        d, dry = ctx.read_brickdist()
+        results.append((slot, v, g, at, m, c1, c2, d, dry, ctx.atlas_state()))
+    for x, y in zip(*results):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    assert (results[0][0][n0:n0 + 800] >= 0).any()

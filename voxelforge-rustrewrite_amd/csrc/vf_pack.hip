@@ -143,6 +143,45 @@ struct DevTemp {
     }
 };
 
+// The caller's bricks: voxels and slots at byte strides (packed arrays, or the fields of brick records).
+struct BrickView {
+    const uint8_t *vox;
+    size_t vstride;
+    int32_t *slot;
+    size_t sstride;
+    const uint8_t *at(size_t b) const { return vox + b * vstride; }
+    int32_t get_slot(size_t b) const {
+        int32_t v;
+        std::memcpy(&v, reinterpret_cast<const uint8_t *>(slot) + b * sstride, 4);
+        return v;
+    }
+    void set_slot(size_t b, int32_t v) const { std::memcpy(reinterpret_cast<uint8_t *>(slot) + b * sstride, &v, 4); }
+};
+
+// Checks the strides (0 = packed) and fills the view.
+int brick_view(vf_ctx *ctx, const char *what, const uint8_t *vox, uint32_t vstride, int32_t *slot, uint32_t sstride,
+               BrickView &v) {
+    if (vstride && vstride < 512u) return vf::fail(ctx, VF_E_INVALID, "%s: brick stride %u < 512", what, vstride);
+    if (sstride && (sstride % 4u)) return vf::fail(ctx, VF_E_INVALID, "%s: slot stride %u is not a multiple of 4", what, sstride);
+    v.vox = vox;
+    v.vstride = vstride ? vstride : 512u;
+    v.slot = slot;
+    v.sstride = sstride ? sstride : 4u;
+    return VF_OK;
+}
+
+// Copies n bricks' voxels into a packed array (threads for large counts); idx == NULL: bricks 0..n-1.
+void gather_bricks(const BrickView &v, const uint32_t *idx, size_t n, uint8_t *out) {
+    const unsigned nt = n >= 4096u ? std::max(1u, std::min(8u, std::thread::hardware_concurrency())) : 1u;
+    const auto work = [&](unsigned t) {
+        for (size_t l = t; l < n; l += nt) std::memcpy(out + l * 512u, v.at(idx ? idx[l] : l), 512);
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot_of_brick) {
@@ -160,8 +199,10 @@ extern "C" int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot
         if (src->grid[i] >= 256u && src->grid[i] - 256u >= src->n_bricks)
             return vf::fail(ctx, VF_E_INVALID, "vf_world_pack: grid[%lld] = %u names brick %u of %u", (long long)i, src->grid[i],
                             src->grid[i] - 256u, src->n_bricks);
-    int rc = vf::set_device(ctx);
+    BrickView bv;
+    int rc = brick_view(ctx, "vf_world_pack", src->bricks, src->brick_stride, slot_of_brick, src->slot_stride, bv);
     if (rc) return rc;
+    if ((rc = vf::set_device(ctx))) return rc;
     const int64_t nb = std::max<int64_t>(src->n_bricks, 1);
     DevTemp t;
     uint32_t *d_src = nullptr, *first = nullptr, *flag = nullptr, *rank = nullptr, *major = nullptr;
@@ -174,8 +215,14 @@ extern "C" int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot
         return vf::fail(ctx, VF_E_OOM, "vf_world_pack: hipMalloc: %s", hipGetErrorString(e));
     hipStream_t st = ctx->stream;
     VF_HIP(ctx, hipMemcpyAsync(d_src, src->grid, (size_t)n * 4u, hipMemcpyHostToDevice, st));
+    std::vector<uint8_t> packed;   // bricks given as records: their voxels gathered first
+    if (src->n_bricks && bv.vstride != 512u) {
+        packed.resize((size_t)src->n_bricks * 512u);
+        gather_bricks(bv, nullptr, src->n_bricks, packed.data());
+    }
     if (src->n_bricks)
-        VF_HIP(ctx, hipMemcpyAsync(d_bricks, src->bricks, (size_t)src->n_bricks * 512u, hipMemcpyHostToDevice, st));
+        VF_HIP(ctx, hipMemcpyAsync(d_bricks, packed.empty() ? src->bricks : packed.data(), (size_t)src->n_bricks * 512u,
+                                   hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_fill_i32, dim3(vf::blocks_for(nb)), dim3(256), 0, st, (int32_t *)first, -1, nb);
     hipLaunchKernelGGL(k_fill_i32, dim3(vf::blocks_for(nb)), dim3(256), 0, st, slot_of, -1, nb);
     hipLaunchKernelGGL(k_first_cell, dim3(vf::blocks_for(n)), dim3(256), 0, st, (const uint32_t *)d_src, n, first);
@@ -222,12 +269,10 @@ extern "C" int vf_world_pack(vf_ctx *ctx, const vf_world_src *src, int32_t *slot
     ctx->host_occ.resize((size_t)cap * 8u);
     VF_HIP(ctx, hipMemcpyAsync(ctx->host_grid.data(), ctx->grid, (size_t)n * 4u, hipMemcpyDeviceToHost, st));
     if (cap) VF_HIP(ctx, hipMemcpyAsync(ctx->host_occ.data(), ctx->occ, (size_t)cap * 64u, hipMemcpyDeviceToHost, st));
-    if (slot_of_brick && src->n_bricks)
-        VF_HIP(ctx, hipMemcpyAsync(slot_of_brick, slot_of, (size_t)src->n_bricks * 4u, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> slots(slot_of_brick ? (size_t)src->n_bricks : 0u);
+    if (!slots.empty()) VF_HIP(ctx, hipMemcpyAsync(slots.data(), slot_of, slots.size() * 4u, hipMemcpyDeviceToHost, st));
     VF_HIP(ctx, hipStreamSynchronize(st));
-    if (slot_of_brick)
-        for (uint32_t b = 0; b < src->n_bricks; ++b)
-            if (slot_of_brick[b] < 0) slot_of_brick[b] = -1;
+    for (size_t b = 0; b < slots.size(); ++b) bv.set_slot(b, slots[b] < 0 ? -1 : slots[b]);
     return VF_OK;
 }
 
@@ -386,7 +431,8 @@ extern "C" int vf_world_atlas_state(vf_ctx *ctx, uint32_t *used, uint32_t *n_fre
 }
 
 extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t n_chunks, const uint8_t *bricks, uint32_t n_bricks,
-                                    int32_t *gpu_slot, const int32_t *released, int32_t n_released) {
+                                    uint32_t brick_stride, int32_t *gpu_slot, uint32_t slot_stride, const int32_t *released,
+                                    int32_t n_released) {
     int rc = vf::check_ready(ctx, "vf_world_sync_chunks");
     if (rc) return rc;
     if ((rc = vf::no_capture(ctx, "vf_world_sync_chunks"))) return rc;
@@ -397,6 +443,8 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
                         W.wb[0], W.wb[2]);
     if (n_chunks < 0 || n_released < 0 || (n_chunks && !chunks) || (n_released && !released) || (n_bricks && (!bricks || !gpu_slot)))
         return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_chunks: null or negative argument");
+    BrickView bv;
+    if ((rc = brick_view(ctx, "vf_world_sync_chunks", bricks, brick_stride, gpu_slot, slot_stride, bv))) return rc;
     const int32_t ring_w = W.wb[0] / kChunkB, ring_d = W.wb[2] / kChunkB;
     const int64_t per = (int64_t)kChunkB * W.wb[1] * kChunkB;
     // chunks once per ring slot, in order; loaded ones concatenated with their bricks renumbered by first
@@ -453,17 +501,8 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
     uint8_t *pb = ctx->pin;
     int32_t *pslot = reinterpret_cast<int32_t *>(pb + off_slot);
     uint32_t *pcells = reinterpret_cast<uint32_t *>(pb + off_cells);
-    {
-        const unsigned nt = nl >= 4096u ? std::max(1u, std::min(8u, std::thread::hardware_concurrency())) : 1u;
-        std::vector<std::thread> th;
-        for (unsigned t = 1; t < nt; ++t)
-            th.emplace_back([&, t] {
-                for (size_t l = t; l < nl; l += nt) std::memcpy(pb + l * 512u, bricks + (size_t)named[l] * 512u, 512);
-            });
-        for (size_t l = 0; l < nl; l += nt) std::memcpy(pb + l * 512u, bricks + (size_t)named[l] * 512u, 512);
-        for (auto &x : th) x.join();
-    }
-    for (uint32_t l = 0; l < nl; ++l) pslot[l] = gpu_slot[named[l]];
+    gather_bricks(bv, named.data(), nl, pb);
+    for (uint32_t l = 0; l < nl; ++l) pslot[l] = bv.get_slot(named[l]);
     if (n) std::memcpy(pcells, cat.data(), (size_t)n * 4u);
     // released slots that hold world bricks, highest first (gpu_world.rs:1199-1207); taken below, once
     // nothing can fail before the world changes
@@ -603,7 +642,7 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
     ctx->free_slots.resize(nf - pops);
     ctx->atlas_used += total_new > nf ? std::min<uint32_t>(total_new - (uint32_t)nf, A.cap - A.used) : 0u;
     for (uint32_t l = 0; l < nl; ++l) {
-        gpu_slot[named[l]] = out[l] >= 0 ? out[l] : -1;
+        bv.set_slot(named[l], out[l] >= 0 ? out[l] : -1);
         if (out[l] < 0) continue;
         ctx->world_slot[(size_t)out[l]] = 1;
         std::memcpy(&ctx->host_occ[(size_t)out[l] * 8u], hocc + (size_t)l * 16u, 64);
@@ -676,13 +715,15 @@ struct BrickBox {
 }  // namespace
 
 extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t *values, uint32_t n, const uint8_t *bricks,
-                                   uint32_t n_bricks, int32_t *gpu_slot) {
+                                   uint32_t n_bricks, uint32_t brick_stride, int32_t *gpu_slot, uint32_t slot_stride) {
     int rc = vf::check_ready(ctx, "vf_world_sync_edits");
     if (rc) return rc;
     if ((rc = vf::no_capture(ctx, "vf_world_sync_edits"))) return rc;
     if (!n) return VF_OK;
     if (!cells || !values || (n_bricks && (!bricks || !gpu_slot)))
         return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_edits: null argument");
+    BrickView bv;
+    if ((rc = brick_view(ctx, "vf_world_sync_edits", bricks, brick_stride, gpu_slot, slot_stride, bv))) return rc;
     const DevWorld &W = ctx->W;
     for (uint32_t i = 0; i < n; ++i) {
         if (cells[i] >= ctx->n_cells) return vf::fail(ctx, VF_E_INVALID, "vf_world_sync_edits: cell %u of %zu", cells[i], ctx->n_cells);
@@ -713,14 +754,14 @@ extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t 
     std::unordered_map<uint32_t, int32_t> slot_of;   // bricks whose slot changes
     const auto cur_slot = [&](uint32_t b) {
         auto it = slot_of.find(b);
-        return it != slot_of.end() ? it->second : gpu_slot[b];
+        return it != slot_of.end() ? it->second : bv.get_slot(b);
     };
     std::vector<std::pair<uint32_t, uint8_t>> ws_ops;   // world_slot writes in order
     std::vector<uint32_t> enc(nd), up;
     for (size_t j = 0; j < nd; ++j) {
         if (val[j] >= 256u) {
             const uint32_t b = val[j] - 256u;
-            const uint8_t *v = bricks + (size_t)b * 512u;
+            const uint8_t *v = bv.at(b);
             bool uni = true;
             for (int k = 1; k < 512 && uni; ++k) uni = v[k] == v[0];
             if (uni) {
@@ -755,7 +796,7 @@ extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t 
             enc[j] = 256u + (uint32_t)s;
             up.push_back(b);
         } else {
-            enc[j] = majority_host(bricks + (size_t)b * 512u);
+            enc[j] = majority_host(bv.at(b));
         }
     }
     std::sort(up.begin(), up.end());
@@ -837,10 +878,8 @@ extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t 
     }
     uint8_t *pb = ctx->pin, *db = ctx->sync_ws;
     int32_t *pslot = (int32_t *)(pb + h_sl);
-    for (size_t u = 0; u < nu; ++u) {
-        std::memcpy(pb + h_br + u * 512u, bricks + (size_t)up[u] * 512u, 512);
-        pslot[u] = cur_slot(up[u]);
-    }
+    gather_bricks(bv, up.data(), nu, pb + h_br);
+    for (size_t u = 0; u < nu; ++u) pslot[u] = cur_slot(up[u]);
     std::memcpy(pb + h_idx, dirty.data(), nd * 4u);
     std::memcpy(pb + h_enc, enc.data(), nd * 4u);
     std::memcpy(pb + h_src, val.data(), nd * 4u);
@@ -881,7 +920,7 @@ extern "C" int vf_world_sync_edits(vf_ctx *ctx, const uint32_t *cells, uint32_t 
     for (const auto &op : ws_ops) ctx->world_slot[op.first] = op.second;
     for (size_t j = 0; j < nd; ++j) ctx->host_grid[dirty[j]] = enc[j];
     for (size_t u = 0; u < nu; ++u) std::memcpy(&ctx->host_occ[(size_t)pslot[u] * 8u], pb + h_occ + u * 64u, 64);
-    for (const auto &kv : slot_of) gpu_slot[kv.first] = kv.second;
+    for (const auto &kv : slot_of) bv.set_slot(kv.first, kv.second);
     for (uint32_t i = 0; i < n; ++i) values[i] = val[pos_of[i]];
     return VF_OK;
 }

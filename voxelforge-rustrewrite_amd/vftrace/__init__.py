@@ -54,9 +54,10 @@ _SIGS = {
     "vf_world_upload": (C.c_int, [_P, C.POINTER(WorldDesc)]),
     "vf_world_pack": (C.c_int, [_P, C.POINTER(WorldSrc), _P]),
     "vf_world_read": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
-    "vf_world_sync_chunks": (C.c_int, [_P, C.POINTER(Chunk), C.c_int32, _P, C.c_uint32, _P, _P, C.c_int32]),
+    "vf_world_sync_chunks": (C.c_int, [_P, C.POINTER(Chunk), C.c_int32, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P,
+                                       C.c_int32]),
     "vf_world_atlas_state": (C.c_int, [_P, _P, _P, _P]),
-    "vf_world_sync_edits": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P]),
+    "vf_world_sync_edits": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
     "vf_read_coarse": (C.c_int, [_P, _P, _P]),
     "vf_world_update": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.c_uint32]),
     "vf_build_brickdist": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -429,23 +430,27 @@ class Context:
 
     def pack_world(self, world_bricks, world_win, grid, bricks, palette, atlas_slots=0):
         """vf_world_pack: the world from the CPU side's grid (0 / 1-255 / 256 + brick index) and bricks
-        ([n, 512] u8), packed on the device.  Returns each brick's slot (-1: unnamed or did not fit)."""
+        ([n, 512] u8, or brick records as `_brick_args` takes them, whose gpu_slot fields are written),
+        packed on the device.  Returns each brick's slot (-1: unnamed or did not fit)."""
         wb = tuple(int(x) for x in world_bricks)
         g = np.ascontiguousarray(grid, dtype=np.uint32)
-        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1)
+        rec = getattr(bricks, "dtype", None) is not None and bricks.dtype.names is not None
+        bp, nb, bs, sp, ss, hold, slot = self._brick_args(bricks, None if rec else
+                                                          np.zeros(np.asarray(bricks).size // 512, np.int32))
         pal = np.ascontiguousarray(palette, dtype=np.float32).reshape(256, 12)
         d = WorldSrc()
         d.world_bricks[:] = wb
         d.world_win[:] = tuple(int(x) for x in world_win)
         d.grid = g.ctypes.data_as(C.POINTER(C.c_uint32))
-        d.bricks = b.ctypes.data_as(C.POINTER(C.c_uint8)) if b.size else None
-        d.n_bricks = b.size // 512
+        d.bricks = C.cast(bp, C.POINTER(C.c_uint8)) if bp else None
+        d.n_bricks = nb
         d.atlas_slots = int(atlas_slots)
         d.palette = pal.ctypes.data_as(C.POINTER(Material))
-        slots = np.zeros(max(d.n_bricks, 1), np.int32)
-        self._ck(lib().vf_world_pack(self._h, C.byref(d), slots.ctypes.data))
+        d.brick_stride, d.slot_stride = bs, ss
+        self._ck(lib().vf_world_pack(self._h, C.byref(d), sp))
         self.world_bricks = wb
-        return slots[:d.n_bricks]
+        del hold
+        return np.array(slot, np.int32)
 
     def world_read(self, n_slots):
         """The device world's encoded grid, its first n_slots atlas slots and their brick masks."""
@@ -455,6 +460,27 @@ class Context:
         masks = np.zeros((max(n_slots, 1), 16), np.uint32)
         self._ck(lib().vf_world_read(self._h, grid.ctypes.data, atlas.ctypes.data, int(n_slots), masks.ctypes.data))
         return grid, atlas[:n_slots], masks[:n_slots]
+
+    @staticmethod
+    def _brick_args(bricks, gpu_slot):
+        """(voxel pointer, count, stride, slot pointer, slot stride, keep-alive, slots) for the world calls:
+        packed bricks [n, 512] u8 with gpu_slot [n] i32 (copied), or an array of brick records (a structured
+        array with fields `vox` (512 u8) and `gpu_slot` (i32), like World::bricks), read and written in place."""
+        if getattr(bricks, "dtype", None) is not None and bricks.dtype.names and "vox" in bricks.dtype.names:
+            if gpu_slot is not None:
+                raise ValueError("brick records carry their own gpu_slot")
+            if not bricks.flags.c_contiguous:
+                raise ValueError("brick records must be contiguous")
+            base = bricks.ctypes.data
+            vo, so = bricks.dtype.fields["vox"][1], bricks.dtype.fields["gpu_slot"][1]
+            n = bricks.shape[0]
+            return (base + vo if n else None, n, bricks.dtype.itemsize, base + so if n else None, bricks.dtype.itemsize,
+                    bricks, bricks["gpu_slot"])
+        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1, 512)
+        slot = np.array(gpu_slot, dtype=np.int32).reshape(-1)
+        if slot.size != b.shape[0]:
+            raise ValueError(f"gpu_slot has {slot.size} entries for {b.shape[0]} bricks")
+        return (b.ctypes.data if b.size else None, b.shape[0], 512, slot.ctypes.data if slot.size else None, 4, b, slot)
 
     def sync_chunks(self, chunks, bricks, gpu_slot, released=()):
         """vf_world_sync_chunks (GpuWorld::sync_chunks): chunks = [(cx, cz, cells or None)], cells [16*wby*16] u32
@@ -470,15 +496,11 @@ class Context:
                 c = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
                 keep.append(c)
                 arr[i].loaded, arr[i].cells = 1, c.ctypes.data_as(C.POINTER(C.c_uint32))
-        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1, 512)
-        slot = np.array(gpu_slot, dtype=np.int32).reshape(-1)
-        if slot.size != b.shape[0]:
-            raise ValueError(f"gpu_slot has {slot.size} entries for {b.shape[0]} bricks")
+        bp, nb, bs, sp, ss, hold, slot = self._brick_args(bricks, gpu_slot)
         rel = np.ascontiguousarray(released, dtype=np.int32).reshape(-1)
-        self._ck(lib().vf_world_sync_chunks(self._h, arr, len(chunks), b.ctypes.data if b.size else None, b.shape[0],
-                                            slot.ctypes.data if slot.size else None, rel.ctypes.data if rel.size else None,
-                                            rel.size))
-        del keep
+        self._ck(lib().vf_world_sync_chunks(self._h, arr, len(chunks), bp, nb, bs, sp, ss,
+                                            rel.ctypes.data if rel.size else None, rel.size))
+        del keep, hold
         return slot
 
     def sync_edits(self, cells, values, bricks, gpu_slot):
@@ -489,13 +511,10 @@ class Context:
         v = np.array(values, dtype=np.uint32).reshape(-1)
         if v.size != c.size:
             raise ValueError(f"{v.size} values for {c.size} cells")
-        b = np.ascontiguousarray(bricks, dtype=np.uint8).reshape(-1, 512)
-        slot = np.array(gpu_slot, dtype=np.int32).reshape(-1)
-        if slot.size != b.shape[0]:
-            raise ValueError(f"gpu_slot has {slot.size} entries for {b.shape[0]} bricks")
+        bp, nb, bs, sp, ss, hold, slot = self._brick_args(bricks, gpu_slot)
         self._ck(lib().vf_world_sync_edits(self._h, c.ctypes.data if c.size else None, v.ctypes.data if v.size else None,
-                                           c.size, b.ctypes.data if b.size else None, b.shape[0],
-                                           slot.ctypes.data if slot.size else None))
+                                           c.size, bp, nb, bs, sp, ss))
+        del hold
         return v, slot
 
     def atlas_state(self):

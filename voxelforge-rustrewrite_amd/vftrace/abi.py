@@ -104,6 +104,7 @@ class WorldSrc(C.Structure):
         ("world_bricks", i3), ("world_win", i4),
         ("grid", C.POINTER(C.c_uint32)), ("bricks", C.POINTER(C.c_uint8)), ("n_bricks", C.c_uint32),
         ("atlas_slots", C.c_uint32), ("palette", C.POINTER(Material)),
+        ("brick_stride", C.c_uint32), ("slot_stride", C.c_uint32),
     ]
 
 
