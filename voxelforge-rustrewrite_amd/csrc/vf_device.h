@@ -95,13 +95,15 @@ __device__ __forceinline__ void mask_layer(const uint4 &m0, const uint4 &m1, con
 // lane with thread id t at vf_vmask[k * 256 + t] (consecutive lanes in consecutive banks).  Only
 // kernels of 256 threads that instantiate brick_voxels<., true> reference it (16 KB per workgroup).
 static __shared__ uint32_t vf_vmask[16 * 256];
+// The same for kernels of 64 threads (LS = 64: 4 KB per workgroup).
+static __shared__ uint32_t vf_vmask64[16 * 64];
 
 // brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
 // arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
 // LDSM: the mask goes to the lane's LDS slots and a step along z reads the new 64-bit layer from
 // there (one LDS read instead of a 16-to-2 word select in registers, which a wave executes on almost
 // every step since some lane steps along z).
-template <bool COUNT, bool LDSM = false>
+template <bool COUNT, bool LDSM = false, int LS = 256>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
                                              const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c,
@@ -109,12 +111,12 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     const uint32_t slot = g - 256u;
     const uint4 *mp = (skip == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
     const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
-    uint32_t *sm = vf_vmask + threadIdx.x;
+    uint32_t *sm = (LS == 64 ? vf_vmask64 : vf_vmask) + threadIdx.x;
     if (LDSM) {
-        sm[0 * 256] = m0.x; sm[1 * 256] = m0.y; sm[2 * 256] = m0.z; sm[3 * 256] = m0.w;
-        sm[4 * 256] = m1.x; sm[5 * 256] = m1.y; sm[6 * 256] = m1.z; sm[7 * 256] = m1.w;
-        sm[8 * 256] = m2.x; sm[9 * 256] = m2.y; sm[10 * 256] = m2.z; sm[11 * 256] = m2.w;
-        sm[12 * 256] = m3.x; sm[13 * 256] = m3.y; sm[14 * 256] = m3.z; sm[15 * 256] = m3.w;
+        sm[0 * LS] = m0.x; sm[1 * LS] = m0.y; sm[2 * LS] = m0.z; sm[3 * LS] = m0.w;
+        sm[4 * LS] = m1.x; sm[5 * LS] = m1.y; sm[6 * LS] = m1.z; sm[7 * LS] = m1.w;
+        sm[8 * LS] = m2.x; sm[9 * LS] = m2.y; sm[10 * LS] = m2.z; sm[11 * LS] = m2.w;
+        sm[12 * LS] = m3.x; sm[13 * LS] = m3.y; sm[14 * LS] = m3.z; sm[15 * LS] = m3.w;
     }
     int32_t bmin[3], vc[3];
     float tM[3], tD[3];
@@ -130,8 +132,8 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
     uint32_t wlo, whi;
     if (LDSM) {
-        wlo = sm[(lz * 2) * 256];
-        whi = sm[(lz * 2 + 1) * 256];
+        wlo = sm[(lz * 2) * LS];
+        whi = sm[(lz * 2 + 1) * LS];
     } else {
         mask_layer(m0, m1, m2, m3, lz, wlo, whi);
     }
@@ -169,8 +171,8 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         tMz = a2 ? nz : tMz;
         lx = nlx; ly = nly; lz = nlz;
         if (LDSM) {   // the layer of the new voxel, read every trip (no branch; unchanged unless a z step)
-            wlo = sm[(lz * 2) * 256];
-            whi = sm[(lz * 2 + 1) * 256];
+            wlo = sm[(lz * 2) * LS];
+            whi = sm[(lz * 2 + 1) * LS];
         } else if (a2) {
             mask_layer(m0, m1, m2, m3, lz, wlo, whi);
         }
@@ -188,7 +190,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
 
 // traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5), or -1 to
 // take it from skip_rt (one inlined copy serving both).
-template <int SKIP, bool COUNT, bool LDSM = false>
+template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
                                             Cnt &c, const int skip_rt = 0) {
     const int sk = SKIP >= 0 ? SKIP : skip_rt;
@@ -247,7 +249,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             }
         }
         if (g >= 256u) {
-            if (brick_voxels<COUNT, LDSM>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
+            if (brick_voxels<COUNT, LDSM, LS>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
         } else if (g != 0u && g != (uint32_t)sk) {
             const float e = t + 1e-3f;
             h.hit = 1; h.t = t; h.face = face; h.mat = g;
@@ -335,6 +337,16 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
 // themselves.
 __device__ __forceinline__ int32_t xcd_remap(int32_t i, int32_t n) {
     constexpr int32_t C = 16;
+    const int32_t full = (n / (8 * C)) * (8 * C);
+    if (i >= full) return i;
+    const int32_t x = i & 7, k = i >> 3;
+    return ((k / C) * 8 + x) * C + (k % C);
+}
+
+// xcd_remap for 64-thread blocks that each take a quarter of a 16x16 tile: runs of 64 blocks (16 tiles)
+// per XCD, as xcd_remap deals 16 whole tiles.
+__device__ __forceinline__ int32_t xcd_remap64(int32_t i, int32_t n) {
+    constexpr int32_t C = 64;
     const int32_t full = (n / (8 * C)) * (8 * C);
     if (i >= full) return i;
     const int32_t x = i & 7, k = i >> 3;

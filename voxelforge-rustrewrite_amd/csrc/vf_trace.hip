@@ -67,11 +67,11 @@ __device__ __forceinline__ void primary_dir(const PrimArgs &A, int32_t px, int32
     normalize3(rd);
 }
 
-template <bool COUNT, bool SPECIAL>
+template <bool COUNT, bool SPECIAL, int LS = 256>
 __device__ __forceinline__ void primary_hit(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, const float rd[3],
                                             Hit &h, Cnt &c) {
     if (SPECIAL) primary_trace<COUNT, true>(W, A.pos, cam_mat, rd, h, c);
-    else trace_world<0, COUNT, true>(W, A.pos, rd, 1e30f, h, c);
+    else trace_world<0, COUNT, true, LS>(W, A.pos, rd, 1e30f, h, c);
 }
 
 // Tile L's pixel for this thread (8x8 per wave, 16x16 per workgroup).
@@ -148,7 +148,7 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
 // deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
 // to the first blocker.
-template <bool COUNT, bool BODIES = false, bool LDSM = false>
+template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr) {
     float vis = 0.0f, tb = 0.0f;
@@ -162,7 +162,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         } else {
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT, BODIES, LDSM>(W, so, sun, leaf, seed, tb, hits, i, c, B);
+            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS>(W, so, sun, leaf, seed, tb, hits, i, c, B);
         }
     }
     const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
@@ -191,14 +191,14 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
 // gbuffer.comp then shadow_trace.comp for the same pixel in one kernel: the sun ray starts from the
 // G-buffer values this lane has just written (position, RGBA8-quantised normal), exactly what the
 // separate shadow pass would read back, so the results are those of the two passes.
-template <bool COUNT, bool SPECIAL>
+template <bool COUNT, bool SPECIAL, int LS = 256>
 __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px,
                                                      int32_t py, const float sun[3], vf_hit *hits, float4 *gbpos,
                                                      uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c) {
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
-    primary_hit<COUNT, SPECIAL>(W, A, cam_mat, rd, h, c);
+    primary_hit<COUNT, SPECIAL, LS>(W, A, cam_mat, rd, h, c);
     if (camera_special(cam_mat) != SPECIAL) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     store_hit(hits + i, h);
@@ -216,7 +216,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT, false, true>(W, p, nq, sun, A.leaf, shits, i, c);
+    shadow[i] = shadow_texel<COUNT, false, true, LS>(W, p, nq, sun, A.leaf, shits, i, c);
 }
 
 template <bool COUNT, bool SPECIAL>
@@ -241,6 +241,24 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
         }
     }
     if (camera_special(cam_mat) == SPECIAL) flush_count<COUNT>(c, cnt);   // only the variant whose results stand
+}
+
+// The fused kernel for a camera in air with one 8x8-pixel wave per workgroup: a wave that finishes
+// frees its slot at once instead of waiting for the other three of a 16x16 tile.  Block L is quadrant
+// L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.
+template <bool COUNT>
+__global__ void __launch_bounds__(64) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
+                                                         float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
+                                                         unsigned long long *cnt) {
+    const float sun[3] = {sx, sy, sz};
+    Cnt c = {0, 0, 0, 0};
+    const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
+    const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
+    const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
+    if (px < A.w && py < A.h)
+        primary_shadow_pixel<COUNT, false, 64>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c);
+    flush_count<COUNT>(c, cnt);
 }
 
 // gbuffer.comp over the scene (traceScenePrimary = traceSceneS, common.glsl:787-793: the world, then the
@@ -846,7 +864,10 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
                            uint32_t *shadow, vf_hit *shits) {
     const int cls = camera_class(ctx, A.pos);
     A.cam_air = cls == 0 ? 1 : 0;
-    if (cls != 1)
+    if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
+        hipLaunchKernelGGL((k_primary_shadow64<COUNT>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1],
+                           s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
+    else if (cls != 1)
         hipLaunchKernelGGL((k_primary_shadow<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0],
                            s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
     const int32_t g = A.ntiles < ctx->n_cu ? A.ntiles : ctx->n_cu;
