@@ -53,11 +53,13 @@ __device__ __forceinline__ int32_t imin(int32_t x, int32_t y) { return y < x ? y
 __device__ __forceinline__ int32_t imax(int32_t x, int32_t y) { return x < y ? y : x; }
 __device__ __forceinline__ int32_t iclamp(int32_t x, int32_t lo, int32_t hi) { return imin(imax(x, lo), hi); }
 __device__ __forceinline__ float fclampg(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+// float -> int32 truncating, NaN -> 0, saturating at both ends: exactly what v_cvt_i32_f32 does, as one
+// instruction (a C++ cast is undefined out of range, and the checks written out compiled to three nested
+// branches per conversion).
 __device__ __forceinline__ int32_t f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT32_MAX;
-    if (x <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 __device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 __device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {

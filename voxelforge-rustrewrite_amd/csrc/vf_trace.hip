@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
 // frees its slot at once instead of waiting for the other three of a 16x16 tile.  Block L is quadrant
 // L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.
 template <bool COUNT>
-__global__ void __launch_bounds__(64) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
+__global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
                                                          float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
                                                          unsigned long long *cnt) {
     const float sun[3] = {sx, sy, sz};
