@@ -338,6 +338,8 @@ int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int
  * fit) and how many lanes must be idle before a wavefront fetches new rays (1..64; 0 = chunk mode: a
  * wavefront traces its claimed chunk 64 rays at a time, each to completion).  Results never depend
  * on these. */
+/* refill -1 (vf_trace_gi_batch only): no queue, one lane per pixel traces its GI ray and then its sun ray
+ * (the autotuner measures it beside the queue configurations). */
 typedef struct vf_queue_cfg {
     int32_t tile_w, tile_h, chunk, sort, groups_per_cu, refill;
 } vf_queue_cfg;

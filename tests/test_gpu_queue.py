@@ -38,7 +38,7 @@ def gbuffer(ctx, pu, W, H):
 
 @pytest.mark.parametrize("tile,chunk,sort,refill", [((8, 8), 64, False, 32), ((16, 8), 256, True, 32),
                                                     ((16, 16), 1024, True, 48), ((8, 8), 512, True, 0),
-                                                    ((8, 8), 64, False, 0)])
+                                                    ((8, 8), 64, False, 0), ((16, 16), 64, False, -1)])
 def test_gi_batch_matches_oracle(ctx, tile, chunk, sort, refill):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=6)
@@ -58,7 +58,7 @@ def test_gi_batch_matches_oracle(ctx, tile, chunk, sort, refill):
     assert 0.05 < g["hit"].mean() < 0.95 and s["hit"].sum() > 0
 
 
-@pytest.mark.parametrize("refill", [0, 32])
+@pytest.mark.parametrize("refill", [0, 32, -1])
 def test_gi_batch_counters_match_oracle(ctx, refill):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=2)
@@ -162,7 +162,7 @@ def test_autotuner_picks_a_measured_config(ctx):
     gbp, gbn = gbuffer(ctx, pu, W, H)
     gi, sun = dev_hits(W * H), dev_hits(W * H)
     best, table = ctx.autotune_gi_batch(pu, W, H, 640.0, gbp, gbn, gi, sun)
-    assert len(table) == 2 * 3 * 5 * 2
+    assert len(table) == 2 * 3 * 5 * 2 + 1   # + the queue-free configuration
     fastest = min(table, key=lambda r: r[3])
     assert (best.tile_w, best.tile_h) == fastest[0] and best.chunk == fastest[1] and bool(best.sort) == fastest[2]
     assert all(r[3] > 0 for r in table)

@@ -66,6 +66,60 @@ struct GenArgs {
 // `sort` it writes them grouped by direction octant (a counting sort in LDS), so the rays of a
 // wavefront start close together AND share a direction octant.
 constexpr int kGenPer = 4;
+__device__ __forceinline__ void gi_ray(const GenArgs &A, const float4 *gbpos, const uint32_t *gbn, int32_t px, int32_t py,
+                                       size_t fi, float o[3], float d[3]) {
+    const float4 P4 = gbpos[fi];
+    const uint32_t nb = gbn[fi];
+    const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
+    const uint32_t salt = 0x68e31da4u;
+    const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
+    const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
+    const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
+    const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
+    const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
+    cosine_hemisphere_u(N, u0, u1, d);
+    o[0] = P4.x + N[0] * 3e-2f; o[1] = P4.y + N[1] * 3e-2f; o[2] = P4.z + N[2] * 3e-2f;
+}
+
+// The batch without a queue (refill -1): one lane per pixel of a 16x16 tile (8x8 per wave) traces its GI
+// ray and then its sun ray, with the same rays and rules as the list kernels; for even work such as one
+// ray per pixel the launch's own scheduling is enough.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_gi_direct(DevWorld W, GenArgs A, const float4 *gbpos, const uint32_t *gbn,
+                                                   vf_hit *gi_hits, vf_hit *sun_hits, float sx, float sy, float sz,
+                                                   unsigned long long *cnt) {
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t tiles_x = (A.w + 15) / 16;
+    const int32_t px = (L % tiles_x) * 16 + (wv & 1) * 8 + (lane & 7);
+    const int32_t py = (L / tiles_x) * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (px >= A.w || py >= A.h) return;
+    const size_t fi = (size_t)py * A.w + px;
+    Cnt c = {0, 0, 0, 0};
+    Hit h, sh;
+    miss(sh);
+    if (gbpos[fi].w < 0.0f) {
+        miss(h);
+    } else {
+        float o[3], d[3];
+        gi_ray(A, gbpos, gbn, px, py, fi, o, d);
+        trace_world<0, COUNT, true>(W, o, d, A.reach, h, c);
+        const float sun_dir[3] = {sx, sy, sz};
+        if (h.hit && h.mat != 5u) {
+            float hn[3];
+            face_normal(h.face, hn);
+            if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
+                float so[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) so[a] = (o[a] + d[a] * h.t) + hn[a] * 3e-2f;
+                trace_world<0, COUNT, true>(W, so, sun_dir, 1e30f, sh, c);
+            }
+        }
+    }
+    store_hit(gi_hits + fi, h);
+    store_hit(sun_hits + fi, sh);
+    flush_count<COUNT>(c, cnt);
+}
 __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, const uint32_t *gbn, float4 *rays,
                                                 unsigned long long *ctr, vf_hit *gi_hits, vf_hit *sun_hits) {
     __shared__ uint32_t oct_count[8], oct_base[8];
@@ -93,17 +147,7 @@ __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, 
         o[k][0] = o[k][1] = o[k][2] = 0.0f;
         d[k][0] = d[k][1] = d[k][2] = 0.0f;
         if (valid[k]) {
-            const float4 P4 = gbpos[fi];
-            const uint32_t nb = gbn[fi];
-            const float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
-            const uint32_t salt = 0x68e31da4u;
-            const uint32_t hh = pcg((uint32_t)px + pcg((uint32_t)py + salt));
-            const float rx = (float)(hh >> 8u) * (1.0f / 16777216.0f);
-            const float ry = (float)(pcg(hh) >> 8u) * (1.0f / 16777216.0f);
-            const float kf = (float)((A.frame >> 2u) & 0xFFFFu);
-            const float u0 = fractg(rx + kf * 0.7548776662f), u1 = fractg(ry + kf * 0.5698402910f);
-            cosine_hemisphere_u(N, u0, u1, d[k]);
-            o[k][0] = P4.x + N[0] * 3e-2f; o[k][1] = P4.y + N[1] * 3e-2f; o[k][2] = P4.z + N[2] * 3e-2f;
+            gi_ray(A, gbpos, gbn, px, py, fi, o[k], d[k]);
             oct[k] = A.sort ? ((d[k][0] < 0.0f ? 1u : 0u) | (d[k][1] < 0.0f ? 2u : 0u) | (d[k][2] < 0.0f ? 4u : 0u)) : 0u;
             local[k] = atomicAdd(&oct_count[oct[k]], 1u);
         } else if (inside) {
@@ -640,8 +684,8 @@ int check_cfg(vf_ctx *ctx, const vf_queue_cfg *cfg, const char *what) {
     if (!tiles) return vf::fail(ctx, VF_E_INVALID, "%s: tile %dx%d not supported", what, cfg->tile_w, cfg->tile_h);
     if (cfg->chunk < 64 || cfg->chunk > 4096 || (cfg->chunk & 63))
         return vf::fail(ctx, VF_E_INVALID, "%s: chunk %d must be a multiple of 64 in [64, 4096]", what, cfg->chunk);
-    if (cfg->refill < 0 || cfg->refill > 64)
-        return vf::fail(ctx, VF_E_INVALID, "%s: refill %d must be in [0, 64]", what, cfg->refill);
+    if (cfg->refill < -1 || cfg->refill > 64)
+        return vf::fail(ctx, VF_E_INVALID, "%s: refill %d must be in [0, 64], or -1 (no queue)", what, cfg->refill);
     return VF_OK;
 }
 
@@ -687,6 +731,19 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     const float *s = ctx->S.sun_dir;
     unsigned long long *ctr = ctx->q_ctr;
     if ((rc = vf::begin_timed(ctx))) return rc;
+    if (cfg.refill < 0) {   // no queue
+        const int32_t nt = ((w + 15) / 16) * ((h + 15) / 16);
+        if (ctx->counting)
+            hipLaunchKernelGGL((k_gi_direct<true>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
+                               reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
+                               ctx->d_cnt);
+        else
+            hipLaunchKernelGGL((k_gi_direct<false>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
+                               reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
+                               ctx->d_cnt);
+        if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
+        return vf::end_timed(ctx);
+    }
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(256), 0, ctx->stream, ctr, nullptr, 0);
     hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen, 256 * kGenPer)), dim3(256), 0, ctx->stream, A,
                        reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctr, gi_hits, sun_hits);
@@ -730,6 +787,8 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
     if (!reqs || n_req <= 0 || n_req > kMaxReq)
         return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: 1..%d requests required", kMaxReq);
     if ((rc = check_cfg(ctx, cfg_in, "vf_trace_batch"))) return rc;
+    if (cfg_in && cfg_in->refill < 0)
+        return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: refill -1 (no queue) is a GI-batch mode only");
     const vf_queue_cfg cfg = cfg_in ? *cfg_in : default_cfg();
     BatchArgs B;
     std::memset(&B, 0, sizeof(B));
@@ -822,7 +881,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     static const int tiles[3][2] = {{8, 8}, {16, 8}, {16, 16}};
     static const int chunks[5] = {64, 128, 256, 512, 1024};
     const int sorts = workload == VF_TUNE_GI_BATCH ? 2 : 1;
-    static const int refills[2] = {0, 32};
+    static const int refills[3] = {-1, 0, 32};   // -1: no queue (GI batch only, one configuration)
     int k = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
@@ -833,6 +892,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     for (int so = 0; so < sorts && rc == VF_OK; ++so)
         for (const auto &t : tiles)
             for (int ch : chunks) {
+                if (rf < 0 && (workload != VF_TUNE_GI_BATCH || so != 0 || t[0] != 16 || t[1] != 16 || ch != 64)) continue;
                 vf_queue_cfg c = default_cfg();
                 c.tile_w = t[0]; c.tile_h = t[1]; c.chunk = ch; c.sort = so;
                 c.refill = rf;
