@@ -99,8 +99,8 @@ def main():
                     help="run a plain device copy of this many MiB first (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: rays are independent and the path would shard "
-                                   "by image tiles with the world replicated per GPU (see DESIGN.md); "
+        print(json.dumps({"error": "this benchmark runs on one GPU: rays are independent and the path would shard "
+                                   "by image tiles with the world replicated per GPU; "
                                    f"--gpus {args.gpus} is not supported"}))
         sys.exit(2)
     if not torch.cuda.is_available():
