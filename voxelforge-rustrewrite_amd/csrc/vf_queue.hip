@@ -85,7 +85,7 @@ __device__ __forceinline__ void gi_ray(const GenArgs &A, const float4 *gbpos, co
 // ray and then its sun ray, with the same rays and rules as the list kernels; for even work such as one
 // ray per pixel the launch's own scheduling is enough.
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_gi_direct(DevWorld W, GenArgs A, const float4 *gbpos, const uint32_t *gbn,
+__global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, GenArgs A, const float4 *gbpos, const uint32_t *gbn,
                                                    vf_hit *gi_hits, vf_hit *sun_hits, float sx, float sy, float sz,
                                                    unsigned long long *cnt) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
