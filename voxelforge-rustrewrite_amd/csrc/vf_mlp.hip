@@ -612,7 +612,7 @@ __device__ __forceinline__ void push_query(bool want, unsigned *qcount, float4 *
 // per sample the path up to its tail (pre += its light; a query at the tail, or tailv[slot] = 0);
 // terminate 0: nothing (k_mlp_infer<2> answers at the G-buffer points).
 template <bool BODIES>
-__global__ void __launch_bounds__(256) k_gc_render(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
+__global__ void __launch_bounds__(256, 3) k_gc_render(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
                                                    const uint32_t *gbn, const DevStats *st, float4 *out, float4 *pre,
                                                    float4 *tailv, float4 *queries, unsigned *qcount) {
     int32_t px, py;
