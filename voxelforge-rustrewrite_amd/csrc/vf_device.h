@@ -192,9 +192,11 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
 
 // traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5), or -1 to
 // take it from skip_rt (one inlined copy serving both).
-template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256>
+// PRE: pinv holds 1 / rd (each component clamped away from 0 as below), computed once by the caller for a
+// direction shared by many rays (the sun's): the same IEEE quotients, without three divisions per ray.
+template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256, bool PRE = false>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
-                                            Cnt &c, const int skip_rt = 0) {
+                                            Cnt &c, const int skip_rt = 0, const float *pinv = nullptr) {
     const int sk = SKIP >= 0 ? SKIP : skip_rt;
     h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
     if (COUNT) c.rays++;
@@ -203,7 +205,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
-        inv[a] = 1.0f / sd[a];
+        inv[a] = PRE ? pinv[a] : 1.0f / sd[a];
         const float lo = (float)(W.minb[a] * 8);
         const float dims = (float)(W.wb[a] * 8);
         const float t0 = (lo - ro[a]) * inv[a];

@@ -148,9 +148,10 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
 // deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
 // to the first blocker.
-template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256>
+template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
-                                                 vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr) {
+                                                 vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr,
+                                                 const float *pinv = nullptr) {
     float vis = 0.0f, tb = 0.0f;
     if (p.w < 0.0f) {
         vis = 1.0f;
@@ -162,7 +163,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         } else {
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS>(W, so, sun, leaf, seed, tb, hits, i, c, B);
+            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE>(W, so, sun, leaf, seed, tb, hits, i, c, B, pinv);
         }
     }
     const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
@@ -191,10 +192,11 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
 // gbuffer.comp then shadow_trace.comp for the same pixel in one kernel: the sun ray starts from the
 // G-buffer values this lane has just written (position, RGBA8-quantised normal), exactly what the
 // separate shadow pass would read back, so the results are those of the two passes.
-template <bool COUNT, bool SPECIAL, int LS = 256>
+template <bool COUNT, bool SPECIAL, int LS = 256, bool PRE = false>
 __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px,
                                                      int32_t py, const float sun[3], vf_hit *hits, float4 *gbpos,
-                                                     uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c) {
+                                                     uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c,
+                                                     const float *pinv = nullptr) {
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
@@ -216,7 +218,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT, false, true, LS>(W, p, nq, sun, A.leaf, shits, i, c);
+    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv);
 }
 
 template <bool COUNT, bool SPECIAL>
@@ -249,15 +251,16 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
 template <bool COUNT>
 __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
                                                          float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
-                                                         unsigned long long *cnt) {
+                                                         unsigned long long *cnt, float ix, float iy, float iz) {
     const float sun[3] = {sx, sy, sz};
+    const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
     Cnt c = {0, 0, 0, 0};
     const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
     const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
     const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
     if (px < A.w && py < A.h)
-        primary_shadow_pixel<COUNT, false, 64>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c);
+        primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -864,9 +867,11 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
                            uint32_t *shadow, vf_hit *shits) {
     const int cls = camera_class(ctx, A.pos);
     A.cam_air = cls == 0 ? 1 : 0;
+    float si[3];   // the sun ray's 1 / direction as trace_world computes it (IEEE float division on the host)
+    for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
     if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
         hipLaunchKernelGGL((k_primary_shadow64<COUNT>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1],
-                           s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
+                           s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
     else if (cls != 1)
         hipLaunchKernelGGL((k_primary_shadow<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0],
                            s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
