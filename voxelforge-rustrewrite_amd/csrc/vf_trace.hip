@@ -175,7 +175,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
 template <bool COUNT, bool BODIES>
 __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArgs A, float sx, float sy, float sz,
                                                 const float4 *gbpos, const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
-                                                unsigned long long *cnt) {
+                                                unsigned long long *cnt, float ix, float iy, float iz) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -185,7 +185,8 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     const float sun[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
-    shadow[i] = shadow_texel<COUNT, BODIES, true>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B);
+    const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
+    shadow[i] = shadow_texel<COUNT, BODIES, true, 256, true>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B, sinv);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -1505,20 +1506,22 @@ int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, con
     const float *s = ctx->S.sun_dir;
     const DevBodies B = vf::dev_bodies(ctx);
     const float4 *gp = reinterpret_cast<const float4 *>(gb_pos);
+    float si[3];   // as the traversal computes it: 1 / the clamped direction, IEEE float division
+    for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
     if ((rc = begin_timed(ctx))) return rc;
     if (B.n > 0) {
         if (ctx->counting)
             hipLaunchKernelGGL((k_shadow<true, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
-                               gp, gb_normal, shadow, hits, ctx->d_cnt);
+                               gp, gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
         else
             hipLaunchKernelGGL((k_shadow<false, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
-                               gp, gb_normal, shadow, hits, ctx->d_cnt);
+                               gp, gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
     } else if (ctx->counting) {
         hipLaunchKernelGGL((k_shadow<true, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
-                           gb_normal, shadow, hits, ctx->d_cnt);
+                           gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
     } else {
         hipLaunchKernelGGL((k_shadow<false, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
-                           gb_normal, shadow, hits, ctx->d_cnt);
+                           gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
     }
     if ((rc = launch_check(ctx, "k_shadow"))) return rc;
     return end_timed(ctx);
