@@ -107,3 +107,24 @@ def test_endless_shape_generator_small():
     assert loaded_cols.sum() <= 3 * 16 * 3 * 16
     # trees: wood and leaves exist above the ground
     assert (a["atlas"] == 6).any() and (a["atlas"] == 7).any()
+
+
+def test_brick_arguments_packed_and_as_records():
+    """The world calls' brick arguments: packed arrays (512-byte stride, gpu_slot copied) or brick records
+    whose vox / gpu_slot fields are passed in place at the record stride (how World::bricks is handed over)."""
+    packed = np.arange(3 * 512, dtype=np.uint8).reshape(3, 512)
+    bp, n, bs, sp, ss, hold, slot = vf.Context._brick_args(packed, [5, -1, 7])
+    assert (n, bs, ss) == (3, 512, 4) and bp == hold.ctypes.data and slot.tolist() == [5, -1, 7]
+    assert sp == slot.ctypes.data
+    rec_t = np.dtype([("vox", np.uint8, 512), ("gpu_slot", np.int32), ("mask_ok", np.uint8), ("mask", np.uint32, 16)],
+                     align=True)
+    rec = np.zeros(4, rec_t)
+    bp, n, bs, sp, ss, hold, slot = vf.Context._brick_args(rec, None)
+    assert (n, bs, ss) == (4, rec_t.itemsize, rec_t.itemsize) and hold is rec
+    assert bp == rec.ctypes.data + rec_t.fields["vox"][1] and sp == rec.ctypes.data + rec_t.fields["gpu_slot"][1]
+    slot[2] = 9                                      # a view of the records' own field
+    assert rec["gpu_slot"][2] == 9
+    with pytest.raises(ValueError):
+        vf.Context._brick_args(rec, [0, 0, 0, 0])    # records carry their own slots
+    with pytest.raises(ValueError):
+        vf.Context._brick_args(packed, [0, 0])       # one slot per brick
