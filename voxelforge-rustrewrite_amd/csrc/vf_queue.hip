@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -87,7 +88,7 @@ __device__ __forceinline__ void gi_ray(const GenArgs &A, const float4 *gbpos, co
 template <bool COUNT>
 __global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, GenArgs A, const float4 *gbpos, const uint32_t *gbn,
                                                    vf_hit *gi_hits, vf_hit *sun_hits, float sx, float sy, float sz,
-                                                   unsigned long long *cnt) {
+                                                   unsigned long long *cnt, float ix, float iy, float iz) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t tiles_x = (A.w + 15) / 16;
@@ -112,7 +113,8 @@ __global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, GenArgs A, con
                 float so[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) so[a] = (o[a] + d[a] * h.t) + hn[a] * 3e-2f;
-                trace_world<0, COUNT, true>(W, so, sun_dir, 1e30f, sh, c);
+                const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
+                trace_world<0, COUNT, true, 256, true>(W, so, sun_dir, 1e30f, sh, c, 0, sinv);
             }
         }
     }
@@ -733,14 +735,16 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     if ((rc = vf::begin_timed(ctx))) return rc;
     if (cfg.refill < 0) {   // no queue
         const int32_t nt = ((w + 15) / 16) * ((h + 15) / 16);
+        float si[3];   // as the traversal computes it: 1 / the clamped direction, IEEE float division
+        for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
         if (ctx->counting)
             hipLaunchKernelGGL((k_gi_direct<true>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
                                reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
-                               ctx->d_cnt);
+                               ctx->d_cnt, si[0], si[1], si[2]);
         else
             hipLaunchKernelGGL((k_gi_direct<false>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
                                reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
-                               ctx->d_cnt);
+                               ctx->d_cnt, si[0], si[1], si[2]);
         if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
         return vf::end_timed(ctx);
     }
