@@ -148,16 +148,18 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
 // deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
 // to the first blocker.
-template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false>
+// NDEC: N holds the decoded normal already (nq is not read).
+template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false, bool NDEC = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr,
-                                                 const float *pinv = nullptr) {
+                                                 const float *pinv = nullptr, const float *Ndec = nullptr) {
     float vis = 0.0f, tb = 0.0f;
     if (p.w < 0.0f) {
         vis = 1.0f;
         if (hits) store_miss(hits + i);
     } else {
-        const float N[3] = {unorm8_dec(nq & 255u), unorm8_dec((nq >> 8) & 255u), unorm8_dec((nq >> 16) & 255u)};
+        const float N[3] = {NDEC ? Ndec[0] : unorm8_dec(nq & 255u), NDEC ? Ndec[1] : unorm8_dec((nq >> 8) & 255u),
+                            NDEC ? Ndec[2] : unorm8_dec((nq >> 16) & 255u)};
         if (dot3(N, sun) <= 0.0f) {
             if (hits) store_miss(hits + i);
         } else {
@@ -207,19 +209,25 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     store_hit(hits + i, h);
     float4 p;
     uint32_t nq = 0u;
+    // the quantised face normal as the shadow pass decodes it: unorm8 bytes 0 / 128 / 255 give exactly
+    // -1, (128 / 255) * 2 - 1 and 1 (the middle value folded at compile time, IEEE like the division)
+    constexpr float kDec128 = (128.0f / 255.0f) * 2.0f - 1.0f;
+    float Nd[3] = {kDec128, kDec128, kDec128};
     if (h.hit) {
         p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
         float n[3];
         face_normal(h.face, n);
         nq = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
              (255u << 24);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Nd[a] = n[a] > 0.0f ? 1.0f : (n[a] < 0.0f ? -1.0f : kDec128);
     } else {
         p = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     }
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv);
+    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE, true>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv, Nd);
 }
 
 template <bool COUNT, bool SPECIAL>
