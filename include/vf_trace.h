@@ -377,6 +377,14 @@ typedef struct vf_request {
 /* All requests in ONE persistent launch; each request's completion time is stamped on the device.
  * Request order is the service order (put the latency-critical ones first).  Recordable into a graph. */
 int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg);
+/* vf_trace_batch without any allocation: no reference counterpart.  For callers that cannot allocate device
+ * memory (code under HIP graph capture before the batch ever ran eagerly, serving loops and harnesses that
+ * forbid hipMalloc).  Same requests, outputs and limits as vf_trace_batch (1..VF_MAX_REQUESTS requests, else
+ * VF_E_INVALID); the batch's counters and completion stamps live in a 512-byte array that the library keeps
+ * on each GPU until the process ends, instead of the context's queue buffers.  Every call on one GPU shares
+ * that array (the batch clears it first): calls that overlap on one GPU, from any context or stream, race.
+ * vf_read_batch_times reads the stamps of whichever batch ran last on the context. */
+int vf_trace_batch_noalloc(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg);
 /* Per-request completion times of the last batch, ms after the launch started (host, synchronises). */
 int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms);
 

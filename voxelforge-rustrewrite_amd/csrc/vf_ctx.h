@@ -63,6 +63,10 @@ struct vf_ctx {
     unsigned long long *q_ctr = nullptr, *q_stamps = nullptr;
     float4 *q_rays = nullptr, *q_sun = nullptr;
     int64_t q_cap = 0;
+    // the allocation-free batch's counters and stamps: this device's copy of a file-scope __device__ array
+    // (vf_queue.hip), its address taken once per device at context creation; never freed
+    unsigned long long *batch_scratch = nullptr;
+    const unsigned long long *batch_stamps = nullptr;   // where the last batch on this context stamped
     // bodies (vf_bodies.hip): repacked headers, grids, BVH; grown, never shrunk, so frame graphs that
     // recorded them stay valid while the body count is unchanged
     int32_t n_bodies = 0;
@@ -93,6 +97,7 @@ int end_timed(vf_ctx *ctx);
 int check_ready(vf_ctx *ctx, const char *what);
 int no_capture(vf_ctx *ctx, const char *what);
 void free_queue(vf_ctx *ctx);
+int queue_scratch_init(vf_ctx *ctx);   // vf_queue.hip: the device address of the batch scratch (device current)
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 // bodies (vf_bodies.hip): the view the kernels get (n = 0 unless the scene's num_objects is set), and the
 // launch-time check that the scene's num_objects matches the uploaded bodies and their bricks exist

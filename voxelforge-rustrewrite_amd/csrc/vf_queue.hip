@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -29,12 +30,19 @@
 #include "vf_device.h"
 #include "vf_internal.h"
 
+// Counters and stamps of vf_trace_batch_noalloc.  One copy per GPU, held until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned long long scratch[64];
+
 namespace {
 
 constexpr int kMaxReq = VF_MAX_REQUESTS;
 // counters: [0] ray-list length / batch head, [1] GI head, [2] sun tail, [3] sun head,
 // [8 + r] remaining rays of request r
 constexpr int kCtr = 8 + kMaxReq;
+// The allocation-free batch's scratch: kCtr counters, then kMaxReq + 1 completion stamps.
+constexpr int kScratchWords = 64;   // 512 B
+static_assert(kCtr + kMaxReq + 1 <= kScratchWords, "batch scratch too small for the request table");
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
@@ -633,7 +641,28 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, 
 // ------------------------------------------------------------------------------------------------
 namespace vf {
 
+// The device address of `scratch` on the context's device, taken once per device (with that device
+// current) and kept for every later context on it.
+int queue_scratch_init(vf_ctx *ctx) {
+    static_assert(sizeof(scratch) == kScratchWords * sizeof(unsigned long long), "scratch size");
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned long long *per_device[kMaxDevices] = {};
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return VF_OK;   // the allocation-free batch then refuses
+    std::lock_guard<std::mutex> lock(mu);
+    if (!per_device[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return vf::fail(ctx, VF_E_DEVICE, "batch scratch: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+        per_device[ctx->device] = static_cast<unsigned long long *>(p);
+    }
+    ctx->batch_scratch = per_device[ctx->device];
+    return VF_OK;
+}
+
 void free_queue(vf_ctx *ctx) {
+    if (ctx->batch_stamps == ctx->q_stamps) ctx->batch_stamps = nullptr;
     void *ptrs[] = {ctx->q_ctr, ctx->q_stamps, ctx->q_rays, ctx->q_sun};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -783,16 +812,26 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     return vf::end_timed(ctx);
 }
 
-int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in) {
-    int rc = vf::check_ready(ctx, "vf_trace_batch");
+}  // extern "C"
+
+namespace {
+
+// vf_trace_batch's work with its counters and stamps either in the context's queue buffers (allocated on
+// first use) or, with noalloc, in this device's `scratch` (nothing allocated).
+int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in, bool noalloc) {
+    const char *what = noalloc ? "vf_trace_batch_noalloc" : "vf_trace_batch";
+    int rc = vf::check_ready(ctx, what);
     if (rc) return rc;
     if (ctx->scene.num_objects != 0)
-        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_trace_batch: the persistent queue traces the world only (the scene has bodies)");
+        return vf::fail(ctx, VF_E_UNSUPPORTED, "%s: the persistent queue traces the world only (the scene has bodies)", what);
     if (!reqs || n_req <= 0 || n_req > kMaxReq)
-        return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: 1..%d requests required", kMaxReq);
-    if ((rc = check_cfg(ctx, cfg_in, "vf_trace_batch"))) return rc;
+        return vf::fail(ctx, VF_E_INVALID, "%s: 1..%d requests required (the batch's counters and stamps are sized for %d)",
+                        what, kMaxReq, kMaxReq);
+    if (noalloc && !ctx->batch_scratch)
+        return vf::fail(ctx, VF_E_UNSUPPORTED, "%s: no batch scratch on device %d", what, ctx->device);
+    if ((rc = check_cfg(ctx, cfg_in, what))) return rc;
     if (cfg_in && cfg_in->refill < 0)
-        return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: refill -1 (no queue) is a GI-batch mode only");
+        return vf::fail(ctx, VF_E_INVALID, "%s: refill -1 (no queue) is a GI-batch mode only", what);
     const vf_queue_cfg cfg = cfg_in ? *cfg_in : default_cfg();
     BatchArgs B;
     std::memset(&B, 0, sizeof(B));
@@ -802,12 +841,12 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
     for (int k = 0; k < n_req; ++k) {
         const vf_request &q = reqs[k];
         ReqDev &d = B.req[k];
-        if (!q.d_hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d has no output", k);
+        if (!q.d_hits) return vf::fail(ctx, VF_E_INVALID, "%s: request %d has no output", what, k);
         d.kind = q.kind;
         d.hits = q.d_hits;
         if (q.kind == VF_REQ_VIEW) {
             if (q.width <= 0 || q.height <= 0 || (int64_t)q.width * q.height > (1ll << 28))
-                return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: bad view size", k);
+                return vf::fail(ctx, VF_E_INVALID, "%s: request %d: bad view size", what, k);
             d.w = q.width; d.h = q.height;
             d.tile_w = cfg.tile_w; d.tile_h = cfg.tile_h;
             d.tiles_x = (q.width + cfg.tile_w - 1) / cfg.tile_w;
@@ -822,7 +861,7 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
             counts[k] = d.count;
         } else if (q.kind == VF_REQ_PROBES) {
             if (q.n_rays <= 0 || q.n_rays > 65536 || q.dims[0] <= 0 || q.dims[1] <= 0 || q.dims[2] <= 0)
-                return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: bad probe grid", k);
+                return vf::fail(ctx, VF_E_INVALID, "%s: request %d: bad probe grid", what, k);
             for (int a = 0; a < 3; ++a) { d.v[a] = q.origin[a]; d.dims[a] = q.dims[a]; }
             d.v[3] = q.spacing;
             d.n_rays = q.n_rays;
@@ -831,47 +870,68 @@ int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_
             d.count = (uint64_t)q.dims[0] * q.dims[1] * q.dims[2] * (uint64_t)q.n_rays;
             counts[k] = d.count;
         } else {
-            return vf::fail(ctx, VF_E_INVALID, "vf_trace_batch: request %d: unknown kind %d", k, q.kind);
+            return vf::fail(ctx, VF_E_INVALID, "%s: request %d: unknown kind %d", what, k, q.kind);
         }
         d.first = at;
         at += (d.count + (uint64_t)cfg.chunk - 1) / (uint64_t)cfg.chunk * (uint64_t)cfg.chunk;
     }
     B.total = at;
     (void)counts;
-    if ((rc = reserve_queue(ctx, 1))) return rc;
+    unsigned long long *ctr, *stamps;
+    if (noalloc) {
+        ctr = ctx->batch_scratch;
+        stamps = ctx->batch_scratch + kCtr;
+    } else {
+        if ((rc = reserve_queue(ctx, 1))) return rc;
+        ctr = ctx->q_ctr;
+        stamps = ctx->q_stamps;
+    }
     if ((rc = vf::begin_timed(ctx))) return rc;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctx->q_ctr, ctx->q_stamps);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctr, stamps);
     if (cfg.refill == 0) {
         const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_chunked<true>)
                                        : reinterpret_cast<const void *>(k_batch_chunked<false>);
         const int g = persistent_groups(ctx, kp, &cfg);
         if (ctx->counting)
-            hipLaunchKernelGGL(k_batch_chunked<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                               cfg.chunk, ctx->d_cnt);
+            hipLaunchKernelGGL(k_batch_chunked<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
+                               ctx->d_cnt);
         else
-            hipLaunchKernelGGL(k_batch_chunked<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                               cfg.chunk, ctx->d_cnt);
+            hipLaunchKernelGGL(k_batch_chunked<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
+                               ctx->d_cnt);
     } else {
         const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_persist<true>)
                                        : reinterpret_cast<const void *>(k_batch_persist<false>);
         const int g = persistent_groups(ctx, kp, &cfg);
         if (ctx->counting)
-            hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                               cfg.chunk, cfg.refill, ctx->d_cnt);
+            hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
+                               cfg.refill, ctx->d_cnt);
         else
-            hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctx->q_ctr, ctx->q_stamps,
-                               cfg.chunk, cfg.refill, ctx->d_cnt);
+            hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
+                               cfg.refill, ctx->d_cnt);
     }
-    if ((rc = vf::launch_check(ctx, "vf_trace_batch"))) return rc;
+    if ((rc = vf::launch_check(ctx, what))) return rc;
+    ctx->batch_stamps = stamps;
     return vf::end_timed(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vf_trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in) {
+    return trace_batch(ctx, reqs, n_req, cfg_in, false);
+}
+
+int vf_trace_batch_noalloc(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_queue_cfg *cfg_in) {
+    return trace_batch(ctx, reqs, n_req, cfg_in, true);
 }
 
 int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms) {
     if (!ctx || !ms || n_req <= 0 || n_req > kMaxReq) return vf::fail(ctx, VF_E_INVALID, "vf_read_batch_times: bad argument");
     if (int rc = vf::no_capture(ctx, "vf_read_batch_times")) return rc;
-    if (!ctx->q_stamps) return vf::fail(ctx, VF_E_INVALID, "vf_read_batch_times: no batch has run");
+    if (!ctx->batch_stamps) return vf::fail(ctx, VF_E_INVALID, "vf_read_batch_times: no batch has run");
     unsigned long long st[kMaxReq + 1];
-    VF_HIP(ctx, hipMemcpyAsync(st, ctx->q_stamps, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    VF_HIP(ctx, hipMemcpyAsync(st, ctx->batch_stamps, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double tick_ms = 1.0 / (double)ctx->wall_khz;
     for (int k = 0; k < n_req; ++k) ms[k] = st[k] ? (double)(st[k] - st[kMaxReq]) * tick_ms : -1.0;

@@ -1159,6 +1159,10 @@ int vf_ctx_create(int device, vf_ctx **out) {
         return rc;
     }
     ctx->stream = ctx->own_stream;
+    if ((rc = vf::queue_scratch_init(ctx))) {
+        vf_ctx_destroy(ctx);
+        return rc;
+    }
     {
         int cu = 0, khz = 0;
         if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) ctx->n_cu = cu;

@@ -15,7 +15,7 @@ import numpy as np
 
 from .abi import (BVH_NODE_DTYPE, BvhNode, GenParams, GI_RECORD_DTYPE, GICACHE_INPUTS, GICACHE_PARAMS, GiCacheParams,
                   GiCacheStats, Hit, HIT_DTYPE, Material, OBJ_HEADER_DTYPE, ObjHeader, PassUbo, QueueCfg, Ray, RAY_DTYPE,
-                  Request, SceneUbo, TuneBatchArgs, TuneGiArgs, TuneResult, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH,
+                  Request, SceneUbo, TuneBatchArgs, TuneGiArgs, TuneResult, VF_MAX_REQUESTS, VF_REQ_PROBES, VF_REQ_VIEW, VF_TUNE_BATCH,
                   VF_TUNE_GI_BATCH, WorldDesc, WorldSrc, Chunk)
 
 __all__ = ["VfError", "lib", "Context", "GenWorld", "SceneUbo", "PassUbo", "Material", "Ray", "Hit", "HIT_DTYPE",
@@ -84,6 +84,7 @@ _SIGS = {
     "vf_trace_gi_batch": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_float, _P, _P, _P, _P,
                                     C.POINTER(QueueCfg)]),
     "vf_trace_batch": (C.c_int, [_P, C.POINTER(Request), C.c_int32, C.POINTER(QueueCfg)]),
+    "vf_trace_batch_noalloc": (C.c_int, [_P, C.POINTER(Request), C.c_int32, C.POINTER(QueueCfg)]),
     "vf_read_batch_times": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     "vf_autotune": (C.c_int, [_P, C.c_int32, _P, C.POINTER(QueueCfg), C.POINTER(QueueCfg), C.POINTER(TuneResult),
                               C.c_int32, C.POINTER(C.c_int32)]),
@@ -673,6 +674,12 @@ class Context:
     def trace_batch(self, requests, cfg=None):
         arr = (Request * len(requests))(*requests)
         self._ck(lib().vf_trace_batch(self._h, arr, len(requests), C.byref(cfg) if cfg is not None else None))
+
+    def trace_batch_noalloc(self, requests, cfg=None):
+        """vf_trace_batch_noalloc: the same batch with its counters in the library's per-GPU scratch (no
+        device allocation)."""
+        arr = (Request * len(requests))(*requests)
+        self._ck(lib().vf_trace_batch_noalloc(self._h, arr, len(requests), C.byref(cfg) if cfg is not None else None))
 
     def batch_times(self, n):
         out = (C.c_double * n)()

@@ -165,6 +165,7 @@ GICACHE_PARAMS = 23043
 GICACHE_INPUTS = 96
 
 VF_REQ_VIEW, VF_REQ_PROBES = 0, 1
+VF_MAX_REQUESTS = 24   # requests per uneven batch (include/vf_trace.h)
 VF_TUNE_GI_BATCH, VF_TUNE_BATCH = 0, 1
 
 # numpy views of the 24-byte ray and 20-byte hit records
