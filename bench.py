@@ -272,8 +272,13 @@ def main():
     }
 
     if not args.no_cpu:
+        # one more (untimed) fused launch that also writes the sun rays' first-segment hits, for the check
+        shh = torch.empty(n * HIT_BYTES, dtype=torch.uint8, device=dev)
+        ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh, shh)
+        torch.cuda.synchronize()
         result["cpu_baseline"] = cpu_baseline(world, win, arrays_wb, cam_pos, right, up, fwd, pu.tan_half_fov, scene,
-                                              hits, args.cpu_seconds)
+                                              hits, gbn, sh, shh, args.cpu_seconds)
+        del shh
     for g in (g_step, g_two, g_prim, g_shad):
         g.close()
     if not args.no_extra:
@@ -713,15 +718,27 @@ def _random_queries(n, dev):
     return rec.to(dev).view(torch.uint8).view(-1)
 
 
-def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, min_seconds):
-    """The C oracle (OpenMP over the host cores) on the same workload: the whole frame's primary and
-    sun rays, repeated until min_seconds have passed.  Its hits double as a hit-exact check of the
-    GPU's whole frame."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, gpu_gbn, gpu_sh, gpu_shh, min_seconds):
+    """The C oracle (OpenMP over the host cores this process may use, at most the 16 a one-GPU share of
+    the machine gets) on the same workload: the whole frame's primary and sun rays, repeated until
+    min_seconds have passed.  Its outputs double as a bit-exact check of the GPU's whole frame: primary
+    hits, gbNormal (foliage alpha included), shadowRaw and the sun rays' first-segment hits."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    affinity = len(os.sched_getaffinity(0))
+    cores = min(affinity, 16)
     t0 = time.time()
     ow = O.OracleWorld(world.arrays(), build_skytop=False)
     log(f"oracle world built in {time.time() - t0:.1f}s")
@@ -731,7 +748,7 @@ def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, 
     t0 = time.perf_counter()
     while True:
         oh, opos, onrm, cp = ow.pass_primary_rows(cam, W, H, 0, 1, H, threads=cores)
-        osh, _, cs = ow.pass_shadow(sun, W, H, opos, onrm, threads=cores)
+        osh, oshh, cs = ow.pass_shadow(sun, W, H, opos, onrm, threads=cores)
         rays += cp["rays"] + cs["rays"]
         frames += 1
         dt = time.perf_counter() - t0
@@ -739,16 +756,32 @@ def cpu_baseline(world, win, wb, cam_pos, right, up, fwd, thf, scene, gpu_hits, 
             break
     g = vf.hits_to_numpy(gpu_hits)
     mism = int(((g.view(np.uint8).reshape(-1, 20) != oh.view(np.uint8).reshape(-1, 20)).any(1)).sum())
+    gn = gpu_gbn.cpu().numpy().view(np.uint32)
+    nrm_mism = int((gn != onrm).sum())
+    want_sh = (osh[:, 0].astype(np.float16).view(np.uint16).astype(np.uint32) |
+               (osh[:, 1].astype(np.float16).view(np.uint16).astype(np.uint32) << 16))
+    sh_mism = int((gpu_sh.cpu().numpy().view(np.uint32) != want_sh).sum())
+    gs = vf.hits_to_numpy(gpu_shh)
+    shh_mism = int(((gs.view(np.uint8).reshape(-1, 20) != oshh.view(np.uint8).reshape(-1, 20)).any(1)).sum())
+    nb = onrm
+    N = np.stack([((nb >> (8 * k)) & 255).astype(np.float32) / 255 * 2 - 1 for k in range(3)], 1)
+    back_lit = int(((opos[:, 3] >= 0) & ((nb >> 24) == 204) & ((N * sun).sum(1) <= 0)).sum())
     return {
         "value": round(rays / dt / 1e9, 6),
         "unit": "Grays/s",
         "cores": cores,
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": affinity,
         "kind": "port",
         "sample": f"the whole 1920x1080 frame ({cp['rays']} primary + {cs['rays']} sun rays) x {frames} repeats; "
-                  f"the project's C oracle (FP32 restatement of traceVolumeS) with OpenMP; lavapipe baseline "
-                  f"unavailable (no cargo / shaderc / Vulkan loader on the box)",
+                  f"the project's C oracle (FP32 restatement of traceVolumeS) with OpenMP on {cores} threads (the "
+                  f"CPU share of one GPU, of {affinity} usable); lavapipe baseline unavailable (no cargo / shaderc / "
+                  f"Vulkan loader on the box)",
         "seconds": round(dt, 3),
-        "hit_exact_check": {"rays": int(g.size), "mismatches": mism},
+        "hit_exact_check": {"rays": int(g.size), "mismatches": mism, "gb_normal_mismatches": nrm_mism,
+                            "sun_rays": int(cs["rays"]), "back_lit_foliage_rays": back_lit,
+                            "sun_mismatches": sh_mism, "sun_hit_mismatches": shh_mism},
     }
 
 

@@ -282,7 +282,8 @@ int vf_set_scene(vf_ctx *ctx, const vf_scene_ubo *scene);
  * scene (the world, and the bodies when the scene's num_objects is set).  A camera inside solid terrain starts each ray where it leaves the solid (exitSolid,
  * common.glsl:990-1016; t counts from the camera, rays that only leave the world are misses); a
  * camera in water traces with skipMat 5.  Writes hits [w*h], and when non-NULL gbPos (RGBA32F: P, t;
- * sky (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5, a=255; sky 0).  Row 0 is the image bottom.
+ * sky (0,0,0,-1)) and gbNormal (RGBA8 unorm: n*0.5+0.5; a = matFlag, gbuffer.comp:524-526: 0.8 (204) on a
+ * foliage material (palette kind 3), 1.0 (255) otherwise; sky 0).  Row 0 is the image bottom.
  * A frame graph records the camera case with the pass: record it again after an edit of the
  * camera's voxel. */
 int vf_trace_primary(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, vf_hit *d_hits,
@@ -298,12 +299,17 @@ int vf_trace_gbuffer(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_
 /* P::ShTrace (record.rs:463; shadow_trace.comp:17-105 without the hybrid map, grass and under-water
  * refraction): one sun ray per G-buffer hit from P + N*0.03 through the dry field (water clear), with
  * FOLIAGE_LIGHT: sunVisibilityT (common.glsl:1964-1981, deterministic) lets pass->look_p[3]
- * (r.foliage.light) of the light through each leaf voxel, up to 6 (0 = a hard shadow ray).  shadowRaw
- * is RG16F (visibility, min(tFirstBlocker, 60000)); d_hits (nullable) gets the first segment's hit. */
+ * (r.foliage.light) of the light through each leaf voxel, up to 6 (0 = a hard shadow ray).  A texel
+ * facing away from the sun gets no ray, unless it is foliage (gbNormal.a in (0.62, 0.88)) with the sun
+ * above the horizon: FOLIAGE_SSS (shadow_trace.comp:35-46, the reference default) traces sunVisibility
+ * from foliageSunOrigin (P - N*0.5 + sun*0.9; a blade, a in (0.62, 0.75): P + sun*0.05) and stores no
+ * blocker distance.  shadowRaw is RG16F (visibility, min(tFirstBlocker, 60000)); d_hits (nullable) gets
+ * the first segment's hit. */
 int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, const float *d_gb_pos,
                     const uint32_t *d_gb_normal, uint32_t *d_shadow_raw, vf_hit *d_hits);
-/* The SVGF lighting pass's sun ray (lighting.comp:44-84): per lit G-buffer texel one ray along
- * sampleCone(sunDir, scene sun_angle) (common.glsl:1259-1265) seeded pixelSeed(pix, frame_idx, 0),
+/* The SVGF lighting pass's sun ray (lighting.comp:44-84): per lit G-buffer texel, or back-lit foliage
+ * texel with the sun above the horizon (FOLIAGE_SSS, from foliageSunOrigin as in vf_trace_shadow), one ray
+ * along sampleCone(sunDir, scene sun_angle) (common.glsl:1259-1265) seeded pixelSeed(pix, frame_idx, 0),
  * through leaves with probability look_p[3] (sunVisibility, stochastic).  d_shadow: float per pixel,
  * the raw sample (1 sky, 0 facing away / blocked, 1 lit); d_hits (nullable): the first segment. */
 int vf_trace_sun_cone(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, const float *d_gb_pos,

@@ -22,7 +22,8 @@ class VfoWorld(C.Structure):
                 ("n_slots", C.c_uint32), ("dist", C.c_void_p), ("dry", C.c_void_p), ("coarse1", C.c_void_p),
                 ("coarse2", C.c_void_p), ("c1", C.c_int32 * 3), ("c2", C.c_int32 * 3), ("use_dist", C.c_int32),
                 ("coarse_skip", C.c_int32), ("n_bodies", C.c_int32), ("bodies", C.c_void_p), ("obj_grid", C.c_void_p),
-                ("bvh", C.c_void_p), ("n_nodes", C.c_int32), ("bvh_prims", C.c_void_p), ("use_bvh", C.c_int32)]
+                ("bvh", C.c_void_p), ("n_nodes", C.c_int32), ("bvh_prims", C.c_void_p), ("use_bvh", C.c_int32),
+                ("palette", C.c_void_p)]
 
 
 class VfoHit(C.Structure):
@@ -191,6 +192,7 @@ class OracleWorld:
         self.s.c2[:] = self.c2d
         self.s.use_dist = int(bool(use_dist))
         self.s.coarse_skip = int(bool(coarse_skip))
+        self.s.palette = self.palette.ctypes.data
         self.skytop = np.zeros(self.wb[0] * 8 * self.wb[2] * 8, np.uint16)
         if build_skytop:
             kinds = np.ascontiguousarray(self.palette[:, 6])

@@ -946,11 +946,13 @@ static void primary_trace(const vfo_world *w, const float cam[3], uint32_t cam_m
 static inline uint32_t unorm8(float v) { return (uint32_t)floorf(fclamp(v, 0.0f, 1.0f) * 255.0f + 0.5f); }
 static inline float unorm8_dec(uint32_t b) { return ((float)b / 255.0f) * 2.0f - 1.0f; }
 /* gbNormal RGBA8 of the hit's world normal (n * 0.5 + 0.5; a NaN component stores 0, as Vulkan's
- * float -> unorm conversion does). */
+ * float -> unorm conversion does) with gbNormal.a = matFlag (gbuffer.comp:524-526, WATER_CLEAR on): 0.8 for
+ * a foliage material (palette kind 3), else 1.0. */
 static uint32_t pack_normal(const vfo_world *w, const vfo_hit *h) {
     float n[3];
     vfo_hit_normal(w, h, n);
-    uint32_t v = 255u << 24;
+    const float mat_flag = (w->palette && w->palette[(size_t)h->mat * 12 + 6] == 3.0f) ? 0.8f : 1.0f;
+    uint32_t v = unorm8(mat_flag) << 24;
     for (int a = 0; a < 3; ++a) {
         float x = n[a] * 0.5f + 0.5f;
         v |= (x == x ? unorm8(x) : 0u) << (8 * a);
@@ -1025,6 +1027,14 @@ void vfo_pass_primary_rows(const vfo_world *w, const vfo_cam *c, int32_t W, int3
     if (cnt) add_cnt(cnt, &total);
 }
 
+/* isFoliageFlag / isBladeFlag (common.glsl:1953-1954) of the sampled gbNormal.a, and foliageSunOrigin
+ * (common.glsl:1987-1989). */
+static inline int foliage_flag(float a) { return a > 0.62f && a < 0.88f; }
+static inline int blade_flag(float a) { return a > 0.62f && a < 0.75f; }
+static void foliage_sun_origin(const float P[3], const float N[3], const float sun[3], int blade, float o[3]) {
+    for (int a = 0; a < 3; ++a) o[a] = blade ? P[a] + sun[a] * 0.05f : (P[a] - N[a] * 0.5f) + sun[a] * 0.9f;
+}
+
 void vfo_pass_shadow(const vfo_world *w, const float *palette, const float sun[3], float leaf_light, int32_t W,
                      int32_t H, const float *gb_pos, const uint32_t *gb_normal, float *shadow, vfo_hitrec *hits,
                      vfo_count *cnt, int threads) {
@@ -1046,6 +1056,15 @@ void vfo_pass_shadow(const vfo_world *w, const float *palette, const float sun[3
                 float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
                 if (dot3(N, sun) <= 0.0f) {
                     vis = 0.0f; tb = 0.0f;
+                    /* FOLIAGE_SSS (shadow_trace.comp:35-46): back-lit foliage with the sun above the horizon
+                     * gets sunVisibility from foliageSunOrigin, stored without a blocker distance */
+                    const float fa = (float)(nb >> 24) / 255.0f;
+                    if (foliage_flag(fa) && sun[1] > 0.0f) {
+                        float so[3], tf;
+                        uint32_t seed = 0u;
+                        foliage_sun_origin(pt, N, sun, blade_flag(fa), so);
+                        vis = vfo_sun_visibility_t(w, palette, so, sun, leaf_light, &seed, 0, &tf, &h, &cc);
+                    }
                 } else {
                     /* FOLIAGE_LIGHT (shadow_trace.comp:96-99): sunVisibilityT, deterministic */
                     float so[3];
@@ -1198,8 +1217,9 @@ void vfo_sample_cone(const float dir[3], float ang, uint32_t *state, float out[3
     normalize3(out);
 }
 
-/* lighting.comp's sun sample (lighting.comp:44-84): 1 sky, 0 facing away, else stochastic sunVisibility
- * along sampleCone(sunDir, sun_angle), seed pixelSeed(pix, frame, 0). */
+/* lighting.comp's sun sample (lighting.comp:44-84): 1 sky; a lit texel, or a back-lit foliage texel with the
+ * sun above the horizon (FOLIAGE_SSS, from foliageSunOrigin), gets a stochastic sunVisibility along
+ * sampleCone(sunDir, sun_angle), seed pixelSeed(pix, frame, 0); any other texel facing away 0. */
 void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun[3], float sun_angle, float leaf_light,
                        uint32_t frame, int32_t W, int32_t H, const float *gb_pos, const uint32_t *gb_normal, float *out,
                        vfo_hitrec *hits, int threads) {
@@ -1216,10 +1236,20 @@ void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun
             float N[3] = {unorm8_dec(nb & 255u), unorm8_dec((nb >> 8) & 255u), unorm8_dec((nb >> 16) & 255u)};
             uint32_t seed = vfo_pixel_seed(px, py, frame, 0u);
             v = 0.0f;
-            if (gmax(dot3(N, sun), 0.0f) > 0.0f) {
-                float sd[3], so[3], tf;
+            const int lit = gmax(dot3(N, sun), 0.0f) > 0.0f;
+            int ray = lit;
+            float so[3];
+            for (int a = 0; a < 3; ++a) so[a] = pt[a] + N[a] * 3e-2f;
+            if (!lit) {   /* FOLIAGE_SSS (lighting.comp:66-71) */
+                const float fa = (float)(nb >> 24) / 255.0f;
+                if (foliage_flag(fa) && sun[1] > 0.0f) {
+                    ray = 1;
+                    foliage_sun_origin(pt, N, sun, blade_flag(fa), so);
+                }
+            }
+            if (ray) {
+                float sd[3], tf;
                 vfo_sample_cone(sun, sun_angle, &seed, sd);
-                for (int a = 0; a < 3; ++a) so[a] = pt[a] + N[a] * 3e-2f;
                 v = vfo_sun_visibility_t(w, palette, so, sd, leaf_light, &seed, 1, &tf, &h, NULL);
             }
         }

@@ -58,6 +58,7 @@ typedef struct vfo_world {
     int32_t n_nodes;
     const uint32_t *bvh_prims;
     int32_t use_bvh;          /* USE_OBJ_BVH (default on, renderer/mod.rs:96); 0 = the loop over every body */
+    const float *palette;     /* materials[256] (12 floats each, kind at [6]): gbNormal.a; NULL = no foliage */
 } vfo_world;
 
 typedef struct vfo_hit {

@@ -448,6 +448,53 @@ def test_shadow_through_leaves_matches_oracle(ctx, leaf):
     assert (partial > 0.01) == (leaf > 0.0), partial
 
 
+def test_back_lit_foliage_matches_oracle(ctx):
+    """FOLIAGE_SSS, the reference default (shadow_trace.comp:35-46, lighting.comp:66-71): on the forest with the
+    camera facing the sun, leaf texels facing away from the sun (gbNormal.a = 0.8) trace the sun through the
+    leaf from foliageSunOrigin.  The fused pass, the G-buffer + shadow pair and the SVGF sun cone match the
+    oracle bit for bit, G-buffer alpha included, and such rays are more than 1 % of the frame."""
+    w, a, ow, scene, pos = _forest(ctx)
+    sun = np.array(scene.sun_dir, np.float32)
+    W, H = 320, 200
+    yaw = float(np.arctan2(sun[0], sun[2]))
+    views = [(pos, -0.05), (pos, 0.2), None, ([32.5, 9.5, 32.5], 0.1)]
+    lit_some = False
+    for v in views:
+        if v is None:   # single leaves over a floor: back-lit leaves with the sun through 0..5 further leaves
+            w, a, ow = _scattered_leaves()
+            scene = upload(ctx, w, a)
+            continue
+        pu, cam = camera(v[0], yaw, v[1], frame=3)
+        hits, gbp, gbn = dev_hits(W * H), torch.zeros(W * H * 4, dtype=torch.float32, device=DEV), \
+            torch.zeros(W * H, dtype=torch.int32, device=DEV)
+        sh, shh = torch.zeros(W * H, dtype=torch.int32, device=DEV), dev_hits(W * H)
+        ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh, shh)
+        sh2, shh2 = torch.zeros(W * H, dtype=torch.int32, device=DEV), dev_hits(W * H)
+        ctx.trace_shadow(W, H, gbp, gbn, sh2, shh2, pass_ubo=pu)
+        cone, coneh = torch.zeros(W * H, dtype=torch.float32, device=DEV), dev_hits(W * H)
+        ctx.trace_sun_cone(pu, W, H, gbp, gbn, cone, coneh)
+        ctx.sync()
+        oh, opos, onrm, _ = ow.pass_primary(cam, W, H)
+        assert hit_diff(vf.hits_to_numpy(hits), oh).size == 0
+        assert np.array_equal(gbn.cpu().numpy().view(np.uint32), onrm)
+        osh, ohits, ocnt = ow.pass_shadow(sun, W, H, opos, onrm, leaf_light=0.6)
+        want = ((osh[:, 0].astype(np.float16).view(np.uint16).astype(np.uint32)) |
+                (osh[:, 1].astype(np.float16).view(np.uint16).astype(np.uint32) << 16))
+        for g, gh in ((sh, shh), (sh2, shh2)):
+            assert np.array_equal(g.cpu().numpy().view(np.uint32), want)
+            assert hit_diff(vf.hits_to_numpy(gh), ohits).size == 0
+        oc, och = ow.pass_sun_cone(sun, scene.sun_angle, 0.6, 3, W, H, opos, onrm)
+        assert np.array_equal(cone.cpu().numpy(), oc)
+        assert hit_diff(vf.hits_to_numpy(coneh), och).size == 0
+        nb = onrm.view(np.uint32)
+        N = np.stack([((nb >> (8 * k)) & 255).astype(np.float32) / 255 * 2 - 1 for k in range(3)], 1)
+        back_lit = (opos[:, 3] >= 0) & ((nb >> 24) == 204) & ((N * sun).sum(1) <= 0)
+        assert back_lit.mean() > 0.01, back_lit.mean()
+        assert (osh[back_lit, 1] == 0).all()
+        lit_some |= bool((osh[back_lit, 0] > 0).any())
+    assert lit_some
+
+
 def test_sun_cone_matches_oracle(ctx):
     """lighting.comp's cone-jittered, stochastic sun sample: bit-identical to the oracle."""
     w, a, ow = _scattered_leaves()

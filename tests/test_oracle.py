@@ -493,3 +493,118 @@ def test_sample_cone_stays_in_the_sun_disk():
             r2.append(math.tan(th) ** 2)
         # r = ang * sqrt(u): uniform over the disk, E[r^2] = ang^2 / 2
         assert abs(np.mean(r2) / (ang * ang) - 0.5) < 0.06
+
+
+# --- back-lit foliage (FOLIAGE_SSS: shadow_trace.comp:35-46, lighting.comp:66-71, common.glsl:1953-1989) ---
+def _nbytes(n, alpha):
+    b = [int(math.floor(min(max(c * 0.5 + 0.5, 0.0), 1.0) * 255.0 + 0.5)) for c in n]
+    return np.uint32(b[0] | (b[1] << 8) | (b[2] << 16) | (alpha << 24))
+
+
+def _leaf_slabs_world(ks, y0=20):
+    """64^3 voxels: stone floor at y < 8; column j (x = 4 + 4j, z = 20) holds a slab of ks[j] leaf voxels
+    from y0 up.  Indexed (x, y, z)."""
+    vox = np.zeros((64, 64, 64), np.uint8)
+    vox[:, 0:8, :] = 3
+    for j, k in enumerate(ks):
+        vox[4 + 4 * j, y0:y0 + k, 20] = 7
+    return vox
+
+
+LEAF_A, BLADE_A = 204, 178   # unorm8(0.8), unorm8(0.7): gbNormal.a of a leaf voxel and of a grass blade
+
+
+def test_back_lit_leaf_closed_form():
+    """A G-buffer texel on the underside of a slab of k leaf voxels, the sun straight up: the texel faces
+    away from the sun, so only FOLIAGE_SSS traces.  A leaf voxel's ray starts at its centre stepped 0.9
+    toward the sun (foliageSunOrigin: P - N*0.5 + sun*0.9), inside the second layer, so it crosses k - 1
+    leaves: visibility leafLight^(k-1), 0 from the sixth leaf on; a blade starts just off the surface
+    (P + sun*0.05), inside the first layer: leafLight^k.  No blocker distance is stored.  An opaque
+    texel (alpha 255) and a sun below the horizon get no ray."""
+    ks = [1, 2, 3, 4, 5, 6, 7]
+    _, _, ow = dense_world(_leaf_slabs_world(ks).transpose(2, 1, 0))
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    down_n = (0.0, -1.0, 0.0)
+    cases = []   # (P, normal bytes, expected visibility)
+    for j, k in enumerate(ks):
+        P = (4 + 4 * j + 0.5, 20.0, 20.5)
+        m = k - 1
+        cases.append((P, _nbytes(down_n, LEAF_A), float(np.float32(0.6) ** m) if m <= 5 else 0.0))
+        cases.append((P, _nbytes(down_n, BLADE_A), float(np.float32(0.6) ** k) if k <= 5 else 0.0))
+        cases.append((P, _nbytes(down_n, 255), 0.0))
+    n = len(cases)
+    pos = np.array([[*c[0], 5.0] for c in cases], np.float32)
+    nrm = np.array([c[1] for c in cases], np.uint32)
+    sh, _, cnt = ow.pass_shadow(up, n, 1, pos, nrm, leaf_light=0.6)
+    for i, c in enumerate(cases):
+        assert sh[i, 0] == pytest.approx(c[2], rel=1e-6), (i, c, sh[i])
+        assert sh[i, 1] == 0.0
+    assert cnt["rays"] == 2 * len(ks)   # one ray per foliage texel, however many leaves it crosses
+    # a lone leaf voxel's side face, the sun grazing it from behind: a ray (nothing beyond: lit) only while the
+    # sun is above the horizon
+    _, _, ow = dense_world(_leaf_slabs_world([1]).transpose(2, 1, 0))
+    side = _nbytes((-1.0, 0.0, 0.0), LEAF_A)
+    P = np.array([[4.0, 20.5, 20.5, 5.0]], np.float32)
+    for sy, want in ((-0.1, 0.0), (0.1, 1.0)):
+        sun = np.array([1.0, sy, 0.0], np.float32)
+        sun /= np.linalg.norm(sun)
+        sh, _, cnt = ow.pass_shadow(sun, 1, 1, P, np.array([side], np.uint32), leaf_light=0.6)
+        assert sh[0, 0] == want and cnt["rays"] == (1 if sy > 0 else 0), (sy, sh)
+
+
+def test_back_lit_leaf_mistakes_are_caught():
+    """The closed form separates foliageSunOrigin from its plausible misreadings: the lit-face origin
+    P + N*0.03 or a flipped N*0.5 start below or inside the first layer and cross all k leaves."""
+    k = 3
+    _, _, ow = dense_world(_leaf_slabs_world([k]).transpose(2, 1, 0))
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    P = np.array([4.5, 20.0, 20.5], np.float32)
+    N = np.array([128 / 255 * 2 - 1, -1.0, 128 / 255 * 2 - 1], np.float32)
+    right = ow.sun_visibility((P - N * np.float32(0.5)) + up * np.float32(0.9), up, 0.6)[0]
+    assert right == pytest.approx(0.36, rel=1e-6)
+    for wrong_o in (P + N * np.float32(0.03), (P + N * np.float32(0.5)) + up * np.float32(0.9)):
+        assert ow.sun_visibility(wrong_o, up, 0.6)[0] == pytest.approx(0.216, rel=1e-6)
+
+
+def test_back_lit_leaf_sun_cone_mean():
+    """lighting.comp's stochastic sample from the underside of a wide slab of 3 leaf voxels: each leaf
+    passes the ray with probability 0.6, so the mean over many texels is 0.6^2 (the origin is in the
+    second layer); texels that are not foliage stay 0."""
+    vox = np.zeros((64, 64, 64), np.uint8)
+    vox[:, 0:8, :] = 3
+    vox[8:56, 20:23, 8:56] = 7
+    _, _, ow = dense_world(vox.transpose(2, 1, 0))
+    sun = np.array([0.05, 1.0, 0.03], np.float32)
+    sun /= np.linalg.norm(sun)
+    rng = np.random.default_rng(3)
+    W, H = 64, 64
+    xz = rng.uniform(16, 48, (W * H, 2)).astype(np.float32)
+    pos = np.stack([xz[:, 0], np.full(W * H, 20.0, np.float32), xz[:, 1], np.full(W * H, 5.0, np.float32)], 1)
+    nrm = np.full(W * H, _nbytes((0.0, -1.0, 0.0), LEAF_A), np.uint32)
+    out, _ = ow.pass_sun_cone(sun, 0.02, 0.6, 7, W, H, pos, nrm)
+    assert set(np.unique(out)) <= {0.0, 1.0}
+    assert abs(out.mean() - 0.36) < 0.03
+    nrm[:] = _nbytes((0.0, -1.0, 0.0), 255)
+    out, _ = ow.pass_sun_cone(sun, 0.02, 0.6, 7, W, H, pos, nrm)
+    assert not out.any()
+
+
+def test_gbuffer_alpha_marks_foliage():
+    """gbuffer.comp:526 (WATER_CLEAR on): gbNormal.a = 0.8 (byte 204) on a hit whose material is palette
+    kind 3, 1.0 (255) on every other hit, 0 on the sky."""
+    _, a, ow = world(1)
+    pu, cam = config1_camera()
+    hits, pos, nrm, _ = ow.pass_primary(cam, 160, 90)
+    kinds = ow.palette[hits["mat"], 6]
+    alpha = nrm >> 24
+    hit = hits["hit"] == 1
+    assert (alpha[~hit] == 0).all()
+    assert (alpha[hit & (kinds == 3)] == LEAF_A).all() and (alpha[hit & (kinds != 3)] == 255).all()
+    vox = _leaf_slabs_world([2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2])
+    _, _, ow2 = dense_world(vox.transpose(2, 1, 0))
+    cam2 = {"pos": [30.5, 40.0, 20.7], "right": [1.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0], "fwd": [0.0, -1.0, 0.0],
+            "tan_half_fov": 0.7}
+    h2, _, n2, _ = ow2.pass_primary(cam2, 64, 64)
+    leaf = (h2["hit"] == 1) & (h2["mat"] == 7)
+    assert leaf.sum() > 20 and ((n2 >> 24)[leaf] == LEAF_A).all()
+    assert ((n2 >> 24)[(h2["hit"] == 1) & (h2["mat"] == 3)] == 255).all()

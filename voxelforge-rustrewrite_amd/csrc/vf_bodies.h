@@ -44,9 +44,10 @@ __device__ __forceinline__ void hit_normal(const DevBodies *B, const Hit &h, flo
     else face_normal(h.face, n);
 }
 
-// RGBA8 unorm of n * 0.5 + 0.5 (a = 255); NaN components store 0 (Vulkan's float -> unorm rule).
-__device__ __forceinline__ uint32_t pack_normal(const float n[3]) {
-    uint32_t v = 255u << 24;
+// RGBA8 unorm of n * 0.5 + 0.5 with alpha byte `a` (gbuffer.comp's matFlag); NaN components store 0 (Vulkan's
+// float -> unorm rule).
+__device__ __forceinline__ uint32_t pack_normal(const float n[3], uint32_t a = 255u) {
+    uint32_t v = a << 24;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float x = n[a] * 0.5f + 0.5f;

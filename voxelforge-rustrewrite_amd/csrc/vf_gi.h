@@ -245,6 +245,24 @@ __device__ __forceinline__ void sample_cone(const float dir[3], float ang, uint3
     normalize3(out);
 }
 
+// gbNormal.a (gbuffer.comp:524-526, WATER_CLEAR on): 0.8 for a foliage material (palette kind 3), else 1;
+// stored as RGBA8 unorm, so 204 or 255.
+constexpr uint32_t kFoliageAlpha = 204u;   // unorm8(0.8f)
+__device__ __forceinline__ uint32_t mat_flag_byte(const DevWorld &W, uint32_t mat) {
+    return W.palette[mat * 3u + 1u].z == 3.0f ? kFoliageAlpha : 255u;
+}
+// isFoliageFlag / isBladeFlag (common.glsl:1953-1954) of a gbNormal texel's alpha as the sampler returns it.
+__device__ __forceinline__ float gbn_alpha(uint32_t nq) { return (float)(nq >> 24) / 255.0f; }
+__device__ __forceinline__ bool foliage_flag(float a) { return a > 0.62f && a < 0.88f; }
+__device__ __forceinline__ bool blade_flag(float a) { return a > 0.62f && a < 0.75f; }
+// foliageSunOrigin (common.glsl:1987-1989): a leaf voxel is lit through itself from its centre stepped out
+// toward the sun; a blade from just off its surface.
+__device__ __forceinline__ void foliage_sun_origin(const float P[3], const float N[3], const float sun[3], bool blade,
+                                                   float o[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) o[a] = blade ? P[a] + sun[a] * 0.05f : (P[a] - N[a] * 0.5f) + sun[a] * 0.9f;
+}
+
 // sunVisibilityT (common.glsl:1964-1981) over the world (and the bodies: traceSceneShadow): a leaf voxel (palette kind 3) lets the light
 // through with probability (STOCHASTIC) or fraction leaf_light, and the ray goes on from where it
 // leaves that voxel; up to 6 leaves.  tfirst = distance to the first blocker (0 = none).  The far

@@ -105,7 +105,7 @@ __device__ __forceinline__ void primary_pixel(const DevWorld &W, const PrimArgs 
             float n[3];
             face_normal(h.face, n);
             v = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
-                (255u << 24);
+                (mat_flag_byte(W, h.mat) << 24);
         }
         gbn[i] = v;
     }
@@ -144,11 +144,12 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
     store_hit(out, m);
 }
 
-// shadow_trace.comp's result for one G-buffer texel (shadow_trace.comp:21-34, 87, 96-105): sky -> lit,
-// facing away from the sun -> shadowed without a ray, else sunVisibilityT (FOLIAGE_LIGHT,
-// deterministic) from P + N*0.03: visibility (leaves let leaf of the light through) and the distance
-// to the first blocker.
-// NDEC: N holds the decoded normal already (nq is not read).
+// shadow_trace.comp's result for one G-buffer texel (shadow_trace.comp:21-49, 87, 96-105): sky -> lit;
+// facing away from the sun -> shadowed without a ray, unless the texel is foliage (gbNormal.a) and the sun
+// is above the horizon: then FOLIAGE_SSS traces sunVisibility from foliageSunOrigin (the light through the
+// leaf, no blocker distance); else sunVisibilityT (FOLIAGE_LIGHT, deterministic) from P + N*0.03:
+// visibility (leaves let leaf of the light through) and the distance to the first blocker.
+// NDEC: N holds the decoded normal already (only nq's alpha is read, which is then 204 or 255).
 template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false, bool NDEC = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr,
@@ -161,7 +162,16 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         const float N[3] = {NDEC ? Ndec[0] : unorm8_dec(nq & 255u), NDEC ? Ndec[1] : unorm8_dec((nq >> 8) & 255u),
                             NDEC ? Ndec[2] : unorm8_dec((nq >> 16) & 255u)};
         if (dot3(N, sun) <= 0.0f) {
-            if (hits) store_miss(hits + i);
+            const float a = NDEC ? ((nq >> 24) == kFoliageAlpha ? 0.8f : 1.0f) : gbn_alpha(nq);
+            if (foliage_flag(a) && sun[1] > 0.0f) {   // FOLIAGE_SSS (shadow_trace.comp:35-46): back-lit foliage
+                const float P[3] = {p.x, p.y, p.z};
+                float so[3], tf;
+                foliage_sun_origin(P, N, sun, blade_flag(a), so);
+                uint32_t seed = 0u;
+                vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE>(W, so, sun, leaf, seed, tf, hits, i, c, B, pinv);
+            } else if (hits) {
+                store_miss(hits + i);
+            }
         } else {
             const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             uint32_t seed = 0u;
@@ -218,7 +228,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
         float n[3];
         face_normal(h.face, n);
         nq = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
-             (255u << 24);
+             (mat_flag_byte(W, h.mat) << 24);
 #pragma unroll
         for (int a = 0; a < 3; ++a) Nd[a] = n[a] > 0.0f ? 1.0f : (n[a] < 0.0f ? -1.0f : kDec128);
     } else {
@@ -298,7 +308,7 @@ __global__ void __launch_bounds__(256) k_scene(DevWorld W, DevBodies B, PrimArgs
         p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
         float n[3];
         hit_normal<true>(&B, h, n);
-        nq = pack_normal(n);
+        nq = pack_normal(n, mat_flag_byte(W, h.mat));
     } else {
         p = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     }
@@ -370,7 +380,8 @@ __global__ void __launch_bounds__(256) k_gi(DevWorld W, SkyParams S, DevBodies B
 
 // lighting.comp's sun ray (SVGF, lighting.comp:44-84): per pixel seed pixelSeed(pix, frameIdx, 0), a
 // lit texel (N.L > 0) traces sunVisibility (FOLIAGE_LIGHT, stochastic) from P + N*0.03 along
-// sampleCone(sunDir, sunAngle); the raw sample is 1 for sky, 0 for a texel facing away, else 0 or 1.
+// sampleCone(sunDir, sunAngle), a back-lit foliage texel (FOLIAGE_SSS, sun above the horizon) from
+// foliageSunOrigin; the raw sample is 1 for sky, 0 for another texel facing away, else 0 or 1.
 template <bool COUNT, bool BODIES>
 __global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, DevBodies B, TileArgs A, float sx, float sy, float sz, float ang,
                                                   uint32_t frame, const float4 *gbpos, const uint32_t *gbn, float *out,
@@ -392,10 +403,20 @@ __global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, DevBodies B, TileA
         const float sun[3] = {sx, sy, sz};
         uint32_t seed = pixel_seed(px, py, frame, 0u);
         v = 0.0f;
-        if (gmax(dot3(N, sun), 0.0f) > 0.0f) {
+        const bool lit = gmax(dot3(N, sun), 0.0f) > 0.0f;
+        bool ray = lit;
+        float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+        if (!lit) {   // FOLIAGE_SSS (lighting.comp:66-71): back-lit foliage traces from foliageSunOrigin
+            const float a = gbn_alpha(nq);
+            if (foliage_flag(a) && sun[1] > 0.0f) {
+                const float P[3] = {p.x, p.y, p.z};
+                foliage_sun_origin(P, N, sun, blade_flag(a), so);
+                ray = true;
+            }
+        }
+        if (ray) {
             float sd[3], tf;
             sample_cone(sun, ang, seed, sd);
-            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
             v = sun_visibility_t<true, COUNT, BODIES, true>(W, so, sd, A.leaf, seed, tf, hits, i, c, &B);
             stored = true;
         }
