@@ -104,6 +104,30 @@ typedef struct vf_pass_ubo {
     float facet_p[4];
 } vf_pass_ubo;
 
+/* The layout asserts of the reference (src/gfx/ubo.rs:76-86, 132-139; src/voxel/world.rs:31), checked by
+ * every C or C++ compiler that includes this header. */
+#ifdef __cplusplus
+#define VF_LAYOUT_ASSERT(c, m) static_assert(c, m)
+#else
+#define VF_LAYOUT_ASSERT(c, m) _Static_assert(c, m)
+#endif
+VF_LAYOUT_ASSERT(sizeof(vf_material) == 48, "Material is 48 B");
+VF_LAYOUT_ASSERT(sizeof(vf_scene_ubo) == 1536, "SceneUbo is 1536 B");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, fog_height) == 144, "SceneUbo.fog_height @144");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, sh_win_a) == 240, "SceneUbo.sh_win_a @240");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, probe_origin) == 272, "SceneUbo.probe_origin @272");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, world_win) == 352, "SceneUbo.world_win @352");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, chunk_fade) == 400, "SceneUbo.chunk_fade @400");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, gi_cache_cam) == 1424, "SceneUbo.gi_cache_cam @1424");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, hor_c) == 1472, "SceneUbo.hor_c @1472");
+VF_LAYOUT_ASSERT(offsetof(vf_scene_ubo, rain_a) == 1520, "SceneUbo.rain_a @1520");
+VF_LAYOUT_ASSERT(sizeof(vf_pass_ubo) == 288, "PassUbo is 288 B");
+VF_LAYOUT_ASSERT(offsetof(vf_pass_ubo, prev_tan_half_fov) == 76, "PassUbo.prev_tan_half_fov @76");
+VF_LAYOUT_ASSERT(offsetof(vf_pass_ubo, prev_cam_fwd) == 112, "PassUbo.prev_cam_fwd @112");
+VF_LAYOUT_ASSERT(offsetof(vf_pass_ubo, frame_idx) == 124, "PassUbo.frame_idx @124");
+VF_LAYOUT_ASSERT(offsetof(vf_pass_ubo, jitter) == 152, "PassUbo.jitter @152");
+VF_LAYOUT_ASSERT(offsetof(vf_pass_ubo, facet_p) == 272, "PassUbo.facet_p @272");
+
 /* Explicit ray, 24 B: origin and direction (voxel units; direction need not be normalised). */
 typedef struct vf_ray { float o[3]; float d[3]; } vf_ray;
 

@@ -128,3 +128,15 @@ def test_brick_arguments_packed_and_as_records():
         vf.Context._brick_args(rec, [0, 0, 0, 0])    # records carry their own slots
     with pytest.raises(ValueError):
         vf.Context._brick_args(packed, [0, 0])       # one slot per brick
+
+
+def test_buffer_checks_refuse_host_and_undersized_tensors():
+    """The Python host checks device buffers before a call: a host tensor, a non-contiguous one or one
+    smaller than the pass writes is a ValueError, never a device write out of bounds."""
+    import torch
+    from vftrace import _dev
+    assert _dev(None, 100, "x") is None
+    with pytest.raises(ValueError, match="device"):
+        _dev(torch.zeros(64, dtype=torch.uint8), 16, "hits")
+    with pytest.raises(ValueError, match="device"):
+        _dev(np.zeros(64, np.uint8), 16, "hits")

@@ -166,3 +166,66 @@ def test_autotuner_picks_a_measured_config(ctx):
     fastest = min(table, key=lambda r: r[3])
     assert (best.tile_w, best.tile_h) == fastest[0] and best.chunk == fastest[1] and bool(best.sort) == fastest[2]
     assert all(r[3] > 0 for r in table)
+
+
+def test_full_config5_batch_matches_plain_launches_and_oracle(ctx):
+    """The uneven batch at its full size: 16 seeded views drawn from {256^2, 512x288, 960x540, 1024^2} and a
+    32x16x32 probe grid x 256 spherical-Fibonacci rays (about 13M rays) in one launch, in chunk mode and in
+    refill mode.  Every request's hit records equal its own plain launch byte for byte, and every 97th ray of
+    every request equals the oracle's traceVolumeS on the same ray."""
+    import math
+    _, a, ow = world(1)
+    rng = np.random.default_rng(1337)
+    sizes = [(256, 256), (512, 288), (960, 540), (1024, 1024)]
+    views = []
+    for _ in range(16):
+        vw, vh = sizes[int(rng.integers(0, 4))]
+        pos = [float(rng.uniform(20, 236)), float(rng.uniform(75, 120)), float(rng.uniform(20, 236))]
+        vp, vr, vu, vfw = vf.fly_camera(pos, float(rng.uniform(0, 2 * math.pi)), float(rng.uniform(-0.7, 0.1)))
+        assert ow.voxel(*[int(math.floor(c)) for c in vp]) == 0   # cameras in air: rays start at the camera
+        views.append((vw, vh, vf.make_pass(vp, vr, vu, vfw, 70.0, 0), (vp, vr, vu, vfw)))
+    views.sort(key=lambda v: v[0] * v[1])
+    assert {(v[0], v[1]) for v in views} >= {(1024, 1024)}
+    dims, nr, frame, reach, spacing = (32, 16, 32), 256, 7, 256.0, 20.0
+    origin = (128.0 - 310.0 + 0.5, 64.0 - 150.0 + 0.5, 128.0 - 310.0 + 0.5)
+    n_probe = int(np.prod(dims)) * nr
+    outs = [dev_hits(vw * vh) for vw, vh, _, _ in views] + [dev_hits(n_probe)]
+    reqs = [ctx.view_request(vp, vw, vh, o) for (vw, vh, vp, _), o in zip(views, outs)]
+    reqs.append(ctx.probe_request(origin, spacing, dims, nr, frame, reach, outs[-1]))
+    ref = [dev_hits(vw * vh) for vw, vh, _, _ in views] + [dev_hits(n_probe)]
+    for (vw, vh, vp, _), r in zip(views, ref):
+        ctx.trace_primary(vp, vw, vh, r)
+    ctx.trace_probes(origin, spacing, dims, nr, frame, reach, ref[-1])
+    ctx.sync()
+    for cfg in (ctx.queue_cfg((16, 8), 512, refill=0), ctx.queue_cfg((8, 8), 256, refill=32)):
+        for o in outs:
+            o.zero_()
+        ctx.trace_batch(reqs, cfg)
+        ctx.sync()
+        times = ctx.batch_times(len(reqs))
+        assert all(t > 0 for t in times), times
+        for k, (o, r) in enumerate(zip(outs, ref)):
+            assert torch.equal(o, r), (k, cfg.refill)
+    # every 97th ray of each request against the oracle
+    L = O.lib()
+    jit = np.zeros(2, np.float32)
+    tmp = np.zeros(3, np.float32)
+    for (vw, vh, pu, (vp, vr, vu, vfw)), o in zip(views, outs):
+        idx = np.arange(0, vw * vh, 97)
+        rays = np.zeros((idx.size, 6), np.float32)
+        R, U, F = (np.ascontiguousarray(x, np.float32) for x in (vr, vu, vfw))
+        for j, i in enumerate(idx):
+            L.vfo_primary_dir(R.ctypes.data, U.ctypes.data, F.ctypes.data, float(pu.tan_half_fov),
+                              vw, vh, int(i % vw), int(i // vw), jit.ctypes.data, tmp.ctypes.data)
+            rays[j, :3], rays[j, 3:] = vp, tmp
+        oh, _ = ow.trace_rays(rays, 1e30, 0)
+        g = vf.hits_to_numpy(o)[idx]
+        assert hit_diff(g, oh).size == 0, (vw, vh)
+    dirs = O.probe_dirs(nr, frame)
+    idx = np.arange(0, n_probe, 97)
+    p, i = idx // nr, idx % nr
+    pc = np.stack([p % dims[0], (p // dims[0]) % dims[1], p // (dims[0] * dims[1])], 1).astype(np.float32)
+    o3 = np.array(origin, np.float32)[None, :] + np.float32(spacing) * pc
+    oh, _ = ow.trace_rays(np.concatenate([o3, dirs[i]], 1).astype(np.float32), reach, 5)
+    assert hit_diff(vf.hits_to_numpy(outs[-1])[idx], oh).size == 0
+    assert 0.05 < oh["hit"].mean() < 0.95

@@ -273,6 +273,125 @@ def test_cosine_hemisphere_distribution():
     assert abs(cosines.mean() - 2.0 / 3.0) < 0.02     # E[cos] of a cosine-weighted hemisphere
 
 
+def _ld(px, py, k, salt):
+    out = np.zeros(2, np.float32)
+    O.lib().vfo_ld_sample2(int(px), int(py), int(k) & 0xFFFFFFFF, int(salt) & 0xFFFFFFFF, out.ctypes.data)
+    return out
+
+
+def _r2_constants():
+    """Roberts' R2 sequence step (1/g, 1/g^2), g the real root of x^3 = x + 1 (the plastic number), as float32."""
+    g = max(r.real for r in np.roots([1.0, 0.0, -1.0, -1.0]) if abs(r.imag) < 1e-12)
+    assert abs(g ** 3 - g - 1.0) < 1e-14
+    return np.array([1.0 / g, 1.0 / (g * g)], np.float32)
+
+
+def _r2_sequence(u0, ks, alpha):
+    x = u0[None, :] + (ks & 0xFFFF).astype(np.float32)[:, None] * alpha[None, :]   # float32 product, then sum
+    return (x - np.floor(x)).astype(np.float32)                                     # GLSL fract
+
+
+def test_ld_sample2_is_the_r2_sequence():
+    """ldSample2 (common.glsl:1284-1288): sample 0 of a pixel is its rotation, a multiple of 2^-24 per axis; sample
+    k is fract(rotation + float(k & 0xFFFF) * (1/g, 1/g^2)) bit for bit in float32, with g the plastic number
+    computed here from its defining cubic; k wraps at 2^16.  A constant one float32 ulp off fails."""
+    alpha = _r2_constants()
+    ks = np.arange(1, 3000, dtype=np.int64)
+    for px, py, salt in ((0, 0, 0), (17, 931, 0x68e31da4), (1919, 1079, 0x5EED), (5, 7, 123456789)):
+        u0 = _ld(px, py, 0, salt)
+        assert np.all(u0 * np.float32(16777216.0) == np.floor(u0 * np.float32(16777216.0)))
+        got = np.array([_ld(px, py, k, salt) for k in ks])
+        assert np.array_equal(got, _r2_sequence(u0, ks, alpha)), (px, py)
+        assert np.array_equal(_ld(px, py, 65536 + 77, salt), got[76])
+        for ax in (0, 1):
+            bad = alpha.copy()
+            bad[ax] = np.nextafter(bad[ax], np.float32(1.0))
+            assert not np.array_equal(got, _r2_sequence(u0, ks, bad))
+
+
+def _fibonacci_set(n, golden):
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - (2.0 * i + 1.0) / n
+    rr = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    ph = i * golden
+    return np.stack([np.cos(ph) * rr, z, np.sin(ph) * rr], 1)
+
+
+def test_probe_dirs_are_the_rotated_fibonacci_set():
+    """probeRayDir (probes.glsl:155-170): direction i of n is the spherical-Fibonacci point z = 1 - (2i+1)/n,
+    phi = i * golden angle (pi (3 - sqrt 5)), turned by one rotation per frame; so every pair's dot product
+    equals that of the closed-form set, index for index.  A one-digit slip in the golden angle or the
+    z spacing fails."""
+    golden = np.pi * (3.0 - np.sqrt(5.0))
+    # the float32 angle i * golden is off by up to ~ulp(i * golden): 6e-6 at n = 64, 2.5e-5 at n = 256
+    for n, tol in ((64, 1e-5), (256, 4e-5)):
+        b = _fibonacci_set(n, golden)
+        for frame in (0, 3, 11):
+            d = O.probe_dirs(n, frame).astype(np.float64)
+            assert np.abs(d @ d.T - b @ b.T).max() < tol, (n, frame)
+        wrong_golden = _fibonacci_set(n, 2.39996423)
+        assert np.abs(d @ d.T - wrong_golden @ wrong_golden.T).max() > 2 * tol
+        i = np.arange(n)
+        wz = np.stack([np.cos(i * golden) * np.sqrt(1 - (1 - 2 * i / n) ** 2), 1 - 2 * i / n,
+                       np.sin(i * golden) * np.sqrt(1 - (1 - 2 * i / n) ** 2)], 1)
+        assert np.abs(d @ d.T - wz @ wz.T).max() > 1e-3
+
+
+def _sky_closed_form(scene, rd, model, glow_exp=8.0, ph_exp=0.7, sunset_k=9.0):
+    rd = np.asarray(rd, np.float64)
+    sun = np.asarray(scene.sun_dir, np.float64)
+    h = min(max(rd[1] * 0.5 + 0.5, 0.0), 1.0)
+    ph = h ** ph_exp
+    mu = max(float(rd @ sun), 0.0)
+    glow = mu ** glow_exp
+    tint = np.array([1.0, 0.9, 0.7])
+    if model == 0:
+        sky = np.array([0.75, 0.82, 0.90]) * (1 - ph) + np.array([0.25, 0.45, 0.85]) * ph
+        out = sky + tint * (glow * 0.25)
+    else:
+        sky = np.array(scene.sky_horizon, np.float64) * (1 - ph) + np.array(scene.sky_zenith, np.float64) * ph
+        sky = sky + np.array(scene.sunset_rgb, np.float64) * (mu ** 3 * np.exp(-max(rd[1], 0.0) * 5.0) +
+                                                             0.25 * np.exp(-abs(rd[1]) * sunset_k))
+        srgb = np.array(scene.sun_rgb, np.float64)
+        out = sky + srgb / max(srgb[0], srgb[1], 1e-3) * tint * (glow * 0.25)
+    return np.minimum(out * scene.sky_scale, 3.0)
+
+
+def test_sky_color_closed_forms():
+    """skyColor(rd, 0) (common.glsl:1082-1099) as a GI ray's escape radiance in an empty world (giEscape 1,
+    clamped to 3): both sky models against their closed forms at the zenith, near the horizon, toward the sun
+    and around it; a changed exponent (glow 8 -> 16, horizon ramp 0.7 -> 0.8, sunset falloff 9 -> 8) fails."""
+    vox = np.zeros((64, 64, 64), np.uint8)
+    _, a, ow = dense_world(vox)
+    P = np.array([30.5, 10.0, 30.5], np.float32)
+    N = np.array([0.0, 1.0, 0.0], np.float32)
+    L = np.zeros(3, np.float32)
+    rng = np.random.default_rng(2)
+    for model in (0, 1):
+        scene = vf.default_scene((8, 8, 8), sky_model=model)
+        sun = np.array(scene.sun_dir, np.float32)
+        dirs = [np.array([0.0, 1.0, 0.0]), sun, np.array([0.6, 0.03, -0.8]), np.array([-0.99, 0.02, 0.1])]
+        for _ in range(12):
+            v = rng.normal(size=3)
+            v[1] = abs(v[1]) + 0.01
+            dirs.append(v)
+        t = sun + np.float32(0.2) * np.array([sun[2], 0.0, -sun[0]], np.float32)
+        dirs.append(t)
+        s = ow.sky(scene)
+        worst = {"glow": 0.0, "ph": 0.0, "sunset": 0.0}
+        for d in dirs:
+            d = (np.asarray(d, np.float64) / np.linalg.norm(d)).astype(np.float32)
+            O.lib().vfo_gi_path(ctypes.byref(ow.s), ctypes.byref(s), P.ctypes.data, N.ctypes.data, d.ctypes.data,
+                                L.ctypes.data, None, None)
+            want = _sky_closed_form(scene, d, model)
+            assert np.allclose(L, want, rtol=2e-5, atol=1e-6), (model, d, L, want)
+            for k, kw in (("glow", {"glow_exp": 16.0}), ("ph", {"ph_exp": 0.8}), ("sunset", {"sunset_k": 8.0})):
+                worst[k] = max(worst[k], float(np.abs(L - _sky_closed_form(scene, d, model, **kw)).max()))
+        assert worst["glow"] > 1e-2 and worst["ph"] > 1e-2, worst
+        if model == 1:
+            assert worst["sunset"] > 1e-3, worst
+
+
 def test_probe_dirs_spherical_fibonacci():
     d = O.probe_dirs(256, 3)
     assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-5)

@@ -155,6 +155,26 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _dev(t, nbytes, what):
+    """Device pointer of a torch tensor that is on the GPU, contiguous and holds at least nbytes (None
+    passes through): an undersized or host buffer is a ValueError here, not an out-of-bounds device
+    write in the kernel."""
+    if t is None:
+        return None
+    if not hasattr(t, "data_ptr") or not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what}: expected a device (cuda) tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: tensor is not contiguous")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise ValueError(f"{what}: {have} bytes, {int(nbytes)} needed")
+    return C.c_void_p(t.data_ptr())
+
+
+def _half(n):
+    return (int(n) + 1) // 2
+
+
 def _i3(v):
     return (C.c_int32 * 3)(*[int(x) for x in v])
 
@@ -568,71 +588,102 @@ class Context:
 
     # passes (device buffers: torch tensors)
     def trace_primary(self, pass_ubo, width, height, hits, gb_pos=None, gb_normal=None):
-        self._ck(lib().vf_trace_primary(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
-                                        _ptr(gb_pos), _ptr(gb_normal)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_trace_primary(self._h, C.byref(pass_ubo), int(width), int(height), _dev(hits, 20 * n, "hits"),
+                                        _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal")))
 
     def trace_gbuffer(self, pass_ubo, width, height, hits, gb_pos=None, gb_normal=None, body=None, shadow_raw=None,
                       shadow_hits=None):
         """P::Gbuf over the scene (+ P::ShTrace when shadow_raw is given) with the body index per pixel."""
-        self._ck(lib().vf_trace_gbuffer(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits), _ptr(gb_pos),
-                                        _ptr(gb_normal), _ptr(body), _ptr(shadow_raw), _ptr(shadow_hits)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_trace_gbuffer(self._h, C.byref(pass_ubo), int(width), int(height), _dev(hits, 20 * n, "hits"),
+                                        _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal"),
+                                        _dev(body, 4 * n, "body"), _dev(shadow_raw, 4 * n, "shadow_raw"),
+                                        _dev(shadow_hits, 20 * n, "shadow_hits")))
 
     def trace_shadow(self, width, height, gb_pos, gb_normal, shadow_raw, hits=None, pass_ubo=None):
         """shadow_trace.comp; the pass supplies look_p[3] (r.foliage.light, the reference default 0.6)."""
         p = pass_ubo if pass_ubo is not None else _DEFAULT_PASS
-        self._ck(lib().vf_trace_shadow(self._h, C.byref(p), int(width), int(height), _ptr(gb_pos), _ptr(gb_normal),
-                                       _ptr(shadow_raw), _ptr(hits)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_trace_shadow(self._h, C.byref(p), int(width), int(height), _dev(gb_pos, 16 * n, "gb_pos"),
+                                       _dev(gb_normal, 4 * n, "gb_normal"), _dev(shadow_raw, 4 * n, "shadow_raw"),
+                                       _dev(hits, 20 * n, "hits")))
 
     def trace_sun_cone(self, pass_ubo, width, height, gb_pos, gb_normal, shadow, hits=None):
         """lighting.comp's cone-jittered sun ray (float per pixel)."""
-        self._ck(lib().vf_trace_sun_cone(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
-                                         _ptr(gb_normal), _ptr(shadow), _ptr(hits)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_trace_sun_cone(self._h, C.byref(pass_ubo), int(width), int(height),
+                                         _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal"),
+                                         _dev(shadow, 4 * n, "shadow"), _dev(hits, 20 * n, "hits")))
 
     def trace_primary_shadow(self, pass_ubo, width, height, hits, gb_pos, gb_normal, shadow_raw, shadow_hits=None):
-        self._ck(lib().vf_trace_primary_shadow(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(hits),
-                                               _ptr(gb_pos), _ptr(gb_normal), _ptr(shadow_raw), _ptr(shadow_hits)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_trace_primary_shadow(self._h, C.byref(pass_ubo), int(width), int(height),
+                                               _dev(hits, 20 * n, "hits"), _dev(gb_pos, 16 * n, "gb_pos"),
+                                               _dev(gb_normal, 4 * n, "gb_normal"),
+                                               _dev(shadow_raw, 4 * n, "shadow_raw"),
+                                               _dev(shadow_hits, 20 * n, "shadow_hits")))
 
     def trace_gi(self, pass_ubo, width, height, gb_pos, gb_normal, radiance, bounce=None, half_res=True,
                  gi_reach=48.0, lights_nee=True):
+        n = int(width) * int(height)
+        no = _half(width) * _half(height) if half_res else n
         self._ck(lib().vf_trace_gi(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(half_res)),
-                                   float(gi_reach), int(bool(lights_nee)), _ptr(gb_pos), _ptr(gb_normal),
-                                   _ptr(radiance), _ptr(bounce)))
+                                   float(gi_reach), int(bool(lights_nee)), _dev(gb_pos, 16 * n, "gb_pos"),
+                                   _dev(gb_normal, 4 * n, "gb_normal"), _dev(radiance, 16 * no, "radiance"),
+                                   _dev(bounce, 20 * no, "bounce")))
 
     # temporal consumers (include/vf_temporal.h)
     def gbuffer_motion(self, pass_ubo, width, height, gb_pos, gb_motion, gb_delta, body=None):
         """gbuffer.comp's gbMotion (RG16F bits per pixel, int32 tensor) and gbDelta (RGBA16F bits)."""
-        self._ck(lib().vf_gbuffer_motion(self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
-                                         _ptr(body), _ptr(gb_motion), _ptr(gb_delta)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_gbuffer_motion(self._h, C.byref(pass_ubo), int(width), int(height),
+                                         _dev(gb_pos, 16 * n, "gb_pos"), _dev(body, 4 * n, "body"),
+                                         _dev(gb_motion, 4 * n, "gb_motion"), _dev(gb_delta, 8 * n, "gb_delta")))
 
     def gi_accumulate(self, pass_ubo, width, height, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
                       gb_delta, indirect, hist_prev, mom_prev, hist_cur, mom_cur, guide, bicubic=True):
         """gi.comp's reprojected accumulation of the traced half-res GI (RGBA16F history as int16/uint16
         tensors of half-float bits, RGBA32F moments, RGBA32UI guide)."""
+        n, nh = int(width) * int(height), _half(width) * _half(height)
+        sizes = (("gb_pos", 16 * n), ("gb_normal", 4 * n), ("gb_pos_prev", 16 * n), ("gb_normal_prev", 4 * n),
+                 ("gb_motion", 4 * n), ("gb_delta", 8 * n), ("indirect", 16 * nh), ("hist_prev", 8 * nh),
+                 ("mom_prev", 16 * nh), ("hist_cur", 8 * nh), ("mom_cur", 16 * nh), ("guide", 16 * nh))
+        ts = (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion, gb_delta, indirect, hist_prev, mom_prev,
+              hist_cur, mom_cur, guide)
         self._ck(lib().vf_gi_accumulate(self._h, C.byref(pass_ubo), int(width), int(height), int(bool(bicubic)),
-                                        *[_ptr(t) for t in (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
-                                                            gb_delta, indirect, hist_prev, mom_prev, hist_cur, mom_cur,
-                                                            guide)]))
+                                        *[_dev(t, b, w) for t, (w, b) in zip(ts, sizes)]))
 
     def light_accumulate(self, pass_ubo, width, height, gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion,
                          gb_delta, shadow, hist_prev, hist_cur):
         """lighting.comp's sun-shadow history (RGBA32F per pixel)."""
+        n = int(width) * int(height)
+        sizes = (("gb_pos", 16 * n), ("gb_normal", 4 * n), ("gb_pos_prev", 16 * n), ("gb_normal_prev", 4 * n),
+                 ("gb_motion", 4 * n), ("gb_delta", 8 * n), ("shadow", 4 * n), ("hist_prev", 16 * n),
+                 ("hist_cur", 16 * n))
+        ts = (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev, gb_motion, gb_delta, shadow, hist_prev, hist_cur)
         self._ck(lib().vf_light_accumulate(self._h, C.byref(pass_ubo), int(width), int(height),
-                                           *[_ptr(t) for t in (gb_pos, gb_normal, gb_pos_prev, gb_normal_prev,
-                                                               gb_motion, gb_delta, shadow, hist_prev, hist_cur)]))
+                                           *[_dev(t, b, w) for t, (w, b) in zip(ts, sizes)]))
 
     def gi_atrous(self, width, height, levels, gb_pos, gb_normal, gi_mom, gi_hist, guide, filt0, filt1,
                   histfix=True):
         """`levels` a-trous levels; level i writes filt[i & 1] (odd counts end in filt0)."""
+        n, nh = int(width) * int(height), _half(width) * _half(height)
+        sizes = (("gb_pos", 16 * n), ("gb_normal", 4 * n), ("gi_mom", 16 * nh), ("gi_hist", 8 * nh),
+                 ("guide", 16 * nh), ("filt0", 8 * nh), ("filt1", 8 * nh))
+        ts = (gb_pos, gb_normal, gi_mom, gi_hist, guide, filt0, filt1)
         self._ck(lib().vf_gi_atrous(self._h, int(width), int(height), int(levels), int(bool(histfix)),
-                                    *[_ptr(t) for t in (gb_pos, gb_normal, gi_mom, gi_hist, guide, filt0, filt1)]))
+                                    *[_dev(t, b, w) for t, (w, b) in zip(ts, sizes)]))
 
     def trace_rays(self, rays, n, hits, t_cap=1e30, skip_mat=0):
-        self._ck(lib().vf_trace_rays(self._h, _ptr(rays), int(n), float(t_cap), int(skip_mat), _ptr(hits)))
+        self._ck(lib().vf_trace_rays(self._h, _dev(rays, 24 * int(n), "rays"), int(n), float(t_cap), int(skip_mat),
+                                     _dev(hits, 20 * int(n), "hits")))
 
     def trace_probes(self, origin, spacing, dims, n_rays, frame, reach, hits):
         o = (C.c_float * 3)(*[float(x) for x in origin])
+        nh = int(np.prod([int(x) for x in dims])) * int(n_rays)
         self._ck(lib().vf_trace_probes(self._h, o, float(spacing), _i3(dims), int(n_rays), int(frame), float(reach),
-                                       _ptr(hits)))
+                                       _dev(hits, 20 * max(nh, 0), "hits")))
 
     # persistent-queue batches
     @staticmethod
@@ -643,8 +694,10 @@ class Context:
         return c
 
     def trace_gi_batch(self, pass_ubo, width, height, reach, gb_pos, gb_normal, gi_hits, sun_hits, cfg=None):
+        n = int(width) * int(height)
         self._ck(lib().vf_trace_gi_batch(self._h, C.byref(pass_ubo), int(width), int(height), float(reach),
-                                         _ptr(gb_pos), _ptr(gb_normal), _ptr(gi_hits), _ptr(sun_hits),
+                                         _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal"),
+                                         _dev(gi_hits, 20 * n, "gi_hits"), _dev(sun_hits, 20 * n, "sun_hits"),
                                          C.byref(cfg) if cfg is not None else None))
 
     @staticmethod
@@ -657,7 +710,7 @@ class Context:
             r.cam_up[i], r.cam_fwd[i] = pass_ubo.cam_up[i], pass_ubo.cam_fwd[i]
         r.tan_half_fov = pass_ubo.tan_half_fov
         r.jitter[0], r.jitter[1] = pass_ubo.jitter[0], pass_ubo.jitter[1]
-        r.d_hits = hits.data_ptr()
+        r.d_hits = _dev(hits, 20 * int(width) * int(height), "hits").value
         return r
 
     @staticmethod
@@ -668,7 +721,7 @@ class Context:
             r.origin[i] = float(origin[i])
             r.dims[i] = int(dims[i])
         r.spacing, r.n_rays, r.frame, r.reach = float(spacing), int(n_rays), int(frame), float(reach)
-        r.d_hits = hits.data_ptr()
+        r.d_hits = _dev(hits, 20 * int(np.prod([int(x) for x in dims])) * int(n_rays), "hits").value
         return r
 
     def trace_batch(self, requests, cfg=None):
@@ -690,8 +743,9 @@ class Context:
         a = TuneGiArgs()
         a.pass_ = C.cast(C.pointer(pass_ubo), C.c_void_p)
         a.width, a.height, a.reach = int(width), int(height), float(reach)
-        a.d_gb_pos, a.d_gb_normal = gb_pos.data_ptr(), gb_normal.data_ptr()
-        a.d_gi_hits, a.d_sun_hits = gi_hits.data_ptr(), sun_hits.data_ptr()
+        n = int(width) * int(height)
+        a.d_gb_pos, a.d_gb_normal = _dev(gb_pos, 16 * n, "gb_pos").value, _dev(gb_normal, 4 * n, "gb_normal").value
+        a.d_gi_hits, a.d_sun_hits = _dev(gi_hits, 20 * n, "gi_hits").value, _dev(sun_hits, 20 * n, "sun_hits").value
         return self._autotune(VF_TUNE_GI_BATCH, a)
 
     def autotune_batch(self, requests):
@@ -714,9 +768,11 @@ class Context:
     def trace_gi_path(self, pass_ubo, width, height, gb_pos, gb_normal, out_rgba, frame=0, spp=1, salt=0x68e31da4,
                       gi_reach=48.0, lights_nee=True):
         """Plain traced two-bounce GI (mean of spp paths per pixel) into out_rgba (float32, 4 per pixel)."""
+        n = int(width) * int(height)
         self._ck(lib().vf_trace_gi_path(self._h, C.byref(pass_ubo), int(width), int(height), float(gi_reach),
-                                        int(bool(lights_nee)), _ptr(gb_pos), _ptr(gb_normal), int(frame) & 0xFFFFFFFF,
-                                        int(spp), int(salt) & 0xFFFFFFFF, _ptr(out_rgba)))
+                                        int(bool(lights_nee)), _dev(gb_pos, 16 * n, "gb_pos"),
+                                        _dev(gb_normal, 4 * n, "gb_normal"), int(frame) & 0xFFFFFFFF,
+                                        int(spp), int(salt) & 0xFFFFFFFF, _dev(out_rgba, 16 * n, "out_rgba")))
 
     # frame graphs
     def graph_begin(self):
@@ -785,8 +841,10 @@ class GiCache:
         self._ck(lib().vf_gicache_reset(self.ctx._h, self._h))
 
     def frame(self, pass_ubo, width, height, gb_pos, gb_normal, out_rgba, frame):
-        self._ck(lib().vf_gicache_frame(self.ctx._h, self._h, C.byref(pass_ubo), int(width), int(height), _ptr(gb_pos),
-                                        _ptr(gb_normal), int(frame) & 0xFFFFFFFF, _ptr(out_rgba)))
+        n = int(width) * int(height)
+        self._ck(lib().vf_gicache_frame(self.ctx._h, self._h, C.byref(pass_ubo), int(width), int(height),
+                                        _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal"),
+                                        int(frame) & 0xFFFFFFFF, _dev(out_rgba, 16 * n, "out_rgba")))
 
     def stats(self):
         s = GiCacheStats()
