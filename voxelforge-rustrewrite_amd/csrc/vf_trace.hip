@@ -606,6 +606,339 @@ __global__ void __launch_bounds__(256) k_dist(DistArgs A, const uint32_t *grid, 
     dst[ci] = (uint8_t)best;
 }
 
+// The separable passes staged in LDS.  A workgroup takes a tile of kDX consecutive x cells; a tap outside the
+// world reads the sentinel kOut, larger than any max(|k|, f), so it never wins (the shader's `continue`).
+// A tap at |k| = 16 gives max(16, f) >= the cap 16, so only |k| <= 15 can lower a result: the taps run in
+// ascending |k| and a wavefront stops once every lane's best is <= |k| (no later tap can lower it).
+constexpr int kDX = 32;          // x cells per tile; the x pass's window is kDX + 32 = 64 cells (one 64-bit mask)
+constexpr int kDMaxY = 64;       // box y extent the fused x+y kernel holds in LDS
+constexpr uint32_t kOut = 255u;
+
+typedef unsigned short vf_us2 __attribute__((ext_vector_type(2)));
+
+// Tiles start at a multiple of 4 cells in x (x0 & 3 == 0, wrapped or not), so with dims.x % 4 == 0 (VEC) every
+// 4-cell quad of a row is one aligned 32-bit word: loaded and stored whole.  Cells of a tile outside the box
+// are read (inside the world) but never written.
+__device__ __forceinline__ int32_t dist_tile_x0(const DistArgs &A, int32_t tile, int32_t width) {
+    return (A.bmin[0] & ~3) + tile * width;
+}
+
+// 4 cells of a row (x .. x + 3, x % 4 == 0) as bytes of one word, each outside the world = kOut.
+template <bool VEC>
+__device__ __forceinline__ uint32_t load_quad(const DistArgs &A, const uint8_t *src, int32_t zl, int32_t y, int32_t x,
+                                              bool zin) {
+    const size_t row = ((size_t)zl * A.dims[1] + y) * A.dims[0];
+    if (VEC) {
+        int32_t xq = x;
+        if (A.ring) xq &= A.dims[0] - 1;
+        const bool in = zin && (A.ring || xq < A.dims[0]);   // dims.x % 4 == 0: a quad is all in or all out
+        const uint32_t w = reinterpret_cast<const uint32_t *>(src + row)[(in ? xq : 0) >> 2];
+        return in ? w : 0xFFFFFFFFu;
+    }
+    uint32_t w = 0u;
+#pragma unroll
+    for (int32_t e = 0; e < 4; ++e) {
+        int32_t xe = x + e;
+        if (A.ring) xe &= A.dims[0] - 1;
+        const bool in = zin && (A.ring || xe < A.dims[0]);
+        const uint32_t b = src[row + (in ? xe : 0)];
+        w |= (in ? b : kOut) << (8 * e);
+    }
+    return w;
+}
+
+// Writes the cells of quad (x .. x + 3) that lie in the box [bmin.x, bmax.x] (absolute x).
+template <bool VEC>
+__device__ __forceinline__ void store_quad(const DistArgs &A, uint8_t *dst, int32_t zw, int32_t y, int32_t x, uint32_t w) {
+    const size_t row = ((size_t)zw * A.dims[1] + y) * A.dims[0];
+    if (VEC && x >= A.bmin[0] && x + 3 <= A.bmax[0]) {
+        const int32_t xq = A.ring ? (x & (A.dims[0] - 1)) : x;
+        reinterpret_cast<uint32_t *>(dst + row)[xq >> 2] = w;
+        return;
+    }
+#pragma unroll
+    for (int32_t e = 0; e < 4; ++e) {
+        const int32_t xe = x + e;
+        if (xe < A.bmin[0] || xe > A.bmax[0]) continue;
+        dst[row + (A.ring ? (xe & (A.dims[0] - 1)) : xe)] = (uint8_t)(w >> (8 * e));
+    }
+}
+
+
+// x pass of one row from its window (window cell j = x0 - 16 + j; output col = window cell col + 16).  A binary
+// row (every value 0, 16 or outside): min(16, distance to the nearest 0) from the row's occupancy mask.
+__device__ __forceinline__ uint32_t dist_x(const uint8_t *F, uint64_t occ, bool binary, int32_t col) {
+    const int32_t p = col + 16;
+    if (binary) {
+        const uint64_t r = occ >> p, l = occ << (63 - p);
+        const uint32_t dr = r ? (uint32_t)__builtin_ctzll(r) : 64u, dl = l ? (uint32_t)__builtin_clzll(l) : 64u;
+        return min(16u, min(dl, dr));
+    }
+    uint32_t best = 16u;
+#pragma unroll
+    for (int32_t k = -15; k <= 15; ++k) best = min(best, max((uint32_t)(k < 0 ? -k : k), (uint32_t)F[p + k]));
+    return best;
+}
+
+// brickdist.comp's x pass and y pass of one field in one kernel (record_distance_field passes 0-1 or 4-5)
+// for a tile of kDX x cells x the box's y extent at one z.  x pass: f from the grid (FIRST: occupied = g != 0)
+// or from the dry occupancy field `src`, over x +-16 (any x of the world: the source is complete).  y pass:
+// a row inside the box reads the x pass's value (held in LDS: the x pass's field is overwritten by the z pass
+// over the same box, so it is never stored); a row outside it reads `stale`, the field that pass reads in the
+// reference (d0 / d2 as they stand).  Writes the y pass's field `dst` (d1) over the box.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_dist_xy(DistArgs A, const uint32_t *grid, const uint8_t *src, const uint8_t *stale,
+                                                 uint8_t *dst) {
+    __shared__ uint8_t F[kDMaxY][kDX + 32];
+    __shared__ uint8_t X[kDMaxY][kDX];
+    __shared__ uint64_t M[kDMaxY];
+    __shared__ int32_t nonbinary;
+    const int32_t ey = A.bmax[1] - A.bmin[1] + 1;
+    const int32_t x0 = A.bmin[0] + (int32_t)blockIdx.x * kDX;
+    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
+    const int32_t t = (int32_t)threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (t == 0) nonbinary = 0;
+    __syncthreads();
+    // window rows: wavefront w takes rows w, w + 4, ..., lane j = window cell j.  All its loads are issued
+    // first (up to kDMaxY / 4 per lane), then each row's occupancy becomes a ballot.
+    {
+        int32_t xa = x0 - 16 + lane;
+        if (A.ring) xa &= A.dims[0] - 1;
+        const bool inside = A.ring || (xa >= 0 && xa < A.dims[0]);
+        uint32_t v[kDMaxY / 4];
+        const int32_t xl = inside ? xa : 0;
+#pragma unroll
+        for (int32_t j = 0; j < kDMaxY / 4; ++j) {   // unconditional loads from valid addresses, masked after
+            const int32_t row = min(wv + 4 * j, ey - 1);
+            const size_t qi = ((size_t)zw * A.dims[1] + (A.bmin[1] + row)) * A.dims[0] + xl;
+            v[j] = FIRST ? grid[qi] : (uint32_t)src[qi];
+        }
+#pragma unroll
+        for (int32_t j = 0; j < kDMaxY / 4; ++j)
+            if (!(wv + 4 * j < ey && inside)) v[j] = kOut;
+#pragma unroll
+        for (int32_t j = 0; j < kDMaxY / 4; ++j) {
+            const int32_t row = wv + 4 * j;
+            if (row >= ey) break;   // wave-uniform
+            const uint32_t f = FIRST ? (v[j] == kOut ? kOut : (v[j] != 0u ? 0u : 16u)) : v[j];
+            F[row][lane] = (uint8_t)f;
+            const uint64_t occ = __ballot(f == 0u);
+            if (!FIRST && __any(f != 0u && f != 16u && f != kOut) && lane == 0) nonbinary = 1;
+            if (lane == 0) M[row] = occ;
+        }
+    }
+    __syncthreads();
+    const bool binary = FIRST || nonbinary == 0;
+    for (int32_t i = t; i < ey * kDX; i += 256) {
+        const int32_t row = i / kDX, col = i - row * kDX;
+        X[row][col] = (uint8_t)dist_x(F[row], M[row], binary, col);
+    }
+    __syncthreads();
+    const int32_t nx = min(kDX, A.bmax[0] - x0 + 1);
+    for (int32_t i0 = 0; i0 < ey * kDX; i0 += 256) {   // the same trip count in every lane (wavefront votes)
+        const int32_t i = i0 + t;
+        const int32_t row = i / kDX, col = i - row * kDX;
+        const bool act = i < ey * kDX && col < nx;
+        const int32_t y = A.bmin[1] + row;
+        const int32_t xa = x0 + col, xw = A.ring ? (xa & (A.dims[0] - 1)) : xa;
+        uint32_t best = act ? 16u : 0u;
+        for (int32_t k = 0; k < 16; ++k) {
+            if (!__any(best > (uint32_t)k)) break;
+            if (!act) continue;
+#pragma unroll
+            for (int32_t sg = 0; sg < 2; ++sg) {
+                if (sg == 1 && k == 0) break;
+                const int32_t qy = sg ? y + k : y - k;
+                if (qy < 0 || qy >= A.dims[1]) continue;
+                const uint32_t f = (qy >= A.bmin[1] && qy <= A.bmax[1])
+                                       ? (uint32_t)X[qy - A.bmin[1]][col]
+                                       : (uint32_t)stale[((size_t)zw * A.dims[1] + qy) * A.dims[0] + xw];
+                best = min(best, max((uint32_t)k, f));
+            }
+        }
+        if (act) dst[((size_t)zw * A.dims[1] + y) * A.dims[0] + xw] = (uint8_t)best;
+    }
+}
+
+// brickdist.comp's x and y passes of one field for a box spanning the whole world height (every y tap is inside
+// the box: no stale rows), one wavefront per tile of kDX x cells at one z.  With a binary x-pass input
+// (occupancy: the grid's g != 0, or the dry occupancy field holding 0 / 16) the x+y result is min(16, Chebyshev
+// distance in the xy plane to the nearest occupied cell): it is <= r exactly when the (2r+1)^2 square around
+// the cell holds an occupied cell.  So the result = the number of radii r in [0, 15] whose square dilation of
+// the occupancy misses the cell.  Lane j holds window column x0 - 16 + j as a bit mask over the rows; each
+// radius is one more dilation step (rows: a shift within the lane; columns: the neighbour lanes through DPP
+// wavefront shifts), counted by a 5-plane bit-sliced counter per lane.  A lane's error from the window's edge
+// moves one lane per step, so the 32 output lanes 16..47 are exact after 15 steps.  A non-binary dry input
+// falls back to the passes' taps.  MaskT: 32 or 64 rows.
+__device__ __forceinline__ uint32_t lane_from_below(uint32_t v) {   // lane i gets lane i - 1's value, lane 0 gets 0
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ uint32_t lane_from_above(uint32_t v) {   // lane i gets lane i + 1's value, lane 63 gets 0
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ uint64_t lane_from_below(uint64_t v) {
+    return (uint64_t)lane_from_below((uint32_t)v) | ((uint64_t)lane_from_below((uint32_t)(v >> 32)) << 32);
+}
+__device__ __forceinline__ uint64_t lane_from_above(uint64_t v) {
+    return (uint64_t)lane_from_above((uint32_t)v) | ((uint64_t)lane_from_above((uint32_t)(v >> 32)) << 32);
+}
+
+template <bool FIRST, typename MaskT>
+__global__ void __launch_bounds__(64) k_dist_xy_mask(DistArgs A, const uint32_t *grid, const uint8_t *src, uint8_t *dst) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    __shared__ uint8_t F[kRows][kDX + 32];   // the window values (the non-binary fallback only)
+    __shared__ uint8_t X[kRows][kDX];
+    __shared__ uint32_t D[kRows][kDX / 4];   // the results, rows x quads (vector stores)
+    const int32_t ey = A.dims[1];
+    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kDX);
+    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
+    const int32_t lane = (int32_t)threadIdx.x;
+    int32_t xa = x0 - 16 + lane;
+    if (A.ring) xa &= A.dims[0] - 1;
+    const bool inside = A.ring || (xa >= 0 && xa < A.dims[0]);
+    const int32_t xl = inside ? xa : 0;   // a valid address for every lane; outside lanes read the sentinel
+    MaskT occ = 0;
+    bool nonbinary = false;
+    // every row's value loaded from a valid address with no branch around it (a load the compiler sinks into
+    // a conditional block is waited for before the next one), then masked
+    uint32_t raw[kRows];
+    const size_t base = (size_t)zw * A.dims[1] * A.dims[0] + xl;
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) {
+        const size_t qi = base + (size_t)(j < ey ? j : ey - 1) * A.dims[0];
+        raw[j] = FIRST ? grid[qi] : (uint32_t)src[qi];
+    }
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) {
+        const uint32_t valid = (uint32_t)(inside && j < ey);
+        if (FIRST) {
+            occ |= (MaskT)((raw[j] != 0u ? 1u : 0u) & valid) << j;
+        } else {
+            const uint32_t f = valid ? raw[j] : kOut;
+            occ |= (MaskT)((f == 0u ? 1u : 0u)) << j;
+            nonbinary = nonbinary || (f != 0u && f != 16u && f != kOut);
+            if (j < ey) F[j][lane] = (uint8_t)f;
+        }
+    }
+    if (!FIRST && __any(nonbinary)) {   // the passes' taps over LDS (a dry field that is not 0 / 16)
+        __syncthreads();
+        for (int32_t i = lane; i < ey * kDX; i += 64) {
+            const int32_t row = i / kDX, col = i - row * kDX;
+            X[row][col] = (uint8_t)dist_x(F[row], 0ull, false, col);
+        }
+        __syncthreads();
+        for (int32_t i = lane; i < ey * kDX; i += 64) {
+            const int32_t row = i / kDX, col = i - row * kDX;
+            const int32_t xc = x0 + col;
+            if (xc < A.bmin[0] || xc > A.bmax[0]) continue;
+            uint32_t best = 16u;
+            for (int32_t k = -15; k <= 15; ++k) {
+                const int32_t qy = row + k;
+                if (qy < 0 || qy >= ey) continue;
+                best = min(best, max((uint32_t)(k < 0 ? -k : k), (uint32_t)X[qy][col]));
+            }
+            const int32_t xw = A.ring ? (xc & (A.dims[0] - 1)) : xc;
+            dst[((size_t)zw * A.dims[1] + row) * A.dims[0] + xw] = (uint8_t)best;
+        }
+        return;
+    }
+    const MaskT rows = ey >= kRows ? ~(MaskT)0 : (((MaskT)1 << ey) - (MaskT)1);
+    MaskT S = occ;
+    MaskT c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+#pragma unroll
+    for (int32_t r = 0; r < 16; ++r) {
+        if (r > 0) {
+            const MaskT t = (S | (S << 1) | (S >> 1)) & rows;   // rows r - 1 -> r (no row outside the world)
+            S = t | lane_from_below(t) | lane_from_above(t);     // columns
+        }
+        MaskT carry = ~S, q;   // rows of this column not yet within r of an occupied cell
+        q = c0 & carry; c0 ^= carry; carry = q;
+        q = c1 & carry; c1 ^= carry; carry = q;
+        q = c2 & carry; c2 ^= carry; carry = q;
+        q = c3 & carry; c3 ^= carry; carry = q;
+        c4 ^= carry;
+    }
+    // lanes 16..47 hold the tile's columns: their results go through LDS as rows of 32-bit quads
+    const int32_t col = lane - 16;
+    if (col >= 0 && col < kDX) {
+        uint8_t *Db = reinterpret_cast<uint8_t *>(&D[0][0]);
+#pragma unroll
+        for (int32_t row = 0; row < kRows; ++row) {
+            const uint32_t d = (uint32_t)((c0 >> row) & 1u) | ((uint32_t)((c1 >> row) & 1u) << 1) |
+                               ((uint32_t)((c2 >> row) & 1u) << 2) | ((uint32_t)((c3 >> row) & 1u) << 3) |
+                               ((uint32_t)((c4 >> row) & 1u) << 4);
+            Db[row * kDX + col] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    for (int32_t i = lane; i < ey * (kDX / 4); i += 64) {
+        const int32_t row = i / (kDX / 4), q = i - row * (kDX / 4);
+        if (A.dims[0] % 4 == 0) store_quad<true>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
+        else store_quad<false>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
+    }
+}
+
+constexpr int kZX = 64;   // x cells per z-pass tile (16 quads)
+constexpr int kZZ = 16;   // z cells per z-pass tile
+
+// brickdist.comp's z pass (record_distance_field passes 2 and 6) over a tile of kZX x cells x kZZ z cells at one
+// y: `src` (d1) over z +-16 staged in LDS as 16-bit values, one quad (4 x cells) per lane with packed 16-bit
+// max / min; written to `dst` (d0 / d2) over the box.  A tile whose window holds no value below 16 (open sky)
+// is 16 throughout: every tap's max(|k|, f) is >= 16.
+template <bool VEC>
+__global__ void __launch_bounds__(256) k_dist_z(DistArgs A, const uint8_t *src, uint8_t *dst) {
+    constexpr int32_t Q = kZX / 4, ROWS = kZZ + 32, PER = ROWS * Q / 256;
+    __shared__ uint2 F[ROWS][Q];
+    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kZX);
+    const int32_t y = A.bmin[1] + (int32_t)blockIdx.y;
+    const int32_t z0 = A.bmin[2] + (int32_t)blockIdx.z * kZZ;
+    const int32_t t = (int32_t)threadIdx.x;
+    bool low = false;
+    {
+        uint32_t v[PER];
+#pragma unroll
+        for (int32_t j = 0; j < PER; ++j) {   // every load issued before the first is used
+            const int32_t i = t + 256 * j, row = i / Q, q = i - row * Q;
+            int32_t za = z0 - 16 + row;
+            if (A.ring) za &= A.dims[2] - 1;
+            const bool zin = A.ring || (za >= 0 && za < A.dims[2]);
+            v[j] = load_quad<VEC>(A, src, zin ? za : 0, y, x0 + 4 * q, zin);
+        }
+#pragma unroll
+        for (int32_t j = 0; j < PER; ++j) {
+            const int32_t i = t + 256 * j, row = i / Q, q = i - row * Q;
+            const uint32_t w = v[j];
+            // bytes < 16 (values are 0..16 or kOut): a set bit below bit 4 of the byte, or a zero byte
+            low = low || ((w & 0xFFu) < 16u) || (((w >> 8) & 0xFFu) < 16u) || (((w >> 16) & 0xFFu) < 16u) || ((w >> 24) < 16u);
+            F[row][q] = make_uint2((w & 0xFFu) | ((w & 0xFF00u) << 8), ((w >> 16) & 0xFFu) | ((w >> 8) & 0xFF0000u));
+        }
+    }
+    const bool any_low = __syncthreads_or(low) != 0;
+    const int32_t row = t / Q, q = t - (t / Q) * Q;
+    const int32_t za = z0 + row;
+    if (za > A.bmax[2]) return;
+    const int32_t zw = A.ring ? (za & (A.dims[2] - 1)) : za;
+    if (!any_low) {
+        store_quad<VEC>(A, dst, zw, y, x0 + 4 * q, 0x10101010u);
+        return;
+    }
+    vf_us2 b01 = vf_us2{16, 16}, b23 = b01;
+    for (int32_t k = 0; k < 16; ++k) {
+        const vf_us2 mx = __builtin_elementwise_max(b01, b23);
+        if (!__any((uint32_t)max(mx.x, mx.y) > (uint32_t)k)) break;
+        const vf_us2 kk = {(unsigned short)k, (unsigned short)k};
+        const uint2 lo = F[row + 16 - k][q], hi = F[row + 16 + k][q];
+        b01 = __builtin_elementwise_min(b01, __builtin_elementwise_max(kk, __builtin_bit_cast(vf_us2, lo.x)));
+        b23 = __builtin_elementwise_min(b23, __builtin_elementwise_max(kk, __builtin_bit_cast(vf_us2, lo.y)));
+        b01 = __builtin_elementwise_min(b01, __builtin_elementwise_max(kk, __builtin_bit_cast(vf_us2, hi.x)));
+        b23 = __builtin_elementwise_min(b23, __builtin_elementwise_max(kk, __builtin_bit_cast(vf_us2, hi.y)));
+    }
+    store_quad<VEC>(A, dst, zw, y, x0 + 4 * q,
+                    (uint32_t)b01.x | ((uint32_t)b01.y << 8) | ((uint32_t)b23.x << 16) | ((uint32_t)b23.y << 24));
+}
+
 // Cell words over the whole grid: grid value, both distance fields and the coarse-cell flags.
 __global__ void __launch_bounds__(256) k_pack(const uint32_t *grid, const uint8_t *d0, const uint8_t *d2, const uint8_t *c1,
                                               const uint8_t *c2, int32_t wbx, int32_t wby, int32_t wbz, int32_t c1x,
@@ -817,14 +1150,56 @@ int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], 
     }
     uint8_t **d = ctx->dist;
     int rc;
-    if ((rc = dist_pass(ctx, lo, hi, 0, 1, nullptr, d[0]))) return rc;
-    if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[0], d[1]))) return rc;
-    if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[0]))) return rc;
+    if (hi[1] - lo[1] + 1 > kDMaxY) {   // a box taller than the fused kernel holds: the seven passes one by one
+        if ((rc = dist_pass(ctx, lo, hi, 0, 1, nullptr, d[0]))) return rc;
+        if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[0], d[1]))) return rc;
+        if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[0]))) return rc;
+        if ((rc = dist_pass(ctx, dlo, dhi, 0, 2, nullptr, d[3]))) return rc;
+        if ((rc = dist_pass(ctx, lo, hi, 0, 0, d[3], d[2]))) return rc;
+        if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[2], d[1]))) return rc;
+        if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[2]))) return rc;
+        return VF_OK;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (hi[a] < lo[a]) return VF_OK;
+    DistArgs A;
+    for (int a = 0; a < 3; ++a) {
+        A.bmin[a] = lo[a];
+        A.bmax[a] = hi[a];
+        A.dims[a] = wb[a];
+    }
+    A.axis = 0;
+    A.fp = 0;
+    A.ring = ctx->W.ring;
+    const int32_t ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    const int32_t exa = hi[0] - (lo[0] & ~3) + 1;   // from the 4-aligned tile origin
+    const dim3 gxy((unsigned)((ex + kDX - 1) / kDX), (unsigned)ez),
+        gxym((unsigned)((exa + kDX - 1) / kDX), (unsigned)ez),
+        gz((unsigned)((exa + kZX - 1) / kZX), (unsigned)ey, (unsigned)((ez + kZZ - 1) / kZZ));
+    using ZKernel = void (*)(DistArgs, const uint8_t *, uint8_t *);
+    const ZKernel kz = wb[0] % 4 == 0 ? k_dist_z<true> : k_dist_z<false>;
+    // a box over the world's whole height reads no row outside itself: the square-dilation count
+    const bool full_height = lo[1] == 0 && hi[1] == wb[1] - 1 && wb[1] <= 64;
+    using MaskKernel = void (*)(DistArgs, const uint32_t *, const uint8_t *, uint8_t *);
+    const MaskKernel mask_first = wb[1] <= 32 ? k_dist_xy_mask<true, uint32_t> : k_dist_xy_mask<true, uint64_t>;
+    const MaskKernel mask_dry = wb[1] <= 32 ? k_dist_xy_mask<false, uint32_t> : k_dist_xy_mask<false, uint64_t>;
+    // passes 0-2: x and y (d0's x values live in LDS only), then z into d0
+    if (full_height)
+        hipLaunchKernelGGL(mask_first, gxym, dim3(64), 0, ctx->stream, A, ctx->grid, nullptr, d[1]);
+    else
+        hipLaunchKernelGGL(k_dist_xy<true>, gxy, dim3(256), 0, ctx->stream, A, ctx->grid, nullptr, d[0], d[1]);
+    if ((rc = launch_check(ctx, "brickdist x+y"))) return rc;
+    hipLaunchKernelGGL(kz, gz, dim3(256), 0, ctx->stream, A, d[1], d[0]);
+    if ((rc = launch_check(ctx, "brickdist z"))) return rc;
+    // pass 3: the dry occupancy over the dirty box; passes 4-6 as above on it, into d2
     if ((rc = dist_pass(ctx, dlo, dhi, 0, 2, nullptr, d[3]))) return rc;
-    if ((rc = dist_pass(ctx, lo, hi, 0, 0, d[3], d[2]))) return rc;
-    if ((rc = dist_pass(ctx, lo, hi, 1, 0, d[2], d[1]))) return rc;
-    if ((rc = dist_pass(ctx, lo, hi, 2, 0, d[1], d[2]))) return rc;
-    return VF_OK;
+    if (full_height)
+        hipLaunchKernelGGL(mask_dry, gxym, dim3(64), 0, ctx->stream, A, ctx->grid, d[3], d[1]);
+    else
+        hipLaunchKernelGGL(k_dist_xy<false>, gxy, dim3(256), 0, ctx->stream, A, ctx->grid, d[3], d[2], d[1]);
+    if ((rc = launch_check(ctx, "brickdist dry x+y"))) return rc;
+    hipLaunchKernelGGL(kz, gz, dim3(256), 0, ctx->stream, A, d[1], d[2]);
+    return launch_check(ctx, "brickdist dry z");
 }
 
 
