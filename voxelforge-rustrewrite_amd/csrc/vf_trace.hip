@@ -161,21 +161,26 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
     } else {
         const float N[3] = {NDEC ? Ndec[0] : unorm8_dec(nq & 255u), NDEC ? Ndec[1] : unorm8_dec((nq >> 8) & 255u),
                             NDEC ? Ndec[2] : unorm8_dec((nq >> 16) & 255u)};
-        if (dot3(N, sun) <= 0.0f) {
+        // the ray's origin: P + N*0.03 for a texel facing the sun; FOLIAGE_SSS (shadow_trace.comp:35-46):
+        // foliageSunOrigin for back-lit foliage with the sun above the horizon, which stores no blocker
+        // distance; one call site of the traversal for both
+        const bool lit = dot3(N, sun) > 0.0f;
+        bool trace = lit;
+        float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+        if (!lit) {
             const float a = NDEC ? ((nq >> 24) == kFoliageAlpha ? 0.8f : 1.0f) : gbn_alpha(nq);
-            if (foliage_flag(a) && sun[1] > 0.0f) {   // FOLIAGE_SSS (shadow_trace.comp:35-46): back-lit foliage
+            if (foliage_flag(a) && sun[1] > 0.0f) {
                 const float P[3] = {p.x, p.y, p.z};
-                float so[3], tf;
                 foliage_sun_origin(P, N, sun, blade_flag(a), so);
-                uint32_t seed = 0u;
-                vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE>(W, so, sun, leaf, seed, tf, hits, i, c, B, pinv);
-            } else if (hits) {
-                store_miss(hits + i);
+                trace = true;
             }
-        } else {
-            const float so[3] = {p.x + N[0] * 3e-2f, p.y + N[1] * 3e-2f, p.z + N[2] * 3e-2f};
+        }
+        if (trace) {
             uint32_t seed = 0u;
             vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE>(W, so, sun, leaf, seed, tb, hits, i, c, B, pinv);
+            if (!lit) tb = 0.0f;
+        } else if (hits) {
+            store_miss(hits + i);
         }
     }
     const uint32_t hv = (uint32_t)__half_as_ushort(__float2half_rn(vis));
