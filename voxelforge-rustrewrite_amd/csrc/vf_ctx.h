@@ -34,6 +34,8 @@ struct vf_ctx {
     uint8_t *atlas = nullptr;
     uint32_t *occ = nullptr, *occdry = nullptr, *masks_ref = nullptr;
     uint8_t *dist[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t *dist_tmp = nullptr;   // the normal field's x+y values of a two-launch build (not a reference field)
+    uint8_t *dry_any = nullptr;    // per atlas slot: 1 if the brick holds a voxel that is neither air nor water
     uint8_t *c1 = nullptr, *c2 = nullptr;
     uint8_t *srcnz = nullptr;   // per cell: the caller's (source) cell is not air; the coarse pyramid's input
     uint16_t *skytop = nullptr;

@@ -510,22 +510,27 @@ __global__ void __launch_bounds__(256) k_probes(DevWorld W, DevBodies B, ProbeAr
 // ------------------------------------------------------------------------------------------------
 // One thread per (slot, 32-bit word): 32 voxels = 4 rows of 8.
 __global__ void __launch_bounds__(256) k_masks(const uint8_t *atlas, const uint32_t *slots, uint32_t n_list, uint32_t n_slots,
-                                               uint32_t *occ, uint32_t *occdry) {
+                                               uint32_t *occ, uint32_t *occdry, uint8_t *dry_any) {
     const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     const uint64_t k = gid >> 4;
     const uint32_t wd = (uint32_t)(gid & 15u);
-    if (k >= (slots ? n_list : n_slots)) return;
-    const uint32_t slot = slots ? slots[k] : (uint32_t)k;
-    if (slot >= n_slots) return;
-    const uint8_t *b = atlas + (size_t)slot * 512u + (size_t)wd * 32u;
+    const bool live = k < (slots ? n_list : n_slots);
+    const uint32_t slot = live ? (slots ? slots[k] : (uint32_t)k) : 0u;
+    const bool ok = live && slot < n_slots;
     uint32_t m0 = 0u, m5 = 0u;
-    for (int i = 0; i < 32; ++i) {
-        const uint32_t v = b[i];
-        m0 |= (uint32_t)(v != 0u) << i;
-        m5 |= (uint32_t)(v != 0u && v != 5u) << i;
+    if (ok) {
+        const uint8_t *b = atlas + (size_t)slot * 512u + (size_t)wd * 32u;
+        for (int i = 0; i < 32; ++i) {
+            const uint32_t v = b[i];
+            m0 |= (uint32_t)(v != 0u) << i;
+            m5 |= (uint32_t)(v != 0u && v != 5u) << i;
+        }
+        occ[(size_t)slot * 16u + wd] = m0;
+        occdry[(size_t)slot * 16u + wd] = m5;
     }
-    occ[(size_t)slot * 16u + wd] = m0;
-    occdry[(size_t)slot * 16u + wd] = m5;
+    // the slot's 16 words are 16 consecutive lanes: whether any of them has a dry voxel
+    const uint64_t b = __ballot(m5 != 0u);
+    if (ok && wd == 0u) dry_any[slot] = ((b >> (__lane_id() & 48u)) & 0xFFFFull) != 0ull ? 1u : 0u;
 }
 
 // coarse1: one thread per 4^3-brick cell (gpu_world.rs:300-313).
@@ -789,6 +794,79 @@ __device__ __forceinline__ uint64_t lane_from_above(uint64_t v) {
     return (uint64_t)lane_from_above((uint32_t)v) | ((uint64_t)lane_from_above((uint32_t)(v >> 32)) << 32);
 }
 
+// The square-dilation count of one tile (lane j = window column x0 - 16 + j, MaskT bits = rows) into 5 counter
+// planes, then the result bytes through LDS as rows of 32-bit quads to `dst` over the box.
+template <typename MaskT>
+__device__ __forceinline__ void dist_xy_square(MaskT occ, int32_t ey, MaskT &c0, MaskT &c1, MaskT &c2, MaskT &c3, MaskT &c4) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    const int32_t lane = (int32_t)threadIdx.x;
+    const MaskT rows = ey >= kRows ? ~(MaskT)0 : (((MaskT)1 << ey) - (MaskT)1);
+    MaskT S = occ;
+    c0 = 0; c1 = 0; c2 = 0; c3 = 0; c4 = 0;
+#pragma unroll
+    for (int32_t r = 0; r < 16; ++r) {
+        if (r > 0) {
+            const MaskT t = (S | (S << 1) | (S >> 1)) & rows;   // rows r - 1 -> r (no row outside the world)
+            S = t | lane_from_below(t) | lane_from_above(t);     // columns
+        }
+        MaskT carry = ~S, q;   // rows of this column not yet within r of an occupied cell
+        q = c0 & carry; c0 ^= carry; carry = q;
+        q = c1 & carry; c1 ^= carry; carry = q;
+        q = c2 & carry; c2 ^= carry; carry = q;
+        q = c3 & carry; c3 ^= carry; carry = q;
+        c4 ^= carry;
+    }
+    (void)lane;
+}
+
+template <typename MaskT>
+__device__ __forceinline__ void dist_xy_emit(const DistArgs &A, int32_t x0, int32_t zw, int32_t ey, MaskT c0, MaskT c1,
+                                             MaskT c2, MaskT c3, MaskT c4, uint32_t (*D)[kDX / 4], uint8_t *dst) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    const int32_t lane = (int32_t)threadIdx.x, col = lane - 16;
+    if (col >= 0 && col < kDX) {   // lanes 16..47 hold the tile's columns
+        uint8_t *Db = reinterpret_cast<uint8_t *>(&D[0][0]);
+#pragma unroll
+        for (int32_t row = 0; row < kRows; ++row) {
+            const uint32_t d = (uint32_t)((c0 >> row) & 1u) | ((uint32_t)((c1 >> row) & 1u) << 1) |
+                               ((uint32_t)((c2 >> row) & 1u) << 2) | ((uint32_t)((c3 >> row) & 1u) << 3) |
+                               ((uint32_t)((c4 >> row) & 1u) << 4);
+            Db[row * kDX + col] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    for (int32_t i = lane; i < ey * (kDX / 4); i += 64) {
+        const int32_t row = i / (kDX / 4), q = i - row * (kDX / 4);
+        if (A.dims[0] % 4 == 0) store_quad<true>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
+        else store_quad<false>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
+    }
+}
+
+// The passes' taps over LDS for a tile whose x-pass input is not 0 / 16 (F: the window values).
+__device__ __forceinline__ void dist_xy_taps(const DistArgs &A, int32_t x0, int32_t zw, int32_t ey, const uint8_t (*F)[kDX + 32],
+                                             uint8_t (*X)[kDX], uint8_t *dst) {
+    const int32_t lane = (int32_t)threadIdx.x;
+    __syncthreads();
+    for (int32_t i = lane; i < ey * kDX; i += 64) {
+        const int32_t row = i / kDX, col = i - row * kDX;
+        X[row][col] = (uint8_t)dist_x(F[row], 0ull, false, col);
+    }
+    __syncthreads();
+    for (int32_t i = lane; i < ey * kDX; i += 64) {
+        const int32_t row = i / kDX, col = i - row * kDX;
+        const int32_t xc = x0 + col;
+        if (xc < A.bmin[0] || xc > A.bmax[0]) continue;
+        uint32_t best = 16u;
+        for (int32_t k = -15; k <= 15; ++k) {
+            const int32_t qy = row + k;
+            if (qy < 0 || qy >= ey) continue;
+            best = min(best, max((uint32_t)(k < 0 ? -k : k), (uint32_t)X[qy][col]));
+        }
+        const int32_t xw = A.ring ? (xc & (A.dims[0] - 1)) : xc;
+        dst[((size_t)zw * A.dims[1] + row) * A.dims[0] + xw] = (uint8_t)best;
+    }
+}
+
 template <bool FIRST, typename MaskT>
 __global__ void __launch_bounds__(64) k_dist_xy_mask(DistArgs A, const uint32_t *grid, const uint8_t *src, uint8_t *dst) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
@@ -827,78 +905,150 @@ __global__ void __launch_bounds__(64) k_dist_xy_mask(DistArgs A, const uint32_t 
             if (j < ey) F[j][lane] = (uint8_t)f;
         }
     }
-    if (!FIRST && __any(nonbinary)) {   // the passes' taps over LDS (a dry field that is not 0 / 16)
-        __syncthreads();
-        for (int32_t i = lane; i < ey * kDX; i += 64) {
-            const int32_t row = i / kDX, col = i - row * kDX;
-            X[row][col] = (uint8_t)dist_x(F[row], 0ull, false, col);
-        }
-        __syncthreads();
-        for (int32_t i = lane; i < ey * kDX; i += 64) {
-            const int32_t row = i / kDX, col = i - row * kDX;
-            const int32_t xc = x0 + col;
-            if (xc < A.bmin[0] || xc > A.bmax[0]) continue;
-            uint32_t best = 16u;
-            for (int32_t k = -15; k <= 15; ++k) {
-                const int32_t qy = row + k;
-                if (qy < 0 || qy >= ey) continue;
-                best = min(best, max((uint32_t)(k < 0 ? -k : k), (uint32_t)X[qy][col]));
-            }
-            const int32_t xw = A.ring ? (xc & (A.dims[0] - 1)) : xc;
-            dst[((size_t)zw * A.dims[1] + row) * A.dims[0] + xw] = (uint8_t)best;
-        }
+    if (!FIRST && __any(nonbinary)) {
+        dist_xy_taps(A, x0, zw, ey, F, X, dst);
         return;
     }
+    MaskT c0, c1, c2, c3, c4;
+    dist_xy_square<MaskT>(occ, ey, c0, c1, c2, c3, c4);
+    dist_xy_emit<MaskT>(A, x0, zw, ey, c0, c1, c2, c3, c4, D, dst);
+}
+
+struct DirtyBox {
+    int32_t lo[3], hi[3];
+};
+
+// Is absolute brick coordinate v of axis a inside [lo, hi] of that axis, on the ring (x / z of a ring world:
+// modulo the grid; the box is at most one ring wide) or plainly?
+__device__ __forceinline__ bool in_span(const DistArgs &A, int a, int32_t v, int32_t lo, int32_t hi) {
+    if (A.ring && a != 1) return (uint32_t)((v - lo) & (A.dims[a] - 1)) <= (uint32_t)(hi - lo);
+    return v >= lo && v <= hi;
+}
+
+// Both fields' x and y passes in one launch (blockIdx.z: 0 = the normal field, 1 = the dry field) for a box
+// over the world's whole height that contains the dirty box.  Normal: as k_dist_xy_mask<true> into `tmp`.
+// Dry: the x-pass input is the dry occupancy of record_distance_field's pass 3 inside the dirty box (computed
+// here: a uniform brick other than air and water, or a detailed brick with a dry voxel, dry_any) and the
+// dry occupancy field `d3` as it stands outside it; the pass-3 values of the tile's cells in the dirty box are
+// written to d3; the x+y result goes to d1.
+template <typename MaskT>
+__device__ __forceinline__ void dist_xy2_dry(const DistArgs &A, const DirtyBox &Db, const uint32_t *grid,
+                                             const uint8_t *dry_any, uint8_t *d3, uint8_t *d1, uint8_t (*F)[kDX + 32],
+                                             uint8_t (*X)[kDX], uint32_t (*D)[kDX / 4]) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    const int32_t ey = A.dims[1];
+    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kDX);
+    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
+    const int32_t lane = (int32_t)threadIdx.x;
+    const int32_t xabs = x0 - 16 + lane;
+    int32_t xa = xabs;
+    if (A.ring) xa &= A.dims[0] - 1;
+    const bool inside = A.ring || (xa >= 0 && xa < A.dims[0]);
+    const int32_t xl = inside ? xa : 0;
+    const bool dirty_xz = in_span(A, 0, xabs, Db.lo[0], Db.hi[0]) && in_span(A, 2, z, Db.lo[2], Db.hi[2]);
+    const size_t base = (size_t)zw * A.dims[1] * A.dims[0] + xl;
+    uint32_t g[kRows], o[kRows];
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) {   // all loads first, then the slots' dry flags they name
+        const size_t qi = base + (size_t)(j < ey ? j : ey - 1) * A.dims[0];
+        g[j] = grid[qi];
+        o[j] = d3[qi];
+    }
+    uint32_t a[kRows];
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) a[j] = dry_any[g[j] >= 256u ? g[j] - 256u : 0u];
+    MaskT solid = 0;
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) {   // arithmetic on every loaded flag (a select would sink each load into a branch)
+        const uint32_t det = (uint32_t)(g[j] >= 256u), uni = (uint32_t)(g[j] != 0u && g[j] != 5u);
+        const uint32_t sj = (a[j] & det) | (uni & (det ^ 1u));
+        solid |= (MaskT)(sj & 1u) << j;
+    }
+    const MaskT drows = Db.hi[1] >= kRows - 1 ? ~(MaskT)0 << Db.lo[1]
+                                              : ((((MaskT)1 << (Db.hi[1] + 1)) - (MaskT)1) & (~(MaskT)0 << Db.lo[1]));
     const MaskT rows = ey >= kRows ? ~(MaskT)0 : (((MaskT)1 << ey) - (MaskT)1);
-    MaskT S = occ;
-    MaskT c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    // x-pass input: pass 3's value in the dirty box (0 where solid), d3 elsewhere; outside the world: none
+    const MaskT indirty = dirty_xz ? drows : (MaskT)0;
+    MaskT occ = 0;
+    bool nonbinary = false;
 #pragma unroll
-    for (int32_t r = 0; r < 16; ++r) {
-        if (r > 0) {
-            const MaskT t = (S | (S << 1) | (S >> 1)) & rows;   // rows r - 1 -> r (no row outside the world)
-            S = t | lane_from_below(t) | lane_from_above(t);     // columns
-        }
-        MaskT carry = ~S, q;   // rows of this column not yet within r of an occupied cell
-        q = c0 & carry; c0 ^= carry; carry = q;
-        q = c1 & carry; c1 ^= carry; carry = q;
-        q = c2 & carry; c2 ^= carry; carry = q;
-        q = c3 & carry; c3 ^= carry; carry = q;
-        c4 ^= carry;
+    for (int32_t j = 0; j < kRows; ++j) {
+        const bool outside_old = ((indirty >> j) & 1) == 0;
+        occ |= (MaskT)(outside_old && o[j] == 0u ? 1u : 0u) << j;
+        nonbinary = nonbinary || (outside_old && j < ey && o[j] != 0u && o[j] != 16u);
     }
-    // lanes 16..47 hold the tile's columns: their results go through LDS as rows of 32-bit quads
-    const int32_t col = lane - 16;
-    if (col >= 0 && col < kDX) {
-        uint8_t *Db = reinterpret_cast<uint8_t *>(&D[0][0]);
+    occ = inside ? ((occ & ~indirty) | (solid & indirty)) & rows : (MaskT)0;
+    if (lane >= 16 && lane < 16 + kDX && dirty_xz) {   // pass 3's writes of this tile's columns
+        for (int32_t j = Db.lo[1]; j <= Db.hi[1]; ++j) d3[base + (size_t)j * A.dims[0]] = ((solid >> j) & 1) ? 0u : 16u;
+    }
+    if (__any(nonbinary)) {   // a d3 value that is not 0 / 16: the passes' taps
 #pragma unroll
-        for (int32_t row = 0; row < kRows; ++row) {
-            const uint32_t d = (uint32_t)((c0 >> row) & 1u) | ((uint32_t)((c1 >> row) & 1u) << 1) |
-                               ((uint32_t)((c2 >> row) & 1u) << 2) | ((uint32_t)((c3 >> row) & 1u) << 3) |
-                               ((uint32_t)((c4 >> row) & 1u) << 4);
-            Db[row * kDX + col] = (uint8_t)d;
+        for (int32_t j = 0; j < kRows; ++j) {
+            if (j >= ey) continue;
+            const bool dj = ((indirty >> j) & 1) != 0;
+            const uint32_t f = !inside ? kOut : (dj ? (((solid >> j) & 1) ? 0u : 16u) : o[j]);
+            F[j][lane] = (uint8_t)f;
         }
+        dist_xy_taps(A, x0, zw, ey, F, X, d1);
+        return;
     }
-    __syncthreads();
-    for (int32_t i = lane; i < ey * (kDX / 4); i += 64) {
-        const int32_t row = i / (kDX / 4), q = i - row * (kDX / 4);
-        if (A.dims[0] % 4 == 0) store_quad<true>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
-        else store_quad<false>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
-    }
+    MaskT c0, c1, c2, c3, c4;
+    dist_xy_square<MaskT>(occ, ey, c0, c1, c2, c3, c4);
+    dist_xy_emit<MaskT>(A, x0, zw, ey, c0, c1, c2, c3, c4, D, d1);
+}
+
+template <typename MaskT>
+__device__ __forceinline__ void dist_xy2_normal(const DistArgs &A, const uint32_t *grid, uint8_t *tmp, uint32_t (*D)[kDX / 4]) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    const int32_t ey = A.dims[1];
+    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kDX);
+    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
+    const int32_t lane = (int32_t)threadIdx.x;
+    int32_t xa = x0 - 16 + lane;
+    if (A.ring) xa &= A.dims[0] - 1;
+    const bool inside = A.ring || (xa >= 0 && xa < A.dims[0]);
+    const size_t base = (size_t)zw * A.dims[1] * A.dims[0] + (inside ? xa : 0);
+    uint32_t g[kRows];
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) g[j] = grid[base + (size_t)(j < ey ? j : ey - 1) * A.dims[0]];
+    MaskT occ = 0;
+#pragma unroll
+    for (int32_t j = 0; j < kRows; ++j) occ |= (MaskT)((g[j] != 0u ? 1u : 0u) & (uint32_t)(inside && j < ey)) << j;
+    MaskT c0, c1, c2, c3, c4;
+    dist_xy_square<MaskT>(occ, ey, c0, c1, c2, c3, c4);
+    dist_xy_emit<MaskT>(A, x0, zw, ey, c0, c1, c2, c3, c4, D, tmp);
+}
+
+template <typename MaskT>
+__global__ void __launch_bounds__(64) k_dist_xy2(DistArgs A, DirtyBox Db, const uint32_t *grid, const uint8_t *dry_any,
+                                                 uint8_t *d3, uint8_t *tmp, uint8_t *d1) {
+    constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
+    __shared__ uint8_t F[kRows][kDX + 32];
+    __shared__ uint8_t X[kRows][kDX];
+    __shared__ uint32_t D[kRows][kDX / 4];
+    if (blockIdx.z == 0) dist_xy2_normal<MaskT>(A, grid, tmp, D);
+    else dist_xy2_dry<MaskT>(A, Db, grid, dry_any, d3, d1, F, X, D);
 }
 
 constexpr int kZX = 64;   // x cells per z-pass tile (16 quads)
 constexpr int kZZ = 16;   // z cells per z-pass tile
 
 // brickdist.comp's z pass (record_distance_field passes 2 and 6) over a tile of kZX x cells x kZZ z cells at one
-// y: `src` (d1) over z +-16 staged in LDS as 16-bit values, one quad (4 x cells) per lane with packed 16-bit
-// max / min; written to `dst` (d0 / d2) over the box.  A tile whose window holds no value below 16 (open sky)
-// is 16 throughout: every tap's max(|k|, f) is >= 16.
+// y: the source over z +-16 staged in LDS as 16-bit values, one quad (4 x cells) per lane with packed 16-bit
+// max / min; written to `dst` over the box.  A window row inside the box's z span reads `src_in`, one outside
+// it `src_out` (one field for the seven-pass build; the two-launch build keeps the normal field's x+y values
+// apart from d1).  A tile whose window holds no value below 16 (open sky) is 16 throughout: every tap's
+// max(|k|, f) is >= 16.
 template <bool VEC>
-__global__ void __launch_bounds__(256) k_dist_z(DistArgs A, const uint8_t *src, uint8_t *dst) {
+__device__ __forceinline__ void dist_z_tile(const DistArgs &A, int32_t ztile, const uint8_t *src_in, const uint8_t *src_out,
+                                            uint8_t *dst) {
     constexpr int32_t Q = kZX / 4, ROWS = kZZ + 32, PER = ROWS * Q / 256;
     __shared__ uint2 F[ROWS][Q];
     const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kZX);
     const int32_t y = A.bmin[1] + (int32_t)blockIdx.y;
-    const int32_t z0 = A.bmin[2] + (int32_t)blockIdx.z * kZZ;
+    const int32_t z0 = A.bmin[2] + ztile * kZZ;
     const int32_t t = (int32_t)threadIdx.x;
     bool low = false;
     {
@@ -906,16 +1056,17 @@ __global__ void __launch_bounds__(256) k_dist_z(DistArgs A, const uint8_t *src, 
 #pragma unroll
         for (int32_t j = 0; j < PER; ++j) {   // every load issued before the first is used
             const int32_t i = t + 256 * j, row = i / Q, q = i - row * Q;
-            int32_t za = z0 - 16 + row;
+            const int32_t zabs = z0 - 16 + row;
+            int32_t za = zabs;
             if (A.ring) za &= A.dims[2] - 1;
             const bool zin = A.ring || (za >= 0 && za < A.dims[2]);
+            const uint8_t *src = in_span(A, 2, zabs, A.bmin[2], A.bmax[2]) ? src_in : src_out;
             v[j] = load_quad<VEC>(A, src, zin ? za : 0, y, x0 + 4 * q, zin);
         }
 #pragma unroll
         for (int32_t j = 0; j < PER; ++j) {
             const int32_t i = t + 256 * j, row = i / Q, q = i - row * Q;
             const uint32_t w = v[j];
-            // bytes < 16 (values are 0..16 or kOut): a set bit below bit 4 of the byte, or a zero byte
             low = low || ((w & 0xFFu) < 16u) || (((w >> 8) & 0xFFu) < 16u) || (((w >> 16) & 0xFFu) < 16u) || ((w >> 24) < 16u);
             F[row][q] = make_uint2((w & 0xFFu) | ((w & 0xFF00u) << 8), ((w >> 16) & 0xFFu) | ((w >> 8) & 0xFF0000u));
         }
@@ -942,6 +1093,20 @@ __global__ void __launch_bounds__(256) k_dist_z(DistArgs A, const uint8_t *src, 
     }
     store_quad<VEC>(A, dst, zw, y, x0 + 4 * q,
                     (uint32_t)b01.x | ((uint32_t)b01.y << 8) | ((uint32_t)b23.x << 16) | ((uint32_t)b23.y << 24));
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(256) k_dist_z(DistArgs A, const uint8_t *src, uint8_t *dst) {
+    dist_z_tile<VEC>(A, (int32_t)blockIdx.z, src, src, dst);
+}
+
+// Both fields' z passes in one launch (blockIdx.z = 2 x z tile + field): the normal field reads its x+y values
+// (`tmp`) inside the box and d1 outside it, as record_distance_field's pass 2 finds them, into d0; the dry
+// field reads d1 into d2 (pass 6).
+template <bool VEC>
+__global__ void __launch_bounds__(256) k_dist_z2(DistArgs A, const uint8_t *tmp, const uint8_t *d1, uint8_t *d0, uint8_t *d2) {
+    const bool dry = (blockIdx.z & 1u) != 0u;
+    dist_z_tile<VEC>(A, (int32_t)(blockIdx.z >> 1), dry ? d1 : tmp, d1, dry ? d2 : d0);
 }
 
 // Cell words over the whole grid: grid value, both distance fields and the coarse-cell flags.
@@ -1075,11 +1240,14 @@ namespace {
 
 void free_world(vf_ctx *ctx) {
     void *ptrs[] = {ctx->grid, ctx->cells, ctx->atlas, ctx->occ, ctx->occdry, ctx->masks_ref, ctx->dist[0], ctx->dist[1],
-                    ctx->dist[2], ctx->dist[3], ctx->c1, ctx->c2, ctx->skytop, ctx->palette, ctx->srcnz};
+                    ctx->dist[2], ctx->dist[3], ctx->c1, ctx->c2, ctx->skytop, ctx->palette, ctx->srcnz, ctx->dry_any,
+                    ctx->dist_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ctx->grid = nullptr; ctx->cells = nullptr; ctx->atlas = nullptr; ctx->occ = nullptr; ctx->occdry = nullptr;
     ctx->masks_ref = nullptr;
+    ctx->dry_any = nullptr;
+    ctx->dist_tmp = nullptr;
     for (auto &d : ctx->dist) d = nullptr;
     ctx->c1 = ctx->c2 = nullptr;
     ctx->srcnz = nullptr;
@@ -1185,6 +1353,26 @@ int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], 
     const ZKernel kz = wb[0] % 4 == 0 ? k_dist_z<true> : k_dist_z<false>;
     // a box over the world's whole height reads no row outside itself: the square-dilation count
     const bool full_height = lo[1] == 0 && hi[1] == wb[1] - 1 && wb[1] <= 64;
+    bool dirty_inside = true;
+    for (int a = 0; a < 3; ++a) dirty_inside = dirty_inside && dlo[a] >= lo[a] && dhi[a] <= hi[a] && dlo[a] <= dhi[a];
+    if (full_height && dirty_inside) {   // both fields in two launches: x+y, then z
+        DirtyBox Db;
+        for (int a = 0; a < 3; ++a) { Db.lo[a] = dlo[a]; Db.hi[a] = dhi[a]; }
+        const dim3 g2((unsigned)((exa + kDX - 1) / kDX), (unsigned)ez, 2u);
+        if (wb[1] <= 32)
+            hipLaunchKernelGGL(k_dist_xy2<uint32_t>, g2, dim3(64), 0, ctx->stream, A, Db, ctx->grid, ctx->dry_any, d[3],
+                               ctx->dist_tmp, d[1]);
+        else
+            hipLaunchKernelGGL(k_dist_xy2<uint64_t>, g2, dim3(64), 0, ctx->stream, A, Db, ctx->grid, ctx->dry_any, d[3],
+                               ctx->dist_tmp, d[1]);
+        if ((rc = launch_check(ctx, "brickdist x+y"))) return rc;
+        const dim3 gz2(gz.x, gz.y, gz.z * 2u);
+        if (wb[0] % 4 == 0)
+            hipLaunchKernelGGL(k_dist_z2<true>, gz2, dim3(256), 0, ctx->stream, A, ctx->dist_tmp, d[1], d[0], d[2]);
+        else
+            hipLaunchKernelGGL(k_dist_z2<false>, gz2, dim3(256), 0, ctx->stream, A, ctx->dist_tmp, d[1], d[0], d[2]);
+        return launch_check(ctx, "brickdist z");
+    }
     using MaskKernel = void (*)(DistArgs, const uint32_t *, const uint8_t *, uint8_t *);
     const MaskKernel mask_first = wb[1] <= 32 ? k_dist_xy_mask<true, uint32_t> : k_dist_xy_mask<true, uint64_t>;
     const MaskKernel mask_dry = wb[1] <= 32 ? k_dist_xy_mask<false, uint32_t> : k_dist_xy_mask<false, uint64_t>;
@@ -1416,6 +1604,8 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     VF_ALLOC(ctx->atlas, slots * 512);
     VF_ALLOC(ctx->occ, slots * 64);
     VF_ALLOC(ctx->occdry, slots * 64);
+    VF_ALLOC(ctx->dry_any, slots);
+    VF_ALLOC(ctx->dist_tmp, n);
     VF_ALLOC(ctx->masks_ref, slots * 64);
     for (auto &p : ctx->dist) VF_ALLOC(p, n);
     VF_ALLOC(ctx->c1, n1);
@@ -1429,6 +1619,7 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     VF_HIP(ctx, hipMemsetAsync(ctx->masks_ref, 0, slots * 64, ctx->stream));
     VF_HIP(ctx, hipMemsetAsync(ctx->occ, 0, slots * 64, ctx->stream));
     VF_HIP(ctx, hipMemsetAsync(ctx->occdry, 0, slots * 64, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->dry_any, 0, slots, ctx->stream));
     return VF_OK;
 }
 
@@ -1444,7 +1635,7 @@ int world_finish(vf_ctx *ctx, bool coarse_given, bool source_given) {
     }
     if (ctx->n_slots) {
         hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)ctx->n_slots * 16)), dim3(256), 0, ctx->stream, ctx->atlas, nullptr,
-                           0u, ctx->n_slots, ctx->occ, ctx->occdry);
+                           0u, ctx->n_slots, ctx->occ, ctx->occdry, ctx->dry_any);
         if ((rc = launch_check(ctx, "occupancy masks"))) return rc;
     }
     if (!coarse_given && (rc = rebuild_coarse(ctx))) return rc;
@@ -1483,7 +1674,7 @@ int build_coarse(vf_ctx *ctx, const uint32_t *d_grid) {
 int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n) {
     if (!n) return VF_OK;
     hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)n * 16)), dim3(256), 0, ctx->stream, ctx->atlas, d_slots, n, ctx->n_slots,
-                       ctx->occ, ctx->occdry);
+                       ctx->occ, ctx->occdry, ctx->dry_any);
     return launch_check(ctx, "occupancy masks");
 }
 
@@ -1740,7 +1931,7 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
         hipLaunchKernelGGL(k_update_slots, dim3(blocks_for((int64_t)n_slot_updates * 128)), dim3(256), 0, ctx->stream, ctx->atlas,
                            (const uint32_t *)d_si, (const uint8_t *)d_sv, n_slot_updates);
         hipLaunchKernelGGL(k_masks, dim3(blocks_for((int64_t)n_slot_updates * 16)), dim3(256), 0, ctx->stream, ctx->atlas,
-                           (const uint32_t *)d_si, n_slot_updates, ctx->n_slots, ctx->occ, ctx->occdry);
+                           (const uint32_t *)d_si, n_slot_updates, ctx->n_slots, ctx->occ, ctx->occdry, ctx->dry_any);
     }
     if ((rc = launch_check(ctx, "world update"))) { cleanup(); return rc; }
     if ((rc = rebuild_coarse(ctx)) || (rc = repack(ctx)) || (rc = rebuild_skytop(ctx))) { cleanup(); return rc; }
