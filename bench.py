@@ -86,10 +86,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the open-forest, fixed-world, GI-batch, learned-cache and uneven-batch lines")
-    ap.add_argument("--extras", default="temporal,pack,forest,other",
-                    help="comma list of the extra measurements: temporal (the passes after the trace), pack (the "
-                         "world packed on the device), forest (open forest with bodies), other (fixed world, GI "
-                         "batch, GI cache, probes, uneven batch)")
+    ap.add_argument("--extras", default="temporal,dist,pack,forest,other",
+                    help="comma list of the extra measurements: temporal (the passes after the trace), dist (the "
+                         "brick distance field over the whole world), pack (the world packed on the device), forest "
+                         "(open forest with bodies), other (fixed world, GI batch, GI cache, probes, uneven batch)")
     ap.add_argument("--no-graph", action="store_true",
                     help="record no graphs at all (also the kernel-duration loop launches eagerly)")
     ap.add_argument("--graph-steps", action="store_true",
@@ -285,6 +285,8 @@ def main():
         extras = set(args.extras.split(","))
         if "temporal" in extras:
             result["temporal_1080p"] = temporal_config(ctx, stream, args, pu)
+        if "dist" in extras:
+            result["brickdist_full"] = dist_config(ctx, world)
         if "pack" in extras:
             result["world_pack"] = pack_config(ctx, world, scene)
         if "forest" in extras:
@@ -293,6 +295,33 @@ def main():
             result["other_configs"] = other_configs(ctx, args)
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def dist_config(ctx, world, reps=20):
+    """record_distance_field over the whole headline world (both fields), timed with HIP events around its
+    passes (vf_last_kernel_ms).  Bytes: the passes' own traffic, 15 B per brick (normal field: grid 4 B read,
+    the x, y, z passes' reads and writes 5 B; dry field: occupancy 1 B + 5 B), and the same plus the
+    reference's dry-occupancy scan of 512 atlas bytes per detailed brick, which this build replaces by a
+    per-slot flag kept with the occupancy masks."""
+    d = world.desc()
+    wb, win = tuple(d.world_bricks), tuple(d.world_win)
+    lo = [win[0], 0, win[2]]
+    hi = [win[0] + wb[0] - 1, wb[1] - 1, win[2] + wb[2] - 1]
+    ctx.build_brickdist(lo, hi)
+    ms = []
+    for _ in range(reps):
+        ctx.build_brickdist(lo, hi)
+        ms.append(ctx.last_kernel_ms())
+    t = float(np.median(ms))
+    n = wb[0] * wb[1] * wb[2]
+    detailed = int((np.asarray(world.arrays()["grid"], np.uint32) >= 256).sum())
+    own, scan = 15 * n, 15 * n + 512 * detailed
+    res = {"bricks": n, "detailed_bricks": detailed, "ms": round(t, 5), "launches": 2,
+           "pass_bytes": own, "pass_GB/s": round(own / (t * 1e-3) / 1e9, 1), "pass_frac": round(own / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "with_reference_scan_bytes": scan,
+           "with_reference_scan_frac": round(scan / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"distance field: {res}")
+    return res
 
 
 def pack_config(ctx, world, scene):

@@ -246,7 +246,8 @@ int vf_world_update(vf_ctx *ctx, const uint32_t *cell_idx, const uint32_t *cell_
                     const uint32_t *slot_idx, const uint8_t *slot_vox, uint32_t n_slot_updates);
 /* GpuWorld::record_distance_field (gpu_world.rs:1376-1428): the separable L-inf brick distance
  * field (brickdist.comp) over bricks [lo, hi] and the dry occupancy over [dirty_lo, dirty_hi],
- * inclusive, absolute brick coordinates.  Also refreshes the coarse pyramid and sky-top columns. */
+ * inclusive, absolute brick coordinates.  Then refreshes the cell words the traversal reads.  vf_last_kernel_ms
+ * afterwards gives the distance-field passes' time. */
 int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dirty_lo[3],
                        const int32_t dirty_hi[3]);
 /* Read back the two distance fields the traversal uses (brickDistTex, brickDistDryTex), host. */

@@ -794,12 +794,19 @@ __device__ __forceinline__ uint64_t lane_from_above(uint64_t v) {
     return (uint64_t)lane_from_above((uint32_t)v) | ((uint64_t)lane_from_above((uint32_t)(v >> 32)) << 32);
 }
 
+// LDS written by one wavefront and read by its other lanes: ordered within the wavefront, and visible once its
+// own LDS operations are done (each wavefront of these kernels has LDS of its own, so no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The square-dilation count of one tile (lane j = window column x0 - 16 + j, MaskT bits = rows) into 5 counter
 // planes, then the result bytes through LDS as rows of 32-bit quads to `dst` over the box.
 template <typename MaskT>
 __device__ __forceinline__ void dist_xy_square(MaskT occ, int32_t ey, MaskT &c0, MaskT &c1, MaskT &c2, MaskT &c3, MaskT &c4) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
-    const int32_t lane = (int32_t)threadIdx.x;
     const MaskT rows = ey >= kRows ? ~(MaskT)0 : (((MaskT)1 << ey) - (MaskT)1);
     MaskT S = occ;
     c0 = 0; c1 = 0; c2 = 0; c3 = 0; c4 = 0;
@@ -816,14 +823,13 @@ __device__ __forceinline__ void dist_xy_square(MaskT occ, int32_t ey, MaskT &c0,
         q = c3 & carry; c3 ^= carry; carry = q;
         c4 ^= carry;
     }
-    (void)lane;
 }
 
 template <typename MaskT>
 __device__ __forceinline__ void dist_xy_emit(const DistArgs &A, int32_t x0, int32_t zw, int32_t ey, MaskT c0, MaskT c1,
                                              MaskT c2, MaskT c3, MaskT c4, uint32_t (*D)[kDX / 4], uint8_t *dst) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
-    const int32_t lane = (int32_t)threadIdx.x, col = lane - 16;
+    const int32_t lane = (int32_t)(threadIdx.x & 63u), col = lane - 16;
     if (col >= 0 && col < kDX) {   // lanes 16..47 hold the tile's columns
         uint8_t *Db = reinterpret_cast<uint8_t *>(&D[0][0]);
 #pragma unroll
@@ -834,7 +840,7 @@ __device__ __forceinline__ void dist_xy_emit(const DistArgs &A, int32_t x0, int3
             Db[row * kDX + col] = (uint8_t)d;
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     for (int32_t i = lane; i < ey * (kDX / 4); i += 64) {
         const int32_t row = i / (kDX / 4), q = i - row * (kDX / 4);
         if (A.dims[0] % 4 == 0) store_quad<true>(A, dst, zw, row, x0 + 4 * q, D[row][q]);
@@ -845,13 +851,13 @@ __device__ __forceinline__ void dist_xy_emit(const DistArgs &A, int32_t x0, int3
 // The passes' taps over LDS for a tile whose x-pass input is not 0 / 16 (F: the window values).
 __device__ __forceinline__ void dist_xy_taps(const DistArgs &A, int32_t x0, int32_t zw, int32_t ey, const uint8_t (*F)[kDX + 32],
                                              uint8_t (*X)[kDX], uint8_t *dst) {
-    const int32_t lane = (int32_t)threadIdx.x;
-    __syncthreads();
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    wave_lds_sync();
     for (int32_t i = lane; i < ey * kDX; i += 64) {
         const int32_t row = i / kDX, col = i - row * kDX;
         X[row][col] = (uint8_t)dist_x(F[row], 0ull, false, col);
     }
-    __syncthreads();
+    wave_lds_sync();
     for (int32_t i = lane; i < ey * kDX; i += 64) {
         const int32_t row = i / kDX, col = i - row * kDX;
         const int32_t xc = x0 + col;
@@ -932,15 +938,15 @@ __device__ __forceinline__ bool in_span(const DistArgs &A, int a, int32_t v, int
 // dry occupancy field `d3` as it stands outside it; the pass-3 values of the tile's cells in the dirty box are
 // written to d3; the x+y result goes to d1.
 template <typename MaskT>
-__device__ __forceinline__ void dist_xy2_dry(const DistArgs &A, const DirtyBox &Db, const uint32_t *grid,
-                                             const uint8_t *dry_any, uint8_t *d3, uint8_t *d1, uint8_t (*F)[kDX + 32],
-                                             uint8_t (*X)[kDX], uint32_t (*D)[kDX / 4]) {
+__device__ __forceinline__ void dist_xy2_dry(const DistArgs &A, const DirtyBox &Db, int32_t tx, int32_t tz,
+                                             const uint32_t *grid, const uint8_t *dry_any, uint8_t *d3, uint8_t *d1,
+                                             uint8_t (*F)[kDX + 32], uint8_t (*X)[kDX], uint32_t (*D)[kDX / 4]) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
     const int32_t ey = A.dims[1];
-    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kDX);
-    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t x0 = dist_tile_x0(A, tx, kDX);
+    const int32_t z = A.bmin[2] + tz;
     const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
-    const int32_t lane = (int32_t)threadIdx.x;
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
     const int32_t xabs = x0 - 16 + lane;
     int32_t xa = xabs;
     if (A.ring) xa &= A.dims[0] - 1;
@@ -999,13 +1005,14 @@ __device__ __forceinline__ void dist_xy2_dry(const DistArgs &A, const DirtyBox &
 }
 
 template <typename MaskT>
-__device__ __forceinline__ void dist_xy2_normal(const DistArgs &A, const uint32_t *grid, uint8_t *tmp, uint32_t (*D)[kDX / 4]) {
+__device__ __forceinline__ void dist_xy2_normal(const DistArgs &A, int32_t tx, int32_t tz, const uint32_t *grid, uint8_t *tmp,
+                                                uint32_t (*D)[kDX / 4]) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
     const int32_t ey = A.dims[1];
-    const int32_t x0 = dist_tile_x0(A, (int32_t)blockIdx.x, kDX);
-    const int32_t z = A.bmin[2] + (int32_t)blockIdx.y;
+    const int32_t x0 = dist_tile_x0(A, tx, kDX);
+    const int32_t z = A.bmin[2] + tz;
     const int32_t zw = A.ring ? (z & (A.dims[2] - 1)) : z;
-    const int32_t lane = (int32_t)threadIdx.x;
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
     int32_t xa = x0 - 16 + lane;
     if (A.ring) xa &= A.dims[0] - 1;
     const bool inside = A.ring || (xa >= 0 && xa < A.dims[0]);
@@ -1021,15 +1028,22 @@ __device__ __forceinline__ void dist_xy2_normal(const DistArgs &A, const uint32_
     dist_xy_emit<MaskT>(A, x0, zw, ey, c0, c1, c2, c3, c4, D, tmp);
 }
 
+// One workgroup of two wavefronts per tile: wavefront 0 the normal field, 1 the dry field (the second one's grid
+// loads hit the cache the first one filled).  Workgroups are dealt round-robin to the 8 XCDs; each XCD gets a
+// contiguous run of tiles (whole z slices), so the windows that neighbouring tiles share stay in one L2.
 template <typename MaskT>
-__global__ void __launch_bounds__(64) k_dist_xy2(DistArgs A, DirtyBox Db, const uint32_t *grid, const uint8_t *dry_any,
-                                                 uint8_t *d3, uint8_t *tmp, uint8_t *d1) {
+__global__ void __launch_bounds__(128) k_dist_xy2(DistArgs A, DirtyBox Db, int32_t tiles_x, int32_t n_tiles,
+                                                  const uint32_t *grid, const uint8_t *dry_any, uint8_t *d3, uint8_t *tmp,
+                                                  uint8_t *d1) {
     constexpr int32_t kRows = (int32_t)sizeof(MaskT) * 8;
     __shared__ uint8_t F[kRows][kDX + 32];
     __shared__ uint8_t X[kRows][kDX];
-    __shared__ uint32_t D[kRows][kDX / 4];
-    if (blockIdx.z == 0) dist_xy2_normal<MaskT>(A, grid, tmp, D);
-    else dist_xy2_dry<MaskT>(A, Db, grid, dry_any, d3, d1, F, X, D);
+    __shared__ uint32_t D[2][kRows][kDX / 4];
+    const int32_t L = (int32_t)blockIdx.x, full = (n_tiles / 8) * 8;
+    const int32_t T = L < full ? (L & 7) * (full / 8) + (L >> 3) : L;
+    const int32_t tx = T % tiles_x, tz = T / tiles_x;
+    if (threadIdx.x < 64) dist_xy2_normal<MaskT>(A, tx, tz, grid, tmp, D[0]);
+    else dist_xy2_dry<MaskT>(A, Db, tx, tz, grid, dry_any, d3, d1, F, X, D[1]);
 }
 
 constexpr int kZX = 64;   // x cells per z-pass tile (16 quads)
@@ -1358,13 +1372,13 @@ int distance_field(vf_ctx *ctx, const int32_t lo_in[3], const int32_t hi_in[3], 
     if (full_height && dirty_inside) {   // both fields in two launches: x+y, then z
         DirtyBox Db;
         for (int a = 0; a < 3; ++a) { Db.lo[a] = dlo[a]; Db.hi[a] = dhi[a]; }
-        const dim3 g2((unsigned)((exa + kDX - 1) / kDX), (unsigned)ez, 2u);
+        const int32_t tiles_x = (exa + kDX - 1) / kDX, n_tiles = tiles_x * ez;
         if (wb[1] <= 32)
-            hipLaunchKernelGGL(k_dist_xy2<uint32_t>, g2, dim3(64), 0, ctx->stream, A, Db, ctx->grid, ctx->dry_any, d[3],
-                               ctx->dist_tmp, d[1]);
+            hipLaunchKernelGGL(k_dist_xy2<uint32_t>, dim3((unsigned)n_tiles), dim3(128), 0, ctx->stream, A, Db, tiles_x, n_tiles,
+                               ctx->grid, ctx->dry_any, d[3], ctx->dist_tmp, d[1]);
         else
-            hipLaunchKernelGGL(k_dist_xy2<uint64_t>, g2, dim3(64), 0, ctx->stream, A, Db, ctx->grid, ctx->dry_any, d[3],
-                               ctx->dist_tmp, d[1]);
+            hipLaunchKernelGGL(k_dist_xy2<uint64_t>, dim3((unsigned)n_tiles), dim3(128), 0, ctx->stream, A, Db, tiles_x, n_tiles,
+                               ctx->grid, ctx->dry_any, d[3], ctx->dist_tmp, d[1]);
         if ((rc = launch_check(ctx, "brickdist x+y"))) return rc;
         const dim3 gz2(gz.x, gz.y, gz.z * 2u);
         if (wb[0] % 4 == 0)
@@ -2002,10 +2016,10 @@ int vf_build_brickdist(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], co
     for (int a = 0; a < 3; ++a)
         if (hi[a] - lo[a] > (1 << 20) || dhi[a] - dlo[a] > (1 << 20))
             return vf::fail(ctx, VF_E_INVALID, "vf_build_brickdist: box too large");
-    if ((rc = begin_timed(ctx))) return rc;
+    if ((rc = begin_timed(ctx))) return rc;   // the timed region: the distance-field passes
     if ((rc = distance_field(ctx, lo, hi, dlo, dhi))) return rc;
-    if ((rc = repack(ctx))) return rc;
-    return end_timed(ctx);
+    if ((rc = end_timed(ctx))) return rc;
+    return repack(ctx);
 }
 
 int vf_read_brickdist(vf_ctx *ctx, uint8_t *dist, uint8_t *dry) {
