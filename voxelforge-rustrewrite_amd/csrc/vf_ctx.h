@@ -116,6 +116,7 @@ int masks_for(vf_ctx *ctx, const uint32_t *d_slots, uint32_t n);
 int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const int32_t dlo[3], const int32_t dhi[3]);
 int refresh_world(vf_ctx *ctx);
 int skytop_columns(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t nx, int32_t nz);
+int skytop_windows(vf_ctx *ctx, const int32_t *x0, const int32_t *z0, int32_t n, int32_t nx, int32_t nz);
 int cell_words(vf_ctx *ctx);
 int skytop_box(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t x1, int32_t z1);
 }  // namespace vf

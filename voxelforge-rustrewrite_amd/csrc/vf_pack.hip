@@ -378,25 +378,39 @@ __global__ void k_sync_occ(const int32_t *written, int64_t n_words, const uint32
     out[i] = s >= 0 ? occ[(size_t)s * 16u + (size_t)(i & 15)] : 0u;
 }
 
+// Up to 64 chunks of one sync, one per blockIdx.y: their grid origins (bricks, wrapped into the ring) and the
+// offset of their cells in the staged cell block (-1: an unloaded chunk). A sync's chunks sit in distinct ring
+// slots, so their grid cells and coarse cells are disjoint.
+struct SyncChunks {
+    int32_t n;
+    int2 o[64];
+    int64_t cells[64];
+};
+
 // an unloaded chunk's grid cells -> 0
-__global__ void k_sync_clear(const SyncArgs A, int2 o, uint32_t *grid, uint8_t *srcnz) {
+__global__ void k_sync_clear(const SyncArgs A, const SyncChunks C, uint32_t *grid, uint8_t *srcnz) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.cells_per_chunk) return;
+    const int32_t w = (int32_t)blockIdx.y;
+    if (k >= A.cells_per_chunk || w >= C.n || C.cells[w] >= 0) return;
+    const int2 o = C.o[w];
     const int32_t x = (int32_t)(k % kChunkB), y = (int32_t)((k / kChunkB) % A.wby), z = (int32_t)(k / ((int64_t)kChunkB * A.wby));
     const size_t pos = ((size_t)(o.y + z) * A.wby + y) * A.wbx + (size_t)(o.x + x);
     grid[pos] = 0u;
     srcnz[pos] = 0u;
 }
 
-// coarse_cell1 (gpu_world.rs:944-957) of the chunk's 4^3-brick cells from its own cells (cells == NULL:
-// an unloaded chunk, all 0)
-__global__ void k_sync_coarse1(const SyncArgs A, int2 o, const uint32_t *cells, int32_t c1x, int32_t c1y, uint8_t *c1) {
+// coarse_cell1 (gpu_world.rs:944-957) of each chunk's 4^3-brick cells from its own cells (an unloaded chunk: all 0)
+__global__ void k_sync_coarse1(const SyncArgs A, const SyncChunks C, const uint32_t *staged, int32_t c1x, int32_t c1y,
+                               uint8_t *c1) {
     const int32_t per = (kChunkB / 4) * c1y * (kChunkB / 4);
     const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= per) return;
+    const int32_t w = (int32_t)blockIdx.y;
+    if (k >= per || w >= C.n) return;
+    const int2 o = C.o[w];
     const int32_t cx = k % 4, cy = (k / 4) % c1y, cz = k / (4 * c1y);
     uint8_t occ = 0u;
-    if (cells) {
+    if (C.cells[w] >= 0) {
+        const uint32_t *cells = staged + C.cells[w];
         for (int32_t z = cz * 4; z < cz * 4 + 4 && !occ; ++z)
             for (int32_t y = cy * 4; y < min(cy * 4 + 4, A.wby) && !occ; ++y)
                 for (int32_t x = cx * 4; x < cx * 4 + 4; ++x)
@@ -405,12 +419,13 @@ __global__ void k_sync_coarse1(const SyncArgs A, int2 o, const uint32_t *cells, 
     c1[((size_t)(o.y / 4 + cz) * c1y + cy) * c1x + (size_t)(o.x / 4 + cx)] = occ;
 }
 
-// coarse_cell2 (gpu_world.rs:959-973) of the chunk's column of 16^3-brick cells from coarse1
-__global__ void k_sync_coarse2(int2 o, int32_t c1x, int32_t c1y, int32_t c1z, int32_t c2x, int32_t c2y, const uint8_t *c1,
-                               uint8_t *c2) {
+// coarse_cell2 (gpu_world.rs:959-973) of each chunk's column of 16^3-brick cells from coarse1
+__global__ void k_sync_coarse2(const SyncChunks C, int32_t c1x, int32_t c1y, int32_t c1z, int32_t c2x, int32_t c2y,
+                               const uint8_t *c1, uint8_t *c2) {
     const int32_t cy = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (cy >= c2y) return;
-    const int32_t cx = o.x / 16, cz = o.y / 16;
+    const int32_t w = (int32_t)blockIdx.y;
+    if (cy >= c2y || w >= C.n) return;
+    const int32_t cx = C.o[w].x / 16, cz = C.o[w].y / 16;
     uint8_t occ = 0u;
     for (int32_t z = cz * 4; z < min(cz * 4 + 4, c1z) && !occ; ++z)
         for (int32_t y = cy * 4; y < min(cy * 4 + 4, c1y) && !occ; ++y)
@@ -590,17 +605,25 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
                                (const uint32_t *)ctx->occ, gocc);
         }
     }
-    // cleared chunks, coarse cells of every synced chunk
+    // cleared chunks, coarse cells of every synced chunk: 64 chunks per launch
     int64_t li = 0;
-    for (const vf_chunk *c : list) {
-        const int2 o = make_int2((c->cx * kChunkB) & (W.wb[0] - 1), (c->cz * kChunkB) & (W.wb[2] - 1));
-        const uint32_t *cells = nullptr;
-        if (c->loaded) cells = d_cells + (li++) * per;
-        else hipLaunchKernelGGL(k_sync_clear, dim3(vf::blocks_for(per)), dim3(256), 0, st, A, o, ctx->grid, ctx->srcnz);
-        const int32_t c1per = (kChunkB / 4) * ctx->c1d[1] * (kChunkB / 4);
-        hipLaunchKernelGGL(k_sync_coarse1, dim3(vf::blocks_for(c1per)), dim3(256), 0, st, A, o, cells, ctx->c1d[0], ctx->c1d[1], ctx->c1);
-        hipLaunchKernelGGL(k_sync_coarse2, dim3(vf::blocks_for(ctx->c2d[1])), dim3(256), 0, st, o, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2],
-                           ctx->c2d[0], ctx->c2d[1], (const uint8_t *)ctx->c1, ctx->c2);
+    const int32_t c1per = (kChunkB / 4) * ctx->c1d[1] * (kChunkB / 4);
+    for (size_t c0 = 0; c0 < list.size(); c0 += 64) {
+        SyncChunks C;
+        C.n = (int32_t)std::min<size_t>(64, list.size() - c0);
+        bool any_clear = false;
+        for (int32_t w = 0; w < C.n; ++w) {
+            const vf_chunk *c = list[c0 + (size_t)w];
+            C.o[w] = make_int2((c->cx * kChunkB) & (W.wb[0] - 1), (c->cz * kChunkB) & (W.wb[2] - 1));
+            C.cells[w] = c->loaded ? (li++) * per : -1;
+            any_clear |= !c->loaded;
+        }
+        const dim3 g1(vf::blocks_for(per), (unsigned)C.n), gc1(vf::blocks_for(c1per), (unsigned)C.n),
+            gc2(vf::blocks_for(ctx->c2d[1]), (unsigned)C.n);
+        if (any_clear) hipLaunchKernelGGL(k_sync_clear, g1, dim3(256), 0, st, A, C, ctx->grid, ctx->srcnz);
+        hipLaunchKernelGGL(k_sync_coarse1, gc1, dim3(256), 0, st, A, C, (const uint32_t *)d_cells, ctx->c1d[0], ctx->c1d[1], ctx->c1);
+        hipLaunchKernelGGL(k_sync_coarse2, gc2, dim3(256), 0, st, C, ctx->c1d[0], ctx->c1d[1], ctx->c1d[2], ctx->c2d[0],
+                           ctx->c2d[1], (const uint8_t *)ctx->c1, ctx->c2);
     }
     if ((rc = vf::launch_check(ctx, "chunk sync"))) return rc;
     // the distance field over each chunk's box +-16 bricks (gpu_world.rs:1355-1361), then cell words and sky-top
@@ -611,9 +634,14 @@ extern "C" int vf_world_sync_chunks(vf_ctx *ctx, const vf_chunk *chunks, int32_t
         if ((rc = vf::distance_box(ctx, flo, fhi, lo, hi))) return rc;
     }
     if ((rc = vf::cell_words(ctx))) return rc;
-    for (const vf_chunk *c : list) {   // record_grass_top over each chunk's columns (gpu_world.rs:1358)
-        const int32_t tx = (c->cx * kChunkB) & (W.wb[0] - 1), tz = (c->cz * kChunkB) & (W.wb[2] - 1);
-        if ((rc = vf::skytop_columns(ctx, tx * 8, tz * 8, kChunkB * 8, kChunkB * 8))) return rc;
+    {   // record_grass_top over each chunk's columns (gpu_world.rs:1358): one ring slot each, so the windows are
+        // disjoint and go in one launch
+        std::vector<int32_t> wx, wz;
+        for (const vf_chunk *c : list) {
+            wx.push_back(((c->cx * kChunkB) & (W.wb[0] - 1)) * 8);
+            wz.push_back(((c->cz * kChunkB) & (W.wb[2] - 1)) * 8);
+        }
+        if ((rc = vf::skytop_windows(ctx, wx.data(), wz.data(), (int32_t)wx.size(), kChunkB * 8, kChunkB * 8))) return rc;
     }
     // bookkeeping: the host grid copy, the slots popped and handed out, world-owned slots, the caller's gpu_slot
     // back through the staging block: slots over the brick voxels' place, occupancy rows after them

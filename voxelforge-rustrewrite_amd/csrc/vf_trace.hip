@@ -1141,27 +1141,55 @@ __global__ void __launch_bounds__(256) k_pack(const uint32_t *grid, const uint8_
 // Sky-top map (grasstop.comp:121-146 skyTop): 1 + the highest voxel of the column whose material
 // kind is neither water (1) nor foliage (3); 0 if none.  Columns in grid (wrapped) coordinates.
 // One thread per column of the window [x0, x0 + nx) x [z0, z0 + nz) (voxels).
+__device__ __forceinline__ void skytop_column(const uint32_t *grid, const uint8_t *atlas, const float4 *palette, int32_t wbx,
+                                              int32_t wby, int32_t wbz, int32_t xa, int32_t za, uint16_t *skytop) {
+    // grid (wrapped) column: a ring world's window may start at negative absolute voxels and cross its edge
+    const int32_t sx = wbx * 8, sz = wbz * 8;
+    const int32_t x = (xa % sx + sx) % sx, z = (za % sz + sz) % sz;
+    const int64_t i = (int64_t)z * sx + x;
+    const uint32_t *col = grid + (size_t)(z >> 3) * wby * wbx + (x >> 3);
+    uint32_t sky = 0u;
+    // the column's bricks 8 at a time, top down: their grid words are loaded together (one wait per group)
+    for (int32_t top = wby - 1; top >= 0 && sky == 0u; top -= 8) {
+        uint32_t g[8];
+#pragma unroll
+        for (int32_t k = 0; k < 8; ++k) g[k] = col[(size_t)max(top - k, 0) * wbx];
+        for (int32_t k = 0; k < 8 && sky == 0u; ++k) {
+            const int32_t by = top - k;
+            if (by < 0 || g[k] == 0u) continue;
+            for (int32_t ly = 7; ly >= 0; --ly) {
+                const uint32_t m = g[k] < 256u ? g[k] : atlas[(size_t)(g[k] - 256u) * 512u + (uint32_t)((z & 7) * 64 + ly * 8 + (x & 7))];
+                if (m == 0u) continue;
+                const float kind = palette[m * 3u + 1u].z;
+                if (kind != 1.0f && kind != 3.0f) { sky = (uint32_t)(by * 8 + ly + 1); break; }
+            }
+        }
+    }
+    skytop[i] = (uint16_t)sky;
+}
+
 __global__ void __launch_bounds__(256) k_skytop(const uint32_t *grid, const uint8_t *atlas, const float4 *palette,
                                                 int32_t wbx, int32_t wby, int32_t wbz, int32_t x0, int32_t z0, int32_t nx,
                                                 int32_t nz, uint16_t *skytop) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= (int64_t)nx * nz) return;
-    // grid (wrapped) column: a ring world's window may start at negative absolute voxels and cross its edge
-    const int32_t sx = wbx * 8, sz = wbz * 8;
-    const int32_t x = ((x0 + (int32_t)(j % nx)) % sx + sx) % sx, z = ((z0 + (int32_t)(j / nx)) % sz + sz) % sz;
-    const int64_t i = (int64_t)z * sx + x;
-    uint32_t sky = 0u;
-    for (int32_t by = wby - 1; by >= 0 && sky == 0u; --by) {
-        const uint32_t g = grid[((size_t)(z >> 3) * wby + by) * wbx + (x >> 3)];
-        if (g == 0u) continue;
-        for (int32_t ly = 7; ly >= 0; --ly) {
-            const uint32_t m = g < 256u ? g : atlas[(size_t)(g - 256u) * 512u + (uint32_t)((z & 7) * 64 + ly * 8 + (x & 7))];
-            if (m == 0u) continue;
-            const float kind = palette[m * 3u + 1u].z;
-            if (kind != 1.0f && kind != 3.0f) { sky = (uint32_t)(by * 8 + ly + 1); break; }
-        }
-    }
-    skytop[i] = (uint16_t)sky;
+    skytop_column(grid, atlas, palette, wbx, wby, wbz, x0 + (int32_t)(j % nx), z0 + (int32_t)(j / nx), skytop);
+}
+
+// The same over several windows of equal size in one launch (blockIdx.y = window; the windows of one call must
+// not share a column).
+struct SkyWindows {
+    int32_t n, nx, nz;
+    int32_t x0[64], z0[64];
+};
+__global__ void __launch_bounds__(256) k_skytop_windows(const uint32_t *grid, const uint8_t *atlas, const float4 *palette,
+                                                        int32_t wbx, int32_t wby, int32_t wbz, SkyWindows Wn,
+                                                        uint16_t *skytop) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int32_t w = (int32_t)blockIdx.y;
+    if (j >= (int64_t)Wn.nx * Wn.nz || w >= Wn.n) return;
+    skytop_column(grid, atlas, palette, wbx, wby, wbz, Wn.x0[w] + (int32_t)(j % Wn.nx), Wn.z0[w] + (int32_t)(j / Wn.nx),
+                  skytop);
 }
 
 __global__ void k_update_cells(uint32_t *grid, uint8_t *srcnz, const uint32_t *idx, const uint32_t *val, uint32_t n) {
@@ -1699,6 +1727,22 @@ int distance_box(vf_ctx *ctx, const int32_t lo[3], const int32_t hi[3], const in
 
 // The sky-top map over the voxel columns [x0, x0 + nx) x [z0, z0 + nz) (grid coordinates).
 int skytop_columns(vf_ctx *ctx, int32_t x0, int32_t z0, int32_t nx, int32_t nz) { return skytop_window(ctx, x0, z0, nx, nz); }
+
+// The sky-top map over n windows of nx x nz columns at (x0[k], z0[k]) that share no column, 64 per launch.
+int skytop_windows(vf_ctx *ctx, const int32_t *x0, const int32_t *z0, int32_t n, int32_t nx, int32_t nz) {
+    const int32_t *wb = ctx->W.wb;
+    for (int32_t k0 = 0; k0 < n; k0 += 64) {
+        SkyWindows Wn;
+        Wn.n = std::min(64, n - k0);
+        Wn.nx = nx;
+        Wn.nz = nz;
+        for (int32_t k = 0; k < Wn.n; ++k) { Wn.x0[k] = x0[k0 + k]; Wn.z0[k] = z0[k0 + k]; }
+        hipLaunchKernelGGL(k_skytop_windows, dim3(blocks_for((int64_t)nx * nz), (unsigned)Wn.n), dim3(256), 0, ctx->stream,
+                           ctx->grid, ctx->atlas, ctx->palette, wb[0], wb[1], wb[2], Wn, ctx->skytop);
+        if (int rc = launch_check(ctx, "sky-top")) return rc;
+    }
+    return VF_OK;
+}
 
 // The cell words (grid value, distance fields, coarse flags) of the whole grid.
 int cell_words(vf_ctx *ctx) { return repack(ctx); }
