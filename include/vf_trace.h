@@ -33,7 +33,7 @@ enum {
     VF_E_DEVICE = -2,      /* HIP runtime error */
     VF_E_NO_WORLD = -3,    /* a trace before vf_world_upload */
     VF_E_OOM = -4,         /* device allocation failed */
-    VF_E_UNSUPPORTED = -5  /* a feature outside this library (e.g. bodies in the persistent-queue batches) */
+    VF_E_UNSUPPORTED = -5  /* a feature outside this library (e.g. the allocation-free batch on a device it cannot reach) */
 };
 
 /* ---- layouts shared with the reference host ------------------------------------------------ */
@@ -382,7 +382,9 @@ int vf_queue_defaults(vf_queue_cfg *cfg);
  * sunDir, no cap) per GI hit that is not water and faces the sun.  Both go through the persistent
  * queue kernel.  gi_hits / sun_hits: one record per pixel (miss where no ray).  Not a reference
  * entry point: it batches what P::Gi traces, for incoherent-ray throughput. */
-/* The persistent-queue entry points trace the world only: a scene with bodies is VF_E_UNSUPPORTED. */
+/* With bodies in the scene (GI_OBJECTS, common.glsl:780, 2267-2271) a GI ray also meets the bodies nearer than
+ * min(its world hit or reach, 16); a body hit's sun ray leaves along the body's rotated normal; a sun ray that
+ * missed the world meets any body.  Body hits carry the body flag. */
 int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, float reach,
                       const float *d_gb_pos, const uint32_t *d_gb_normal, vf_hit *d_gi_hits, vf_hit *d_sun_hits,
                       const vf_queue_cfg *cfg);
@@ -390,7 +392,8 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32
 #define VF_MAX_REQUESTS 24
 enum { VF_REQ_VIEW = 0, VF_REQ_PROBES = 1 };
 /* One request of an uneven batch: a camera view (primary rays as vf_trace_primary, hits [w*h]) or a
- * DDGI probe grid (as vf_trace_probes, hits [n_probes*n_rays]). */
+ * DDGI probe grid (as vf_trace_probes, hits [n_probes*n_rays]).  With bodies in the scene both meet them
+ * as those entry points do (traceSceneS for a view, GI_ADD_OBJECTS for a probe ray). */
 typedef struct vf_request {
     int32_t kind;
     int32_t width, height;

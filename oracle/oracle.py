@@ -534,15 +534,21 @@ def gi_rays(width, height, frame, gb_pos, gb_normal):
     return rays, valid.astype(bool)
 
 
-def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_counts=False):
+def _body_record(b):
+    return ((b.cell[0], b.cell[1], b.cell[2]), b.t, b.mat, face_code(b.n), 1, 1)
+
+
+def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_counts=False, bodies=False):
     """The GI batch on the oracle: GI rays (t_cap = reach, skipMat 0), then for each hit that is not
-    water and faces the sun, its sun ray from hit + N*0.03 (giPathImpl stage 1).  Per-pixel hits."""
+    water and faces the sun, its sun ray from hit + N*0.03 (giPathImpl stage 1).  Per-pixel hits.
+    bodies (GI_OBJECTS, common.glsl:780, 2267-2271): a GI ray also meets the bodies nearer than min(its
+    world hit or reach, 16), its sun origin then uses the body's rotated normal, and a sun ray that missed
+    the world meets any body (counters then cover the world walks only)."""
     rays, valid = gi_rays(width, height, frame, gb_pos, gb_normal)
     gi = np.zeros(width * height, HIT_DTYPE)
     gi["t"] = -1.0
     idx = np.nonzero(valid)[0]
     h, cnt = ow.trace_rays(rays[idx], reach, 0)
-    gi[idx] = h
     sun = np.asarray(sun, np.float32)
     face = h["face"].astype(np.int64)
     hn = np.zeros((len(h), 3), np.float32)
@@ -550,8 +556,17 @@ def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_cou
     sgn = np.where(face % 2 == 1, 1.0, -1.0).astype(np.float32)
     has = face > 0
     hn[np.nonzero(has)[0], ax[has]] = sgn[has]
-    ndl = (hn[:, 0] * sun[0] + hn[:, 1] * sun[1]) + hn[:, 2] * sun[2]
-    emit = (h["hit"] == 1) & (h["mat"] != 5) & (np.maximum(ndl, np.float32(0.0)) > 0)
+    if bodies:
+        for k in range(len(h)):
+            lim = min(float(h["t"][k]) if h["hit"][k] else float(reach), 16.0)
+            b = ow.objects_only(rays[idx[k], :3], rays[idx[k], 3:], lim)
+            if b.hit:
+                h[k] = _body_record(b)
+                hn[k] = ow.hit_normal(b)
+    gi[idx] = h
+    with np.errstate(invalid="ignore"):
+        ndl = (hn[:, 0] * sun[0] + hn[:, 1] * sun[1]) + hn[:, 2] * sun[2]
+        emit = (h["hit"] == 1) & (h["mat"] != 5) & (np.maximum(ndl, np.float32(0.0)) > 0)
     o = rays[idx, :3]
     d = rays[idx, 3:]
     t = h["t"].astype(np.float32)[:, None]
@@ -562,6 +577,11 @@ def gi_batch(ow, width, height, frame, gb_pos, gb_normal, sun, reach, return_cou
     sh["t"] = -1.0
     if len(sun_rays):
         s_hits, c2 = ow.trace_rays(sun_rays, 1e30, 0)
+        if bodies:
+            for k in np.nonzero(s_hits["hit"] == 0)[0]:
+                b = ow.objects_only(sun_rays[k, :3], sun_rays[k, 3:], 1e30, any_hit=True)
+                if b.hit:
+                    s_hits[k] = _body_record(b)
         sh[idx[emit]] = s_hits
         cnt = {k: cnt[k] + c2[k] for k in cnt}
     if return_counts:

@@ -300,19 +300,22 @@ def test_body_grid_update_and_scene_switch(ctx):
     oh1, _, _, _ = ow1.pass_primary(cam, W, H)
     assert hit_diff(vf.hits_to_numpy(hits), oh1).size == 0
     assert (body.cpu().numpy() == -1).all()
-    # a scene whose num_objects differs from the uploaded bodies is refused; so is the persistent queue
+    # a scene whose num_objects differs from the uploaded bodies is refused, by the passes and the batches
+    _, gbp, gbn, _, _, _ = gbuffer_gpu(ctx, pu, W, H)
+    gi_h, sun_h = dev_hits(W * H), dev_hits(W * H)
     scene.num_objects = len(hd) + 1
     ctx.set_scene(scene)
     with pytest.raises(vf.VfError) as e:
         gbuffer_gpu(ctx, pu, W, H)
     assert "num_objects" in str(e.value)
-    scene.num_objects = len(hd)
-    ctx.set_scene(scene)
-    gi_h, sun_h = dev_hits(W * H), dev_hits(W * H)
-    _, gbp, gbn, _, _, _ = gbuffer_gpu(ctx, pu, W, H)
     with pytest.raises(vf.VfError) as e:
         ctx.trace_gi_batch(pu, W, H, 48.0, gbp, gbn, gi_h, sun_h)
-    assert e.value.code == -5
+    assert "num_objects" in str(e.value)
+    with pytest.raises(vf.VfError) as e:
+        ctx.trace_batch([ctx.view_request(pu, W, H, gi_h)])
+    assert "num_objects" in str(e.value)
+    scene.num_objects = len(hd)
+    ctx.set_scene(scene)
 
 
 def test_caller_bvh_checks(ctx):
@@ -348,3 +351,73 @@ def test_caller_bvh_checks(ctx):
         ctx.objects_set_bvh(chain, np.zeros(1, np.uint32))
     assert "stack" in str(e.value)
     ctx.objects_set_bvh(nodes, ow.bvh_prims)
+
+
+_GI_BATCH_ORACLE = {}
+
+
+@pytest.mark.parametrize("refill", [32, 0, -1])
+def test_gi_batch_with_bodies(ctx, refill):
+    """vf_trace_gi_batch over a scene with bodies, every queue mode: GI rays meet the bodies nearer than
+    min(world hit or reach, 16), a body hit's sun ray leaves along the body's rotated normal, a sun ray that
+    missed the world meets any body (GI_OBJECTS, common.glsl:780, 2267-2271)."""
+    scene, ow, _ = upload(ctx)
+    pu, _ = config1_camera(3)
+    W, H, reach = 320, 180, 96.0
+    _, gbp, gbn, _, _, _ = gbuffer_gpu(ctx, pu, W, H)
+    gi, sun = dev_hits(W * H), dev_hits(W * H)
+    ctx.trace_gi_batch(pu, W, H, reach, gbp, gbn, gi, sun, ctx.queue_cfg((16, 8), 128, True, refill=refill))
+    ctx.sync()
+    if "o" not in _GI_BATCH_ORACLE:
+        _GI_BATCH_ORACLE["o"] = O.gi_batch(ow, W, H, 3, gbp.cpu().numpy().reshape(-1, 4), gbn.cpu().numpy().view(np.uint32),
+                                           np.array(scene.sun_dir, np.float32), reach, bodies=True)
+    og, osun = _GI_BATCH_ORACLE["o"]
+    g, s = vf.hits_to_numpy(gi)[:W * H], vf.hits_to_numpy(sun)[:W * H]
+    assert hit_diff(g, og).size == 0
+    assert hit_diff(s, osun).size == 0
+    assert (g["body"] == 1).sum() > 20, "too few GI rays meet bodies"
+    assert ((s["body"] == 1) & (s["hit"] == 1)).sum() > 0, "no sun ray is blocked by a body"
+
+
+def test_uneven_batch_with_bodies_matches_plain_launches(ctx):
+    """vf_trace_batch over a scene with bodies: each view equals its plain primary launch (traceSceneS from
+    the camera, with the exitSolid pre-step for a camera in terrain), each probe grid its plain probe
+    launch (GI_ADD_OBJECTS), in the persistent and the chunk modes."""
+    _, ow, hd = upload(ctx)
+    w, _, _ = world(1)
+    pu0, cam = config1_camera()
+    pos = np.asarray(cam["pos"], np.float32)
+    fwd = np.asarray(cam["fwd"], np.float32)
+    gx, gz = 100, 120
+    in_solid = np.array([gx + 0.5, w.ground(gx, gz) - 3.5, gz + 0.5], np.float32)
+    in_body = ((hd["aabb_min"][0, :3] + hd["aabb_max"][0, :3]) * 0.5).astype(np.float32)
+    views = [(pu0, 320, 180), (camera(in_solid, 0.4, 0.1)[0], 96, 64), (camera(in_body, 2.2, -0.3)[0], 130, 75)]
+    origin = (pos + fwd * 60.0 - 24.0).astype(np.float32)
+    probes = [(origin, 16.0, (4, 3, 4), 128, 3, 96.0)]
+    reqs, outs = [], []
+    for (pu, vw, vh) in views:
+        outs.append(dev_hits(vw * vh))
+        reqs.append(ctx.view_request(pu, vw, vh, outs[-1]))
+    for (o, sp, dims, nr, fr, rc) in probes:
+        outs.append(dev_hits(int(np.prod(dims)) * nr))
+        reqs.append(ctx.probe_request(o, sp, dims, nr, fr, rc, outs[-1]))
+    refs = []
+    for (pu, vw, vh) in views:
+        r = dev_hits(vw * vh)
+        ctx.trace_primary(pu, vw, vh, r)
+        refs.append((r, vw * vh))
+    for (o, sp, dims, nr, fr, rc) in probes:
+        n = int(np.prod(dims)) * nr
+        r = dev_hits(n)
+        ctx.trace_probes(o, sp, dims, nr, fr, rc, r)
+        refs.append((r, n))
+    ctx.sync()
+    body_hits = sum(int((vf.hits_to_numpy(r)[:n]["body"] == 1).sum()) for r, n in refs)
+    assert body_hits > 50, body_hits
+    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 512, refill=0)):
+        for o in outs:
+            o.zero_()
+        ctx.trace_batch(reqs, cfg)
+        ctx.sync()
+        for k, (r, n) in enumerate(refs):
+            assert torch.equal(outs[k][:n * 20], r[:n * 20]), (k, cfg.refill)

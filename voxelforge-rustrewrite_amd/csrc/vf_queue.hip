@@ -14,6 +14,9 @@
 //   * vf_trace_batch: camera views of any size + DDGI probe grids in ONE persistent launch.  Each
 //     request owns a chunk-aligned id range (smallest views first); rays are generated on the fly;
 //     the wave that retires a request's last ray stamps its completion time (100 MHz wall clock).
+// With bodies in the scene (GI_OBJECTS) a ray meets them once its world walk is done, by the rules of the
+// pass it stands for: a GI or probe ray the bodies nearer than min(its world hit or reach, 16), a sun ray
+// that missed the world any body, a view ray the nearest body closer than its world hit.
 // Queue counters are reset by a kernel (no memset), so a batch can be recorded into a graph.
 #include <hip/hip_runtime.h>
 
@@ -26,6 +29,7 @@
 #include <vector>
 
 #include "../../include/vf_trace.h"
+#include "vf_bodies.h"
 #include "vf_ctx.h"
 #include "vf_device.h"
 #include "vf_internal.h"
@@ -49,7 +53,36 @@ __device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
     return (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
 }
 __device__ __forceinline__ void miss(Hit &h) {
-    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
+    h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0; h.obj = -1;
+}
+
+// The bodies after a ray's world walk (h: its world hit).  A bounce ray (gi.comp / probe_trace.comp with
+// GI_OBJECTS, common.glsl:780, 2267-2271): the nearest body closer than min(the world hit or tcap, 16).
+// A sun ray: if it missed the world, any body (common.glsl:2268-2270).
+template <bool COUNT>
+__device__ __forceinline__ void bounce_bodies(const DevWorld &W, const DevBodies &Bo, const float ro[3], const float rd[3],
+                                              float tcap, Hit &h, Cnt &c) {
+    h.obj = -1;
+    trace_objects<COUNT>(W, Bo, ro, rd, h, gmin(h.hit ? h.t : tcap, 16.0f), false, c);
+}
+template <bool COUNT>
+__device__ __forceinline__ void sun_bodies(const DevWorld &W, const DevBodies &Bo, const float ro[3], const float rd[3],
+                                           Hit &h, Cnt &c) {
+    h.obj = -1;
+    if (!h.hit) trace_objects<COUNT>(W, Bo, ro, rd, h, 1e30f, true, c);
+}
+// The GI hit's sun ray origin (giPathImpl stage 1, common.glsl:2318-2330): a hit that is not water and
+// whose normal (a body's rotated face normal for a body hit) faces the sun starts one at hit + N * 0.03.
+template <bool BODIES>
+__device__ __forceinline__ bool sun_origin(const DevBodies &Bo, const Hit &h, const float ro[3], const float rd[3],
+                                           const float sun_dir[3], float so[3]) {
+    if (!h.hit || h.mat == 5u) return false;
+    float hn[3];
+    hit_normal<BODIES>(&Bo, h, hn);
+    if (!(gmax(dot3(hn, sun_dir), 0.0f) > 0.0f)) return false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) so[a] = (ro[a] + rd[a] * h.t) + hn[a] * 3e-2f;
+    return true;
 }
 
 // Clears the GI-batch counters and the sort histogram.
@@ -93,10 +126,11 @@ __device__ __forceinline__ void gi_ray(const GenArgs &A, const float4 *gbpos, co
 // The batch without a queue (refill -1): one lane per pixel of a 16x16 tile (8x8 per wave) traces its GI
 // ray and then its sun ray, with the same rays and rules as the list kernels; for even work such as one
 // ray per pixel the launch's own scheduling is enough.
-template <bool COUNT>
-__global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, GenArgs A, const float4 *gbpos, const uint32_t *gbn,
-                                                   vf_hit *gi_hits, vf_hit *sun_hits, float sx, float sy, float sz,
-                                                   unsigned long long *cnt, float ix, float iy, float iz) {
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, DevBodies Bo, GenArgs A, const float4 *gbpos,
+                                                   const uint32_t *gbn, vf_hit *gi_hits, vf_hit *sun_hits, float sx,
+                                                   float sy, float sz, unsigned long long *cnt, float ix, float iy,
+                                                   float iz) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t tiles_x = (A.w + 15) / 16;
@@ -113,21 +147,17 @@ __global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, GenArgs A, con
         float o[3], d[3];
         gi_ray(A, gbpos, gbn, px, py, fi, o, d);
         trace_world<0, COUNT, true>(W, o, d, A.reach, h, c);
+        if (BODIES) bounce_bodies<COUNT>(W, Bo, o, d, A.reach, h, c);
         const float sun_dir[3] = {sx, sy, sz};
-        if (h.hit && h.mat != 5u) {
-            float hn[3];
-            face_normal(h.face, hn);
-            if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
-                float so[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) so[a] = (o[a] + d[a] * h.t) + hn[a] * 3e-2f;
-                const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
-                trace_world<0, COUNT, true, 256, true>(W, so, sun_dir, 1e30f, sh, c, 0, sinv);
-            }
+        float so[3];
+        if (sun_origin<BODIES>(Bo, h, o, d, sun_dir, so)) {
+            const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
+            trace_world<0, COUNT, true, 256, true>(W, so, sun_dir, 1e30f, sh, c, 0, sinv);
+            if (BODIES) sun_bodies<COUNT>(W, Bo, so, sun_dir, sh, c);
         }
     }
-    store_hit(gi_hits + fi, h);
-    store_hit(sun_hits + fi, sh);
+    store_hit(gi_hits + fi, h, BODIES && h.hit && h.obj >= 0);
+    store_hit(sun_hits + fi, sh, BODIES && sh.hit && sh.obj >= 0);
     flush_count<COUNT>(c, cnt);
 }
 __global__ void __launch_bounds__(256) k_gi_gen(GenArgs A, const float4 *gbpos, const uint32_t *gbn, float4 *rays,
@@ -198,8 +228,9 @@ __device__ __forceinline__ bool claim(unsigned long long *head, uint64_t n, int3
 // Traces a ray list (o.xyz, tag) (d.xyz, tcap) of length *len.  SUN: the list is the GI batch, and
 // each GI hit that is not water and faces the sun appends its sun ray (giPathImpl stage 1,
 // common.glsl:2318-2330) to sun_list; every GI ray writes one sun_hits record (miss when no ray).
-template <bool SUN, bool COUNT>
-__global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *rays, const unsigned long long *len,
+// BODIES: the GI list's rays meet the bodies as bounce rays, the sun list's as sun rays.
+template <bool SUN, bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_list_persist(DevWorld W, DevBodies Bo, const float4 *rays, const unsigned long long *len,
                                                       unsigned long long *head, vf_hit *hits, float4 *sun_list,
                                                       unsigned long long *sun_tail, vf_hit *sun_hits, float sx, float sy,
                                                       float sz, int32_t chunk, int32_t refill, unsigned long long *cnt) {
@@ -209,6 +240,7 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
     RayState r;
     Hit h;
     float rd[3] = {0.0f, 0.0f, 0.0f};
+    float tcap = 0.0f;
     bool active = false;
     uint32_t tag = 0;
     uint64_t pool = 0, pool_end = 0;
@@ -234,6 +266,7 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
                 const float4 a = rays[id * 2 + 0], b = rays[id * 2 + 1];
                 const float ro[3] = {a.x, a.y, a.z};
                 rd[0] = b.x; rd[1] = b.y; rd[2] = b.z;
+                tcap = b.w;
                 tag = __float_as_uint(a.w);
                 if (ray_begin<COUNT>(W, ro, rd, b.w, r, h, c) == RAY_RUNNING) active = true;
                 else finished = true;   // missed the world box: done at once
@@ -252,20 +285,15 @@ __global__ void __launch_bounds__(256) k_list_persist(DevWorld W, const float4 *
         bool emit = false;
         float so[3] = {0.0f, 0.0f, 0.0f};
         if (finished) {
-            store_hit(hits + tag, h);
+            if (BODIES) {
+                const float ro[3] = {r.ro[0], r.ro[1], r.ro[2]};
+                if (SUN) bounce_bodies<COUNT>(W, Bo, ro, rd, tcap, h, c);
+                else sun_bodies<COUNT>(W, Bo, ro, rd, h, c);
+            }
+            store_hit(hits + tag, h, BODIES && h.hit && h.obj >= 0);
             if (SUN) {
-                if (h.hit && h.mat != 5u) {
-                    float hn[3];
-                    face_normal(h.face, hn);
-                    if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
-                        emit = true;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const float hp = r.ro[a] + rd[a] * h.t;
-                            so[a] = hp + hn[a] * 3e-2f;
-                        }
-                    }
-                }
+                const float ro[3] = {r.ro[0], r.ro[1], r.ro[2]};
+                emit = sun_origin<BODIES>(Bo, h, ro, rd, sun_dir, so);
                 if (!emit) {
                     Hit mh;
                     miss(mh);
@@ -429,8 +457,18 @@ __device__ __forceinline__ void retire(unsigned long long *ctr, unsigned long lo
     }
 }
 
+// The bodies after a batch ray's world walk: a view ray (traceScenePrimary = traceSceneS, gbuffer.comp:233-266)
+// the nearest body closer than its world hit, from the origin its world walk started at; a probe ray
+// (probe_trace.comp:47) the bodies nearer than min(its world hit or reach, 16).
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, unsigned long long *ctr,
+__device__ __forceinline__ void batch_bodies(const DevWorld &W, const DevBodies &Bo, const ReqDev &q, const float ro[3],
+                                             const float rd[3], Hit &h, Cnt &c) {
+    if (q.kind == 0) add_bodies<COUNT>(W, Bo, ro, rd, 1e30f, h, c);
+    else bounce_bodies<COUNT>(W, Bo, ro, rd, q.reach, h, c);
+}
+
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, DevBodies Bo, BatchArgs B, unsigned long long *ctr,
                                                        unsigned long long *stamps, int32_t chunk, int32_t refill,
                                                        unsigned long long *cnt) {
     if (threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
@@ -439,10 +477,11 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
     Cnt c = {0, 0, 0, 0};
     RayState r;
     Hit h;
-    bool active = false;
+    bool active = false, vray = false;
     int32_t req = 0, skip = 0;
     uint32_t out = 0;
     float tskip = 0.0f;
+    float rd[3] = {0.0f, 0.0f, 0.0f};   // the ray's direction as generated (the bodies need it unclamped)
     uint64_t pool = 0, pool_end = 0;
     int32_t pool_req = 0;
     bool drained = false;
@@ -475,9 +514,10 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
             if (((idle >> lane_id()) & 1ull) != 0ull && rk < take) {
                 const uint64_t id = pool + rk;
                 req = pool_req;
-                float ro[3], rd[3], tcap;
+                float ro[3], tcap;
                 bool void_ray;
                 batch_ray(W, B.req[req], id - B.req[req].first, ro, rd, tcap, out, skip, tskip, void_ray);
+                vray = void_ray;
                 if (out == 0xFFFFFFFFu) {
                     finished = true;   // tile padding outside the view: no ray
                 } else if (void_ray) {
@@ -501,8 +541,12 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
             }
         }
         if (finished && out != 0xFFFFFFFFu) {
+            if (BODIES && !vray) {   // a void ray is a miss whatever it meets
+                const float ro[3] = {r.ro[0], r.ro[1], r.ro[2]};
+                batch_bodies<COUNT>(W, Bo, B.req[req], ro, rd, h, c);
+            }
             h.t = h.t + tskip;
-            store_hit(B.req[req].hits + out, h);
+            store_hit(B.req[req].hits + out, h, BODIES && h.hit && h.obj >= 0);
         }
         // retired rays are counted per wave (wave-uniform acc_req / acc_cnt) and subtracted from the
         // request's remaining count in one atomic when the request changes, the wave claimed a new
@@ -534,8 +578,8 @@ __global__ void __launch_bounds__(256) k_batch_persist(DevWorld W, BatchArgs B, 
 // Chunk mode (refill == 0): a wavefront claims `chunk` consecutive rays and traces them 64 at a time,
 // each lane to completion with trace_world, then claims again.  No per-iteration bookkeeping, the
 // register footprint of the plain kernels; the sun rays of 64 GI rays go out with one atomic.
-template <bool SUN, bool COUNT>
-__global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, const float4 *rays, const unsigned long long *len,
+template <bool SUN, bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, DevBodies Bo, const float4 *rays, const unsigned long long *len,
                                                       unsigned long long *head, vf_hit *hits, float4 *sun_list,
                                                       unsigned long long *sun_tail, vf_hit *sun_hits, float sx, float sy,
                                                       float sz, int32_t chunk, unsigned long long *cnt) {
@@ -555,20 +599,13 @@ __global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, const float4 *
                 tag = __float_as_uint(a.w);
                 Hit h;
                 trace_world<0, COUNT, true>(W, ro, rd, b.w, h, c);
-                store_hit(hits + tag, h);
+                if (BODIES) {
+                    if (SUN) bounce_bodies<COUNT>(W, Bo, ro, rd, b.w, h, c);
+                    else sun_bodies<COUNT>(W, Bo, ro, rd, h, c);
+                }
+                store_hit(hits + tag, h, BODIES && h.hit && h.obj >= 0);
                 if (SUN) {
-                    if (h.hit && h.mat != 5u) {
-                        float hn[3];
-                        face_normal(h.face, hn);
-                        if (gmax(dot3(hn, sun_dir), 0.0f) > 0.0f) {
-                            emit = true;
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                const float hp = ro[k] + rd[k] * h.t;
-                                so[k] = hp + hn[k] * 3e-2f;
-                            }
-                        }
-                    }
+                    emit = sun_origin<BODIES>(Bo, h, ro, rd, sun_dir, so);
                     if (!emit) {
                         Hit mh;
                         miss(mh);
@@ -596,8 +633,8 @@ __global__ void __launch_bounds__(256) k_list_chunked(DevWorld W, const float4 *
 
 // Chunk mode of the uneven batch: a claimed chunk belongs to one request (wave-uniform), so the
 // request's fields are scalar loads and views (skipMat 0) and probes (skipMat 5) never diverge.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, unsigned long long *ctr,
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, DevBodies Bo, BatchArgs B, unsigned long long *ctr,
                                                        unsigned long long *stamps, int32_t chunk,
                                                        unsigned long long *cnt) {
     if (threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
@@ -623,9 +660,10 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, BatchArgs B, 
                     Hit h;
                     if (skip == 0) trace_world<0, COUNT, true>(W, ro, rd, tcap, h, c);
                     else trace_world<5, COUNT, true>(W, ro, rd, tcap, h, c);
+                    if (BODIES) batch_bodies<COUNT>(W, Bo, q, ro, rd, h, c);
                     h.t = h.t + tskip;
                     if (void_ray) h.hit = 0;
-                    store_hit(q.hits + out, h);
+                    store_hit(q.hits + out, h, BODIES && h.hit && h.obj >= 0);
                 }
             }
         }
@@ -740,8 +778,7 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
                       const uint32_t *gb_normal, vf_hit *gi_hits, vf_hit *sun_hits, const vf_queue_cfg *cfg_in) {
     int rc = vf::check_ready(ctx, "vf_trace_gi_batch");
     if (rc) return rc;
-    if (ctx->scene.num_objects != 0)
-        return vf::fail(ctx, VF_E_UNSUPPORTED, "vf_trace_gi_batch: the persistent queue traces the world only (the scene has bodies)");
+    if ((rc = vf::check_bodies(ctx, "vf_trace_gi_batch"))) return rc;
     if (!ctx->has_scene) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: no scene (vf_set_scene)");
     if (!pass || !gb_pos || !gb_normal || !gi_hits || !sun_hits) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: null buffer");
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_batch: bad size %dx%d", w, h);
@@ -760,53 +797,60 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     for (int a = 0; a < 3; ++a) A.minb[a] = ctx->W.minb[a];
     const int64_t ngen = (int64_t)A.tiles_x * tiles_y * cfg.tile_w * cfg.tile_h;
     const float *s = ctx->S.sun_dir;
+    const DevBodies Bo = vf::dev_bodies(ctx);
+    const bool bodies = Bo.n > 0, counting = ctx->counting != 0;
     unsigned long long *ctr = ctx->q_ctr;
     if ((rc = vf::begin_timed(ctx))) return rc;
     if (cfg.refill < 0) {   // no queue
         const int32_t nt = ((w + 15) / 16) * ((h + 15) / 16);
         float si[3];   // as the traversal computes it: 1 / the clamped direction, IEEE float division
         for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
-        if (ctx->counting)
-            hipLaunchKernelGGL((k_gi_direct<true>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
-                               reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
-                               ctx->d_cnt, si[0], si[1], si[2]);
-        else
-            hipLaunchKernelGGL((k_gi_direct<false>), dim3(nt), dim3(256), 0, ctx->stream, ctx->W, A,
-                               reinterpret_cast<const float4 *>(gb_pos), gb_normal, gi_hits, sun_hits, s[0], s[1], s[2],
-                               ctx->d_cnt, si[0], si[1], si[2]);
+        using DirectFn = void (*)(DevWorld, DevBodies, GenArgs, const float4 *, const uint32_t *, vf_hit *, vf_hit *, float,
+                                  float, float, unsigned long long *, float, float, float);
+        const DirectFn fn = bodies ? (counting ? k_gi_direct<true, true> : k_gi_direct<false, true>)
+                                   : (counting ? k_gi_direct<true, false> : k_gi_direct<false, false>);
+        hipLaunchKernelGGL(fn, dim3(nt), dim3(256), 0, ctx->stream, ctx->W, Bo, A, reinterpret_cast<const float4 *>(gb_pos),
+                           gb_normal, gi_hits, sun_hits, s[0], s[1], s[2], ctx->d_cnt, si[0], si[1], si[2]);
         if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
         return vf::end_timed(ctx);
     }
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(256), 0, ctx->stream, ctr, nullptr, 0);
     hipLaunchKernelGGL(k_gi_gen, dim3(vf::blocks_for(ngen, 256 * kGenPer)), dim3(256), 0, ctx->stream, A,
                        reinterpret_cast<const float4 *>(gb_pos), gb_normal, ctx->q_rays, ctr, gi_hits, sun_hits);
-    const float4 *list = ctx->q_rays;
-    {
-        const void *kp = cfg.refill == 0 ? reinterpret_cast<const void *>(k_list_chunked<true, false>)
-                                         : reinterpret_cast<const void *>(k_list_persist<true, false>);
-        const int g = persistent_groups(ctx, kp, &cfg);
-        unsigned long long *C0 = ctx->d_cnt;
-        if (cfg.refill == 0 && ctx->counting) {
-            hipLaunchKernelGGL((k_list_chunked<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
-                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, C0);
-            hipLaunchKernelGGL((k_list_chunked<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
-                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, C0);
-        } else if (cfg.refill == 0) {
-            hipLaunchKernelGGL((k_list_chunked<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
-                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, C0);
-            hipLaunchKernelGGL((k_list_chunked<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
-                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, C0);
-        } else if (ctx->counting) {
-            hipLaunchKernelGGL((k_list_persist<true, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
-                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
-            hipLaunchKernelGGL((k_list_persist<false, true>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
-                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+    unsigned long long *C0 = ctx->d_cnt;
+    if (cfg.refill == 0) {
+        using ListFn = void (*)(DevWorld, DevBodies, const float4 *, const unsigned long long *, unsigned long long *, vf_hit *,
+                                float4 *, unsigned long long *, vf_hit *, float, float, float, int32_t, unsigned long long *);
+        ListFn gi_fn, sun_fn;
+        if (bodies) {
+            gi_fn = counting ? k_list_chunked<true, true, true> : k_list_chunked<true, false, true>;
+            sun_fn = counting ? k_list_chunked<false, true, true> : k_list_chunked<false, false, true>;
         } else {
-            hipLaunchKernelGGL((k_list_persist<true, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, list, ctr, ctr + 1,
-                               gi_hits, ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
-            hipLaunchKernelGGL((k_list_persist<false, false>), dim3(g), dim3(256), 0, ctx->stream, ctx->W, ctx->q_sun, ctr + 2,
-                               ctr + 3, sun_hits, nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+            gi_fn = counting ? k_list_chunked<true, true, false> : k_list_chunked<true, false, false>;
+            sun_fn = counting ? k_list_chunked<false, true, false> : k_list_chunked<false, false, false>;
         }
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(gi_fn), &cfg);
+        hipLaunchKernelGGL(gi_fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, ctx->q_rays, ctr, ctr + 1, gi_hits,
+                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, C0);
+        hipLaunchKernelGGL(sun_fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, ctx->q_sun, ctr + 2, ctr + 3, sun_hits,
+                           nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, C0);
+    } else {
+        using ListFn = void (*)(DevWorld, DevBodies, const float4 *, const unsigned long long *, unsigned long long *, vf_hit *,
+                                float4 *, unsigned long long *, vf_hit *, float, float, float, int32_t, int32_t,
+                                unsigned long long *);
+        ListFn gi_fn, sun_fn;
+        if (bodies) {
+            gi_fn = counting ? k_list_persist<true, true, true> : k_list_persist<true, false, true>;
+            sun_fn = counting ? k_list_persist<false, true, true> : k_list_persist<false, false, true>;
+        } else {
+            gi_fn = counting ? k_list_persist<true, true, false> : k_list_persist<true, false, false>;
+            sun_fn = counting ? k_list_persist<false, true, false> : k_list_persist<false, false, false>;
+        }
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(gi_fn), &cfg);
+        hipLaunchKernelGGL(gi_fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, ctx->q_rays, ctr, ctr + 1, gi_hits,
+                           ctx->q_sun, ctr + 2, sun_hits, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
+        hipLaunchKernelGGL(sun_fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, ctx->q_sun, ctr + 2, ctr + 3, sun_hits,
+                           nullptr, nullptr, nullptr, s[0], s[1], s[2], cfg.chunk, cfg.refill, C0);
     }
     if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
     return vf::end_timed(ctx);
@@ -822,8 +866,7 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
     const char *what = noalloc ? "vf_trace_batch_noalloc" : "vf_trace_batch";
     int rc = vf::check_ready(ctx, what);
     if (rc) return rc;
-    if (ctx->scene.num_objects != 0)
-        return vf::fail(ctx, VF_E_UNSUPPORTED, "%s: the persistent queue traces the world only (the scene has bodies)", what);
+    if ((rc = vf::check_bodies(ctx, what))) return rc;
     if (!reqs || n_req <= 0 || n_req > kMaxReq)
         return vf::fail(ctx, VF_E_INVALID, "%s: 1..%d requests required (the batch's counters and stamps are sized for %d)",
                         what, kMaxReq, kMaxReq);
@@ -886,28 +929,25 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
         ctr = ctx->q_ctr;
         stamps = ctx->q_stamps;
     }
+    const DevBodies Bo = vf::dev_bodies(ctx);
+    const bool bodies = Bo.n > 0, counting = ctx->counting != 0;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctr, stamps);
     if (cfg.refill == 0) {
-        const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_chunked<true>)
-                                       : reinterpret_cast<const void *>(k_batch_chunked<false>);
-        const int g = persistent_groups(ctx, kp, &cfg);
-        if (ctx->counting)
-            hipLaunchKernelGGL(k_batch_chunked<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
-                               ctx->d_cnt);
-        else
-            hipLaunchKernelGGL(k_batch_chunked<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
-                               ctx->d_cnt);
+        using ChunkFn = void (*)(DevWorld, DevBodies, BatchArgs, unsigned long long *, unsigned long long *, int32_t,
+                                 unsigned long long *);
+        const ChunkFn fn = bodies ? (counting ? k_batch_chunked<true, true> : k_batch_chunked<false, true>)
+                                  : (counting ? k_batch_chunked<true, false> : k_batch_chunked<false, false>);
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(fn), &cfg);
+        hipLaunchKernelGGL(fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, B, ctr, stamps, cfg.chunk, ctx->d_cnt);
     } else {
-        const void *kp = ctx->counting ? reinterpret_cast<const void *>(k_batch_persist<true>)
-                                       : reinterpret_cast<const void *>(k_batch_persist<false>);
-        const int g = persistent_groups(ctx, kp, &cfg);
-        if (ctx->counting)
-            hipLaunchKernelGGL(k_batch_persist<true>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
-                               cfg.refill, ctx->d_cnt);
-        else
-            hipLaunchKernelGGL(k_batch_persist<false>, dim3(g), dim3(256), 0, ctx->stream, ctx->W, B, ctr, stamps, cfg.chunk,
-                               cfg.refill, ctx->d_cnt);
+        using PersistFn = void (*)(DevWorld, DevBodies, BatchArgs, unsigned long long *, unsigned long long *, int32_t, int32_t,
+                                   unsigned long long *);
+        const PersistFn fn = bodies ? (counting ? k_batch_persist<true, true> : k_batch_persist<false, true>)
+                                    : (counting ? k_batch_persist<true, false> : k_batch_persist<false, false>);
+        const int g = persistent_groups(ctx, reinterpret_cast<const void *>(fn), &cfg);
+        hipLaunchKernelGGL(fn, dim3(g), dim3(256), 0, ctx->stream, ctx->W, Bo, B, ctr, stamps, cfg.chunk, cfg.refill,
+                           ctx->d_cnt);
     }
     if ((rc = vf::launch_check(ctx, what))) return rc;
     ctx->batch_stamps = stamps;
