@@ -63,6 +63,10 @@ typedef struct vf_gicache_params {
     uint32_t salt;           /* training-sample and render random streams */
     float gi_reach;          /* GI_REACH (48) */
     int32_t lights_nee;      /* LIGHTS_NEE: emitters with emission >= 0.5 add nothing along the path (1) */
+    int32_t bounces;         /* bounces of the GI the cache stands in for (2): 2 = the reference's giPath; 3 = the
+                              * three-bounce path of vf_trace_gi_path_n.  Training targets, the fallback path and
+                              * (terminate 1) the tail's share follow it: with 3 the tail after bounce 1 has two
+                              * bounces, so a terminate-1 sample is a traced three-bounce path */
 } vf_gicache_params;
 
 typedef struct vf_gicache_stats {
@@ -77,7 +81,7 @@ typedef struct vf_gicache_stats {
 } vf_gicache_stats;
 
 /* Defaults: terminate 0, 65536 samples in 4 steps, spp 1, Adam (1e-3, 0.9, 0.99, 1e-8), loss_eps 0.01,
- * fallback 2x, warmup 16, enc_shift 8, seed 1337, salt 0x2f6b9a31, gi_reach 48, lights_nee 1. */
+ * fallback 2x, warmup 16, enc_shift 8, seed 1337, salt 0x2f6b9a31, gi_reach 48, lights_nee 1, bounces 2. */
 int vf_gicache_defaults(vf_gicache_params *p);
 int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out);
 void vf_gicache_destroy(vf_gicache *c);
@@ -92,6 +96,13 @@ int vf_gicache_reset(vf_ctx *ctx, vf_gicache *c);
 int vf_trace_gi_path(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, float gi_reach,
                      int32_t lights_nee, const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp,
                      uint32_t salt, float *out_rgba);
+/* vf_trace_gi_path with 2 or 3 bounces.  The three-bounce path has no counterpart in the reference (its giPath
+ * stops after two): it is the two-bounce path up to where bounce 2 starts (at the bounce-1 hit, throughput
+ * thr), then a two-bounce path from that point (its first direction the next cosine draw of the same seed,
+ * never traced on past the reach): L = L_head + thr * L_rest.  bounces 2 = vf_trace_gi_path. */
+int vf_trace_gi_path_n(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t width, int32_t height, float gi_reach,
+                       int32_t lights_nee, const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp,
+                       uint32_t salt, int32_t bounces, float *out_rgba);
 
 /* One frame of cached GI: trace params.train_samples fresh samples (paths from stratified G-buffer
  * pixels; target = the traced radiance at the cache's query point), take train_steps Adam steps on

@@ -45,7 +45,7 @@ class VfoCam(C.Structure):
 
 class VfoTail(C.Structure):
     _fields_ = [("valid", C.c_int32), ("p", C.c_float * 3), ("n", C.c_float * 3), ("dir", C.c_float * 3),
-                ("thr", C.c_float * 3)]
+                ("thr", C.c_float * 3), ("body", C.c_int32)]
 
 
 # 240-byte body header (ObjHeader, common.glsl:65-72; the C ABI's vf_obj_header)
@@ -121,12 +121,13 @@ def lib():
             "vfo_voxel_mat_at": (C.c_uint32, [P, P]),
             "vfo_exit_solid": (C.c_float, [P, P, P, C.c_int32]),
             "vfo_gi_path_ex": (None, [P, P, P, P, P, P, C.c_int32, C.c_int32, P, P, P, P]),
-            "vfo_pass_gi_path": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32, P,
-                                        C.c_int]),
-            "vfo_gicache_prepass": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32, P, P,
-                                           C.c_int]),
+            "vfo_gi_path3": (None, [P, P, P, P, P, P, P, P]),
+            "vfo_pass_gi_path": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32,
+                                        C.c_int32, P, C.c_int]),
+            "vfo_gicache_prepass": (None, [P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_uint32,
+                                           C.c_int32, P, P, C.c_int]),
             "vfo_gicache_samples": (None, [P, P, P, C.c_int32, C.c_int32, P, P, C.c_uint32, C.c_int32, C.c_int32,
-                                           C.c_uint32, P, C.c_int]),
+                                           C.c_uint32, C.c_int32, P, C.c_int]),
             "vfo_exp": (C.c_float, [C.c_float]),
             "vfo_pow16": (C.c_float, [C.c_float]),
             "vfo_pow64": (C.c_float, [C.c_float]),
@@ -419,38 +420,48 @@ class OracleWorld:
                   "dir": np.array(t.dir[:], np.float32), "thr": np.array(t.thr[:], np.float32)}
         return L, sd.value, tl
 
+    def gi_path3(self, scene, P, N, d, seed, gi_reach=48.0, lights_nee=True):
+        """One three-bounce path (vfo_gi_path3); returns (L, seed after)."""
+        s = self.sky(scene, gi_reach, lights_nee)
+        P32, N32, d32 = f32(P), f32(N), f32(d)
+        L = np.zeros(3, np.float32)
+        sd = C.c_uint32(int(seed) & 0xFFFFFFFF)
+        lib().vfo_gi_path3(C.byref(self.s), C.byref(s), _p(P32), _p(N32), _p(d32), C.byref(sd), _p(L), None)
+        return L, sd.value
+
     def pass_gi_path(self, scene, width, height, gb_pos, gb_normal, frame, spp, salt, gi_reach=48.0, lights_nee=True,
-                     threads=0):
-        """Plain traced two-bounce GI: RGBA per pixel (mean of spp paths)."""
+                     threads=0, bounces=2):
+        """Plain traced GI, two bounces (giPath) or three: RGBA per pixel (mean of spp paths)."""
         s = self.sky(scene, gi_reach, lights_nee)
         out = np.zeros((width * height, 4), np.float32)
         pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
         lib().vfo_pass_gi_path(C.byref(self.s), C.byref(s), int(width), int(height), _p(pos32), _p(nrm),
-                               int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, _p(out), int(threads))
+                               int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, int(bounces), _p(out),
+                               int(threads))
         return out
 
     def gicache_prepass(self, scene, width, height, gb_pos, gb_normal, frame, spp, salt, gi_reach=48.0,
-                        lights_nee=True, threads=0):
+                        lights_nee=True, threads=0, bounces=2):
         """The cached render's traced part: (pre RGBA per pixel, query record per pixel*spp slot)."""
         s = self.sky(scene, gi_reach, lights_nee)
         pre = np.zeros((width * height, 4), np.float32)
         q = np.zeros(width * height * spp, RECORD_DTYPE)
         pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
         lib().vfo_gicache_prepass(C.byref(self.s), C.byref(s), int(width), int(height), _p(pos32), _p(nrm),
-                                  int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, _p(pre), _p(q),
-                                  int(threads))
+                                  int(frame) & 0xFFFFFFFF, int(spp), int(salt) & 0xFFFFFFFF, int(bounces), _p(pre),
+                                  _p(q), int(threads))
         return pre, q
 
     def gicache_samples(self, scene, cam_pos, width, height, gb_pos, gb_normal, frame, n, terminate, salt,
-                        gi_reach=48.0, lights_nee=True, threads=0):
+                        gi_reach=48.0, lights_nee=True, threads=0, bounces=2):
         """The training records of one frame."""
         s = self.sky(scene, gi_reach, lights_nee)
         out = np.zeros(n, RECORD_DTYPE)
         cam = f32(cam_pos)
         pos32, nrm = f32(gb_pos), np.ascontiguousarray(gb_normal, dtype=np.uint32)
         lib().vfo_gicache_samples(C.byref(self.s), C.byref(s), _p(cam), int(width), int(height), _p(pos32), _p(nrm),
-                                  int(frame) & 0xFFFFFFFF, int(n), int(terminate), int(salt) & 0xFFFFFFFF, _p(out),
-                                  int(threads))
+                                  int(frame) & 0xFFFFFFFF, int(n), int(terminate), int(salt) & 0xFFFFFFFF,
+                                  int(bounces), _p(out), int(threads))
         return out
 
 

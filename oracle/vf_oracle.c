@@ -1263,9 +1263,9 @@ void vfo_pass_sun_cone(const vfo_world *w, const float *palette, const float sun
  * seed: the path's RNG state (bounce 2's direction, cosineHemisphere(hitN, seed)).  With tail non-null
  * the path stops where bounce 2 would start (after `thr *= hitAlb * giScale`, common.glsl:2277 and
  * :2339) and reports that point; L then holds the light gathered before it. */
-void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
-                    uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
-                    vfo_count *cnt) {
+static void gi_path_impl(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                         uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
+                         int tail_any, vfo_count *cnt) {
     float thr[3] = {1.0f, 1.0f, 1.0f}, ro[3], rdir[3], hitAlb[3] = {0, 0, 0}, hitP[3] = {0, 0, 0};
     float hitN[3] = {0.0f, 1.0f, 0.0f}, hitD[3] = {0, 0, 0};
     float hitNdl = 0.0f, reach = s->gi_reach;
@@ -1338,8 +1338,9 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
         }
         if (next_bounce) {   /* bounce 2 from the first hit */
             for (int a = 0; a < 3; ++a) thr[a] = thr[a] * (hitAlb[a] * s->gi_scale);
-            if (tail && !hit_body) {   /* a body's bounce-1 hit is never a cache query (common.glsl:2283) */
+            if (tail && (tail_any || !hit_body)) {   /* a body's bounce-1 hit is never a cache query (common.glsl:2283) */
                 tail->valid = 1;
+                tail->body = hit_body;
                 for (int a = 0; a < 3; ++a) {
                     tail->p[a] = hitP[a];
                     tail->n[a] = hitN[a];
@@ -1355,6 +1356,29 @@ void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], cons
             extended = 0;
         }
     }
+}
+
+void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                    uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
+                    vfo_count *cnt) {
+    gi_path_impl(w, s, P, N, dir, seed, last_stage, trace_on, L, bounce, tail, 0, cnt);
+}
+
+/* The rest of a path from its tail: a two-bounce path from the bounce-1 hit (its first direction the next
+ * cosine draw of the seed, no trace-on), added with the tail's throughput: L += thr * L_rest. */
+static void continue_tail(const vfo_world *w, const vfo_sky *s, const vfo_tail *t, uint32_t *seed, float L[3],
+                          vfo_count *cnt) {
+    float d2[3], L2[3];
+    vfo_cosine_hemisphere(t->n, seed, d2);
+    gi_path_impl(w, s, t->p, t->n, d2, seed, 3, 0, L2, NULL, NULL, 0, cnt);
+    for (int a = 0; a < 3; ++a) L[a] += t->thr[a] * L2[a];
+}
+
+void vfo_gi_path3(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                  uint32_t *seed, float L[3], vfo_count *cnt) {
+    vfo_tail t;
+    gi_path_impl(w, s, P, N, dir, seed, 3, 1, L, NULL, &t, 1, cnt);
+    if (t.valid) continue_tail(w, s, &t, seed, L, cnt);
 }
 
 void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
@@ -1382,7 +1406,8 @@ static void gi_sample_start(const float N[3], int32_t px, int32_t py, uint32_t k
 }
 
 void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H, const float *gb_pos,
-                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *out, int threads) {
+                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, int32_t bounces, float *out,
+                      int threads) {
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
     for (int64_t i = 0; i < (int64_t)W * H; ++i) {
         int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
@@ -1395,7 +1420,8 @@ void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H
                 float dir[3], L[3];
                 uint32_t seed;
                 gi_sample_start(N, px, py, frame * (uint32_t)spp + (uint32_t)sp, salt, dir, &seed);
-                vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
+                if (bounces == 3) vfo_gi_path3(w, s, P, N, dir, &seed, L, NULL);
+                else vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
                 for (int a = 0; a < 3; ++a) sum[a] += L[a];
             }
         }
@@ -1408,8 +1434,8 @@ void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H
  * before the tail summed over the spp samples (a = 1, or 0 for sky), and per slot pixel*spp + s the
  * query record (tag = slot, or 0xFFFFFFFF when the sample has no tail). */
 void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t W, int32_t H, const float *gb_pos,
-                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *pre,
-                         vfo_record *queries, int threads) {
+                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, int32_t bounces,
+                         float *pre, vfo_record *queries, int threads) {
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
     for (int64_t i = 0; i < (int64_t)W * H; ++i) {
         int32_t px = (int32_t)(i % W), py = (int32_t)(i / W);
@@ -1426,7 +1452,11 @@ void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t W, int32_
             uint32_t seed;
             vfo_tail t;
             gi_sample_start(N, px, py, frame * (uint32_t)spp + (uint32_t)sp, salt, dir, &seed);
-            vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, &t, NULL);
+            gi_path_impl(w, s, P, N, dir, &seed, 3, 1, L, NULL, &t, bounces == 3, NULL);
+            if (t.valid && t.body) {   /* three bounces: a body's bounce-1 hit is traced on, never queried */
+                continue_tail(w, s, &t, &seed, L, NULL);
+                t.valid = 0;
+            }
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
             if (t.valid) {
                 q->tag = (uint32_t)(i * spp + sp);
@@ -1448,10 +1478,11 @@ void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t W, int32_
  * two-bounce path.  terminate 0: the record is the G-buffer point (view = towards the camera) with the
  * path's radiance; terminate 1: the bounce-1 hit (view = back along bounce 1) with the radiance of the
  * path's remainder from there (giPathImpl(hit, n, cosineHemisphere(n, seed), seed, 1, false): bounce 2 and
- * its sun ray).  tag = 0 for a sky pixel or a sample without that hit. */
+ * its sun ray).  bounces 3: the path is vfo_gi_path3 and the remainder has two bounces (lastStage 3).  tag = 0
+ * for a sky pixel or a sample without that hit. */
 void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_pos[3], int32_t W, int32_t H,
                          const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t n,
-                         int32_t terminate, uint32_t salt, vfo_record *out, int threads) {
+                         int32_t terminate, uint32_t salt, int32_t bounces, vfo_record *out, int threads) {
     const float stratum = (float)((int64_t)W * H) / (float)n;
     const uint32_t base = vfo_pcg(frame ^ salt);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads(threads))
@@ -1467,7 +1498,8 @@ void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_p
         gbuffer_point(gb_pos, gb_normal, (size_t)pix, P, N);
         vfo_cosine_hemisphere(N, &seed, dir);
         if (terminate == 0) {
-            vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
+            if (bounces == 3) vfo_gi_path3(w, s, P, N, dir, &seed, L, NULL);
+            else vfo_gi_path_ex(w, s, P, N, dir, &seed, 3, 1, L, NULL, NULL, NULL);
             float v[3] = {cam_pos[0] - P[0], cam_pos[1] - P[1], cam_pos[2] - P[2]};
             normalize3(v);
             r->tag = 1u;
@@ -1479,7 +1511,7 @@ void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_p
         if (!t.valid) continue;
         float d2[3], L2[3];
         vfo_cosine_hemisphere(t.n, &seed, d2);
-        vfo_gi_path_ex(w, s, t.p, t.n, d2, &seed, 1, 0, L2, NULL, NULL, NULL);
+        vfo_gi_path_ex(w, s, t.p, t.n, d2, &seed, bounces == 3 ? 3 : 1, 0, L2, NULL, NULL, NULL);
         r->tag = 1u;
         for (int a = 0; a < 3; ++a) { r->pos[a] = t.p[a]; r->n[a] = t.n[a]; r->view[a] = -t.dir[a]; r->rgb[a] = L2[a]; }
     }

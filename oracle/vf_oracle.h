@@ -192,7 +192,7 @@ void vfo_gi_path(const vfo_world *w, const vfo_sky *s, const float P[3], const f
                  float L[3], vfo_hit *bounce, vfo_count *cnt);
 /* The point where a two-bounce path's bounce 2 would start: hit point, its normal, the bounce-1 ray's
  * direction and the throughput after `thr *= hitAlb * giScale`. */
-typedef struct vfo_tail { int32_t valid; float p[3], n[3], dir[3], thr[3]; } vfo_tail;
+typedef struct vfo_tail { int32_t valid; float p[3], n[3], dir[3], thr[3]; int32_t body; } vfo_tail;
 /* Query / training record, the same 64 B as the C-ABI's vf_gi_record. */
 typedef struct vfo_record {
     float pos[3]; uint32_t tag; float n[3]; float pad0; float view[3]; float pad1; float rgb[3]; float pad2;
@@ -202,17 +202,23 @@ void vfo_cosine_hemisphere(const float n[3], uint32_t *state, float out[3]);
 void vfo_gi_path_ex(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
                     uint32_t *seed, int32_t last_stage, int32_t trace_on, float L[3], vfo_hit *bounce, vfo_tail *tail,
                     vfo_count *cnt);
-/* Plain traced two-bounce GI with spp samples per pixel (RGBA32F out; a = 0 for sky). */
+/* A three-bounce path (the library's, not the reference's: its giPath stops after two): the two-bounce path
+ * stopped where bounce 2 would start (a body hit included), then a two-bounce path from there with the next
+ * cosine draw of the seed and no trace-on, weighted by the tail's throughput. */
+void vfo_gi_path3(const vfo_world *w, const vfo_sky *s, const float P[3], const float N[3], const float dir[3],
+                  uint32_t *seed, float L[3], vfo_count *cnt);
+/* Plain traced GI (bounces 2: giPath; 3: vfo_gi_path3) with spp samples per pixel (RGBA32F out; a = 0 for sky). */
 void vfo_pass_gi_path(const vfo_world *w, const vfo_sky *s, int32_t width, int32_t height, const float *gb_pos,
-                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *out, int threads);
-/* Cached render, traced part (tail at the bounce-1 hit). */
+                      const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, int32_t bounces, float *out,
+                      int threads);
+/* Cached render, traced part (tail at the bounce-1 hit; bounces 3: a body's tail is traced on instead). */
 void vfo_gicache_prepass(const vfo_world *w, const vfo_sky *s, int32_t width, int32_t height, const float *gb_pos,
-                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, float *pre,
-                         vfo_record *queries, int threads);
+                         const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt, int32_t bounces,
+                         float *pre, vfo_record *queries, int threads);
 /* Training samples of one frame (terminate 0: at the G-buffer point; 1: at the bounce-1 hit). */
 void vfo_gicache_samples(const vfo_world *w, const vfo_sky *s, const float cam_pos[3], int32_t width, int32_t height,
                          const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t n,
-                         int32_t terminate, uint32_t salt, vfo_record *out, int threads);
+                         int32_t terminate, uint32_t salt, int32_t bounces, vfo_record *out, int threads);
 float vfo_sky_open(const vfo_world *w, const vfo_sky *s, const float p[3]);
 /* GI rays of a full-resolution G-buffer (the GI batch: one per non-sky pixel). */
 void vfo_gi_rays(int32_t width, int32_t height, uint32_t frame, const float *gb_pos, const uint32_t *gb_normal,

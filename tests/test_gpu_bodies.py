@@ -173,16 +173,17 @@ def test_gi_with_bodies(ctx, half_res):
     np.testing.assert_allclose(rad.cpu().numpy().reshape(-1, 4), orad, rtol=2e-4, atol=2e-5)
 
 
-def test_gi_path_with_bodies(ctx):
+@pytest.mark.parametrize("bounces", [2, 3])
+def test_gi_path_with_bodies(ctx, bounces):
     scene, ow, _ = upload(ctx)
     pu, cam = config1_camera()
     W, H = 160, 90
     _, gbp, gbn, _, _, _ = gbuffer_gpu(ctx, pu, W, H)
     out = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
-    ctx.trace_gi_path(pu, W, H, gbp, gbn, out, frame=1, spp=2)
+    ctx.trace_gi_path(pu, W, H, gbp, gbn, out, frame=1, spp=2, bounces=bounces)
     ctx.sync()
     _, opos, onrm, _ = ow.pass_primary(cam, W, H)
-    o = ow.pass_gi_path(scene, W, H, opos, onrm, 1, 2, 0x68e31da4)
+    o = ow.pass_gi_path(scene, W, H, opos, onrm, 1, 2, 0x68e31da4, bounces=bounces)
     np.testing.assert_allclose(out.cpu().numpy().reshape(-1, 4), o, rtol=2e-4, atol=2e-5)
 
 

@@ -138,35 +138,36 @@ def test_adam_matches_oracle_and_updates_the_network(net):
     assert np.all(np.abs(gpu_infer(net, recs)[:, :3] - y) <= 2e-3 + 2e-3 * np.abs(y))
 
 
-def test_plain_two_bounce_gi_matches_oracle(ctx):
+@pytest.mark.parametrize("bounces", [2, 3])
+def test_plain_two_bounce_gi_matches_oracle(ctx, bounces):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=2)
     W, H = 160, 90
     gbp, gbn, gp, gn = gbuffer(ctx, pu, W, H)
     out = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
-    ctx.trace_gi_path(pu, W, H, gbp, gbn, out, frame=3, spp=2, salt=0x1234)
+    ctx.trace_gi_path(pu, W, H, gbp, gbn, out, frame=3, spp=2, salt=0x1234, bounces=bounces)
     ctx.sync()
     got = out.cpu().numpy().reshape(-1, 4)
-    want = ow.pass_gi_path(vf.default_scene(a["world_bricks"]), W, H, gp, gn, 3, 2, 0x1234)
+    want = ow.pass_gi_path(vf.default_scene(a["world_bricks"]), W, H, gp, gn, 3, 2, 0x1234, bounces=bounces)
     assert np.array_equal(got[:, 3], want[:, 3])
     assert np.allclose(got, want, rtol=2e-4, atol=2e-6)
     assert got[:, :3].max() > 0.1 and (got[:, 3] == 1.0).mean() > 0.5
 
 
-@pytest.mark.parametrize("terminate", [0, 1])
-def test_training_samples_match_oracle(ctx, terminate):
+@pytest.mark.parametrize("terminate,bounces", [(0, 2), (1, 2), (0, 3), (1, 3)])
+def test_training_samples_match_oracle(ctx, terminate, bounces):
     _, a, ow = world(1)
     pu, _ = config1_camera(frame=1)
     W, H = 160, 90
     gbp, gbn, gp, gn = gbuffer(ctx, pu, W, H)
-    g = vf.GiCache(ctx, terminate=terminate, train_samples=8192, train_steps=2)
+    g = vf.GiCache(ctx, terminate=terminate, train_samples=8192, train_steps=2, bounces=bounces)
     try:
         out = torch.zeros(8192 * 64, dtype=torch.uint8, device=DEV)
         g.samples(pu, W, H, gbp, gbn, 5, out)
         ctx.sync()
         got = out.cpu().numpy().view(O.RECORD_DTYPE)
         want = ow.gicache_samples(vf.default_scene(a["world_bricks"]), list(pu.cam_pos), W, H, gp, gn, 5, 8192,
-                                  terminate, g.params.salt)
+                                  terminate, g.params.salt, bounces=bounces)
     finally:
         g.close()
     assert np.array_equal(got["tag"], want["tag"])
@@ -188,14 +189,15 @@ def _oracle_render(ow, scene, net_params, X_of, pre, queries, spp, npix):
     return out, int(valid.sum())
 
 
-@pytest.mark.parametrize("terminate,spp", [(1, 1), (1, 2), (0, 1)])
-def test_cached_render_matches_oracle(ctx, terminate, spp):
+@pytest.mark.parametrize("terminate,spp,bounces", [(1, 1, 2), (1, 2, 2), (0, 1, 2), (1, 1, 3)])
+def test_cached_render_matches_oracle(ctx, terminate, spp, bounces):
     _, a, ow = world(1)
     scene = vf.default_scene(a["world_bricks"])
     pu, _ = config1_camera(frame=4)
     W, H = 128, 72
     gbp, gbn, gp, gn = gbuffer(ctx, pu, W, H)
-    g = vf.GiCache(ctx, terminate=terminate, spp=spp, warmup_frames=0, train_samples=4096, train_steps=1)
+    g = vf.GiCache(ctx, terminate=terminate, spp=spp, warmup_frames=0, train_samples=4096, train_steps=1,
+                   bounces=bounces)
     try:
         out = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
         g.frame(pu, W, H, gbp, gbn, out, 7)
@@ -203,7 +205,7 @@ def test_cached_render_matches_oracle(ctx, terminate, spp):
         params = g.get_params()
         got = out.cpu().numpy().reshape(-1, 4)
         if terminate == 1:
-            pre, queries = ow.gicache_prepass(scene, W, H, gp, gn, 7, spp, g.params.salt)
+            pre, queries = ow.gicache_prepass(scene, W, H, gp, gn, 7, spp, g.params.salt, bounces=bounces)
         else:
             npix = W * H
             pre = np.zeros((npix, 4), np.float32)
@@ -235,18 +237,19 @@ def test_cached_render_matches_oracle(ctx, terminate, spp):
     assert np.mean(err <= 2e-4 + 3e-3 * np.abs(want)) > 0.99
 
 
-def test_warmup_frames_render_the_traced_path(ctx):
+@pytest.mark.parametrize("bounces", [2, 3])
+def test_warmup_frames_render_the_traced_path(ctx, bounces):
     _, a, _ = world(1)
     pu, _ = config1_camera(frame=6)
     W, H = 128, 72
     gbp, gbn, _, _ = gbuffer(ctx, pu, W, H)
-    g = vf.GiCache(ctx, warmup_frames=2, spp=1, train_samples=4096, train_steps=1)
+    g = vf.GiCache(ctx, warmup_frames=2, spp=1, train_samples=4096, train_steps=1, bounces=bounces)
     try:
         out = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
         ref = torch.zeros_like(out)
         for f in range(2):
             g.frame(pu, W, H, gbp, gbn, out, f)
-            ctx.trace_gi_path(pu, W, H, gbp, gbn, ref, frame=f, spp=1, salt=g.params.salt)
+            ctx.trace_gi_path(pu, W, H, gbp, gbn, ref, frame=f, spp=1, salt=g.params.salt, bounces=bounces)
             ctx.sync()
             assert g.stats()["fallback"] == 1
             assert torch.equal(out, ref), f

@@ -483,6 +483,39 @@ def test_two_bounce_path_is_its_head_plus_tail():
     assert tails > 40
 
 
+def test_three_bounce_path_is_quadratic_in_the_bounce_weight():
+    """The three-bounce path's light is A + g*B + g^2*C in the bounce weight g = giScale (the throughput picks
+    up albedo x g at each of bounce 1's and bounce 2's hits, and nothing else depends on g).  Pinned from three
+    evaluations: g = 0 is the one-bounce path exactly; B equals the two-bounce path's own bounce-2 term (so the
+    continuation draws bounce 2's direction from the same seed state, starts at the same point and carries the
+    same throughput); C >= 0 and is nonzero for many paths (bounce 3 adds light)."""
+    ow, a, P, N = _gbuffer_points(1500)
+    rng = np.random.default_rng(9)
+    scenes = {}
+    for g in (0.0, 1.0, 2.0):
+        sc = vf.default_scene(a["world_bricks"])
+        sc.gi_scale = g
+        scenes[g] = sc
+    third = 0
+    for i in range(len(P)):
+        seed0 = int(rng.integers(0, 2 ** 32))
+        d, seed = O.cosine_hemisphere(N[i], seed0)
+        L0, _ = ow.gi_path3(scenes[0.0], P[i], N[i], d, seed)
+        L1p, _, _ = ow.gi_path(scenes[0.0], P[i], N[i], d, seed, last_stage=1)
+        assert np.array_equal(L0, L1p), i
+        A = L0.astype(np.float64)
+        y1 = ow.gi_path3(scenes[1.0], P[i], N[i], d, seed)[0].astype(np.float64) - A
+        y2 = ow.gi_path3(scenes[2.0], P[i], N[i], d, seed)[0].astype(np.float64) - A
+        C = (y2 - 2.0 * y1) / 2.0
+        B = y1 - C
+        B2 = ow.gi_path(scenes[1.0], P[i], N[i], d, seed, last_stage=3)[0].astype(np.float64) - A
+        scale = 1e-5 * (1.0 + np.abs(y2).max())
+        assert np.allclose(B, B2, atol=scale), (i, B, B2)
+        assert np.all(C >= -scale), (i, C)
+        third += int(C.max() > 10 * scale)
+    assert third > 30, third
+
+
 # --- camera inside solid terrain (gbuffer.comp:233-266, exitSolid common.glsl:990-1016) ------------
 def _solid_world():
     """64^3 voxels: a solid block [8,40) x [8,24) x [8,40) of material 3 (whole bricks inside, partial ones

@@ -110,6 +110,7 @@ __device__ __forceinline__ void ld_sample2(int32_t px, int32_t py, uint32_t k, u
 // direction and the throughput after `thr *= hitAlb * giScale`.
 struct GiTail {
     bool valid;
+    bool body;   // the bounce-1 hit is a body (reported with TAIL_ANY only)
     float p[3], n[3], dir[3], thr[3];
 };
 
@@ -122,9 +123,10 @@ struct GiPathCfg {
 // bounce (+ its sun ray); 3: two bounces.  trace_on: the rising-ray trace-on past the reach
 // (common.glsl:2309-2320).  TAIL: stop where bounce 2 would start and report it in tail, unless the
 // bounce-1 hit is a body (a moving surface is never answered by a cache, common.glsl:2327, 2283): that
-// path is traced to its end.  BODIES (GI_OBJECTS, common.glsl:2267-2271): a bounce ray also meets the
-// bodies nearer than min(its world hit or reach, 16), a sun ray that missed the world any body.
-template <int LAST_STAGE, bool TAIL, bool COUNT, bool BODIES = false, bool LDSM = false>
+// path is traced to its end; with TAIL_ANY it stops there too).  BODIES (GI_OBJECTS, common.glsl:2267-2271): a
+// bounce ray also meets the bodies nearer than min(its world hit or reach, 16), a sun ray that missed the world
+// any body.
+template <int LAST_STAGE, bool TAIL, bool COUNT, bool BODIES = false, bool LDSM = false, bool TAIL_ANY = false>
 __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &G, const float P[3], const float N[3],
                         const float dir0[3], uint32_t &seed, bool trace_on, float L[3], Hit *bounce, GiTail *tail,
                         Cnt &c, const DevBodies *B = nullptr) {
@@ -135,7 +137,10 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
     int stage = 0;
     bool extended = false, hit_body = false;
     L[0] = L[1] = L[2] = 0.0f;
-    if (TAIL) tail->valid = false;
+    if (TAIL) {
+        tail->valid = false;
+        tail->body = false;
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) { ro[a] = P[a] + N[a] * 3e-2f; rdir[a] = dir0[a]; }
     for (int it = 0; it < 6; ++it) {
@@ -209,8 +214,9 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
         // bounce 2 from the first hit
 #pragma unroll
         for (int a = 0; a < 3; ++a) thr[a] = thr[a] * (hitAlb[a] * S.gi_scale);
-        if (TAIL && !hit_body) {
+        if (TAIL && (TAIL_ANY || !hit_body)) {
             tail->valid = true;
+            tail->body = hit_body;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 tail->p[a] = hitP[a];
@@ -227,6 +233,23 @@ __device__ void gi_path(const DevWorld &W, const SkyParams &S, const GiPathCfg &
         reach = G.reach;
         extended = false;
     }
+}
+
+// A three-bounce path (the learned cache's three-bounce samples; the reference's giPath has two bounces at
+// most): the two-bounce path up to where its bounce 2 would start, then a two-bounce path from there (bounce 2
+// as a path's first bounce, its direction the next draw of the same seed, never traced on past the reach), its
+// light weighted by the throughput reached: L = L_head + thr * L_rest.
+template <bool COUNT, bool BODIES = false, bool LDSM = false>
+__device__ void gi_path3(const DevWorld &W, const SkyParams &S, const GiPathCfg &G, const float P[3], const float N[3],
+                         const float dir0[3], uint32_t &seed, float L[3], Cnt &c, const DevBodies *B = nullptr) {
+    GiTail t;
+    gi_path<3, true, COUNT, BODIES, LDSM, true>(W, S, G, P, N, dir0, seed, true, L, nullptr, &t, c, B);
+    if (!t.valid) return;
+    float d2[3], L2[3];
+    cosine_hemisphere(t.n, seed, d2);
+    gi_path<3, false, COUNT, BODIES, LDSM>(W, S, G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, B);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) L[a] += t.thr[a] * L2[a];
 }
 
 // sampleCone (common.glsl:1259-1265): dir jittered within a cone of angular radius ang.

@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(64) k_gc_stats(StatsArgs A) {
 // ---- GI kernels -------------------------------------------------------------------------------
 
 struct PathArgs {
-    int32_t w, h, tiles_x, ntiles, spp, terminate;
+    int32_t w, h, tiles_x, ntiles, spp, terminate, bounces;
     uint32_t frame, salt;
     GiPathCfg G;
     float cam[3];
@@ -558,8 +558,9 @@ __device__ __forceinline__ void sample_start(const float N[3], int32_t px, int32
     seed = pixel_seed(px, py, k, salt);
 }
 
-// Plain traced two-bounce GI: the mean of spp full paths per pixel.
-template <bool BODIES>
+// Plain traced GI: the mean of spp full paths per pixel, two bounces (the reference's giPath) or, with B3,
+// three (gi_path3).
+template <bool BODIES, bool B3>
 __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &S, const DevBodies &B, const PathArgs &A,
                                             const float4 *gbpos, const uint32_t *gbn, int32_t px, int32_t py,
                                             float4 *out) {
@@ -574,7 +575,8 @@ __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
+            if (B3) gi_path3<false, BODIES, true>(W, S, A.G, P, N, dir, seed, L, c, &B);
+            else gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -583,12 +585,12 @@ __device__ __forceinline__ void plain_pixel(const DevWorld &W, const SkyParams &
     out[i] = sky ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(sum[0] / fs, sum[1] / fs, sum[2] / fs, 1.0f);
 }
 
-template <bool BODIES>
+template <bool BODIES, bool B3>
 __global__ void __launch_bounds__(256) k_gi_plain(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
                                                   const uint32_t *gbn, float4 *out) {
     int32_t px, py;
     if (!tile_pixel(A, px, py)) return;
-    plain_pixel<BODIES>(W, S, B, A, gbpos, gbn, px, py, out);
+    plain_pixel<BODIES, B3>(W, S, B, A, gbpos, gbn, px, py, out);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
@@ -610,15 +612,16 @@ __device__ __forceinline__ void push_query(bool want, unsigned *qcount, float4 *
 
 // Cached render, traced part.  Fallback frames: the plain path, written to out.  Otherwise terminate 1:
 // per sample the path up to its tail (pre += its light; a query at the tail, or tailv[slot] = 0);
-// terminate 0: nothing (k_mlp_infer<2> answers at the G-buffer points).
-template <bool BODIES>
+// terminate 0: nothing (k_mlp_infer<2> answers at the G-buffer points).  B3 (three-bounce GI): a path
+// whose bounce-1 hit is a body is traced on to its end (gi_path3's continuation) instead of querying.
+template <bool BODIES, bool B3>
 __global__ void __launch_bounds__(256, 3) k_gc_render(DevWorld W, SkyParams S, DevBodies B, PathArgs A, const float4 *gbpos,
                                                    const uint32_t *gbn, const DevStats *st, float4 *out, float4 *pre,
                                                    float4 *tailv, float4 *queries, unsigned *qcount) {
     int32_t px, py;
     const bool inside = tile_pixel(A, px, py);
     if (st->fallback) {
-        if (inside) plain_pixel<BODIES>(W, S, B, A, gbpos, gbn, px, py, out);
+        if (inside) plain_pixel<BODIES, B3>(W, S, B, A, gbpos, gbn, px, py, out);
         return;
     }
     if (A.terminate == 0) return;   // answered at the G-buffer point by k_mlp_infer<2>
@@ -636,7 +639,15 @@ __global__ void __launch_bounds__(256, 3) k_gc_render(DevWorld W, SkyParams S, D
             float dir[3], L[3];
             uint32_t seed;
             sample_start(N, px, py, A.frame * (uint32_t)A.spp + (uint32_t)s, A.salt, dir, seed);
-            gi_path<3, true, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
+            gi_path<3, true, false, BODIES, true, B3>(W, S, A.G, P, N, dir, seed, true, L, nullptr, &t, c, &B);
+            if (B3 && t.valid && t.body) {   // a moving surface is never answered by the cache
+                float d2[3], L2[3];
+                cosine_hemisphere(t.n, seed, d2);
+                gi_path<3, false, false, BODIES, true>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) L[a] += t.thr[a] * L2[a];
+                t.valid = false;
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a) sum[a] += L[a];
         }
@@ -669,15 +680,18 @@ __global__ void __launch_bounds__(256) k_gc_resolve(const DevStats *st, int32_t 
 }
 
 struct SampleArgs {
-    int32_t w, h, n, terminate;
+    int32_t w, h, n, terminate, bounces;
     uint32_t frame, salt;
     GiPathCfg G;
     float cam[3];
 };
 
 // Training samples: sample i takes pixel floor((i + r) * W*H / n) (r from its own stream
-// pcg(i + pcg(frame ^ salt))) and a cosine-hemisphere bounce-1 direction from the same stream.
-template <bool BODIES>
+// pcg(i + pcg(frame ^ salt))) and a cosine-hemisphere bounce-1 direction from the same stream.  The target
+// is the traced radiance at the query point: terminate 0, the whole path from the G-buffer point (two
+// bounces, or three with B3); terminate 1, the rest of the path from the bounce-1 hit (one bounce and its
+// sun ray, or two with B3).
+template <bool BODIES, bool B3>
 __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, DevBodies B, SampleArgs A, const float4 *gbpos,
                                                     const uint32_t *gbn, float4 *out) {
     const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
@@ -699,7 +713,8 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Dev
     cosine_hemisphere(N, seed, dir);
     Cnt c = {0, 0, 0, 0};
     if (A.terminate == 0) {
-        gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
+        if (B3) gi_path3<false, BODIES, true>(W, S, A.G, P, N, dir, seed, L, c, &B);
+        else gi_path<3, false, false, BODIES, true>(W, S, A.G, P, N, dir, seed, true, L, nullptr, nullptr, c, &B);
         float v[3] = {A.cam[0] - P[0], A.cam[1] - P[1], A.cam[2] - P[2]};
         normalize3(v);
         r[0] = make_float4(P[0], P[1], P[2], __uint_as_float(1u));
@@ -716,7 +731,8 @@ __global__ void __launch_bounds__(256) k_gc_samples(DevWorld W, SkyParams S, Dev
     }
     float d2[3], L2[3];
     cosine_hemisphere(t.n, seed, d2);
-    gi_path<1, false, false, BODIES, true>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
+    if (B3) gi_path<3, false, false, BODIES, true>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
+    else gi_path<1, false, false, BODIES, true>(W, S, A.G, t.p, t.n, d2, seed, false, L2, nullptr, nullptr, c, &B);
     r[0] = make_float4(t.p[0], t.p[1], t.p[2], __uint_as_float(1u));
     r[1] = make_float4(t.n[0], t.n[1], t.n[2], 0.0f);
     r[2] = make_float4(-t.dir[0], -t.dir[1], -t.dir[2], 0.0f);
@@ -1066,6 +1082,7 @@ int check_params(vf_ctx *ctx, const vf_gicache_params *p) {
     if (p->warmup_frames < 0 || p->enc_shift < 0 || p->enc_shift > 24)
         return vf::fail(ctx, VF_E_INVALID, "vf_gicache: warmup_frames or enc_shift out of range");
     if (!(p->gi_reach > 0.0f) || p->gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: gi_reach out of range");
+    if (p->bounces != 2 && p->bounces != 3) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: bounces must be 2 or 3");
     return VF_OK;
 }
 
@@ -1080,13 +1097,14 @@ int check_gbuffer(vf_ctx *ctx, const char *what, const vf_pass_ubo *pass, int32_
 }
 
 PathArgs path_args(const vf_pass_ubo *pass, int32_t w, int32_t h, uint32_t frame, int32_t spp, uint32_t salt, float reach,
-                   int32_t nee, int32_t terminate) {
+                   int32_t nee, int32_t terminate, int32_t bounces) {
     PathArgs A;
     A.w = w; A.h = h;
     A.tiles_x = (w + 15) / 16;
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     A.spp = spp;
     A.terminate = terminate;
+    A.bounces = bounces;
     A.frame = frame;
     A.salt = salt;
     A.G.reach = reach;
@@ -1101,18 +1119,19 @@ int launch_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t 
     A.w = w; A.h = h;
     A.n = c->p.train_samples;
     A.terminate = c->p.terminate;
+    A.bounces = c->p.bounces;
     A.frame = frame;
     A.salt = c->p.salt;
     A.G.reach = c->p.gi_reach;
     A.G.lights_nee = c->p.lights_nee ? 1 : 0;
     for (int a = 0; a < 3; ++a) A.cam[a] = pass->cam_pos[a];
     const DevBodies B = vf::dev_bodies(ctx);
-    if (B.n > 0)
-        hipLaunchKernelGGL(k_gc_samples<true>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
-    else
-        hipLaunchKernelGGL(k_gc_samples<false>, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
+    using Fn = void (*)(DevWorld, SkyParams, DevBodies, SampleArgs, const float4 *, const uint32_t *, float4 *);
+    const bool b3 = A.bounces == 3;
+    const Fn fn = B.n > 0 ? (b3 ? k_gc_samples<true, true> : k_gc_samples<true, false>)
+                          : (b3 ? k_gc_samples<false, true> : k_gc_samples<false, false>);
+    hipLaunchKernelGGL(fn, dim3(blocks_for(A.n)), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, out);
     return launch_check(ctx, "k_gc_samples");
 }
 
@@ -1139,6 +1158,7 @@ int vf_gicache_defaults(vf_gicache_params *p) {
     p->salt = 0x2f6b9a31u;
     p->gi_reach = 48.0f;
     p->lights_nee = 1;
+    p->bounces = 2;
     return VF_OK;
 }
 
@@ -1210,25 +1230,31 @@ int vf_gicache_reset(vf_ctx *ctx, vf_gicache *c) {
     return reset_stats(ctx, c);
 }
 
-int vf_trace_gi_path(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h, float gi_reach, int32_t lights_nee,
-                     const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt,
-                     float *out_rgba) {
+int vf_trace_gi_path_n(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h, float gi_reach, int32_t lights_nee,
+                       const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt,
+                       int32_t bounces, float *out_rgba) {
     int rc = check_gbuffer(ctx, "vf_trace_gi_path", pass, w, h, gb_pos, gb_normal);
     if (rc) return rc;
     if (!out_rgba) return gc_fail_null(ctx, "vf_trace_gi_path");
     if (spp < 1 || spp > 4096) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: spp %d out of [1, 4096]", spp);
     if (!(gi_reach > 0.0f) || gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: gi_reach out of range");
-    const PathArgs A = path_args(pass, w, h, frame, spp, salt, gi_reach, lights_nee, 1);
+    if (bounces != 2 && bounces != 3) return vf::fail(ctx, VF_E_INVALID, "vf_trace_gi_path: bounces must be 2 or 3");
+    const PathArgs A = path_args(pass, w, h, frame, spp, salt, gi_reach, lights_nee, 1, bounces);
     if ((rc = vf::begin_timed(ctx))) return rc;
     const DevBodies B = vf::dev_bodies(ctx);
-    if (B.n > 0)
-        hipLaunchKernelGGL(k_gi_plain<true>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
-    else
-        hipLaunchKernelGGL(k_gi_plain<false>, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
+    using Fn = void (*)(DevWorld, SkyParams, DevBodies, PathArgs, const float4 *, const uint32_t *, float4 *);
+    const Fn fn = B.n > 0 ? (bounces == 3 ? k_gi_plain<true, true> : k_gi_plain<true, false>)
+                          : (bounces == 3 ? k_gi_plain<false, true> : k_gi_plain<false, false>);
+    hipLaunchKernelGGL(fn, dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, A,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, reinterpret_cast<float4 *>(out_rgba));
     if ((rc = launch_check(ctx, "k_gi_plain"))) return rc;
     return vf::end_timed(ctx);
+}
+
+int vf_trace_gi_path(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h, float gi_reach, int32_t lights_nee,
+                     const float *gb_pos, const uint32_t *gb_normal, uint32_t frame, int32_t spp, uint32_t salt,
+                     float *out_rgba) {
+    return vf_trace_gi_path_n(ctx, pass, w, h, gi_reach, lights_nee, gb_pos, gb_normal, frame, spp, salt, 2, out_rgba);
 }
 
 int vf_gicache_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_t w, int32_t h, const float *gb_pos,
@@ -1273,16 +1299,17 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
     }
     // 3. render: traced part + queries, the network on the queries, resolve
     const int32_t spp = c->p.spp;
-    const PathArgs R = path_args(pass, w, h, frame, spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate);
+    const PathArgs R =
+        path_args(pass, w, h, frame, spp, c->p.salt, c->p.gi_reach, c->p.lights_nee, c->p.terminate, c->p.bounces);
     const DevBodies B = vf::dev_bodies(ctx);
-    if (B.n > 0)
-        hipLaunchKernelGGL(k_gc_render<true>, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, R,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
-                           c->pre, c->tailv, c->queries, c->qcount);
-    else
-        hipLaunchKernelGGL(k_gc_render<false>, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, R,
-                           reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
-                           c->pre, c->tailv, c->queries, c->qcount);
+    using Fn = void (*)(DevWorld, SkyParams, DevBodies, PathArgs, const float4 *, const uint32_t *, const DevStats *,
+                        float4 *, float4 *, float4 *, float4 *, unsigned *);
+    const bool b3 = c->p.bounces == 3;
+    const Fn fn = B.n > 0 ? (b3 ? k_gc_render<true, true> : k_gc_render<true, false>)
+                          : (b3 ? k_gc_render<false, true> : k_gc_render<false, false>);
+    hipLaunchKernelGGL(fn, dim3(R.ntiles), dim3(256), 0, ctx->stream, ctx->W, ctx->S, B, R,
+                       reinterpret_cast<const float4 *>(gb_pos), gb_normal, c->st, reinterpret_cast<float4 *>(out_rgba),
+                       c->pre, c->tailv, c->queries, c->qcount);
     if ((rc = launch_check(ctx, "k_gc_render"))) return rc;
     InferArgs I;
     std::memset(&I, 0, sizeof I);

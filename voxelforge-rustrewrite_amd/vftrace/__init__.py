@@ -101,6 +101,8 @@ _SIGS = {
     "vf_gicache_reset": (C.c_int, [_P, _P]),
     "vf_trace_gi_path": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
                                    C.c_uint32, C.c_int32, C.c_uint32, _P]),
+    "vf_trace_gi_path_n": (C.c_int, [_P, C.POINTER(PassUbo), C.c_int32, C.c_int32, C.c_float, C.c_int32, _P, _P,
+                                     C.c_uint32, C.c_int32, C.c_uint32, C.c_int32, _P]),
     "vf_gicache_frame": (C.c_int, [_P, _P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, C.c_uint32, _P]),
     "vf_gicache_read_stats": (C.c_int, [_P, _P, C.POINTER(GiCacheStats)]),
     "vf_gicache_samples": (C.c_int, [_P, _P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, C.c_uint32, _P]),
@@ -766,13 +768,18 @@ class Context:
         return best, table
 
     def trace_gi_path(self, pass_ubo, width, height, gb_pos, gb_normal, out_rgba, frame=0, spp=1, salt=0x68e31da4,
-                      gi_reach=48.0, lights_nee=True):
-        """Plain traced two-bounce GI (mean of spp paths per pixel) into out_rgba (float32, 4 per pixel)."""
+                      gi_reach=48.0, lights_nee=True, bounces=2):
+        """Plain traced GI, two bounces (the reference's giPath) or three (mean of spp paths per pixel) into
+        out_rgba (float32, 4 per pixel)."""
         n = int(width) * int(height)
-        self._ck(lib().vf_trace_gi_path(self._h, C.byref(pass_ubo), int(width), int(height), float(gi_reach),
-                                        int(bool(lights_nee)), _dev(gb_pos, 16 * n, "gb_pos"),
-                                        _dev(gb_normal, 4 * n, "gb_normal"), int(frame) & 0xFFFFFFFF,
-                                        int(spp), int(salt) & 0xFFFFFFFF, _dev(out_rgba, 16 * n, "out_rgba")))
+        args = (self._h, C.byref(pass_ubo), int(width), int(height), float(gi_reach), int(bool(lights_nee)),
+                _dev(gb_pos, 16 * n, "gb_pos"), _dev(gb_normal, 4 * n, "gb_normal"), int(frame) & 0xFFFFFFFF, int(spp),
+                int(salt) & 0xFFFFFFFF)
+        out = _dev(out_rgba, 16 * n, "out_rgba")
+        if bounces == 2:
+            self._ck(lib().vf_trace_gi_path(*args, out))
+        else:
+            self._ck(lib().vf_trace_gi_path_n(*args, int(bounces), out))
 
     # frame graphs
     def graph_begin(self):
