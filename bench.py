@@ -127,6 +127,7 @@ def main():
     ctx = vf.Context(0)
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
+    torch.cuda.set_stream(stream)   # torch's own work on the buffers stays ordered with the library's launches
     t0 = time.time()
     ctx.upload(world)
     log(f"upload + distance field + sky-top: {time.time() - t0:.2f}s")
@@ -572,6 +573,8 @@ def other_configs(ctx, args):
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
+    prev = torch.cuda.current_stream()
+    torch.cuda.set_stream(stream)
     out = {}
     world = vf.GenWorld.generate(1, seed=args.seed)
     ctx.upload(world)
@@ -656,7 +659,8 @@ def other_configs(ctx, args):
                            "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill},
                            "sequential_plain_launches_ms": round(ms5p, 5),
                            "view_sizes": [[vw, vh] for vw, vh, _ in views]}
-    ctx.set_stream(None)
+    ctx.set_stream(prev)
+    torch.cuda.set_stream(prev)
     return out
 
 
@@ -667,66 +671,138 @@ def _errors(x, ref, mask):
     return float((d * d).mean()), float((d * d / (r * r + 1e-2)).mean())
 
 
+def _reference(ctx, pu, W, H, gbp, gbn, bounces):
+    """64-spp traced GI (samples 0..63 of salt 0x5EED, 8 per launch)."""
+    ref = torch.zeros(W * H * 4, dtype=torch.float32, device=gbp.device)
+    part = torch.zeros_like(ref)
+    for c in range(8):
+        ctx.trace_gi_path(pu, W, H, gbp, gbn, part, frame=c, spp=8, salt=0x5EED, bounces=bounces)
+        ref += part
+    return ref / 8.0
+
+
 def gi_cache_config(ctx, stream, args, pu, W, H, gbp, gbn):
-    """Learned GI radiance cache on the fixed world at 1920x1080: the two-bounce traced GI at 4 spp
-    against the cached GI (online training on 65,536 traced samples per frame in 4 Adam steps, then
-    the render), both judged by MSE / relMSE against a 64-spp traced reference of the same frame.
-    Each cache variant first trains for --cache-frames frames on a static camera; the timed frame
-    includes its sample tracing, training and render."""
+    """Learned GI radiance cache on the fixed world at 1920x1080: traced GI at 4 spp against the cached GI
+    (online training on 65,536 traced samples per frame in 4 Adam steps, then the render), both judged by
+    MSE / relMSE against a 64-spp traced reference of the same frame, for the two-bounce GI of the reference's
+    giPath and for three-bounce GI (the cache's three-bounce samples: with terminate 1 each training target
+    traces two bounces after bounce 1).  Static camera: each cache variant first trains for --cache-frames
+    frames; the timed frame includes its sample tracing, training and render.  Moving camera: every method
+    runs a 64-frame fly-through (2 voxels per frame) from a cold cache and is judged on its last 32 frames."""
     dev = torch.device("cuda:0")
     n = W * H
     mask = (gbp.view(-1, 4)[:, 3] >= 0.0)
-    ref = torch.zeros(n * 4, dtype=torch.float32, device=dev)
-    part = torch.zeros_like(ref)
-    for c in range(8):   # samples 0..63 of the reference stream, 8 per launch
-        ctx.trace_gi_path(pu, W, H, gbp, gbn, part, frame=c, spp=8, salt=0x5EED)
-        ref += part
-    ref /= 8.0
-    refv = ref.view(-1, 4)
     frame = args.cache_frames - 1
-    res = {"reference": "64-spp traced two-bounce GI (samples 0..63 of salt 0x5EED)",
+    res = {"reference": "64-spp traced GI (samples 0..63 of salt 0x5EED) with the method's bounce count",
            "pixels_with_gi": int(mask.sum().item())}
-    plain = {}
-    for spp in (1, 4):
-        o = torch.zeros_like(ref)
-        ms = graph_ms(ctx, stream, lambda: ctx.trace_gi_path(pu, W, H, gbp, gbn, o, frame=frame, spp=spp,
-                                                             salt=0x2f6b9a31), 20)
-        mse, rel = _errors(o.view(-1, 4), refv, mask)
-        plain[spp] = ms
-        res[f"traced_{spp}spp"] = {"ms_per_frame": round(ms, 4), "mse": round(mse, 6), "relmse": round(rel, 5)}
-    for term in (0, 1):
-        g = vf.GiCache(ctx, terminate=term)
-        o = torch.zeros_like(ref)
-        for f in range(args.cache_frames):
-            g.frame(pu, W, H, gbp, gbn, o, f)
-        st = g.stats()
-        mse, rel = _errors(o.view(-1, 4), refv, mask)
-        ms = graph_ms(ctx, stream, lambda: g.frame(pu, W, H, gbp, gbn, o, frame), 20)
-        key = "cache_at_gbuffer_point" if term == 0 else "cache_after_bounce1"
-        res[key] = {"terminate": term, "ms_per_frame": round(ms, 4), "mse": round(mse, 6), "relmse": round(rel, 5),
-                    "speedup_vs_traced_4spp": round(plain[4] / ms, 3),
-                    "mse_vs_traced_4spp": round(mse / res["traced_4spp"]["mse"], 4),
-                    "frames_trained": args.cache_frames, "fallback": st["fallback"], "queries": st["n_queries"],
-                    "running_error": round(st["running_error"], 5)}
-        if term == 0:
-            # matrix-core rate: the training step and the inference on their own
-            samples = torch.zeros(65536 * 64, dtype=torch.uint8, device=dev)
-            g.samples(pu, W, H, gbp, gbn, frame, samples)
-            ms_t = graph_ms(ctx, stream, lambda: g.train(samples, 65536), 20)
-            q = _random_queries(n, dev)
-            h = torch.zeros(n * 4, dtype=torch.int16, device=dev)
-            ms_i = graph_ms(ctx, stream, lambda: g.infer(q, n, h), 20)
-            fl_t, fl_i = 65536 * 3 * FWD_FLOPS, n * FWD_FLOPS
-            res["mfma"] = {"peak_tflops_bf16_dense": 2500.0,
-                           "train_65536": {"ms": round(ms_t, 4), "tflops": round(fl_t / ms_t / 1e9, 2),
-                                           "frac": round(fl_t / ms_t / 1e9 / 2500.0, 5)},
-                           f"infer_{n}": {"ms": round(ms_i, 4), "tflops": round(fl_i / ms_i / 1e9, 2),
-                                          "frac": round(fl_i / ms_i / 1e9 / 2500.0, 5)},
-                           "flops_per_query": FWD_FLOPS}
-        g.close()
+    for bounces in (2, 3):
+        refv = _reference(ctx, pu, W, H, gbp, gbn, bounces).view(-1, 4)
+        sfx = "" if bounces == 2 else "_3bounce"
+        plain = {}
+        for spp in (1, 4):
+            o = torch.zeros(n * 4, dtype=torch.float32, device=dev)
+            ms = graph_ms(ctx, stream, lambda: ctx.trace_gi_path(pu, W, H, gbp, gbn, o, frame=frame, spp=spp,
+                                                                 salt=0x2f6b9a31, bounces=bounces), 20)
+            mse, rel = _errors(o.view(-1, 4), refv, mask)
+            plain[spp] = (ms, mse)
+            res[f"traced_{spp}spp{sfx}"] = {"ms_per_frame": round(ms, 4), "mse": round(mse, 6), "relmse": round(rel, 5)}
+        for term in (0, 1):
+            g = vf.GiCache(ctx, terminate=term, bounces=bounces)
+            o = torch.zeros(n * 4, dtype=torch.float32, device=dev)
+            for f in range(args.cache_frames):
+                g.frame(pu, W, H, gbp, gbn, o, f)
+            st = g.stats()
+            mse, rel = _errors(o.view(-1, 4), refv, mask)
+            ms = graph_ms(ctx, stream, lambda: g.frame(pu, W, H, gbp, gbn, o, frame), 20)
+            key = ("cache_at_gbuffer_point" if term == 0 else "cache_after_bounce1") + sfx
+            res[key] = {"terminate": term, "bounces": bounces, "ms_per_frame": round(ms, 4), "mse": round(mse, 6),
+                        "relmse": round(rel, 5), "speedup_vs_traced_4spp": round(plain[4][0] / ms, 3),
+                        "mse_vs_traced_4spp": round(mse / plain[4][1], 4),
+                        "frames_trained": args.cache_frames, "fallback": st["fallback"], "queries": st["n_queries"],
+                        "running_error": round(st["running_error"], 5)}
+            if term == 0 and bounces == 2:
+                # matrix-core rate: the training step and the inference on their own
+                samples = torch.zeros(65536 * 64, dtype=torch.uint8, device=dev)
+                g.samples(pu, W, H, gbp, gbn, frame, samples)
+                ms_t = graph_ms(ctx, stream, lambda: g.train(samples, 65536), 20)
+                q = _random_queries(n, dev)
+                h = torch.zeros(n * 4, dtype=torch.int16, device=dev)
+                ms_i = graph_ms(ctx, stream, lambda: g.infer(q, n, h), 20)
+                fl_t, fl_i = 65536 * 3 * FWD_FLOPS, n * FWD_FLOPS
+                res["mfma"] = {"peak_tflops_bf16_dense": 2500.0,
+                               "train_65536": {"ms": round(ms_t, 4), "tflops": round(fl_t / ms_t / 1e9, 2),
+                                               "frac": round(fl_t / ms_t / 1e9 / 2500.0, 5)},
+                               f"infer_{n}": {"ms": round(ms_i, 4), "tflops": round(fl_i / ms_i / 1e9, 2),
+                                              "frac": round(fl_i / ms_i / 1e9 / 2500.0, 5)},
+                               "flops_per_query": FWD_FLOPS}
+            g.close()
+    res["moving_camera"] = gi_cache_moving(ctx, stream, W, H)
     log(f"GI cache: traced 4spp {res['traced_4spp']}, cache@P {res['cache_at_gbuffer_point']}, "
-        f"cache@bounce1 {res['cache_after_bounce1']}")
+        f"cache@bounce1 {res['cache_after_bounce1']}, 3-bounce cache@bounce1 {res['cache_after_bounce1_3bounce']}")
     return res
+
+
+def gi_cache_moving(ctx, stream, W, H, frames=64, judged=32, step=2.0):
+    """The GI methods on a fly-through of the fixed world: the camera moves `step` voxels per frame along its
+    horizontal heading (yaw 0.8, pitch -0.35 from (37.3, 96.4, 41.7)); each frame traces its G-buffer, then
+    every method renders it (caches from cold, warm-up frames included).  Reported per method: mean device
+    ms per frame and mean MSE / relMSE against the frame's 64-spp reference over the last `judged` frames."""
+    dev = torch.device("cuda:0")
+    n = W * H
+    hits = torch.empty(n * HIT_BYTES, dtype=torch.uint8, device=dev)
+    gbp = torch.empty(n * 4, dtype=torch.float32, device=dev)
+    gbn = torch.empty(n, dtype=torch.int32, device=dev)
+    yaw, pitch = 0.8, -0.35
+    p0 = np.array([37.3, 96.4, 41.7], np.float64)
+    head = np.array([math.sin(yaw), 0.0, math.cos(yaw)])
+    methods = {}
+    for bounces in (2, 3):
+        sfx = "" if bounces == 2 else "_3bounce"
+        methods[f"traced_4spp{sfx}"] = ("traced", bounces, None)
+        for term in (0, 1):
+            key = ("cache_at_gbuffer_point" if term == 0 else "cache_after_bounce1") + sfx
+            methods[key] = ("cache", bounces, vf.GiCache(ctx, terminate=term, bounces=bounces))
+    outs = {k: torch.zeros(n * 4, dtype=torch.float32, device=dev) for k in methods}
+    acc = {k: {"ms": 0.0, "mse": 0.0, "relmse": 0.0, "fallback_frames": 0} for k in methods}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for f in range(frames):
+        pos, right, up, fwd = vf.fly_camera(p0 + head * (step * f), yaw, pitch)
+        pu = vf.make_pass(pos, right, up, fwd, 70.0, f)
+        ctx.trace_primary(pu, W, H, hits, gbp, gbn)
+        judge = f >= frames - judged
+        refs = {b: _reference(ctx, pu, W, H, gbp, gbn, b).view(-1, 4) for b in (2, 3)} if judge else None
+        mask = gbp.view(-1, 4)[:, 3] >= 0.0
+        for k, (kind, bounces, g) in methods.items():
+            o = outs[k]
+            e0.record(stream)
+            if kind == "traced":
+                ctx.trace_gi_path(pu, W, H, gbp, gbn, o, frame=f, spp=4, salt=0x2f6b9a31, bounces=bounces)
+            else:
+                g.frame(pu, W, H, gbp, gbn, o, f)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if not judge:
+                continue
+            acc[k]["ms"] += e0.elapsed_time(e1)
+            mse, rel = _errors(o.view(-1, 4), refs[bounces], mask)
+            acc[k]["mse"] += mse
+            acc[k]["relmse"] += rel
+            if g is not None:
+                acc[k]["fallback_frames"] += int(g.stats()["fallback"])
+    out = {"frames": frames, "judged_frames": judged, "voxels_per_frame": step}
+    for k, (kind, bounces, g) in methods.items():
+        a = acc[k]
+        out[k] = {"bounces": bounces, "ms_per_frame": round(a["ms"] / judged, 4), "mse": round(a["mse"] / judged, 6),
+                  "relmse": round(a["relmse"] / judged, 5)}
+        if g is not None:
+            out[k]["fallback_frames"] = a["fallback_frames"]
+            g.close()
+    for k in methods:
+        if k.startswith("cache"):
+            base = out["traced_4spp" + ("_3bounce" if k.endswith("_3bounce") else "")]
+            out[k]["speedup_vs_traced_4spp"] = round(base["ms_per_frame"] / out[k]["ms_per_frame"], 3)
+            out[k]["mse_vs_traced_4spp"] = round(out[k]["mse"] / base["mse"], 4)
+    return out
 
 
 # forward FLOPs per query: 2 * (96*64 + 4*64*64 + 64*3)
