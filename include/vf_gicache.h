@@ -54,7 +54,7 @@ typedef struct vf_gicache_params {
     int32_t train_samples;   /* traced samples per frame (65536) */
     int32_t train_steps;     /* optimizer steps the frame's samples are split into (4) */
     int32_t spp;             /* bounce-1 samples per pixel when rendering with terminate = 1 (1..8) */
-    float lr, beta1, beta2, eps;   /* Adam (1e-3, 0.9, 0.99, 1e-8) */
+    float lr, beta1, beta2, eps;   /* Adam (1e-2, 0.9, 0.9, 1e-8) */
     float loss_eps;          /* relative L2 loss (y - t)^2 / (sg(mean(y))^2 + loss_eps) (0.01) */
     float fallback_ratio;    /* render the traced path while the running error > ratio x baseline (2) */
     int32_t warmup_frames;   /* frames rendered with the traced path while the cache first trains (16) */
@@ -67,6 +67,8 @@ typedef struct vf_gicache_params {
                               * three-bounce path of vf_trace_gi_path_n.  Training targets, the fallback path and
                               * (terminate 1) the tail's share follow it: with 3 the tail after bounce 1 has two
                               * bounces, so a terminate-1 sample is a traced three-bounce path */
+    float weight_ema;        /* the render's network weights: an exponential mean of the trained weights updated
+                              * at every Adam step, we = ema * we + (1 - ema) * w (0: the trained weights) */
 } vf_gicache_params;
 
 typedef struct vf_gicache_stats {
@@ -80,8 +82,10 @@ typedef struct vf_gicache_stats {
     int32_t n_valid_samples; /* valid training samples of the last frame */
 } vf_gicache_stats;
 
-/* Defaults: terminate 0, 65536 samples in 4 steps, spp 1, Adam (1e-3, 0.9, 0.99, 1e-8), loss_eps 0.01,
- * fallback 2x, warmup 16, enc_shift 8, seed 1337, salt 0x2f6b9a31, gi_reach 48, lights_nee 1, bounces 2. */
+/* Defaults: terminate 0, 65536 samples in 4 steps, spp 1, Adam (lr 1e-2, 0.9, 0.9, 1e-8), loss_eps 0.01,
+ * fallback 2x, warmup 16, enc_shift 8, seed 1337, salt 0x2f6b9a31, gi_reach 48, lights_nee 1, bounces 2,
+ * weight_ema 0.9.  (The fast learning rate tracks a moving camera; the weights' exponential mean removes
+ * most of the step noise it adds to the render.) */
 int vf_gicache_defaults(vf_gicache_params *p);
 int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out);
 void vf_gicache_destroy(vf_gicache *c);
@@ -121,10 +125,14 @@ int vf_gicache_samples(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int3
 /* --- the network, for checks and for callers with their own samples --- */
 /* FP32 master weights in parameter order (synchronous). set_params also clears the Adam moments. */
 int vf_gicache_get_params(vf_ctx *ctx, vf_gicache *c, float *host_out);
+/* The weights the render and vf_gicache_infer use (parameter order, synchronous): the exponential mean of the
+ * trained weights with weight_ema > 0, else the trained weights; set_params and reset set them equal. */
+int vf_gicache_get_render_params(vf_ctx *ctx, vf_gicache *c, float *host_out);
 int vf_gicache_set_params(vf_ctx *ctx, vf_gicache *c, const float *host_in);
 /* The 96 input features of n records (FP32, n x 96). */
 int vf_gicache_encode(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64_t n, float *enc);
-/* Network output for n records: RGB as FP16 in 4 halves per record (the 4th is 0). */
+/* Network output for n records with the render's weights (the trained ones, or their exponential mean with
+ * weight_ema > 0): RGB as FP16 in 4 halves per record (the 4th is 0). */
 int vf_gicache_infer(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64_t n, uint16_t *out_rgbh);
 /* Gradient (parameter order) of the mean loss over n samples at the current weights, and that loss
  * (sum over valid samples of the per-sample loss, divided by n); no update. */

@@ -126,6 +126,12 @@ def loss_grad(params, recs, shift, loss_eps, emulate, X=None, denom=None):
     return loss, pack(grads), denom
 
 
+def weight_ema(we, w, ema):
+    """The render's weights after one step: ema * we + (1 - ema) * w in FP32, the kernel's order."""
+    f = np.float32
+    return (f(ema) * np.asarray(we, np.float32) + (f(1.0) - f(ema)) * np.asarray(w, np.float32)).astype(np.float32)
+
+
 def adam(w, m, v, g, step, lr, b1, b2, eps):
     """One Adam step in FP32 with the kernel's operation order; returns (w, m, v)."""
     f = np.float32

@@ -126,15 +126,23 @@ def test_adam_matches_oracle_and_updates_the_network(net):
     m = np.zeros_like(w0)
     v = np.zeros_like(w0)
     w = w0
+    assert np.array_equal(net.get_render_params(), w0)
+    we = w0
+    ema = np.float32(p.weight_ema)
+    assert ema > 0
     for step in (1, 2):
         gh = (rng.normal(size=M.N_PARAMS) * 0.05).astype(np.float32)
         net.adam(torch.from_numpy(gh).to(DEV))
         w, m, v = M.adam(w, m, v, gh, step, p.lr, p.beta1, p.beta2, p.eps)
         got = net.get_params()
         assert np.array_equal(got, w), (step, np.abs(got - w).max())
-    # the BF16 copies the kernels read were rewritten
+        # the render's weights: we = ema * we + (1 - ema) * w, FP32 in that order
+        we = M.weight_ema(we, w, ema)
+        assert np.array_equal(net.get_render_params(), we), step
+    assert not np.array_equal(we, w)
+    # the BF16 copies the kernels read were rewritten: inference runs the render's weights
     recs = M.random_records(500, 4)
-    _, y = M.forward(w, gpu_encode(net, recs), emulate=True)
+    _, y = M.forward(we, gpu_encode(net, recs), emulate=True)
     assert np.all(np.abs(gpu_infer(net, recs)[:, :3] - y) <= 2e-3 + 2e-3 * np.abs(y))
 
 
@@ -202,7 +210,8 @@ def test_cached_render_matches_oracle(ctx, terminate, spp, bounces):
         out = torch.zeros(W * H * 4, dtype=torch.float32, device=DEV)
         g.frame(pu, W, H, gbp, gbn, out, 7)
         st = g.stats()
-        params = g.get_params()
+        params = g.get_render_params()
+        assert not np.array_equal(params, g.get_params())   # the exponential mean, one step behind
         got = out.cpu().numpy().reshape(-1, 4)
         if terminate == 1:
             pre, queries = ow.gicache_prepass(scene, W, H, gp, gn, 7, spp, g.params.salt, bounces=bounces)

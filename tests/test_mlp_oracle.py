@@ -118,3 +118,19 @@ def test_adam_first_steps_closed_form():
     # zero gradient from zero moments leaves the weights unchanged
     w3, m3, v3 = M.adam(w0, m0, m0, np.zeros_like(g), 1, lr, b1, b2, eps)
     assert np.array_equal(w3, w0) and not m3.any() and not v3.any()
+
+
+def test_weight_ema_closed_form():
+    """The render weights' exponential mean: from we_0 toward a fixed w, we_k = w + ema^k (we_0 - w); ema 0
+    is w itself; a constant sequence is a fixed point."""
+    rng = np.random.default_rng(3)
+    we0 = rng.normal(size=500).astype(np.float32)
+    w = rng.normal(size=500).astype(np.float32)
+    ema = 0.9
+    we = we0
+    for k in range(1, 6):
+        we = M.weight_ema(we, w, ema)
+        want = w.astype(np.float64) + (np.float64(np.float32(ema)) ** k) * (we0.astype(np.float64) - w)
+        assert np.allclose(we, want, rtol=1e-5, atol=1e-6), k
+    assert np.array_equal(M.weight_ema(we0, w, 0.0), w)
+    assert np.allclose(M.weight_ema(w, w, ema), w, rtol=4e-7, atol=0)   # FP32 rounding of the two products only

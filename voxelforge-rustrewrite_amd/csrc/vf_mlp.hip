@@ -404,6 +404,8 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int32_t n
 
 struct AdamArgs {
     float *w, *m, *v;
+    float *we;             // the inference weights: an exponential mean of w (w itself with ema 0)
+    float ema;
     const float *part;     // [kSlices][kPP]: the first-level sums (mode 1, 2 without grad_in)
     int32_t nparts;
     const float *grad_in;  // mode 2: the gradient instead of the partials
@@ -413,22 +415,23 @@ struct AdamArgs {
     bf16 *ffrag, *bfrag, *ifrag;
 };
 
-__device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) {
+// wv: the training weight (forward and backward fragments); wi: the inference weight.
+__device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv, float wi) {
     int l = 0;
     while (l < 5 && p >= w_off(l + 1)) ++l;
     if (p >= b_off(l)) return;   // a bias: no fragment
     const int q = p - w_off(l), nout = out_dim(l);
     const int in = q / nout, out = q % nout;
-    const bf16 bv = (bf16)wv;
+    const bf16 bv = (bf16)wv, bi = (bf16)wi;
     {
         const int mt = out / 16, kb = in / 32, ln = (out & 15) + 16 * ((in & 31) >> 3), i = in & 7;
         A.ffrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
         if (l == 0) {
-            A.ifrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bv;
+            A.ifrag[((size_t)(fwd_base(l) + mt * fwd_kb(l) + kb) * 64 + ln) * 8 + i] = bi;
         } else {   // in = perm_in(kb, g, j)
             const int t = in >> 4, r = in & 15;
             const int kb2 = t >> 1, g2 = r >> 2, j = ((t & 1) << 2) | (r & 3);
-            A.ifrag[((size_t)(fwd_base(l) + mt * 2 + kb2) * 64 + (out & 15) + 16 * g2) * 8 + j] = bv;
+            A.ifrag[((size_t)(fwd_base(l) + mt * 2 + kb2) * 64 + (out & 15) + 16 * g2) * 8 + j] = bi;
         }
     }
     if (l >= 1) {
@@ -438,12 +441,15 @@ __device__ __forceinline__ void write_frags(const AdamArgs &A, int p, float wv) 
 }
 
 // One parameter per thread: the gradient (the first-level sums added in slice order, or grad_in), then
-// Adam on the FP32 master weight and the BF16 fragment copies.
+// Adam on the FP32 master weight, the inference weight we = ema * we + (1 - ema) * w, and the BF16 fragment
+// copies (training fragments from w, inference fragments from we).  Mode 0 sets we = w.
 __global__ void __launch_bounds__(256) k_mlp_adam(AdamArgs A) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= kP) return;
     if (A.mode == 0) {
-        write_frags(A, p, A.w[p]);
+        const float w0 = A.w[p];
+        A.we[p] = w0;
+        write_frags(A, p, w0, w0);
         return;
     }
     float g;
@@ -464,7 +470,9 @@ __global__ void __launch_bounds__(256) k_mlp_adam(AdamArgs A) {
     A.m[p] = m;
     A.v[p] = v;
     A.w[p] = wv;
-    write_frags(A, p, wv);
+    const float wi = A.ema > 0.0f ? A.ema * A.we[p] + (1.0f - A.ema) * wv : wv;
+    A.we[p] = wi;
+    write_frags(A, p, wv, wi);
 }
 
 // Device statistics of the cache (the fallback decision stays on the device, so a frame needs no sync).
@@ -933,7 +941,7 @@ uint32_t host_pcg(uint32_t v) {
 struct vf_gicache {
     vf_ctx *ctx = nullptr;
     vf_gicache_params p{};
-    float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *we = nullptr;
     bf16 *ffrag = nullptr, *bfrag = nullptr, *ifrag = nullptr;
     float *part = nullptr, *part2 = nullptr, *loss_part = nullptr;
     int32_t *valid_part = nullptr;
@@ -957,7 +965,7 @@ int check_cache(vf_ctx *ctx, vf_gicache *c, const char *what) {
 }
 
 void free_all(vf_gicache *c) {
-    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->ffrag, c->bfrag, c->ifrag, c->part, c->part2, c->loss_part, c->valid_part,
+    void *ptrs[] = {c->w, c->m, c->v, c->grad, c->we, c->ffrag, c->bfrag, c->ifrag, c->part, c->part2, c->loss_part, c->valid_part,
                     c->samples, c->st, c->qcount, c->queries, c->pre, c->tailv};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -972,6 +980,8 @@ int launch_adam(vf_gicache *c, int mode, const float *grad_in, float *grad_out, 
     }
     AdamArgs A;
     A.w = c->w; A.m = c->m; A.v = c->v;
+    A.we = c->we;
+    A.ema = c->p.weight_ema;
     A.part = c->part2;
     A.nparts = nparts;
     A.grad_in = grad_in;
@@ -1083,6 +1093,8 @@ int check_params(vf_ctx *ctx, const vf_gicache_params *p) {
         return vf::fail(ctx, VF_E_INVALID, "vf_gicache: warmup_frames or enc_shift out of range");
     if (!(p->gi_reach > 0.0f) || p->gi_reach > 1e6f) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: gi_reach out of range");
     if (p->bounces != 2 && p->bounces != 3) return vf::fail(ctx, VF_E_INVALID, "vf_gicache: bounces must be 2 or 3");
+    if (!(p->weight_ema >= 0.0f && p->weight_ema < 1.0f))
+        return vf::fail(ctx, VF_E_INVALID, "vf_gicache: weight_ema must be in [0, 1)");
     return VF_OK;
 }
 
@@ -1146,9 +1158,9 @@ int vf_gicache_defaults(vf_gicache_params *p) {
     p->train_samples = 65536;
     p->train_steps = 4;
     p->spp = 1;
-    p->lr = 1e-3f;
+    p->lr = 1e-2f;
     p->beta1 = 0.9f;
-    p->beta2 = 0.99f;
+    p->beta2 = 0.9f;
     p->eps = 1e-8f;
     p->loss_eps = 0.01f;
     p->fallback_ratio = 2.0f;
@@ -1159,6 +1171,7 @@ int vf_gicache_defaults(vf_gicache_params *p) {
     p->gi_reach = 48.0f;
     p->lights_nee = 1;
     p->bounces = 2;
+    p->weight_ema = 0.9f;
     return VF_OK;
 }
 
@@ -1179,6 +1192,7 @@ int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out)
 #define VF_GC_ALLOC(ptr, bytes)                                              \
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes))
     VF_GC_ALLOC(c->w, sizeof(float) * kP);
+    VF_GC_ALLOC(c->we, sizeof(float) * kP);
     VF_GC_ALLOC(c->m, sizeof(float) * kP);
     VF_GC_ALLOC(c->v, sizeof(float) * kP);
     VF_GC_ALLOC(c->grad, sizeof(float) * kP);
@@ -1316,7 +1330,7 @@ int vf_gicache_frame(vf_ctx *ctx, vf_gicache *c, const vf_pass_ubo *pass, int32_
     I.count = c->qcount;
     I.shift = c->p.enc_shift;
     I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
-    I.w = c->w;
+    I.w = c->we;
     if (c->p.terminate == 0) {
         I.n = npix;
         I.gbpos = reinterpret_cast<const float4 *>(gb_pos);
@@ -1360,14 +1374,24 @@ int vf_gicache_read_stats(vf_ctx *ctx, vf_gicache *c, vf_gicache_stats *out) {
     return VF_OK;
 }
 
-int vf_gicache_get_params(vf_ctx *ctx, vf_gicache *c, float *host_out) {
-    int rc = check_cache(ctx, c, "vf_gicache_get_params");
+namespace {
+int read_params(vf_ctx *ctx, vf_gicache *c, const float *src, float *host_out, const char *what) {
+    int rc = check_cache(ctx, c, what);
     if (rc) return rc;
-    if (!host_out) return gc_fail_null(ctx, "vf_gicache_get_params");
-    if ((rc = vf::no_capture(ctx, "vf_gicache_get_params"))) return rc;
-    VF_HIP(ctx, hipMemcpyAsync(host_out, c->w, sizeof(float) * kP, hipMemcpyDeviceToHost, ctx->stream));
+    if (!host_out) return gc_fail_null(ctx, what);
+    if ((rc = vf::no_capture(ctx, what))) return rc;
+    VF_HIP(ctx, hipMemcpyAsync(host_out, src, sizeof(float) * kP, hipMemcpyDeviceToHost, ctx->stream));
     VF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return VF_OK;
+}
+}  // namespace
+
+int vf_gicache_get_params(vf_ctx *ctx, vf_gicache *c, float *host_out) {
+    return read_params(ctx, c, c ? c->w : nullptr, host_out, "vf_gicache_get_params");
+}
+
+int vf_gicache_get_render_params(vf_ctx *ctx, vf_gicache *c, float *host_out) {
+    return read_params(ctx, c, c ? c->we : nullptr, host_out, "vf_gicache_get_render_params");
 }
 
 int vf_gicache_set_params(vf_ctx *ctx, vf_gicache *c, const float *host_in) {
@@ -1401,7 +1425,7 @@ int vf_gicache_infer(vf_ctx *ctx, vf_gicache *c, const vf_gi_record *recs, int64
     I.n = n;
     I.shift = c->p.enc_shift;
     I.ifrag = reinterpret_cast<const bf16x8 *>(c->ifrag);
-    I.w = c->w;
+    I.w = c->we;
     I.out_h = out_rgbh;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_mlp_infer<0>, dim3(infer_groups(c, n)), dim3(256), 0, ctx->stream, I);

@@ -107,6 +107,7 @@ _SIGS = {
     "vf_gicache_read_stats": (C.c_int, [_P, _P, C.POINTER(GiCacheStats)]),
     "vf_gicache_samples": (C.c_int, [_P, _P, C.POINTER(PassUbo), C.c_int32, C.c_int32, _P, _P, C.c_uint32, _P]),
     "vf_gicache_get_params": (C.c_int, [_P, _P, _P]),
+    "vf_gicache_get_render_params": (C.c_int, [_P, _P, _P]),
     "vf_gicache_set_params": (C.c_int, [_P, _P, _P]),
     "vf_gicache_encode": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
     "vf_gicache_infer": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
@@ -865,6 +866,12 @@ class GiCache:
     def get_params(self):
         out = np.zeros(GICACHE_PARAMS, np.float32)
         self._ck(lib().vf_gicache_get_params(self.ctx._h, self._h, out.ctypes.data))
+        return out
+
+    def get_render_params(self):
+        """The weights the render uses (the exponential mean of the trained ones with weight_ema > 0)."""
+        out = np.zeros(GICACHE_PARAMS, np.float32)
+        self._ck(lib().vf_gicache_get_render_params(self.ctx._h, self._h, out.ctypes.data))
         return out
 
     def set_params(self, w):

@@ -152,7 +152,7 @@ class GiCacheParams(C.Structure):
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("loss_eps", C.c_float), ("fallback_ratio", C.c_float), ("warmup_frames", C.c_int32),
                 ("enc_shift", C.c_int32), ("seed", C.c_uint32), ("salt", C.c_uint32), ("gi_reach", C.c_float),
-                ("lights_nee", C.c_int32), ("bounces", C.c_int32)]
+                ("lights_nee", C.c_int32), ("bounces", C.c_int32), ("weight_ema", C.c_float)]
 
 
 class GiCacheStats(C.Structure):
@@ -201,7 +201,7 @@ assert C.sizeof(Material) == 48
 assert C.sizeof(Ray) == 24 and RAY_DTYPE.itemsize == 24
 assert C.sizeof(Hit) == 20 and HIT_DTYPE.itemsize == 20
 assert GI_RECORD_DTYPE.itemsize == 64
-assert C.sizeof(GiCacheParams) == 68 and C.sizeof(GiCacheStats) == 40
+assert C.sizeof(GiCacheParams) == 72 and C.sizeof(GiCacheStats) == 40
 
 assert C.sizeof(ObjHeader) == 240 and OBJ_HEADER_DTYPE.itemsize == 240
 assert C.sizeof(BvhNode) == 32 and BVH_NODE_DTYPE.itemsize == 32
