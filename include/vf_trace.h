@@ -369,8 +369,10 @@ int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int
  * fit) and how many lanes must be idle before a wavefront fetches new rays (1..64; 0 = chunk mode: a
  * wavefront traces its claimed chunk 64 rays at a time, each to completion).  Results never depend
  * on these. */
-/* refill -1 (vf_trace_gi_batch only): no queue, one lane per pixel traces its GI ray and then its sun ray
- * (the autotuner measures it beside the queue configurations). */
+/* refill -1: no queue.  vf_trace_gi_batch: one lane per pixel traces its GI ray and then its sun ray.
+ * vf_trace_batch: one launch with a wavefront per 64 consecutive ray ids of the requests (in order, tile
+ * order within a view), each traced to completion; completion stamps as in the queue modes.  The autotuner
+ * measures it beside the queue configurations. */
 typedef struct vf_queue_cfg {
     int32_t tile_w, tile_h, chunk, sort, groups_per_cu, refill;
 } vf_queue_cfg;
@@ -423,8 +425,9 @@ int vf_trace_batch_noalloc(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, c
 int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms);
 
 /* Autotuner: times every refill mode {chunk mode, refill 32} x tile shape {8x8, 16x8, 16x16} x chunk
- * {64..1024} (x sort off/on for the GI batch) on the given batch (median of 3 runs) and returns the
- * fastest configuration; base (nullable) supplies groups_per_cu. */
+ * {64..1024} (x sort off/on for the GI batch), plus refill -1 (no queue: the GI batch at 16x16, the batch
+ * at each tile shape), on the given batch (median of 3 runs) and returns the fastest configuration; base
+ * (nullable) supplies groups_per_cu. */
 enum { VF_TUNE_GI_BATCH = 0, VF_TUNE_BATCH = 1 };
 typedef struct vf_tune_result { vf_queue_cfg cfg; float ms; } vf_tune_result;
 typedef struct vf_tune_gi_args {

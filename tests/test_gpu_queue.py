@@ -101,7 +101,8 @@ def test_uneven_batch_matches_plain_launches(ctx):
     probes = [((-64.5, -32.25, -64.75), 20.0, (8, 6, 8), 256, 3, 256.0)]
     reqs, outs, passes = _requests(ctx, sizes, probes)
     for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 1024), ctx.queue_cfg((16, 8), 256),
-                ctx.queue_cfg((8, 8), 64, refill=0), ctx.queue_cfg((16, 16), 512, refill=0)):
+                ctx.queue_cfg((8, 8), 64, refill=0), ctx.queue_cfg((16, 16), 512, refill=0),
+                ctx.queue_cfg((8, 8), 64, refill=-1), ctx.queue_cfg((16, 8), 256, refill=-1)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)
@@ -171,7 +172,7 @@ def test_autotuner_picks_a_measured_config(ctx):
 def test_full_config5_batch_matches_plain_launches_and_oracle(ctx):
     """The uneven batch at its full size: 16 seeded views drawn from {256^2, 512x288, 960x540, 1024^2} and a
     32x16x32 probe grid x 256 spherical-Fibonacci rays (about 13M rays) in one launch, in chunk mode and in
-    refill mode.  Every request's hit records equal its own plain launch byte for byte, and every 97th ray of
+    refill mode and in the flat one-launch mode.  Every request's hit records equal its own plain launch byte for byte, and every 97th ray of
     every request equals the oracle's traceVolumeS on the same ray."""
     import math
     _, a, ow = world(1)
@@ -197,7 +198,8 @@ def test_full_config5_batch_matches_plain_launches_and_oracle(ctx):
         ctx.trace_primary(vp, vw, vh, r)
     ctx.trace_probes(origin, spacing, dims, nr, frame, reach, ref[-1])
     ctx.sync()
-    for cfg in (ctx.queue_cfg((16, 8), 512, refill=0), ctx.queue_cfg((8, 8), 256, refill=32)):
+    for cfg in (ctx.queue_cfg((16, 8), 512, refill=0), ctx.queue_cfg((8, 8), 256, refill=32),
+                ctx.queue_cfg((16, 16), 64, refill=-1)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)

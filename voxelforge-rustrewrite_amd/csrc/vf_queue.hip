@@ -672,6 +672,45 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, DevBodies Bo,
     flush_count<COUNT>(c, cnt);
 }
 
+// Flat mode of the uneven batch (refill -1): no queue.  One launch with a workgroup per 256 consecutive ray
+// ids of the batch (requests in their service order, each 256-aligned, so a workgroup serves one request),
+// dealt to the XCDs in runs like the plain passes; each lane traces one id to completion, and the workgroup
+// subtracts its ids from the request's count in one atomic, so the request's last workgroup stamps its
+// completion as in the queue modes.  (A per-wave atomic serialised on the few counters: 3x slower.)  The
+// start stamp is the first workgroup's.
+template <bool COUNT, bool BODIES>
+__global__ void __launch_bounds__(256, BODIES ? 1 : 7) k_batch_flat(DevWorld W, DevBodies Bo, BatchArgs B, unsigned long long *ctr,
+                                                    unsigned long long *stamps, unsigned long long *cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
+    const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const uint64_t b = (uint64_t)L * 256u;   // the workgroup's first id
+    if (b >= B.total) return;
+    const int rq = find_req(B, b);
+    const ReqDev &q = B.req[rq];
+    const uint64_t rend = q.first + q.count;
+    if (b >= rend) return;   // the alignment padding after a request
+    Cnt c = {0, 0, 0, 0};
+    const uint64_t id = b + threadIdx.x;
+    if (id < rend) {
+        float ro[3], rd[3], tcap, tskip;
+        uint32_t out;
+        int skip;
+        bool void_ray;
+        batch_ray(W, q, id - q.first, ro, rd, tcap, out, skip, tskip, void_ray);
+        if (out != 0xFFFFFFFFu) {
+            Hit h;
+            trace_world<-1, COUNT, true>(W, ro, rd, tcap, h, c, skip);
+            if (BODIES) batch_bodies<COUNT>(W, Bo, q, ro, rd, h, c);
+            h.t = h.t + tskip;
+            if (void_ray) h.hit = 0;
+            store_hit(q.hits + out, h, BODIES && h.hit && h.obj >= 0);
+        }
+    }
+    flush_count<COUNT>(c, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) retire(ctr, stamps, rq, (unsigned long long)((rend - b) < 256u ? (rend - b) : 256u));
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -873,8 +912,6 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
     if (noalloc && !ctx->batch_scratch)
         return vf::fail(ctx, VF_E_UNSUPPORTED, "%s: no batch scratch on device %d", what, ctx->device);
     if ((rc = check_cfg(ctx, cfg_in, what))) return rc;
-    if (cfg_in && cfg_in->refill < 0)
-        return vf::fail(ctx, VF_E_INVALID, "%s: refill -1 (no queue) is a GI-batch mode only", what);
     const vf_queue_cfg cfg = cfg_in ? *cfg_in : default_cfg();
     BatchArgs B;
     std::memset(&B, 0, sizeof(B));
@@ -916,7 +953,8 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
             return vf::fail(ctx, VF_E_INVALID, "%s: request %d: unknown kind %d", what, k, q.kind);
         }
         d.first = at;
-        at += (d.count + (uint64_t)cfg.chunk - 1) / (uint64_t)cfg.chunk * (uint64_t)cfg.chunk;
+        const uint64_t align = cfg.refill < 0 ? 256u : (uint64_t)cfg.chunk;   // flat: one workgroup, one request
+        at += (d.count + align - 1) / align * align;
     }
     B.total = at;
     (void)counts;
@@ -933,7 +971,14 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
     const bool bodies = Bo.n > 0, counting = ctx->counting != 0;
     if ((rc = vf::begin_timed(ctx))) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, ctx->stream, B, ctr, stamps);
-    if (cfg.refill == 0) {
+    if (cfg.refill < 0) {   // flat: one workgroup per 256 ids, no queue
+        using FlatFn = void (*)(DevWorld, DevBodies, BatchArgs, unsigned long long *, unsigned long long *,
+                                unsigned long long *);
+        const FlatFn fn = bodies ? (counting ? k_batch_flat<true, true> : k_batch_flat<false, true>)
+                                 : (counting ? k_batch_flat<true, false> : k_batch_flat<false, false>);
+        hipLaunchKernelGGL(fn, dim3((unsigned)(B.total / 256u)), dim3(256), 0, ctx->stream, ctx->W, Bo, B, ctr, stamps,
+                           ctx->d_cnt);
+    } else if (cfg.refill == 0) {
         using ChunkFn = void (*)(DevWorld, DevBodies, BatchArgs, unsigned long long *, unsigned long long *, int32_t,
                                  unsigned long long *);
         const ChunkFn fn = bodies ? (counting ? k_batch_chunked<true, true> : k_batch_chunked<false, true>)
@@ -985,7 +1030,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     static const int tiles[3][2] = {{8, 8}, {16, 8}, {16, 16}};
     static const int chunks[5] = {64, 128, 256, 512, 1024};
     const int sorts = workload == VF_TUNE_GI_BATCH ? 2 : 1;
-    static const int refills[3] = {-1, 0, 32};   // -1: no queue (GI batch only, one configuration)
+    static const int refills[3] = {-1, 0, 32};   // -1: no queue (GI batch: one configuration; batch: per tile)
     int k = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
@@ -996,7 +1041,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     for (int so = 0; so < sorts && rc == VF_OK; ++so)
         for (const auto &t : tiles)
             for (int ch : chunks) {
-                if (rf < 0 && (workload != VF_TUNE_GI_BATCH || so != 0 || t[0] != 16 || t[1] != 16 || ch != 64)) continue;
+                if (rf < 0 && (ch != 64 || (workload == VF_TUNE_GI_BATCH && (so != 0 || t[0] != 16 || t[1] != 16)))) continue;
                 vf_queue_cfg c = default_cfg();
                 c.tile_w = t[0]; c.tile_h = t[1]; c.chunk = ch; c.sort = so;
                 c.refill = rf;
