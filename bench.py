@@ -89,7 +89,9 @@ def main():
     ap.add_argument("--extras", default="temporal,dist,pack,forest,other",
                     help="comma list of the extra measurements: temporal (the passes after the trace), dist (the "
                          "brick distance field over the whole world), pack (the world packed on the device), forest "
-                         "(open forest with bodies), other (fixed world, GI batch, GI cache, probes, uneven batch)")
+                         "(open forest with bodies), other (all of the fixed world's lines: fixed = primary + sun "
+                         "rays at 640x360, gibatch = the GI batch, cache = the learned GI cache, batch = probes + "
+                         "16 views in one launch; each may be named on its own)")
     ap.add_argument("--no-graph", action="store_true",
                     help="record no graphs at all (also the kernel-duration loop launches eagerly)")
     ap.add_argument("--graph-steps", action="store_true",
@@ -292,8 +294,9 @@ def main():
             result["world_pack"] = pack_config(ctx, world, scene)
         if "forest" in extras:
             result["open_forest_1080p"] = open_forest_config(ctx, stream, args, hits, gbp, gbn, sh)
-        if "other" in extras:
-            result["other_configs"] = other_configs(ctx, args)
+        parts = {"fixed", "gibatch", "cache", "batch"} if "other" in extras else extras & {"fixed", "gibatch", "cache", "batch"}
+        if parts:
+            result["other_configs"] = other_configs(ctx, args, parts)
     ctx.close()
     print(json.dumps(result), flush=True)
 
@@ -565,11 +568,11 @@ def graph_ms(ctx, stream, fn, k):
     return e0.elapsed_time(e1) / k
 
 
-def other_configs(ctx, args):
-    """The fixed world (256x128x256 voxels): primary + sun rays at 640x360, the GI batch at 1920x1080
-    (1 GI ray per pixel, reach 640 voxels, 1 sun ray per GI hit; autotuned queue), the learned GI
-    cache at 1920x1080, and one launch of a 32x16x32 DDGI probe grid (256 rays each) plus 16 views of
-    mixed sizes."""
+def other_configs(ctx, args, parts):
+    """The fixed world (256x128x256 voxels): primary + sun rays at 640x360 ("fixed"; with the fused kernel's
+    time at 320x180 and 1280x720 beside it), the GI batch at 1920x1080 ("gibatch": 1 GI ray per pixel, reach
+    640 voxels, 1 sun ray per GI hit; autotuned queue), the learned GI cache at 1920x1080 ("cache"), and one
+    launch of a 32x16x32 DDGI probe grid (256 rays each) plus 16 views of mixed sizes ("batch")."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     ctx.set_stream(stream)
@@ -581,22 +584,52 @@ def other_configs(ctx, args):
     scene = vf.default_scene((32, 16, 32))
     ctx.set_scene(scene)
     cam_pos, right, up, fwd = vf.fly_camera([37.3, 96.4, 41.7], 0.8, -0.35)
-    # fixed world, 640x360
-    w1, h1 = 640, 360
-    n1 = w1 * h1
+    if "fixed" in parts:
+        out.update(fixed_world_config(ctx, stream, args, cam_pos, right, up, fwd))
+    if parts & {"gibatch", "cache"}:
+        gibatch_and_cache(ctx, stream, args, cam_pos, right, up, fwd, parts, out)
+    if "batch" in parts:
+        out["uneven_batch"] = uneven_batch_config(ctx, stream, args)
+    ctx.set_stream(prev)
+    torch.cuda.set_stream(prev)
+    return out
+
+
+def fixed_world_config(ctx, stream, args, cam_pos, right, up, fwd):
+    """Primary + sun rays on the fixed world at 640x360 (the fused pass in a graph of one launch, so the
+    launch is included), and the same frame at 320x180 and 1280x720 with the fused kernel alone (20
+    back-to-back launches in one graph): its time hardly changes from 320x180 to 640x360, the latency of
+    the frame's longest wavefronts."""
+    dev = torch.device("cuda:0")
+    out = {}
     pu = vf.make_pass(cam_pos, right, up, fwd, 70.0, 0)
-    hits = torch.empty(n1 * HIT_BYTES, dtype=torch.uint8, device=dev)
-    gbp = torch.empty(n1 * 4, dtype=torch.float32, device=dev)
-    gbn = torch.empty(n1, dtype=torch.int32, device=dev)
-    sh = torch.empty(n1, dtype=torch.int32, device=dev)
-    ctx.set_counting(True)
-    ctx.counters(reset=True)
-    ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh)
-    c1 = ctx.counters(reset=True)
-    ctx.set_counting(False)
-    ms = graph_ms(ctx, stream, lambda: ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh), args.steps)
-    out["fixed_world_640x360"] = {"rays": int(c1["rays"]), "ms": round(ms, 5),
-                                  "grays": round(c1["rays"] / (ms * 1e-3) / 1e9, 4)}
+    scaling = {}
+    for (w1, h1) in ((640, 360), (320, 180), (1280, 720)):
+        n1 = w1 * h1
+        hits = torch.empty(n1 * HIT_BYTES, dtype=torch.uint8, device=dev)
+        gbp = torch.empty(n1 * 4, dtype=torch.float32, device=dev)
+        gbn = torch.empty(n1, dtype=torch.int32, device=dev)
+        sh = torch.empty(n1, dtype=torch.int32, device=dev)
+        ctx.set_counting(True)
+        ctx.counters(reset=True)
+        ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh)
+        c1 = ctx.counters(reset=True)
+        ctx.set_counting(False)
+        fn = lambda: ctx.trace_primary_shadow(pu, w1, h1, hits, gbp, gbn, sh)   # noqa: E731
+        if (w1, h1) == (640, 360):
+            ms = graph_ms(ctx, stream, fn, args.steps)
+            out["fixed_world_640x360"] = {"rays": int(c1["rays"]), "ms": round(ms, 5),
+                                          "grays": round(c1["rays"] / (ms * 1e-3) / 1e9, 4)}
+        reps = 20
+        ms_k = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
+        scaling[f"{w1}x{h1}"] = {"rays": int(c1["rays"]), "kernel_us": round(ms_k * 1e3, 2),
+                                 "grays": round(c1["rays"] / (ms_k * 1e-3) / 1e9, 4)}
+    out["fixed_world_frame_sizes"] = scaling
+    return out
+
+
+def gibatch_and_cache(ctx, stream, args, cam_pos, right, up, fwd, parts, out):
+    dev = torch.device("cuda:0")
     # GI batch at 1080p
     w3, h3 = 1920, 1080
     n3 = w3 * h3
@@ -605,6 +638,10 @@ def other_configs(ctx, args):
     gbp3 = torch.empty(n3 * 4, dtype=torch.float32, device=dev)
     gbn3 = torch.empty(n3, dtype=torch.int32, device=dev)
     ctx.trace_primary(pu3, w3, h3, hits3, gbp3, gbn3)
+    if "cache" in parts:
+        out["learned_gi_cache_1080p"] = gi_cache_config(ctx, stream, args, pu3, w3, h3, gbp3, gbn3)
+    if "gibatch" not in parts:
+        return
     gi = torch.empty(n3 * HIT_BYTES, dtype=torch.uint8, device=dev)
     sun = torch.empty(n3 * HIT_BYTES, dtype=torch.uint8, device=dev)
     best, table = ctx.autotune_gi_batch(pu3, w3, h3, 640.0, gbp3, gbn3, gi, sun)
@@ -621,8 +658,13 @@ def other_configs(ctx, args):
                                        "refill": best.refill},
                              "autotune_candidates": len(table),
                              "plain_gi_pass_ms": round(ms3p, 5)}
-    out["learned_gi_cache_1080p"] = gi_cache_config(ctx, stream, args, pu3, w3, h3, gbp3, gbn3)
-    # uneven batch: probes + 16 views, one launch
+
+
+def uneven_batch_config(ctx, stream, args):
+    """One launch of a 32x16x32 DDGI probe grid (256 rays each, 2 m = 20 voxel spacing) plus 16 seeded views
+    drawn from {256^2, 512x288, 960x540, 1024^2} on the fixed world (autotuned mode), against the same
+    requests as sequential plain launches (one graph each)."""
+    dev = torch.device("cuda:0")
     rng = np.random.default_rng(args.seed)
     sizes = [(256, 256), (512, 288), (960, 540), (1024, 1024)]
     views = []
@@ -652,16 +694,13 @@ def other_configs(ctx, args):
         ctx.trace_probes(origin, 20.0, dims, nr, 7, 256.0, probe_out)
 
     ms5p = graph_ms(ctx, stream, plain, 20)
-    out["uneven_batch"] = {"rays": total_rays, "ms": round(ms5, 5), "grays": round(total_rays / (ms5 * 1e-3) / 1e9, 4),
-                           "view_latency_ms_p50": round(float(np.percentile(view_t, 50)), 5),
-                           "view_latency_ms_p99": round(float(np.percentile(view_t, 99)), 5),
-                           "probes_done_ms": round(float(times[16]), 5),
-                           "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill},
-                           "sequential_plain_launches_ms": round(ms5p, 5),
-                           "view_sizes": [[vw, vh] for vw, vh, _ in views]}
-    ctx.set_stream(prev)
-    torch.cuda.set_stream(prev)
-    return out
+    return {"rays": total_rays, "ms": round(ms5, 5), "grays": round(total_rays / (ms5 * 1e-3) / 1e9, 4),
+            "view_latency_ms_p50": round(float(np.percentile(view_t, 50)), 5),
+            "view_latency_ms_p99": round(float(np.percentile(view_t, 99)), 5),
+            "probes_done_ms": round(float(times[16]), 5),
+            "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill},
+            "sequential_plain_launches_ms": round(ms5p, 5),
+            "view_sizes": [[vw, vh] for vw, vh, _ in views]}
 
 
 def _errors(x, ref, mask):
