@@ -44,7 +44,7 @@ struct vf_ctx {
     uint32_t n_slots = 0;
     std::vector<uint32_t> host_grid;   // host copies of the grid and of the slots' occupancy bits
     std::vector<uint64_t> host_occ;    // (8 words per slot): classify the camera voxel without a device read
-    size_t n_cells = 0, n_cells_tiled = 0;
+    size_t n_cells = 0;
     // the atlas bookkeeping of GpuWorld (gpu_world.rs:146-186): slots handed out so far, the free list
     // (popped from the back), which slots hold world bricks (only those are freed by chunk syncs)
     uint32_t atlas_used = 0;

@@ -1126,16 +1126,14 @@ __global__ void __launch_bounds__(256) k_dist_z2(DistArgs A, const uint8_t *tmp,
 // Cell words over the whole grid: grid value, both distance fields and the coarse-cell flags.
 __global__ void __launch_bounds__(256) k_pack(const uint32_t *grid, const uint8_t *d0, const uint8_t *d2, const uint8_t *c1,
                                               const uint8_t *c2, int32_t wbx, int32_t wby, int32_t wbz, int32_t c1x,
-                                              int32_t c1y, int32_t c2x, int32_t c2y, int32_t tdx, int32_t tdy,
-                                              uint2 *cells) {
+                                              int32_t c1y, int32_t c2x, int32_t c2y, uint2 *cells) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)wbx * wby * wbz) return;
     const int32_t x = (int32_t)(i % wbx), y = (int32_t)((i / wbx) % wby), z = (int32_t)(i / ((int64_t)wbx * wby));
     const uint32_t e2 = c2[((size_t)(z >> 4) * c2y + (y >> 4)) * c2x + (x >> 4)] == 0u ? 1u : 0u;
     const uint32_t e1 = c1[((size_t)(z >> 2) * c1y + (y >> 2)) * c1x + (x >> 2)] == 0u ? 1u : 0u;
     const uint32_t meta = (uint32_t)d0[i] | ((uint32_t)d2[i] << 8) | (e2 << 16) | (e1 << 17);
-    const uint32_t tile = ((uint32_t)(z >> 2) * (uint32_t)tdy + (uint32_t)(y >> 2)) * (uint32_t)tdx + (uint32_t)(x >> 2);
-    cells[(size_t)tile * 64u + (uint32_t)(((z & 3) << 4) | ((y & 3) << 2) | (x & 3))] = make_uint2(grid[i], meta);
+    cells[i] = make_uint2(grid[i], meta);
 }
 
 // Sky-top map (grasstop.comp:121-146 skyTop): 1 + the highest voxel of the column whose material
@@ -1315,7 +1313,7 @@ int repack(vf_ctx *ctx) {
     const int32_t *wb = ctx->W.wb;
     hipLaunchKernelGGL(k_pack, dim3(blocks_for((int64_t)ctx->n_cells)), dim3(256), 0, ctx->stream, ctx->grid, ctx->dist[0],
                        ctx->dist[2], ctx->c1, ctx->c2, wb[0], wb[1], wb[2], ctx->c1d[0], ctx->c1d[1], ctx->c2d[0],
-                       ctx->c2d[1], ctx->W.tdim[0], ctx->W.tdim[1], ctx->cells);
+                       ctx->c2d[1], ctx->cells);
     return launch_check(ctx, "cell words");
 }
 
@@ -1616,7 +1614,6 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     W = DevWorld{};
     for (int a = 0; a < 3; ++a) {
         W.wb[a] = wb_in[a];
-        W.tdim[a] = (W.wb[a] + 3) / 4;
         ctx->c1d[a] = (W.wb[a] + 3) / 4;
         ctx->c2d[a] = (W.wb[a] + 15) / 16;
     }
@@ -1628,7 +1625,6 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     W.max_outer = 3 * (W.wb[0] + W.wb[1] + W.wb[2]) + 4;
     const size_t n = (size_t)W.wb[0] * W.wb[1] * W.wb[2];
     ctx->n_cells = n;
-    ctx->n_cells_tiled = (size_t)W.tdim[0] * W.tdim[1] * W.tdim[2] * 64;
     ctx->n_slots = n_slots;
     const size_t n1 = (size_t)ctx->c1d[0] * ctx->c1d[1] * ctx->c1d[2], n2 = (size_t)ctx->c2d[0] * ctx->c2d[1] * ctx->c2d[2];
     const size_t slots = std::max<size_t>(n_slots, 1);
@@ -1642,7 +1638,7 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
         }                                                                                                        \
     } while (0)
     VF_ALLOC(ctx->grid, n * 4);
-    VF_ALLOC(ctx->cells, ctx->n_cells_tiled * 8);
+    VF_ALLOC(ctx->cells, n * 8);
     VF_ALLOC(ctx->atlas, slots * 512);
     VF_ALLOC(ctx->occ, slots * 64);
     VF_ALLOC(ctx->occdry, slots * 64);
@@ -1656,7 +1652,7 @@ int world_prepare(vf_ctx *ctx, const int32_t wb_in[3], const int32_t win[4], uin
     VF_ALLOC(ctx->skytop, nsky * 2);
     VF_ALLOC(ctx->palette, 256 * 48);
 #undef VF_ALLOC
-    VF_HIP(ctx, hipMemsetAsync(ctx->cells, 0, ctx->n_cells_tiled * 8, ctx->stream));
+    VF_HIP(ctx, hipMemsetAsync(ctx->cells, 0, n * 8, ctx->stream));
     VF_HIP(ctx, hipMemsetAsync(ctx->atlas, 0, slots * 512, ctx->stream));
     VF_HIP(ctx, hipMemsetAsync(ctx->masks_ref, 0, slots * 64, ctx->stream));
     VF_HIP(ctx, hipMemsetAsync(ctx->occ, 0, slots * 64, ctx->stream));
