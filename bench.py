@@ -271,6 +271,10 @@ def main():
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 5),
             "traffic": traffic,
+            # what binds the kernel instead (no MFMA work exists on this path): its SQ counters show the
+            # VALU issue slots filled for the launch's whole duration
+            "binding": "VALU issue (SQ_ACTIVE_INST_VALU x 4 / SIMDs = the launch's cycles; "
+                       "profiles/k_primary_shadow_endless_1080p.md)",
         },
     }
 
