@@ -49,7 +49,8 @@ CALLS = [
 def test_noalloc_batch_matches_the_allocating_batch():
     ctx = _fresh_ctx()
     try:
-        for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 512, refill=0)):
+        for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 512, refill=0),
+                    ctx.queue_cfg((8, 8), 64, refill=-1)):
             for sizes, probes, seed in CALLS:   # the second call is the smaller one
                 reqs, outs = _batch(ctx, sizes, probes, seed)
                 ctx.trace_batch_noalloc(reqs, cfg)
