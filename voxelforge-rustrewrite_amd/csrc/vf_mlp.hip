@@ -47,6 +47,33 @@ __host__ __device__ constexpr int out_dim(int l) { return l == 5 ? 3 : 64; }
 __host__ __device__ constexpr int w_off(int l) { return l == 0 ? 0 : 96 * 64 + 64 + (l - 1) * (64 * 64 + 64); }
 __host__ __device__ constexpr int b_off(int l) { return w_off(l) + in_dim(l) * out_dim(l); }
 static_assert(b_off(5) + 3 == kP, "parameter count");
+// The training kernel's gradient partials keep each layer's weights in the order its MFMA tiles hold them:
+// weight (row, out) of layer l < 5 at w_off(l) + ((row / 16 * 4 + out / 16) * 64 + lane) * 4 + row % 4 with
+// lane = (row % 16 / 4) * 16 + out % 16; of layer 5 at w_off(5) + ((row / 4) * 3 + out) * 4 + row % 4.
+// Biases keep their places.
+__host__ __device__ constexpr int part_index(int p) {
+    if (p >= w_off(5)) {
+        if (p >= b_off(5)) return p;
+        const int q = p - w_off(5), row = q / 3, o = q % 3;
+        return w_off(5) + ((row >> 2) * 3 + o) * 4 + (row & 3);
+    }
+    const int l = p < w_off(1) ? 0 : 1 + (p - w_off(1)) / (64 * 64 + 64);
+    if (p >= b_off(l)) return p;
+    const int q = p - w_off(l), row = q >> 6, o = q & 63;
+    return w_off(l) + (((row >> 4) * 4 + (o >> 4)) * 64 + ((row & 15) >> 2) * 16 + (o & 15)) * 4 + (row & 3);
+}
+constexpr bool part_index_is_permutation() {
+    bool seen[kP] = {};
+    for (int p = 0; p < kP; ++p) {
+        const int q = part_index(p);
+        if (q < 0 || q >= kP || seen[q]) return false;
+        seen[q] = true;
+    }
+    return true;
+}
+static_assert(part_index_is_permutation(), "the partials' tile order is a permutation of the parameters");
+static_assert(w_off(1) % 4 == 0 && w_off(2) % 4 == 0 && w_off(3) % 4 == 0 && w_off(4) % 4 == 0 && w_off(5) % 4 == 0,
+              "16-byte tile stores");
 // Forward fragments of layer l: (out padded to 16)/16 row tiles x in/32 K blocks; backward fragments
 // (layers 1..5): in/16 row tiles x (out padded to 32)/32 K blocks.  1 KB each.
 __host__ __device__ constexpr int fwd_mt(int l) { return l == 5 ? 1 : 4; }
@@ -67,7 +94,9 @@ static_assert(bwd_base(5) + 4 * bwd_kb(5) == kBwdFrags, "backward fragments");
 constexpr int kS0 = 104;   // encoded input, 96 features
 constexpr int kSH = 72;    // hidden, 64 features
 constexpr int kSG = 40;    // output gradient, 3 used of 32
-constexpr int kTrainLds = 64 * kS0 + 5 * 64 * kSH + 64 * kSG;   // BF16 elements per training wavefront
+// BF16 elements of a training workgroup's tiles: the encoded input, five hidden activations, the output
+// gradient and two hidden gradients.
+constexpr int kTrainLds = 64 * kS0 + 5 * 64 * kSH + 64 * kSG + 2 * 64 * kSH;
 constexpr int kBiasLds = 5 * 64 + 4;
 
 // sin(pi u), cos(pi u): u - rint(u) is exact, then Taylor polynomials on [-pi/2, pi/2] (terms to x^11
@@ -132,147 +161,162 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f3
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Hidden layer l (KB = in/32): act_out = bf16(relu(W^T . act_in + b)) for the chunk's 64 samples; wave
-// w of nw computes the output row tiles mt = w, w + nw, ...
-template <int KB>
-__device__ __forceinline__ void hidden_layer(const bf16 *act_in, int s_in, bf16 *act_out, const bf16x8 *frag,
-                                             const float *bias, int lane, int w, int nw) {
-    const int col = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int mt = w; mt < 4; mt += nw) {
-        bf16x8 a[KB];
+// A training workgroup is four wavefronts sharing one 64-sample chunk at a time.  Wave w owns row tile w
+// (features 16 w .. 16 w + 15) of every 64-wide product of the forward and backward passes, and the
+// weight-gradient tiles (k, w) of every layer (row tile k of the layer's inputs, column tile w of its
+// outputs; tile (w, 0) of the 3-output layer).
+constexpr int kTrainWaves = 4;
+
+// What a training wavefront reads of the weights, loaded into registers once per launch (not at the start
+// of each of the eleven products): its forward fragments (layers 0..4 and the output layer), its backward
+// fragments (layers 5..1) and its biases.
+struct TrainWeights {
+    bf16x8 f0[3];      // layer 0, K blocks 0..2
+    bf16x8 fh[4][2];   // layers 1..4
+    bf16x8 fo[2];      // output layer
+    bf16x8 b5[1];      // backward of layer 5 (one K block)
+    bf16x8 bh[4][2];   // backward of layers 1..4
+    float bz[5][4];    // hidden biases 16 w + 4 g + i
+    float bo[3];       // output biases
+};
+
+__device__ __forceinline__ void load_train_weights(const bf16x8 *ff, const bf16x8 *bf, const float *w, int lane, int wv,
+                                                   TrainWeights &T) {
+    const int g = lane >> 4;
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
-        float bz[4];
+    for (int kb = 0; kb < 3; ++kb) T.f0[kb] = ff[(fwd_base(0) + wv * 3 + kb) * 64 + lane];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) bz[i] = bias[mt * 16 + 4 * g + i];
-        f32x4 acc[4];
+    for (int l = 1; l < 5; ++l)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(act_in + (nt * 16 + col) * s_in + kb * 32 + 8 * g);
-                acc[nt] = mfma(a[kb], b, acc[nt]);
-            }
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            bf16x4 o;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float v = acc[nt][i] + bz[i];
-                o[i] = (bf16)(v > 0.0f ? v : 0.0f);
-            }
-            *reinterpret_cast<bf16x4 *>(act_out + (nt * 16 + col) * kSH + mt * 16 + 4 * g) = o;
+        for (int kb = 0; kb < 2; ++kb) {
+            T.fh[l - 1][kb] = ff[(fwd_base(l) + wv * 2 + kb) * 64 + lane];
+            T.bh[l - 1][kb] = bf[(bwd_base(l) + wv * 2 + kb) * 64 + lane];
         }
-    }
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) T.fo[kb] = ff[(fwd_base(5) + kb) * 64 + lane];
+    T.b5[0] = bf[(bwd_base(5) + wv) * 64 + lane];
+#pragma unroll
+    for (int l = 0; l < 5; ++l)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T.bz[l][i] = w[b_off(l) + wv * 16 + 4 * g + i];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) T.bo[o] = w[b_off(5) + o];
 }
 
-// Output layer (64 -> 3, rows padded to 16): y[nt][i] = output 4g + i of sample nt*16 + col (only g = 0,
-// i < 3 are outputs).
-__device__ __forceinline__ void output_layer(const bf16 *act5, const bf16x8 *frag, const float *bias, int lane,
-                                             f32x4 y[4]) {
+// Row tile mt of hidden layer l (KB = in/32): act_out = bf16(relu(W^T . act_in + b)) for the chunk's 64
+// samples.
+template <int KB>
+__device__ __forceinline__ void hidden_tile(const bf16 *act_in, int s_in, bf16 *act_out, const bf16x8 (&a)[KB],
+                                            const float (&bz)[4], int lane, int mt) {
     const int col = lane & 15, g = lane >> 4;
-    const bf16x8 a0 = frag[0 * 64 + lane], a1 = frag[1 * 64 + lane];
+    f32x4 acc[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(act_in + (nt * 16 + col) * s_in + kb * 32 + 8 * g);
+            acc[nt] = mfma(a[kb], b, acc[nt]);
+        }
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        const bf16 *row = act5 + (nt * 16 + col) * kSH + 8 * g;
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = mfma(a0, *reinterpret_cast<const bf16x8 *>(row), acc);
-        acc = mfma(a1, *reinterpret_cast<const bf16x8 *>(row + 32), acc);
+        bf16x4 o;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int o = 4 * g + i;
-            y[nt][i] = acc[i] + (o < 3 ? bias[o] : 0.0f);
+            const float v = acc[nt][i] + bz[i];
+            o[i] = (bf16)(v > 0.0f ? v : 0.0f);
         }
+        *reinterpret_cast<bf16x4 *>(act_out + (nt * 16 + col) * kSH + mt * 16 + 4 * g) = o;
     }
 }
 
-// The forward pass of the chunk's 64 encoded samples by the workgroup's nw waves; hidden activations
-// stay in act[1..5]; wave 0 computes the outputs.
-__device__ __forceinline__ void forward(bf16 *act0, bf16 *const act[6], const bf16x8 *ffrag, const float *w, int lane,
-                                        int wv, int nw, f32x4 y[4]) {
-    hidden_layer<3>(act0, kS0, act[1], ffrag + fwd_base(0) * 64, w + b_off(0), lane, wv, nw);
+// Output layer (64 -> 3, rows padded to 16) for samples 16 nt .. 16 nt + 15: y[i] = output 4g + i of
+// sample 16 nt + col (only g = 0, i < 3 are outputs).
+__device__ __forceinline__ void output_tile(const bf16 *act5, const TrainWeights &T, int lane, int nt, f32x4 &y) {
+    const int col = lane & 15, g = lane >> 4;
+    const bf16 *row = act5 + (nt * 16 + col) * kSH + 8 * g;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = mfma(T.fo[0], *reinterpret_cast<const bf16x8 *>(row), acc);
+    acc = mfma(T.fo[1], *reinterpret_cast<const bf16x8 *>(row + 32), acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = acc[i] + ((g == 0 && i < 3) ? T.bo[i < 3 ? i : 0] : 0.0f);
+}
+
+// The forward pass of the chunk's 64 encoded samples; hidden activations stay in act[1..5]; wave w
+// computes the outputs of samples 16 w .. 16 w + 15.
+__device__ __forceinline__ void forward(bf16 *act0, bf16 *const act[6], const TrainWeights &T, int lane, int wv,
+                                        f32x4 &y) {
+    hidden_tile<3>(act0, kS0, act[1], T.f0, T.bz[0], lane, wv);
     __syncthreads();
-#pragma unroll 1
+#pragma unroll
     for (int l = 1; l < 5; ++l) {
-        hidden_layer<2>(act[l], kSH, act[l + 1], ffrag + fwd_base(l) * 64, w + b_off(l), lane, wv, nw);
+        hidden_tile<2>(act[l], kSH, act[l + 1], T.fh[l - 1], T.bz[l], lane, wv);
         __syncthreads();
     }
-    if (wv == 0) output_layer(act[5], ffrag + fwd_base(5) * 64, w + b_off(5), lane, y);
+    output_tile(act[5], T, lane, wv, y);
 }
 
-// Gradient into the previous layer, in place: act[l] := bf16((W_l . G^T)^T * (act[l] > 0)), where G
-// (the gradient at layer l's output, KB = out/32 blocks) is read from g_in.
+// Row tile mt of the gradient into the previous layer: g_out := bf16((W_l . G^T)^T * (act[l] > 0)), where G
+// (the gradient at layer l's output, KB = out/32 blocks) is read from g_in and act is layer l's input.
 template <int KB>
-__device__ __forceinline__ void backward_layer(const bf16 *g_in, int s_g, bf16 *act, const bf16x8 *frag, int lane, int w,
-                                               int nw) {
+__device__ __forceinline__ void backward_tile(const bf16 *g_in, int s_g, const bf16 *act, bf16 *g_out,
+                                              const bf16x8 (&a)[KB], int lane, int mt) {
     const int col = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int mt = w; mt < 4; mt += nw) {
-        bf16x8 a[KB];
+    f32x4 acc[4];
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) a[kb] = frag[(mt * KB + kb) * 64 + lane];
-        f32x4 acc[4];
+    for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(g_in + (nt * 16 + col) * s_g + kb * 32 + 8 * g);
-                acc[nt] = mfma(a[kb], b, acc[nt]);
-            }
+    for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
-            bf16x4 *p = reinterpret_cast<bf16x4 *>(act + (nt * 16 + col) * kSH + mt * 16 + 4 * g);
-            const bf16x4 m = *p;
-            bf16x4 o;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) o[i] = (float)m[i] > 0.0f ? (bf16)acc[nt][i] : (bf16)0.0f;
-            *p = o;
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(g_in + (nt * 16 + col) * s_g + kb * 32 + 8 * g);
+            acc[nt] = mfma(a[kb], b, acc[nt]);
         }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int off = (nt * 16 + col) * kSH + mt * 16 + 4 * g;
+        const bf16x4 m = *reinterpret_cast<const bf16x4 *>(act + off);
+        bf16x4 o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = (float)m[i] > 0.0f ? (bf16)acc[nt][i] : (bf16)0.0f;
+        *reinterpret_cast<bf16x4 *>(g_out + off) = o;
     }
 }
 
-// Weight gradient of layer l over the chunk's 64 samples: dW[in][out] += sum_s act[s][in] G[s][out] (MFMA
-// with the samples as K; NMT = in/16 row tiles x NNT = out column tiles, tile t handled by wave t % nw),
-// bias gradient db[out] += sum_s G[s][out] by wave bw; written to (or added into) the workgroup's
-// partial.
-template <int NMT, int NNT>
-__device__ __forceinline__ void weight_grad(const bf16 *act, int s_act, const bf16 *gr, int s_g, int nout, float *pw,
-                                            float *pb, bool accumulate, int lane, int w, int nw, int bw) {
+// Weight-gradient tile (mt, nt) over the chunk's 64 samples, added to acc: dW[in][out] += sum_s act[s][in]
+// G[s][out] (MFMA with the samples as K).
+__device__ __forceinline__ void wgrad_tile(const bf16 *act, int s_act, const bf16 *gr, int s_g, int mt, int nt,
+                                           int lane, f32x4 &acc) {
     const int col = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int t = w; t < NMT * NNT; t += nw) {
-        const int mt = t / NNT, nt = t % NNT;
-        bf16x8 a[2], b[2];
+    bf16x8 a[2], b[2];
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
+    for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                a[kb][i] = act[(kb * 32 + 8 * g + i) * s_act + mt * 16 + col];
-                b[kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
-            }
-        const int o = nt * 16 + col;
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (accumulate && o < nout) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] = pw[(mt * 16 + 4 * g + i) * nout + o];
+        for (int i = 0; i < 8; ++i) {
+            a[kb][i] = act[(kb * 32 + 8 * g + i) * s_act + mt * 16 + col];
+            b[kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
         }
-        acc = mfma(a[0], b[0], acc);
-        acc = mfma(a[1], b[1], acc);
-        if (o < nout) {
+    const f32x4 t = mfma(a[0], b[0], acc);
+    acc = mfma(a[1], b[1], t);
+}
+
+// Bias gradient of column tile nt over the chunk, sum_s G[s][16 nt + col], added to db (set by the first
+// chunk): an MFMA product with an all-ones A operand, whose every row is the column sum; lane (g, col)
+// gets output 16 nt + col.  (Its B operand is the one the weight-gradient tiles of column tile nt read.)
+__device__ __forceinline__ void bias_tile(const bf16 *gr, int s_g, int nt, int lane, bool first, float &db) {
+    const int col = lane & 15, g = lane >> 4;
+    bf16x8 one, b[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) pw[(mt * 16 + 4 * g + i) * nout + o] = acc[i];
-        }
-    }
-    if (w == bw && lane < nout) {
-        float s = 0.0f;
-        for (int r = 0; r < 64; ++r) s += (float)gr[r * s_g + lane];
-        pb[lane] = accumulate ? pb[lane] + s : s;
-    }
+    for (int i = 0; i < 8; ++i) one[i] = (bf16)1.0f;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) b[kb][i] = gr[(kb * 32 + 8 * g + i) * s_g + nt * 16 + col];
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = mfma(one, b[0], acc);
+    acc = mfma(one, b[1], acc);
+    db = first ? acc[0] : db + acc[0];
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -294,89 +338,144 @@ struct TrainArgs {
     int32_t *valid_part;     // [gridDim.x]
 };
 
-// Forward + backward of the relative L2 loss over 64-sample chunks: the workgroup's four waves share
-// each chunk (every layer's tiles are split between them) and accumulate the weight gradients of their
-// chunks, strided over the grid, into the workgroup's partial.
-constexpr int kTrainWaves = 4;
+// The training record of sample s of chunk ch (zeros past the end).
+__device__ __forceinline__ void load_record(const TrainArgs &A, int ch, int s, float4 &r0, float4 &r1, float4 &r2,
+                                            float4 &r3) {
+    const long long sidx = (long long)ch * 64 + s;
+    r0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    r1 = r0; r2 = r0; r3 = r0;
+    if (ch < A.n_chunks && sidx < A.n) {
+        const float4 *rec = A.samples + sidx * 4;
+        r0 = rec[0]; r1 = rec[1]; r2 = rec[2]; r3 = rec[3];
+    }
+}
+
+// Forward + backward of the relative L2 loss over 64-sample chunks, strided over the grid.  The weight
+// gradients of the workgroup's chunks are summed, chunk by chunk, in registers (each wave its own tiles,
+// bias sums by one wave per layer) and written to the workgroup's partial once, at the end.
 __global__ void __launch_bounds__(256) k_mlp_train(TrainArgs A) {
     __shared__ __attribute__((aligned(16))) bf16 lds[kTrainLds];
-    __shared__ float4 tgt[64];   // per sample: target RGB, valid
+    __shared__ float4 tgt[64];    // per sample: target RGB, valid
+    __shared__ float2 lterm[64];  // per sample: its loss term, valid
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    constexpr int nw = kTrainWaves;
+    static_assert(kTrainWaves == 4, "one row tile of each 64-wide layer per wave");
+    // encoding: four lanes per sample, each three of its twelve feature groups.  The first chunk's records
+    // are requested before the weights, so the encoding waits for them only; each later chunk's while the
+    // one before it is trained.
+    const int es = wv * 16 + (lane >> 2), pq = lane & 3;
+    float4 r0, r1, r2, r3;
+    load_record(A, blockIdx.x, es, r0, r1, r2, r3);
+    TrainWeights T;
+    load_train_weights(A.ffrag, A.bfrag, A.w, lane, wv, T);
     bf16 *act0 = lds;
     bf16 *act[6];
     act[0] = act0;
 #pragma unroll
     for (int l = 1; l < 6; ++l) act[l] = lds + 64 * kS0 + (l - 1) * 64 * kSH;
     bf16 *g6 = lds + 64 * kS0 + 5 * 64 * kSH;
+    bf16 *const gb[2] = {g6 + 64 * kSG, g6 + 64 * kSG + 64 * kSH};   // hidden gradients, alternating
     for (int j = tid; j < 64 * kSG; j += 256) g6[j] = (bf16)0.0f;
-    float *pw = A.part + (size_t)blockIdx.x * kPP;
+    // gradient accumulators: layer 0 tiles (k, wv) k < 6, layers 1..4 tiles (k, wv) k < 4, layer 5 tile
+    // (wv, 0); bias sums (layer l's by wave l & 3; layer 5's by wave 3)
+    f32x4 g0[6], gh[4][4], g5;
+    float db[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g0[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gh[l][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    g5 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 6; ++l) db[l] = 0.0f;
     float loss = 0.0f, nvalid = 0.0f;
     bool first = true;
     for (int ch = blockIdx.x; ch < A.n_chunks; ch += gridDim.x) {
-        // encoding: four lanes per sample, each three of its twelve feature groups
         {
-            const int s = wv * 16 + (lane >> 2), pq = lane & 3;
-            const long long sidx = (long long)ch * 64 + s;
-            const bool present = sidx < A.n;
-            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
-            if (present) {
-                const float4 *rec = A.samples + sidx * 4;
-                r0 = rec[0]; r1 = rec[1]; r2 = rec[2]; r3 = rec[3];
-            }
+            const bool present = (long long)ch * 64 + es < A.n;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 float f[8];
                 encode_group(r0, r1, r2, A.shift, pq + 4 * k, f);
-                *reinterpret_cast<bf16x8 *>(act0 + s * kS0 + 8 * (pq + 4 * k)) = to_bf16x8(f);
+                *reinterpret_cast<bf16x8 *>(act0 + es * kS0 + 8 * (pq + 4 * k)) = to_bf16x8(f);
             }
-            if (pq == 0) tgt[s] = make_float4(r3.x, r3.y, r3.z, (present && __float_as_uint(r0.w) != 0u) ? 1.0f : 0.0f);
+            if (pq == 0) tgt[es] = make_float4(r3.x, r3.y, r3.z, (present && __float_as_uint(r0.w) != 0u) ? 1.0f : 0.0f);
+        }
+        // the next chunk's records, in flight while this one is trained
+        load_record(A, ch + gridDim.x, es, r0, r1, r2, r3);
+        __syncthreads();
+        f32x4 y;
+        forward(act0, act, T, lane, wv, y);
+        // loss gradient: lanes 0..15 of wave w hold outputs 0..2 of samples 16 w + lane
+        if (lane < 16) {
+            const int s = wv * 16 + lane;
+            const float4 t = tgt[s];
+            const float y0 = y[0], y1 = y[1], y2 = y[2];
+            const float mean = ((y0 + y1) + y2) / 3.0f;
+            const float den = mean * mean + A.loss_eps;
+            const float d0 = y0 - t.x, d1 = y1 - t.y, d2 = y2 - t.z;
+            bf16x4 o = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+            lterm[s] = make_float2(t.w != 0.0f ? ((d0 * d0 + d1 * d1) + d2 * d2) / den / 3.0f : 0.0f, t.w);
+            if (t.w != 0.0f) {
+                const float k = (2.0f / 3.0f) * A.inv_n / den;
+                o[0] = (bf16)(d0 * k);
+                o[1] = (bf16)(d1 * k);
+                o[2] = (bf16)(d2 * k);
+            }
+            *reinterpret_cast<bf16x4 *>(g6 + s * kSG) = o;
         }
         __syncthreads();
-        f32x4 y[4];
-        forward(act0, act, A.ffrag, A.w, lane, wv, nw, y);
-        // loss gradient (wave 0): lanes 0..15 hold outputs 0..2 of samples nt*16 + lane
+        // the chunk's loss terms into wave 0's lane sums, samples lane, 16 + lane, 32 + lane, 48 + lane in turn
         if (wv == 0 && lane < 16) {
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                const int s = nt * 16 + lane;
-                const float4 t = tgt[s];
-                const float y0 = y[nt][0], y1 = y[nt][1], y2 = y[nt][2];
-                const float mean = ((y0 + y1) + y2) / 3.0f;
-                const float den = mean * mean + A.loss_eps;
-                const float d0 = y0 - t.x, d1 = y1 - t.y, d2 = y2 - t.z;
-                bf16x4 o = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
-                if (t.w != 0.0f) {
+                const float2 lt = lterm[nt * 16 + lane];
+                if (lt.y != 0.0f) {
                     nvalid += 1.0f;
-                    loss += ((d0 * d0 + d1 * d1) + d2 * d2) / den / 3.0f;
-                    const float k = (2.0f / 3.0f) * A.inv_n / den;
-                    o[0] = (bf16)(d0 * k);
-                    o[1] = (bf16)(d1 * k);
-                    o[2] = (bf16)(d2 * k);
+                    loss += lt.x;
                 }
-                *reinterpret_cast<bf16x4 *>(g6 + s * kSG) = o;
             }
         }
+        // Layer 5 (64 -> 3), then 4..1, then 0: one phase per layer computes its weight and bias gradients
+        // and the gradient into the layer below, which goes to the other of the two gradient tiles.
+        wgrad_tile(act[5], kSH, g6, kSG, wv, 0, lane, g5);
+        if (wv == 0) bias_tile(g6, kSG, 0, lane, first, db[5]);
+        backward_tile<1>(g6, kSG, act[5], gb[0], T.b5, lane, wv);
         __syncthreads();
-        // layer 5 (64 -> 3), then 4..1, then 0
-        weight_grad<4, 1>(act[5], kSH, g6, kSG, 3, pw + w_off(5), pw + b_off(5), !first, lane, wv, nw, 3);
-        __syncthreads();
-        backward_layer<1>(g6, kSG, act[5], A.bfrag + bwd_base(5) * 64, lane, wv, nw);
-        __syncthreads();
-#pragma unroll 1
+#pragma unroll
         for (int l = 4; l >= 1; --l) {
-            weight_grad<4, 4>(act[l], kSH, act[l + 1], kSH, 64, pw + w_off(l), pw + b_off(l), !first, lane, wv, nw,
-                              l & 3);
-            __syncthreads();
-            backward_layer<2>(act[l + 1], kSH, act[l], A.bfrag + bwd_base(l) * 64, lane, wv, nw);
+            const bf16 *gl = gb[(4 - l) & 1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wgrad_tile(act[l], kSH, gl, kSH, k, wv, lane, gh[l - 1][k]);
+            bias_tile(gl, kSH, wv, lane, first, db[l]);
+            backward_tile<2>(gl, kSH, act[l], gb[(5 - l) & 1], T.bh[l - 1], lane, wv);
             __syncthreads();
         }
-        weight_grad<6, 4>(act0, kS0, act[1], kSH, 64, pw + w_off(0), pw + b_off(0), !first, lane, wv, nw, 0);
+        // gb[0]: the gradient at layer 0's output
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wgrad_tile(act0, kS0, gb[0], kSH, k, wv, lane, g0[k]);
+        bias_tile(gb[0], kSH, wv, lane, first, db[0]);
         __syncthreads();
         first = false;
     }
-    if (first) {   // no chunk for this workgroup: its partial is zero
-        for (int p = tid; p < kPP; p += 256) pw[p] = 0.0f;
+    // The workgroup's partial (zero for a workgroup without a chunk): each gradient tile as the wave holds
+    // it, one 16-byte store per lane (the partial's tile order, part_index).
+    {
+        float *pw = A.part + (size_t)blockIdx.x * kPP;
+        const int col = lane & 15, g = lane >> 4;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) *reinterpret_cast<f32x4 *>(pw + w_off(0) + ((k * 4 + wv) * 64 + lane) * 4) = g0[k];
+#pragma unroll
+        for (int l = 1; l < 5; ++l)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                *reinterpret_cast<f32x4 *>(pw + w_off(l) + ((k * 4 + wv) * 64 + lane) * 4) = gh[l - 1][k];
+        if (col < 3) *reinterpret_cast<f32x4 *>(pw + w_off(5) + ((wv * 4 + g) * 3 + col) * 4) = g5;
+        if (g == 0) {
+#pragma unroll
+            for (int l = 0; l < 5; ++l) pw[b_off(l) + wv * 16 + col] = db[l];
+            if (wv == 0 && col < 3) pw[b_off(5) + col] = db[5];
+        }
     }
     if (wv == 0) {
         loss = wave_sum(loss);
@@ -394,10 +493,18 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int32_t n
     const int p4 = blockIdx.x * 256 + threadIdx.x;
     if (p4 * 4 >= kPP) return;
     const int r0 = (int)((long long)nparts * blockIdx.y / kSlices), r1 = (int)((long long)nparts * (blockIdx.y + 1) / kSlices);
+    const float4 *src = reinterpret_cast<const float4 *>(part) + p4;
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int r = r0; r < r1; ++r) {
-        const float4 v = reinterpret_cast<const float4 *>(part + (size_t)r * kPP)[p4];
-        s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    // rows in order, eight loads in flight at a time
+    for (int r = r0; r < r1; r += 8) {
+        float4 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = r + j < r1 ? src[(size_t)(r + j) * (kPP / 4)] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (r + j < r1) {
+                s.x += v[j].x; s.y += v[j].y; s.z += v[j].z; s.w += v[j].w;
+            }
     }
     reinterpret_cast<float4 *>(part2 + (size_t)blockIdx.y * kPP)[p4] = s;
 }
@@ -457,7 +564,12 @@ __global__ void __launch_bounds__(256) k_mlp_adam(AdamArgs A) {
         g = A.grad_in[p];
     } else {
         g = 0.0f;
-        for (int q = 0; q < kSlices; ++q) g += A.part[(size_t)q * kPP + p];
+        const int pi = part_index(p);
+        float v[kSlices];
+#pragma unroll
+        for (int q = 0; q < kSlices; ++q) v[q] = A.part[(size_t)q * kPP + pi];
+#pragma unroll
+        for (int q = 0; q < kSlices; ++q) g += v[q];
     }
     if (A.mode == 1) {
         A.grad_out[p] = g;
@@ -1187,7 +1299,7 @@ int vf_gicache_create(vf_ctx *ctx, const vf_gicache_params *p, vf_gicache **out)
     if (!c) return vf::fail(ctx, VF_E_OOM, "vf_gicache_create: out of host memory");
     c->ctx = ctx;
     c->p = *p;
-    c->part_rows = 2 * ctx->n_cu;
+    c->part_rows = ctx->n_cu;   // one training workgroup per compute unit (its registers hold the weights)
     hipError_t e = hipSuccess;
 #define VF_GC_ALLOC(ptr, bytes)                                              \
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes))
