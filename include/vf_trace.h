@@ -460,7 +460,11 @@ void vf_graph_destroy(vf_graph *graph);
 int vf_set_counting(vf_ctx *ctx, int32_t on);
 /* counters: {rays, brick steps, voxel steps, fetch bytes}; reset = 1 zeroes them after reading. */
 int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset);
-/* Kernel time of the last trace entry point on the context's stream, from HIP events (ms). */
+/* When on, every trace entry point records a HIP event before and after its launches, for
+ * vf_last_kernel_ms.  Off by default: the two event records add GPU time to each call. */
+int vf_set_timing(vf_ctx *ctx, int32_t on);
+/* Kernel time of the last trace entry point on the context's stream, from HIP events (ms); needs
+ * vf_set_timing(ctx, 1) before that call. */
 int vf_last_kernel_ms(vf_ctx *ctx, float *ms);
 
 #ifdef __cplusplus

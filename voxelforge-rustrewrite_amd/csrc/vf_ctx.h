@@ -58,6 +58,7 @@ struct vf_ctx {
     unsigned long long *d_cnt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    bool timing = false;   // vf_set_timing: record ev0 / ev1 around each trace entry point
     bool capturing = false;   // inside vf_graph_begin / vf_graph_end: no syncs, no timing events
     int n_cu = 256;           // compute units (persistent grids)
     long long wall_khz = 100000;   // wall clock rate (batch completion stamps)

@@ -151,9 +151,11 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     const int32_t s0 = st[0], s1 = st[1], s2 = st[2];
     // One exit per trip: the next step is computed first and committed only when the ray neither hits
     // the current voxel nor leaves the brick with it (a step out of the brick is not used: the caller's
-    // brick DDA continues from its own state).
+    // brick DDA continues from its own state).  The reference caps the walk at 26 trips; it cannot reach
+    // the cap: every step moves one coordinate by the ray's fixed sign, so each axis leaves [0, 7] after at
+    // most 8 steps and the walk exits by trip 22.  The loop therefore has the single exit hit | out.
 #pragma unroll 1
-    for (int j = 0; j < 26; ++j) {
+    for (;;) {
         if (COUNT) { c.voxel++; c.bytes += 1; }
         const uint32_t idx = ((uint32_t)ly << 3) | (uint32_t)lx;   // bit of (lx, ly) in the 64-bit layer
         const uint64_t layer = ((uint64_t)whi << 32) | (uint64_t)wlo;
@@ -163,6 +165,13 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         const bool a1 = !a0 && lt_yz;
         const bool a2 = !a0 && !lt_yz;
         const int32_t nlx = lx + (a0 ? s0 : 0), nly = ly + (a1 ? s1 : 0), nlz = lz + (a2 ? s2 : 0);
+        // the next voxel's layer, read as soon as its z is known (the slot index wraps to stay in the
+        // lane's 8 layers when the step leaves the brick; that value is never used, and the layer is
+        // not needed after the walk, so it is replaced before the exit test)
+        if (LDSM) {
+            wlo = sm[((nlz & 7) * 2) * LS];
+            whi = sm[((nlz & 7) * 2 + 1) * LS];
+        }
         const bool out = ((uint32_t)nlx | (uint32_t)nly | (uint32_t)nlz) > 7u;
         if (hit | out) break;
         tv = a0 ? tMx : (a1 ? tMy : tMz);
@@ -172,12 +181,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         tMy = a1 ? ny : tMy;
         tMz = a2 ? nz : tMz;
         lx = nlx; ly = nly; lz = nlz;
-        if (LDSM) {   // the layer of the new voxel, read every trip (no branch; unchanged unless a z step)
-            wlo = sm[(lz * 2) * LS];
-            whi = sm[(lz * 2 + 1) * LS];
-        } else if (a2) {
-            mask_layer(m0, m1, m2, m3, lz, wlo, whi);
-        }
+        if (!LDSM && a2) mask_layer(m0, m1, m2, m3, lz, wlo, whi);
     }
     if (hit) {
         h.hit = 1;

@@ -1242,13 +1242,13 @@ int launch_check(vf_ctx *ctx, const char *what) {
 }
 
 int begin_timed(vf_ctx *ctx) {
-    if (ctx->capturing) return VF_OK;
+    if (ctx->capturing || !ctx->timing) return VF_OK;
     VF_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     return VF_OK;
 }
 
 int end_timed(vf_ctx *ctx) {
-    if (ctx->capturing) return VF_OK;
+    if (ctx->capturing || !ctx->timing) return VF_OK;
     VF_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     return VF_OK;
@@ -2129,9 +2129,17 @@ int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset) {
     return VF_OK;
 }
 
+int vf_set_timing(vf_ctx *ctx, int32_t on) {
+    if (!ctx) return vf::fail(ctx, VF_E_INVALID, "vf_set_timing: null context");
+    ctx->timing = on != 0;
+    if (!ctx->timing) ctx->timed = false;
+    return VF_OK;
+}
+
 int vf_last_kernel_ms(vf_ctx *ctx, float *ms) {
     if (!ctx || !ms) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: null argument");
     if (int rc = no_capture(ctx, "vf_last_kernel_ms")) return rc;
+    if (!ctx->timing) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: timing is off (vf_set_timing)");
     if (!ctx->timed) return vf::fail(ctx, VF_E_INVALID, "vf_last_kernel_ms: nothing timed yet");
     int rc = set_device(ctx);
     if (rc) return rc;
