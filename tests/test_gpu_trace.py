@@ -18,9 +18,12 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
 
-@pytest.fixture(scope="module")
-def ctx():
+@pytest.fixture(scope="module", params=[1, 2], ids=["plain", "small_frame"])
+def ctx(request):
+    """Every test runs with the plain traversal and with the small-frame one (vf_set_traversal 1 / 2), which
+    skips the walks its 2^3-block test rules out: both must give the oracle's hits."""
     c = vf.Context(0)
+    c.set_traversal(request.param)
     yield c
     c.close()
 
@@ -89,6 +92,30 @@ def test_shadow_config1(ctx):
     assert np.array_equal(vis, osh[:, 0].astype(np.float16))
     assert np.array_equal(tb, osh[:, 1].astype(np.float16))
     assert 0.0 < (vis == 0).mean() < 1.0
+
+
+@pytest.mark.parametrize("size", [(640, 360), (320, 180), (1280, 720)])
+def test_fused_frame_config1_matches_oracle(ctx, size):
+    """The fused primary + sun pass on the fixed world (config 1's frame and the sizes around it): hits,
+    G-buffer, shadowRaw and sun-ray hits equal the oracle's two passes."""
+    w, a, ow = world(1)
+    scene = upload(ctx, w, a)
+    pu, cam = config1_camera()
+    W, H = size
+    hits, gbp, gbn = dev_hits(W * H), torch.zeros(W * H * 4, dtype=torch.float32, device=DEV), \
+        torch.zeros(W * H, dtype=torch.int32, device=DEV)
+    sh, shh = torch.zeros(W * H, dtype=torch.int32, device=DEV), dev_hits(W * H)
+    ctx.trace_primary_shadow(pu, W, H, hits, gbp, gbn, sh, shh)
+    ctx.sync()
+    oh, opos, onrm, _ = ow.pass_primary(cam, W, H)
+    osh, ohits, _ = ow.pass_shadow(np.array(scene.sun_dir, np.float32), W, H, opos, onrm)
+    assert hit_diff(vf.hits_to_numpy(hits), oh).size == 0
+    assert np.array_equal(gbp.cpu().numpy().reshape(-1, 4).view(np.uint32), opos.view(np.uint32))
+    assert np.array_equal(gbn.cpu().numpy().view(np.uint32), onrm)
+    assert hit_diff(vf.hits_to_numpy(shh), ohits).size == 0
+    gsh = sh.cpu().numpy().view(np.uint32)
+    assert np.array_equal((gsh & 0xFFFF).astype(np.uint16).view(np.float16), osh[:, 0].astype(np.float16))
+    assert np.array_equal((gsh >> 16).astype(np.uint16).view(np.float16), osh[:, 1].astype(np.float16))
 
 
 @pytest.mark.parametrize("half_res,frame", [(True, 0), (True, 5), (False, 2)])

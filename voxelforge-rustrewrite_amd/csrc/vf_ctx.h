@@ -61,6 +61,7 @@ struct vf_ctx {
     bool timing = false;   // vf_set_timing: record ev0 / ev1 around each trace entry point
     bool capturing = false;   // inside vf_graph_begin / vf_graph_end: no syncs, no timing events
     int n_cu = 256;           // compute units (persistent grids)
+    int32_t traversal = 0;    // vf_set_traversal: 0 by launch size, 1 the plain traversal, 2 the small-frame one
     long long wall_khz = 100000;   // wall clock rate (batch completion stamps)
     // persistent-queue buffers (vf_queue.hip), grown on first use outside a capture
     unsigned long long *q_ctr = nullptr, *q_stamps = nullptr;
@@ -102,6 +103,13 @@ int no_capture(vf_ctx *ctx, const char *what);
 void free_queue(vf_ctx *ctx);
 int queue_scratch_init(vf_ctx *ctx);   // vf_queue.hip: the device address of the batch scratch (device current)
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+// The small-frame traversal for a launch of `waves` wavefronts: by the context's setting, or, by default,
+// for launches of at most 16 waves per SIMD (4 SIMDs per compute unit), whose time is that of their
+// slowest waves rather than the chip's issue rate.
+inline bool small_frame(const vf_ctx *ctx, int64_t waves) {
+    if (ctx->counting || ctx->traversal == 1) return false;
+    return ctx->traversal == 2 || waves <= (int64_t)ctx->n_cu * 4 * 16;
+}
 // bodies (vf_bodies.hip): the view the kernels get (n = 0 unless the scene's num_objects is set), and the
 // launch-time check that the scene's num_objects matches the uploaded bodies and their bricks exist
 DevBodies dev_bodies(const vf_ctx *ctx);

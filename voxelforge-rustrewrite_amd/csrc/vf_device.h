@@ -105,16 +105,20 @@ static __shared__ uint32_t vf_vmask64[16 * 64];
 // LDSM: the mask goes to the lane's LDS slots and a step along z reads the new 64-bit layer from
 // there (one LDS read instead of a 16-to-2 word select in registers, which a wave executes on almost
 // every step since some lane steps along z).
-template <bool COUNT, bool LDSM = false, int LS = 256>
+// BOX (the small-frame traversal, LDSM only): before the walk, the brick's mask is tested over the box of
+// voxels the walk can visit; tout = the brick DDA's time of leaving the brick (the least of its tMax).  A
+// walk that cannot hit is skipped: its result is the same miss, after which the brick DDA goes on from its
+// own state as it does after any walk.
+template <bool COUNT, bool LDSM = false, int LS = 256, bool BOX = false>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
                                              const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c,
-                                             const int skip) {
+                                             const int skip, const float tout = 0.0f) {
     const uint32_t slot = g - 256u;
     const uint4 *mp = (skip == 5 ? W.occdry : W.occ) + (size_t)slot * 4u;
     const uint4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
     uint32_t *sm = (LS == 64 ? vf_vmask64 : vf_vmask) + threadIdx.x;
-    if (LDSM) {
+    if (LDSM && !BOX) {
         sm[0 * LS] = m0.x; sm[1 * LS] = m0.y; sm[2 * LS] = m0.z; sm[3 * LS] = m0.w;
         sm[4 * LS] = m1.x; sm[5 * LS] = m1.y; sm[6 * LS] = m1.z; sm[7 * LS] = m1.w;
         sm[8 * LS] = m2.x; sm[9 * LS] = m2.y; sm[10 * LS] = m2.z; sm[11 * LS] = m2.w;
@@ -132,6 +136,36 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         tD[a] = fabsf(inv[a]);
     }
     int32_t lx = vc[0] - bmin[0], ly = vc[1] - bmin[1], lz = vc[2] - bmin[2];
+    if (BOX) {
+        // Every step of the walk moves one coordinate by the ray's fixed sign, from the entry voxel to the
+        // voxel where the ray leaves the brick: per axis between lv and the coordinate at tout.  That
+        // coordinate is taken 0.05 voxel further along the ray (the walk's and the brick DDA's tMax
+        // differ by rounding only, ~1e-6 of t) and clamped into the brick.  No solid voxel in that box
+        // (rows and columns of each 64-bit layer, layers blo[2]..bhi[2]): the walk cannot hit.
+        int32_t blo[3], bhi[3];
+        const int32_t lv[3] = {lx, ly, lz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float q = (ro[a] + sd[a] * tout) - (float)bmin[a];
+            const int32_t ev = iclamp(f2i(floorf(st[a] > 0 ? q + 0.05f : q - 0.05f)), 0, 7);
+            blo[a] = imin(lv[a], ev);
+            bhi[a] = imax(lv[a], ev);
+        }
+        const uint32_t rep = ((2u << bhi[0]) - (1u << blo[0])) * 0x01010101u;   // columns, in every row
+        const uint64_t top = bhi[1] == 7 ? ~0ull : ((1ull << (8 * (bhi[1] + 1))) - 1ull);
+        const uint64_t rows = top & ~((1ull << (8 * blo[1])) - 1ull);
+        const uint32_t rlo = rep & (uint32_t)rows, rhi = rep & (uint32_t)(rows >> 32);
+        const uint32_t w[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                                m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+        uint32_t any = 0u;
+#pragma unroll
+        for (int z = 0; z < 8; ++z) any |= (z >= blo[2] && z <= bhi[2]) ? ((w[2 * z] & rlo) | (w[2 * z + 1] & rhi)) : 0u;
+        if (any == 0u) return false;
+        sm[0 * LS] = m0.x; sm[1 * LS] = m0.y; sm[2 * LS] = m0.z; sm[3 * LS] = m0.w;
+        sm[4 * LS] = m1.x; sm[5 * LS] = m1.y; sm[6 * LS] = m1.z; sm[7 * LS] = m1.w;
+        sm[8 * LS] = m2.x; sm[9 * LS] = m2.y; sm[10 * LS] = m2.z; sm[11 * LS] = m2.w;
+        sm[12 * LS] = m3.x; sm[13 * LS] = m3.y; sm[14 * LS] = m3.z; sm[15 * LS] = m3.w;
+    }
     uint32_t wlo, whi;
     if (LDSM) {
         wlo = sm[(lz * 2) * LS];
@@ -198,7 +232,11 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
 // take it from skip_rt (one inlined copy serving both).
 // PRE: pinv holds 1 / rd (each component clamped away from 0 as below), computed once by the caller for a
 // direction shared by many rays (the sun's): the same IEEE quotients, without three divisions per ray.
-template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256, bool PRE = false>
+// BOX: the small-frame traversal (LDSM only, never counting): each detailed brick's walk is first tested
+// against its mask over the voxels it can visit, and a wave raises its issue priority after 4, 10 and 16
+// brick steps, so the few long rays that set a small frame's time are not held back by the frame's other
+// waves (the priority stays raised for the wave's later rays).
+template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256, bool PRE = false, bool BOX = false>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
                                             Cnt &c, const int skip_rt = 0, const float *pinv = nullptr) {
     const int sk = SKIP >= 0 ? SKIP : skip_rt;
@@ -243,8 +281,12 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
         else if (tEnter == tmin3[1]) face = faceA[1];
         else face = faceA[2];
     }
+    static_assert(!BOX || (LDSM && !COUNT), "the small-frame traversal is an LDS-mask, non-counting path");
     for (int i = 0; i < W.max_outer; ++i) {
         if (t > tcap) return;
+        if (BOX && i == 4) __builtin_amdgcn_s_setprio(1);
+        if (BOX && i == 10) __builtin_amdgcn_s_setprio(2);
+        if (BOX && i == 16) __builtin_amdgcn_s_setprio(3);
         const uint2 cw = W.cells[cell_index(W, bc[0], bc[1], bc[2])];
         uint32_t g = cw.x;
         const uint32_t meta = cw.y;
@@ -257,7 +299,9 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             }
         }
         if (g >= 256u) {
-            if (brick_voxels<COUNT, LDSM, LS>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk)) return;
+            if (brick_voxels<COUNT, LDSM, LS, BOX>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk,
+                                                   BOX ? gmin(gmin(tM8[0], tM8[1]), tM8[2]) : 0.0f))
+                return;
         } else if (g != 0u && g != (uint32_t)sk) {
             const float e = t + 1e-3f;
             h.hit = 1; h.t = t; h.face = face; h.mat = g;

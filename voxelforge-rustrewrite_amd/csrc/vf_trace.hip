@@ -67,11 +67,11 @@ __device__ __forceinline__ void primary_dir(const PrimArgs &A, int32_t px, int32
     normalize3(rd);
 }
 
-template <bool COUNT, bool SPECIAL, int LS = 256>
+template <bool COUNT, bool SPECIAL, int LS = 256, bool BOX = false>
 __device__ __forceinline__ void primary_hit(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, const float rd[3],
                                             Hit &h, Cnt &c) {
     if (SPECIAL) primary_trace<COUNT, true>(W, A.pos, cam_mat, rd, h, c);
-    else trace_world<0, COUNT, true, LS>(W, A.pos, rd, 1e30f, h, c);
+    else trace_world<0, COUNT, true, LS, false, BOX>(W, A.pos, rd, 1e30f, h, c);
 }
 
 // Tile L's pixel for this thread (8x8 per wave, 16x16 per workgroup).
@@ -150,7 +150,8 @@ __device__ __forceinline__ void store_miss(vf_hit *out) {
 // leaf, no blocker distance); else sunVisibilityT (FOLIAGE_LIGHT, deterministic) from P + N*0.03:
 // visibility (leaves let leaf of the light through) and the distance to the first blocker.
 // NDEC: N holds the decoded normal already (only nq's alpha is read, which is then 204 or 255).
-template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false, bool NDEC = false>
+template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false, bool NDEC = false,
+          bool BOX = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr,
                                                  const float *pinv = nullptr, const float *Ndec = nullptr) {
@@ -177,7 +178,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         }
         if (trace) {
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE>(W, so, sun, leaf, seed, tb, hits, i, c, B, pinv);
+            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE, BOX>(W, so, sun, leaf, seed, tb, hits, i, c, B, pinv);
             if (!lit) tb = 0.0f;
         } else if (hits) {
             store_miss(hits + i);
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
 // gbuffer.comp then shadow_trace.comp for the same pixel in one kernel: the sun ray starts from the
 // G-buffer values this lane has just written (position, RGBA8-quantised normal), exactly what the
 // separate shadow pass would read back, so the results are those of the two passes.
-template <bool COUNT, bool SPECIAL, int LS = 256, bool PRE = false>
+template <bool COUNT, bool SPECIAL, int LS = 256, bool PRE = false, bool BOX = false>
 __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px,
                                                      int32_t py, const float sun[3], vf_hit *hits, float4 *gbpos,
                                                      uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c,
@@ -218,7 +219,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
-    primary_hit<COUNT, SPECIAL, LS>(W, A, cam_mat, rd, h, c);
+    primary_hit<COUNT, SPECIAL, LS, BOX>(W, A, cam_mat, rd, h, c);
     if (camera_special(cam_mat) != SPECIAL) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     store_hit(hits + i, h);
@@ -242,7 +243,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE, true>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv, Nd);
+    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE, true, BOX>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv, Nd);
 }
 
 template <bool COUNT, bool SPECIAL>
@@ -271,8 +272,9 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
 
 // The fused kernel for a camera in air with one 8x8-pixel wave per workgroup: a wave that finishes
 // frees its slot at once instead of waiting for the other three of a 16x16 tile.  Block L is quadrant
-// L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.
-template <bool COUNT>
+// L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.  BOX: the
+// small-frame traversal (trace_world).
+template <bool COUNT, bool BOX = false>
 __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
                                                          float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
                                                          unsigned long long *cnt, float ix, float iy, float iz) {
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs
     const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
     const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
     if (px < A.w && py < A.h)
-        primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
+        primary_shadow_pixel<COUNT, false, 64, true, BOX>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -431,10 +433,12 @@ __global__ void __launch_bounds__(256) k_sun_cone(DevWorld W, DevBodies B, TileA
     flush_count<COUNT>(c, cnt);
 }
 
-// Explicit ray batch: traceVolumeS per ray (1-D, XCD-remapped 256-ray blocks).
-template <int SKIP, bool COUNT>
+// Explicit ray batch: traceVolumeS per ray (1-D, XCD-remapped 256-ray blocks); BOXT: the small-frame
+// traversal (not in the counting variant).
+template <int SKIP, bool COUNT, bool BOXT = false>
 __global__ void __launch_bounds__(256) k_rays(DevWorld W, const vf_ray *rays, long long n, float tcap, vf_hit *hits,
                                               unsigned long long *cnt) {
+    constexpr bool BOX = BOXT && !COUNT;
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const long long i = (long long)L * 256 + threadIdx.x;
     if (i >= n) return;
@@ -442,7 +446,7 @@ __global__ void __launch_bounds__(256) k_rays(DevWorld W, const vf_ray *rays, lo
     const float ro[3] = {r[0], r[1], r[2]}, rd[3] = {r[3], r[4], r[5]};
     Cnt c = {0, 0, 0, 0};
     Hit h;
-    trace_world<SKIP, COUNT, true>(W, ro, rd, tcap, h, c);
+    trace_world<SKIP, COUNT, true, 256, false, BOX>(W, ro, rd, tcap, h, c);
     store_hit(hits + i, h);
     flush_count<COUNT>(c, cnt);
 }
@@ -1507,7 +1511,10 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
     A.cam_air = cls == 0 ? 1 : 0;
     float si[3];   // the sun ray's 1 / direction as trace_world computes it (IEEE float division on the host)
     for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
-    if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
+    if (cls == 0 && !COUNT && vf::small_frame(ctx, (int64_t)A.ntiles * 4))
+        hipLaunchKernelGGL((k_primary_shadow64<false, true>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0],
+                           s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
+    else if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
         hipLaunchKernelGGL((k_primary_shadow64<COUNT>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1],
                            s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
     else if (cls != 1)
@@ -2129,6 +2136,13 @@ int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset) {
     return VF_OK;
 }
 
+int vf_set_traversal(vf_ctx *ctx, int32_t mode) {
+    if (!ctx) return vf::fail(ctx, VF_E_INVALID, "vf_set_traversal: null context");
+    if (mode < 0 || mode > 2) return vf::fail(ctx, VF_E_INVALID, "vf_set_traversal: mode %d is not 0, 1 or 2", mode);
+    ctx->traversal = mode;
+    return VF_OK;
+}
+
 int vf_set_timing(vf_ctx *ctx, int32_t on) {
     if (!ctx) return vf::fail(ctx, VF_E_INVALID, "vf_set_timing: null context");
     ctx->timing = on != 0;
@@ -2282,14 +2296,19 @@ int vf_trace_rays(vf_ctx *ctx, const vf_ray *rays, int64_t n, float tcap, uint32
     if (skip_mat != 0u && skip_mat != 5u) return vf::fail(ctx, VF_E_INVALID, "vf_trace_rays: skip_mat must be 0 or 5");
     const unsigned nb = blocks_for(n);
     if ((rc = begin_timed(ctx))) return rc;
+    const bool box = vf::small_frame(ctx, (n + 63) / 64);
     if (skip_mat == 5u) {
         if (ctx->counting)
             hipLaunchKernelGGL((k_rays<5, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+        else if (box)
+            hipLaunchKernelGGL((k_rays<5, false, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
         else
             hipLaunchKernelGGL((k_rays<5, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
     } else {
         if (ctx->counting)
             hipLaunchKernelGGL((k_rays<0, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
+        else if (box)
+            hipLaunchKernelGGL((k_rays<0, false, true>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
         else
             hipLaunchKernelGGL((k_rays<0, false>), dim3(nb), dim3(256), 0, ctx->stream, ctx->W, rays, (long long)n, tcap, hits, ctx->d_cnt);
     }

@@ -94,6 +94,7 @@ _SIGS = {
     "vf_graph_destroy": (None, [_P]),
     "vf_set_counting": (C.c_int, [_P, C.c_int32]),
     "vf_set_timing": (C.c_int, [_P, C.c_int32]),
+    "vf_set_traversal": (C.c_int, [_P, C.c_int32]),
     "vf_read_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int32]),
     "vf_last_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "vf_gicache_defaults": (C.c_int, [C.POINTER(GiCacheParams)]),
@@ -803,6 +804,10 @@ class Context:
         out = (C.c_uint64 * 4)()
         self._ck(lib().vf_read_counters(self._h, out, int(bool(reset))))
         return {"rays": out[0], "outer": out[1], "voxel": out[2], "bytes": out[3]}
+
+    def set_traversal(self, mode):
+        """0: by launch size (default), 1: the plain traversal, 2: the small-frame traversal (same hits)."""
+        self._ck(lib().vf_set_traversal(self._h, int(mode)))
 
     def set_timing(self, on):
         """Record HIP events around each trace call (for last_kernel_ms); off by default."""
