@@ -272,9 +272,8 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
 
 // The fused kernel for a camera in air with one 8x8-pixel wave per workgroup: a wave that finishes
 // frees its slot at once instead of waiting for the other three of a 16x16 tile.  Block L is quadrant
-// L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.  BOX: the
-// small-frame traversal (trace_world).
-template <bool COUNT, bool BOX = false>
+// L & 3 of 16x16 tile L >> 2, so the pixels of a wave are those of the 256-thread kernel.
+template <bool COUNT>
 __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
                                                          float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
                                                          unsigned long long *cnt, float ix, float iy, float iz) {
@@ -286,8 +285,24 @@ __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs
     const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
     const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
     if (px < A.w && py < A.h)
-        primary_shadow_pixel<COUNT, false, 64, true, BOX>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
+        primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
     flush_count<COUNT>(c, cnt);
+}
+
+// The same pass with the small-frame traversal (trace_world's BOX), for launches of at most 16 waves per
+// SIMD: held at 5 waves per SIMD (up to 102 VGPRs; at 7 the box test spilled registers to scratch).
+__global__ void __launch_bounds__(64, 5) k_primary_shadow_small(DevWorld W, PrimArgs A, float sx, float sy, float sz,
+                                                               vf_hit *hits, float4 *gbpos, uint32_t *gbn, uint32_t *shadow,
+                                                               vf_hit *shits, float ix, float iy, float iz) {
+    const float sun[3] = {sx, sy, sz};
+    const float sinv[3] = {ix, iy, iz};
+    Cnt c = {0, 0, 0, 0};
+    const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
+    const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
+    const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
+    if (px < A.w && py < A.h)
+        primary_shadow_pixel<false, false, 64, true, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
 }
 
 // gbuffer.comp over the scene (traceScenePrimary = traceSceneS, common.glsl:787-793: the world, then the
@@ -1512,8 +1527,8 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
     float si[3];   // the sun ray's 1 / direction as trace_world computes it (IEEE float division on the host)
     for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
     if (cls == 0 && !COUNT && vf::small_frame(ctx, (int64_t)A.ntiles * 4))
-        hipLaunchKernelGGL((k_primary_shadow64<false, true>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0],
-                           s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
+        hipLaunchKernelGGL(k_primary_shadow_small, dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
+                           hits, gbpos, gbn, shadow, shits, si[0], si[1], si[2]);
     else if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
         hipLaunchKernelGGL((k_primary_shadow64<COUNT>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1],
                            s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
