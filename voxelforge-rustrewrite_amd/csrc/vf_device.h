@@ -140,12 +140,13 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         // Every step of the walk moves one coordinate by the ray's fixed sign, from the entry voxel to the
         // voxel where the ray leaves the brick: per axis between lv and the coordinate at tout.  That
         // coordinate is taken 0.05 voxel further along the ray (the walk's and the brick DDA's tMax
-        // differ by rounding only, ~1e-6 of t) and clamped into the brick.  No solid voxel in that box
-        // (rows and columns of each 64-bit layer, layers blo[2]..bhi[2]): the walk cannot hit.
-        int32_t blo[3], bhi[3];
-        const int32_t lv[3] = {lx, ly, lz};
+        // differ by rounding only, ~1e-6 of t) and clamped into the brick.  No solid voxel in the box's rows
+        // and columns of any of the 8 layers: the walk cannot hit.  (Its z range is not applied: testing
+        // every layer costs fewer instructions than the range, and measured faster.)
+        int32_t blo[2], bhi[2];
+        const int32_t lv[2] = {lx, ly};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 2; ++a) {
             const float q = (ro[a] + sd[a] * tout) - (float)bmin[a];
             const int32_t ev = iclamp(f2i(floorf(st[a] > 0 ? q + 0.05f : q - 0.05f)), 0, 7);
             blo[a] = imin(lv[a], ev);
@@ -159,7 +160,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
                                 m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
         uint32_t any = 0u;
 #pragma unroll
-        for (int z = 0; z < 8; ++z) any |= (z >= blo[2] && z <= bhi[2]) ? ((w[2 * z] & rlo) | (w[2 * z + 1] & rhi)) : 0u;
+        for (int z = 0; z < 8; ++z) any |= (w[2 * z] & rlo) | (w[2 * z + 1] & rhi);
         if (any == 0u) return false;
         sm[0 * LS] = m0.x; sm[1 * LS] = m0.y; sm[2 * LS] = m0.z; sm[3 * LS] = m0.w;
         sm[4 * LS] = m1.x; sm[5 * LS] = m1.y; sm[6 * LS] = m1.z; sm[7 * LS] = m1.w;
@@ -188,6 +189,55 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     // brick DDA continues from its own state).  The reference caps the walk at 26 trips; it cannot reach
     // the cap: every step moves one coordinate by the ray's fixed sign, so each axis leaves [0, 7] after at
     // most 8 steps and the walk exits by trip 22.  The loop therefore has the single exit hit | out.
+    if (BOX) {
+        // The small-frame walk: two steps per trip, so a small frame's long walks pay the loop's exit test,
+        // branch and LDS wait once per two voxels (its time is the latency of its slowest walks).  The
+        // layers of z and z + s2 are held in registers (wlo/whi, ulo/uhi): the trip's second voxel is in
+        // one of them; the two for the next trip are read from LDS as soon as its z is known.  The steps,
+        // their order of comparisons and additions and the exits are those of the loop below; a trip
+        // that ends on its second voxel reports that voxel with its step's t and face.
+        uint32_t ulo = sm[(((lz + s2) & 7) * 2) * LS], uhi = sm[(((lz + s2) & 7) * 2 + 1) * LS];
+        bool hitA = false, outA = false, hitB = false;
+        int32_t lx1 = lx, ly1 = ly, lz1 = lz;
+        float tvA = tv;
+        int32_t f2A = f2;
+#pragma unroll 1
+        for (;;) {
+            hitA = (((((uint64_t)whi << 32) | (uint64_t)wlo) >> (((uint32_t)ly << 3) | (uint32_t)lx)) & 1ull) != 0ull;
+            const bool a0 = tMx < tMy && tMx < tMz;
+            const bool a1 = !a0 && tMy < tMz;
+            const bool a2 = !a0 && !a1;
+            lx1 = lx + (a0 ? s0 : 0); ly1 = ly + (a1 ? s1 : 0); lz1 = lz + (a2 ? s2 : 0);
+            outA = ((uint32_t)lx1 | (uint32_t)ly1 | (uint32_t)lz1) > 7u;
+            tvA = a0 ? tMx : (a1 ? tMy : tMz);
+            f2A = a0 ? fA0 : (a1 ? fA1 : fA2);
+            const float x1 = a0 ? tMx + tDx : tMx, y1 = a1 ? tMy + tDy : tMy, z1 = a2 ? tMz + tDz : tMz;
+            const uint32_t blo = a2 ? ulo : wlo, bhi = a2 ? uhi : whi;
+            hitB = (((((uint64_t)bhi << 32) | (uint64_t)blo) >> ((((uint32_t)ly1 << 3) | (uint32_t)lx1) & 63u)) & 1ull) != 0ull;
+            const bool b0 = x1 < y1 && x1 < z1;
+            const bool b1 = !b0 && y1 < z1;
+            const bool b2 = !b0 && !b1;
+            const int32_t lx2 = lx1 + (b0 ? s0 : 0), ly2 = ly1 + (b1 ? s1 : 0), lz2 = lz1 + (b2 ? s2 : 0);
+            const bool outB = ((uint32_t)lx2 | (uint32_t)ly2 | (uint32_t)lz2) > 7u;
+            wlo = sm[((lz2 & 7) * 2) * LS];
+            whi = sm[((lz2 & 7) * 2 + 1) * LS];
+            ulo = sm[(((lz2 + s2) & 7) * 2) * LS];
+            uhi = sm[(((lz2 + s2) & 7) * 2 + 1) * LS];
+            if (hitA | outA | hitB | outB) break;
+            tv = b0 ? x1 : (b1 ? y1 : z1);
+            f2 = b0 ? fA0 : (b1 ? fA1 : fA2);
+            tMx = b0 ? x1 + tDx : x1;
+            tMy = b1 ? y1 + tDy : y1;
+            tMz = b2 ? z1 + tDz : z1;
+            lx = lx2; ly = ly2; lz = lz2;
+        }
+        if (!hitA && !outA && hitB) {   // the hit is the trip's second voxel
+            lx = lx1; ly = ly1; lz = lz1;
+            tv = tvA;
+            f2 = f2A;
+        }
+        hit = hitA || (!outA && hitB);
+    } else
 #pragma unroll 1
     for (;;) {
         if (COUNT) { c.voxel++; c.bytes += 1; }
