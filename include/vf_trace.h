@@ -460,13 +460,14 @@ void vf_graph_destroy(vf_graph *graph);
 int vf_set_counting(vf_ctx *ctx, int32_t on);
 /* counters: {rays, brick steps, voxel steps, fetch bytes}; reset = 1 zeroes them after reading. */
 int vf_read_counters(vf_ctx *ctx, uint64_t out[4], int32_t reset);
-/* Which traversal the primary+sun pass (vf_trace_primary_shadow, camera in air) and vf_trace_rays launch;
- * both return the same hits.  0 (default): by launch size: launches of at most 16 wavefronts per SIMD
- * (small frames, whose time is that of their slowest rays) take the small-frame traversal, which first
- * tests each detailed brick's occupancy mask over the box of voxels its walk can visit and skips walks
- * that cannot hit; larger launches (issue-bound) take the plain one.  1: always the plain one.  2: always the
- * small-frame one.  The counting variants (vf_set_counting) always take the plain one.  No reference
- * counterpart: a kernel-selection knob of this library. */
+/* Which traversal the primary and primary+sun passes (vf_trace_primary, vf_trace_primary_shadow; camera in
+ * air) and vf_trace_rays launch; both return the same hits.  0 (default): by launch size.  Launches of at
+ * most 16 wavefronts per SIMD (small frames, whose time is that of their slowest rays) take the
+ * small-frame traversal: it tests each detailed brick's occupancy mask over the rows and columns its walk
+ * can visit and skips walks that cannot hit, walks two voxels per loop trip, and raises the issue priority
+ * of long-running waves.  Larger launches (issue-bound) take the plain one.  1: always the plain one.
+ * 2: always the small-frame one.  The counting variants (vf_set_counting) always take the plain one.  No
+ * reference counterpart: a kernel-selection knob of this library. */
 int vf_set_traversal(vf_ctx *ctx, int32_t mode);
 /* When on, every trace entry point records a HIP event before and after its launches, for
  * vf_last_kernel_ms.  Off by default: the two event records add GPU time to each call. */

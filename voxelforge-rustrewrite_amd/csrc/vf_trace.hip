@@ -83,13 +83,13 @@ __device__ __forceinline__ void tile_pixel_of(int32_t L, int32_t tiles_x, int32_
 }
 
 // gbuffer.comp primary ray (gbuffer.comp:194-202, 233-266, 470-480) for a 16x16 tile per workgroup.
-template <bool COUNT, bool SPECIAL>
+template <bool COUNT, bool SPECIAL, int LS = 256, bool BOX = false>
 __device__ __forceinline__ void primary_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px, int32_t py,
                                               vf_hit *hits, float4 *gbpos, uint32_t *gbn, Cnt &c) {
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
-    primary_hit<COUNT, SPECIAL>(W, A, cam_mat, rd, h, c);
+    primary_hit<COUNT, SPECIAL, LS, BOX>(W, A, cam_mat, rd, h, c);
     if (camera_special(cam_mat) != SPECIAL) return;
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     store_hit(hits + i, h);
@@ -287,6 +287,17 @@ __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs
     if (px < A.w && py < A.h)
         primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
     flush_count<COUNT>(c, cnt);
+}
+
+// gbuffer.comp's primary pass with the small-frame traversal (trace_world's BOX) for a camera in air, one
+// 8x8-pixel wave per workgroup, for launches of at most 16 waves per SIMD.
+__global__ void __launch_bounds__(64, 5) k_primary_small(DevWorld W, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn) {
+    Cnt c = {0, 0, 0, 0};
+    const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
+    const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
+    const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
+    const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
+    if (px < A.w && py < A.h) primary_pixel<false, false, 64, true>(W, A, 0u, px, py, hits, gbpos, gbn, c);
 }
 
 // The same pass with the small-frame traversal (trace_world's BOX), for launches of at most 16 waves per
@@ -1511,7 +1522,9 @@ template <bool COUNT>
 void launch_primary(vf_ctx *ctx, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn) {
     const int cls = camera_class(ctx, A.pos);
     A.cam_air = cls == 0 ? 1 : 0;
-    if (cls != 1)
+    if (cls == 0 && !COUNT && vf::small_frame(ctx, (int64_t)A.ntiles * 4))
+        hipLaunchKernelGGL(k_primary_small, dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, hits, gbpos, gbn);
+    else if (cls != 1)
         hipLaunchKernelGGL((k_primary<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, hits, gbpos,
                            gbn, ctx->d_cnt);
     const int32_t g = A.ntiles < ctx->n_cu ? A.ntiles : ctx->n_cu;
