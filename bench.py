@@ -704,7 +704,8 @@ def uneven_batch_config(ctx, stream, args):
             "view_latency_ms_p50": round(float(np.percentile(view_t, 50)), 5),
             "view_latency_ms_p99": round(float(np.percentile(view_t, 99)), 5),
             "probes_done_ms": round(float(times[16]), 5),
-            "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill},
+            "queue": {"tile": [bbest.tile_w, bbest.tile_h], "chunk": bbest.chunk, "refill": bbest.refill,
+                      "traversal": bbest.traversal},
             "sequential_plain_launches_ms": round(ms5p, 5),
             "view_sizes": [[vw, vh] for vw, vh, _ in views]}
 

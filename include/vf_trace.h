@@ -373,8 +373,11 @@ int vf_trace_probes(vf_ctx *ctx, const float origin[3], float spacing, const int
  * vf_trace_batch: one launch with a wavefront per 64 consecutive ray ids of the requests (in order, tile
  * order within a view), each traced to completion; completion stamps as in the queue modes.  The autotuner
  * measures it beside the queue configurations. */
+/* traversal (vf_trace_batch with refill -1 only): 0 the plain traversal, 1 the small-frame one (see
+ * vf_set_traversal); the same hits either way. */
 typedef struct vf_queue_cfg {
     int32_t tile_w, tile_h, chunk, sort, groups_per_cu, refill;
+    int32_t traversal;
 } vf_queue_cfg;
 int vf_queue_defaults(vf_queue_cfg *cfg);
 
@@ -426,8 +429,8 @@ int vf_read_batch_times(vf_ctx *ctx, int32_t n_req, double *ms);
 
 /* Autotuner: times every refill mode {chunk mode, refill 32} x tile shape {8x8, 16x8, 16x16} x chunk
  * {64..1024} (x sort off/on for the GI batch), plus refill -1 (no queue: the GI batch at 16x16, the batch
- * at each tile shape), on the given batch (median of 3 runs) and returns the fastest configuration; base
- * (nullable) supplies groups_per_cu. */
+ * at each tile shape with either traversal), on the given batch (median of 3 runs) and returns the fastest
+ * configuration; base (nullable) supplies groups_per_cu. */
 enum { VF_TUNE_GI_BATCH = 0, VF_TUNE_BATCH = 1 };
 typedef struct vf_tune_result { vf_queue_cfg cfg; float ms; } vf_tune_result;
 typedef struct vf_tune_gi_args {

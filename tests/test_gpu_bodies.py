@@ -415,7 +415,8 @@ def test_uneven_batch_with_bodies_matches_plain_launches(ctx):
     ctx.sync()
     body_hits = sum(int((vf.hits_to_numpy(r)[:n]["body"] == 1).sum()) for r, n in refs)
     assert body_hits > 50, body_hits
-    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 512, refill=0), ctx.queue_cfg((8, 8), 64, refill=-1)):
+    for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 512, refill=0), ctx.queue_cfg((8, 8), 64, refill=-1),
+                ctx.queue_cfg((8, 8), 64, refill=-1, traversal=1)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)

@@ -102,7 +102,8 @@ def test_uneven_batch_matches_plain_launches(ctx):
     reqs, outs, passes = _requests(ctx, sizes, probes)
     for cfg in (ctx.queue_cfg((8, 8), 64), ctx.queue_cfg((16, 16), 1024), ctx.queue_cfg((16, 8), 256),
                 ctx.queue_cfg((8, 8), 64, refill=0), ctx.queue_cfg((16, 16), 512, refill=0),
-                ctx.queue_cfg((8, 8), 64, refill=-1), ctx.queue_cfg((16, 8), 256, refill=-1)):
+                ctx.queue_cfg((8, 8), 64, refill=-1), ctx.queue_cfg((16, 8), 256, refill=-1),
+                ctx.queue_cfg((16, 8), 64, refill=-1, traversal=1)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)
@@ -199,7 +200,7 @@ def test_full_config5_batch_matches_plain_launches_and_oracle(ctx):
     ctx.trace_probes(origin, spacing, dims, nr, frame, reach, ref[-1])
     ctx.sync()
     for cfg in (ctx.queue_cfg((16, 8), 512, refill=0), ctx.queue_cfg((8, 8), 256, refill=32),
-                ctx.queue_cfg((16, 16), 64, refill=-1)):
+                ctx.queue_cfg((16, 16), 64, refill=-1), ctx.queue_cfg((16, 8), 64, refill=-1, traversal=1)):
         for o in outs:
             o.zero_()
         ctx.trace_batch(reqs, cfg)

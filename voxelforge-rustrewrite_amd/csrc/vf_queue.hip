@@ -677,9 +677,10 @@ __global__ void __launch_bounds__(256) k_batch_chunked(DevWorld W, DevBodies Bo,
 // dealt to the XCDs in runs like the plain passes; each lane traces one id to completion, and the workgroup
 // subtracts its ids from the request's count in one atomic, so the request's last workgroup stamps its
 // completion as in the queue modes.  (A per-wave atomic serialised on the few counters: 3x slower.)  The
-// start stamp is the first workgroup's.
-template <bool COUNT, bool BODIES>
-__global__ void __launch_bounds__(256, BODIES ? 1 : 7) k_batch_flat(DevWorld W, DevBodies Bo, BatchArgs B, unsigned long long *ctr,
+// start stamp is the first workgroup's.  BOX: the small-frame traversal (trace_world), held at 5 waves per
+// SIMD (its box test would spill registers at 7).
+template <bool COUNT, bool BODIES, bool BOX = false>
+__global__ void __launch_bounds__(256, BODIES ? 1 : (BOX ? 5 : 7)) k_batch_flat(DevWorld W, DevBodies Bo, BatchArgs B, unsigned long long *ctr,
                                                     unsigned long long *stamps, unsigned long long *cnt) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMin(stamps + kMaxReq, (unsigned long long)wall_clock64());
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
@@ -699,7 +700,7 @@ __global__ void __launch_bounds__(256, BODIES ? 1 : 7) k_batch_flat(DevWorld W, 
         batch_ray(W, q, id - q.first, ro, rd, tcap, out, skip, tskip, void_ray);
         if (out != 0xFFFFFFFFu) {
             Hit h;
-            trace_world<-1, COUNT, true>(W, ro, rd, tcap, h, c, skip);
+            trace_world<-1, COUNT, true, 256, false, BOX && !COUNT>(W, ro, rd, tcap, h, c, skip);
             if (BODIES) batch_bodies<COUNT>(W, Bo, q, ro, rd, h, c);
             h.t = h.t + tskip;
             if (void_ray) h.hit = 0;
@@ -800,6 +801,7 @@ int check_cfg(vf_ctx *ctx, const vf_queue_cfg *cfg, const char *what) {
 vf_queue_cfg default_cfg() {
     vf_queue_cfg c;
     c.tile_w = 8; c.tile_h = 8; c.chunk = 256; c.sort = 1; c.groups_per_cu = 0; c.refill = 32;
+    c.traversal = 0;
     return c;
 }
 
@@ -974,8 +976,9 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
     if (cfg.refill < 0) {   // flat: one workgroup per 256 ids, no queue
         using FlatFn = void (*)(DevWorld, DevBodies, BatchArgs, unsigned long long *, unsigned long long *,
                                 unsigned long long *);
-        const FlatFn fn = bodies ? (counting ? k_batch_flat<true, true> : k_batch_flat<false, true>)
-                                 : (counting ? k_batch_flat<true, false> : k_batch_flat<false, false>);
+        const bool box = cfg.traversal == 1 && !counting;
+        const FlatFn fn = bodies ? (counting ? k_batch_flat<true, true> : box ? k_batch_flat<false, true, true> : k_batch_flat<false, true>)
+                                 : (counting ? k_batch_flat<true, false> : box ? k_batch_flat<false, false, true> : k_batch_flat<false, false>);
         hipLaunchKernelGGL(fn, dim3((unsigned)(B.total / 256u)), dim3(256), 0, ctx->stream, ctx->W, Bo, B, ctr, stamps,
                            ctx->d_cnt);
     } else if (cfg.refill == 0) {
@@ -1031,6 +1034,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     static const int chunks[5] = {64, 128, 256, 512, 1024};
     const int sorts = workload == VF_TUNE_GI_BATCH ? 2 : 1;
     static const int refills[3] = {-1, 0, 32};   // -1: no queue (GI batch: one configuration; batch: per tile)
+    const int travs = workload == VF_TUNE_BATCH ? 2 : 1;   // the batch's flat mode with either traversal
     int k = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
@@ -1038,6 +1042,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
     VF_HIP(ctx, hipEventCreate(&e1));
     int rc = VF_OK;
     for (int rf : refills)
+    for (int tv = 0; tv < (rf < 0 ? travs : 1); ++tv)
     for (int so = 0; so < sorts && rc == VF_OK; ++so)
         for (const auto &t : tiles)
             for (int ch : chunks) {
@@ -1045,6 +1050,7 @@ int vf_autotune(vf_ctx *ctx, int32_t workload, const void *args, const vf_queue_
                 vf_queue_cfg c = default_cfg();
                 c.tile_w = t[0]; c.tile_h = t[1]; c.chunk = ch; c.sort = so;
                 c.refill = rf;
+                c.traversal = tv;
                 if (base_cfg) c.groups_per_cu = base_cfg->groups_per_cu;
                 auto run = [&]() -> int {
                     if (workload == VF_TUNE_GI_BATCH) {

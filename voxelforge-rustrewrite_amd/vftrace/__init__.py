@@ -692,10 +692,11 @@ class Context:
 
     # persistent-queue batches
     @staticmethod
-    def queue_cfg(tile=(8, 8), chunk=256, sort=True, groups_per_cu=0, refill=32):
+    def queue_cfg(tile=(8, 8), chunk=256, sort=True, groups_per_cu=0, refill=32, traversal=0):
         c = QueueCfg()
         c.tile_w, c.tile_h = int(tile[0]), int(tile[1])
         c.chunk, c.sort, c.groups_per_cu, c.refill = int(chunk), int(bool(sort)), int(groups_per_cu), int(refill)
+        c.traversal = int(traversal)
         return c
 
     def trace_gi_batch(self, pass_ubo, width, height, reach, gb_pos, gb_normal, gi_hits, sun_hits, cfg=None):

@@ -123,7 +123,7 @@ class GenParams(C.Structure):
 
 class QueueCfg(C.Structure):
     _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("chunk", C.c_int32), ("sort", C.c_int32),
-                ("groups_per_cu", C.c_int32), ("refill", C.c_int32)]
+                ("groups_per_cu", C.c_int32), ("refill", C.c_int32), ("traversal", C.c_int32)]
 
 
 class Request(C.Structure):
