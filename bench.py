@@ -605,7 +605,8 @@ def fixed_world_config(ctx, stream, args, cam_pos, right, up, fwd):
     """Primary + sun rays on the fixed world at 640x360 (the fused pass launched eagerly each step, as the
     headline's frames are, so the launch is included), and the same frame at 320x180 and 1280x720 with the
     fused kernel alone (20 back-to-back launches in one graph): its time hardly changes from 320x180 to
-    640x360, the latency of the frame's longest wavefronts."""
+    640x360, the latency of the frame's longest wavefronts.  These launches take the small-frame traversal
+    (vf_set_traversal); the 640x360 frame is also timed with the plain one."""
     dev = torch.device("cuda:0")
     out = {}
     pu = vf.make_pass(cam_pos, right, up, fwd, 70.0, 0)
@@ -630,6 +631,12 @@ def fixed_world_config(ctx, stream, args, cam_pos, right, up, fwd):
         ms_k = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
         scaling[f"{w1}x{h1}"] = {"rays": int(c1["rays"]), "kernel_us": round(ms_k * 1e3, 2),
                                  "grays": round(c1["rays"] / (ms_k * 1e-3) / 1e9, 4)}
+        if (w1, h1) == (640, 360):   # the same frame with the plain traversal (vf_set_traversal 1)
+            ctx.set_traversal(1)
+            ms_p = graph_ms(ctx, stream, lambda: [fn() for _ in range(reps)], max(1, args.steps // reps)) / reps
+            ctx.set_traversal(0)
+            scaling["640x360_plain_traversal"] = {"rays": int(c1["rays"]), "kernel_us": round(ms_p * 1e3, 2),
+                                                  "grays": round(c1["rays"] / (ms_p * 1e-3) / 1e9, 4)}
     out["fixed_world_frame_sizes"] = scaling
     return out
 
