@@ -83,6 +83,7 @@ def lib():
         sig = {
             "vfo_trace": (None, [P, P, P, C.c_float, C.c_uint32, P, P]),
             "vfo_trace_batch": (None, [P, P, C.c_int64, C.c_float, C.c_uint32, P, P, C.c_int]),
+            "vfo_trace_box_batch": (None, [P, P, C.c_int64, C.c_float, C.c_uint32, C.c_float, C.c_int, P, P, C.c_int]),
             "vfo_march_f64": (None, [P, P, P, C.c_double, C.c_uint32, P, P, P, P, P]),
             "vfo_raycast_cpu": (None, [P, P, P, C.c_float, C.c_uint32, P, P, P, P, P]),
             "vfo_voxel": (C.c_uint32, [P, C.c_int32, C.c_int32, C.c_int32]),
@@ -288,6 +289,16 @@ class OracleWorld:
         lib().vfo_trace_batch(C.byref(self.s), _p(r), r.shape[0], float(t_cap), int(skip_mat), _p(out), C.byref(cnt),
                               int(threads))
         return out, cnt.as_dict()
+
+    def trace_rays_box(self, rays, t_cap=1e30, skip_mat=0, margin=0.05, mode=0, threads=0):
+        """trace_rays with the library's small-frame mask test before each walk -> (hits, walks skipped,
+        walks run); mode 1 plants a mistake in the test (the entry voxel left out of the box)."""
+        r = f32(rays).reshape(-1, 6)
+        out = np.zeros(r.shape[0], HIT_DTYPE)
+        st = np.zeros(2, np.int64)
+        lib().vfo_trace_box_batch(C.byref(self.s), _p(r), r.shape[0], float(t_cap), int(skip_mat), float(margin),
+                                  int(mode), _p(out), _p(st), int(threads))
+        return out, int(st[0]), int(st[1])
 
     def trace_one(self, ro, rd, t_cap=1e30, skip_mat=0):
         h = VfoHit()

@@ -110,9 +110,48 @@ static int brick_voxels(const vfo_world *w, const float ro[3], const float sd[3]
     return 0;
 }
 
-/* ---- traceVolumeS over the world (common.glsl:541-658), WATER_CLEAR and USE_DIST_FIELD on ---- */
+/* ---- the library's small-frame mask test, restated to check that it is conservative ----
+ * Before a detailed brick's walk the library's small-frame traversal tests the brick's solid voxels (voxel
+ * != 0 and != skip) in the rows and columns, over all 8 layers, of the box from the walk's entry voxel to
+ * the voxel at the brick DDA's exit time (least tMax), that coordinate taken `margin` voxel further along
+ * the ray, and skips a walk with none.  mode 1 drops the entry voxel from the box, a planted mistake the
+ * tests must catch.  Returns 1 when the walk may be skipped. */
+static int box_may_skip(const vfo_world *w, const float ro[3], const float sd[3], const int32_t st[3], const int32_t bc[3],
+                        uint32_t g, float t, const float tM8[3], uint32_t skip, float margin, int mode) {
+    const uint8_t *brick = w->atlas + (size_t)(g - 256u) * 512u;
+    const float e = t + 1e-3f;
+    const float tout = gmin(gmin(tM8[0], tM8[1]), tM8[2]);
+    int32_t lo[2], hi[2];
+    for (int a = 0; a < 2; ++a) {
+        const int32_t bmin = bc[a] * 8;
+        const int32_t lv = iclamp(f2i(floorf(ro[a] + sd[a] * e)), bmin, bmin + 7) - bmin;
+        const float q = (ro[a] + sd[a] * tout) - (float)bmin;
+        const int32_t ev = iclamp(f2i(floorf(st[a] > 0 ? q + margin : q - margin)), 0, 7);
+        lo[a] = mode == 1 ? ev : imin(lv, ev);
+        hi[a] = mode == 1 ? ev : imax(lv, ev);
+    }
+    for (int z = 0; z < 8; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                const uint32_t m = brick[z * 64 + y * 8 + x];
+                if (m != 0u && m != skip) return 0;
+            }
+    return 1;
+}
+
+/* ---- traceVolumeS over the world (common.glsl:541-658), WATER_CLEAR and USE_DIST_FIELD on ----
+ * box_mode < 0: the reference's traversal; else the walks box_may_skip rules out are skipped (counted in
+ * box_stats[0], walks run in box_stats[1]). */
+static void trace_impl(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip,
+                       vfo_hit *h, vfo_count *cnt, float box_margin, int box_mode, int64_t *box_stats);
+
 void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip,
                vfo_hit *h, vfo_count *cnt) {
+    trace_impl(w, ro, rd, tcap, skip, h, cnt, 0.0f, -1, NULL);
+}
+
+static void trace_impl(const vfo_world *w, const float ro[3], const float rd[3], float tcap, uint32_t skip,
+                       vfo_hit *h, vfo_count *cnt, float box_margin, int box_mode, int64_t *box_stats) {
     memset(h, 0, sizeof(*h));
     h->t = -1.0f;
     if (cnt) cnt->rays++;
@@ -167,7 +206,12 @@ void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float t
             }
         }
         if (g >= 256u) {
-            if (brick_voxels(w, ro, sd, inv, st, up, bc, g, t, normal, skip, h, cnt)) return;
+            if (box_mode >= 0 && box_may_skip(w, ro, sd, st, bc, g, t, tM8, skip, box_margin, box_mode)) {
+                box_stats[0]++;
+            } else {
+                if (box_stats) box_stats[1]++;
+                if (brick_voxels(w, ro, sd, inv, st, up, bc, g, t, normal, skip, h, cnt)) return;
+            }
         } else if (g != 0u && g != skip) {
             float e = t + 1e-3f;
             h->hit = 1; h->t = t; h->mat = g;
@@ -506,6 +550,22 @@ void vfo_trace_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float t
         add_cnt(&total, &c);
     }
     if (cnt) add_cnt(cnt, &total);
+}
+
+void vfo_trace_box_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip, float margin,
+                         int mode, vfo_hitrec *out, int64_t stats[2], int threads) {
+    int64_t skipped = 0, walked = 0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads(threads)) reduction(+ : skipped, walked)
+    for (int64_t i = 0; i < n; ++i) {
+        vfo_hit h;
+        int64_t st[2] = {0, 0};
+        trace_impl(w, rays[i].o, rays[i].d, tcap, skip, &h, NULL, margin, mode, st);
+        to_rec(&h, &out[i]);
+        skipped += st[0];
+        walked += st[1];
+    }
+    stats[0] = skipped;
+    stats[1] = walked;
 }
 
 /* traceSceneS per ray (world + bodies), with the body index per ray (-1 = world or miss; nullable). */

@@ -85,6 +85,11 @@ void vfo_trace(const vfo_world *w, const float ro[3], const float rd[3], float t
                vfo_hit *out, vfo_count *cnt);
 void vfo_trace_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip_mat,
                      vfo_hitrec *out, vfo_count *cnt, int threads);
+/* vfo_trace_batch with the library's small-frame mask test before each walk (margin 0.05 and mode 0 as the
+ * library; mode 1 a planted mistake); stats = {walks skipped, walks run}.  Checks that the test never
+ * skips a walk that hits. */
+void vfo_trace_box_batch(const vfo_world *w, const vfo_ray *rays, int64_t n, float tcap, uint32_t skip_mat, float margin,
+                         int mode, vfo_hitrec *out, int64_t stats[2], int threads);
 /* --- bodies (common.glsl:664-720, 787-793; gpu_world.rs:550-600) --- */
 /* build_bvh over the bodies' world boxes: nodes (8 floats each, at most 2n - 1) and prims (n). */
 void vfo_build_bvh(const float *bodies, int32_t n, float *nodes, int32_t *n_nodes, uint32_t *prims);

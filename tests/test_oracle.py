@@ -159,6 +159,54 @@ def test_skips_equal_plain_brick_dda(kind):
         assert c["outer"] < cp["outer"]   # the skips do skip
 
 
+def _config1_frame_rays(ow, scene_sun):
+    """The config-1 frame at 640x360: its primary rays and the sun rays of its lit texels."""
+    pu, cam = config1_camera()
+    W, H = 640, 360
+    _, pos, nrm, _ = ow.pass_primary(cam, W, H)
+    dirs = O.primary_dirs(cam, W, H)
+    prim = np.concatenate([np.broadcast_to(np.asarray(cam["pos"], np.float32), dirs.shape), dirs], 1)
+    N = np.stack([((nrm >> (8 * k)) & 255) / np.float32(255.0) * 2 - 1 for k in range(3)], 1).astype(np.float32)
+    sun = np.asarray(scene_sun, np.float32)
+    lit = (pos[:, 3] >= 0) & ((N @ sun) > 0)
+    so = pos[lit, :3] + N[lit] * np.float32(0.03)
+    return prim.astype(np.float32), np.concatenate([so, np.broadcast_to(sun, so.shape)], 1).astype(np.float32)
+
+
+def test_small_frame_mask_test_never_skips_a_hit():
+    """The library's small-frame traversal skips a detailed brick's walk when the brick's mask has no solid
+    voxel in the rows and columns of the box from the walk's entry voxel to the voxel at the brick DDA's
+    exit time (taken 0.05 voxel further).  Restated in the oracle: every hit equals the reference
+    traversal's on the config-1 frame's primary and sun rays, on random rays with tiny and zero direction
+    components, and on the ring world; the test does skip walks; and a planted mistake in it (the entry
+    voxel left out of the box, or the exit coordinate taken 0.6 voxel short) changes hits."""
+    _, a, ow = world(1)
+    sun = vf.default_scene((32, 16, 32)).sun_dir
+    prim, sunr = _config1_frame_rays(ow, sun)
+    cases = [(prim, 0), (sunr, 5), (random_rays(60_000, [-30, -10, -30], [286, 140, 286], seed=17), 0),
+             (random_rays(60_000, [0, 30, 0], [256, 128, 256], seed=18), 5)]
+    skipped_total = walked_total = 0
+    for rays, skip in cases:
+        ref, _ = ow.trace_rays(rays, 1e30, skip)
+        got, skipped, walked = ow.trace_rays_box(rays, 1e30, skip)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+        skipped_total += skipped
+        walked_total += walked
+    assert skipped_total > 0.2 * (skipped_total + walked_total)
+    _, a2, ow2 = world(2, world_bricks=(64, 32, 64), chunks=3, trees_per_chunk=60)
+    win = a2["world_win"]
+    rays = random_rays(60_000, [win[0] * 8, 0, win[2] * 8], [(win[0] + 64) * 8, 256, (win[2] + 64) * 8], seed=19)
+    for skip in (0, 5):
+        ref, _ = ow2.trace_rays(rays, 1e30, skip)
+        got, _, _ = ow2.trace_rays_box(rays, 1e30, skip)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+    # planted mistakes must change hits
+    ref, _ = ow.trace_rays(prim, 1e30, 0)
+    for margin, mode in ((0.05, 1), (-0.6, 0)):
+        bad, _, _ = ow.trace_rays_box(prim, 1e30, 0, margin=margin, mode=mode)
+        assert (bad.view(np.uint8).reshape(-1, 20) != ref.view(np.uint8).reshape(-1, 20)).any(1).sum() > 100
+
+
 def test_fp64_remarch_agrees():
     """Independent FP64 voxel march: same voxel except at near-ties, |t32 - t64| <= 1e-5 relative."""
     _, a, ow = world(1)
