@@ -58,8 +58,9 @@ __device__ __forceinline__ uint32_t pack_normal(const float n[3], uint32_t a = 2
 
 // traceVolume over body oi (common.glsl:541-658 with useTex false, skipMat kSkipMat = 5): the box
 // [0, dims*8), the body's own grid, plain brick steps (no distance field or coarse skips, no water
-// test); detailed bricks are the world atlas's, walked by brick_voxels.  ro / rd in body space.
-template <bool COUNT>
+// test); detailed bricks are the world atlas's, walked by brick_voxels (LDSM: with the brick's mask in the
+// lane's LDS slots, in 256-thread kernels).  ro / rd in body space.
+template <bool COUNT, bool LDSM = false>
 __device__ bool trace_body(const DevWorld &W, const DevBodies &B, int32_t oi, const float ro[3], const float rd[3],
                            float tcap, Hit &h, Cnt &c) {
     const float4 *hd = B.hdr + (size_t)oi * kBodyWords;
@@ -110,7 +111,7 @@ __device__ bool trace_body(const DevWorld &W, const DevBodies &B, int32_t oi, co
         const uint32_t g = B.grid[goff + (uint32_t)((bc[2] * db[1] + bc[1]) * db[0] + bc[0])];
         if (COUNT) { c.outer++; c.bytes += 4; }
         if (g >= 256u) {
-            if (brick_voxels<COUNT>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, 5)) return true;
+            if (brick_voxels<COUNT, LDSM>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, 5)) return true;
         } else if (g != 0u && g != 5u) {
             const float e = t + 1e-3f;
             h.hit = 1; h.t = t; h.face = face; h.mat = g;
@@ -147,7 +148,7 @@ __device__ __forceinline__ void box_span(const float4 lo, const float4 hi, const
 
 // traceObject (common.glsl:665-683): the body's world box, then its volume in body space; a hit closer
 // than limit becomes the best hit (body-space cell and face, obj = oi) and the new limit.
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ void trace_object(const DevWorld &W, const DevBodies &B, int32_t oi, const float ro[3],
                                              const float rd[3], const float inv[3], Hit &best, float &limit, Cnt &c) {
     const float4 *hd = B.hdr + (size_t)oi * kBodyWords;
@@ -162,7 +163,7 @@ __device__ __forceinline__ void trace_object(const DevWorld &W, const DevBodies 
                           (r1.x * rd[0] + r1.y * rd[1]) + r1.z * rd[2],
                           (r2.x * rd[0] + r2.y * rd[1]) + r2.z * rd[2]};
     Hit oh;
-    trace_body<COUNT>(W, B, oi, roO, rdO, limit, oh, c);
+    trace_body<COUNT, LDSM>(W, B, oi, roO, rdO, limit, oh, c);
     if (oh.hit && oh.t < limit) {
         best = oh;
         best.obj = oi;
@@ -174,7 +175,7 @@ __device__ __forceinline__ void trace_object(const DevWorld &W, const DevBodies 
 // best; any_hit returns at the first body hit.  The walk visits the nodes in the reference's order
 // (a visited interior node continues with its first child; a leaf or a culled node continues with the
 // node its stack would pop) and stops after 512 nodes as the reference's loop does.
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ void trace_objects(const DevWorld &W, const DevBodies &B, const float ro[3], const float rd[3], Hit &best,
                               float limit, bool any_hit, Cnt &c) {
     if (B.n == 0) return;
@@ -193,7 +194,8 @@ __device__ void trace_objects(const DevWorld &W, const DevBodies &B, const float
             continue;
         }
         if (visit)
-            for (uint32_t k = 0; k < count; ++k) trace_object<COUNT>(W, B, (int32_t)B.prims[first + k], ro, rd, inv, best, limit, c);
+            for (uint32_t k = 0; k < count; ++k)
+                trace_object<COUNT, LDSM>(W, B, (int32_t)B.prims[first + k], ro, rd, inv, best, limit, c);
         if (any_hit && best.hit) return;
         node = B.esc[node];
         if (node < 0) return;
@@ -201,11 +203,11 @@ __device__ void trace_objects(const DevWorld &W, const DevBodies &B, const float
 }
 
 // traceSceneS (common.glsl:787-793) after the world's part: h is the world hit (tCap when it missed).
-template <bool COUNT>
+template <bool COUNT, bool LDSM = false>
 __device__ __forceinline__ void add_bodies(const DevWorld &W, const DevBodies &B, const float ro[3], const float rd[3],
                                            float tcap, Hit &h, Cnt &c) {
     h.obj = -1;
-    trace_objects<COUNT>(W, B, ro, rd, h, h.hit ? h.t : tcap, false, c);
+    trace_objects<COUNT, LDSM>(W, B, ro, rd, h, h.hit ? h.t : tcap, false, c);
 }
 
 // gbuffer.comp's primary ray over the scene (gbuffer.comp:233-266 with traceScenePrimary = traceSceneS):
@@ -226,7 +228,7 @@ __device__ __forceinline__ void scene_primary_trace(const DevWorld &W, const Dev
         for (int a = 0; a < 3; ++a) ro[a] = cam[a] + rd[a] * tskip;
     }
     trace_world<-1, COUNT, LDSM>(W, ro, rd, 1e30f, h, c, cam_mat == 5u ? 5 : 0);
-    add_bodies<COUNT>(W, B, ro, rd, 1e30f, h, c);
+    add_bodies<COUNT, LDSM>(W, B, ro, rd, 1e30f, h, c);
     h.t = h.t + tskip;
     if (void_ray) h.hit = 0;
 }

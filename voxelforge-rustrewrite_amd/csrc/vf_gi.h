@@ -300,7 +300,7 @@ __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float
     // the first segment: the plain sun ray (most rays end here: no hit, or a hit that is not a leaf)
     Hit h;
     trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o0, d, 1e30f, h, c, 0, pinv);
-    if (BODIES) add_bodies<COUNT>(W, *B, o0, d, 1e30f, h, c);   // traceSceneShadow: a body always blocks
+    if (BODIES) add_bodies<COUNT, LDSM && LS == 256>(W, *B, o0, d, 1e30f, h, c);   // traceSceneShadow: a body always blocks
     if (first_base) store_hit(first_base + idx, h, BODIES && h.obj >= 0);
     tfirst = 0.0f;
     if (!h.hit) return 1.0f;
@@ -329,7 +329,7 @@ __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float
 #pragma unroll
         for (int a = 0; a < 3; ++a) o[a] = p[a] + d[a] * tm;
         trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o, d, 1e30f, h, c, 0, pinv);
-        if (BODIES) add_bodies<COUNT>(W, *B, o, d, 1e30f, h, c);
+        if (BODIES) add_bodies<COUNT, LDSM && LS == 256>(W, *B, o, d, 1e30f, h, c);
         if (COUNT) c.rays--;
         if (!h.hit) return T;
         if ((BODIES && h.obj >= 0) || W.palette[h.mat * 3u + 1u].z != 3.0f) return 0.0f;

@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(256) k_scene(DevWorld W, DevBodies B, PrimArgs
     if (body) body[i] = h.hit ? h.obj : -1;
     if (SHADOW) {
         const float sun[3] = {sx, sy, sz};
-        shadow[i] = shadow_texel<COUNT, true>(W, p, nq, sun, A.leaf, shits, i, c, &B);
+        shadow[i] = shadow_texel<COUNT, true, true>(W, p, nq, sun, A.leaf, shits, i, c, &B);
     }
     flush_count<COUNT>(c, cnt);
 }
