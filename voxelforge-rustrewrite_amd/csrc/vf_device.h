@@ -237,6 +237,57 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
             f2 = f2A;
         }
         hit = hitA || (!outA && hitB);
+    } else if (LDSM) {
+        // The walk in mirrored coordinates: u_a = l_a when the ray steps +1 along axis a, 7 - l_a when it
+        // steps -1, so every step adds 1 to one u_a.  The three are packed in one word (u_x | u_y << 4 |
+        // u_z << 8): a step is one addition, the step that leaves the brick is the one that takes a field
+        // to 8 (bit 3 of its nibble), and the voxel's own coordinates are q = u ^ mir.  A trip takes a step,
+        // then tests the voxel it stepped into; the entry voxel is tested before the first trip.  The
+        // entry time and face of a hit come from the last step's axis (its increment d) and the tMax
+        // values, which are advanced only when the walk goes on.  Steps, comparisons and additions are
+        // those of the one-step loop below.
+        const uint32_t mir = (s0 > 0 ? 0u : 7u) | (s1 > 0 ? 0u : 0x70u) | (s2 > 0 ? 0u : 0x700u);
+        uint32_t q = (uint32_t)lx | ((uint32_t)ly << 4) | ((uint32_t)lz << 8);
+        uint32_t u = q ^ mir, d = 0u;
+        if (COUNT) { c.voxel++; c.bytes += 1; }
+        // bit of (lx, ly) in the 64-bit layer: bits 0-2 and 4-6 of q
+        hit = ((((uint64_t)whi << 32) | (uint64_t)wlo) >> (((q >> 1) & 0x38u) | (q & 7u)) & 1ull) != 0ull;
+        if (!hit) {
+            // One basic block per trip: the loop's only branch is its exit, the tMax values are advanced by
+            // selects that keep them on the exit trip, and hit / out are values in registers.
+            uint32_t hb, ob;
+#pragma unroll 1
+            for (;;) {
+                const bool a0 = tMx < tMy && tMx < tMz;
+                const bool a1 = !a0 && tMy < tMz;
+                const bool a2 = !a0 && !a1;
+                d = a0 ? 1u : (a1 ? 16u : 256u);
+                u += d;
+                q = u ^ mir;
+                // the voxel's layer (its slot index wraps into the lane's 8 layers when the step left the
+                // brick; that value is then not used)
+                const uint32_t nz = (q >> 8) & 7u;
+                const uint32_t lo = sm[(nz * 2) * LS], hi2 = sm[(nz * 2 + 1) * LS];
+                hb = (uint32_t)((((uint64_t)hi2 << 32) | (uint64_t)lo) >> (((q >> 1) & 0x38u) | (q & 7u))) & 1u;
+                ob = u & 0x888u;
+                const bool stop = (hb | ob) != 0u;
+                if (COUNT && ob == 0u) { c.voxel++; c.bytes += 1; }   // a voxel inside the brick was tested
+                // adding -0 leaves every value as it is (-0 included)
+                tMx = tMx + ((a0 && !stop) ? tDx : -0.0f);
+                tMy = tMy + ((a1 && !stop) ? tDy : -0.0f);
+                tMz = tMz + ((a2 && !stop) ? tDz : -0.0f);
+                if (stop) break;
+            }
+            asm volatile("" : "+v"(d));   // read d as the register it is (not as the loop's axis conditions)
+            hit = ob == 0u;   // the loop stopped on a solid voxel inside the brick
+            if (hit) {
+                // the step into the hit voxel: its axis from d; tMax of that axis is still its value before
+                // the step (the exit trip does not advance it)
+                tv = d == 1u ? tMx : (d == 16u ? tMy : tMz);
+                f2 = d == 1u ? fA0 : (d == 16u ? fA1 : fA2);
+            }
+        }
+        if (hit) { lx = (int32_t)(q & 15u); ly = (int32_t)((q >> 4) & 15u); lz = (int32_t)(q >> 8); }
     } else
 #pragma unroll 1
     for (;;) {
