@@ -109,6 +109,17 @@ static __shared__ uint32_t vf_vmask64[16 * 64];
 // voxels the walk can visit; tout = the brick DDA's time of leaving the brick (the least of its tMax).  A
 // walk that cannot hit is skipped: its result is the same miss, after which the brick DDA goes on from its
 // own state as it does after any walk.
+// Occupancy bit of voxel (lx, ly) in a 64-bit layer (lo, hi), for q = lx | ly << 4 (higher bits ignored): bit
+// lx + 8 ly.  The shift is written as the instruction, which takes its amount modulo 64, so the index is
+// q's bits 0-2 merged with bits 3-5 of q >> 1 (one bitfield insert) with nothing masked off above them.
+__device__ __forceinline__ uint32_t layer_bit(uint32_t lo, uint32_t hi, uint32_t q) {
+    uint32_t idx;
+    asm("v_bfi_b32 %0, 7, %1, %2" : "=v"(idx) : "v"(q), "v"(q >> 1));   // (q & 7) | ((q >> 1) & ~7)
+    uint64_t r;
+    asm("v_lshrrev_b64 %0, %1, %2" : "=v"(r) : "v"(idx), "v"(((uint64_t)hi << 32) | (uint64_t)lo));
+    return (uint32_t)r & 1u;
+}
+
 template <bool COUNT, bool LDSM = false, int LS = 256, bool BOX = false>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
@@ -250,8 +261,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         uint32_t q = (uint32_t)lx | ((uint32_t)ly << 4) | ((uint32_t)lz << 8);
         uint32_t u = q ^ mir, d = 0u;
         if (COUNT) { c.voxel++; c.bytes += 1; }
-        // bit of (lx, ly) in the 64-bit layer: bits 0-2 and 4-6 of q
-        hit = ((((uint64_t)whi << 32) | (uint64_t)wlo) >> (((q >> 1) & 0x38u) | (q & 7u)) & 1ull) != 0ull;
+        hit = layer_bit(wlo, whi, q) != 0u;
         if (!hit) {
             // One basic block per trip: the loop's only branch is its exit, the tMax values are advanced by
             // selects that keep them on the exit trip, and hit / out are values in registers.
@@ -268,7 +278,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
                 // brick; that value is then not used)
                 const uint32_t nz = (q >> 8) & 7u;
                 const uint32_t lo = sm[(nz * 2) * LS], hi2 = sm[(nz * 2 + 1) * LS];
-                hb = (uint32_t)((((uint64_t)hi2 << 32) | (uint64_t)lo) >> (((q >> 1) & 0x38u) | (q & 7u))) & 1u;
+                hb = layer_bit(lo, hi2, q);
                 ob = u & 0x888u;
                 const bool stop = (hb | ob) != 0u;
                 if (COUNT && ob == 0u) { c.voxel++; c.bytes += 1; }   // a voxel inside the brick was tested
@@ -329,17 +339,38 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     return false;
 }
 
+// The per-direction set-up of trace_world for a direction shared by every ray of a launch (the sun's),
+// computed once on the host by dir_pre with trace_world's own operations: the clamped direction, its
+// reciprocal (IEEE float division), the step sign, the upper-plane flag, the face code of each axis
+// (normal -st) and the brick tDelta.
+struct DirPre {
+    float sd[3], inv[3], up[3], tD8[3];
+    int32_t st[3], faceA[3];
+};
+__host__ __device__ inline DirPre dir_pre(const float d[3]) {
+    DirPre p;
+    for (int a = 0; a < 3; ++a) {
+        p.sd[a] = fabsf(d[a]) < 1e-5f ? 1e-5f : d[a];
+        p.inv[a] = 1.0f / p.sd[a];
+        p.st[a] = p.sd[a] > 0.0f ? 1 : -1;
+        p.up[a] = p.sd[a] < 0.0f ? 0.0f : 1.0f;
+        p.faceA[a] = 2 * a + (p.st[a] > 0 ? 2 : 1);
+        p.tD8[a] = fabsf(p.inv[a]) * 8.0f;
+    }
+    return p;
+}
+
 // traceVolumeS over the world (common.glsl:541-658) with WATER_CLEAR; SKIP is skipMat (0 or 5), or -1 to
 // take it from skip_rt (one inlined copy serving both).
-// PRE: pinv holds 1 / rd (each component clamped away from 0 as below), computed once by the caller for a
-// direction shared by many rays (the sun's): the same IEEE quotients, without three divisions per ray.
+// PRE: pre holds the set-up of rd (dir_pre), so rays of a shared direction skip three divisions and the
+// per-axis selects.
 // BOX: the small-frame traversal (LDSM only, never counting): each detailed brick's walk is first tested
 // against its mask over the voxels it can visit, and a wave raises its issue priority after 4, 10 and 16
 // brick steps, so the few long rays that set a small frame's time are not held back by the frame's other
 // waves (the priority stays raised for the wave's later rays).
 template <int SKIP, bool COUNT, bool LDSM = false, int LS = 256, bool PRE = false, bool BOX = false>
 __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3], const float rd[3], float tcap, Hit &h,
-                                            Cnt &c, const int skip_rt = 0, const float *pinv = nullptr) {
+                                            Cnt &c, const int skip_rt = 0, const DirPre *pre = nullptr) {
     const int sk = SKIP >= 0 ? SKIP : skip_rt;
     h.hit = 0; h.t = -1.0f; h.face = 0; h.mat = 0; h.cell[0] = h.cell[1] = h.cell[2] = 0;
     if (COUNT) c.rays++;
@@ -347,17 +378,17 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
     int32_t st[3], faceA[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        sd[a] = fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a];
-        inv[a] = PRE ? pinv[a] : 1.0f / sd[a];
+        sd[a] = PRE ? pre->sd[a] : (fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a]);
+        inv[a] = PRE ? pre->inv[a] : 1.0f / sd[a];
         const float lo = (float)(W.minb[a] * 8);
         const float dims = (float)(W.wb[a] * 8);
         const float t0 = (lo - ro[a]) * inv[a];
         const float t1 = ((lo + dims) - ro[a]) * inv[a];
         tmin3[a] = gmin(t0, t1);
         tmax3[a] = gmax(t0, t1);
-        st[a] = sd[a] > 0.0f ? 1 : -1;
-        up[a] = sd[a] < 0.0f ? 0.0f : 1.0f;
-        faceA[a] = 2 * a + (st[a] > 0 ? 2 : 1);   // face code of normal -st on axis a
+        st[a] = PRE ? pre->st[a] : (sd[a] > 0.0f ? 1 : -1);
+        up[a] = PRE ? pre->up[a] : (sd[a] < 0.0f ? 0.0f : 1.0f);
+        faceA[a] = PRE ? pre->faceA[a] : 2 * a + (st[a] > 0 ? 2 : 1);   // face code of normal -st on axis a
     }
     const float tEnter = gmax(gmax(tmin3[0], tmin3[1]), tmin3[2]);
     const float tExit = gmin(gmin(tmax3[0], tmax3[1]), tmax3[2]);
@@ -372,7 +403,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
         for (int a = 0; a < 3; ++a) {
             const float entry = ro[a] + sd[a] * e;
             bc[a] = iclamp(f2i(floorf(entry * 0.125f)), W.minb[a], W.minb[a] + W.wb[a] - 1);
-            tD8[a] = fabsf(inv[a]) * 8.0f;
+            tD8[a] = PRE ? pre->tD8[a] : fabsf(inv[a]) * 8.0f;
             tM8[a] = (((float)bc[a] + up[a]) * 8.0f - ro[a]) * inv[a];
         }
     }
@@ -473,8 +504,9 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             bc[1] += a1 ? st[1] : 0;
             bc[2] += a2 ? st[2] : 0;
         }
-        if (bc[0] < W.minb[0] || bc[1] < W.minb[1] || bc[2] < W.minb[2] || bc[0] >= W.minb[0] + W.wb[0] ||
-            bc[1] >= W.minb[1] + W.wb[1] || bc[2] >= W.minb[2] + W.wb[2])
+        // outside the loaded box on any axis (one unsigned comparison per axis, no short-circuit branches)
+        if (((uint32_t)(bc[0] - W.minb[0]) >= (uint32_t)W.wb[0]) | ((uint32_t)(bc[1] - W.minb[1]) >= (uint32_t)W.wb[1]) |
+            ((uint32_t)(bc[2] - W.minb[2]) >= (uint32_t)W.wb[2]))
             return;
     }
 }
@@ -535,6 +567,15 @@ __device__ __forceinline__ float unorm8_dec(uint32_t b) { return ((float)b / 255
 __device__ __forceinline__ void face_normal(int32_t face, float n[3]) {
     n[0] = n[1] = n[2] = 0.0f;
     if (face > 0) n[(face - 1) >> 1] = (face & 1) ? 1.0f : -1.0f;
+}
+// unorm8(n * 0.5 + 0.5) of face_normal(face)'s components, packed x | y << 8 | z << 16 (the G-buffer's
+// RGBA8 normal without its alpha).  The components are -1, 0 or +1, whose bytes are exactly 0, 128 and 255
+// (floor(0 * 255 + 0.5), floor(0.5 * 255 + 0.5), floor(255 + 0.5)), so the bytes come from the face code
+// with integer operations instead of three float conversions.
+__device__ __forceinline__ uint32_t face_normal_bytes(int32_t face) {
+    const uint32_t sh = ((uint32_t)((face - 1) >> 1) & 3u) * 8u;   // (& 3: a valid shift for face 0 too)
+    const uint32_t b = (face & 1) ? 0xFFu : 0u;
+    return face > 0 ? ((0x808080u & ~(0xFFu << sh)) | (b << sh)) : 0x808080u;
 }
 
 // PCG hash and samplers (common.glsl:1233-1288)

@@ -296,10 +296,10 @@ __device__ __forceinline__ void foliage_sun_origin(const float P[3], const float
 template <bool STOCHASTIC, bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool PRE = false, bool BOX = false>
 __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float o0[3], const float d[3], float leaf_light,
                                                   uint32_t &seed, float &tfirst, vf_hit *first_base, size_t idx, Cnt &c,
-                                                  const DevBodies *B = nullptr, const float *pinv = nullptr) {
+                                                  const DevBodies *B = nullptr, const DirPre *pre = nullptr) {
     // the first segment: the plain sun ray (most rays end here: no hit, or a hit that is not a leaf)
     Hit h;
-    trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o0, d, 1e30f, h, c, 0, pinv);
+    trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o0, d, 1e30f, h, c, 0, pre);
     if (BODIES) add_bodies<COUNT, LDSM && LS == 256>(W, *B, o0, d, 1e30f, h, c);   // traceSceneShadow: a body always blocks
     if (first_base) store_hit(first_base + idx, h, BODIES && h.obj >= 0);
     tfirst = 0.0f;
@@ -328,7 +328,7 @@ __device__ __forceinline__ float sun_visibility_t(const DevWorld &W, const float
         const float tm = gmin(gmin(tx[0], tx[1]), tx[2]) + 1e-3f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) o[a] = p[a] + d[a] * tm;
-        trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o, d, 1e30f, h, c, 0, pinv);
+        trace_world<5, COUNT, LDSM, LS, PRE, BOX>(W, o, d, 1e30f, h, c, 0, pre);
         if (BODIES) add_bodies<COUNT, LDSM && LS == 256>(W, *B, o, d, 1e30f, h, c);
         if (COUNT) c.rays--;
         if (!h.hit) return T;

@@ -129,8 +129,7 @@ __device__ __forceinline__ void gi_ray(const GenArgs &A, const float4 *gbpos, co
 template <bool COUNT, bool BODIES>
 __global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, DevBodies Bo, GenArgs A, const float4 *gbpos,
                                                    const uint32_t *gbn, vf_hit *gi_hits, vf_hit *sun_hits, float sx,
-                                                   float sy, float sz, unsigned long long *cnt, float ix, float iy,
-                                                   float iz) {
+                                                   float sy, float sz, unsigned long long *cnt, DirPre sp) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int32_t tiles_x = (A.w + 15) / 16;
@@ -151,8 +150,7 @@ __global__ void __launch_bounds__(256, 7) k_gi_direct(DevWorld W, DevBodies Bo, 
         const float sun_dir[3] = {sx, sy, sz};
         float so[3];
         if (sun_origin<BODIES>(Bo, h, o, d, sun_dir, so)) {
-            const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
-            trace_world<0, COUNT, true, 256, true>(W, so, sun_dir, 1e30f, sh, c, 0, sinv);
+            trace_world<0, COUNT, true, 256, true>(W, so, sun_dir, 1e30f, sh, c, 0, &sp);
             if (BODIES) sun_bodies<COUNT>(W, Bo, so, sun_dir, sh, c);
         }
     }
@@ -844,14 +842,13 @@ int vf_trace_gi_batch(vf_ctx *ctx, const vf_pass_ubo *pass, int32_t w, int32_t h
     if ((rc = vf::begin_timed(ctx))) return rc;
     if (cfg.refill < 0) {   // no queue
         const int32_t nt = ((w + 15) / 16) * ((h + 15) / 16);
-        float si[3];   // as the traversal computes it: 1 / the clamped direction, IEEE float division
-        for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
+        const DirPre sp = dir_pre(s);   // the sun ray's set-up, as trace_world computes it
         using DirectFn = void (*)(DevWorld, DevBodies, GenArgs, const float4 *, const uint32_t *, vf_hit *, vf_hit *, float,
-                                  float, float, unsigned long long *, float, float, float);
+                                  float, float, unsigned long long *, DirPre);
         const DirectFn fn = bodies ? (counting ? k_gi_direct<true, true> : k_gi_direct<false, true>)
                                    : (counting ? k_gi_direct<true, false> : k_gi_direct<false, false>);
         hipLaunchKernelGGL(fn, dim3(nt), dim3(256), 0, ctx->stream, ctx->W, Bo, A, reinterpret_cast<const float4 *>(gb_pos),
-                           gb_normal, gi_hits, sun_hits, s[0], s[1], s[2], ctx->d_cnt, si[0], si[1], si[2]);
+                           gb_normal, gi_hits, sun_hits, s[0], s[1], s[2], ctx->d_cnt, sp);
         if ((rc = vf::launch_check(ctx, "vf_trace_gi_batch"))) return rc;
         return vf::end_timed(ctx);
     }

@@ -101,12 +101,7 @@ __device__ __forceinline__ void primary_pixel(const DevWorld &W, const PrimArgs 
     }
     if (gbn) {
         uint32_t v = 0u;
-        if (h.hit) {
-            float n[3];
-            face_normal(h.face, n);
-            v = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
-                (mat_flag_byte(W, h.mat) << 24);
-        }
+        if (h.hit) v = face_normal_bytes(h.face) | (mat_flag_byte(W, h.mat) << 24);
         gbn[i] = v;
     }
 }
@@ -154,7 +149,7 @@ template <bool COUNT, bool BODIES = false, bool LDSM = false, int LS = 256, bool
           bool BOX = false>
 __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, uint32_t nq, const float sun[3], float leaf,
                                                  vf_hit *hits, size_t i, Cnt &c, const DevBodies *B = nullptr,
-                                                 const float *pinv = nullptr, const float *Ndec = nullptr) {
+                                                 const DirPre *pre = nullptr, const float *Ndec = nullptr) {
     float vis = 0.0f, tb = 0.0f;
     if (p.w < 0.0f) {
         vis = 1.0f;
@@ -178,7 +173,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
         }
         if (trace) {
             uint32_t seed = 0u;
-            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE, BOX>(W, so, sun, leaf, seed, tb, hits, i, c, B, pinv);
+            vis = sun_visibility_t<false, COUNT, BODIES, LDSM, LS, PRE, BOX>(W, so, sun, leaf, seed, tb, hits, i, c, B, pre);
             if (!lit) tb = 0.0f;
         } else if (hits) {
             store_miss(hits + i);
@@ -193,7 +188,7 @@ __device__ __forceinline__ uint32_t shadow_texel(const DevWorld &W, float4 p, ui
 template <bool COUNT, bool BODIES>
 __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArgs A, float sx, float sy, float sz,
                                                 const float4 *gbpos, const uint32_t *gbn, uint32_t *shadow, vf_hit *hits,
-                                                unsigned long long *cnt, float ix, float iy, float iz) {
+                                                unsigned long long *cnt, DirPre sp) {
     const int32_t L = xcd_remap((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t tx = L % A.tiles_x, ty = L / A.tiles_x;
     const int32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -203,8 +198,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevWorld W, DevBodies B, TileArg
     const size_t i = (size_t)py * (size_t)A.w + (size_t)px;
     const float sun[3] = {sx, sy, sz};
     Cnt c = {0, 0, 0, 0};
-    const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
-    shadow[i] = shadow_texel<COUNT, BODIES, true, 256, true>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B, sinv);
+    shadow[i] = shadow_texel<COUNT, BODIES, true, 256, true>(W, gbpos[i], gbn[i], sun, A.leaf, hits, i, c, &B, &sp);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -215,7 +209,7 @@ template <bool COUNT, bool SPECIAL, int LS = 256, bool PRE = false, bool BOX = f
 __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const PrimArgs &A, uint32_t cam_mat, int32_t px,
                                                      int32_t py, const float sun[3], vf_hit *hits, float4 *gbpos,
                                                      uint32_t *gbn, uint32_t *shadow, vf_hit *shits, Cnt &c,
-                                                     const float *pinv = nullptr) {
+                                                     const DirPre *pre = nullptr) {
     float rd[3];
     primary_dir(A, px, py, rd);
     Hit h;
@@ -233,8 +227,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
         p = make_float4(A.pos[0] + rd[0] * h.t, A.pos[1] + rd[1] * h.t, A.pos[2] + rd[2] * h.t, h.t);
         float n[3];
         face_normal(h.face, n);
-        nq = unorm8(n[0] * 0.5f + 0.5f) | (unorm8(n[1] * 0.5f + 0.5f) << 8) | (unorm8(n[2] * 0.5f + 0.5f) << 16) |
-             (mat_flag_byte(W, h.mat) << 24);
+        nq = face_normal_bytes(h.face) | (mat_flag_byte(W, h.mat) << 24);
 #pragma unroll
         for (int a = 0; a < 3; ++a) Nd[a] = n[a] > 0.0f ? 1.0f : (n[a] < 0.0f ? -1.0f : kDec128);
     } else {
@@ -243,7 +236,7 @@ __device__ __forceinline__ void primary_shadow_pixel(const DevWorld &W, const Pr
     if (gbpos) gbpos[i] = p;
     if (gbn) gbn[i] = nq;
     // shadow_trace.comp on (p, nq)
-    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE, true, BOX>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pinv, Nd);
+    shadow[i] = shadow_texel<COUNT, false, true, LS, PRE, true, BOX>(W, p, nq, sun, A.leaf, shits, i, c, nullptr, pre, Nd);
 }
 
 template <bool COUNT, bool SPECIAL>
@@ -276,16 +269,15 @@ __global__ void __launch_bounds__(256) k_primary_shadow(DevWorld W, PrimArgs A, 
 template <bool COUNT>
 __global__ void __launch_bounds__(64, 7) k_primary_shadow64(DevWorld W, PrimArgs A, float sx, float sy, float sz, vf_hit *hits,
                                                          float4 *gbpos, uint32_t *gbn, uint32_t *shadow, vf_hit *shits,
-                                                         unsigned long long *cnt, float ix, float iy, float iz) {
+                                                         unsigned long long *cnt, DirPre sp) {
     const float sun[3] = {sx, sy, sz};
-    const float sinv[3] = {ix, iy, iz};   // 1 / the sun direction, computed on the host
     Cnt c = {0, 0, 0, 0};
     const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
     const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
     const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
     if (px < A.w && py < A.h)
-        primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
+        primary_shadow_pixel<COUNT, false, 64, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, &sp);
     flush_count<COUNT>(c, cnt);
 }
 
@@ -304,16 +296,15 @@ __global__ void __launch_bounds__(64, 5) k_primary_small(DevWorld W, PrimArgs A,
 // SIMD: held at 5 waves per SIMD (up to 102 VGPRs; at 7 the box test spilled registers to scratch).
 __global__ void __launch_bounds__(64, 5) k_primary_shadow_small(DevWorld W, PrimArgs A, float sx, float sy, float sz,
                                                                vf_hit *hits, float4 *gbpos, uint32_t *gbn, uint32_t *shadow,
-                                                               vf_hit *shits, float ix, float iy, float iz) {
+                                                               vf_hit *shits, DirPre sp) {
     const float sun[3] = {sx, sy, sz};
-    const float sinv[3] = {ix, iy, iz};
     Cnt c = {0, 0, 0, 0};
     const int32_t L = xcd_remap64((int32_t)blockIdx.x, (int32_t)gridDim.x);
     const int32_t T = L >> 2, q = L & 3, lane = (int32_t)threadIdx.x;
     const int32_t px = (T % A.tiles_x) * 16 + (q & 1) * 8 + (lane & 7);
     const int32_t py = (T / A.tiles_x) * 16 + (q >> 1) * 8 + (lane >> 3);
     if (px < A.w && py < A.h)
-        primary_shadow_pixel<false, false, 64, true, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, sinv);
+        primary_shadow_pixel<false, false, 64, true, true>(W, A, 0u, px, py, sun, hits, gbpos, gbn, shadow, shits, c, &sp);
 }
 
 // gbuffer.comp over the scene (traceScenePrimary = traceSceneS, common.glsl:787-793: the world, then the
@@ -1537,14 +1528,13 @@ void launch_primary_shadow(vf_ctx *ctx, PrimArgs A, const float *s, vf_hit *hits
                            uint32_t *shadow, vf_hit *shits) {
     const int cls = camera_class(ctx, A.pos);
     A.cam_air = cls == 0 ? 1 : 0;
-    float si[3];   // the sun ray's 1 / direction as trace_world computes it (IEEE float division on the host)
-    for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
+    const DirPre sp = dir_pre(s);   // the sun ray's set-up, as trace_world computes it
     if (cls == 0 && !COUNT && vf::small_frame(ctx, (int64_t)A.ntiles * 4))
         hipLaunchKernelGGL(k_primary_shadow_small, dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1], s[2],
-                           hits, gbpos, gbn, shadow, shits, si[0], si[1], si[2]);
+                           hits, gbpos, gbn, shadow, shits, sp);
     else if (cls == 0)   // camera in air: one wave per workgroup (measured 0.5 % faster than 16x16 tiles)
         hipLaunchKernelGGL((k_primary_shadow64<COUNT>), dim3(A.ntiles * 4), dim3(64), 0, ctx->stream, ctx->W, A, s[0], s[1],
-                           s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, si[0], si[1], si[2]);
+                           s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt, sp);
     else if (cls != 1)
         hipLaunchKernelGGL((k_primary_shadow<COUNT, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, A, s[0],
                            s[1], s[2], hits, gbpos, gbn, shadow, shits, ctx->d_cnt);
@@ -2214,22 +2204,21 @@ int vf_trace_shadow(vf_ctx *ctx, const vf_pass_ubo *p, int32_t w, int32_t h, con
     const float *s = ctx->S.sun_dir;
     const DevBodies B = vf::dev_bodies(ctx);
     const float4 *gp = reinterpret_cast<const float4 *>(gb_pos);
-    float si[3];   // as the traversal computes it: 1 / the clamped direction, IEEE float division
-    for (int a = 0; a < 3; ++a) si[a] = 1.0f / (std::fabs(s[a]) < 1e-5f ? 1e-5f : s[a]);
+    const DirPre sp = dir_pre(s);   // the sun ray's set-up, as trace_world computes it
     if ((rc = begin_timed(ctx))) return rc;
     if (B.n > 0) {
         if (ctx->counting)
             hipLaunchKernelGGL((k_shadow<true, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
-                               gp, gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
+                               gp, gb_normal, shadow, hits, ctx->d_cnt, sp);
         else
             hipLaunchKernelGGL((k_shadow<false, true>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2],
-                               gp, gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
+                               gp, gb_normal, shadow, hits, ctx->d_cnt, sp);
     } else if (ctx->counting) {
         hipLaunchKernelGGL((k_shadow<true, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
-                           gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
+                           gb_normal, shadow, hits, ctx->d_cnt, sp);
     } else {
         hipLaunchKernelGGL((k_shadow<false, false>), dim3(A.ntiles), dim3(256), 0, ctx->stream, ctx->W, B, A, s[0], s[1], s[2], gp,
-                           gb_normal, shadow, hits, ctx->d_cnt, si[0], si[1], si[2]);
+                           gb_normal, shadow, hits, ctx->d_cnt, sp);
     }
     if ((rc = launch_check(ctx, "k_shadow"))) return rc;
     return end_timed(ctx);
