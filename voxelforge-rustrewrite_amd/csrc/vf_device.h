@@ -75,9 +75,17 @@ __device__ __forceinline__ void normalize3(float v[3]) {
 // Cell word index of absolute brick (x, y, z) inside the window (gridWrap, common.glsl:134-136): the grid's
 // own x-fastest order.  (4x4x4-brick tiles cost 10 more VALU instructions per brick step for no
 // measurable cache gain: the traversal kernels are issue-bound and their world reads hit the L1.)
+// Both products are of factors below 2^24 (brick dims <= 4096, so z * wby + y < 2^24): 24-bit multiply-adds
+// (one full-rate instruction each, written as the instruction: the compiler widens the first to a 64-bit
+// multiply-add), the same low 32 bits as 32-bit multiplies.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
 __device__ __forceinline__ uint32_t cell_index(const DevWorld &W, int32_t x, int32_t y, int32_t z) {
     if (W.ring) { x &= W.wb[0] - 1; z &= W.wb[2] - 1; }
-    return ((uint32_t)z * (uint32_t)W.wb[1] + (uint32_t)y) * (uint32_t)W.wb[0] + (uint32_t)x;
+    return mad_u24(mad_u24((uint32_t)z, (uint32_t)W.wb[1], (uint32_t)y), (uint32_t)W.wb[0], (uint32_t)x);
 }
 
 // voxelMatAt (common.glsl:976-985): the world voxel's material at p (0 = air or outside the loaded box).

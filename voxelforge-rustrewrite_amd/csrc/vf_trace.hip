@@ -43,6 +43,7 @@ struct PrimArgs {
     int32_t w, h, tiles_x, ntiles;
     int32_t cam_air;   // the host's grid copy shows the camera in an empty brick (or outside the world)
     float leaf;        // lookP.w = r.foliage.light: sunlight passing through each leaf (FOLIAGE_LIGHT)
+    float aspect;      // (float)w / (float)h, the IEEE quotient, computed once on the host
 };
 
 // The camera voxel decides which primary kernel variant's results stand (gbuffer.comp:233-241).  A camera
@@ -59,8 +60,7 @@ __device__ __forceinline__ void primary_dir(const PrimArgs &A, int32_t px, int32
     const float sy = ((float)py + 0.5f) + A.jit[1];
     const float nx = (sx / (float)A.w) * 2.0f - 1.0f;
     const float ny = (sy / (float)A.h) * 2.0f - 1.0f;
-    const float aspect = (float)A.w / (float)A.h;
-    const float a = (nx * aspect) * A.thf;
+    const float a = (nx * A.aspect) * A.thf;
     const float b = ny * A.thf;
 #pragma unroll
     for (int i = 0; i < 3; ++i) rd[i] = (A.fwd[i] + A.right[i] * a) + A.up[i] * b;
@@ -1565,6 +1565,7 @@ PrimArgs prim_args(const vf_pass_ubo *p, int32_t w, int32_t h) {
     A.ntiles = A.tiles_x * ((h + 15) / 16);
     A.cam_air = 0;
     A.leaf = p->look_p[3];
+    A.aspect = (float)w / (float)h;
     return A;
 }
 
