@@ -108,15 +108,6 @@ static __shared__ uint32_t vf_vmask[16 * 256];
 // The same for kernels of 64 threads (LS = 64: 4 KB per workgroup).
 static __shared__ uint32_t vf_vmask64[16 * 64];
 
-// brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
-// arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
-// LDSM: the mask goes to the lane's LDS slots and a step along z reads the new 64-bit layer from
-// there (one LDS read instead of a 16-to-2 word select in registers, which a wave executes on almost
-// every step since some lane steps along z).
-// BOX (the small-frame traversal, LDSM only): before the walk, the brick's mask is tested over the box of
-// voxels the walk can visit; tout = the brick DDA's time of leaving the brick (the least of its tMax).  A
-// walk that cannot hit is skipped: its result is the same miss, after which the brick DDA goes on from its
-// own state as it does after any walk.
 // Occupancy bit of voxel (lx, ly) in a 64-bit layer (lo, hi), for q = lx | ly << 4 (higher bits ignored): bit
 // lx + 8 ly.  The shift is written as the instruction, which takes its amount modulo 64, so the index is
 // q's bits 0-2 merged with bits 3-5 of q >> 1 (one bitfield insert) with nothing masked off above them.
@@ -128,6 +119,15 @@ __device__ __forceinline__ uint32_t layer_bit(uint32_t lo, uint32_t hi, uint32_t
     return (uint32_t)r & 1u;
 }
 
+// brickVoxels (common.glsl:474-539), USE_BRICK_MASKS off: the same voxel sequence and tMax
+// arithmetic; solidity comes from the brick's occupancy mask instead of one atlas fetch per step.
+// LDSM: the mask goes to the lane's LDS slots and a step along z reads the new 64-bit layer from
+// there (one LDS read instead of a 16-to-2 word select in registers, which a wave executes on almost
+// every step since some lane steps along z).
+// BOX (the small-frame traversal, LDSM only): before the walk, the brick's mask is tested over the box of
+// voxels the walk can visit; tout = the brick DDA's time of leaving the brick (the least of its tMax).  A
+// walk that cannot hit is skipped: its result is the same miss, after which the brick DDA goes on from its
+// own state as it does after any walk.
 template <bool COUNT, bool LDSM = false, int LS = 256, bool BOX = false>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
@@ -196,67 +196,18 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
     float tv = t;
     int32_t f2 = face;
     bool hit = false;
-    // One DDA step per trip, written with selects instead of a 3-way branch: the axis choice and
-    // the updates are the reference's (strict <, ties x then y then z; tMax += tDelta); the mask
-    // layer is re-selected only after a step along z.  Scalars, not arrays, so nothing is indexed.
+    // Both walks take one DDA step per trip, written with selects instead of a 3-way branch: the axis
+    // choice and the updates are the reference's (x when its tMax is below both others, else y when below
+    // z's, else z: strict <, so a tie goes to the later axis; tMax += tDelta).  Scalars, not arrays, so
+    // nothing is indexed.  A step out of the brick ends the walk without a hit (the caller's brick DDA goes
+    // on from its own state).  The reference caps the walk at 26 trips; it cannot reach the cap: every step
+    // moves one coordinate by the ray's fixed sign, so each axis leaves [0, 7] after at most 8 steps and
+    // the walk ends within 22 steps.  Each loop therefore has the single exit hit | out.
     float tMx = tM[0], tMy = tM[1], tMz = tM[2];
     const float tDx = tD[0], tDy = tD[1], tDz = tD[2];
     const int32_t fA0 = faceA[0], fA1 = faceA[1], fA2 = faceA[2];
     const int32_t s0 = st[0], s1 = st[1], s2 = st[2];
-    // One exit per trip: the next step is computed first and committed only when the ray neither hits
-    // the current voxel nor leaves the brick with it (a step out of the brick is not used: the caller's
-    // brick DDA continues from its own state).  The reference caps the walk at 26 trips; it cannot reach
-    // the cap: every step moves one coordinate by the ray's fixed sign, so each axis leaves [0, 7] after at
-    // most 8 steps and the walk exits by trip 22.  The loop therefore has the single exit hit | out.
-    if (BOX) {
-        // The small-frame walk: two steps per trip, so a small frame's long walks pay the loop's exit test,
-        // branch and LDS wait once per two voxels (its time is the latency of its slowest walks).  The
-        // layers of z and z + s2 are held in registers (wlo/whi, ulo/uhi): the trip's second voxel is in
-        // one of them; the two for the next trip are read from LDS as soon as its z is known.  The steps,
-        // their order of comparisons and additions and the exits are those of the loop below; a trip
-        // that ends on its second voxel reports that voxel with its step's t and face.
-        uint32_t ulo = sm[(((lz + s2) & 7) * 2) * LS], uhi = sm[(((lz + s2) & 7) * 2 + 1) * LS];
-        bool hitA = false, outA = false, hitB = false;
-        int32_t lx1 = lx, ly1 = ly, lz1 = lz;
-        float tvA = tv;
-        int32_t f2A = f2;
-#pragma unroll 1
-        for (;;) {
-            hitA = (((((uint64_t)whi << 32) | (uint64_t)wlo) >> (((uint32_t)ly << 3) | (uint32_t)lx)) & 1ull) != 0ull;
-            const bool a0 = tMx < tMy && tMx < tMz;
-            const bool a1 = !a0 && tMy < tMz;
-            const bool a2 = !a0 && !a1;
-            lx1 = lx + (a0 ? s0 : 0); ly1 = ly + (a1 ? s1 : 0); lz1 = lz + (a2 ? s2 : 0);
-            outA = ((uint32_t)lx1 | (uint32_t)ly1 | (uint32_t)lz1) > 7u;
-            tvA = a0 ? tMx : (a1 ? tMy : tMz);
-            f2A = a0 ? fA0 : (a1 ? fA1 : fA2);
-            const float x1 = a0 ? tMx + tDx : tMx, y1 = a1 ? tMy + tDy : tMy, z1 = a2 ? tMz + tDz : tMz;
-            const uint32_t blo = a2 ? ulo : wlo, bhi = a2 ? uhi : whi;
-            hitB = (((((uint64_t)bhi << 32) | (uint64_t)blo) >> ((((uint32_t)ly1 << 3) | (uint32_t)lx1) & 63u)) & 1ull) != 0ull;
-            const bool b0 = x1 < y1 && x1 < z1;
-            const bool b1 = !b0 && y1 < z1;
-            const bool b2 = !b0 && !b1;
-            const int32_t lx2 = lx1 + (b0 ? s0 : 0), ly2 = ly1 + (b1 ? s1 : 0), lz2 = lz1 + (b2 ? s2 : 0);
-            const bool outB = ((uint32_t)lx2 | (uint32_t)ly2 | (uint32_t)lz2) > 7u;
-            wlo = sm[((lz2 & 7) * 2) * LS];
-            whi = sm[((lz2 & 7) * 2 + 1) * LS];
-            ulo = sm[(((lz2 + s2) & 7) * 2) * LS];
-            uhi = sm[(((lz2 + s2) & 7) * 2 + 1) * LS];
-            if (hitA | outA | hitB | outB) break;
-            tv = b0 ? x1 : (b1 ? y1 : z1);
-            f2 = b0 ? fA0 : (b1 ? fA1 : fA2);
-            tMx = b0 ? x1 + tDx : x1;
-            tMy = b1 ? y1 + tDy : y1;
-            tMz = b2 ? z1 + tDz : z1;
-            lx = lx2; ly = ly2; lz = lz2;
-        }
-        if (!hitA && !outA && hitB) {   // the hit is the trip's second voxel
-            lx = lx1; ly = ly1; lz = lz1;
-            tv = tvA;
-            f2 = f2A;
-        }
-        hit = hitA || (!outA && hitB);
-    } else if (LDSM) {
+    if (LDSM) {
         // The walk in mirrored coordinates: u_a = l_a when the ray steps +1 along axis a, 7 - l_a when it
         // steps -1, so every step adds 1 to one u_a.  The three are packed in one word (u_x | u_y << 4 |
         // u_z << 8): a step is one addition, the step that leaves the brick is the one that takes a field
@@ -308,7 +259,7 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         if (hit) { lx = (int32_t)(q & 15u); ly = (int32_t)((q >> 4) & 15u); lz = (int32_t)(q >> 8); }
     } else
 #pragma unroll 1
-    for (;;) {
+    for (;;) {   // registers only: the layer is re-selected from the mask words after a step along z
         if (COUNT) { c.voxel++; c.bytes += 1; }
         const uint32_t idx = ((uint32_t)ly << 3) | (uint32_t)lx;   // bit of (lx, ly) in the 64-bit layer
         const uint64_t layer = ((uint64_t)whi << 32) | (uint64_t)wlo;
@@ -498,7 +449,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
                 stepped = true;
             }
         }
-        if (!stepped) {   // one brick step, strict <, ties x then y then z (common.glsl:644-655)
+        if (!stepped) {   // one brick step, the walk's axis rule (common.glsl:644-655)
             const bool a0 = tM8[0] < tM8[1] && tM8[0] < tM8[2];
             const bool a1 = !a0 && tM8[1] < tM8[2];
             const bool a2 = !a0 && !a1;

@@ -293,7 +293,8 @@ __global__ void __launch_bounds__(64, 5) k_primary_small(DevWorld W, PrimArgs A,
 }
 
 // The same pass with the small-frame traversal (trace_world's BOX), for launches of at most 16 waves per
-// SIMD: held at 5 waves per SIMD (up to 102 VGPRs; at 7 the box test spilled registers to scratch).
+// SIMD: bounded at 5 waves per SIMD (it takes 78 VGPRs; bounds of 6 and 7 measured no faster on these
+// launches, which hold at most 16 waves per SIMD in all).
 __global__ void __launch_bounds__(64, 5) k_primary_shadow_small(DevWorld W, PrimArgs A, float sx, float sy, float sz,
                                                                vf_hit *hits, float4 *gbpos, uint32_t *gbn, uint32_t *shadow,
                                                                vf_hit *shits, DirPre sp) {
