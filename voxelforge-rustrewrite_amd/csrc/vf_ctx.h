@@ -99,6 +99,9 @@ int launch_check(vf_ctx *ctx, const char *what);
 int begin_timed(vf_ctx *ctx);
 int end_timed(vf_ctx *ctx);
 int check_ready(vf_ctx *ctx, const char *what);
+// the host's grid copy shows world position pos in air (or outside the loaded box): a camera there needs
+// no per-ray camera-voxel lookup (vf_trace.hip)
+bool camera_in_air(const vf_ctx *ctx, const float pos[3]);
 int no_capture(vf_ctx *ctx, const char *what);
 void free_queue(vf_ctx *ctx);
 int queue_scratch_init(vf_ctx *ctx);   // vf_queue.hip: the device address of the batch scratch (device current)

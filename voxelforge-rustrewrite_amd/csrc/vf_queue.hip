@@ -345,6 +345,8 @@ struct ReqDev {
     int32_t w, h, tiles_x, tile_w, tile_h;
     float v[12];    // view: pos, right, up, fwd;  probes: origin (3), spacing
     float thf, jit0, jit1, reach;
+    float aspect;      // view: (float)w / (float)h, the IEEE quotient, computed on the host
+    int32_t cam_air;   // view: the host's grid copy shows the camera in air (no per-ray camera-voxel lookup)
     int32_t dims[3], n_rays;
     uint32_t frame;
     uint64_t first, count;   // id range [first, first + count); first is chunk-aligned
@@ -366,9 +368,10 @@ __global__ void k_batch_reset(BatchArgs B, unsigned long long *ctr, unsigned lon
 }
 
 __device__ __forceinline__ int find_req(const BatchArgs &B, uint64_t id) {
-    int r = 0;
-    for (int k = 1; k < B.n_req; ++k)
-        if (id >= B.req[k].first) r = k;
+    int r = 0;   // the request with the greatest first <= id (the id ranges need not follow request order)
+    uint64_t best = 0;
+    for (int k = 0; k < B.n_req; ++k)
+        if (B.req[k].first <= id && B.req[k].first >= best) { best = B.req[k].first; r = k; }
     return r;
 }
 
@@ -389,8 +392,7 @@ __device__ __forceinline__ void batch_ray(const DevWorld &W, const ReqDev &q, ui
         const float sy = ((float)py + 0.5f) + q.jit1;
         const float nx = (sx / (float)q.w) * 2.0f - 1.0f;
         const float ny = (sy / (float)q.h) * 2.0f - 1.0f;
-        const float aspect = (float)q.w / (float)q.h;
-        const float a = (nx * aspect) * q.thf;
+        const float a = (nx * q.aspect) * q.thf;
         const float b = ny * q.thf;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -399,7 +401,7 @@ __device__ __forceinline__ void batch_ray(const DevWorld &W, const ReqDev &q, ui
         }
         normalize3(rd);
         tcap = 1e30f;
-        const uint32_t cam_mat = voxel_mat_at(W, ro);
+        const uint32_t cam_mat = q.cam_air ? 0u : voxel_mat_at(W, ro);
         skip = cam_mat == 5u ? 5 : 0;
         if (cam_mat != 0u && cam_mat != 5u) {
             const float cam[3] = {ro[0], ro[1], ro[2]};
@@ -934,6 +936,8 @@ int trace_batch(vf_ctx *ctx, const vf_request *reqs, int32_t n_req, const vf_que
                 d.v[a] = q.cam_pos[a]; d.v[3 + a] = q.cam_right[a]; d.v[6 + a] = q.cam_up[a]; d.v[9 + a] = q.cam_fwd[a];
             }
             d.thf = q.tan_half_fov;
+            d.aspect = (float)q.width / (float)q.height;
+            d.cam_air = vf::camera_in_air(ctx, q.cam_pos) ? 1 : 0;
             d.jit0 = q.jitter[0];
             d.jit1 = q.jitter[1];
             d.count = (uint64_t)d.tiles_x * ty * cfg.tile_w * cfg.tile_h;   // tile padding: no output

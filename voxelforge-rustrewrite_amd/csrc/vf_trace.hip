@@ -1510,6 +1510,12 @@ int camera_class(const vf_ctx *ctx, const float pos[3]) {
     return ((ctx->host_occ[(size_t)(g - 256u) * 8u + (k >> 6)] >> (k & 63u)) & 1ull) ? 1 : 0;
 }
 
+}  // namespace
+namespace vf {
+bool camera_in_air(const vf_ctx *ctx, const float pos[3]) { return camera_class(ctx, pos) == 0; }
+}  // namespace vf
+namespace {
+
 template <bool COUNT>
 void launch_primary(vf_ctx *ctx, PrimArgs A, vf_hit *hits, float4 *gbpos, uint32_t *gbn) {
     const int cls = camera_class(ctx, A.pos);
