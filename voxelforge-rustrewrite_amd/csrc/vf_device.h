@@ -223,7 +223,12 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         hit = layer_bit(wlo, whi, q) != 0u;
         if (!hit) {
             // One basic block per trip: the loop's only branch is its exit, the tMax values are advanced by
-            // selects that keep them on the exit trip, and hit / out are values in registers.
+            // selects that keep them on the exit trip, and hit / out are values in registers.  The tDelta
+            // values are pinned to vector registers first: for a direction shared by the launch (the sun's)
+            // they are scalars, and a select cannot take a scalar operand beside its scalar condition, so
+            // each trip would copy them again.
+            float tDxv = tDx, tDyv = tDy, tDzv = tDz;
+            asm volatile("" : "+v"(tDxv), "+v"(tDyv), "+v"(tDzv));
             uint32_t hb, ob;
 #pragma unroll 1
             for (;;) {
@@ -242,9 +247,9 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
                 const bool stop = (hb | ob) != 0u;
                 if (COUNT && ob == 0u) { c.voxel++; c.bytes += 1; }   // a voxel inside the brick was tested
                 // adding -0 leaves every value as it is (-0 included)
-                tMx = tMx + ((a0 && !stop) ? tDx : -0.0f);
-                tMy = tMy + ((a1 && !stop) ? tDy : -0.0f);
-                tMz = tMz + ((a2 && !stop) ? tDz : -0.0f);
+                tMx = tMx + ((a0 && !stop) ? tDxv : -0.0f);
+                tMy = tMy + ((a1 && !stop) ? tDyv : -0.0f);
+                tMz = tMz + ((a2 && !stop) ? tDzv : -0.0f);
                 if (stop) break;
             }
             asm volatile("" : "+v"(d));   // read d as the register it is (not as the loop's axis conditions)
