@@ -128,7 +128,7 @@ __device__ __forceinline__ uint32_t layer_bit(uint32_t lo, uint32_t hi, uint32_t
 // voxels the walk can visit; tout = the brick DDA's time of leaving the brick (the least of its tMax).  A
 // walk that cannot hit is skipped: its result is the same miss, after which the brick DDA goes on from its
 // own state as it does after any walk.
-template <bool COUNT, bool LDSM = false, int LS = 256, bool BOX = false>
+template <bool COUNT, bool LDSM = false, int LS = 256, bool BOX = false, bool UNI = false>
 __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3], const float sd[3], const float inv[3],
                                              const int32_t st[3], const float up[3], const int32_t faceA[3],
                                              const int32_t bc[3], uint32_t g, float t, int32_t face, Hit &h, Cnt &c,
@@ -223,12 +223,12 @@ __device__ __forceinline__ bool brick_voxels(const DevWorld &W, const float ro[3
         hit = layer_bit(wlo, whi, q) != 0u;
         if (!hit) {
             // One basic block per trip: the loop's only branch is its exit, the tMax values are advanced by
-            // selects that keep them on the exit trip, and hit / out are values in registers.  The tDelta
-            // values are pinned to vector registers first: for a direction shared by the launch (the sun's)
-            // they are scalars, and a select cannot take a scalar operand beside its scalar condition, so
-            // each trip would copy them again.
+            // selects that keep them on the exit trip, and hit / out are values in registers.  UNI (a
+            // direction shared by the launch, the sun's): the tDelta values are scalars, and a select cannot
+            // take a scalar operand beside its scalar condition, so they are copied to vector registers once
+            // here rather than on every trip.
             float tDxv = tDx, tDyv = tDy, tDzv = tDz;
-            asm volatile("" : "+v"(tDxv), "+v"(tDyv), "+v"(tDzv));
+            if (UNI) asm volatile("" : "+v"(tDxv), "+v"(tDyv), "+v"(tDzv));
             uint32_t hb, ob;
 #pragma unroll 1
             for (;;) {
@@ -395,7 +395,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
             }
         }
         if (g >= 256u) {
-            if (brick_voxels<COUNT, LDSM, LS, BOX>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk,
+            if (brick_voxels<COUNT, LDSM, LS, BOX, PRE>(W, ro, sd, inv, st, up, faceA, bc, g, t, face, h, c, sk,
                                                    BOX ? gmin(gmin(tM8[0], tM8[1]), tM8[2]) : 0.0f))
                 return;
         } else if (g != 0u && g != (uint32_t)sk) {
