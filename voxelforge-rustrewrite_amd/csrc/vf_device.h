@@ -60,6 +60,17 @@ __device__ __forceinline__ int32_t f2i(float x) {
     asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;
 }
+// 1 / x rounded to nearest, for |x| >= 1e-5 (the traversal clamps direction components there): the hardware
+// reciprocal refined by one Newton-Raphson step with fused multiply-adds (e = 1 - x*y, y + y*e).  For every
+// float with 1e-5 <= |x| < 2^126, both signs, this equals the IEEE quotient 1.0f / x bit for bit (checked
+// exhaustively on gfx950: 1.2 G values); from 2^126 on, where 1/x is subnormal, it does not, and the
+// division is taken instead.  Three instructions in place of the division's eleven.
+__device__ __forceinline__ float rcp_rn(float x) {
+    const float y = __builtin_amdgcn_rcpf(x);
+    float r = __builtin_fmaf(__builtin_fmaf(-x, y, 1.0f), y, y);
+    if (__builtin_expect(fabsf(x) >= 0x1p126f, 0)) r = 1.0f / x;
+    return r;
+}
 __device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 __device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3]) {
     float x = a[1] * b[2] - b[1] * a[2];
@@ -343,7 +354,7 @@ __device__ __forceinline__ void trace_world(const DevWorld &W, const float ro[3]
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         sd[a] = PRE ? pre->sd[a] : (fabsf(rd[a]) < 1e-5f ? 1e-5f : rd[a]);
-        inv[a] = PRE ? pre->inv[a] : 1.0f / sd[a];
+        inv[a] = PRE ? pre->inv[a] : rcp_rn(sd[a]);
         const float lo = (float)(W.minb[a] * 8);
         const float dims = (float)(W.wb[a] * 8);
         const float t0 = (lo - ro[a]) * inv[a];
